@@ -1,0 +1,174 @@
+/* spectre_amd: MI355X (gfx950) implementation of SpECTRE's DG volume
+ * time-derivative path behind a C ABI.
+ *
+ * Conventions shared by every entry point
+ *  - All field pointers are DEVICE pointers to FP64 data in SpECTRE's
+ *    Variables layout: per element, nc contiguous blocks of Np doubles, grid
+ *    index i + N0 (j + N1 k) (DataStructures/Index.hpp:177-183).  A batch of
+ *    E elements of equal extents is E such buffers back to back.
+ *  - Tensor components are in SpECTRE storage order
+ *    (DataStructures/Tensor/Structure.hpp:98-195).
+ *  - The inverse Jacobian is 9 blocks per element, block j + 3 i holding
+ *    d xi^j / d x^i (NumericalAlgorithms/LinearOperators/PartialDerivatives.tpp:70-77).
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream).  All
+ *    device work is enqueued on it; calls return after enqueueing.
+ *  - Return value: SXD_OK (0) or a negative status.  The reference reports
+ *    the same conditions through ERROR/ASSERT; here the caller decides.
+ *    sxd_last_error() gives a message for the calling thread's last failure.
+ *  - Meshes are Legendre / Gauss-Lobatto in every dimension with 2..20
+ *    points per dimension (Spectral.hpp:100-102).
+ */
+#ifndef SPECTRE_AMD_H
+#define SPECTRE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SXD_OK = 0,
+  SXD_ERR_INVALID_ARGUMENT = -1,
+  SXD_ERR_UNSUPPORTED = -2,
+  SXD_ERR_DEVICE = -3,
+  SXD_ERR_OUT_OF_MEMORY = -4
+};
+
+/* Message describing the calling thread's most recent failure ("" if none). */
+const char* sxd_last_error(void);
+
+/* Library version string. */
+const char* sxd_version(void);
+
+/* ---- Spectral (host functions, no device needed) ------------------------ */
+
+/* LGL collocation points and quadrature weights for n points.
+ * Replaces Spectral::collocation_points / quadrature_weights for
+ * Basis::Legendre, Quadrature::GaussLobatto
+ * (NumericalAlgorithms/Spectral/Spectral.hpp:170-190, Legendre.cpp:187-232). */
+int sxd_lgl_points_and_weights(int n, double* x, double* w);
+
+/* LGL differentiation matrix, row-major d[i * n + j] = D_ij.
+ * Replaces Spectral::differentiation_matrix(const Mesh<1>&)
+ * (NumericalAlgorithms/Spectral/Spectral.cpp:113-150, 859-878). */
+int sxd_lgl_differentiation_matrix(int n, double* d);
+
+/* ---- Linear operators ------------------------------------------------- */
+
+/* Logical partial derivatives: dlogical = [3][E][nc][Np] (d/dxi, d/deta,
+ * d/dzeta, each a batch of E Variables buffers).
+ * Replaces logical_partial_derivatives
+ * (NumericalAlgorithms/LinearOperators/PartialDerivatives.tpp:137-184). */
+int sxd_logical_partial_derivatives(int num_elements, const int32_t extents[3],
+                                    int nc, const double* u, double* dlogical,
+                                    void* stream);
+
+/* Partial derivatives with respect to inertial coordinates:
+ * du = [E][3 nc][Np], block 3 c + i = d_i u_c.
+ * Replaces partial_derivatives(not_null<Variables<deriv tags>*>,
+ * const Variables<u tags>&, const Mesh<3>&, const InverseJacobian&)
+ * (NumericalAlgorithms/LinearOperators/PartialDerivatives.tpp:191-241). */
+int sxd_partial_derivatives(int num_elements, const int32_t extents[3], int nc,
+                            const double* u, const double* inv_jac, double* du,
+                            void* stream);
+
+/* ---- Point-wise system terms ------------------------------------------ */
+
+/* ScalarWave::TimeDerivative<3>::apply over num_points points
+ * (Evolution/Systems/ScalarWave/TimeDerivative.cpp:14-45).
+ * u [5][n] (psi, Pi, Phi_i), du [15][n] (d psi, d Pi, d_i Phi_j at 6+i+3j),
+ * gamma2 [n]; dt [5][n]; gamma2_temp [n] (the temporary copy, may be NULL). */
+int sxd_scalar_wave_time_derivative(int64_t num_points, const double* u,
+                                    const double* du, const double* gamma2,
+                                    double* dt, double* gamma2_temp,
+                                    void* stream);
+
+/* gh::TimeDerivative<3>::apply, harmonic gauge, static mesh
+ * (Evolution/Systems/GeneralizedHarmonic/TimeDerivative.cpp:30-407).
+ * u [50][n] (g_ab, Pi_ab, Phi_iab), du [150][n] (partial_derivatives layout),
+ * gammas [3][n] (gamma0, gamma1, gamma2); dt [50][n];
+ * temps [12][n] (gamma1, gamma2, lapse, shift^i, inverse spatial metric), may
+ * be NULL. */
+int sxd_gh_time_derivative(int64_t num_points, const double* u,
+                           const double* du, const double* gammas, double* dt,
+                           double* temps, void* stream);
+
+/* ---- Boundary correction and lift -------------------------------------- */
+
+/* ScalarWave UpwindPenalty<3>: dg_package_data on both sides of n face
+ * points followed by dg_boundary_terms
+ * (Evolution/Systems/ScalarWave/BoundaryCorrections/UpwindPenalty.cpp:36-205).
+ * u_int/u_ext [5][n], gamma2_int/ext [n], normal_int/ext [3][n] unit normal
+ * covectors, each side pointing out of its own element; bc [5][n]. */
+int sxd_scalar_wave_upwind_penalty(int64_t num_points, const double* u_int,
+                                   const double* gamma2_int,
+                                   const double* normal_int,
+                                   const double* u_ext,
+                                   const double* gamma2_ext,
+                                   const double* normal_ext, double* bc,
+                                   void* stream);
+
+/* GH UpwindPenalty<3> on n face points
+ * (Evolution/Systems/GeneralizedHarmonic/BoundaryCorrections/UpwindPenalty.cpp:36-275),
+ * including the metric normalisation of the normal
+ * (Evolution/DiscontinuousGalerkin/Actions/NormalCovectorAndMagnitude.hpp:47-81).
+ * u_* [50][n]; temps_* [12][n] as written by sxd_gh_time_derivative;
+ * normal_* [3][n] UNNORMALIZED covectors; bc [50][n]. */
+int sxd_gh_upwind_penalty(int64_t num_points, const double* u_int,
+                          const double* temps_int, const double* normal_int,
+                          const double* u_ext, const double* temps_ext,
+                          const double* normal_ext, double* bc, void* stream);
+
+/* dg::lift_flux for Gauss-Lobatto points: bc *= -0.5 N (N - 1) |n|, in place,
+ * nc components of n face points
+ * (NumericalAlgorithms/DiscontinuousGalerkin/LiftFlux.hpp:41-62). */
+int sxd_lift_flux(int64_t num_points, int nc, double* bc,
+                  int extent_perpendicular, const double* magnitude_of_normal,
+                  void* stream);
+
+/* ---- Whole DG right-hand side on a periodic brick ----------------------- */
+
+/* A set of conforming, aligned hexahedral elements with equal extents and a
+ * neighbour table.  Replaces the per-element Mesh<3> + neighbour/mortar
+ * bookkeeping of ComputeTimeDerivative / ApplyBoundaryCorrections
+ * (Evolution/DiscontinuousGalerkin/Actions/ComputeTimeDerivative.hpp:383-650,
+ * ApplyBoundaryCorrections.hpp:722-1050) for a single device.
+ * neighbors: host array [6 E], entry 6 e + 2 d + side (side 0 = lower,
+ * 1 = upper) is the element across that face.  The table is copied to the
+ * device; this call allocates device memory. */
+typedef struct sxd_domain sxd_domain;
+int sxd_domain_create(sxd_domain** domain, int num_elements,
+                      const int32_t extents[3], const int32_t* neighbors);
+int sxd_domain_destroy(sxd_domain* domain);
+
+/* Full ScalarWave DG time derivative (volume terms + UpwindPenalty + LGL lift)
+ * for every element of the domain: ComputeTimeDerivative followed by
+ * ApplyBoundaryCorrectionsToTimeDerivative.
+ * u, dt [E][5][Np]; gamma2 [E][Np]; inv_jac [E][9][Np]. */
+int sxd_scalar_wave_rhs(const sxd_domain* domain, const double* u,
+                        const double* gamma2, const double* inv_jac,
+                        double* dt, void* stream);
+
+/* Full generalized-harmonic DG time derivative (harmonic gauge).
+ * u, dt [E][50][Np]; gammas [E][3][Np]; inv_jac [E][9][Np]. */
+int sxd_gh_rhs(const sxd_domain* domain, const double* u, const double* gammas,
+               const double* inv_jac, double* dt, void* stream);
+
+/* ---- Time stepping ------------------------------------------------------ */
+
+/* One substep of TimeSteppers::Rk3HesthavenSsp::update_u
+ * (Time/TimeSteppers/Rk3HesthavenSsp.cpp:55-81) on n doubles:
+ *   substep 0: out = u0 + dt lk
+ *   substep 1: out = 0.25 (3 u0 + uk + dt lk)
+ *   substep 2: out = (1/3) (u0 + 2 uk + 2 dt lk)               */
+int sxd_rk3_substep(int substep, int64_t n, double dt, const double* u0,
+                    const double* uk, const double* lk, double* out,
+                    void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPECTRE_AMD_H */

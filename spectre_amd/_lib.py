@@ -1,0 +1,71 @@
+"""Loader of the native library libspectre_amd.so (HIP kernels + C ABI).
+
+The product path runs only through this library: if it is missing or fails to
+load, every entry point raises; there is no CPU fallback.
+"""
+import ctypes
+import pathlib
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libspectre_amd.so"
+
+_lib = None
+
+
+class SpectreAmdError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, status, message):
+        super().__init__(f"spectre_amd status {status}: {message}")
+        self.status = status
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+
+_SIGNATURES = {
+    "sxd_last_error": (ctypes.c_char_p, []),
+    "sxd_version": (ctypes.c_char_p, []),
+    "sxd_lgl_points_and_weights": (ctypes.c_int, [ctypes.c_int, _dp, _dp]),
+    "sxd_lgl_differentiation_matrix": (ctypes.c_int, [ctypes.c_int, _dp]),
+    "sxd_logical_partial_derivatives": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _vp, _vp, _vp]),
+    "sxd_partial_derivatives": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _vp, _vp, _vp, _vp]),
+    "sxd_scalar_wave_time_derivative": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sxd_gh_time_derivative": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sxd_scalar_wave_upwind_penalty": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sxd_gh_upwind_penalty": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sxd_lift_flux": (ctypes.c_int, [_i64, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp]),
+    "sxd_domain_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _ip, _ip]),
+    "sxd_domain_destroy": (ctypes.c_int, [_vp]),
+    "sxd_scalar_wave_rhs": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "sxd_gh_rhs": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "sxd_rk3_substep": (ctypes.c_int, [ctypes.c_int, _i64, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]),
+}
+
+
+def exported_symbols():
+    """Names of the C-ABI functions this binding expects the library to export."""
+    return sorted(_SIGNATURES)
+
+
+def lib():
+    """The loaded library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise SpectreAmdError(-2, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        handle = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(status):
+    if status != 0:
+        raise SpectreAmdError(status, lib().sxd_last_error().decode())
+    return status

@@ -1,0 +1,391 @@
+// C ABI of spectre_amd: argument validation, error reporting, device tables
+// and the domain handle.  Host code; the kernels live in kernels.hip and
+// gh_fused.hip.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/spectre_amd.h"
+#include "device_common.hpp"
+#include "kernels.hpp"
+#include "spectral.hpp"
+
+struct sxd_domain {
+  int device = 0;
+  int num_elements = 0;
+  int32_t ext[3] = {0, 0, 0};
+  int32_t* d_nbr = nullptr;
+  // Work arrays of the composite (non-fused) paths, allocated on first use.
+  double* d_scratch_du = nullptr;
+  double* d_scratch_temps = nullptr;
+  size_t scratch_du_doubles = 0;
+  size_t scratch_temps_doubles = 0;
+  std::mutex scratch_mutex;
+};
+
+namespace sxd {
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return status;
+}
+
+int ok() {
+  g_last_error.clear();
+  return SXD_OK;
+}
+
+bool valid_extents(const int32_t* ext) {
+  if (ext == nullptr) return false;
+  for (int d = 0; d < 3; ++d)
+    if (ext[d] < kMinPoints || ext[d] > kMaxPoints) return false;
+  return true;
+}
+
+// True if p is device (or managed) memory of the current device's address
+// space.  Host pointers passed by mistake are rejected before any launch.
+bool is_device_pointer(const void* p) {
+  if (p == nullptr) return false;
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
+         attr.type == hipMemoryTypeUnified;
+}
+
+#define SXD_REQUIRE_DEVICE(ptr)                                                     \
+  do {                                                                              \
+    if (!is_device_pointer(ptr))                                                    \
+      return fail(SXD_ERR_INVALID_ARGUMENT, "%s: '%s' is not a device pointer",     \
+                  __func__, #ptr);                                                  \
+  } while (0)
+
+int finish(int rc, const char* where) {
+  if (rc == SXD_OK) return ok();
+  if (rc == SXD_ERR_DEVICE) {
+    const hipError_t err = hipGetLastError();
+    return fail(rc, "%s: kernel launch failed: %s", where, hipGetErrorString(err));
+  }
+  if (rc == SXD_ERR_UNSUPPORTED) return fail(rc, "%s: unsupported configuration", where);
+  return fail(rc, "%s: failed with status %d", where, rc);
+}
+
+struct DeviceTables {
+  std::mutex mutex;
+  std::vector<double*> per_device;
+};
+
+DeviceTables& tables() {
+  static DeviceTables t;
+  return t;
+}
+
+}  // namespace
+
+const double* device_diff_matrices() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  DeviceTables& t = tables();
+  std::lock_guard<std::mutex> lock(t.mutex);
+  if (static_cast<int>(t.per_device.size()) <= dev) t.per_device.resize(dev + 1, nullptr);
+  if (t.per_device[dev] == nullptr) {
+    const std::vector<double>& host = packed_differentiation_matrices();
+    double* d = nullptr;
+    if (hipMalloc(&d, host.size() * sizeof(double)) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice) !=
+        hipSuccess) {
+      (void)hipFree(d);
+      return nullptr;
+    }
+    t.per_device[dev] = d;
+  }
+  return t.per_device[dev];
+}
+
+}  // namespace sxd
+
+using namespace sxd;
+
+extern "C" {
+
+const char* sxd_last_error(void) { return g_last_error.c_str(); }
+
+const char* sxd_version(void) { return "spectre_amd 0.1 (gfx950, FP64)"; }
+
+int sxd_lgl_points_and_weights(int n, double* x, double* w) {
+  if (n < kMinPoints || n > kMaxPoints)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "LGL quadrature needs 2..20 points, got %d", n);
+  if (x == nullptr || w == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null output");
+  lgl_points_and_weights(n, x, w);
+  return ok();
+}
+
+int sxd_lgl_differentiation_matrix(int n, double* d) {
+  if (n < kMinPoints || n > kMaxPoints)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "LGL quadrature needs 2..20 points, got %d", n);
+  if (d == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null output");
+  const std::vector<double>& t = packed_differentiation_matrices();
+  std::memcpy(d, t.data() + packed_d_offset(n), sizeof(double) * n * n);
+  return ok();
+}
+
+int sxd_logical_partial_derivatives(int num_elements, const int32_t extents[3], int nc,
+                                    const double* u, double* dlogical, void* stream) {
+  if (!valid_extents(extents))
+    return fail(SXD_ERR_INVALID_ARGUMENT, "extents must be 2..20 in each dimension");
+  if (num_elements < 0 || nc < 1)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "bad element or component count");
+  if (num_elements == 0) return ok();
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(dlogical);
+  return finish(launch_partial_derivatives(num_elements, extents, nc, u, nullptr, nullptr,
+                                           dlogical, static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+int sxd_partial_derivatives(int num_elements, const int32_t extents[3], int nc,
+                            const double* u, const double* inv_jac, double* du,
+                            void* stream) {
+  if (!valid_extents(extents))
+    return fail(SXD_ERR_INVALID_ARGUMENT, "extents must be 2..20 in each dimension");
+  if (num_elements < 0 || nc < 1)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "bad element or component count");
+  if (num_elements == 0) return ok();
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  SXD_REQUIRE_DEVICE(du);
+  return finish(launch_partial_derivatives(num_elements, extents, nc, u, inv_jac, du,
+                                           nullptr, static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+int sxd_scalar_wave_time_derivative(int64_t num_points, const double* u, const double* du,
+                                    const double* gamma2, double* dt, double* gamma2_temp,
+                                    void* stream) {
+  if (num_points < 0) return fail(SXD_ERR_INVALID_ARGUMENT, "negative point count");
+  if (num_points == 0) return ok();
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(du);
+  SXD_REQUIRE_DEVICE(gamma2);
+  SXD_REQUIRE_DEVICE(dt);
+  if (gamma2_temp != nullptr) SXD_REQUIRE_DEVICE(gamma2_temp);
+  return finish(launch_sw_time_derivative(num_points, u, du, gamma2, dt, gamma2_temp,
+                                          static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+int sxd_gh_time_derivative(int64_t num_points, const double* u, const double* du,
+                           const double* gammas, double* dt, double* temps, void* stream) {
+  if (num_points < 0) return fail(SXD_ERR_INVALID_ARGUMENT, "negative point count");
+  if (num_points == 0) return ok();
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(du);
+  SXD_REQUIRE_DEVICE(gammas);
+  SXD_REQUIRE_DEVICE(dt);
+  if (temps != nullptr) SXD_REQUIRE_DEVICE(temps);
+  return finish(launch_gh_time_derivative(num_points, u, du, gammas, dt, temps,
+                                          static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+int sxd_scalar_wave_upwind_penalty(int64_t num_points, const double* u_int,
+                                   const double* gamma2_int, const double* normal_int,
+                                   const double* u_ext, const double* gamma2_ext,
+                                   const double* normal_ext, double* bc, void* stream) {
+  if (num_points < 0) return fail(SXD_ERR_INVALID_ARGUMENT, "negative point count");
+  if (num_points == 0) return ok();
+  SXD_REQUIRE_DEVICE(u_int);
+  SXD_REQUIRE_DEVICE(gamma2_int);
+  SXD_REQUIRE_DEVICE(normal_int);
+  SXD_REQUIRE_DEVICE(u_ext);
+  SXD_REQUIRE_DEVICE(gamma2_ext);
+  SXD_REQUIRE_DEVICE(normal_ext);
+  SXD_REQUIRE_DEVICE(bc);
+  return finish(launch_sw_upwind(num_points, u_int, gamma2_int, normal_int, u_ext,
+                                 gamma2_ext, normal_ext, bc,
+                                 static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+int sxd_gh_upwind_penalty(int64_t num_points, const double* u_int, const double* temps_int,
+                          const double* normal_int, const double* u_ext,
+                          const double* temps_ext, const double* normal_ext, double* bc,
+                          void* stream) {
+  if (num_points < 0) return fail(SXD_ERR_INVALID_ARGUMENT, "negative point count");
+  if (num_points == 0) return ok();
+  SXD_REQUIRE_DEVICE(u_int);
+  SXD_REQUIRE_DEVICE(temps_int);
+  SXD_REQUIRE_DEVICE(normal_int);
+  SXD_REQUIRE_DEVICE(u_ext);
+  SXD_REQUIRE_DEVICE(temps_ext);
+  SXD_REQUIRE_DEVICE(normal_ext);
+  SXD_REQUIRE_DEVICE(bc);
+  return finish(launch_gh_upwind(num_points, u_int, temps_int, normal_int, u_ext, temps_ext,
+                                 normal_ext, bc, static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+int sxd_lift_flux(int64_t num_points, int nc, double* bc, int extent_perpendicular,
+                  const double* magnitude_of_normal, void* stream) {
+  if (num_points < 0 || nc < 1)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "bad point or component count");
+  if (extent_perpendicular < kMinPoints || extent_perpendicular > kMaxPoints)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "perpendicular extent must be 2..20");
+  if (num_points == 0) return ok();
+  SXD_REQUIRE_DEVICE(bc);
+  SXD_REQUIRE_DEVICE(magnitude_of_normal);
+  return finish(launch_lift_flux(num_points, nc, bc, extent_perpendicular,
+                                 magnitude_of_normal, static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+int sxd_domain_create(sxd_domain** domain, int num_elements, const int32_t extents[3],
+                      const int32_t* neighbors) {
+  if (domain == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null domain output");
+  *domain = nullptr;
+  if (num_elements < 1) return fail(SXD_ERR_INVALID_ARGUMENT, "need at least one element");
+  if (!valid_extents(extents))
+    return fail(SXD_ERR_INVALID_ARGUMENT, "extents must be 2..20 in each dimension");
+  if (neighbors == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null neighbour table");
+  for (int64_t i = 0; i < 6LL * num_elements; ++i)
+    if (neighbors[i] < 0 || neighbors[i] >= num_elements)
+      return fail(SXD_ERR_INVALID_ARGUMENT, "neighbour %d of element %d out of range",
+                  neighbors[i], static_cast<int>(i / 6));
+  sxd_domain* d = new (std::nothrow) sxd_domain;
+  if (d == nullptr) return fail(SXD_ERR_OUT_OF_MEMORY, "host allocation failed");
+  if (hipGetDevice(&d->device) != hipSuccess) {
+    delete d;
+    return fail(SXD_ERR_DEVICE, "no HIP device");
+  }
+  d->num_elements = num_elements;
+  for (int i = 0; i < 3; ++i) d->ext[i] = extents[i];
+  const size_t bytes = sizeof(int32_t) * 6 * static_cast<size_t>(num_elements);
+  if (hipMalloc(&d->d_nbr, bytes) != hipSuccess) {
+    delete d;
+    return fail(SXD_ERR_OUT_OF_MEMORY, "device allocation of the neighbour table failed");
+  }
+  if (hipMemcpy(d->d_nbr, neighbors, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d->d_nbr);
+    delete d;
+    return fail(SXD_ERR_DEVICE, "copy of the neighbour table failed");
+  }
+  if (device_diff_matrices() == nullptr) {
+    (void)hipFree(d->d_nbr);
+    delete d;
+    return fail(SXD_ERR_DEVICE, "upload of the spectral tables failed");
+  }
+  *domain = d;
+  return ok();
+}
+
+int sxd_domain_destroy(sxd_domain* domain) {
+  if (domain == nullptr) return ok();
+  if (domain->d_nbr != nullptr) (void)hipFree(domain->d_nbr);
+  if (domain->d_scratch_du != nullptr) (void)hipFree(domain->d_scratch_du);
+  if (domain->d_scratch_temps != nullptr) (void)hipFree(domain->d_scratch_temps);
+  delete domain;
+  return ok();
+}
+
+namespace {
+int ensure_scratch(sxd_domain* d, size_t du_doubles, size_t temps_doubles) {
+  std::lock_guard<std::mutex> lock(d->scratch_mutex);
+  if (d->scratch_du_doubles < du_doubles) {
+    if (d->d_scratch_du != nullptr) (void)hipFree(d->d_scratch_du);
+    d->d_scratch_du = nullptr;
+    d->scratch_du_doubles = 0;
+    if (hipMalloc(&d->d_scratch_du, du_doubles * sizeof(double)) != hipSuccess)
+      return SXD_ERR_OUT_OF_MEMORY;
+    d->scratch_du_doubles = du_doubles;
+  }
+  if (d->scratch_temps_doubles < temps_doubles) {
+    if (d->d_scratch_temps != nullptr) (void)hipFree(d->d_scratch_temps);
+    d->d_scratch_temps = nullptr;
+    d->scratch_temps_doubles = 0;
+    if (hipMalloc(&d->d_scratch_temps, temps_doubles * sizeof(double)) != hipSuccess)
+      return SXD_ERR_OUT_OF_MEMORY;
+    d->scratch_temps_doubles = temps_doubles;
+  }
+  return SXD_OK;
+}
+
+int check_domain(const sxd_domain* d) {
+  if (d == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null domain");
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != d->device)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "domain belongs to device %d, current is %d",
+                d->device, dev);
+  return SXD_OK;
+}
+}  // namespace
+
+int sxd_scalar_wave_rhs(const sxd_domain* domain, const double* u, const double* gamma2,
+                        const double* inv_jac, double* dt, void* stream) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(gamma2);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  SXD_REQUIRE_DEVICE(dt);
+  sxd_domain* d = const_cast<sxd_domain*>(domain);
+  const bool fused = d->ext[0] == d->ext[1] && d->ext[1] == d->ext[2] && d->ext[0] <= 10;
+  double* scratch = nullptr;
+  if (!fused) {
+    const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
+    if (ensure_scratch(d, 15 * np * d->num_elements, 0) != SXD_OK)
+      return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
+    scratch = d->d_scratch_du;
+  }
+  return finish(launch_sw_rhs(d->num_elements, d->ext, d->d_nbr, u, gamma2, inv_jac, dt,
+                              scratch, static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+int sxd_gh_rhs(const sxd_domain* domain, const double* u, const double* gammas,
+               const double* inv_jac, double* dt, void* stream) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(gammas);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  SXD_REQUIRE_DEVICE(dt);
+  sxd_domain* d = const_cast<sxd_domain*>(domain);
+  const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
+  if (ensure_scratch(d, 150 * np * d->num_elements, 12 * np * d->num_elements) != SXD_OK)
+    return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
+  return finish(launch_gh_rhs_composite(d->num_elements, d->ext, d->d_nbr, u, gammas,
+                                        inv_jac, dt, d->d_scratch_du, d->d_scratch_temps,
+                                        static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+int sxd_rk3_substep(int substep, int64_t n, double dt, const double* u0, const double* uk,
+                    const double* lk, double* out, void* stream) {
+  if (substep < 0 || substep > 2)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "substep must be 0, 1 or 2");
+  if (n < 0) return fail(SXD_ERR_INVALID_ARGUMENT, "negative size");
+  if (n == 0) return ok();
+  SXD_REQUIRE_DEVICE(u0);
+  if (substep > 0) SXD_REQUIRE_DEVICE(uk);
+  SXD_REQUIRE_DEVICE(lk);
+  SXD_REQUIRE_DEVICE(out);
+  return finish(launch_rk3_substep(substep, n, dt, u0, substep > 0 ? uk : u0, lk, out,
+                                   static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+// Shared device-side definitions for the CDNA4 kernels.
+//
+// Floating point: everything is FP64.  Kernels are compiled with
+// -ffp-contract=off; every accumulation statement s += a * b of the
+// reference is written as an explicit fma(a, b, s), everything else is
+// evaluated unfused in the reference's order.  This makes the kernels
+// reproduce the reference's arithmetic step for step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "spectral.hpp"
+
+namespace sxd {
+
+// Device copy of the packed LGL differentiation matrices, n = 2..20
+// (row-major, see spectral.hpp); kernels take its address as an argument.
+__device__ __forceinline__ const double* diff_matrix(const double* table, int n) {
+  return table + packed_d_offset(n);
+}
+
+// Tensor storage order (DataStructures/Tensor/Structure.hpp:98-195).
+// Symmetric 4-d pair (a <= b): (0,0)(0,1)(0,2)(0,3)(1,1)(1,2)(1,3)(2,2)(2,3)(3,3)
+__host__ __device__ constexpr int sym4(int a, int b) {
+  return a <= b ? (a == 0 ? b : a == 1 ? 3 + b : a == 2 ? 5 + b : 9)
+                : (b == 0 ? a : b == 1 ? 3 + a : b == 2 ? 5 + a : 9);
+}
+// Symmetric 3-d pair: (0,0)(0,1)(0,2)(1,1)(1,2)(2,2)
+__host__ __device__ constexpr int sym3(int a, int b) {
+  return a <= b ? (a == 0 ? b : a == 1 ? 2 + b : 5) : (b == 0 ? a : b == 1 ? 2 + a : 5);
+}
+// First (row) index of each symmetric 4-d pair in storage order.
+__host__ __device__ constexpr int pair_a(int p) {
+  return p < 4 ? 0 : p < 7 ? 1 : p < 9 ? 2 : 3;
+}
+__host__ __device__ constexpr int pair_b(int p) {
+  return p < 4 ? p : p < 7 ? p - 3 : p < 9 ? p - 5 : 3;
+}
+// InverseJacobian(logical j, inertial i) block.
+__host__ __device__ constexpr int ij3(int j, int i) { return j + 3 * i; }
+
+// Heaviside step of UpwindPenalty (Blaze/StepFunction.hpp:46-48).
+__device__ __forceinline__ double step_fn(double x) { return x < 0.0 ? 0.0 : 1.0; }
+
+// Lift factor of dg::lift_flux (LiftFlux.hpp:56-60): -0.5 N (N-1) |n|.
+__device__ __forceinline__ double lift_factor(int n_perp, double mag) {
+  return (-0.5 * static_cast<double>(n_perp * (n_perp - 1))) * mag;
+}
+
+// Device address of the packed D tables on the current device, uploading
+// them on first use (one small allocation per device for the life of the
+// process).  Returns nullptr on failure.
+const double* device_diff_matrices();
+
+}  // namespace sxd

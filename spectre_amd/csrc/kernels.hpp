@@ -1,0 +1,40 @@
+// Host-side launchers of the device kernels (defined in kernels.hip and
+// gh_fused.hip).  Arguments are validated by the C ABI layer (capi.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/spectre_amd.h"
+
+namespace sxd {
+
+int launch_partial_derivatives(int num_elements, const int32_t ext[3], int nc,
+                               const double* u, const double* inv_jac, double* du,
+                               double* dlogical, hipStream_t stream);
+int launch_sw_time_derivative(int64_t n, const double* u, const double* du,
+                              const double* gamma2, double* dt, double* gamma2_temp,
+                              hipStream_t stream);
+int launch_gh_time_derivative(int64_t n, const double* u, const double* du,
+                              const double* gammas, double* dt, double* temps,
+                              hipStream_t stream);
+int launch_sw_upwind(int64_t n, const double* u_int, const double* g2_int,
+                     const double* n_int, const double* u_ext, const double* g2_ext,
+                     const double* n_ext, double* bc, hipStream_t stream);
+int launch_gh_upwind(int64_t n, const double* u_int, const double* t_int,
+                     const double* n_int, const double* u_ext, const double* t_ext,
+                     const double* n_ext, double* bc, hipStream_t stream);
+int launch_lift_flux(int64_t n, int nc, double* bc, int n_perp, const double* mag,
+                     hipStream_t stream);
+int launch_sw_rhs(int num_elements, const int32_t ext[3], const int32_t* nbr,
+                  const double* u, const double* gamma2, const double* inv_jac, double* dt,
+                  double* scratch_du, hipStream_t stream);
+int launch_gh_rhs_composite(int num_elements, const int32_t ext[3], const int32_t* nbr,
+                            const double* u, const double* gammas, const double* inv_jac,
+                            double* dt, double* scratch_du, double* scratch_temps,
+                            hipStream_t stream);
+int launch_rk3_substep(int substep, int64_t n, double dt, const double* u0, const double* uk,
+                       const double* lk, double* out, hipStream_t stream);
+
+}  // namespace sxd

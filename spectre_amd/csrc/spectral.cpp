@@ -1,0 +1,122 @@
+// Legendre-Gauss-Lobatto points, weights and differentiation matrices for the
+// device tables.  Host code.
+//
+// Points: Kopriva Alg. 25 (NumericalAlgorithms/Spectral/Legendre.cpp:187-232):
+// the interior roots of q = L_{p+1} - L_{p-1}, bracketed as in
+// Legendre.cpp:207-214 and refined by safeguarded Newton steps
+// (q' = (2p+1) L_p).  Weights w = 2 / (p (p+1) L_p^2).
+// Barycentric weights: Kopriva Alg. 30 (Spectral.cpp:83-101).
+// D_ij = (b_j / b_i) / (x_i - x_j), D_ii = -sum_{j != i} D_ij
+// (Kopriva Alg. 37, Spectral.cpp:113-150).
+#include "spectral.hpp"
+
+#include <cmath>
+#include <mutex>
+
+namespace sxd {
+namespace {
+
+struct Legendre {
+  double q, l;  // q = L_{p+1} - L_{p-1}, l = L_p
+};
+
+Legendre legendre_q(int p, double x) {
+  double prev = 1.0, cur = x;  // L_0, L_1
+  for (int k = 2; k <= p; ++k) {
+    const double next = ((2.0 * k - 1.0) * x * cur - (k - 1.0) * prev) / k;
+    prev = cur;
+    cur = next;
+  }
+  const int k = p + 1;
+  const double up = ((2.0 * k - 1.0) * x * cur - (k - 1.0) * prev) / k;
+  return {up - prev, cur};
+}
+
+double interior_root(int p, int j) {
+  const double pi = 3.14159265358979323846;
+  double lo = -std::cos((j - 0.25) * pi / p - 0.375 / (p * pi * (j - 0.25)));
+  double hi = -std::cos((j + 0.75) * pi / p - 0.375 / (p * pi * (j + 0.75)));
+  const double q_lo_sign = legendre_q(p, lo).q;
+  legendre_q(p, hi);
+  double x = 0.5 * (lo + hi);
+  for (int it = 0; it < 200; ++it) {
+    const Legendre v = legendre_q(p, x);
+    if (v.q == 0.0) break;
+    if ((v.q < 0.0) == (q_lo_sign < 0.0)) {
+      lo = x;
+    } else {
+      hi = x;
+    }
+    double next = x - v.q / ((2.0 * p + 1.0) * v.l);
+    if (!(next > lo && next < hi)) next = 0.5 * (lo + hi);
+    const bool done = next == x || hi - lo <= 4.0 * 2.220446049250313e-16 * std::fabs(x);
+    x = next;
+    if (done) break;
+  }
+  const Legendre v = legendre_q(p, x);
+  const double next = x - v.q / ((2.0 * p + 1.0) * v.l);
+  if (std::fabs(legendre_q(p, next).q) < std::fabs(v.q)) x = next;
+  return x;
+}
+
+}  // namespace
+
+void lgl_points_and_weights(int n, double* x, double* w) {
+  const int p = n - 1;
+  for (int i = 0; i < n; ++i) x[i] = w[i] = 0.0;
+  if (p == 1) {
+    x[0] = -1.0;
+    x[1] = 1.0;
+    w[0] = w[1] = 1.0;
+    return;
+  }
+  x[0] = -1.0;
+  x[p] = 1.0;
+  w[0] = w[p] = 2.0 / (p * (p + 1.0));
+  for (int j = 1; j < (p + 1) / 2; ++j) {
+    const double r = interior_root(p, j);
+    const double l = legendre_q(p, r).l;
+    x[j] = r;
+    x[p - j] = -r;
+    w[j] = w[p - j] = 2.0 / (p * (p + 1.0) * (l * l));
+  }
+  if (p % 2 == 0) {
+    const double l = legendre_q(p, 0.0).l;
+    x[p / 2] = 0.0;
+    w[p / 2] = 2.0 / (p * (p + 1.0) * (l * l));
+  }
+}
+
+void lgl_differentiation_matrix(int n, double* d) {
+  double x[kMaxPoints], w[kMaxPoints], b[kMaxPoints];
+  lgl_points_and_weights(n, x, w);
+  for (int k = 0; k < n; ++k) b[k] = 1.0;
+  for (int j = 1; j < n; ++j)
+    for (int k = 0; k < j; ++k) {
+      b[k] *= x[k] - x[j];
+      b[j] *= x[j] - x[k];
+    }
+  for (int k = 0; k < n; ++k) b[k] = 1.0 / b[k];
+  for (int i = 0; i < n; ++i) {
+    double diag = 0.0;
+    for (int j = 0; j < n; ++j) {
+      if (j == i) continue;
+      d[i * n + j] = b[j] / (b[i] * (x[i] - x[j]));
+      diag -= d[i * n + j];
+    }
+    d[i * n + i] = diag;
+  }
+}
+
+const std::vector<double>& packed_differentiation_matrices() {
+  static std::vector<double> table;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    table.assign(kPackedDSize, 0.0);
+    for (int n = kMinPoints; n <= kMaxPoints; ++n)
+      lgl_differentiation_matrix(n, table.data() + packed_d_offset(n));
+  });
+  return table;
+}
+
+}  // namespace sxd

@@ -1,0 +1,75 @@
+// Point-wise ScalarWave algebra on the device: TimeDerivative and the
+// UpwindPenalty boundary correction.
+#pragma once
+
+#include "device_common.hpp"
+
+namespace sxd {
+
+// ScalarWave::TimeDerivative<3>::apply
+// (Evolution/Systems/ScalarWave/TimeDerivative.cpp:36-44).
+// u = (psi, Pi, Phi_i); du = (d_i psi, d_i Pi, d_i Phi_j at 6 + i + 3 j).
+__device__ __forceinline__ void sw_time_derivative(double dt[5], const double u[5],
+                                                   const double du[15], double gamma2) {
+  dt[0] = -u[1];
+  double s = -du[6 + 0];
+  s = s - du[6 + 4];
+  s = s - du[6 + 8];
+  dt[1] = s;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) dt[2 + d] = -du[3 + d] + gamma2 * (du[d] - u[2 + d]);
+}
+
+// Flat-space unit normal and magnitude (NormalCovectorAndMagnitude.hpp:82-90).
+__device__ __forceinline__ void flat_normalize(double n[3], double* mag) {
+  double m = n[0] * n[0];
+  m = fma(n[1], n[1], m);
+  m = fma(n[2], n[2], m);
+  *mag = sqrt(m);
+  const double one_over = 1.0 / *mag;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) n[i] = n[i] * one_over;
+}
+
+// dg_package_data, static mesh (ScalarWave/BoundaryCorrections/UpwindPenalty.cpp:60-108).
+// pk: [0]=v_psi [1..3]=v_zero [4]=v_plus [5]=v_minus [6..8]=n v_plus
+// [9..11]=n v_minus [12]=gamma2 v_psi; speeds are the constants (0, 1, -1).
+__device__ __forceinline__ void sw_package(double pk[13], const double u[5], double gamma2,
+                                           const double n[3]) {
+  const double l0 = 0.0, l1 = 1.0, l2 = -1.0;
+  const double g2psi = gamma2 * u[0];
+  double ndphi = n[0] * u[2];
+  ndphi = fma(n[1], u[3], ndphi);
+  ndphi = fma(n[2], u[4], ndphi);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) pk[1 + i] = l0 * (u[2 + i] - n[i] * ndphi);
+  pk[4] = l1 * ((u[1] + ndphi) - g2psi);
+  pk[5] = l2 * ((u[1] - ndphi) - g2psi);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    pk[6 + d] = pk[4] * n[d];
+    pk[9 + d] = pk[5] * n[d];
+  }
+  pk[0] = l0 * u[0];
+  pk[12] = g2psi * l0;
+}
+
+// dg_boundary_terms (UpwindPenalty.cpp:110-205) with the constant speeds:
+// step(-0) = 1 and step(0) = 1 for lambda_psi = lambda_zero = 0,
+// step(-1) = 0, step(1) = 1, step(1) = 1, step(-1) = 0.
+__device__ __forceinline__ void sw_boundary_terms(double bc[5], const double in[13],
+                                                  const double ex[13]) {
+  const double wpsi_i = step_fn(-0.0), wpsi_e = -step_fn(0.0);
+  const double wzero_i = wpsi_i, wzero_e = wpsi_e;
+  const double wplus_i = step_fn(-1.0), wplus_e = -step_fn(1.0);
+  const double wminus_i = step_fn(1.0), wminus_e = -step_fn(-1.0);
+  bc[0] = wpsi_e * ex[0] - wpsi_i * in[0];
+  bc[1] = 0.5 * (wplus_e * ex[4] + wminus_e * ex[5]) + wpsi_e * ex[12] -
+          0.5 * (wplus_i * in[4] + wminus_i * in[5]) - wpsi_i * in[12];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    bc[2 + d] = 0.5 * (wplus_e * ex[6 + d] - wminus_e * ex[9 + d]) + wzero_e * ex[1 + d] -
+                0.5 * (wplus_i * in[6 + d] - wminus_i * in[9 + d]) - wzero_i * in[1 + d];
+}
+
+}  // namespace sxd

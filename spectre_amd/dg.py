@@ -1,0 +1,226 @@
+"""Device entry points on torch tensors (HIP memory, current torch stream).
+
+Every function checks shapes, dtype (float64), device and contiguity on the
+host before calling the C ABI, and raises SpectreAmdError / ValueError instead
+of launching on bad input.  Layouts are SpECTRE's Variables layout
+(element-major, then component, then grid point with xi fastest).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import SpectreAmdError, check, lib
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _require(t, name, shape=None):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.dtype != torch.float64:
+        raise ValueError(f"{name} must be float64, got {t.dtype}")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name} must be a device tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+
+
+def _extents(extents):
+    ext = np.ascontiguousarray(extents, dtype=np.int32)
+    if ext.shape != (3,):
+        raise ValueError("extents must have 3 entries")
+    return ext, ext.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def _np(extents):
+    return int(np.prod(extents))
+
+
+def partial_derivatives(u, extents, inv_jac, out=None):
+    """u [E, nc, Np], inv_jac [E, 9, Np] -> du [E, 3 nc, Np] (block 3 c + i = d_i u_c).
+
+    partial_derivatives (LinearOperators/PartialDerivatives.tpp:191-241)."""
+    if u.dim() == 2:
+        u = u.unsqueeze(0)
+        inv_jac = inv_jac.unsqueeze(0)
+        squeeze = True
+    else:
+        squeeze = False
+    e, nc, npts = u.shape
+    if npts != _np(extents):
+        raise ValueError("u has the wrong number of grid points for these extents")
+    _require(u, "u")
+    _require(inv_jac, "inv_jac", (e, 9, npts))
+    if out is None:
+        out = torch.empty((e, 3 * nc, npts), dtype=torch.float64, device=u.device)
+    _require(out, "out", (e, 3 * nc, npts))
+    ext, ep = _extents(extents)
+    check(lib().sxd_partial_derivatives(e, ep, nc, _ptr(u), _ptr(inv_jac), _ptr(out), _stream()))
+    return out[0] if squeeze else out
+
+
+def logical_partial_derivatives(u, extents, out=None):
+    """u [E, nc, Np] -> [3, E, nc, Np] (d/dxi, d/deta, d/dzeta)."""
+    if u.dim() == 2:
+        u = u.unsqueeze(0)
+    e, nc, npts = u.shape
+    if npts != _np(extents):
+        raise ValueError("u has the wrong number of grid points for these extents")
+    _require(u, "u")
+    if out is None:
+        out = torch.empty((3, e, nc, npts), dtype=torch.float64, device=u.device)
+    _require(out, "out", (3, e, nc, npts))
+    ext, ep = _extents(extents)
+    check(lib().sxd_logical_partial_derivatives(e, ep, nc, _ptr(u), _ptr(out), _stream()))
+    return out
+
+
+def scalar_wave_time_derivative(u, du, gamma2):
+    """u [5, n], du [15, n], gamma2 [n] -> dt [5, n] (TimeDerivative.cpp:14-45)."""
+    n = u.shape[-1]
+    _require(u, "u", (5, n))
+    _require(du, "du", (15, n))
+    _require(gamma2, "gamma2", (n,))
+    dt = torch.empty_like(u)
+    check(lib().sxd_scalar_wave_time_derivative(n, _ptr(u), _ptr(du), _ptr(gamma2), _ptr(dt),
+                                                None, _stream()))
+    return dt
+
+
+def gh_time_derivative(u, du, gammas):
+    """u [50, n], du [150, n], gammas [3, n] -> (dt [50, n], temps [12, n])."""
+    n = u.shape[-1]
+    _require(u, "u", (50, n))
+    _require(du, "du", (150, n))
+    _require(gammas, "gammas", (3, n))
+    dt = torch.empty_like(u)
+    temps = torch.empty((12, n), dtype=torch.float64, device=u.device)
+    check(lib().sxd_gh_time_derivative(n, _ptr(u), _ptr(du), _ptr(gammas), _ptr(dt),
+                                       _ptr(temps), _stream()))
+    return dt, temps
+
+
+def scalar_wave_upwind_penalty(u_int, gamma2_int, n_int, u_ext, gamma2_ext, n_ext):
+    n = u_int.shape[-1]
+    for t, name, shape in ((u_int, "u_int", (5, n)), (gamma2_int, "gamma2_int", (n,)),
+                           (n_int, "n_int", (3, n)), (u_ext, "u_ext", (5, n)),
+                           (gamma2_ext, "gamma2_ext", (n,)), (n_ext, "n_ext", (3, n))):
+        _require(t, name, shape)
+    bc = torch.empty_like(u_int)
+    check(lib().sxd_scalar_wave_upwind_penalty(n, _ptr(u_int), _ptr(gamma2_int), _ptr(n_int),
+                                               _ptr(u_ext), _ptr(gamma2_ext), _ptr(n_ext),
+                                               _ptr(bc), _stream()))
+    return bc
+
+
+def gh_upwind_penalty(u_int, temps_int, n_int, u_ext, temps_ext, n_ext):
+    n = u_int.shape[-1]
+    for t, name, shape in ((u_int, "u_int", (50, n)), (temps_int, "temps_int", (12, n)),
+                           (n_int, "n_int", (3, n)), (u_ext, "u_ext", (50, n)),
+                           (temps_ext, "temps_ext", (12, n)), (n_ext, "n_ext", (3, n))):
+        _require(t, name, shape)
+    bc = torch.empty_like(u_int)
+    check(lib().sxd_gh_upwind_penalty(n, _ptr(u_int), _ptr(temps_int), _ptr(n_int),
+                                      _ptr(u_ext), _ptr(temps_ext), _ptr(n_ext), _ptr(bc),
+                                      _stream()))
+    return bc
+
+
+def lift_flux(bc, extent_perpendicular, magnitude_of_normal):
+    """In place: bc [nc, n] *= -0.5 N (N - 1) |n| (LiftFlux.hpp:41-62)."""
+    nc, n = bc.shape
+    _require(bc, "bc")
+    _require(magnitude_of_normal, "magnitude_of_normal", (n,))
+    check(lib().sxd_lift_flux(n, nc, _ptr(bc), int(extent_perpendicular),
+                              _ptr(magnitude_of_normal), _stream()))
+    return bc
+
+
+def rk3_substep(substep, dt, u0, uk, lk, out=None):
+    """Rk3HesthavenSsp::update_u substep (Rk3HesthavenSsp.cpp:55-81)."""
+    _require(u0, "u0")
+    _require(lk, "lk", tuple(u0.shape))
+    if uk is None:
+        uk = u0
+    _require(uk, "uk", tuple(u0.shape))
+    if out is None:
+        out = torch.empty_like(u0)
+    _require(out, "out", tuple(u0.shape))
+    check(lib().sxd_rk3_substep(int(substep), u0.numel(), float(dt), _ptr(u0), _ptr(uk),
+                                _ptr(lk), _ptr(out), _stream()))
+    return out
+
+
+class Domain:
+    """Device-side element set of a periodic brick (conforming, aligned).
+
+    Wraps sxd_domain: the neighbour table lives on the device; scalar_wave_rhs
+    and gh_rhs run ComputeTimeDerivative + ApplyBoundaryCorrections for all
+    elements in one call."""
+
+    def __init__(self, extents, neighbors):
+        nb = np.ascontiguousarray(neighbors, dtype=np.int32)
+        if nb.ndim != 2 or nb.shape[1] != 6:
+            raise ValueError("neighbors must be [E, 6]")
+        ext, ep = _extents(extents)
+        self.extents = tuple(int(x) for x in ext)
+        self.num_elements = nb.shape[0]
+        self.num_points = _np(self.extents)
+        handle = ctypes.c_void_p()
+        check(lib().sxd_domain_create(ctypes.byref(handle), self.num_elements, ep,
+                                      nb.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        self._handle = handle
+
+    @classmethod
+    def from_brick(cls, brick):
+        return cls(brick.extents, brick.neighbors)
+
+    def close(self):
+        if getattr(self, "_handle", None) is not None and self._handle.value:
+            lib().sxd_domain_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _shapes(self, nc):
+        return (self.num_elements, nc, self.num_points)
+
+    def scalar_wave_rhs(self, u, gamma2, inv_jac, out=None):
+        _require(u, "u", self._shapes(5))
+        _require(gamma2, "gamma2", (self.num_elements, self.num_points))
+        _require(inv_jac, "inv_jac", self._shapes(9))
+        if out is None:
+            out = torch.empty_like(u)
+        _require(out, "out", self._shapes(5))
+        check(lib().sxd_scalar_wave_rhs(self._handle, _ptr(u), _ptr(gamma2), _ptr(inv_jac),
+                                        _ptr(out), _stream()))
+        return out
+
+    def gh_rhs(self, u, gammas, inv_jac, out=None):
+        _require(u, "u", self._shapes(50))
+        _require(gammas, "gammas", self._shapes(3))
+        _require(inv_jac, "inv_jac", self._shapes(9))
+        if out is None:
+            out = torch.empty_like(u)
+        _require(out, "out", self._shapes(50))
+        check(lib().sxd_gh_rhs(self._handle, _ptr(u), _ptr(gammas), _ptr(inv_jac), _ptr(out),
+                               _stream()))
+        return out
+
+
+__all__ = ["SpectreAmdError", "partial_derivatives", "logical_partial_derivatives",
+           "scalar_wave_time_derivative", "gh_time_derivative", "scalar_wave_upwind_penalty",
+           "gh_upwind_penalty", "lift_flux", "rk3_substep", "Domain"]

@@ -1,0 +1,45 @@
+"""Legendre-Gauss-Lobatto tables from the native library (host functions).
+
+Mirrors spectre.Spectral's collocation_points / quadrature_weights /
+differentiation_matrix for Basis.Legendre, Quadrature.GaussLobatto
+(NumericalAlgorithms/Spectral/Python/Spectral.cpp:20-60).
+"""
+import ctypes
+import functools
+
+import numpy as np
+
+from ._lib import check, lib
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+@functools.lru_cache(maxsize=None)
+def _points_and_weights(n):
+    x = np.empty(n)
+    w = np.empty(n)
+    check(lib().sxd_lgl_points_and_weights(n, x.ctypes.data_as(_dp), w.ctypes.data_as(_dp)))
+    x.setflags(write=False)
+    w.setflags(write=False)
+    return x, w
+
+
+def lgl_points(n):
+    return _points_and_weights(n)[0]
+
+
+def lgl_weights(n):
+    return _points_and_weights(n)[1]
+
+
+@functools.lru_cache(maxsize=None)
+def differentiation_matrix(n):
+    d = np.empty((n, n))
+    check(lib().sxd_lgl_differentiation_matrix(n, d.ctypes.data_as(_dp)))
+    d.setflags(write=False)
+    return d
+
+
+# Names used by spectre.Spectral
+collocation_points = lgl_points
+quadrature_weights = lgl_weights

@@ -1,0 +1,344 @@
+"""Synthetic workloads: periodic affine bricks and analytic initial data.
+
+These build the inputs of the DG right-hand side in SpECTRE's layouts:
+
+* element ordering e = ex + Ex (ey + Ey ez); neighbours wrap periodically and
+  are listed per element in the direction order lower_xi, upper_xi,
+  lower_eta, upper_eta, lower_zeta, upper_zeta;
+* per-element fields are Variables buffers [nc][Np] with xi fastest
+  (DataStructures/Index.hpp:177-183);
+* the inverse Jacobian is 9 blocks, block j + 3 i = d xi^j / d x^i
+  (InverseJacobian<DataVector, 3, ElementLogical, Inertial>), for the affine
+  map of Domain/CoordinateMaps/Affine.cpp:23-24.
+
+Initial data:
+* ScalarWave plane wave psi = sin(k.x - w t), Pi = -dt psi, Phi_i = d_i psi
+  (PointwiseFunctions/AnalyticSolutions/WaveEquation/PlaneWave.cpp:56-107 with
+  a Sinusoid profile);
+* generalized-harmonic Kerr-Schild data: g_ab = eta_ab + 2 H l_a l_b,
+  Phi_iab = d_i g_ab (exact, forward-mode dual numbers),
+  Pi_ab = -(dt g_ab - beta^i Phi_iab) / alpha
+  (PointwiseFunctions/GeneralRelativity/GeneralizedHarmonic/Pi.cpp:15,
+  Phi.cpp:15; KerrSchild.hpp:269-272 for the metric form).
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import spectral
+
+
+# ---------------------------------------------------------------------------
+# Nested forward-mode dual numbers (value + 3 partials), over numpy arrays.
+class Dual:
+    __slots__ = ("v", "d")
+
+    def __init__(self, v, d):
+        self.v = v
+        self.d = d
+
+    @staticmethod
+    def _lift(x, like):
+        if isinstance(x, Dual):
+            return x
+        zero = _zero_like(like.v)
+        return Dual(x, [zero, zero, zero])
+
+    def __add__(self, o):
+        o = Dual._lift(o, self)
+        return Dual(self.v + o.v, [a + b for a, b in zip(self.d, o.d)])
+
+    __radd__ = __add__
+
+    def __neg__(self):
+        return Dual(-self.v, [-a for a in self.d])
+
+    def __sub__(self, o):
+        return self + (-Dual._lift(o, self))
+
+    def __rsub__(self, o):
+        return Dual._lift(o, self) - self
+
+    def __mul__(self, o):
+        o = Dual._lift(o, self)
+        return Dual(self.v * o.v, [a * o.v + self.v * b for a, b in zip(self.d, o.d)])
+
+    __rmul__ = __mul__
+
+    def __truediv__(self, o):
+        o = Dual._lift(o, self)
+        inv = 1.0 / o.v
+        val = self.v * inv
+        return Dual(val, [(a - val * b) * inv for a, b in zip(self.d, o.d)])
+
+    def __rtruediv__(self, o):
+        return Dual._lift(o, self) / self
+
+
+def _zero_like(v):
+    if isinstance(v, Dual):
+        z = _zero_like(v.v)
+        return Dual(z, [z, z, z])
+    return np.zeros_like(v) if isinstance(v, np.ndarray) else 0.0
+
+
+def dsqrt(x):
+    if isinstance(x, Dual):
+        s = dsqrt(x.v)
+        return Dual(s, [a / (2.0 * s) for a in x.d])
+    return np.sqrt(x)
+
+
+def dsin(x):
+    if isinstance(x, Dual):
+        c = dcos(x.v)
+        return Dual(dsin(x.v), [c * a for a in x.d])
+    return np.sin(x)
+
+
+def dcos(x):
+    if isinstance(x, Dual):
+        s = dsin(x.v)
+        return Dual(dcos(x.v), [-(s * a) for a in x.d])
+    return np.cos(x)
+
+
+def seed_coordinates(x, order):
+    """Cartesian coordinates as dual numbers carrying `order` levels of
+    derivatives (1: gradient, 2: gradient and Hessian)."""
+    x = [np.asarray(c, dtype=np.float64) for c in x]
+    if order == 0:
+        return x
+    inner = seed_coordinates(x, order - 1)
+    out = []
+    for k in range(3):
+        dk = []
+        for j in range(3):
+            one = _const_like(inner[k], 1.0 if j == k else 0.0)
+            dk.append(one)
+        out.append(Dual(inner[k], dk))
+    return out
+
+
+def _const_like(like, c):
+    if isinstance(like, Dual):
+        z = _const_like(like.v, 0.0)
+        return Dual(_const_like(like.v, c), [z, z, z])
+    return np.full_like(like, c)
+
+
+# ---------------------------------------------------------------------------
+# Tensor storage order (DataStructures/Tensor/Structure.hpp:98-195).
+def sym4(a, b):
+    a, b = min(a, b), max(a, b)
+    return (0, 4, 7, 9)[a] + b - a
+
+
+def sym3(a, b):
+    a, b = min(a, b), max(a, b)
+    return (0, 3, 5)[a] + b - a
+
+
+PAIRS = [(a, b) for a in range(4) for b in range(a, 4)]
+
+
+# ---------------------------------------------------------------------------
+@dataclass
+class Brick:
+    """A periodic brick of Ex x Ey x Ez affine elements with uniform extents."""
+    lower: tuple
+    upper: tuple
+    elements: tuple
+    extents: tuple
+    neighbors: np.ndarray = field(init=False)
+
+    def __post_init__(self):
+        ex, ey, ez = self.elements
+        e = np.arange(ex * ey * ez)
+        ix, iy, iz = e % ex, (e // ex) % ey, e // (ex * ey)
+        idx = lambda a, b, c: (a % ex) + ex * ((b % ey) + ey * (c % ez))
+        self.neighbors = np.stack([
+            idx(ix - 1, iy, iz), idx(ix + 1, iy, iz),
+            idx(ix, iy - 1, iz), idx(ix, iy + 1, iz),
+            idx(ix, iy, iz - 1), idx(ix, iy, iz + 1)], axis=1).astype(np.int32)
+
+    @property
+    def num_elements(self):
+        return int(np.prod(self.elements))
+
+    @property
+    def num_points(self):
+        return int(np.prod(self.extents))
+
+    @property
+    def widths(self):
+        return tuple((self.upper[d] - self.lower[d]) / self.elements[d] for d in range(3))
+
+    def coordinates(self):
+        """Inertial coordinates [E][3][Np]."""
+        n0, n1, n2 = self.extents
+        xs = [spectral.lgl_points(n) for n in self.extents]
+        ex, ey, ez = self.elements
+        h = self.widths
+        e = np.arange(self.num_elements)
+        ie = [e % ex, (e // ex) % ey, e // (ex * ey)]
+        k, j, i = np.meshgrid(np.arange(n2), np.arange(n1), np.arange(n0), indexing="ij")
+        logical = [xs[0][i.ravel()], xs[1][j.ravel()], xs[2][k.ravel()]]
+        out = np.empty((self.num_elements, 3, self.num_points))
+        for d in range(3):
+            lo = self.lower[d] + h[d] * ie[d]
+            out[:, d, :] = lo[:, None] + 0.5 * h[d] * (logical[d][None, :] + 1.0)
+        return out
+
+    def inverse_jacobian(self):
+        """[E][9][Np]: d xi^j / d x^i = 2 / h_j delta_ij at block j + 3 i."""
+        inv = np.zeros((self.num_elements, 9, self.num_points))
+        for d in range(3):
+            inv[:, d + 3 * d, :] = 2.0 / self.widths[d]
+        return inv
+
+
+# ---------------------------------------------------------------------------
+def plane_wave(coords, seed, t=0.0):
+    """ScalarWave plane wave: unit k-hat from the seed, |k| ~ U[1, 3],
+    amplitude 1, psi = sin(k.x - |k| t).  Returns u [E][5][Np]."""
+    rng = np.random.default_rng(seed)
+    khat = rng.normal(size=3)
+    khat /= np.linalg.norm(khat)
+    kmag = rng.uniform(1.0, 3.0)
+    k = kmag * khat
+    omega = kmag
+    phase = np.einsum("i,eip->ep", k, coords) - omega * t
+    u = np.empty((coords.shape[0], 5, coords.shape[2]))
+    u[:, 0] = np.sin(phase)
+    u[:, 1] = omega * np.cos(phase)
+    for i in range(3):
+        u[:, 2 + i] = k[i] * np.cos(phase)
+    return u, k
+
+
+def kerr_schild_metric(x, mass=1.0, spin=0.5):
+    """g_ab (10 comps, storage order) of Kerr-Schild with spin along +z, for
+    coordinates given as arrays or dual numbers."""
+    a = spin * mass
+    X, Y, Z = x
+    rho2 = X * X + Y * Y + Z * Z
+    half = 0.5 * (rho2 - a * a)
+    r2 = half + dsqrt(half * half + a * a * (Z * Z))
+    r = dsqrt(r2)
+    H = mass * r * r2 / (r2 * r2 + a * a * (Z * Z))
+    den = r2 + a * a
+    l = [1.0, (r * X + a * Y) / den, (r * Y - a * X) / den, Z / r]
+    eta = [-1.0, 1.0, 1.0, 1.0]
+    g = [None] * 10
+    for (p, q) in PAIRS:
+        val = 2.0 * H * l[p] * l[q]
+        if p == q:
+            val = val + eta[p]
+        g[sym4(p, q)] = val
+    return g
+
+
+def _lapse_shift_inverse(g):
+    """Lapse, shift and inverse spatial metric from g (numpy arrays)."""
+    spatial = np.empty((3, 3) + np.shape(g[0]))
+    for i in range(3):
+        for j in range(3):
+            spatial[i, j] = g[sym4(i + 1, j + 1)]
+    inv = np.linalg.inv(np.moveaxis(spatial, (0, 1), (-2, -1)))
+    inv = np.moveaxis(inv, (-2, -1), (0, 1))
+    shift = np.einsum("ij...,j...->i...", inv, np.stack([g[sym4(j + 1, 0)] for j in range(3)]))
+    lapse = np.sqrt(-g[0] + sum(shift[i] * g[sym4(i + 1, 0)] for i in range(3)))
+    return lapse, shift, inv
+
+
+def gh_kerr_schild(coords, mass=1.0, spin=0.5):
+    """Stationary Kerr-Schild GH state u [E][50][Np] (g, Pi, Phi)."""
+    xs = seed_coordinates([coords[:, 0], coords[:, 1], coords[:, 2]], 1)
+    gd = kerr_schild_metric(xs, mass, spin)
+    g = [c.v for c in gd]
+    lapse, shift, _ = _lapse_shift_inverse(g)
+    u = np.empty((coords.shape[0], 50, coords.shape[2]))
+    for p in range(10):
+        u[:, p] = g[p]
+        for i in range(3):
+            u[:, 20 + i + 3 * p] = gd[p].d[i]
+        u[:, 10 + p] = sum(shift[i] * gd[p].d[i] for i in range(3)) / lapse
+    return u
+
+
+def gh_kerr_schild_with_derivatives(coords, mass=1.0, spin=0.5):
+    """u [E][50][Np] and its exact inertial derivatives du [E][150][Np] in the
+    partial_derivatives layout (block 3 c + i)."""
+    xs = seed_coordinates([coords[:, 0], coords[:, 1], coords[:, 2]], 2)
+    gd = kerr_schild_metric(xs, mass, spin)
+    # gd[p].v.v = g, gd[p].v.d[i] = d_i g, gd[p].d[j].d[i] = d_i d_j g
+    E, _, npts = coords.shape
+    u = np.empty((E, 50, npts))
+    du = np.empty((E, 150, npts))
+    g1 = [c.v for c in gd]            # first-order duals of g
+    # lapse and shift as first-order duals: compute with dual arithmetic
+    spatial = [[g1[sym4(i + 1, j + 1)] for j in range(3)] for i in range(3)]
+    inv = _inverse3_dual(spatial)
+    shift = [sum((inv[i][j] * g1[sym4(j + 1, 0)] for j in range(1, 3)),
+                 inv[i][0] * g1[sym4(1, 0)]) for i in range(3)]
+    l2 = -g1[0]
+    for i in range(3):
+        l2 = l2 + shift[i] * g1[sym4(i + 1, 0)]
+    lapse = dsqrt(l2)
+    for p in range(10):
+        u[:, p] = gd[p].v.v
+        for i in range(3):
+            du[:, 3 * p + i] = gd[p].v.d[i]
+        for i in range(3):
+            phi_i = gd[p].d[i]          # first-order dual of d_i g_p
+            u[:, 20 + i + 3 * p] = phi_i.v
+            for m in range(3):
+                du[:, 60 + 3 * (i + 3 * p) + m] = phi_i.d[m]
+        pi = sum((shift[i] * gd[p].d[i] for i in range(1, 3)), shift[0] * gd[p].d[0]) / lapse
+        u[:, 10 + p] = pi.v
+        for m in range(3):
+            du[:, 30 + 3 * p + m] = pi.d[m]
+    return u, du
+
+
+def _inverse3_dual(m):
+    a = m[1][1] * m[2][2] - m[1][2] * m[1][2]
+    b = m[1][2] * m[0][2] - m[0][1] * m[2][2]
+    c = m[0][1] * m[1][2] - m[1][1] * m[0][2]
+    det = m[0][0] * a + m[0][1] * b + m[0][2] * c
+    inv00 = a / det
+    inv01 = b / det
+    inv02 = c / det
+    inv11 = (m[2][2] * m[0][0] - m[0][2] * m[0][2]) / det
+    inv12 = (m[0][2] * m[0][1] - m[0][0] * m[1][2]) / det
+    inv22 = (m[0][0] * m[1][1] - m[0][1] * m[0][1]) / det
+    return [[inv00, inv01, inv02], [inv01, inv11, inv12], [inv02, inv12, inv22]]
+
+
+# ---------------------------------------------------------------------------
+# Named workloads.
+def scalar_wave_plane_wave_3d(seed=0, elements=(4, 4, 4), extents=(5, 5, 5)):
+    """Plane wave on the periodic brick [-1,1]^3 with gamma2 = 1."""
+    brick = Brick((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), elements, extents)
+    u, _ = plane_wave(brick.coordinates(), seed)
+    gamma2 = np.ones((brick.num_elements, brick.num_points))
+    return brick, u, gamma2, brick.inverse_jacobian()
+
+
+def gh_kerr_schild_brick(seed=0, elements=(8, 8, 8), extents=(8, 8, 8),
+                         noise=1.0e-6, lower=(3.0, -4.0, -4.0), upper=(11.0, 4.0, 4.0)):
+    """Kerr-Schild (M = 1, chi = 0.5 along z) on the brick x in [3, 11],
+    y, z in [-4, 4] (wholly in r >= 2), with seeded Gaussian noise of relative
+    size `noise`; gamma0 = gamma2 = 1, gamma1 = -1.  Returns
+    (brick, u [E][50][Np], gammas [E][3][Np], inv_jac [E][9][Np])."""
+    brick = Brick(lower, upper, elements, extents)
+    u = gh_kerr_schild(brick.coordinates())
+    if noise:
+        rng = np.random.default_rng(seed)
+        u *= 1.0 + noise * rng.standard_normal(u.shape)
+    gammas = np.empty((brick.num_elements, 3, brick.num_points))
+    gammas[:, 0] = 1.0
+    gammas[:, 1] = -1.0
+    gammas[:, 2] = 1.0
+    return brick, u, gammas, brick.inverse_jacobian()

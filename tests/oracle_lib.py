@@ -1,0 +1,175 @@
+"""ctypes bindings of the oracle (oracle/_build/liboracle.so).
+
+Test infrastructure: the oracle is the CPU restatement of the reference path
+that the HIP kernels are checked against.  Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline use it.
+"""
+import ctypes
+import pathlib
+import subprocess
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+LIB_PATH = ROOT / "oracle" / "_build" / "liboracle.so"
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        _lib = ctypes.CDLL(str(LIB_PATH))
+    return _lib
+
+
+def _d(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def _check(rc):
+    if rc != 0:
+        raise ValueError(f"oracle call failed with status {rc}")
+
+
+def lgl_points_and_weights(n):
+    x = np.empty(n)
+    w = np.empty(n)
+    _check(lib().sxo_lgl_points_and_weights(n, _d(x), _d(w)))
+    return x, w
+
+
+def differentiation_matrix(n):
+    d = np.empty((n, n))
+    _check(lib().sxo_differentiation_matrix(n, _d(d)))
+    return d
+
+
+def partial_derivatives(u, extents, inv_jac):
+    """u [nc, Np], inv_jac [9, Np] -> du [3 nc, Np] (one element)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    inv_jac = np.ascontiguousarray(inv_jac, dtype=np.float64)
+    nc = u.shape[0]
+    du = np.empty((3 * nc, u.shape[1]))
+    ext, ep = _i(extents)
+    _check(lib().sxo_partial_derivatives(_d(u), nc, ep, _d(inv_jac), _d(du)))
+    return du
+
+
+def logical_partial_derivatives(u, extents):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    nc = u.shape[0]
+    out = np.empty((3, nc, u.shape[1]))
+    ext, ep = _i(extents)
+    _check(lib().sxo_logical_partial_derivatives(_d(u), nc, ep, _d(out)))
+    return out
+
+
+def partial_derivatives_batch(u, extents, inv_jac):
+    """u [E, nc, Np], inv_jac [E, 9, Np] -> du [E, 3 nc, Np]."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    inv_jac = np.ascontiguousarray(inv_jac, dtype=np.float64)
+    e, nc, npts = u.shape
+    du = np.empty((e, 3 * nc, npts))
+    ext, ep = _i(extents)
+    _check(lib().sxo_partial_derivatives_batch(e, _d(u), nc, ep, _d(inv_jac), _d(du)))
+    return du
+
+
+def sw_time_derivative(u, du, gamma2):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    du = np.ascontiguousarray(du, dtype=np.float64)
+    gamma2 = np.ascontiguousarray(gamma2, dtype=np.float64)
+    n = u.shape[1]
+    dt = np.empty((5, n))
+    _check(lib().sxo_sw_time_derivative(n, _d(u), _d(du), _d(gamma2), _d(dt)))
+    return dt
+
+
+def gh_time_derivative(u, du, gammas, with_temps=False):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    du = np.ascontiguousarray(du, dtype=np.float64)
+    gammas = np.ascontiguousarray(gammas, dtype=np.float64)
+    n = u.shape[1]
+    dt = np.empty((50, n))
+    temps = np.empty((12, n))
+    _check(lib().sxo_gh_time_derivative(n, _d(u), _d(du), _d(gammas), _d(dt), _d(temps)))
+    return (dt, temps) if with_temps else dt
+
+
+def gh_reference_impl_from_state(u, du, gammas):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    du = np.ascontiguousarray(du, dtype=np.float64)
+    gammas = np.ascontiguousarray(gammas, dtype=np.float64)
+    n = u.shape[1]
+    dt = np.empty((50, n))
+    _check(lib().sxo_gh_reference_impl_from_state(n, _d(u), _d(du), _d(gammas), _d(dt)))
+    return dt
+
+
+def gh_spec_golden():
+    dt = np.empty((50, 2))
+    _check(lib().sxo_gh_spec_golden(_d(dt)))
+    return dt
+
+
+def sw_upwind(u_int, g2_int, n_int, u_ext, g2_ext, n_ext):
+    args = [np.ascontiguousarray(a, dtype=np.float64)
+            for a in (u_int, g2_int, n_int, u_ext, g2_ext, n_ext)]
+    n = args[0].shape[1]
+    bc = np.empty((5, n))
+    _check(lib().sxo_sw_upwind(n, *[_d(a) for a in args], _d(bc)))
+    return bc
+
+
+def gh_upwind(u_int, t_int, n_int, u_ext, t_ext, n_ext):
+    args = [np.ascontiguousarray(a, dtype=np.float64)
+            for a in (u_int, t_int, n_int, u_ext, t_ext, n_ext)]
+    n = args[0].shape[1]
+    bc = np.empty((50, n))
+    _check(lib().sxo_gh_upwind(n, *[_d(a) for a in args], _d(bc)))
+    return bc
+
+
+def sw_rhs(extents, neighbors, u, gamma2, inv_jac):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    gamma2 = np.ascontiguousarray(gamma2, dtype=np.float64)
+    inv_jac = np.ascontiguousarray(inv_jac, dtype=np.float64)
+    dt = np.empty_like(u)
+    ext, ep = _i(extents)
+    nb, nbp = _i(neighbors)
+    _check(lib().sxo_sw_rhs(u.shape[0], ep, nbp, _d(u), _d(gamma2), _d(inv_jac), _d(dt)))
+    return dt
+
+
+def gh_rhs(extents, neighbors, u, gammas, inv_jac):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    gammas = np.ascontiguousarray(gammas, dtype=np.float64)
+    inv_jac = np.ascontiguousarray(inv_jac, dtype=np.float64)
+    dt = np.empty_like(u)
+    ext, ep = _i(extents)
+    nb, nbp = _i(neighbors)
+    _check(lib().sxo_gh_rhs(u.shape[0], ep, nbp, _d(u), _d(gammas), _d(inv_jac), _d(dt)))
+    return dt
+
+
+def rk3_substep(substep, dt_step, u0, uk, lk):
+    u0, uk, lk = (np.ascontiguousarray(a, dtype=np.float64) for a in (u0, uk, lk))
+    out = np.empty_like(u0)
+    _check(lib().sxo_rk3_substep(substep, ctypes.c_long(u0.size), ctypes.c_double(dt_step),
+                                 _d(u0), _d(uk), _d(lk), _d(out)))
+    return out

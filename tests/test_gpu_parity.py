@@ -1,0 +1,177 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the same
+seeded inputs.  Criterion: max-norm relative error
+max|gpu - oracle| / max|oracle| <= 1e-12 for every output array.  Both sides
+execute the same sequence of IEEE operations, so most results are
+bit-identical; the tests record that too.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as ol
+from spectre_amd import dg, workloads as wl
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def rel_err(gpu, ref):
+    ref = np.asarray(ref)
+    scale = np.max(np.abs(ref))
+    return float(np.max(np.abs(gpu - ref)) / (scale if scale > 0 else 1.0))
+
+
+@pytest.mark.parametrize("extents,nc,num_elements", [
+    ((5, 5, 5), 5, 3), ((8, 8, 8), 50, 2), ((4, 7, 3), 3, 4), ((12, 12, 12), 2, 1),
+    ((2, 2, 2), 1, 5), ((20, 3, 9), 2, 1)])
+def test_partial_derivatives(extents, nc, num_elements):
+    npts = int(np.prod(extents))
+    rng = np.random.default_rng(sum(extents) + nc)
+    u = rng.uniform(-1, 1, (num_elements, nc, npts))
+    inv_jac = rng.uniform(-2, 2, (num_elements, 9, npts))
+    ref = ol.partial_derivatives_batch(u, extents, inv_jac)
+    got = host(dg.partial_derivatives(dev(u), extents, dev(inv_jac)))
+    assert rel_err(got, ref) <= TOL
+    assert np.array_equal(got, ref), "expected bit-identical derivatives"
+    logical = host(dg.logical_partial_derivatives(dev(u), extents))
+    for e in range(num_elements):
+        ref_l = ol.logical_partial_derivatives(u[e], extents)
+        assert np.array_equal(logical[:, e], ref_l)
+
+
+def test_scalar_wave_time_derivative():
+    rng = np.random.default_rng(1)
+    n = 1000
+    u, du, g2 = rng.uniform(-1, 1, (5, n)), rng.uniform(-1, 1, (15, n)), rng.uniform(0, 2, n)
+    ref = ol.sw_time_derivative(u, du, g2)
+    got = host(dg.scalar_wave_time_derivative(dev(u), dev(du), dev(g2)))
+    assert rel_err(got, ref) <= TOL
+    assert np.array_equal(got, ref)
+
+
+def _gh_random_state(n, seed):
+    from test_oracle_systems import random_physical_gh_state
+    return random_physical_gh_state(n, seed)
+
+
+def test_gh_time_derivative():
+    u, du, gammas = _gh_random_state(3000, 5)
+    ref, ref_t = ol.gh_time_derivative(u, du, gammas, with_temps=True)
+    got, got_t = dg.gh_time_derivative(dev(u), dev(du), dev(gammas))
+    got, got_t = host(got), host(got_t)
+    for c in range(50):
+        assert rel_err(got[c], ref[c]) <= TOL, c
+    assert rel_err(got_t, ref_t) <= TOL
+    assert np.array_equal(got, ref), "expected bit-identical GH time derivative"
+
+
+def test_upwind_and_lift():
+    rng = np.random.default_rng(2)
+    n = 777
+    ui, ue = rng.uniform(-1, 1, (5, n)), rng.uniform(-1, 1, (5, n))
+    g2i, g2e = rng.uniform(0, 2, n), rng.uniform(0, 2, n)
+    ni = rng.normal(size=(3, n))
+    ni /= np.linalg.norm(ni, axis=0)
+    ne = -ni
+    ref = ol.sw_upwind(ui, g2i, ni, ue, g2e, ne)
+    got = host(dg.scalar_wave_upwind_penalty(dev(ui), dev(g2i), dev(ni), dev(ue), dev(g2e), dev(ne)))
+    assert np.array_equal(got, ref)
+    u1, _, gam1 = _gh_random_state(n, 8)
+    u2, _, gam2 = _gh_random_state(n, 9)
+    _, t1 = ol.gh_time_derivative(u1, np.zeros((150, n)), gam1, with_temps=True)
+    _, t2 = ol.gh_time_derivative(u2, np.zeros((150, n)), gam2, with_temps=True)
+    n1 = rng.normal(size=(3, n))
+    n2 = -n1 * rng.uniform(0.5, 1.5, n)
+    ref = ol.gh_upwind(u1, t1, n1, u2, t2, n2)
+    got = host(dg.gh_upwind_penalty(dev(u1), dev(t1), dev(n1), dev(u2), dev(t2), dev(n2)))
+    assert rel_err(got, ref) <= TOL
+    assert np.array_equal(got, ref)
+    mag = rng.uniform(0.5, 3.0, n)
+    bc = rng.uniform(-1, 1, (7, n))
+    lifted = host(dg.lift_flux(dev(bc), 6, dev(mag)))
+    np.testing.assert_array_equal(lifted, bc * ((-0.5 * 30.0) * mag))
+
+
+def test_scalar_wave_rhs_plane_wave_brick():
+    """4x4x4 elements, N = 5, [-1,1]^3 periodic (the ScalarWave headline case)."""
+    brick, u, gamma2, inv = wl.scalar_wave_plane_wave_3d(seed=0)
+    ref = ol.sw_rhs(brick.extents, brick.neighbors, u, gamma2, inv)
+    domain = dg.Domain.from_brick(brick)
+    got = host(domain.scalar_wave_rhs(dev(u), dev(gamma2), dev(inv)))
+    assert rel_err(got, ref) <= TOL
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("extents", [(6, 6, 6), (3, 5, 4), (11, 11, 11)])
+def test_scalar_wave_rhs_other_extents(extents):
+    brick = wl.Brick((-1.0, 0.0, 0.5), (1.0, 1.0, 2.0), (3, 2, 2), extents)
+    rng = np.random.default_rng(4)
+    u = rng.uniform(-1, 1, (brick.num_elements, 5, brick.num_points))
+    gamma2 = rng.uniform(0, 1, (brick.num_elements, brick.num_points))
+    inv = brick.inverse_jacobian()
+    ref = ol.sw_rhs(brick.extents, brick.neighbors, u, gamma2, inv)
+    domain = dg.Domain.from_brick(brick)
+    got = host(domain.scalar_wave_rhs(dev(u), dev(gamma2), dev(inv)))
+    assert rel_err(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("elements,extents", [((2, 2, 2), (5, 5, 5)), ((2, 3, 2), (4, 6, 5))])
+def test_gh_rhs_small(elements, extents):
+    brick, u, gammas, inv = wl.gh_kerr_schild_brick(seed=1, elements=elements, extents=extents)
+    ref = ol.gh_rhs(brick.extents, brick.neighbors, u, gammas, inv)
+    domain = dg.Domain.from_brick(brick)
+    got = host(domain.gh_rhs(dev(u), dev(gammas), dev(inv)))
+    assert rel_err(got, ref) <= TOL
+
+
+def test_gh_rhs_kerr_schild_512_elements():
+    """GH headline case: 8x8x8 elements, N = 8, Kerr-Schild + 1e-6 noise."""
+    brick, u, gammas, inv = wl.gh_kerr_schild_brick(seed=0)
+    ref = ol.gh_rhs(brick.extents, brick.neighbors, u, gammas, inv)
+    domain = dg.Domain.from_brick(brick)
+    got = host(domain.gh_rhs(dev(u), dev(gammas), dev(inv)))
+    err = rel_err(got, ref)
+    print(f"GH 512x8^3 max-norm relative error {err:.3e}, "
+          f"bit-identical fraction {np.mean(got == ref):.6f}")
+    assert err <= TOL
+
+
+def test_rk3_substeps():
+    rng = np.random.default_rng(6)
+    u0, uk, lk = (rng.uniform(-1, 1, 10007) for _ in range(3))
+    for s in range(3):
+        ref = ol.rk3_substep(s, 0.013, u0, uk, lk)
+        got = host(dg.rk3_substep(s, 0.013, dev(u0), dev(uk), dev(lk)))
+        assert np.array_equal(got, ref)
+
+
+def test_invalid_arguments_are_reported():
+    from spectre_amd import SpectreAmdError
+    u = torch.zeros((1, 1, 8), dtype=torch.float64, device="cuda")
+    inv = torch.zeros((1, 9, 8), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        dg.partial_derivatives(u, (2, 2, 3), inv)
+    from spectre_amd._lib import lib
+    import ctypes
+    ext = (ctypes.c_int32 * 3)(2, 2, 2)
+    host_buf = (ctypes.c_double * 8)()
+    rc = lib().sxd_partial_derivatives(1, ext, 1, ctypes.cast(host_buf, ctypes.c_void_p),
+                                       ctypes.c_void_p(inv.data_ptr()),
+                                       ctypes.c_void_p(u.data_ptr()), None)
+    assert rc == -1 and b"not a device pointer" in lib().sxd_last_error()
+    ext_bad = (ctypes.c_int32 * 3)(2, 21, 2)
+    rc = lib().sxd_partial_derivatives(1, ext_bad, 1, ctypes.c_void_p(u.data_ptr()),
+                                       ctypes.c_void_p(inv.data_ptr()),
+                                       ctypes.c_void_p(u.data_ptr()), None)
+    assert rc == -1
+    with pytest.raises(SpectreAmdError):
+        dg.Domain((5, 5, 5), np.full((2, 6), 7, dtype=np.int32))
