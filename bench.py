@@ -1,0 +1,155 @@
+#!/usr/bin/env python3
+"""Benchmark of the DG right-hand side on one MI355X.
+
+A step is one full DG time derivative (volume terms + UpwindPenalty boundary
+corrections + LGL lift) of every element of the workload, with all inputs
+resident in HBM.  Default workload: generalized harmonic, 512 elements
+(8x8x8 periodic brick, x in [3,11], y, z in [-4,4]), 8^3 LGL points each,
+50 evolved components, Kerr-Schild (M = 1, chi = 0.5) plus 1e-6 relative
+noise, gamma0 = gamma2 = 1, gamma1 = -1, harmonic gauge.  FP64 throughout.
+
+Prints one JSON line: metric, value (grid points per second), roofline of the
+RHS kernel against HBM bandwidth (algorithmic bytes / measured time) and a CPU
+baseline (the oracle, a C++ restatement of the reference path, timed on the
+host cores over a bounded sample).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload gh|sw]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+METRIC = "DG RHS grid-pts/s FP64 on 1 GPU, % roofline; mixed-p batch gain; adjoint/forward cost"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_point(nc, n_aux, n):
+    """Compulsory HBM traffic per grid point: u (nc), inverse Jacobian (9) and
+    damping parameters (n_aux) read once, dt (nc) written once, plus the
+    neighbour's face trace of u (nc per face point, 6 N^2 face points per N^3
+    volume points)."""
+    return 8.0 * (nc + 9 + n_aux + nc) + 8.0 * nc * (6.0 * n * n / n ** 3)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", choices=["gh", "sw"], default="gh")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="wall-clock budget of the CPU baseline sample (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not "
+              "supported (the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    root = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, root)
+    from spectre_amd import dg, workloads as wl
+
+    if args.workload == "gh":
+        brick, u, aux, inv = wl.gh_kerr_schild_brick(seed=0)
+        nc, n_aux = 50, 3
+        name = "gh_kerr_schild_brick_512el_8^3"
+    else:
+        brick, u, aux, inv = wl.scalar_wave_plane_wave_3d(seed=0)
+        nc, n_aux = 5, 1
+        name = "scalar_wave_plane_wave_64el_5^3"
+    n = brick.extents[0]
+    points = brick.num_elements * brick.num_points
+
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    d_u, d_aux, d_inv = dev(u), dev(aux), dev(inv)
+    d_dt = torch.empty_like(d_u)
+    domain = dg.Domain.from_brick(brick)
+    rhs = domain.gh_rhs if args.workload == "gh" else domain.scalar_wave_rhs
+
+    for _ in range(args.warmup):
+        rhs(d_u, d_aux, d_inv, out=d_dt)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        rhs(d_u, d_aux, d_inv, out=d_dt)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = start.elapsed_time(stop) / args.steps
+    value = points / (ms_per_step * 1e-3)
+
+    bpp = algorithmic_bytes_per_point(nc, n_aux, n)
+    bytes_per_launch = bpp * points
+    achieved_gbs = bytes_per_launch / (ms_per_step * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved_gbs, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel": "full RHS per step",
+        "algorithmic_bytes_per_point": bpp,
+        "algorithmic_bytes_per_launch": bytes_per_launch,
+    }
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(root, "tests"))
+        import oracle_lib as ol
+        fn1 = ol.gh_rhs if args.workload == "gh" else ol.sw_rhs
+        fnt = ol.gh_rhs_threads if args.workload == "gh" else ol.sw_rhs_threads
+        out = np.empty_like(u)
+        t = time.perf_counter()
+        fn1(brick.extents, brick.neighbors, u, aux, inv)
+        single = points / (time.perf_counter() - t)
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        reps = 0
+        t = time.perf_counter()
+        while True:
+            fnt(threads, brick.extents, brick.neighbors, u, aux, inv, out=out)
+            reps += 1
+            if time.perf_counter() - t >= args.cpu_seconds:
+                break
+        multi = reps * points / (time.perf_counter() - t)
+        cpu = {"value": round(multi, 1), "unit": "grid-points/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{reps} full RHS evaluations of the same workload on {threads} "
+                         f"host threads (elements split over threads); 1-thread rate "
+                         f"{single:.4g} pts/s from one evaluation",
+               "single_core_value": round(single, 1)}
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "grid-points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": name, "elements": brick.num_elements,
+                   "extents": list(brick.extents), "grid_points": points,
+                   "components": nc, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "wall_s_timed_region": round(wall, 4),
+    }
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -713,13 +713,15 @@ inline T lift_factor(int n_perp, const T& mag) {
 
 // Full ScalarWave DG RHS on a uniform-extent periodic brick.
 //   u      [E][5][Np], gamma2 [E][Np], inv_j [E][9][Np], dt [E][5][Np]
+// The element range [e_begin, e_end) is split into the volume pass and the
+// face pass so that the CPU baseline can distribute elements over threads.
 template <class T>
-void sw_rhs(int num_elements, const int ext[3], const int* nbr, const T* u,
-            const T* gamma2, const T* inv_j, T* dt) {
+void sw_rhs_volume(int e_begin, int e_end, const int ext[3], const T* u,
+                   const T* gamma2, const T* inv_j, T* dt) {
   const size_t np = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
   std::vector<T> du(15 * np);
   // volume terms (VolumeTermsImpl.tpp:119-153)
-  for (int e = 0; e < num_elements; ++e) {
+  for (int e = e_begin; e < e_end; ++e) {
     const T* ue = u + e * 5 * np;
     partial_derivatives(ue, 5, ext, inv_j + e * 9 * np, du.data());
     for (size_t p = 0; p < np; ++p) {
@@ -730,8 +732,14 @@ void sw_rhs(int num_elements, const int ext[3], const int* nbr, const T* u,
       for (int c = 0; c < 5; ++c) dt[(e * 5 + c) * np + p] = dtp[c];
     }
   }
+}
+
+template <class T>
+void sw_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
+                  const T* u, const T* gamma2, const T* inv_j, T* dt) {
+  const size_t np = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
   // boundary corrections
-  for (int e = 0; e < num_elements; ++e) {
+  for (int e = e_begin; e < e_end; ++e) {
     for (int dir = 0; dir < 6; ++dir) {
       const int d = dir / 2, side = dir % 2;
       const int en = nbr[6 * e + dir];
@@ -771,16 +779,22 @@ void sw_rhs(int num_elements, const int ext[3], const int* nbr, const T* u,
   }
 }
 
+template <class T>
+void sw_rhs(int num_elements, const int ext[3], const int* nbr, const T* u,
+            const T* gamma2, const T* inv_j, T* dt) {
+  sw_rhs_volume(0, num_elements, ext, u, gamma2, inv_j, dt);
+  sw_rhs_faces(0, num_elements, ext, nbr, u, gamma2, inv_j, dt);
+}
+
 // Full GH DG RHS on a uniform-extent periodic brick.
 //   u [E][50][Np], gammas [E][3][Np] (gamma0, gamma1, gamma2),
 //   inv_j [E][9][Np], dt [E][50][Np]
 template <class T>
-void gh_rhs(int num_elements, const int ext[3], const int* nbr, const T* u,
-            const T* gammas, const T* inv_j, T* dt) {
+void gh_rhs_volume(int e_begin, int e_end, const int ext[3], const T* u,
+                   const T* gammas, const T* inv_j, T* dt, GhFaceTemps<T>* temps) {
   const size_t np = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
   std::vector<T> du(150 * np);
-  std::vector<GhFaceTemps<T>> temps(static_cast<size_t>(num_elements) * np);
-  for (int e = 0; e < num_elements; ++e) {
+  for (int e = e_begin; e < e_end; ++e) {
     const T* ue = u + e * 50 * np;
     partial_derivatives(ue, 50, ext, inv_j + e * 9 * np, du.data());
     for (size_t p = 0; p < np; ++p) {
@@ -793,7 +807,13 @@ void gh_rhs(int num_elements, const int ext[3], const int* nbr, const T* u,
       for (int c = 0; c < 50; ++c) dt[(e * 50 + c) * np + p] = dtp[c];
     }
   }
-  for (int e = 0; e < num_elements; ++e) {
+}
+
+template <class T>
+void gh_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
+                  const T* u, const T* inv_j, T* dt, const GhFaceTemps<T>* temps) {
+  const size_t np = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
+  for (int e = e_begin; e < e_end; ++e) {
     for (int dir = 0; dir < 6; ++dir) {
       const int d = dir / 2, side = dir % 2;
       const int en = nbr[6 * e + dir];
@@ -840,6 +860,15 @@ void gh_rhs(int num_elements, const int ext[3], const int* nbr, const T* u,
       }
     }
   }
+}
+
+template <class T>
+void gh_rhs(int num_elements, const int ext[3], const int* nbr, const T* u,
+            const T* gammas, const T* inv_j, T* dt) {
+  const size_t np = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
+  std::vector<GhFaceTemps<T>> temps(static_cast<size_t>(num_elements) * np);
+  gh_rhs_volume(0, num_elements, ext, u, gammas, inv_j, dt, temps.data());
+  gh_rhs_faces(0, num_elements, ext, nbr, u, inv_j, dt, temps.data());
 }
 
 // ---------------------------------------------------------------------------

@@ -4,6 +4,7 @@
 // negative value on an invalid argument.
 #include <cstring>
 #include <random>
+#include <thread>
 #include <vector>
 
 #include "gh_reference_impl.hpp"
@@ -14,6 +15,19 @@ bool bad_extents(const int* ext) {
   for (int d = 0; d < 3; ++d)
     if (ext[d] < 2 || ext[d] > 20) return true;
   return false;
+}
+
+template <class F>
+void run_split(int threads, int num_elements, F&& f) {
+  if (threads < 1) threads = 1;
+  if (threads > num_elements) threads = num_elements;
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t) {
+    const int b = static_cast<int>(static_cast<long>(num_elements) * t / threads);
+    const int e = static_cast<int>(static_cast<long>(num_elements) * (t + 1) / threads);
+    pool.emplace_back([&f, b, e] { f(b, e); });
+  }
+  for (auto& th : pool) th.join();
 }
 }  // namespace
 
@@ -288,6 +302,38 @@ int sxo_gh_rhs(int num_elements, const int* ext, const int* nbr, const double* u
                const double* gammas, const double* inv_j, double* dt) {
   if (bad_extents(ext) || num_elements < 1) return -1;
   sxo::gh_rhs(num_elements, ext, nbr, u, gammas, inv_j, dt);
+  return 0;
+}
+
+// The same RHS with the elements split over `threads` host threads (element
+// data parallelism, as the reference distributes elements over cores).  Used
+// for the CPU baseline.  Volume terms per element, then the face terms per
+// element; each element writes only its own dt block.
+int sxo_gh_rhs_threads(int threads, int num_elements, const int* ext, const int* nbr,
+                       const double* u, const double* gammas, const double* inv_j,
+                       double* dt) {
+  if (bad_extents(ext) || num_elements < 1) return -1;
+  const size_t np = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
+  std::vector<sxo::GhFaceTemps<double>> temps(static_cast<size_t>(num_elements) * np);
+  run_split(threads, num_elements, [&](int b, int e) {
+    sxo::gh_rhs_volume(b, e, ext, u, gammas, inv_j, dt, temps.data());
+  });
+  run_split(threads, num_elements, [&](int b, int e) {
+    sxo::gh_rhs_faces(b, e, ext, nbr, u, inv_j, dt, temps.data());
+  });
+  return 0;
+}
+
+int sxo_sw_rhs_threads(int threads, int num_elements, const int* ext, const int* nbr,
+                       const double* u, const double* gamma2, const double* inv_j,
+                       double* dt) {
+  if (bad_extents(ext) || num_elements < 1) return -1;
+  run_split(threads, num_elements, [&](int b, int e) {
+    sxo::sw_rhs_volume(b, e, ext, u, gamma2, inv_j, dt);
+  });
+  run_split(threads, num_elements, [&](int b, int e) {
+    sxo::sw_rhs_faces(b, e, ext, nbr, u, gamma2, inv_j, dt);
+  });
   return 0;
 }
 

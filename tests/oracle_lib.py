@@ -167,6 +167,32 @@ def gh_rhs(extents, neighbors, u, gammas, inv_jac):
     return dt
 
 
+def gh_rhs_threads(threads, extents, neighbors, u, gammas, inv_jac, out=None):
+    """gh_rhs with elements split over host threads (CPU baseline)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    gammas = np.ascontiguousarray(gammas, dtype=np.float64)
+    inv_jac = np.ascontiguousarray(inv_jac, dtype=np.float64)
+    dt = np.empty_like(u) if out is None else out
+    ext, ep = _i(extents)
+    nb, nbp = _i(neighbors)
+    _check(lib().sxo_gh_rhs_threads(threads, u.shape[0], ep, nbp, _d(u), _d(gammas),
+                                    _d(inv_jac), _d(dt)))
+    return dt
+
+
+def sw_rhs_threads(threads, extents, neighbors, u, gamma2, inv_jac, out=None):
+    """sw_rhs with elements split over host threads (CPU baseline)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    gamma2 = np.ascontiguousarray(gamma2, dtype=np.float64)
+    inv_jac = np.ascontiguousarray(inv_jac, dtype=np.float64)
+    dt = np.empty_like(u) if out is None else out
+    ext, ep = _i(extents)
+    nb, nbp = _i(neighbors)
+    _check(lib().sxo_sw_rhs_threads(threads, u.shape[0], ep, nbp, _d(u), _d(gamma2),
+                                    _d(inv_jac), _d(dt)))
+    return dt
+
+
 def rk3_substep(substep, dt_step, u0, uk, lk):
     u0, uk, lk = (np.ascontiguousarray(a, dtype=np.float64) for a in (u0, uk, lk))
     out = np.empty_like(u0)
