@@ -364,6 +364,10 @@ int sxd_gh_rhs(const sxd_domain* domain, const double* u, const double* gammas,
   SXD_REQUIRE_DEVICE(inv_jac);
   SXD_REQUIRE_DEVICE(dt);
   sxd_domain* d = const_cast<sxd_domain*>(domain);
+  if (gh_fused_supported(d->ext))
+    return finish(launch_gh_rhs_fused(d->num_elements, d->ext, d->d_nbr, u, gammas, inv_jac,
+                                      dt, static_cast<hipStream_t>(stream)),
+                  __func__);
   const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
   if (ensure_scratch(d, 150 * np * d->num_elements, 12 * np * d->num_elements) != SXD_OK)
     return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");

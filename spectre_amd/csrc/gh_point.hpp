@@ -67,12 +67,14 @@ __device__ __forceinline__ void gh_lapse_shift(double* lapse, double shift[3],
   *lapse = sqrt(l);
 }
 
-__device__ __forceinline__ void gh_point_setup(GhPointState& st, const double* u,
-                                               double gamma0, double gamma1,
-                                               double gamma2) {
+// u: the point's 50 fields in Variables order, in a thread-local array.  The
+// array is modified only by opaque register moves (see KEEP_OPAQUE_), never
+// in value.
+__device__ __forceinline__ void gh_point_setup(GhPointState& st, double* u, double gamma0,
+                                               double gamma1, double gamma2) {
   const double* g = u;
   const double* pi = u + 10;
-  const double* phi = u + 20;
+  double* phi = u + 20;
 #define PI_(a, b) pi[sym4(a, b)]
 #define PHI_(i, a, b) phi[(i) + 3 * sym4(a, b)]
   gh_lapse_shift(&st.lapse, st.shift, st.inv_sp, g);
@@ -101,26 +103,28 @@ __device__ __forceinline__ void gh_point_setup(GhPointState& st, const double* u
     dtg[p] = s;
   }
 #define DA_(a, b, c) ((a) == 0 ? dtg[sym4(b, c)] : PHI_((a) - 1, b, c))
-  // gr::christoffel_first_kind (Christoffel.cpp:16-28), layout k + 4 sym4(i,j)
-  double chr[40];
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-#pragma unroll
-    for (int p = 0; p < 10; ++p) {
-      const int i = pair_a(p), j = pair_b(p);
-      chr[k + 4 * p] = 0.5 * (DA_(i, j, k) + DA_(j, i, k) - DA_(k, i, j));
-    }
-#define CHR_(a, b, c) chr[(a) + 4 * sym4(b, c)]
+  // gr::christoffel_first_kind (Christoffel.cpp:16-28):
+  // Gamma(k, i, j) = 0.5 ((d_i g_jk + d_j g_ik) - d_k g_ij)
+#define GAM_(k, i, j) (0.5 * (DA_(i, j, k) + DA_(j, i, k) - DA_(k, i, j)))
+  // Compiler-only barrier: afterwards dt g and Phi are "new" values, so the
+  // Christoffel symbols formed from them are recomputed rather than kept.
+#define KEEP_OPAQUE_()                                                  \
+  do {                                                                  \
+    _Pragma("unroll") for (int q_ = 0; q_ < 10; ++q_)                    \
+        asm volatile("" : "+v"(dtg[q_]));                               \
+    _Pragma("unroll") for (int q_ = 0; q_ < 30; ++q_)                    \
+        asm volatile("" : "+v"(phi[q_]));                               \
+  } while (0)
   // trace_last_indices (Tensor/EagerMath/Trace.cpp:14-33) = gauge constraint
   double gc[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    double s = CHR_(i, 0, 0) * IG_(0, 0);
+    double s = GAM_(i, 0, 0) * IG_(0, 0);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      if (j != 0) s = fma(CHR_(i, j, j), IG_(j, j), s);
+      if (j != 0) s = fma(GAM_(i, j, j), IG_(j, j), s);
 #pragma unroll
-      for (int k = j + 1; k < 4; ++k) s = fma(2.0 * CHR_(i, j, k), IG_(j, k), s);
+      for (int k = j + 1; k < 4; ++k) s = fma(2.0 * GAM_(i, j, k), IG_(j, k), s);
     }
     gc[i] = s;
   }
@@ -131,19 +135,8 @@ __device__ __forceinline__ void gh_point_setup(GhPointState& st, const double* u
   for (int i = 0; i < 3; ++i) nv[i + 1] = -st.shift[i] * nv[0];
   st.gamma12 = gamma1 * gamma2;
   st.gamma2 = gamma2;
-  // phi_1_up (TimeDerivative.cpp:134-143)
-  double p1u[30];
-#pragma unroll
-  for (int m = 0; m < 3; ++m)
-#pragma unroll
-    for (int p = 0; p < 10; ++p) {
-      const int mu = pair_a(p), nu = pair_b(p);
-      double s = st.inv_sp[sym3(m, 0)] * PHI_(0, mu, nu);
-      s = fma(st.inv_sp[sym3(m, 1)], PHI_(1, mu, nu), s);
-      s = fma(st.inv_sp[sym3(m, 2)], PHI_(2, mu, nu), s);
-      p1u[m + 3 * p] = s;
-    }
-#define P1U_(m, a, b) p1u[(m) + 3 * sym4(a, b)]
+  // phi_1_up (TimeDerivative.cpp:134-143) is formed where it is used: in the
+  // delta loop below and in gh_point_pair().
   // pi_one_normal, half_pi_two_normals (TimeDerivative.cpp:186-200)
   double pi1n[4];
 #pragma unroll
@@ -209,51 +202,80 @@ __device__ __forceinline__ void gh_point_setup(GhPointState& st, const double* u
     }
     acc[p] = fma(-hpi2n, PI_(mu, nu), s);
   }
+  // Within one delta the terms arrive in three groups (Pi, then Phi for
+  // n = 0, 1, 2, then Christoffel for alpha = 0..3); each group needs only a
+  // 4-wide slice of a raised tensor, so those slices are formed just before
+  // use and all ten accumulators advance one term at a time.
 #pragma unroll
   for (int de = 0; de < 4; ++de) {
-    double pi2up[4];   // pi_2_up(nu, de)
-    double p3u[12];    // phi_3_up(n, nu, de) at n + 3 nu
-    double c_lo[16];   // christoffel_first_kind_3_up(mu, al, de) at mu + 4 al
-    double c_mid[16];  // christoffel_first_kind_3_up(nu, de, al) at nu + 4 al
+    {
+      double pi2up[4];  // pi_2_up(nu, de)
 #pragma unroll
-    for (int nu = 0; nu < 4; ++nu) {
-      double t = IG_(de, 0) * PI_(nu, 0);
-      t = fma(IG_(de, 1), PI_(nu, 1), t);
-      t = fma(IG_(de, 2), PI_(nu, 2), t);
-      t = fma(IG_(de, 3), PI_(nu, 3), t);
-      pi2up[nu] = t;
-#pragma unroll
-      for (int n = 0; n < 3; ++n) {
-        double v = IG_(de, 0) * PHI_(n, nu, 0);
-        v = fma(IG_(de, 1), PHI_(n, nu, 1), v);
-        v = fma(IG_(de, 2), PHI_(n, nu, 2), v);
-        v = fma(IG_(de, 3), PHI_(n, nu, 3), v);
-        p3u[n + 3 * nu] = v;
+      for (int nu = 0; nu < 4; ++nu) {
+        double t = IG_(de, 0) * PI_(nu, 0);
+        t = fma(IG_(de, 1), PI_(nu, 1), t);
+        t = fma(IG_(de, 2), PI_(nu, 2), t);
+        t = fma(IG_(de, 3), PI_(nu, 3), t);
+        pi2up[nu] = t;
       }
 #pragma unroll
-      for (int al = 0; al < 4; ++al) {
-        double c1 = IG_(de, 0) * CHR_(nu, al, 0);
-        c1 = fma(IG_(de, 1), CHR_(nu, al, 1), c1);
-        c1 = fma(IG_(de, 2), CHR_(nu, al, 2), c1);
-        c1 = fma(IG_(de, 3), CHR_(nu, al, 3), c1);
-        c_lo[nu + 4 * al] = c1;
-        double c2 = IG_(al, 0) * CHR_(nu, de, 0);
-        c2 = fma(IG_(al, 1), CHR_(nu, de, 1), c2);
-        c2 = fma(IG_(al, 2), CHR_(nu, de, 2), c2);
-        c2 = fma(IG_(al, 3), CHR_(nu, de, 3), c2);
-        c_mid[nu + 4 * al] = c2;
-      }
+      for (int p = 0; p < 10; ++p)
+        acc[p] = fma(-(2.0 * PI_(pair_a(p), de)), pi2up[pair_b(p)], acc[p]);
     }
 #pragma unroll
-    for (int p = 0; p < 10; ++p) {
-      const int mu = pair_a(p), nu = pair_b(p);
-      double s = acc[p];
-      s = fma(-(2.0 * PI_(mu, de)), pi2up[nu], s);
+    for (int n = 0; n < 3; ++n) {
+      double p1u[4], p3u[4];  // phi_1_up(n, mu, de), phi_3_up(n, nu, de)
 #pragma unroll
-      for (int n = 0; n < 3; ++n) s = fma(2.0 * P1U_(n, mu, de), p3u[n + 3 * nu], s);
+      for (int mu = 0; mu < 4; ++mu) {
+        double s = st.inv_sp[sym3(n, 0)] * PHI_(0, mu, de);
+        s = fma(st.inv_sp[sym3(n, 1)], PHI_(1, mu, de), s);
+        s = fma(st.inv_sp[sym3(n, 2)], PHI_(2, mu, de), s);
+        p1u[mu] = s;
+        double v = IG_(de, 0) * PHI_(n, mu, 0);
+        v = fma(IG_(de, 1), PHI_(n, mu, 1), v);
+        v = fma(IG_(de, 2), PHI_(n, mu, 2), v);
+        v = fma(IG_(de, 3), PHI_(n, mu, 3), v);
+        p3u[mu] = v;
+      }
 #pragma unroll
-      for (int al = 0; al < 4; ++al) s = fma(-(2.0 * c_lo[mu + 4 * al]), c_mid[nu + 4 * al], s);
-      acc[p] = s;
+      for (int p = 0; p < 10; ++p)
+        acc[p] = fma(2.0 * p1u[pair_a(p)], p3u[pair_b(p)], acc[p]);
+    }
+    // Christoffel symbols are re-formed from dt g and Phi in every (delta,
+    // alpha) step instead of being kept (40 values) across the loop; the
+    // expression is the same, so the values are the same.  The opaque moves
+    // stop the compiler from merging the re-formed values with earlier ones
+    // (which would keep them all live again).
+    double gde[16];  // Gamma(mu, de, beta) at mu + 4 beta
+    KEEP_OPAQUE_();
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+      for (int be = 0; be < 4; ++be) gde[mu + 4 * be] = GAM_(mu, de, be);
+#pragma unroll
+    for (int al = 0; al < 4; ++al) {
+      // christoffel_first_kind_3_up(mu, al, de) and (nu, de, al)
+      KEEP_OPAQUE_();
+      double c_lo[4], c_mid[4];
+#pragma unroll
+      for (int mu = 0; mu < 4; ++mu) {
+        double gal[4];  // Gamma(mu, al, beta)
+#pragma unroll
+        for (int be = 0; be < 4; ++be) gal[be] = (al == de) ? gde[mu + 4 * be] : GAM_(mu, al, be);
+        double c1 = IG_(de, 0) * gal[0];
+        c1 = fma(IG_(de, 1), gal[1], c1);
+        c1 = fma(IG_(de, 2), gal[2], c1);
+        c1 = fma(IG_(de, 3), gal[3], c1);
+        c_lo[mu] = c1;
+        double c2 = IG_(al, 0) * gde[mu];
+        c2 = fma(IG_(al, 1), gde[mu + 4], c2);
+        c2 = fma(IG_(al, 2), gde[mu + 8], c2);
+        c2 = fma(IG_(al, 3), gde[mu + 12], c2);
+        c_mid[mu] = c2;
+      }
+#pragma unroll
+      for (int p = 0; p < 10; ++p)
+        acc[p] = fma(-(2.0 * c_lo[pair_a(p)]), c_mid[pair_b(p)], acc[p]);
     }
   }
 #pragma unroll
@@ -262,8 +284,9 @@ __device__ __forceinline__ void gh_point_setup(GhPointState& st, const double* u
 #undef PHI_
 #undef IG_
 #undef DA_
-#undef CHR_
-#undef P1U_
+#undef GAM_
+#undef KEEP_OPAQUE_
+
 }
 
 // One symmetric pair p = (a,b).  Inputs: g_p, Pi_p, Phi_{i,p} (i = 0..2) and
@@ -300,8 +323,12 @@ __device__ __forceinline__ void gh_point_pair(double out[5], const GhPointState&
   dtg = fma(st.shift[1], phi[1], dtg);
   dtg = fma(st.shift[2], phi[2], dtg);
   out[0] = fma(st.gamma1p1, sdc, dtg);
-  // dt Pi, continued (TimeDerivative.cpp:354-380)
-  double s = st.dt_pi_partial[p];
+  // dt Pi, continued (TimeDerivative.cpp:354-380).  The partial sum of pair p
+  // is picked with selects so a run-time p does not force the state into
+  // scratch memory.
+  double s = st.dt_pi_partial[0];
+#pragma unroll
+  for (int q = 1; q < 10; ++q) s = (p == q) ? st.dt_pi_partial[q] : s;
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     s = fma(-st.pi_one_normal[m], p1u[m], s);

@@ -34,6 +34,10 @@ int launch_gh_rhs_composite(int num_elements, const int32_t ext[3], const int32_
                             const double* u, const double* gammas, const double* inv_jac,
                             double* dt, double* scratch_du, double* scratch_temps,
                             hipStream_t stream);
+bool gh_fused_supported(const int32_t ext[3]);
+int launch_gh_rhs_fused(int num_elements, const int32_t ext[3], const int32_t* nbr,
+                        const double* u, const double* gammas, const double* inv_jac,
+                        double* dt, hipStream_t stream);
 int launch_rk3_substep(int substep, int64_t n, double dt, const double* u0, const double* uk,
                        const double* lk, double* out, hipStream_t stream);
 
