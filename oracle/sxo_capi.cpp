@@ -290,6 +290,44 @@ int sxo_gh_upwind(int npts, const double* u_int, const double* t_int,
   return 0;
 }
 
+// GH UpwindPenalty with the face quantities given directly (unit normal
+// covector and vector, lapse, shift, gamma1, gamma2 per side), as the pypp
+// oracle of the reference takes them.  side arrays: u [50][n], aux [9][n] =
+// gamma1, gamma2, lapse, shift (3), ... normals [6][n] = n_i (3), n^i (3).
+int sxo_gh_upwind_direct(int npts, const double* u_int, const double* aux_int,
+                         const double* nrm_int, const double* u_ext, const double* aux_ext,
+                         const double* nrm_ext, double* bc) {
+  for (int p = 0; p < npts; ++p) {
+    double ni[3], nvi[3], ne[3], nve[3], shi[3], she[3];
+    for (int i = 0; i < 3; ++i) {
+      ni[i] = nrm_int[i * npts + p];
+      nvi[i] = nrm_int[(3 + i) * npts + p];
+      ne[i] = nrm_ext[i * npts + p];
+      nve[i] = nrm_ext[(3 + i) * npts + p];
+      shi[i] = aux_int[(3 + i) * npts + p];
+      she[i] = aux_ext[(3 + i) * npts + p];
+    }
+    double li[4], le[4];
+    sxo::gh_char_speeds(li, aux_int[p], aux_int[2 * npts + p], shi, ni);
+    sxo::gh_char_speeds(le, aux_ext[p], aux_ext[2 * npts + p], she, ne);
+    for (int pr = 0; pr < 10; ++pr) {
+      double phii[3], phie[3], pki[13], pke[13], b[5];
+      for (int i = 0; i < 3; ++i) {
+        phii[i] = u_int[(20 + i + 3 * pr) * npts + p];
+        phie[i] = u_ext[(20 + i + 3 * pr) * npts + p];
+      }
+      sxo::gh_package_pair(pki, u_int[pr * npts + p], u_int[(10 + pr) * npts + p], phii,
+                           aux_int[npts + p], li, ni, nvi);
+      sxo::gh_package_pair(pke, u_ext[pr * npts + p], u_ext[(10 + pr) * npts + p], phie,
+                           aux_ext[npts + p], le, ne, nve);
+      sxo::gh_boundary_terms_pair(b, pki, li, pke, le);
+      const int comp[5] = {pr, 10 + pr, 20 + 3 * pr, 21 + 3 * pr, 22 + 3 * pr};
+      for (int c = 0; c < 5; ++c) bc[comp[c] * npts + p] = b[c];
+    }
+  }
+  return 0;
+}
+
 // Full DG RHS on a uniform periodic brick.  nbr [6 E].
 int sxo_sw_rhs(int num_elements, const int* ext, const int* nbr, const double* u,
                const double* gamma2, const double* inv_j, double* dt) {

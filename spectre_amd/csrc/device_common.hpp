@@ -41,6 +41,16 @@ __host__ __device__ constexpr int pair_b(int p) {
 // InverseJacobian(logical j, inertial i) block.
 __host__ __device__ constexpr int ij3(int j, int i) { return j + 3 * i; }
 
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also
+// releases global memory, which makes every wave wait for its outstanding
+// global loads (s_waitcnt vmcnt(0)) at the barrier; with a local-address-space
+// fence, prefetches of the next pass stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Heaviside step of UpwindPenalty (Blaze/StepFunction.hpp:46-48).
 __device__ __forceinline__ double step_fn(double x) { return x < 0.0 ? 0.0 : 1.0; }
 

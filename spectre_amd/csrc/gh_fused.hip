@@ -1,22 +1,29 @@
 // Fused generalized-harmonic DG right-hand side: one workgroup per element,
-// one thread per grid point, every term of ComputeTimeDerivative and
-// ApplyBoundaryCorrections in a single pass over HBM.
+// every term of ComputeTimeDerivative and ApplyBoundaryCorrections in a single
+// pass over HBM.
+//
+// Register budget.  The point-wise GH algebra keeps ~300 FP64 values live per
+// grid point.  A CDNA4 SIMD has 512 VGPRs per lane, so the kernel runs one
+// wave per SIMD (256 threads per element) and each thread owns P = ceil(Np /
+// 256) grid points (P = 2 at 8^3): nothing spills to scratch memory, and the
+// two points give each thread two independent instruction streams.
 //
 // Phases:
-//  A  point-wise setup.  Each thread loads its point's 50 fields and runs
-//     gh_point_setup(): lapse, shift, inverse metrics, Christoffel symbols and
-//     the algebraic part of dt Pi_ab, all in registers.
+//  A  point-wise setup for each owned point: load the 50 fields, run
+//     gh_point_setup() (lapse, shift, inverse metrics, Christoffel symbols and
+//     the algebraic part of dt Pi_ab).
 //  F  face data.  Owners of face points write their side's unit normal,
-//     characteristic speeds, gamma2 and lift factor; threads 0 .. 6 N^2 - 1
-//     compute the neighbour side of one face point each from the neighbour's
-//     g_ab, gamma1, gamma2 and inverse Jacobian (HBM reads of face traces).
+//     characteristic speeds, gamma2 and lift factor to LDS; each face point
+//     of the element (6 N^2 slots, strided over the threads) gets the
+//     neighbour side computed from the neighbour's g_ab, gamma1, gamma2 and
+//     inverse Jacobian at the matching point (HBM reads of face traces).
 //  B  for each symmetric pair (a,b) (10 passes): stage g_ab, Pi_ab, Phi_{i,ab}
-//     of the element in LDS; face threads form the lifted UpwindPenalty
-//     correction of their face point; every thread forms the 15 derivatives
-//     of its point from LDS pencils (sum k = 0..N-1 in order, then the
-//     inverse-Jacobian contraction), the five dt components of the pair, adds
-//     the corrections of the faces it lies on (xi, eta, zeta order) and
-//     writes dt once.
+//     of the element in LDS (the next pass's values are prefetched into
+//     registers while the current pass computes); face slots form the lifted
+//     UpwindPenalty correction; every point forms its 15 derivatives from LDS
+//     pencils (sum k = 0..N-1 in order, then the inverse-Jacobian
+//     contraction), the five dt components of the pair, adds the corrections
+//     of the faces it lies on (xi, eta, zeta order) and writes dt once.
 //
 // Each output follows the reference's operation sequence (see gh_point.hpp),
 // so results equal the point-wise composite path bit for bit.
@@ -31,21 +38,23 @@
 namespace sxd {
 namespace {
 
+constexpr int kGhThreads = 256;
+
 template <int N>
 struct GhFusedLayout {
   static constexpr int NP = N * N * N;
-  static constexpr int NF = N * N;        // points per face
-  static constexpr int SLOTS = 6 * NF;    // face points of the element
-  static constexpr int kIntStride = 12;   // n[3] nv[3] lam[4] gamma2 lift
-  static constexpr int kExtStride = 11;   // n[3] nv[3] lam[4] gamma2
-  // LDS of phases F and B (phase A works in registers)
+  static constexpr int NF = N * N;                   // points per face
+  static constexpr int SLOTS = 6 * NF;               // face points of the element
+  static constexpr int THREADS = NP < kGhThreads ? (NP + 63) / 64 * 64 : kGhThreads;
+  static constexpr int P = (NP + THREADS - 1) / THREADS;          // points per thread
+  static constexpr int PF = (SLOTS + THREADS - 1) / THREADS;      // face slots per thread
+  static constexpr int kIntStride = 12;  // n[3] nv[3] lam[4] gamma2 lift
+  static constexpr int kExtStride = 11;  // n[3] nv[3] lam[4] gamma2
   static constexpr int kFaceInt = 0;
   static constexpr int kFaceExt = kFaceInt + SLOTS * kIntStride;
   static constexpr int kStage = kFaceExt + SLOTS * kExtStride;
   static constexpr int kCorr = kStage + 5 * NP;
-  static constexpr int kPhaseBSize = kCorr + SLOTS * 5;
-  static constexpr int kDoubles = kPhaseBSize;
-  static constexpr int kThreads = ((NP > SLOTS ? NP : SLOTS) + 63) / 64 * 64;
+  static constexpr int kDoubles = kCorr + SLOTS * 5;
 };
 
 // Volume index of face point (a, b) of face `dir` (= 2 d + side).
@@ -59,51 +68,68 @@ __device__ __forceinline__ int face_volume_index(int dir, int a, int b) {
 }
 
 template <int N>
-__global__ void __launch_bounds__(GhFusedLayout<N>::kThreads)
+__global__ void __launch_bounds__(GhFusedLayout<N>::THREADS, 1)
 gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ u,
                     const double* __restrict__ gammas, const double* __restrict__ inv_jac,
                     double* __restrict__ dt, const double* __restrict__ dtab) {
   using L = GhFusedLayout<N>;
-  constexpr int NP = L::NP;
-  constexpr int NF = L::NF;
+  constexpr int NP = L::NP, NF = L::NF, P = L::P, PF = L::PF, T = L::THREADS;
   __shared__ double smem[L::kDoubles];
   const int64_t e = blockIdx.x;
   const int t = threadIdx.x;
-  const bool active = t < NP;
   const double* ue = u + e * 50 * NP;
-  const int i = t % N, j = (t / N) % N, k = t / (N * N);
+
+  // points owned by this thread: pt[q] = t + q T
+  bool act[P];
+  int pi_[P], pj[P], pk[P];
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    const int pt = t + q * T;
+    act[q] = pt < NP;
+    pi_[q] = pt % N;
+    pj[q] = (pt / N) % N;
+    pk[q] = act[q] ? pt / (N * N) : 0;
+  }
 
   // ---- phase A -------------------------------------------------------------
-  GhPointState st;
-  double ij[9];
-  double gamma1 = 0.0;
-  if (active) {
-    double up[50];
+  GhPointState st[P];
+  double ij[P][9];
+  double gamma1[P];
 #pragma unroll
-    for (int c = 0; c < 50; ++c) up[c] = ue[c * NP + t];
-    const double* ga = gammas + e * 3 * NP + t;
-    gamma1 = ga[NP];
-    gh_point_setup(st, up, ga[0], gamma1, ga[2 * NP]);
+  for (int q = 0; q < P; ++q) {
+    gamma1[q] = 0.0;
+    if (act[q]) {
+      const int pt = t + q * T;
+      double up[50];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) ij[c] = inv_jac[(e * 9 + c) * NP + t];
+      for (int c = 0; c < 50; ++c) up[c] = ue[c * NP + pt];
+      const double* ga = gammas + e * 3 * NP + pt;
+      gamma1[q] = ga[NP];
+      gh_point_setup(st[q], up, ga[0], gamma1[q], ga[2 * NP]);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) ij[q][c] = inv_jac[(e * 9 + c) * NP + pt];
+    }
   }
 
   // ---- phase F -------------------------------------------------------------
-  if (active) {
-    const int idx[3] = {i, j, k};
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    if (!act[q]) continue;
+    const int idx[3] = {pi_[q], pj[q], pk[q]};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       if (idx[d] != 0 && idx[d] != N - 1) continue;
       const int side = idx[d] == 0 ? 0 : 1;
-      const int a = d == 0 ? j : i;
-      const int b = d == 2 ? j : k;
+      const int a = d == 0 ? idx[1] : idx[0];
+      const int b = d == 2 ? idx[1] : idx[2];
       double* fi = smem + L::kFaceInt + ((2 * d + side) * NF + a + N * b) * L::kIntStride;
       GhFaceSide s;
       double mag;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) s.n[c] = side == 0 ? ij[ij3(d, c)] * -1.0 : ij[ij3(d, c)];
-      gh_normalize(s.n, s.nv, &mag, st.inv_sp);
-      gh_char_speeds(s.lam, gamma1, st.lapse, st.shift, s.n);
+      for (int c = 0; c < 3; ++c)
+        s.n[c] = side == 0 ? ij[q][ij3(d, c)] * -1.0 : ij[q][ij3(d, c)];
+      gh_normalize(s.n, s.nv, &mag, st[q].inv_sp);
+      gh_char_speeds(s.lam, gamma1[q], st[q].lapse, st[q].shift, s.n);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         fi[c] = s.n[c];
@@ -111,36 +137,41 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
       }
 #pragma unroll
       for (int c = 0; c < 4; ++c) fi[6 + c] = s.lam[c];
-      fi[10] = st.gamma2;
+      fi[10] = st[q].gamma2;
       fi[11] = lift_factor(N, mag);
     }
   }
-  // neighbour side of face slot t
-  int slot_q = 0;        // neighbour volume index of this thread's face slot
-  int64_t slot_en = 0;   // neighbour element
-  const bool face_thread = t < L::SLOTS;
-  if (face_thread) {
-    const int dir = t / NF;
-    const int f = t % NF;
+  // neighbour side of the face slots s = t + f T
+  bool fact[PF];
+  int64_t nb_base[PF];  // neighbour element offset + neighbour volume index
+#pragma unroll
+  for (int f = 0; f < PF; ++f) {
+    const int slot = t + f * T;
+    fact[f] = slot < L::SLOTS;
+    nb_base[f] = 0;
+    if (!fact[f]) continue;
+    const int dir = slot / NF;
+    const int fp = slot % NF;
     const int d = dir >> 1, side = dir & 1;
-    slot_en = nbr[6 * e + dir];
-    slot_q = face_volume_index<N>(dir ^ 1, f % N, f / N);
-    const double* un = u + slot_en * 50 * NP + slot_q;
+    const int64_t en = nbr[6 * e + dir];
+    const int q = face_volume_index<N>(dir ^ 1, fp % N, fp / N);
+    nb_base[f] = en * 50 * NP + q;
+    const double* un = u + nb_base[f];
     double g[10];
 #pragma unroll
     for (int c = 0; c < 10; ++c) g[c] = un[c * NP];
     double lapse, shift[3], inv_sp[6], mag;
     gh_lapse_shift(&lapse, shift, inv_sp, g);
-    const double* gn = gammas + slot_en * 3 * NP + slot_q;
+    const double* gn = gammas + en * 3 * NP + q;
     GhFaceSide s;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      s.n[c] = inv_jac[(slot_en * 9 + ij3(d, c)) * NP + slot_q];
+      s.n[c] = inv_jac[(en * 9 + ij3(d, c)) * NP + q];
       if (side == 1) s.n[c] = s.n[c] * -1.0;
     }
     gh_normalize(s.n, s.nv, &mag, inv_sp);
     gh_char_speeds(s.lam, gn[NP], lapse, shift, s.n);
-    double* fe = smem + L::kFaceExt + t * L::kExtStride;
+    double* fe = smem + L::kFaceExt + slot * L::kExtStride;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       fe[c] = s.n[c];
@@ -151,41 +182,62 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
     fe[10] = gn[2 * NP];
   }
 
-  // differentiation matrix rows of this point
+  // differentiation matrix rows of the owned points
   const double* D = diff_matrix(dtab, N);
-  double Di[N], Dj[N], Dk[N];
+  double Di[P][N], Dj[P][N], Dk[P][N];
 #pragma unroll
-  for (int l = 0; l < N; ++l) {
-    Di[l] = active ? D[i * N + l] : 0.0;
-    Dj[l] = active ? D[j * N + l] : 0.0;
-    Dk[l] = active ? D[k * N + l] : 0.0;
-  }
+  for (int q = 0; q < P; ++q)
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      Di[q][l] = D[pi_[q] * N + l];
+      Dj[q][l] = D[pj[q] * N + l];
+      Dk[q][l] = D[pk[q] * N + l];
+    }
 
   // ---- phase B -------------------------------------------------------------
   double* stage = smem + L::kStage;
   double* corr = smem + L::kCorr;
   double* dte = dt + e * 50 * NP;
-  const int64_t nb_base = slot_en * 50 * NP + slot_q;
+  // prefetched fields of the next pass: own points and neighbour face points
+  double own_next[P][5], ext_next[PF][5];
+  auto load_pass = [&](int p) {
+    const int comp[5] = {p, 10 + p, 20 + 3 * p, 21 + 3 * p, 22 + 3 * p};
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) own_next[q][c] = act[q] ? ue[comp[c] * NP + t + q * T] : 0.0;
+#pragma unroll
+    for (int f = 0; f < PF; ++f)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) ext_next[f][c] = fact[f] ? u[nb_base[f] + comp[c] * NP] : 0.0;
+  };
+  load_pass(0);
 #pragma unroll 1
   for (int p = 0; p < 10; ++p) {
-    const int comp[5] = {p, 10 + p, 20 + 3 * p, 21 + 3 * p, 22 + 3 * p};
-    __syncthreads();  // previous pass done with stage and corr
-    if (active) {
+    lds_barrier();  // previous pass done with stage and corr
 #pragma unroll
-      for (int c = 0; c < 5; ++c) stage[c * NP + t] = ue[comp[c] * NP + t];
-    }
-    double ext_v[5];
-    if (face_thread) {
+    for (int q = 0; q < P; ++q)
+      if (act[q]) {
 #pragma unroll
-      for (int c = 0; c < 5; ++c) ext_v[c] = u[nb_base + comp[c] * NP];
-    }
-    __syncthreads();
-    if (face_thread) {
-      const int dir = t / NF;
-      const int f = t % NF;
-      const int pv = face_volume_index<N>(dir, f % N, f / N);
-      const double* fi = smem + L::kFaceInt + t * L::kIntStride;
-      const double* fe = smem + L::kFaceExt + t * L::kExtStride;
+        for (int c = 0; c < 5; ++c) stage[c * NP + t + q * T] = own_next[q][c];
+      }
+    double ext_v[PF][5];
+#pragma unroll
+    for (int f = 0; f < PF; ++f)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) ext_v[f][c] = ext_next[f][c];
+    if (p + 1 < 10) load_pass(p + 1);
+    lds_barrier();
+    // lifted boundary corrections of the face slots
+#pragma unroll
+    for (int f = 0; f < PF; ++f) {
+      if (!fact[f]) continue;
+      const int slot = t + f * T;
+      const int dir = slot / NF;
+      const int fp = slot % NF;
+      const int pv = face_volume_index<N>(dir, fp % N, fp / N);
+      const double* fi = smem + L::kFaceInt + slot * L::kIntStride;
+      const double* fe = smem + L::kFaceExt + slot * L::kExtStride;
       GhFaceSide si, se;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -206,34 +258,39 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         phi_i[c] = stage[(2 + c) * NP + pv];
-        phi_e[c] = ext_v[2 + c];
+        phi_e[c] = ext_v[f][2 + c];
       }
       gh_package_pair(pki, stage[pv], stage[NP + pv], phi_i, si);
-      gh_package_pair(pke, ext_v[0], ext_v[1], phi_e, se);
+      gh_package_pair(pke, ext_v[f][0], ext_v[f][1], phi_e, se);
       gh_boundary_terms_pair(bc, pki, si.lam, pke, se.lam);
 #pragma unroll
-      for (int c = 0; c < 5; ++c) corr[t * 5 + c] = bc[c] * fac;
+      for (int c = 0; c < 5; ++c) corr[slot * 5 + c] = bc[c] * fac;
     }
-    double out[5];
-    if (active) {
+    // volume terms of the owned points
+    double out[P][5];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      if (!act[q]) continue;
+      const int pt = t + q * T;
+      const int i = pi_[q], j = pj[q], k = pk[q];
       double dlog[15];
 #pragma unroll
       for (int c = 0; c < 5; ++c) {
         const double* sc = stage + c * NP;
-        double d0 = Di[0] * sc[N * (j + N * k)];
+        double d0 = Di[q][0] * sc[N * (j + N * k)];
 #pragma unroll
-        for (int l = 1; l < N; ++l) d0 = fma(Di[l], sc[l + N * (j + N * k)], d0);
-        double d1 = Dj[0] * sc[i + N * N * k];
+        for (int l = 1; l < N; ++l) d0 = fma(Di[q][l], sc[l + N * (j + N * k)], d0);
+        double d1 = Dj[q][0] * sc[i + N * N * k];
 #pragma unroll
-        for (int l = 1; l < N; ++l) d1 = fma(Dj[l], sc[i + N * (l + N * k)], d1);
-        double d2 = Dk[0] * sc[i + N * j];
+        for (int l = 1; l < N; ++l) d1 = fma(Dj[q][l], sc[i + N * (l + N * k)], d1);
+        double d2 = Dk[q][0] * sc[i + N * j];
 #pragma unroll
-        for (int l = 1; l < N; ++l) d2 = fma(Dk[l], sc[i + N * (j + N * l)], d2);
+        for (int l = 1; l < N; ++l) d2 = fma(Dk[q][l], sc[i + N * (j + N * l)], d2);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          double s = ij[ij3(0, a)] * d0;
-          s = fma(ij[ij3(1, a)], d1, s);
-          s = fma(ij[ij3(2, a)], d2, s);
+          double s = ij[q][ij3(0, a)] * d0;
+          s = fma(ij[q][ij3(1, a)], d1, s);
+          s = fma(ij[q][ij3(2, a)], d2, s);
           dlog[3 * c + a] = s;
         }
       }
@@ -241,27 +298,34 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
       double phi[3], dphi[9];
 #pragma unroll
       for (int n = 0; n < 3; ++n) {
-        phi[n] = stage[(2 + n) * NP + t];
+        phi[n] = stage[(2 + n) * NP + pt];
 #pragma unroll
         for (int m = 0; m < 3; ++m) dphi[m + 3 * n] = dlog[3 * (2 + n) + m];
       }
-      gh_point_pair(out, st, p, stage[t], stage[NP + t], phi, dlog, dlog + 3, dphi);
+      gh_point_pair(out[q], st[q], st[q].dt_pi_partial[0], stage[NP + pt], phi, dlog,
+                    dlog + 3, dphi);
+      // rotate so that pass p + 1 finds its partial sum at index 0
+#pragma unroll
+      for (int r = 0; r < 9; ++r) st[q].dt_pi_partial[r] = st[q].dt_pi_partial[r + 1];
     }
-    __syncthreads();  // corrections complete
-    if (active) {
-      const int idx[3] = {i, j, k};
+    lds_barrier();  // corrections complete
+    const int comp[5] = {p, 10 + p, 20 + 3 * p, 21 + 3 * p, 22 + 3 * p};
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      if (!act[q]) continue;
+      const int idx[3] = {pi_[q], pj[q], pk[q]};
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         if (idx[d] != 0 && idx[d] != N - 1) continue;
         const int side = idx[d] == 0 ? 0 : 1;
-        const int a = d == 0 ? j : i;
-        const int b = d == 2 ? j : k;
+        const int a = d == 0 ? idx[1] : idx[0];
+        const int b = d == 2 ? idx[1] : idx[2];
         const double* cr = corr + ((2 * d + side) * NF + a + N * b) * 5;
 #pragma unroll
-        for (int c = 0; c < 5; ++c) out[c] = out[c] + cr[c];
+        for (int c = 0; c < 5; ++c) out[q][c] = out[q][c] + cr[c];
       }
 #pragma unroll
-      for (int c = 0; c < 5; ++c) dte[comp[c] * NP + t] = out[c];
+      for (int c = 0; c < 5; ++c) dte[comp[c] * NP + t + q * T] = out[q][c];
     }
   }
 }
@@ -271,8 +335,8 @@ int launch_gh_fused_n(int num_elements, const int32_t* nbr, const double* u,
                       const double* gammas, const double* inv_jac, double* dt,
                       const double* dtab, hipStream_t stream) {
   hipLaunchKernelGGL(gh_rhs_fused_kernel<N>, dim3(num_elements),
-                     dim3(GhFusedLayout<N>::kThreads), 0, stream, nbr, u, gammas, inv_jac,
-                     dt, dtab);
+                     dim3(GhFusedLayout<N>::THREADS), 0, stream, nbr, u, gammas, inv_jac, dt,
+                     dtab);
   return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
 }
 

@@ -67,14 +67,13 @@ __device__ __forceinline__ void gh_lapse_shift(double* lapse, double shift[3],
   *lapse = sqrt(l);
 }
 
-// u: the point's 50 fields in Variables order, in a thread-local array.  The
-// array is modified only by opaque register moves (see KEEP_OPAQUE_), never
-// in value.
-__device__ __forceinline__ void gh_point_setup(GhPointState& st, double* u, double gamma0,
-                                               double gamma1, double gamma2) {
+// u: the point's 50 fields in Variables order.
+__device__ __forceinline__ void gh_point_setup(GhPointState& st, const double* u,
+                                               double gamma0, double gamma1,
+                                               double gamma2) {
   const double* g = u;
   const double* pi = u + 10;
-  double* phi = u + 20;
+  const double* phi = u + 20;
 #define PI_(a, b) pi[sym4(a, b)]
 #define PHI_(i, a, b) phi[(i) + 3 * sym4(a, b)]
   gh_lapse_shift(&st.lapse, st.shift, st.inv_sp, g);
@@ -103,28 +102,27 @@ __device__ __forceinline__ void gh_point_setup(GhPointState& st, double* u, doub
     dtg[p] = s;
   }
 #define DA_(a, b, c) ((a) == 0 ? dtg[sym4(b, c)] : PHI_((a) - 1, b, c))
-  // gr::christoffel_first_kind (Christoffel.cpp:16-28):
+  // gr::christoffel_first_kind (Christoffel.cpp:16-28), layout k + 4 sym4(i,j):
   // Gamma(k, i, j) = 0.5 ((d_i g_jk + d_j g_ik) - d_k g_ij)
-#define GAM_(k, i, j) (0.5 * (DA_(i, j, k) + DA_(j, i, k) - DA_(k, i, j)))
-  // Compiler-only barrier: afterwards dt g and Phi are "new" values, so the
-  // Christoffel symbols formed from them are recomputed rather than kept.
-#define KEEP_OPAQUE_()                                                  \
-  do {                                                                  \
-    _Pragma("unroll") for (int q_ = 0; q_ < 10; ++q_)                    \
-        asm volatile("" : "+v"(dtg[q_]));                               \
-    _Pragma("unroll") for (int q_ = 0; q_ < 30; ++q_)                    \
-        asm volatile("" : "+v"(phi[q_]));                               \
-  } while (0)
+  double chr[40];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int p = 0; p < 10; ++p) {
+      const int i = pair_a(p), j = pair_b(p);
+      chr[k + 4 * p] = 0.5 * (DA_(i, j, k) + DA_(j, i, k) - DA_(k, i, j));
+    }
+#define CHR_(a, b, c) chr[(a) + 4 * sym4(b, c)]
   // trace_last_indices (Tensor/EagerMath/Trace.cpp:14-33) = gauge constraint
   double gc[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    double s = GAM_(i, 0, 0) * IG_(0, 0);
+    double s = CHR_(i, 0, 0) * IG_(0, 0);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      if (j != 0) s = fma(GAM_(i, j, j), IG_(j, j), s);
+      if (j != 0) s = fma(CHR_(i, j, j), IG_(j, j), s);
 #pragma unroll
-      for (int k = j + 1; k < 4; ++k) s = fma(2.0 * GAM_(i, j, k), IG_(j, k), s);
+      for (int k = j + 1; k < 4; ++k) s = fma(2.0 * CHR_(i, j, k), IG_(j, k), s);
     }
     gc[i] = s;
   }
@@ -241,36 +239,21 @@ __device__ __forceinline__ void gh_point_setup(GhPointState& st, double* u, doub
       for (int p = 0; p < 10; ++p)
         acc[p] = fma(2.0 * p1u[pair_a(p)], p3u[pair_b(p)], acc[p]);
     }
-    // Christoffel symbols are re-formed from dt g and Phi in every (delta,
-    // alpha) step instead of being kept (40 values) across the loop; the
-    // expression is the same, so the values are the same.  The opaque moves
-    // stop the compiler from merging the re-formed values with earlier ones
-    // (which would keep them all live again).
-    double gde[16];  // Gamma(mu, de, beta) at mu + 4 beta
-    KEEP_OPAQUE_();
-#pragma unroll
-    for (int mu = 0; mu < 4; ++mu)
-#pragma unroll
-      for (int be = 0; be < 4; ++be) gde[mu + 4 * be] = GAM_(mu, de, be);
 #pragma unroll
     for (int al = 0; al < 4; ++al) {
       // christoffel_first_kind_3_up(mu, al, de) and (nu, de, al)
-      KEEP_OPAQUE_();
       double c_lo[4], c_mid[4];
 #pragma unroll
       for (int mu = 0; mu < 4; ++mu) {
-        double gal[4];  // Gamma(mu, al, beta)
-#pragma unroll
-        for (int be = 0; be < 4; ++be) gal[be] = (al == de) ? gde[mu + 4 * be] : GAM_(mu, al, be);
-        double c1 = IG_(de, 0) * gal[0];
-        c1 = fma(IG_(de, 1), gal[1], c1);
-        c1 = fma(IG_(de, 2), gal[2], c1);
-        c1 = fma(IG_(de, 3), gal[3], c1);
+        double c1 = IG_(de, 0) * CHR_(mu, al, 0);
+        c1 = fma(IG_(de, 1), CHR_(mu, al, 1), c1);
+        c1 = fma(IG_(de, 2), CHR_(mu, al, 2), c1);
+        c1 = fma(IG_(de, 3), CHR_(mu, al, 3), c1);
         c_lo[mu] = c1;
-        double c2 = IG_(al, 0) * gde[mu];
-        c2 = fma(IG_(al, 1), gde[mu + 4], c2);
-        c2 = fma(IG_(al, 2), gde[mu + 8], c2);
-        c2 = fma(IG_(al, 3), gde[mu + 12], c2);
+        double c2 = IG_(al, 0) * CHR_(mu, de, 0);
+        c2 = fma(IG_(al, 1), CHR_(mu, de, 1), c2);
+        c2 = fma(IG_(al, 2), CHR_(mu, de, 2), c2);
+        c2 = fma(IG_(al, 3), CHR_(mu, de, 3), c2);
         c_mid[mu] = c2;
       }
 #pragma unroll
@@ -284,23 +267,21 @@ __device__ __forceinline__ void gh_point_setup(GhPointState& st, double* u, doub
 #undef PHI_
 #undef IG_
 #undef DA_
-#undef GAM_
-#undef KEEP_OPAQUE_
+#undef CHR_
 
 }
 
-// One symmetric pair p = (a,b).  Inputs: g_p, Pi_p, Phi_{i,p} (i = 0..2) and
-// their inertial derivatives dg[m] = d_m g_p, dpi[m] = d_m Pi_p,
+// One symmetric pair p = (a,b).  Inputs: st.dt_pi_partial[p] (passed by
+// value, so a run-time p never indexes the state), Pi_p, Phi_{i,p} (i = 0..2)
+// and the inertial derivatives dg[m] = d_m g_p, dpi[m] = d_m Pi_p,
 // dphi[m + 3 n] = d_m Phi_{n,p}.  Outputs: out[0] = dt g_p, out[1] = dt Pi_p,
 // out[2 + i] = dt Phi_{i,p}.
 __device__ __forceinline__ void gh_point_pair(double out[5], const GhPointState& st,
-                                              int p, double g, double pi,
+                                              double dt_pi_partial, double pi,
                                               const double phi[3],
                                               const double dg[3],
                                               const double dpi[3],
                                               const double dphi[9]) {
-  (void)p;
-  (void)g;
   // phi_1_up(m, a, b)
   double p1u[3];
 #pragma unroll
@@ -323,12 +304,8 @@ __device__ __forceinline__ void gh_point_pair(double out[5], const GhPointState&
   dtg = fma(st.shift[1], phi[1], dtg);
   dtg = fma(st.shift[2], phi[2], dtg);
   out[0] = fma(st.gamma1p1, sdc, dtg);
-  // dt Pi, continued (TimeDerivative.cpp:354-380).  The partial sum of pair p
-  // is picked with selects so a run-time p does not force the state into
-  // scratch memory.
-  double s = st.dt_pi_partial[0];
-#pragma unroll
-  for (int q = 1; q < 10; ++q) s = (p == q) ? st.dt_pi_partial[q] : s;
+  // dt Pi, continued (TimeDerivative.cpp:354-380) from the pair's partial sum
+  double s = dt_pi_partial;
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     s = fma(-st.pi_one_normal[m], p1u[m], s);

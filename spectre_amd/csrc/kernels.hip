@@ -136,7 +136,7 @@ gh_time_derivative_kernel(int64_t n, const double* __restrict__ u,
     for (int q = 0; q < 3; ++q)
 #pragma unroll
       for (int m = 0; m < 3; ++m) dphi[m + 3 * q] = du[(60 + 3 * (q + 3 * pr) + m) * n + p];
-    gh_point_pair(out, st, pr, up[pr], up[10 + pr], phi, dg, dpi, dphi);
+    gh_point_pair(out, st, st.dt_pi_partial[pr], up[10 + pr], phi, dg, dpi, dphi);
     dt[pr * n + p] = out[0];
     dt[(10 + pr) * n + p] = out[1];
 #pragma unroll
@@ -635,7 +635,7 @@ gh_volume_batched_kernel(int np, int64_t total, const double* __restrict__ u,
     for (int q = 0; q < 3; ++q)
 #pragma unroll
       for (int m = 0; m < 3; ++m) dphi[m + 3 * q] = due[(60 + 3 * (q + 3 * pr) + m) * np];
-    gh_point_pair(out, st, pr, up[pr], up[10 + pr], phi, dg, dpi, dphi);
+    gh_point_pair(out, st, st.dt_pi_partial[pr], up[10 + pr], phi, dg, dpi, dphi);
     dte[pr * np] = out[0];
     dte[(10 + pr) * np] = out[1];
 #pragma unroll

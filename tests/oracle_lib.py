@@ -145,6 +145,16 @@ def gh_upwind(u_int, t_int, n_int, u_ext, t_ext, n_ext):
     return bc
 
 
+def gh_upwind_direct(u_int, aux_int, nrm_int, u_ext, aux_ext, nrm_ext):
+    """aux [6, n] = gamma1, gamma2, lapse, shift^i; nrm [6, n] = n_i, n^i (unit)."""
+    args = [np.ascontiguousarray(a, dtype=np.float64)
+            for a in (u_int, aux_int, nrm_int, u_ext, aux_ext, nrm_ext)]
+    n = args[0].shape[1]
+    bc = np.empty((50, n))
+    _check(lib().sxo_gh_upwind_direct(n, *[_d(a) for a in args], _d(bc)))
+    return bc
+
+
 def sw_rhs(extents, neighbors, u, gamma2, inv_jac):
     u = np.ascontiguousarray(u, dtype=np.float64)
     gamma2 = np.ascontiguousarray(gamma2, dtype=np.float64)
