@@ -24,6 +24,13 @@ import time
 METRIC = "DG RHS grid-pts/s FP64 on 1 GPU, % roofline; mixed-p batch gain; adjoint/forward cost"
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# (each scaled by what the counter reads for a plain copy of known size:
+# FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), for the workload's default kernel;
+# the summaries are profiles/gh_rhs_fused_kernel_gh_kerr_schild_512el_8cubed.md
+# and profiles/sw_rhs_fused_kernel_sw_plane_wave_64el_5cubed.md.
+MEASURED_TRAFFIC = {("gh", "fused"): 636803264.0, ("scalar_wave", "fused"): 3229690.0}
+
 
 def algorithmic_bytes_per_point(nc, n_aux, n):
     """Compulsory HBM traffic per grid point: u (nc), inverse Jacobian (9) and
@@ -41,6 +48,10 @@ def main():
     ap.add_argument("--workload", choices=["gh", "sw"], default="gh")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="wall-clock budget of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--kernel", default="tune",
+                    choices=["tune", "auto", "fused", "fused_mfma", "composite"],
+                    help="kernel organisation of the RHS; 'tune' runs the library's "
+                         "autotuner on the workload first and keeps its choice")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (a plain copy "
                          "kernel of known size, to calibrate HBM byte counters) and exit")
@@ -84,6 +95,12 @@ def main():
     d_dt = torch.empty_like(d_u)
     domain = dg.Domain.from_brick(brick)
     rhs = domain.gh_rhs if args.workload == "gh" else domain.scalar_wave_rhs
+    system = "gh" if args.workload == "gh" else "scalar_wave"
+    if args.kernel == "tune":
+        domain.autotune(system, d_u, d_aux, d_inv, d_dt, repetitions=10)
+    else:
+        domain.set_kernel(system, args.kernel)
+    kernel_used = domain.kernel(system)
 
     for _ in range(args.warmup):
         rhs(d_u, d_aux, d_inv, out=d_dt)
@@ -110,8 +127,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
-        "traffic": None,
-        "kernel": "full RHS per step",
+        "traffic": MEASURED_TRAFFIC.get((system, kernel_used)),
+        "kernel": f"{system} RHS, {kernel_used} kernel organisation",
         "algorithmic_bytes_per_point": bpp,
         "algorithmic_bytes_per_launch": bytes_per_launch,
     }
@@ -157,7 +174,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": name, "elements": brick.num_elements,
                    "extents": list(brick.extents), "grid_points": points,
-                   "components": nc, "parallelism": "single GPU"},
+                   "components": nc, "parallelism": "single GPU",
+                   "kernel": kernel_used, "kernel_choice": args.kernel},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "wall_s_timed_region": round(wall, 4),

@@ -156,6 +156,35 @@ int sxd_scalar_wave_rhs(const sxd_domain* domain, const double* u,
 int sxd_gh_rhs(const sxd_domain* domain, const double* u, const double* gammas,
                const double* inv_jac, double* dt, void* stream);
 
+/* ---- Kernel selection (no counterpart in the reference) ----------------- */
+
+/* The reference has one CPU implementation per call; here a whole-domain
+ * right-hand side can run as one of several kernel organisations with
+ * identical results (every choice is bit-for-bit the same): */
+enum {
+  SXD_KERNEL_AUTO = 0,      /* the library's default for the domain's extents */
+  SXD_KERNEL_COMPOSITE = 1, /* derivative, point-wise, face and lift launches */
+  SXD_KERNEL_FUSED = 2,     /* one launch per RHS, derivatives on the vector ALUs */
+  SXD_KERNEL_FUSED_MFMA = 3 /* one launch per RHS, derivatives on FP64 MFMA (GH) */
+};
+enum { SXD_SYSTEM_SCALAR_WAVE = 0, SXD_SYSTEM_GH = 1 };
+
+/* Pin the kernel used by sxd_scalar_wave_rhs / sxd_gh_rhs on this domain.
+ * SXD_ERR_UNSUPPORTED if the kernel cannot run these extents.  Not
+ * thread-safe against concurrent RHS calls on the same domain. */
+int sxd_domain_set_kernel(sxd_domain* domain, int system, int kernel);
+/* The kernel the next RHS call on this domain will use. */
+int sxd_domain_get_kernel(const sxd_domain* domain, int system, int* kernel);
+/* Autotuner: time every kernel available for (domain, system) on the given
+ * inputs (one warm-up, then `repetitions` timed calls each, HIP events on
+ * `stream`), keep the fastest for later calls and return it in *chosen and
+ * its mean time in *milliseconds (either may be NULL).  params is gamma2
+ * (ScalarWave) or gammas (GH); dt is overwritten.  Synchronises `stream`. */
+int sxd_domain_autotune(sxd_domain* domain, int system, const double* u,
+                        const double* params, const double* inv_jac, double* dt,
+                        int repetitions, void* stream, int* chosen,
+                        float* milliseconds);
+
 /* ---- Time stepping ------------------------------------------------------ */
 
 /* One substep of TimeSteppers::Rk3HesthavenSsp::update_u

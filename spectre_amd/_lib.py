@@ -42,7 +42,15 @@ _SIGNATURES = {
     "sxd_scalar_wave_rhs": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_gh_rhs": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_rk3_substep": (ctypes.c_int, [ctypes.c_int, _i64, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]),
+    "sxd_domain_set_kernel": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "sxd_domain_get_kernel": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+    "sxd_domain_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, _vp,
+                                           _ip, ctypes.POINTER(ctypes.c_float)]),
 }
+
+# Kernel organisations of the whole-domain right-hand sides (SXD_KERNEL_*)
+KERNELS = {"auto": 0, "composite": 1, "fused": 2, "fused_mfma": 3}
+SYSTEMS = {"scalar_wave": 0, "gh": 1}
 
 
 def exported_symbols():

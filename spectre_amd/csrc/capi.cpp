@@ -27,6 +27,8 @@ struct sxd_domain {
   size_t scratch_du_doubles = 0;
   size_t scratch_temps_doubles = 0;
   std::mutex scratch_mutex;
+  // Kernel choice per system (SXD_SYSTEM_*), SXD_KERNEL_AUTO until set.
+  int kernel[2] = {SXD_KERNEL_AUTO, SXD_KERNEL_AUTO};
 };
 
 namespace sxd {
@@ -335,6 +337,63 @@ int check_domain(const sxd_domain* d) {
 }
 }  // namespace
 
+namespace {
+// Kernels a domain can run for a system, fastest expected first; returns
+// their number (at most 3).
+int candidate_kernels(const sxd_domain* d, int system, int* k) {
+  int n = 0;
+  if (system == SXD_SYSTEM_SCALAR_WAVE) {
+    if (sw_fused_supported(d->ext)) k[n++] = SXD_KERNEL_FUSED;
+  } else {
+    if (gh_fused_supported(d->ext)) {
+      k[n++] = SXD_KERNEL_FUSED;
+      k[n++] = SXD_KERNEL_FUSED_MFMA;
+    }
+  }
+  k[n++] = SXD_KERNEL_COMPOSITE;
+  return n;
+}
+
+int resolve_kernel(const sxd_domain* d, int system) {
+  if (d->kernel[system] != SXD_KERNEL_AUTO) return d->kernel[system];
+  int k[3];
+  candidate_kernels(d, system, k);
+  return k[0];
+}
+
+int run_sw(sxd_domain* d, int kernel, const double* u, const double* gamma2,
+           const double* inv_jac, double* dt, hipStream_t stream) {
+  double* scratch = nullptr;
+  if (kernel == SXD_KERNEL_COMPOSITE) {
+    const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
+    if (ensure_scratch(d, 15 * np * d->num_elements, 0) != SXD_OK)
+      return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
+    scratch = d->d_scratch_du;
+  }
+  return launch_sw_rhs(d->num_elements, d->ext, d->d_nbr, u, gamma2, inv_jac, dt,
+                       kernel == SXD_KERNEL_FUSED, scratch, stream);
+}
+
+int run_gh(sxd_domain* d, int kernel, const double* u, const double* gammas,
+           const double* inv_jac, double* dt, hipStream_t stream) {
+  if (kernel == SXD_KERNEL_FUSED || kernel == SXD_KERNEL_FUSED_MFMA)
+    return launch_gh_rhs_fused(d->num_elements, d->ext, d->d_nbr, u, gammas, inv_jac, dt,
+                               kernel == SXD_KERNEL_FUSED ? kGhFusedValu : kGhFusedMfma,
+                               stream);
+  const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
+  if (ensure_scratch(d, 150 * np * d->num_elements, 12 * np * d->num_elements) != SXD_OK)
+    return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
+  return launch_gh_rhs_composite(d->num_elements, d->ext, d->d_nbr, u, gammas, inv_jac, dt,
+                                 d->d_scratch_du, d->d_scratch_temps, stream);
+}
+
+int run_system(sxd_domain* d, int system, int kernel, const double* u, const double* params,
+               const double* inv_jac, double* dt, hipStream_t stream) {
+  return system == SXD_SYSTEM_SCALAR_WAVE ? run_sw(d, kernel, u, params, inv_jac, dt, stream)
+                                          : run_gh(d, kernel, u, params, inv_jac, dt, stream);
+}
+}  // namespace
+
 int sxd_scalar_wave_rhs(const sxd_domain* domain, const double* u, const double* gamma2,
                         const double* inv_jac, double* dt, void* stream) {
   if (int rc = check_domain(domain); rc != SXD_OK) return rc;
@@ -343,16 +402,8 @@ int sxd_scalar_wave_rhs(const sxd_domain* domain, const double* u, const double*
   SXD_REQUIRE_DEVICE(inv_jac);
   SXD_REQUIRE_DEVICE(dt);
   sxd_domain* d = const_cast<sxd_domain*>(domain);
-  const bool fused = d->ext[0] == d->ext[1] && d->ext[1] == d->ext[2] && d->ext[0] <= 10;
-  double* scratch = nullptr;
-  if (!fused) {
-    const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
-    if (ensure_scratch(d, 15 * np * d->num_elements, 0) != SXD_OK)
-      return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
-    scratch = d->d_scratch_du;
-  }
-  return finish(launch_sw_rhs(d->num_elements, d->ext, d->d_nbr, u, gamma2, inv_jac, dt,
-                              scratch, static_cast<hipStream_t>(stream)),
+  return finish(run_sw(d, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,
+                       static_cast<hipStream_t>(stream)),
                 __func__);
 }
 
@@ -364,17 +415,86 @@ int sxd_gh_rhs(const sxd_domain* domain, const double* u, const double* gammas,
   SXD_REQUIRE_DEVICE(inv_jac);
   SXD_REQUIRE_DEVICE(dt);
   sxd_domain* d = const_cast<sxd_domain*>(domain);
-  if (gh_fused_supported(d->ext))
-    return finish(launch_gh_rhs_fused(d->num_elements, d->ext, d->d_nbr, u, gammas, inv_jac,
-                                      dt, static_cast<hipStream_t>(stream)),
-                  __func__);
-  const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
-  if (ensure_scratch(d, 150 * np * d->num_elements, 12 * np * d->num_elements) != SXD_OK)
-    return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
-  return finish(launch_gh_rhs_composite(d->num_elements, d->ext, d->d_nbr, u, gammas,
-                                        inv_jac, dt, d->d_scratch_du, d->d_scratch_temps,
-                                        static_cast<hipStream_t>(stream)),
+  return finish(run_gh(d, resolve_kernel(d, SXD_SYSTEM_GH), u, gammas, inv_jac, dt,
+                       static_cast<hipStream_t>(stream)),
                 __func__);
+}
+
+int sxd_domain_set_kernel(sxd_domain* domain, int system, int kernel) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (system != SXD_SYSTEM_SCALAR_WAVE && system != SXD_SYSTEM_GH)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "unknown system %d", system);
+  if (kernel != SXD_KERNEL_AUTO) {
+    int k[3];
+    const int n = candidate_kernels(domain, system, k);
+    bool found = false;
+    for (int i = 0; i < n; ++i) found = found || k[i] == kernel;
+    if (!found)
+      return fail(SXD_ERR_UNSUPPORTED, "kernel %d is not available for these extents", kernel);
+  }
+  domain->kernel[system] = kernel;
+  return ok();
+}
+
+int sxd_domain_get_kernel(const sxd_domain* domain, int system, int* kernel) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (system != SXD_SYSTEM_SCALAR_WAVE && system != SXD_SYSTEM_GH)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "unknown system %d", system);
+  if (kernel == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null output");
+  *kernel = resolve_kernel(domain, system);
+  return ok();
+}
+
+int sxd_domain_autotune(sxd_domain* domain, int system, const double* u, const double* params,
+                        const double* inv_jac, double* dt, int repetitions, void* stream,
+                        int* chosen, float* milliseconds) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (system != SXD_SYSTEM_SCALAR_WAVE && system != SXD_SYSTEM_GH)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "unknown system %d", system);
+  if (repetitions < 1) return fail(SXD_ERR_INVALID_ARGUMENT, "repetitions must be positive");
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(params);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  SXD_REQUIRE_DEVICE(dt);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipEvent_t ev[2];
+  if (hipEventCreate(&ev[0]) != hipSuccess) return fail(SXD_ERR_DEVICE, "hipEventCreate failed");
+  if (hipEventCreate(&ev[1]) != hipSuccess) {
+    (void)hipEventDestroy(ev[0]);
+    return fail(SXD_ERR_DEVICE, "hipEventCreate failed");
+  }
+  int best = -1;
+  float best_ms = 0.0f;
+  int rc = SXD_OK;
+  int cand[3];
+  const int ncand = candidate_kernels(domain, system, cand);
+  for (int ci = 0; ci < ncand; ++ci) {
+    const int k = cand[ci];
+    rc = run_system(domain, system, k, u, params, inv_jac, dt, s);  // warm-up
+    if (rc != SXD_OK) break;
+    if (hipEventRecord(ev[0], s) != hipSuccess) { rc = SXD_ERR_DEVICE; break; }
+    for (int r = 0; r < repetitions && rc == SXD_OK; ++r)
+      rc = run_system(domain, system, k, u, params, inv_jac, dt, s);
+    if (rc != SXD_OK) break;
+    float ms = 0.0f;
+    if (hipEventRecord(ev[1], s) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+        hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess) {
+      rc = SXD_ERR_DEVICE;
+      break;
+    }
+    ms /= static_cast<float>(repetitions);
+    if (best < 0 || ms < best_ms) {
+      best = k;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (rc != SXD_OK) return finish(rc, __func__);
+  domain->kernel[system] = best;
+  if (chosen != nullptr) *chosen = best;
+  if (milliseconds != nullptr) *milliseconds = best_ms;
+  return ok();
 }
 
 int sxd_rk3_substep(int substep, int64_t n, double dt, const double* u0, const double* uk,

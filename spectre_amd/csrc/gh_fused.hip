@@ -5,8 +5,9 @@
 // Register budget.  The point-wise GH algebra keeps ~300 FP64 values live per
 // grid point.  A CDNA4 SIMD has 512 VGPRs per lane, so the kernel runs one
 // wave per SIMD (256 threads per element) and each thread owns P = ceil(Np /
-// 256) grid points (P = 2 at 8^3): nothing spills to scratch memory, and the
-// two points give each thread two independent instruction streams.
+// 256) grid points (P = 2 at 8^3): with the vector-ALU derivative engine
+// nothing spills to scratch memory, and the two points give each thread two
+// independent instruction streams.
 //
 // Phases:
 //  A  point-wise setup for each owned point: load the 50 fields, run
@@ -20,10 +21,31 @@
 //  B  for each symmetric pair (a,b) (10 passes): stage g_ab, Pi_ab, Phi_{i,ab}
 //     of the element in LDS (the next pass's values are prefetched into
 //     registers while the current pass computes); face slots form the lifted
-//     UpwindPenalty correction; every point forms its 15 derivatives from LDS
-//     pencils (sum k = 0..N-1 in order, then the inverse-Jacobian
-//     contraction), the five dt components of the pair, adds the corrections
-//     of the faces it lies on (xi, eta, zeta order) and writes dt once.
+//     UpwindPenalty correction; every point forms the 15 derivatives of the
+//     pair, contracts them with the inverse Jacobian, forms the five dt
+//     components of the pair, adds the corrections of the faces it lies on
+//     (xi, eta, zeta order) and writes dt once.
+//
+// Two derivative engines (template parameter MFMA):
+//  - vector ALUs: each point sums its three LDS pencils, l = 0 .. N-1 in
+//    order (3 barriers per pass);
+//  - matrix cores, described next (5 barriers per pass, logical derivatives
+//    go through LDS).
+// At 8^3 the vector-ALU engine is faster: the MFMA engine needs the extra
+// LDS round trip and barriers, and the register file is already full with
+// the point-wise GH state, so few MFMA chains can be in flight.
+//
+// Derivatives on the matrix cores.  For direction d the logical derivative of
+// a component is D (N x N) times the component arranged as N x N^2 (rows:
+// the index along d, columns: the N^2 pencils).  v_mfma_f64_16x16x4f64 takes
+// the 16 x 16 A operand as block-diag(D, D, ...) over CPT = 16 / N components
+// (K = CPT N rows of the B operand are those components' pencil values), so
+// one 16 x 16 output tile holds CPT components x N outputs for 16 pencils.
+// The instruction accumulates the four K terms of each step one after the
+// other with one rounding per term, i.e. it performs exactly the sequential
+// fma chain sum_{l=0}^{N-1} D_il u_l of the reference's dgemm order (the zero
+// blocks of block-diag(D, ...) add exact zeros), so the MFMA path is bit for
+// bit the same as the vector-ALU engine.
 //
 // Each output follows the reference's operation sequence (see gh_point.hpp),
 // so results equal the point-wise composite path bit for bit.
@@ -38,23 +60,33 @@
 namespace sxd {
 namespace {
 
+typedef double mfma_acc_t __attribute__((ext_vector_type(4)));
+
 constexpr int kGhThreads = 256;
 
-template <int N>
+template <int N, bool MFMA>
 struct GhFusedLayout {
   static constexpr int NP = N * N * N;
   static constexpr int NF = N * N;                   // points per face
   static constexpr int SLOTS = 6 * NF;               // face points of the element
   static constexpr int THREADS = NP < kGhThreads ? (NP + 63) / 64 * 64 : kGhThreads;
+  static constexpr int WAVES = THREADS / 64;
   static constexpr int P = (NP + THREADS - 1) / THREADS;          // points per thread
   static constexpr int PF = (SLOTS + THREADS - 1) / THREADS;      // face slots per thread
+  // MFMA tiling of the derivative products
+  static constexpr int CPT = 16 / N;                 // components per 16-row tile
+  static constexpr int ROWS = CPT * N;               // used rows (and K) of a tile
+  static constexpr int KSTEPS = (ROWS + 3) / 4;
+  static constexpr int TILES = (NF + 15) / 16;       // 16-pencil column tiles
+  // LDS (doubles)
   static constexpr int kIntStride = 12;  // n[3] nv[3] lam[4] gamma2 lift
   static constexpr int kExtStride = 11;  // n[3] nv[3] lam[4] gamma2
   static constexpr int kFaceInt = 0;
   static constexpr int kFaceExt = kFaceInt + SLOTS * kIntStride;
   static constexpr int kStage = kFaceExt + SLOTS * kExtStride;
   static constexpr int kCorr = kStage + 5 * NP;
-  static constexpr int kDoubles = kCorr + SLOTS * 5;
+  static constexpr int kDeriv = kCorr + SLOTS * 5;   // MFMA: logical derivatives, 3 comps x 3 dirs
+  static constexpr int kDoubles = kDeriv + (MFMA ? 9 * NP : 0);
 };
 
 // Volume index of face point (a, b) of face `dir` (= 2 d + side).
@@ -67,16 +99,90 @@ __device__ __forceinline__ int face_volume_index(int dir, int a, int b) {
   return a + N * (b + N * fixed);
 }
 
+// Volume index of entry l of pencil n (n = a + N b over the two other
+// dimensions, lowest fastest) in direction d.
 template <int N>
-__global__ void __launch_bounds__(GhFusedLayout<N>::THREADS, 1)
+__device__ __forceinline__ int pencil_point(int d, int l, int n) {
+  const int a = n % N, b = n / N;
+  if (d == 0) return l + N * (a + N * b);
+  if (d == 1) return a + N * (l + N * b);
+  return a + N * (b + N * l);
+}
+
+// Logical derivatives of `ncomp` staged components (stage slots first ..
+// first + ncomp - 1) in all three directions, written to
+// deriv[(c * 3 + d) * NP + point] (c = 0 .. ncomp - 1).  The jobs
+// (direction, component group, pencil tile) are spread over the waves.
+template <int N, int NCOMP>
+__device__ __forceinline__ void mfma_logical_derivatives(const double* stage, double* deriv,
+                                                         int first, const double* afrag,
+                                                         int wave, int lane) {
+  using L = GhFusedLayout<N, true>;
+  constexpr int NP = L::NP;
+  constexpr int GROUPS = (NCOMP + L::CPT - 1) / L::CPT;
+  constexpr int JOBS = 3 * GROUPS * L::TILES;
+  constexpr int PER_WAVE = (JOBS + L::WAVES - 1) / L::WAVES;
+  constexpr int BATCH = 1;  // more chains in flight spill registers at N = 8
+  const int m_lane = lane % 16;
+  const int k_lane = lane / 16;
+#pragma unroll
+  for (int j0 = 0; j0 < PER_WAVE; j0 += BATCH) {
+    double b[BATCH][L::KSTEPS];
+    int jd[BATCH], jc0[BATCH], jnc[BATCH], jn[BATCH];
+    bool jok[BATCH];
+    // operand loads of the whole batch first
+#pragma unroll
+    for (int x = 0; x < BATCH; ++x) {
+      const int job = wave + (j0 + x) * L::WAVES;  // wave-uniform
+      jok[x] = (j0 + x < PER_WAVE) && job < JOBS;
+      const int jj = jok[x] ? job : 0;
+      jd[x] = jj % 3;
+      const int gr = (jj / 3) % GROUPS;
+      const int tl = jj / (3 * GROUPS);
+      jc0[x] = gr * L::CPT;
+      jnc[x] = min(L::CPT, NCOMP - jc0[x]);
+      jn[x] = 16 * tl + m_lane;
+#pragma unroll
+      for (int s = 0; s < L::KSTEPS; ++s) {
+        const int kk = 4 * s + k_lane;
+        const int c2 = kk / N, l = kk % N;
+        b[x][s] = (jok[x] && jn[x] < L::NF && kk < jnc[x] * N)
+                      ? stage[(first + jc0[x] + c2) * NP + pencil_point<N>(jd[x], l, jn[x])]
+                      : 0.0;
+      }
+    }
+    mfma_acc_t acc[BATCH];
+#pragma unroll
+    for (int x = 0; x < BATCH; ++x) acc[x] = mfma_acc_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < L::KSTEPS; ++s)
+#pragma unroll
+      for (int x = 0; x < BATCH; ++x)
+        acc[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(afrag[s], b[x][s], acc[x], 0, 0, 0);
+#pragma unroll
+    for (int x = 0; x < BATCH; ++x)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = 4 * r + k_lane;
+        const int c2 = m / N, i = m % N;
+        if (jok[x] && jn[x] < L::NF && m < jnc[x] * N)
+          deriv[((jc0[x] + c2) * 3 + jd[x]) * NP + pencil_point<N>(jd[x], i, jn[x])] =
+              acc[x][r];
+      }
+  }
+}
+
+template <int N, bool MFMA>
+__global__ void __launch_bounds__((GhFusedLayout<N, MFMA>::THREADS), 1)
 gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ u,
                     const double* __restrict__ gammas, const double* __restrict__ inv_jac,
                     double* __restrict__ dt, const double* __restrict__ dtab) {
-  using L = GhFusedLayout<N>;
+  using L = GhFusedLayout<N, MFMA>;
   constexpr int NP = L::NP, NF = L::NF, P = L::P, PF = L::PF, T = L::THREADS;
   __shared__ double smem[L::kDoubles];
   const int64_t e = blockIdx.x;
   const int t = threadIdx.x;
+  const int wave = t / 64, lane = t % 64;
   const double* ue = u + e * 50 * NP;
 
   // points owned by this thread: pt[q] = t + q T
@@ -182,21 +288,39 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
     fe[10] = gn[2 * NP];
   }
 
-  // differentiation matrix rows of the owned points
-  const double* D = diff_matrix(dtab, N);
-  double Di[P][N], Dj[P][N], Dk[P][N];
+  // Derivative operands.  MFMA: the A operand block-diag(D, ..., D), step s
+  // holds A[m][4 s + lane / 16] for m = lane % 16.  VALU: the rows of D that
+  // each owned point needs in the three directions.
+  constexpr int KS = MFMA ? L::KSTEPS : 1;
+  constexpr int DR = MFMA ? 1 : N;
+  double afrag[KS];
+  double Di[P][DR], Dj[P][DR], Dk[P][DR];
+  {
+    const double* D = diff_matrix(dtab, N);
+    if constexpr (MFMA) {
+      const int m = lane % 16;
 #pragma unroll
-  for (int q = 0; q < P; ++q)
+      for (int s = 0; s < L::KSTEPS; ++s) {
+        const int kk = 4 * s + lane / 16;
+        afrag[s] = (m < L::ROWS && kk < L::ROWS && m / N == kk / N) ? D[(m % N) * N + kk % N]
+                                                                     : 0.0;
+      }
+    } else {
 #pragma unroll
-    for (int l = 0; l < N; ++l) {
-      Di[q][l] = D[pi_[q] * N + l];
-      Dj[q][l] = D[pj[q] * N + l];
-      Dk[q][l] = D[pk[q] * N + l];
+      for (int q = 0; q < P; ++q)
+#pragma unroll
+        for (int l = 0; l < N; ++l) {
+          Di[q][l] = D[pi_[q] * N + l];
+          Dj[q][l] = D[pj[q] * N + l];
+          Dk[q][l] = D[pk[q] * N + l];
+        }
     }
+  }
 
   // ---- phase B -------------------------------------------------------------
   double* stage = smem + L::kStage;
   double* corr = smem + L::kCorr;
+  double* deriv = smem + L::kDeriv;
   double* dte = dt + e * 50 * NP;
   // prefetched fields of the next pass: own points and neighbour face points
   double own_next[P][5], ext_next[PF][5];
@@ -211,24 +335,24 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
 #pragma unroll
       for (int c = 0; c < 5; ++c) ext_next[f][c] = fact[f] ? u[nb_base[f] + comp[c] * NP] : 0.0;
   };
-  load_pass(0);
-#pragma unroll 1
-  for (int p = 0; p < 10; ++p) {
-    lds_barrier();  // previous pass done with stage and corr
+  // inertial derivatives of an owned point from its three logical ones
+  auto contract = [&](int q, double d0, double d1, double d2, double* out3) {
 #pragma unroll
-    for (int q = 0; q < P; ++q)
-      if (act[q]) {
-#pragma unroll
-        for (int c = 0; c < 5; ++c) stage[c * NP + t + q * T] = own_next[q][c];
-      }
-    double ext_v[PF][5];
-#pragma unroll
-    for (int f = 0; f < PF; ++f)
-#pragma unroll
-      for (int c = 0; c < 5; ++c) ext_v[f][c] = ext_next[f][c];
-    if (p + 1 < 10) load_pass(p + 1);
-    lds_barrier();
-    // lifted boundary corrections of the face slots
+    for (int a = 0; a < 3; ++a) {
+      double s = ij[q][ij3(0, a)] * d0;
+      s = fma(ij[q][ij3(1, a)], d1, s);
+      s = fma(ij[q][ij3(2, a)], d2, s);
+      out3[a] = s;
+    }
+  };
+  // ... from the logical ones of staged component c_local in LDS
+  auto contract_lds = [&](int q, int c_local, double* out3) {
+    const int pt = t + q * T;
+    contract(q, deriv[(c_local * 3 + 0) * NP + pt], deriv[(c_local * 3 + 1) * NP + pt],
+             deriv[(c_local * 3 + 2) * NP + pt], out3);
+  };
+  // lifted boundary corrections of the face slots of the staged pair
+  auto face_corrections = [&](const double* ext_v) {
 #pragma unroll
     for (int f = 0; f < PF; ++f) {
       if (!fact[f]) continue;
@@ -258,86 +382,227 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         phi_i[c] = stage[(2 + c) * NP + pv];
-        phi_e[c] = ext_v[f][2 + c];
+        phi_e[c] = ext_v[5 * f + 2 + c];
       }
       gh_package_pair(pki, stage[pv], stage[NP + pv], phi_i, si);
-      gh_package_pair(pke, ext_v[f][0], ext_v[f][1], phi_e, se);
+      gh_package_pair(pke, ext_v[5 * f], ext_v[5 * f + 1], phi_e, se);
       gh_boundary_terms_pair(bc, pki, si.lam, pke, se.lam);
 #pragma unroll
       for (int c = 0; c < 5; ++c) corr[slot * 5 + c] = bc[c] * fac;
     }
-    // volume terms of the owned points
-    double out[P][5];
+  };
+  // add the corrections of the faces point q lies on (xi, eta, zeta order)
+  auto add_corrections = [&](int q, double* out) {
+    const int idx[3] = {pi_[q], pj[q], pk[q]};
 #pragma unroll
-    for (int q = 0; q < P; ++q) {
-      if (!act[q]) continue;
-      const int pt = t + q * T;
-      const int i = pi_[q], j = pj[q], k = pk[q];
-      double dlog[15];
+    for (int d = 0; d < 3; ++d) {
+      if (idx[d] != 0 && idx[d] != N - 1) continue;
+      const int side = idx[d] == 0 ? 0 : 1;
+      const int a = d == 0 ? idx[1] : idx[0];
+      const int b = d == 2 ? idx[1] : idx[2];
+      const double* cr = corr + ((2 * d + side) * NF + a + N * b) * 5;
 #pragma unroll
-      for (int c = 0; c < 5; ++c) {
-        const double* sc = stage + c * NP;
-        double d0 = Di[q][0] * sc[N * (j + N * k)];
-#pragma unroll
-        for (int l = 1; l < N; ++l) d0 = fma(Di[q][l], sc[l + N * (j + N * k)], d0);
-        double d1 = Dj[q][0] * sc[i + N * N * k];
-#pragma unroll
-        for (int l = 1; l < N; ++l) d1 = fma(Dj[q][l], sc[i + N * (l + N * k)], d1);
-        double d2 = Dk[q][0] * sc[i + N * j];
-#pragma unroll
-        for (int l = 1; l < N; ++l) d2 = fma(Dk[q][l], sc[i + N * (j + N * l)], d2);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          double s = ij[q][ij3(0, a)] * d0;
-          s = fma(ij[q][ij3(1, a)], d1, s);
-          s = fma(ij[q][ij3(2, a)], d2, s);
-          dlog[3 * c + a] = s;
-        }
-      }
-      // dlog[3 c + m] = d_m of component c; c = 0 g, 1 Pi, 2..4 Phi_n
-      double phi[3], dphi[9];
-#pragma unroll
-      for (int n = 0; n < 3; ++n) {
-        phi[n] = stage[(2 + n) * NP + pt];
-#pragma unroll
-        for (int m = 0; m < 3; ++m) dphi[m + 3 * n] = dlog[3 * (2 + n) + m];
-      }
-      gh_point_pair(out[q], st[q], st[q].dt_pi_partial[0], stage[NP + pt], phi, dlog,
-                    dlog + 3, dphi);
-      // rotate so that pass p + 1 finds its partial sum at index 0
-#pragma unroll
-      for (int r = 0; r < 9; ++r) st[q].dt_pi_partial[r] = st[q].dt_pi_partial[r + 1];
+      for (int c = 0; c < 5; ++c) out[c] = out[c] + cr[c];
     }
-    lds_barrier();  // corrections complete
-    const int comp[5] = {p, 10 + p, 20 + 3 * p, 21 + 3 * p, 22 + 3 * p};
+  };
+  if constexpr (MFMA) {
+    load_pass(0);
+#pragma unroll 1
+    for (int p = 0; p < 10; ++p) {
+      lds_barrier();  // previous pass done with stage, corr and deriv
+      // the pair's own values stay in registers for the point-wise terms
+      double own[P][5], ext_v[PF][5];
 #pragma unroll
-    for (int q = 0; q < P; ++q) {
-      if (!act[q]) continue;
-      const int idx[3] = {pi_[q], pj[q], pk[q]};
+      for (int q = 0; q < P; ++q)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        if (idx[d] != 0 && idx[d] != N - 1) continue;
-        const int side = idx[d] == 0 ? 0 : 1;
-        const int a = d == 0 ? idx[1] : idx[0];
-        const int b = d == 2 ? idx[1] : idx[2];
-        const double* cr = corr + ((2 * d + side) * NF + a + N * b) * 5;
+        for (int c = 0; c < 5; ++c) {
+          own[q][c] = own_next[q][c];
+          if (act[q]) stage[c * NP + t + q * T] = own[q][c];
+        }
 #pragma unroll
-        for (int c = 0; c < 5; ++c) out[q][c] = out[q][c] + cr[c];
+      for (int f = 0; f < PF; ++f)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) ext_v[f][c] = ext_next[f][c];
+      if (p + 1 < 10) load_pass(p + 1);
+      lds_barrier();  // stage complete
+      const int comp[5] = {p, 10 + p, 20 + 3 * p, 21 + 3 * p, 22 + 3 * p};
+      double out[P][5];
+      mfma_logical_derivatives<N, 2>(stage, deriv, 0, afrag, wave, lane);  // g, Pi
+      face_corrections(&ext_v[0][0]);
+      lds_barrier();  // g, Pi derivatives complete
+      double dg[P][3], dpi[P][3];
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+        if (act[q]) {
+          contract_lds(q, 0, dg[q]);
+          contract_lds(q, 1, dpi[q]);
+        }
+      lds_barrier();  // deriv free again
+      mfma_logical_derivatives<N, 3>(stage, deriv, 2, afrag, wave, lane);  // Phi_n
+      lds_barrier();  // Phi derivatives and face corrections complete
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        if (!act[q]) continue;
+        double dphi[9];  // dphi[m + 3 n] = d_m Phi_n
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+          double d3[3];
+          contract_lds(q, n, d3);
+#pragma unroll
+          for (int m = 0; m < 3; ++m) dphi[m + 3 * n] = d3[m];
+        }
+        gh_point_pair(out[q], st[q], st[q].dt_pi_partial[0], own[q][1], own[q] + 2, dg[q],
+                      dpi[q], dphi);
+        // rotate so that pass p + 1 finds its partial sum at index 0
+#pragma unroll
+        for (int r = 0; r < 9; ++r) st[q].dt_pi_partial[r] = st[q].dt_pi_partial[r + 1];
       }
 #pragma unroll
-      for (int c = 0; c < 5; ++c) dte[comp[c] * NP + t + q * T] = out[q][c];
+      for (int q = 0; q < P; ++q) {
+        if (!act[q]) continue;
+        add_corrections(q, out[q]);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) dte[comp[c] * NP + t + q * T] = out[q][c];
+      }
+    }
+  } else {
+    load_pass(0);
+#pragma unroll 1
+    for (int p = 0; p < 10; ++p) {
+      lds_barrier();  // previous pass done with stage and corr
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+        if (act[q]) {
+#pragma unroll
+          for (int c = 0; c < 5; ++c) stage[c * NP + t + q * T] = own_next[q][c];
+        }
+      double ext_v[PF][5];
+#pragma unroll
+      for (int f = 0; f < PF; ++f)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) ext_v[f][c] = ext_next[f][c];
+      if (p + 1 < 10) load_pass(p + 1);
+      lds_barrier();
+      // lifted boundary corrections of the face slots
+#pragma unroll
+      for (int f = 0; f < PF; ++f) {
+        if (!fact[f]) continue;
+        const int slot = t + f * T;
+        const int dir = slot / NF;
+        const int fp = slot % NF;
+        const int pv = face_volume_index<N>(dir, fp % N, fp / N);
+        const double* fi = smem + L::kFaceInt + slot * L::kIntStride;
+        const double* fe = smem + L::kFaceExt + slot * L::kExtStride;
+        GhFaceSide si, se;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          si.n[c] = fi[c];
+          si.nv[c] = fi[3 + c];
+          se.n[c] = fe[c];
+          se.nv[c] = fe[3 + c];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          si.lam[c] = fi[6 + c];
+          se.lam[c] = fe[6 + c];
+        }
+        si.gamma2 = fi[10];
+        se.gamma2 = fe[10];
+        const double fac = fi[11];
+        double phi_i[3], phi_e[3], pki[13], pke[13], bc[5];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          phi_i[c] = stage[(2 + c) * NP + pv];
+          phi_e[c] = ext_v[f][2 + c];
+        }
+        gh_package_pair(pki, stage[pv], stage[NP + pv], phi_i, si);
+        gh_package_pair(pke, ext_v[f][0], ext_v[f][1], phi_e, se);
+        gh_boundary_terms_pair(bc, pki, si.lam, pke, se.lam);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) corr[slot * 5 + c] = bc[c] * fac;
+      }
+      // volume terms of the owned points
+      double out[P][5];
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        if (!act[q]) continue;
+        const int pt = t + q * T;
+        const int i = pi_[q], j = pj[q], k = pk[q];
+        double dlog[15];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+          const double* sc = stage + c * NP;
+          double d0 = Di[q][0] * sc[N * (j + N * k)];
+#pragma unroll
+          for (int l = 1; l < N; ++l) d0 = fma(Di[q][l], sc[l + N * (j + N * k)], d0);
+          double d1 = Dj[q][0] * sc[i + N * N * k];
+#pragma unroll
+          for (int l = 1; l < N; ++l) d1 = fma(Dj[q][l], sc[i + N * (l + N * k)], d1);
+          double d2 = Dk[q][0] * sc[i + N * j];
+#pragma unroll
+          for (int l = 1; l < N; ++l) d2 = fma(Dk[q][l], sc[i + N * (j + N * l)], d2);
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            double s = ij[q][ij3(0, a)] * d0;
+            s = fma(ij[q][ij3(1, a)], d1, s);
+            s = fma(ij[q][ij3(2, a)], d2, s);
+            dlog[3 * c + a] = s;
+          }
+        }
+        // dlog[3 c + m] = d_m of component c; c = 0 g, 1 Pi, 2..4 Phi_n
+        double phi[3], dphi[9];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+          phi[n] = stage[(2 + n) * NP + pt];
+#pragma unroll
+          for (int m = 0; m < 3; ++m) dphi[m + 3 * n] = dlog[3 * (2 + n) + m];
+        }
+        gh_point_pair(out[q], st[q], st[q].dt_pi_partial[0], stage[NP + pt], phi, dlog,
+                      dlog + 3, dphi);
+        // rotate so that pass p + 1 finds its partial sum at index 0
+#pragma unroll
+        for (int r = 0; r < 9; ++r) st[q].dt_pi_partial[r] = st[q].dt_pi_partial[r + 1];
+      }
+      lds_barrier();  // corrections complete
+      const int comp[5] = {p, 10 + p, 20 + 3 * p, 21 + 3 * p, 22 + 3 * p};
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        if (!act[q]) continue;
+        const int idx[3] = {pi_[q], pj[q], pk[q]};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          if (idx[d] != 0 && idx[d] != N - 1) continue;
+          const int side = idx[d] == 0 ? 0 : 1;
+          const int a = d == 0 ? idx[1] : idx[0];
+          const int b = d == 2 ? idx[1] : idx[2];
+          const double* cr = corr + ((2 * d + side) * NF + a + N * b) * 5;
+#pragma unroll
+          for (int c = 0; c < 5; ++c) out[q][c] = out[q][c] + cr[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) dte[comp[c] * NP + t + q * T] = out[q][c];
+      }
     }
   }
 }
 
-template <int N>
+template <int N, bool MFMA>
 int launch_gh_fused_n(int num_elements, const int32_t* nbr, const double* u,
                       const double* gammas, const double* inv_jac, double* dt,
                       const double* dtab, hipStream_t stream) {
-  hipLaunchKernelGGL(gh_rhs_fused_kernel<N>, dim3(num_elements),
-                     dim3(GhFusedLayout<N>::THREADS), 0, stream, nbr, u, gammas, inv_jac, dt,
-                     dtab);
+  hipLaunchKernelGGL((gh_rhs_fused_kernel<N, MFMA>), dim3(num_elements),
+                     dim3(GhFusedLayout<N, MFMA>::THREADS), 0, stream, nbr, u, gammas, inv_jac,
+                     dt, dtab);
   return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
+}
+
+template <int N>
+int launch_gh_fused_variant(int variant, int num_elements, const int32_t* nbr, const double* u,
+                            const double* gammas, const double* inv_jac, double* dt,
+                            const double* dtab, hipStream_t stream) {
+  if (variant == kGhFusedMfma)
+    return launch_gh_fused_n<N, true>(num_elements, nbr, u, gammas, inv_jac, dt, dtab, stream);
+  return launch_gh_fused_n<N, false>(num_elements, nbr, u, gammas, inv_jac, dt, dtab, stream);
 }
 
 }  // namespace
@@ -348,18 +613,23 @@ bool gh_fused_supported(const int32_t ext[3]) {
 
 int launch_gh_rhs_fused(int num_elements, const int32_t ext[3], const int32_t* nbr,
                         const double* u, const double* gammas, const double* inv_jac,
-                        double* dt, hipStream_t stream) {
+                        double* dt, int variant, hipStream_t stream) {
   const double* dtab = device_diff_matrices();
   if (dtab == nullptr) return SXD_ERR_DEVICE;
   if (!gh_fused_supported(ext)) return SXD_ERR_UNSUPPORTED;
+  if (variant != kGhFusedValu && variant != kGhFusedMfma) return SXD_ERR_INVALID_ARGUMENT;
   switch (ext[0]) {
-    case 2: return launch_gh_fused_n<2>(num_elements, nbr, u, gammas, inv_jac, dt, dtab, stream);
-    case 3: return launch_gh_fused_n<3>(num_elements, nbr, u, gammas, inv_jac, dt, dtab, stream);
-    case 4: return launch_gh_fused_n<4>(num_elements, nbr, u, gammas, inv_jac, dt, dtab, stream);
-    case 5: return launch_gh_fused_n<5>(num_elements, nbr, u, gammas, inv_jac, dt, dtab, stream);
-    case 6: return launch_gh_fused_n<6>(num_elements, nbr, u, gammas, inv_jac, dt, dtab, stream);
-    case 7: return launch_gh_fused_n<7>(num_elements, nbr, u, gammas, inv_jac, dt, dtab, stream);
-    case 8: return launch_gh_fused_n<8>(num_elements, nbr, u, gammas, inv_jac, dt, dtab, stream);
+#define SXD_GH_FUSED_CASE(n) \
+  case n:                    \
+    return launch_gh_fused_variant<n>(variant, num_elements, nbr, u, gammas, inv_jac, dt, dtab, stream);
+    SXD_GH_FUSED_CASE(2)
+    SXD_GH_FUSED_CASE(3)
+    SXD_GH_FUSED_CASE(4)
+    SXD_GH_FUSED_CASE(5)
+    SXD_GH_FUSED_CASE(6)
+    SXD_GH_FUSED_CASE(7)
+    SXD_GH_FUSED_CASE(8)
+#undef SXD_GH_FUSED_CASE
     default: return SXD_ERR_UNSUPPORTED;
   }
 }

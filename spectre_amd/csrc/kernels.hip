@@ -568,12 +568,17 @@ static int launch_sw_fused_n(int num_elements, const int32_t* nbr, const double*
   return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
 }
 
+bool sw_fused_supported(const int32_t ext[3]) {
+  return ext[0] == ext[1] && ext[1] == ext[2] && ext[0] >= 2 && ext[0] <= 10;
+}
+
 int launch_sw_rhs(int num_elements, const int32_t ext[3], const int32_t* nbr,
                   const double* u, const double* gamma2, const double* inv_jac, double* dt,
-                  double* scratch_du, hipStream_t stream) {
+                  bool fused, double* scratch_du, hipStream_t stream) {
   const double* dtab = device_diff_matrices();
   if (dtab == nullptr) return SXD_ERR_DEVICE;
-  if (ext[0] == ext[1] && ext[1] == ext[2]) {
+  if (fused) {
+    if (!sw_fused_supported(ext)) return SXD_ERR_UNSUPPORTED;
     switch (ext[0]) {
       case 2: return launch_sw_fused_n<2>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, stream);
       case 3: return launch_sw_fused_n<3>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, stream);
@@ -584,7 +589,7 @@ int launch_sw_rhs(int num_elements, const int32_t ext[3], const int32_t* nbr,
       case 8: return launch_sw_fused_n<8>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, stream);
       case 9: return launch_sw_fused_n<9>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, stream);
       case 10: return launch_sw_fused_n<10>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, stream);
-      default: break;
+      default: return SXD_ERR_UNSUPPORTED;
     }
   }
   // Composite path: derivatives, point-wise terms, then face passes.

@@ -27,17 +27,23 @@ int launch_gh_upwind(int64_t n, const double* u_int, const double* t_int,
                      const double* n_ext, double* bc, hipStream_t stream);
 int launch_lift_flux(int64_t n, int nc, double* bc, int n_perp, const double* mag,
                      hipStream_t stream);
+bool sw_fused_supported(const int32_t ext[3]);
+// fused: one launch per RHS (requires sw_fused_supported); otherwise the
+// composite path, which needs scratch_du [E][15][Np].
 int launch_sw_rhs(int num_elements, const int32_t ext[3], const int32_t* nbr,
                   const double* u, const double* gamma2, const double* inv_jac, double* dt,
-                  double* scratch_du, hipStream_t stream);
+                  bool fused, double* scratch_du, hipStream_t stream);
 int launch_gh_rhs_composite(int num_elements, const int32_t ext[3], const int32_t* nbr,
                             const double* u, const double* gammas, const double* inv_jac,
                             double* dt, double* scratch_du, double* scratch_temps,
                             hipStream_t stream);
+// Derivative engines of the fused GH kernel.
+constexpr int kGhFusedValu = 0;  // LDS pencils on the vector ALUs
+constexpr int kGhFusedMfma = 1;  // v_mfma_f64_16x16x4f64 tiles
 bool gh_fused_supported(const int32_t ext[3]);
 int launch_gh_rhs_fused(int num_elements, const int32_t ext[3], const int32_t* nbr,
                         const double* u, const double* gammas, const double* inv_jac,
-                        double* dt, hipStream_t stream);
+                        double* dt, int variant, hipStream_t stream);
 int launch_rk3_substep(int substep, int64_t n, double dt, const double* u0, const double* uk,
                        const double* lk, double* out, hipStream_t stream);
 

@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import SpectreAmdError, check, lib
+from ._lib import KERNELS, SYSTEMS, SpectreAmdError, check, lib
 
 
 def _stream():
@@ -219,6 +219,34 @@ class Domain:
         check(lib().sxd_gh_rhs(self._handle, _ptr(u), _ptr(gammas), _ptr(inv_jac), _ptr(out),
                                _stream()))
         return out
+
+    def set_kernel(self, system, kernel):
+        """Pin the kernel organisation ("auto", "composite", "fused",
+        "fused_mfma") used for `system` ("scalar_wave" or "gh")."""
+        check(lib().sxd_domain_set_kernel(self._handle, SYSTEMS[system], KERNELS[kernel]))
+
+    def kernel(self, system):
+        k = ctypes.c_int()
+        check(lib().sxd_domain_get_kernel(self._handle, SYSTEMS[system], ctypes.byref(k)))
+        return {v: n for n, v in KERNELS.items()}[k.value]
+
+    def autotune(self, system, u, params, inv_jac, out, repetitions=5):
+        """Time every kernel organisation on these inputs and keep the fastest;
+        returns (kernel name, mean milliseconds)."""
+        nc = 5 if system == "scalar_wave" else 50
+        _require(u, "u", self._shapes(nc))
+        _require(out, "out", self._shapes(nc))
+        _require(inv_jac, "inv_jac", self._shapes(9))
+        if system == "scalar_wave":
+            _require(params, "gamma2", (self.num_elements, self.num_points))
+        else:
+            _require(params, "gammas", self._shapes(3))
+        k = ctypes.c_int()
+        ms = ctypes.c_float()
+        check(lib().sxd_domain_autotune(self._handle, SYSTEMS[system], _ptr(u), _ptr(params),
+                                        _ptr(inv_jac), _ptr(out), repetitions, _stream(),
+                                        ctypes.byref(k), ctypes.byref(ms)))
+        return {v: n for n, v in KERNELS.items()}[k.value], ms.value
 
 
 __all__ = ["SpectreAmdError", "partial_derivatives", "logical_partial_derivatives",

@@ -145,6 +145,51 @@ def test_gh_rhs_kerr_schild_512_elements():
     assert err <= TOL
 
 
+@pytest.mark.parametrize("kernel", ["composite", "fused", "fused_mfma"])
+@pytest.mark.parametrize("elements,extents", [((2, 2, 2), (8, 8, 8)), ((2, 1, 2), (5, 5, 5)),
+                                              ((1, 2, 2), (7, 7, 7))])
+def test_gh_rhs_every_kernel_organisation(kernel, elements, extents):
+    """Each kernel organisation of the GH RHS gives the oracle's result bit for bit."""
+    brick, u, gammas, inv = wl.gh_kerr_schild_brick(seed=2, elements=elements, extents=extents)
+    ref = ol.gh_rhs(brick.extents, brick.neighbors, u, gammas, inv)
+    domain = dg.Domain.from_brick(brick)
+    domain.set_kernel("gh", kernel)
+    assert domain.kernel("gh") == kernel
+    got = host(domain.gh_rhs(dev(u), dev(gammas), dev(inv)))
+    assert rel_err(got, ref) <= TOL
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("kernel", ["composite", "fused"])
+def test_scalar_wave_rhs_every_kernel_organisation(kernel):
+    brick, u, gamma2, inv = wl.scalar_wave_plane_wave_3d(seed=3)
+    ref = ol.sw_rhs(brick.extents, brick.neighbors, u, gamma2, inv)
+    domain = dg.Domain.from_brick(brick)
+    domain.set_kernel("scalar_wave", kernel)
+    got = host(domain.scalar_wave_rhs(dev(u), dev(gamma2), dev(inv)))
+    assert np.array_equal(got, ref)
+
+
+def test_autotune_picks_an_available_kernel_and_keeps_results():
+    from spectre_amd import SpectreAmdError
+    brick, u, gammas, inv = wl.gh_kerr_schild_brick(seed=4, elements=(2, 2, 2),
+                                                    extents=(6, 6, 6))
+    ref = ol.gh_rhs(brick.extents, brick.neighbors, u, gammas, inv)
+    domain = dg.Domain.from_brick(brick)
+    d_u, d_g, d_inv = dev(u), dev(gammas), dev(inv)
+    out = torch.empty_like(d_u)
+    kernel, ms = domain.autotune("gh", d_u, d_g, d_inv, out, repetitions=2)
+    assert kernel in ("composite", "fused", "fused_mfma") and ms > 0
+    assert domain.kernel("gh") == kernel
+    assert np.array_equal(host(out), ref)
+    assert np.array_equal(host(domain.gh_rhs(d_u, d_g, d_inv)), ref)
+    # anisotropic extents have no fused GH kernel
+    aniso = dg.Domain(extents=(4, 5, 6), neighbors=np.zeros((1, 6), dtype=np.int32))
+    with pytest.raises(SpectreAmdError):
+        aniso.set_kernel("gh", "fused")
+    assert aniso.kernel("gh") == "composite"
+
+
 def test_rk3_substeps():
     rng = np.random.default_rng(6)
     u0, uk, lk = (rng.uniform(-1, 1, 10007) for _ in range(3))
