@@ -23,6 +23,7 @@ import time
 
 METRIC = "DG RHS grid-pts/s FP64 on 1 GPU, % roofline; mixed-p batch gain; adjoint/forward cost"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
 # (each scaled by what the counter reads for a plain copy of known size:
@@ -121,6 +122,13 @@ def main():
     bpp = algorithmic_bytes_per_point(nc, n_aux, n)
     bytes_per_launch = bpp * points
     achieved_gbs = bytes_per_launch / (ms_per_step * 1e-3) / 1e9
+    # exact algorithmic flops per point (oracle run with a counting scalar type)
+    flops_pp = None
+    golden = os.path.join(root, "tests", "golden", "flop_counts.json")
+    for entry in json.load(open(golden)):
+        if entry["system"] == args.workload and entry["extents"] == list(brick.extents):
+            flops_pp = entry["flops_per_point"]
+    tflops = flops_pp * points / (ms_per_step * 1e-3) / 1e12 if flops_pp else None
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved_gbs, 2),
@@ -131,6 +139,10 @@ def main():
         "kernel": f"{system} RHS, {kernel_used} kernel organisation",
         "algorithmic_bytes_per_point": bpp,
         "algorithmic_bytes_per_launch": bytes_per_launch,
+        "algorithmic_flops_per_point": flops_pp,
+        "achieved_fp64_tflops": round(tflops, 3) if tflops else None,
+        "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+        "fp64_frac": round(tflops / FP64_PEAK_TFLOPS, 4) if tflops else None,
     }
 
     cpu = None

@@ -51,6 +51,18 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Element handled by workgroup b of a one-workgroup-per-element launch of E
+// workgroups.  The dispatcher hands workgroups to the 8 XCDs of the chip in
+// turn (b % 8), and each XCD has its own L2; this bijection gives XCD x one
+// contiguous run of about E / 8 elements, so that an element and the
+// neighbours whose face traces it reads (numbered next to it) share an L2.
+constexpr int kNumXcds = 8;
+__device__ __forceinline__ int64_t xcd_local_element(int64_t b, int64_t num_elements) {
+  const int64_t x = b % kNumXcds, slot = b / kNumXcds;
+  const int64_t per = num_elements / kNumXcds, rem = num_elements % kNumXcds;
+  return x * per + (x < rem ? x : rem) + slot;
+}
+
 // Heaviside step of UpwindPenalty (Blaze/StepFunction.hpp:46-48).
 __device__ __forceinline__ double step_fn(double x) { return x < 0.0 ? 0.0 : 1.0; }
 

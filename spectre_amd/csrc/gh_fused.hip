@@ -180,7 +180,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
   using L = GhFusedLayout<N, MFMA>;
   constexpr int NP = L::NP, NF = L::NF, P = L::P, PF = L::PF, T = L::THREADS;
   __shared__ double smem[L::kDoubles];
-  const int64_t e = blockIdx.x;
+  const int64_t e = xcd_local_element(blockIdx.x, gridDim.x);
   const int t = threadIdx.x;
   const int wave = t / 64, lane = t % 64;
   const double* ue = u + e * 50 * NP;

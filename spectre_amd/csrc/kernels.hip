@@ -462,7 +462,7 @@ sw_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
                     double* __restrict__ dt, const double* __restrict__ dtab) {
   constexpr int NP = N * N * N;
   __shared__ double s_u[5 * NP];
-  const int64_t e = blockIdx.x;
+  const int64_t e = xcd_local_element(blockIdx.x, gridDim.x);
   const int t = threadIdx.x;
   const double* ue = u + e * 5 * NP;
   if (t < NP) {
