@@ -143,6 +143,24 @@ int sxd_domain_create(sxd_domain** domain, int num_elements,
                       const int32_t extents[3], const int32_t* neighbors);
 int sxd_domain_destroy(sxd_domain* domain);
 
+/* A mixed-p set of conforming, aligned hexahedral elements: element e has its
+ * own LGL extents extents[3 e .. 3 e + 2] (2..20 each; p-refinement).  Faces
+ * between elements of different extents are coupled through mortars with the
+ * larger extents per dimension (dg::mortar_mesh,
+ * NumericalAlgorithms/DiscontinuousGalerkin/MortarHelpers.cpp:22-45), packaged
+ * data interpolated to the mortar and corrections projected back
+ * (Spectral/Projection.cpp, ChildSize::Full), as in
+ * ApplyBoundaryCorrections.hpp:797-907.  Fields passed to the RHS calls of
+ * such a domain use the ragged Variables layout: element e's nc blocks of
+ * Np_e points start at nc * offset_e, offset_e = sum_{e' < e} Np_{e'}
+ * (damping parameters and inverse Jacobian likewise, with their component
+ * counts).  SXD_ERR_UNSUPPORTED if an element or face is too large for the
+ * kernels' LDS plan (element Np up to 4096 and faces up to 12 x 12 fit). */
+int sxd_domain_create_mixed(sxd_domain** domain, int num_elements,
+                            const int32_t* extents, const int32_t* neighbors);
+/* Total grid points of the domain (sum of Np_e). */
+int sxd_domain_num_points(const sxd_domain* domain, int64_t* points);
+
 /* Full ScalarWave DG time derivative (volume terms + UpwindPenalty + LGL lift)
  * for every element of the domain: ComputeTimeDerivative followed by
  * ApplyBoundaryCorrectionsToTimeDerivative.
@@ -165,7 +183,11 @@ enum {
   SXD_KERNEL_AUTO = 0,      /* the library's default for the domain's extents */
   SXD_KERNEL_COMPOSITE = 1, /* derivative, point-wise, face and lift launches */
   SXD_KERNEL_FUSED = 2,     /* one launch per RHS, derivatives on the vector ALUs */
-  SXD_KERNEL_FUSED_MFMA = 3 /* one launch per RHS, derivatives on FP64 MFMA (GH) */
+  SXD_KERNEL_FUSED_MFMA = 3, /* one launch per RHS, derivatives on FP64 MFMA (GH) */
+  /* mixed-p domains: */
+  SXD_KERNEL_MIXED_BATCHED = 4,    /* one launch per kernel over all elements,
+                                      ordered by decreasing cost */
+  SXD_KERNEL_MIXED_PER_EXTENTS = 5 /* one launch per kernel and group of equal extents */
 };
 enum { SXD_SYSTEM_SCALAR_WAVE = 0, SXD_SYSTEM_GH = 1 };
 
@@ -175,6 +197,9 @@ enum { SXD_SYSTEM_SCALAR_WAVE = 0, SXD_SYSTEM_GH = 1 };
 int sxd_domain_set_kernel(sxd_domain* domain, int system, int kernel);
 /* The kernel the next RHS call on this domain will use. */
 int sxd_domain_get_kernel(const sxd_domain* domain, int system, int* kernel);
+/* Threads per workgroup of the mixed-p kernels (64, 128 or 256; default 128). */
+int sxd_domain_set_threads(sxd_domain* domain, int system, int threads);
+int sxd_domain_get_threads(const sxd_domain* domain, int system, int* threads);
 /* Autotuner: time every kernel available for (domain, system) on the given
  * inputs (one warm-up, then `repetitions` timed calls each, HIP events on
  * `stream`), keep the fastest for later calls and return it in *chosen and

@@ -20,6 +20,8 @@
 // counting type (exact flop counts) and on a dual number (exact Jacobians).
 #pragma once
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -869,6 +871,357 @@ void gh_rhs(int num_elements, const int ext[3], const int* nbr, const T* u,
   std::vector<GhFaceTemps<T>> temps(static_cast<size_t>(num_elements) * np);
   gh_rhs_volume(0, num_elements, ext, u, gammas, inv_j, dt, temps.data());
   gh_rhs_faces(0, num_elements, ext, nbr, u, inv_j, dt, temps.data());
+}
+
+// ---------------------------------------------------------------------------
+// Mixed p: elements of different per-dimension LGL extents, coupled through
+// mortars (conforming faces, aligned orientation, ChildSize::Full).
+//
+// Ragged Variables layout: element e has Np_e = N0 N1 N2 points and starts at
+// point offset off_e = sum_{e' < e} Np_{e'}; a field with nc components
+// stores element e's data at [nc off_e, nc off_e + nc Np_e) as nc blocks of
+// Np_e (the Variables layout of that element).
+
+// Legendre polynomial P_k(x), Kopriva Alg. 20 (Spectral/Legendre.cpp:27-45).
+inline double legendre_p(int k, double x) {
+  if (k == 0) return 1.0;
+  if (k == 1) return x;
+  double lm2 = 1.0, lm1 = x, lk = 0.0;
+  for (int j = 2; j <= k; ++j) {
+    lk = ((2.0 * j - 1.0) * x * lm1 - (j - 1.0) * lm2) / j;
+    lm2 = lm1;
+    lm1 = lk;
+  }
+  return lk;
+}
+
+// Barycentric interpolation matrix from the n LGL points to `target`
+// (row-major, rows = targets), Kopriva Alg. 32 as in interpolation_matrix,
+// Spectral/Spectral.cpp:625-667: a target equal within roundoff
+// (|a - b| < (max(|a|, |b|) + 1) 100 eps, Utilities/EqualWithinRoundoff.hpp)
+// to a node gets the unit row.
+inline std::vector<double> lgl_interpolation_matrix(int n, const std::vector<double>& target) {
+  std::vector<double> x, w;
+  lgl_points_and_weights(n, &x, &w);
+  const std::vector<double> b = barycentric_weights(x);
+  const size_t m = target.size();
+  std::vector<double> I(m * n, 0.0);
+  const double eps = 100.0 * 2.220446049250313e-16;
+  for (size_t k = 0; k < m; ++k) {
+    bool match = false;
+    for (int j = 0; j < n; ++j) {
+      const double a = target[k], c = x[j];
+      if (std::fabs(a - c) < (std::max(std::fabs(a), std::fabs(c)) + 1.0) * eps) {
+        I[k * n + j] = 1.0;
+        match = true;
+      }
+    }
+    if (!match) {
+      double sum = 0.0;
+      for (int j = 0; j < n; ++j) {
+        I[k * n + j] = b[j] / (target[k] - x[j]);
+        sum += I[k * n + j];
+      }
+      for (int j = 0; j < n; ++j) I[k * n + j] /= sum;
+    }
+  }
+  return I;
+}
+
+// Element-to-mortar projection, projection_matrix_parent_to_child with
+// ChildSize::Full (Spectral/Projection.cpp, make_interpolators): interpolation
+// from the n_face points to the n_mortar >= n_face mortar points.
+// Row-major [n_mortar][n_face].
+inline std::vector<double> mortar_interpolation_matrix(int n_face, int n_mortar) {
+  std::vector<double> xm, wm;
+  lgl_points_and_weights(n_mortar, &xm, &wm);
+  return lgl_interpolation_matrix(n_face, xm);
+}
+
+// Mortar-to-element projection, projection_matrix_child_to_parent with
+// ChildSize::Full for a non-massive operand (Spectral/Projection.cpp:97-136):
+// modal truncation, Pr(i, j) = sum_{k < n_face} V_face(i, k) Vinv_mortar(k, j)
+// with V(i, k) = P_k(x_i) (ModalToNodalMatrixGenerator, Spectral.cpp:499-515).
+// The reference obtains Vinv for Gauss-Lobatto by numerical inversion of V
+// (NodalToModalMatrixGenerator, Spectral.cpp:519-523); here the exact inverse
+// is used: Vinv(k, j) = w_j P_k(x_j) / gamma_k with the discrete norms
+// gamma_k = 2 / (2k + 1) for k < n - 1 and gamma_{n-1} = 2 / (n - 1)
+// (the LGL rule integrates P_k P_l exactly for k + l <= 2n - 3).
+// Row-major [n_face][n_mortar]; the sum over k is accumulated from k = 0 as a
+// chain of fused multiply-adds starting from 0.
+inline std::vector<double> mortar_projection_matrix(int n_face, int n_mortar) {
+  std::vector<double> xf, wf, xm, wm;
+  lgl_points_and_weights(n_face, &xf, &wf);
+  lgl_points_and_weights(n_mortar, &xm, &wm);
+  std::vector<double> vinv(static_cast<size_t>(n_mortar) * n_mortar);
+  for (int k = 0; k < n_mortar; ++k) {
+    const double gamma = k < n_mortar - 1 ? 2.0 / (2.0 * k + 1.0) : 2.0 / (n_mortar - 1.0);
+    for (int j = 0; j < n_mortar; ++j) vinv[k * n_mortar + j] = wm[j] * legendre_p(k, xm[j]) / gamma;
+  }
+  std::vector<double> pr(static_cast<size_t>(n_face) * n_mortar);
+  for (int i = 0; i < n_face; ++i)
+    for (int j = 0; j < n_mortar; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < n_face; ++k) s = std::fma(legendre_p(k, xf[i]), vinv[k * n_mortar + j], s);
+      pr[i * n_mortar + j] = s;
+    }
+  return pr;
+}
+
+// Apply per-dimension matrices to nc components on a 2-d face/mortar mesh
+// (apply_matrices, DataStructures/ApplyMatrices.cpp: dimension 0 first, then
+// dimension 1; a dimension whose extents agree is the identity and skipped).
+//   in [nc][ia * ib] (a fastest), out [nc][oa * ob]; Ma [oa][ia], Mb [ob][ib].
+// Each output is sum_l M(r, l) v_l: first term a product, then fused
+// multiply-adds in l order.
+template <class T>
+inline void apply_face_matrices(int nc, int ia, int ib, int oa, int ob,
+                                const std::vector<double>& Ma, const std::vector<double>& Mb,
+                                const T* in, T* out) {
+  std::vector<T> tmp(static_cast<size_t>(nc) * oa * ib);
+  for (int c = 0; c < nc; ++c)
+    for (int b = 0; b < ib; ++b)
+      for (int r = 0; r < oa; ++r) {
+        const T* v = in + static_cast<size_t>(c) * ia * ib + static_cast<size_t>(ia) * b;
+        T s;
+        if (ia == oa) {
+          s = v[r];
+        } else {
+          s = Ma[static_cast<size_t>(r) * ia] * v[0];
+          for (int l = 1; l < ia; ++l) s = fma_(Ma[static_cast<size_t>(r) * ia + l], v[l], s);
+        }
+        tmp[static_cast<size_t>(c) * oa * ib + static_cast<size_t>(oa) * b + r] = s;
+      }
+  for (int c = 0; c < nc; ++c)
+    for (int r = 0; r < ob; ++r)
+      for (int a = 0; a < oa; ++a) {
+        const T* v = tmp.data() + static_cast<size_t>(c) * oa * ib + a;
+        T s;
+        if (ib == ob) {
+          s = v[static_cast<size_t>(oa) * r];
+        } else {
+          s = Mb[static_cast<size_t>(r) * ib] * v[0];
+          for (int l = 1; l < ib; ++l)
+            s = fma_(Mb[static_cast<size_t>(r) * ib + l], v[static_cast<size_t>(oa) * l], s);
+        }
+        out[static_cast<size_t>(c) * oa * ob + static_cast<size_t>(oa) * r + a] = s;
+      }
+}
+
+struct RaggedLayout {
+  std::vector<std::array<int, 3>> ext;
+  std::vector<size_t> np, off;
+  size_t total = 0;
+  RaggedLayout(int num_elements, const int* extents) {
+    for (int e = 0; e < num_elements; ++e) {
+      ext.push_back({extents[3 * e], extents[3 * e + 1], extents[3 * e + 2]});
+      np.push_back(static_cast<size_t>(extents[3 * e]) * extents[3 * e + 1] * extents[3 * e + 2]);
+      off.push_back(total);
+      total += np.back();
+    }
+  }
+};
+
+// Face-mesh extents (a, b) of direction d: the two remaining dimensions in
+// increasing order (Mesh::slice_away, Spectral/Mesh.hpp:190).
+inline void face_extents(const std::array<int, 3>& ext, int d, int* na, int* nb) {
+  *na = d == 0 ? ext[1] : ext[0];
+  *nb = d == 2 ? ext[1] : ext[2];
+}
+
+// Mortar coupling of one face of element e in direction (d, side) for a
+// system with npk packaged components: interpolate both sides' packaged data
+// (own face mesh (na, nb), neighbour face mesh (ma, mb)) to the mortar
+// (max per dimension, dg::mortar_mesh, NumericalAlgorithms/
+// DiscontinuousGalerkin/MortarHelpers.cpp:22-45), return the mortar extents
+// and the two mortar arrays.  Projection happens only where a face mesh
+// differs from the mortar mesh (Spectral::needs_projection, Projection.cpp:46-54).
+template <class T>
+inline void to_mortar(int npk, int na, int nb, int ma, int mb, const T* pk_own, const T* pk_nb,
+                      int* ka, int* kb, std::vector<T>* own_m, std::vector<T>* nb_m) {
+  *ka = std::max(na, ma);
+  *kb = std::max(nb, mb);
+  const size_t km = static_cast<size_t>(*ka) * *kb;
+  own_m->assign(npk * km, T(0.0));
+  nb_m->assign(npk * km, T(0.0));
+  apply_face_matrices(npk, na, nb, *ka, *kb, mortar_interpolation_matrix(na, *ka),
+                      mortar_interpolation_matrix(nb, *kb), pk_own, own_m->data());
+  apply_face_matrices(npk, ma, mb, *ka, *kb, mortar_interpolation_matrix(ma, *ka),
+                      mortar_interpolation_matrix(mb, *kb), pk_nb, nb_m->data());
+}
+
+// Full ScalarWave DG RHS on a mixed-p set of elements (ragged layout).
+template <class T>
+void sw_rhs_mixed(int num_elements, const int* extents, const int* nbr, const T* u,
+                  const T* gamma2, const T* inv_j, T* dt) {
+  const RaggedLayout L(num_elements, extents);
+  for (int e = 0; e < num_elements; ++e) {
+    const size_t np = L.np[e], off = L.off[e];
+    std::vector<T> du(15 * np);
+    partial_derivatives(u + 5 * off, 5, L.ext[e].data(), inv_j + 9 * off, du.data());
+    for (size_t p = 0; p < np; ++p) {
+      T up[5], dup[15], dtp[5];
+      for (int c = 0; c < 5; ++c) up[c] = u[5 * off + c * np + p];
+      for (int c = 0; c < 15; ++c) dup[c] = du[c * np + p];
+      sw_time_derivative_point(dtp, up, dup, gamma2[off + p]);
+      for (int c = 0; c < 5; ++c) dt[5 * off + c * np + p] = dtp[c];
+    }
+  }
+  for (int e = 0; e < num_elements; ++e) {
+    const size_t np = L.np[e], off = L.off[e];
+    for (int dir = 0; dir < 6; ++dir) {
+      const int d = dir / 2, side = dir % 2;
+      const int en = nbr[6 * e + dir];
+      const size_t npn = L.np[en], offn = L.off[en];
+      const Face fi = make_face(L.ext[e].data(), d, side);
+      const Face fe = make_face(L.ext[en].data(), d, 1 - side);
+      int na, nb, ma, mb;
+      face_extents(L.ext[e], d, &na, &nb);
+      face_extents(L.ext[en], d, &ma, &mb);
+      const size_t nf = fi.pts.size(), mf = fe.pts.size();
+      std::vector<T> pk_own(16 * nf), pk_nb(16 * mf), mag(nf);
+      for (size_t f = 0; f < nf; ++f) {
+        const size_t p = fi.pts[f];
+        T up[5], n[3], pk[16];
+        for (int c = 0; c < 5; ++c) up[c] = u[5 * off + c * np + p];
+        for (int i = 0; i < 3; ++i) {
+          n[i] = inv_j[9 * off + ij3(d, i) * np + p];
+          if (side == 0) n[i] = n[i] * -1.0;
+        }
+        flat_normalize(n, &mag[f]);
+        sw_package_point(pk, up, gamma2[off + p], n);
+        for (int c = 0; c < 16; ++c) pk_own[c * nf + f] = pk[c];
+      }
+      for (size_t f = 0; f < mf; ++f) {
+        const size_t q = fe.pts[f];
+        T un[5], nn[3], magn, pk[16];
+        for (int c = 0; c < 5; ++c) un[c] = u[5 * offn + c * npn + q];
+        for (int i = 0; i < 3; ++i) {
+          nn[i] = inv_j[9 * offn + ij3(d, i) * npn + q];
+          if (side == 1) nn[i] = nn[i] * -1.0;
+        }
+        flat_normalize(nn, &magn);
+        sw_package_point(pk, un, gamma2[offn + q], nn);
+        for (int c = 0; c < 16; ++c) pk_nb[c * mf + f] = pk[c];
+      }
+      int ka, kb;
+      std::vector<T> own_m, nb_m;
+      to_mortar(16, na, nb, ma, mb, pk_own.data(), pk_nb.data(), &ka, &kb, &own_m, &nb_m);
+      const size_t km = static_cast<size_t>(ka) * kb;
+      std::vector<T> bc_m(5 * km), bc(5 * nf);
+      for (size_t k = 0; k < km; ++k) {
+        T in[16], ex[16], b[5];
+        for (int c = 0; c < 16; ++c) {
+          in[c] = own_m[c * km + k];
+          ex[c] = nb_m[c * km + k];
+        }
+        sw_boundary_terms_point(b, in, ex);
+        for (int c = 0; c < 5; ++c) bc_m[c * km + k] = b[c];
+      }
+      apply_face_matrices(5, ka, kb, na, nb, mortar_projection_matrix(na, ka),
+                          mortar_projection_matrix(nb, kb), bc_m.data(), bc.data());
+      for (size_t f = 0; f < nf; ++f) {
+        const T fac = lift_factor(L.ext[e][d], mag[f]);
+        for (int c = 0; c < 5; ++c) {
+          T& target = dt[5 * off + c * np + fi.pts[f]];
+          target = target + bc[c * nf + f] * fac;
+        }
+      }
+    }
+  }
+}
+
+// Full GH DG RHS on a mixed-p set of elements (ragged layout); gammas
+// [3][Np_e] per element.
+template <class T>
+void gh_rhs_mixed(int num_elements, const int* extents, const int* nbr, const T* u,
+                  const T* gammas, const T* inv_j, T* dt) {
+  const RaggedLayout L(num_elements, extents);
+  std::vector<GhFaceTemps<T>> temps(L.total);
+  for (int e = 0; e < num_elements; ++e) {
+    const size_t np = L.np[e], off = L.off[e];
+    std::vector<T> du(150 * np);
+    partial_derivatives(u + 50 * off, 50, L.ext[e].data(), inv_j + 9 * off, du.data());
+    const T* ga = gammas + 3 * off;
+    for (size_t p = 0; p < np; ++p) {
+      T up[50], dup[150], dtp[50];
+      for (int c = 0; c < 50; ++c) up[c] = u[50 * off + c * np + p];
+      for (int c = 0; c < 150; ++c) dup[c] = du[c * np + p];
+      gh_time_derivative_point(dtp, &temps[off + p], up, dup, ga[p], ga[np + p], ga[2 * np + p]);
+      for (int c = 0; c < 50; ++c) dt[50 * off + c * np + p] = dtp[c];
+    }
+  }
+  for (int e = 0; e < num_elements; ++e) {
+    const size_t np = L.np[e], off = L.off[e];
+    for (int dir = 0; dir < 6; ++dir) {
+      const int d = dir / 2, side = dir % 2;
+      const int en = nbr[6 * e + dir];
+      const size_t npn = L.np[en], offn = L.off[en];
+      const Face fi = make_face(L.ext[e].data(), d, side);
+      const Face fe = make_face(L.ext[en].data(), d, 1 - side);
+      int na, nb, ma, mb;
+      face_extents(L.ext[e], d, &na, &nb);
+      face_extents(L.ext[en], d, &ma, &mb);
+      const size_t nf = fi.pts.size(), mf = fe.pts.size();
+      // packaged data: 10 pairs x 13, then the 4 speeds
+      std::vector<T> pk_own(134 * nf), pk_nb(134 * mf), mag(nf);
+      auto package = [&](const Face& face, size_t f, size_t ee_off, size_t ee_np, int flip_side,
+                         const GhFaceTemps<T>& t, T* out, size_t stride, T* mag_out) {
+        const size_t p = face.pts[f];
+        T n[3], nv[3], mg, lam[4];
+        for (int i = 0; i < 3; ++i) {
+          n[i] = inv_j[9 * ee_off + ij3(d, i) * ee_np + p];
+          if (side == flip_side) n[i] = n[i] * -1.0;
+        }
+        gh_normalize(n, nv, &mg, t.inv_spatial);
+        gh_char_speeds(lam, t.gamma1, t.lapse, t.shift, n);
+        if (mag_out != nullptr) *mag_out = mg;
+        for (int pr = 0; pr < 10; ++pr) {
+          T phi[3], pk[13];
+          for (int i = 0; i < 3; ++i) phi[i] = u[50 * ee_off + (20 + i + 3 * pr) * ee_np + p];
+          gh_package_pair(pk, u[50 * ee_off + pr * ee_np + p],
+                          u[50 * ee_off + (10 + pr) * ee_np + p], phi, t.gamma2, lam, n, nv);
+          for (int c = 0; c < 13; ++c) out[(13 * pr + c) * stride + f] = pk[c];
+        }
+        for (int c = 0; c < 4; ++c) out[(130 + c) * stride + f] = lam[c];
+      };
+      for (size_t f = 0; f < nf; ++f)
+        package(fi, f, off, np, 0, temps[off + fi.pts[f]], pk_own.data(), nf, &mag[f]);
+      for (size_t f = 0; f < mf; ++f)
+        package(fe, f, offn, npn, 1, temps[offn + fe.pts[f]], pk_nb.data(), mf, nullptr);
+      int ka, kb;
+      std::vector<T> own_m, nb_m;
+      to_mortar(134, na, nb, ma, mb, pk_own.data(), pk_nb.data(), &ka, &kb, &own_m, &nb_m);
+      const size_t km = static_cast<size_t>(ka) * kb;
+      std::vector<T> bc_m(50 * km), bc(50 * nf);
+      for (size_t k = 0; k < km; ++k) {
+        T lam_in[4], lam_ex[4];
+        for (int c = 0; c < 4; ++c) {
+          lam_in[c] = own_m[(130 + c) * km + k];
+          lam_ex[c] = nb_m[(130 + c) * km + k];
+        }
+        for (int pr = 0; pr < 10; ++pr) {
+          T in[13], ex[13], b[5];
+          for (int c = 0; c < 13; ++c) {
+            in[c] = own_m[(13 * pr + c) * km + k];
+            ex[c] = nb_m[(13 * pr + c) * km + k];
+          }
+          gh_boundary_terms_pair(b, in, lam_in, ex, lam_ex);
+          const int comp[5] = {pr, 10 + pr, 20 + 3 * pr, 21 + 3 * pr, 22 + 3 * pr};
+          for (int c = 0; c < 5; ++c) bc_m[comp[c] * km + k] = b[c];
+        }
+      }
+      apply_face_matrices(50, ka, kb, na, nb, mortar_projection_matrix(na, ka),
+                          mortar_projection_matrix(nb, kb), bc_m.data(), bc.data());
+      for (size_t f = 0; f < nf; ++f) {
+        const T fac = lift_factor(L.ext[e][d], mag[f]);
+        for (int c = 0; c < 50; ++c) {
+          T& target = dt[50 * off + c * np + fi.pts[f]];
+          target = target + bc[c * nf + f] * fac;
+        }
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -343,6 +343,53 @@ int sxo_gh_rhs(int num_elements, const int* ext, const int* nbr, const double* u
   return 0;
 }
 
+// ---- mixed p ---------------------------------------------------------------
+
+int sxo_mortar_interpolation_matrix(int n_face, int n_mortar, double* out) {
+  if (n_face < 2 || n_mortar > 20 || n_mortar < n_face) return -1;
+  const auto m = sxo::mortar_interpolation_matrix(n_face, n_mortar);
+  std::memcpy(out, m.data(), m.size() * sizeof(double));
+  return 0;
+}
+
+int sxo_mortar_projection_matrix(int n_face, int n_mortar, double* out) {
+  if (n_face < 2 || n_mortar > 20 || n_mortar < n_face) return -1;
+  const auto m = sxo::mortar_projection_matrix(n_face, n_mortar);
+  std::memcpy(out, m.data(), m.size() * sizeof(double));
+  return 0;
+}
+
+int sxo_lgl_interpolation_matrix(int n, int num_targets, const double* targets, double* out) {
+  if (n < 2 || n > 20 || num_targets < 1) return -1;
+  const auto m = sxo::lgl_interpolation_matrix(n, std::vector<double>(targets, targets + num_targets));
+  std::memcpy(out, m.data(), m.size() * sizeof(double));
+  return 0;
+}
+
+static bool bad_ragged(int num_elements, const int* extents, const int* nbr) {
+  if (num_elements < 1) return true;
+  for (int e = 0; e < num_elements; ++e) {
+    if (bad_extents(extents + 3 * e)) return true;
+    for (int k = 0; k < 6; ++k)
+      if (nbr[6 * e + k] < 0 || nbr[6 * e + k] >= num_elements) return true;
+  }
+  return false;
+}
+
+int sxo_sw_rhs_mixed(int num_elements, const int* extents, const int* nbr, const double* u,
+                     const double* gamma2, const double* inv_j, double* dt) {
+  if (bad_ragged(num_elements, extents, nbr)) return -1;
+  sxo::sw_rhs_mixed(num_elements, extents, nbr, u, gamma2, inv_j, dt);
+  return 0;
+}
+
+int sxo_gh_rhs_mixed(int num_elements, const int* extents, const int* nbr, const double* u,
+                     const double* gammas, const double* inv_j, double* dt) {
+  if (bad_ragged(num_elements, extents, nbr)) return -1;
+  sxo::gh_rhs_mixed(num_elements, extents, nbr, u, gammas, inv_j, dt);
+  return 0;
+}
+
 // The same RHS with the elements split over `threads` host threads (element
 // data parallelism, as the reference distributes elements over cores).  Used
 // for the CPU baseline.  Volume terms per element, then the face terms per

@@ -42,6 +42,10 @@ _SIGNATURES = {
     "sxd_scalar_wave_rhs": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_gh_rhs": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_rk3_substep": (ctypes.c_int, [ctypes.c_int, _i64, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]),
+    "sxd_domain_create_mixed": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _ip, _ip]),
+    "sxd_domain_num_points": (ctypes.c_int, [_vp, ctypes.POINTER(_i64)]),
+    "sxd_domain_set_threads": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "sxd_domain_get_threads": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "sxd_domain_set_kernel": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "sxd_domain_get_kernel": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "sxd_domain_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, _vp,
@@ -49,7 +53,8 @@ _SIGNATURES = {
 }
 
 # Kernel organisations of the whole-domain right-hand sides (SXD_KERNEL_*)
-KERNELS = {"auto": 0, "composite": 1, "fused": 2, "fused_mfma": 3}
+KERNELS = {"auto": 0, "composite": 1, "fused": 2, "fused_mfma": 3, "mixed_batched": 4,
+           "mixed_per_extents": 5}
 SYSTEMS = {"scalar_wave": 0, "gh": 1}
 
 

@@ -3,6 +3,7 @@
 // gh_fused.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -27,8 +28,18 @@ struct sxd_domain {
   size_t scratch_du_doubles = 0;
   size_t scratch_temps_doubles = 0;
   std::mutex scratch_mutex;
-  // Kernel choice per system (SXD_SYSTEM_*), SXD_KERNEL_AUTO until set.
+  // Kernel choice per system (SXD_SYSTEM_*), SXD_KERNEL_AUTO until set, and
+  // threads per workgroup of the mixed-p kernels.
   int kernel[2] = {SXD_KERNEL_AUTO, SXD_KERNEL_AUTO};
+  int threads[2] = {128, 128};
+  // Mixed p: per-element extents and offsets (ragged layout).
+  bool mixed = false;
+  int64_t total_points = 0;
+  int np_max = 0, kf = 0, km = 0;
+  sxd::MixedElement* d_el = nullptr;
+  int32_t* d_list = nullptr;          // all elements, decreasing cost
+  int32_t* d_buckets = nullptr;       // elements grouped by extents
+  std::vector<std::pair<int, int>> buckets;  // (start, count) in d_buckets
 };
 
 namespace sxd {
@@ -98,16 +109,18 @@ DeviceTables& tables() {
   return t;
 }
 
-}  // namespace
+DeviceTables& mortar_tables() {
+  static DeviceTables t;
+  return t;
+}
 
-const double* device_diff_matrices() {
+// Upload `host` once per device; the copy lives until the process ends.
+const double* device_table(DeviceTables& t, const std::vector<double>& host) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-  DeviceTables& t = tables();
   std::lock_guard<std::mutex> lock(t.mutex);
   if (static_cast<int>(t.per_device.size()) <= dev) t.per_device.resize(dev + 1, nullptr);
   if (t.per_device[dev] == nullptr) {
-    const std::vector<double>& host = packed_differentiation_matrices();
     double* d = nullptr;
     if (hipMalloc(&d, host.size() * sizeof(double)) != hipSuccess) return nullptr;
     if (hipMemcpy(d, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice) !=
@@ -118,6 +131,16 @@ const double* device_diff_matrices() {
     t.per_device[dev] = d;
   }
   return t.per_device[dev];
+}
+
+}  // namespace
+
+const double* device_diff_matrices() {
+  return device_table(tables(), packed_differentiation_matrices());
+}
+
+const double* device_mortar_matrices() {
+  return device_table(mortar_tables(), packed_mortar_matrices());
 }
 
 }  // namespace sxd
@@ -296,8 +319,125 @@ int sxd_domain_create(sxd_domain** domain, int num_elements, const int32_t exten
   return ok();
 }
 
+int sxd_domain_create_mixed(sxd_domain** domain, int num_elements, const int32_t* extents,
+                            const int32_t* neighbors) {
+  if (domain == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null domain output");
+  *domain = nullptr;
+  if (num_elements < 1) return fail(SXD_ERR_INVALID_ARGUMENT, "need at least one element");
+  if (extents == nullptr || neighbors == nullptr)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "null extents or neighbour table");
+  for (int e = 0; e < num_elements; ++e)
+    if (!valid_extents(extents + 3 * e))
+      return fail(SXD_ERR_INVALID_ARGUMENT, "extents of element %d must be 2..20", e);
+  for (int64_t i = 0; i < 6LL * num_elements; ++i)
+    if (neighbors[i] < 0 || neighbors[i] >= num_elements)
+      return fail(SXD_ERR_INVALID_ARGUMENT, "neighbour %d of element %d out of range",
+                  neighbors[i], static_cast<int>(i / 6));
+  std::vector<MixedElement> el(num_elements);
+  int64_t off = 0;
+  int np_max = 0, kf = 0, km = 0;
+  for (int e = 0; e < num_elements; ++e) {
+    MixedElement& m = el[e];
+    for (int d = 0; d < 3; ++d) m.n[d] = extents[3 * e + d];
+    m.np = m.n[0] * m.n[1] * m.n[2];
+    m.off = off;
+    off += m.np;
+    np_max = std::max(np_max, m.np);
+  }
+  for (int e = 0; e < num_elements; ++e)
+    for (int dir = 0; dir < 6; ++dir) {
+      const int d = dir / 2;
+      const int32_t* a = extents + 3 * e;
+      const int32_t* b = extents + 3 * neighbors[6 * e + dir];
+      const int fa = (d == 0 ? a[1] : a[0]), fb = (d == 2 ? a[1] : a[2]);
+      const int ga = (d == 0 ? b[1] : b[0]), gb = (d == 2 ? b[1] : b[2]);
+      kf = std::max({kf, fa * fb, ga * gb});
+      km = std::max(km, std::max(fa, ga) * std::max(fb, gb));
+    }
+  if (5 * np_max * 8 > 160 * 1024 || mixed_face_lds_bytes(kf, km) > 160 * 1024)
+    return fail(SXD_ERR_UNSUPPORTED, "mixed-p elements or faces too large for the LDS plan");
+  // work lists: decreasing cost; and grouped by extents (per-extents launches)
+  std::vector<int32_t> list(num_elements), grouped(num_elements);
+  for (int e = 0; e < num_elements; ++e) list[e] = e;
+  std::stable_sort(list.begin(), list.end(),
+                   [&](int32_t x, int32_t y) { return el[x].np > el[y].np; });
+  grouped = list;
+  auto key = [&](int32_t x) {
+    return (static_cast<int64_t>(el[x].n[0]) << 40) | (static_cast<int64_t>(el[x].n[1]) << 20) |
+           el[x].n[2];
+  };
+  std::stable_sort(grouped.begin(), grouped.end(),
+                   [&](int32_t x, int32_t y) { return key(x) < key(y); });
+  sxd_domain* d = new (std::nothrow) sxd_domain;
+  if (d == nullptr) return fail(SXD_ERR_OUT_OF_MEMORY, "host allocation failed");
+  for (int i = 0; i < num_elements;) {
+    int j = i;
+    while (j < num_elements && key(grouped[j]) == key(grouped[i])) ++j;
+    d->buckets.emplace_back(i, j - i);
+    i = j;
+  }
+  d->mixed = true;
+  d->num_elements = num_elements;
+  d->total_points = off;
+  d->np_max = np_max;
+  d->kf = kf;
+  d->km = km;
+  int rc = SXD_OK;
+  if (hipGetDevice(&d->device) != hipSuccess) rc = SXD_ERR_DEVICE;
+  auto upload = [&](auto** dst, const auto* src, size_t n) {
+    if (rc != SXD_OK) return;
+    if (hipMalloc(dst, n * sizeof(*src)) != hipSuccess) {
+      rc = SXD_ERR_OUT_OF_MEMORY;
+      return;
+    }
+    if (hipMemcpy(*dst, src, n * sizeof(*src), hipMemcpyHostToDevice) != hipSuccess)
+      rc = SXD_ERR_DEVICE;
+  };
+  upload(&d->d_nbr, neighbors, 6 * static_cast<size_t>(num_elements));
+  upload(&d->d_el, el.data(), el.size());
+  upload(&d->d_list, list.data(), list.size());
+  upload(&d->d_buckets, grouped.data(), grouped.size());
+  if (rc == SXD_OK && (device_diff_matrices() == nullptr || device_mortar_matrices() == nullptr))
+    rc = SXD_ERR_DEVICE;
+  if (rc != SXD_OK) {
+    sxd_domain_destroy(d);
+    return fail(rc, "mixed domain set-up failed");
+  }
+  *domain = d;
+  return ok();
+}
+
+int sxd_domain_num_points(const sxd_domain* domain, int64_t* points) {
+  if (domain == nullptr || points == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null argument");
+  *points = domain->mixed ? domain->total_points
+                          : static_cast<int64_t>(domain->num_elements) * domain->ext[0] *
+                                domain->ext[1] * domain->ext[2];
+  return ok();
+}
+
+int sxd_domain_set_threads(sxd_domain* domain, int system, int threads) {
+  if (domain == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null domain");
+  if (system != SXD_SYSTEM_SCALAR_WAVE && system != SXD_SYSTEM_GH)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "unknown system %d", system);
+  if (threads != 64 && threads != 128 && threads != 256)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "threads per workgroup must be 64, 128 or 256");
+  domain->threads[system] = threads;
+  return ok();
+}
+
+int sxd_domain_get_threads(const sxd_domain* domain, int system, int* threads) {
+  if (domain == nullptr || threads == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null argument");
+  if (system != SXD_SYSTEM_SCALAR_WAVE && system != SXD_SYSTEM_GH)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "unknown system %d", system);
+  *threads = domain->threads[system];
+  return ok();
+}
+
 int sxd_domain_destroy(sxd_domain* domain) {
   if (domain == nullptr) return ok();
+  if (domain->d_el != nullptr) (void)hipFree(domain->d_el);
+  if (domain->d_list != nullptr) (void)hipFree(domain->d_list);
+  if (domain->d_buckets != nullptr) (void)hipFree(domain->d_buckets);
   if (domain->d_nbr != nullptr) (void)hipFree(domain->d_nbr);
   if (domain->d_scratch_du != nullptr) (void)hipFree(domain->d_scratch_du);
   if (domain->d_scratch_temps != nullptr) (void)hipFree(domain->d_scratch_temps);
@@ -342,6 +482,11 @@ namespace {
 // their number (at most 3).
 int candidate_kernels(const sxd_domain* d, int system, int* k) {
   int n = 0;
+  if (d->mixed) {
+    k[n++] = SXD_KERNEL_MIXED_BATCHED;
+    k[n++] = SXD_KERNEL_MIXED_PER_EXTENTS;
+    return n;
+  }
   if (system == SXD_SYSTEM_SCALAR_WAVE) {
     if (sw_fused_supported(d->ext)) k[n++] = SXD_KERNEL_FUSED;
   } else {
@@ -361,8 +506,30 @@ int resolve_kernel(const sxd_domain* d, int system) {
   return k[0];
 }
 
+int run_mixed(sxd_domain* d, bool gh, int kernel, const double* u, const double* params,
+              const double* inv_jac, double* dt, hipStream_t stream) {
+  double* scratch = nullptr;
+  if (gh) {
+    if (ensure_scratch(d, 150 * static_cast<size_t>(d->total_points), 0) != SXD_OK)
+      return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
+    scratch = d->d_scratch_du;
+  }
+  const int threads = d->threads[gh ? SXD_SYSTEM_GH : SXD_SYSTEM_SCALAR_WAVE];
+  if (kernel == SXD_KERNEL_MIXED_BATCHED)
+    return launch_mixed_rhs(gh, d->num_elements, d->d_list, d->d_el, d->d_nbr, d->np_max, d->kf,
+                            d->km, u, params, inv_jac, dt, scratch, threads, stream);
+  for (const auto& b : d->buckets) {
+    const int rc = launch_mixed_rhs(gh, b.second, d->d_buckets + b.first, d->d_el, d->d_nbr,
+                                    d->np_max, d->kf, d->km, u, params, inv_jac, dt, scratch,
+                                    threads, stream);
+    if (rc != SXD_OK) return rc;
+  }
+  return SXD_OK;
+}
+
 int run_sw(sxd_domain* d, int kernel, const double* u, const double* gamma2,
            const double* inv_jac, double* dt, hipStream_t stream) {
+  if (d->mixed) return run_mixed(d, false, kernel, u, gamma2, inv_jac, dt, stream);
   double* scratch = nullptr;
   if (kernel == SXD_KERNEL_COMPOSITE) {
     const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
@@ -376,6 +543,7 @@ int run_sw(sxd_domain* d, int kernel, const double* u, const double* gamma2,
 
 int run_gh(sxd_domain* d, int kernel, const double* u, const double* gammas,
            const double* inv_jac, double* dt, hipStream_t stream) {
+  if (d->mixed) return run_mixed(d, true, kernel, u, gammas, inv_jac, dt, stream);
   if (kernel == SXD_KERNEL_FUSED || kernel == SXD_KERNEL_FUSED_MFMA)
     return launch_gh_rhs_fused(d->num_elements, d->ext, d->d_nbr, u, gammas, inv_jac, dt,
                                kernel == SXD_KERNEL_FUSED ? kGhFusedValu : kGhFusedMfma,
@@ -468,8 +636,14 @@ int sxd_domain_autotune(sxd_domain* domain, int system, const double* u, const d
   int rc = SXD_OK;
   int cand[3];
   const int ncand = candidate_kernels(domain, system, cand);
-  for (int ci = 0; ci < ncand; ++ci) {
+  const int saved_threads = domain->threads[system];
+  int best_threads = saved_threads;
+  const int tile_options[3] = {64, 128, 256};
+  const int ntiles = domain->mixed ? 3 : 1;
+  for (int cj = 0; cj < ncand * ntiles; ++cj) {
+    const int ci = cj / ntiles;
     const int k = cand[ci];
+    if (domain->mixed) domain->threads[system] = tile_options[cj % ntiles];
     rc = run_system(domain, system, k, u, params, inv_jac, dt, s);  // warm-up
     if (rc != SXD_OK) break;
     if (hipEventRecord(ev[0], s) != hipSuccess) { rc = SXD_ERR_DEVICE; break; }
@@ -486,8 +660,10 @@ int sxd_domain_autotune(sxd_domain* domain, int system, const double* u, const d
     if (best < 0 || ms < best_ms) {
       best = k;
       best_ms = ms;
+      best_threads = domain->threads[system];
     }
   }
+  domain->threads[system] = rc == SXD_OK ? best_threads : saved_threads;
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
   if (rc != SXD_OK) return finish(rc, __func__);

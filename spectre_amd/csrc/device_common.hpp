@@ -21,6 +21,15 @@ __device__ __forceinline__ const double* diff_matrix(const double* table, int n)
   return table + packed_d_offset(n);
 }
 
+// Offset of the (nf, nm) block pair in the packed mortar table (the same
+// value as packed_mortar_offset, in O(nf) steps).
+__host__ __device__ inline int mortar_table_offset(int nf, int nm) {
+  constexpr int kSumB = kMaxPoints * (kMaxPoints + 1) / 2;  // sum_{b=1}^{20} b
+  int off = 0;
+  for (int a = kMinPoints; a < nf; ++a) off += 2 * a * (kSumB - a * (a - 1) / 2);
+  return off + 2 * nf * ((nm - 1) * nm / 2 - (nf - 1) * nf / 2);
+}
+
 // Tensor storage order (DataStructures/Tensor/Structure.hpp:98-195).
 // Symmetric 4-d pair (a <= b): (0,0)(0,1)(0,2)(0,3)(1,1)(1,2)(1,3)(2,2)(2,3)(3,3)
 __host__ __device__ constexpr int sym4(int a, int b) {

@@ -44,6 +44,23 @@ bool gh_fused_supported(const int32_t ext[3]);
 int launch_gh_rhs_fused(int num_elements, const int32_t ext[3], const int32_t* nbr,
                         const double* u, const double* gammas, const double* inv_jac,
                         double* dt, int variant, hipStream_t stream);
+// Mixed p.  Per-element record of the ragged layout.
+struct MixedElement {
+  int32_t n[3];  // LGL extents
+  int32_t np;    // n0 n1 n2
+  int64_t off;   // point offset of the element
+};
+const double* device_mortar_matrices();
+// LDS bytes of the face kernel for faces of <= kf and mortars of <= km points.
+int mixed_face_lds_bytes(int kf, int km);
+// Volume terms and mortar-coupled boundary corrections of the `count`
+// elements in `list` (one workgroup each, `threads` threads: 64, 128, 256).
+// gh: params = gammas [3][Np_e], scratch_du = 150 x total points; otherwise
+// params = gamma2 [Np_e].
+int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement* el,
+                     const int32_t* nbr, int np_max, int kf, int km, const double* u,
+                     const double* params, const double* inv_jac, double* dt, double* scratch_du,
+                     int threads, hipStream_t stream);
 int launch_rk3_substep(int substep, int64_t n, double dt, const double* u0, const double* uk,
                        const double* lk, double* out, hipStream_t stream);
 

@@ -1,0 +1,473 @@
+// Mixed-p (p-refinement) DG right-hand sides: elements with different LGL
+// extents per dimension, coupled through mortars.
+//
+// Layout: the ragged Variables layout.  Element e has Np_e = N0 N1 N2 points
+// and starts at point offset off_e; a field with nc components keeps element
+// e's nc blocks of Np_e doubles at [nc off_e, nc (off_e + Np_e)).
+//
+// Launch organisation: every kernel takes a work list of element indices and
+// runs one workgroup per entry.  The batched form is one launch per kernel
+// over the whole domain with the list sorted by decreasing cost, so the
+// dispatcher hands out the expensive elements first; the per-extents form
+// launches the same kernels once per group of equal extents.
+//
+// Face coupling (ApplyBoundaryCorrections.hpp:797-907 with
+// InternalMortarDataImpl.hpp:230-318): for each face of an element, in the
+// order lower/upper xi, eta, zeta, one workgroup
+//   1. packages both sides on their own face meshes (UpwindPenalty
+//      dg_package_data; the neighbour's side from the neighbour's fields),
+//   2. interpolates both packages to the mortar mesh (max extents per
+//      dimension, dg::mortar_mesh) where a face mesh differs from it,
+//      dimension a then dimension b (apply_matrices order),
+//   3. evaluates dg_boundary_terms at the mortar points,
+//   4. projects the correction back to its own face mesh (modal truncation)
+//      where needed, and
+//   5. lifts it into dt (dg::lift_flux, -N(N-1)/2 |n| on the LGL face).
+// Matrix products are sums over the input index l = 0 .. n-1, first term a
+// product and then fused multiply-adds, as in the oracle.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_common.hpp"
+#include "gh_point.hpp"
+#include "kernels.hpp"
+#include "sw_point.hpp"
+
+namespace sxd {
+namespace {
+
+__device__ __forceinline__ int point_index(const MixedElement& m, int i, int j, int k) {
+  return i + m.n[0] * (j + m.n[1] * k);
+}
+
+// Volume index of point (a, b) of the face in dimension d on `side`.
+__device__ __forceinline__ int face_point(const MixedElement& m, int d, int side, int a, int b) {
+  const int fixed = side == 0 ? 0 : m.n[d] - 1;
+  if (d == 0) return point_index(m, fixed, a, b);
+  if (d == 1) return point_index(m, a, fixed, b);
+  return point_index(m, a, b, fixed);
+}
+
+// Logical derivatives of LDS component `sc` at point (i, j, k): sum over
+// l = 0 .. n_d - 1 in order, first term a product.
+__device__ __forceinline__ void logical_derivs(const double* sc, const MixedElement& m,
+                                               const double* D0, const double* D1,
+                                               const double* D2, int i, int j, int k,
+                                               double* d) {
+  const int n0 = m.n[0], n1 = m.n[1], n2 = m.n[2];
+  double d0 = D0[i * n0] * sc[n0 * (j + n1 * k)];
+  for (int l = 1; l < n0; ++l) d0 = fma(D0[i * n0 + l], sc[l + n0 * (j + n1 * k)], d0);
+  double d1 = D1[j * n1] * sc[i + n0 * n1 * k];
+  for (int l = 1; l < n1; ++l) d1 = fma(D1[j * n1 + l], sc[i + n0 * (l + n1 * k)], d1);
+  double d2 = D2[k * n2] * sc[i + n0 * j];
+  for (int l = 1; l < n2; ++l) d2 = fma(D2[k * n2 + l], sc[i + n0 * (j + n1 * l)], d2);
+  d[0] = d0;
+  d[1] = d1;
+  d[2] = d2;
+}
+
+// ---------------------------------------------------------------------------
+// ScalarWave volume terms.  Dynamic LDS: 5 Np_max doubles.
+__global__ void __launch_bounds__(256)
+sw_mixed_volume_kernel(const MixedElement* __restrict__ el, const int32_t* __restrict__ list,
+                       const double* __restrict__ u, const double* __restrict__ gamma2,
+                       const double* __restrict__ inv_jac, double* __restrict__ dt,
+                       const double* __restrict__ dtab) {
+  extern __shared__ double s_u[];
+  const MixedElement m = el[list[blockIdx.x]];
+  const int np = m.np;
+  const double* ue = u + 5 * m.off;
+  for (int p = threadIdx.x; p < 5 * np; p += blockDim.x) s_u[p] = ue[p];
+  __syncthreads();
+  const double* D0 = diff_matrix(dtab, m.n[0]);
+  const double* D1 = diff_matrix(dtab, m.n[1]);
+  const double* D2 = diff_matrix(dtab, m.n[2]);
+  const double* ij = inv_jac + 9 * m.off;
+  for (int p = threadIdx.x; p < np; p += blockDim.x) {
+    const int i = p % m.n[0], j = (p / m.n[0]) % m.n[1], k = p / (m.n[0] * m.n[1]);
+    double up[5], dup[15];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      up[c] = s_u[c * np + p];
+      double d[3];
+      logical_derivs(s_u + c * np, m, D0, D1, D2, i, j, k, d);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        double s = ij[ij3(0, a) * np + p] * d[0];
+        s = fma(ij[ij3(1, a) * np + p], d[1], s);
+        s = fma(ij[ij3(2, a) * np + p], d[2], s);
+        dup[3 * c + a] = s;
+      }
+    }
+    double dtp[5];
+    sw_time_derivative(dtp, up, dup, gamma2[m.off + p]);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) dt[5 * m.off + c * np + p] = dtp[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Partial derivatives of nc components (GH volume terms).  Grid: (list
+// length, component chunks of kRaggedChunk); dynamic LDS: Np_max doubles.
+constexpr int kRaggedChunk = 5;
+
+__global__ void __launch_bounds__(256)
+ragged_partial_derivatives_kernel(const MixedElement* __restrict__ el,
+                                  const int32_t* __restrict__ list, int nc,
+                                  const double* __restrict__ u,
+                                  const double* __restrict__ inv_jac, double* __restrict__ du,
+                                  const double* __restrict__ dtab) {
+  extern __shared__ double s_c[];
+  const MixedElement m = el[list[blockIdx.x]];
+  const int np = m.np;
+  const double* D0 = diff_matrix(dtab, m.n[0]);
+  const double* D1 = diff_matrix(dtab, m.n[1]);
+  const double* D2 = diff_matrix(dtab, m.n[2]);
+  const double* ij = inv_jac + 9 * m.off;
+  const int c_end = min(nc, static_cast<int>(blockIdx.y + 1) * kRaggedChunk);
+  for (int c = blockIdx.y * kRaggedChunk; c < c_end; ++c) {
+    __syncthreads();
+    for (int p = threadIdx.x; p < np; p += blockDim.x) s_c[p] = u[nc * m.off + c * np + p];
+    __syncthreads();
+    for (int p = threadIdx.x; p < np; p += blockDim.x) {
+      const int i = p % m.n[0], j = (p / m.n[0]) % m.n[1], k = p / (m.n[0] * m.n[1]);
+      double d[3];
+      logical_derivs(s_c, m, D0, D1, D2, i, j, k, d);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        double s = ij[ij3(0, a) * np + p] * d[0];
+        s = fma(ij[ij3(1, a) * np + p], d[1], s);
+        s = fma(ij[ij3(2, a) * np + p], d[2], s);
+        du[3 * nc * m.off + (3 * c + a) * np + p] = s;
+      }
+    }
+  }
+}
+
+// GH point-wise terms from u and du (ragged layouts).
+__global__ void __launch_bounds__(256)
+gh_mixed_pointwise_kernel(const MixedElement* __restrict__ el, const int32_t* __restrict__ list,
+                          const double* __restrict__ u, const double* __restrict__ du,
+                          const double* __restrict__ gammas, double* __restrict__ dt) {
+  const MixedElement m = el[list[blockIdx.x]];
+  const int np = m.np;
+  for (int p = threadIdx.x; p < np; p += blockDim.x) {
+    const double* ue = u + 50 * m.off + p;
+    const double* due = du + 150 * m.off + p;
+    double up[50];
+#pragma unroll
+    for (int c = 0; c < 50; ++c) up[c] = ue[c * np];
+    const double* ga = gammas + 3 * m.off + p;
+    GhPointState st;
+    gh_point_setup(st, up, ga[0], ga[np], ga[2 * np]);
+    double* dte = dt + 50 * m.off + p;
+#pragma unroll
+    for (int pr = 0; pr < 10; ++pr) {
+      double phi[3], dg[3], dpi[3], dphi[9], out[5];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        phi[i] = up[20 + i + 3 * pr];
+        dg[i] = due[(3 * pr + i) * np];
+        dpi[i] = due[(30 + 3 * pr + i) * np];
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int mm = 0; mm < 3; ++mm) dphi[mm + 3 * q] = due[(60 + 3 * (q + 3 * pr) + mm) * np];
+      gh_point_pair(out, st, st.dt_pi_partial[pr], up[10 + pr], phi, dg, dpi, dphi);
+      dte[pr * np] = out[0];
+      dte[(10 + pr) * np] = out[1];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) dte[(20 + i + 3 * pr) * np] = out[2 + i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Face coupling.
+//
+// Apply per-dimension matrices to `nc` components on a 2-d mesh, as the
+// oracle's apply_face_matrices: in [nc][ia ib] -> out [nc][oa ob] through
+// tmp [nc][oa ib]; Ma [oa][ia] (skipped when ia == oa), Mb [ob][ib]
+// (skipped when ib == ob).  Includes the barriers between the steps.
+__device__ void apply_face_matrices(int nc, int ia, int ib, int oa, int ob, const double* Ma,
+                                    const double* Mb, const double* in, double* tmp,
+                                    double* out) {
+  const int t = threadIdx.x, T = blockDim.x;
+  const double* src = in;
+  if (ia != oa) {
+    for (int x = t; x < nc * oa * ib; x += T) {
+      const int c = x / (oa * ib), rem = x % (oa * ib), b = rem / oa, r = rem % oa;
+      const double* v = in + c * ia * ib + ia * b;
+      double s = Ma[r * ia] * v[0];
+      for (int l = 1; l < ia; ++l) s = fma(Ma[r * ia + l], v[l], s);
+      tmp[x] = s;
+    }
+    __syncthreads();
+    src = tmp;
+  }
+  for (int x = t; x < nc * oa * ob; x += T) {
+    const int c = x / (oa * ob), rem = x % (oa * ob), r = rem / oa, a = rem % oa;
+    const double* v = src + c * oa * ib + a;
+    double s;
+    if (ib == ob) {
+      s = v[oa * r];
+    } else {
+      s = Mb[r * ib] * v[0];
+      for (int l = 1; l < ib; ++l) s = fma(Mb[r * ib + l], v[oa * l], s);
+    }
+    out[x] = s;
+  }
+  __syncthreads();
+}
+
+// LDS plan of the face kernel for faces of at most kf points and mortars of
+// at most km points (doubles).
+struct FacePlan {
+  int side_own, side_nb, pk_own, pk_nb, tmp, m_own, m_nb, lam_own, lam_nb, bc_m, bc, total;
+  __host__ __device__ FacePlan(int kf, int km, int npk) {
+    side_own = 0;                       // 12 per own face point
+    side_nb = side_own + 12 * kf;       // 11 per neighbour face point
+    pk_own = side_nb + 11 * kf;         // npk per own face point
+    pk_nb = pk_own + npk * kf;          // npk per neighbour face point
+    tmp = pk_nb + npk * kf;             // npk per mortar point
+    m_own = tmp + npk * km;             // packages on the mortar
+    m_nb = m_own + npk * km;
+    lam_own = m_nb + npk * km;          // GH speeds on the mortar
+    lam_nb = lam_own + 4 * km;
+    bc_m = lam_nb + 4 * km;             // 5 corrections per mortar point
+    bc = bc_m + 5 * km;                 // 5 per own face point
+    total = bc + 5 * kf;
+  }
+};
+
+// Side data of face point f: SW n[3], mag; GH n[3], nv[3], lam[4], gamma2 (+ mag).
+template <bool GH>
+__device__ void face_side(const MixedElement& m, int d, int side, bool flip, int f,
+                          const double* __restrict__ u, const double* __restrict__ params,
+                          const double* __restrict__ inv_jac, double* out, bool own) {
+  const int na = d == 0 ? m.n[1] : m.n[0];
+  const int p = face_point(m, d, side, f % na, f / na);
+  const int np = m.np;
+  double n[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    n[i] = inv_jac[9 * m.off + ij3(d, i) * np + p];
+    if (flip) n[i] = n[i] * -1.0;
+  }
+  if constexpr (!GH) {
+    double mag;
+    flat_normalize(n, &mag);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[i] = n[i];
+    out[3] = mag;
+  } else {
+    double g[10];
+#pragma unroll
+    for (int c = 0; c < 10; ++c) g[c] = u[50 * m.off + c * np + p];
+    double lapse, shift[3], inv_sp[6], mag;
+    gh_lapse_shift(&lapse, shift, inv_sp, g);
+    const double* ga = params + 3 * m.off + p;
+    GhFaceSide s;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) s.n[i] = n[i];
+    gh_normalize(s.n, s.nv, &mag, inv_sp);
+    gh_char_speeds(s.lam, ga[np], lapse, shift, s.n);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      out[i] = s.n[i];
+      out[3 + i] = s.nv[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out[6 + i] = s.lam[i];
+    out[10] = ga[2 * np];
+    if (own) out[11] = mag;
+  }
+}
+
+// Package of pass `pr` (GH pair; SW: the single pass) at face point f into
+// pk[c * nf + f].
+template <bool GH>
+__device__ void face_package(const MixedElement& m, int d, int side, int f, int nf, int pr,
+                             const double* __restrict__ u, const double* __restrict__ params,
+                             const double* sd, double* pk) {
+  const int na = d == 0 ? m.n[1] : m.n[0];
+  const int p = face_point(m, d, side, f % na, f / na);
+  const int np = m.np;
+  double out[13];
+  if constexpr (!GH) {
+    double up[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) up[c] = u[5 * m.off + c * np + p];
+    sw_package(out, up, params[m.off + p], sd);
+  } else {
+    GhFaceSide s;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      s.n[i] = sd[i];
+      s.nv[i] = sd[3 + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s.lam[i] = sd[6 + i];
+    s.gamma2 = sd[10];
+    const double* ue = u + 50 * m.off + p;
+    double phi[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) phi[i] = ue[(20 + i + 3 * pr) * np];
+    gh_package_pair(out, ue[pr * np], ue[(10 + pr) * np], phi, s);
+  }
+#pragma unroll
+  for (int c = 0; c < 13; ++c) pk[c * nf + f] = out[c];
+}
+
+// Dynamic LDS: FacePlan(kf, km, 13).total doubles.
+template <bool GH>
+__global__ void __launch_bounds__(256)
+mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restrict__ list,
+                  const int32_t* __restrict__ nbr, const double* __restrict__ u,
+                  const double* __restrict__ params, const double* __restrict__ inv_jac,
+                  double* __restrict__ dt, const double* __restrict__ mortar_tab, int kf,
+                  int km) {
+  extern __shared__ double sm[];
+  constexpr int NC = GH ? 50 : 5;
+  constexpr int SIDE_OWN = GH ? 12 : 4, SIDE_NB = GH ? 11 : 4;
+  const FacePlan L(kf, km, 13);
+  const int t = threadIdx.x, T = blockDim.x;
+  const int e = list[blockIdx.x];
+  const MixedElement m = el[e];
+  for (int dir = 0; dir < 6; ++dir) {
+    const int d = dir >> 1, side = dir & 1;
+    const MixedElement mn = el[nbr[6 * e + dir]];
+    const int na = d == 0 ? m.n[1] : m.n[0], nb = d == 2 ? m.n[1] : m.n[2];
+    const int ma = d == 0 ? mn.n[1] : mn.n[0], mb = d == 2 ? mn.n[1] : mn.n[2];
+    const int ka = max(na, ma), kb = max(nb, mb);
+    const int nf = na * nb, mf = ma * mb, kmf = ka * kb;
+    const double* Ia_own = mortar_tab + mortar_table_offset(na, ka);
+    const double* Ib_own = mortar_tab + mortar_table_offset(nb, kb);
+    const double* Ia_nb = mortar_tab + mortar_table_offset(ma, ka);
+    const double* Ib_nb = mortar_tab + mortar_table_offset(mb, kb);
+    const double* Pa = Ia_own + na * ka;  // projection block follows the interpolation
+    const double* Pb = Ib_own + nb * kb;
+    __syncthreads();  // previous face done with the LDS
+    for (int f = t; f < nf; f += T)
+      face_side<GH>(m, d, side, side == 0, f, u, params, inv_jac, sm + L.side_own + SIDE_OWN * f,
+                    true);
+    for (int f = t; f < mf; f += T)
+      face_side<GH>(mn, d, 1 - side, side == 1, f, u, params, inv_jac,
+                    sm + L.side_nb + SIDE_NB * f, false);
+    __syncthreads();
+    if constexpr (GH) {
+      // characteristic speeds on the mortar (part of the packaged data)
+      for (int f = t; f < nf; f += T)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sm[L.pk_own + c * nf + f] = sm[L.side_own + 12 * f + 6 + c];
+      for (int f = t; f < mf; f += T)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sm[L.pk_nb + c * mf + f] = sm[L.side_nb + 11 * f + 6 + c];
+      __syncthreads();
+      apply_face_matrices(4, na, nb, ka, kb, Ia_own, Ib_own, sm + L.pk_own, sm + L.tmp,
+                          sm + L.lam_own);
+      apply_face_matrices(4, ma, mb, ka, kb, Ia_nb, Ib_nb, sm + L.pk_nb, sm + L.tmp,
+                          sm + L.lam_nb);
+    }
+    constexpr int PASSES = GH ? 10 : 1;
+    for (int pr = 0; pr < PASSES; ++pr) {
+      for (int f = t; f < nf; f += T)
+        face_package<GH>(m, d, side, f, nf, pr, u, params, sm + L.side_own + SIDE_OWN * f,
+                         sm + L.pk_own);
+      for (int f = t; f < mf; f += T)
+        face_package<GH>(mn, d, 1 - side, f, mf, pr, u, params, sm + L.side_nb + SIDE_NB * f,
+                         sm + L.pk_nb);
+      __syncthreads();
+      apply_face_matrices(13, na, nb, ka, kb, Ia_own, Ib_own, sm + L.pk_own, sm + L.tmp,
+                          sm + L.m_own);
+      apply_face_matrices(13, ma, mb, ka, kb, Ia_nb, Ib_nb, sm + L.pk_nb, sm + L.tmp,
+                          sm + L.m_nb);
+      for (int k = t; k < kmf; k += T) {
+        double in[13], ex[13], bc[5];
+#pragma unroll
+        for (int c = 0; c < 13; ++c) {
+          in[c] = sm[L.m_own + c * kmf + k];
+          ex[c] = sm[L.m_nb + c * kmf + k];
+        }
+        if constexpr (GH) {
+          double li[4], le[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            li[c] = sm[L.lam_own + c * kmf + k];
+            le[c] = sm[L.lam_nb + c * kmf + k];
+          }
+          gh_boundary_terms_pair(bc, in, li, ex, le);
+        } else {
+          sw_boundary_terms(bc, in, ex);
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) sm[L.bc_m + c * kmf + k] = bc[c];
+      }
+      __syncthreads();
+      apply_face_matrices(5, ka, kb, na, nb, Pa, Pb, sm + L.bc_m, sm + L.tmp, sm + L.bc);
+      // lift into dt
+      for (int f = t; f < nf; f += T) {
+        const int p = face_point(m, d, side, f % na, f / na);
+        const double fac = lift_factor(m.n[d], sm[L.side_own + SIDE_OWN * f + SIDE_OWN - 1]);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+          // component c of the pass: g/Pi/Phi_{0..2} of pair pr (GH); c (SW)
+          const int comp = !GH ? c : c == 0 ? pr : c == 1 ? 10 + pr : 18 + c + 3 * pr;
+          double& target = dt[NC * m.off + comp * m.np + p];
+          target = target + sm[L.bc + c * nf + f] * fac;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace
+
+int mixed_face_lds_bytes(int kf, int km) { return FacePlan(kf, km, 13).total * 8; }
+
+int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement* el,
+                     const int32_t* nbr, int np_max, int kf, int km, const double* u,
+                     const double* params, const double* inv_jac, double* dt, double* scratch_du,
+                     int threads, hipStream_t stream) {
+  if (count == 0) return SXD_OK;
+  const double* dtab = device_diff_matrices();
+  const double* mtab = device_mortar_matrices();
+  if (dtab == nullptr || mtab == nullptr) return SXD_ERR_DEVICE;
+  if (threads != 64 && threads != 128 && threads != 256) return SXD_ERR_INVALID_ARGUMENT;
+  const int face_lds = mixed_face_lds_bytes(kf, km);
+  if (face_lds > 160 * 1024) return SXD_ERR_UNSUPPORTED;
+  // dynamic LDS beyond the default 64 KiB has to be enabled per kernel
+  const int lds_max = 160 * 1024;
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&sw_mixed_volume_kernel),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_face_kernel<false>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_face_kernel<true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess)
+    return SXD_ERR_DEVICE;
+  if (!gh) {
+    if (5 * np_max * 8 > 160 * 1024) return SXD_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL(sw_mixed_volume_kernel, dim3(count), dim3(threads), 5 * np_max * 8, stream,
+                       el, list, u, params, inv_jac, dt, dtab);
+    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+    hipLaunchKernelGGL(mixed_face_kernel<false>, dim3(count), dim3(threads), face_lds, stream, el,
+                       list, nbr, u, params, inv_jac, dt, mtab, kf, km);
+    return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
+  }
+  if (scratch_du == nullptr) return SXD_ERR_INVALID_ARGUMENT;
+  hipLaunchKernelGGL(ragged_partial_derivatives_kernel,
+                     dim3(count, (50 + kRaggedChunk - 1) / kRaggedChunk), dim3(threads),
+                     np_max * 8, stream, el, list, 50, u, inv_jac, scratch_du, dtab);
+  if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  hipLaunchKernelGGL(gh_mixed_pointwise_kernel, dim3(count), dim3(threads), 0, stream, el, list,
+                     u, scratch_du, params, dt);
+  if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  hipLaunchKernelGGL(mixed_face_kernel<true>, dim3(count), dim3(threads), face_lds, stream, el,
+                     list, nbr, u, params, inv_jac, dt, mtab, kf, km);
+  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
+}
+
+}  // namespace sxd

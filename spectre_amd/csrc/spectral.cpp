@@ -8,6 +8,14 @@
 // Barycentric weights: Kopriva Alg. 30 (Spectral.cpp:83-101).
 // D_ij = (b_j / b_i) / (x_i - x_j), D_ii = -sum_{j != i} D_ij
 // (Kopriva Alg. 37, Spectral.cpp:113-150).
+// Mortar matrices (mixed p, Spectral/Projection.cpp with ChildSize::Full):
+// element-to-mortar = barycentric interpolation onto the mortar's LGL points
+// (Kopriva Alg. 32, interpolation_matrix in Spectral.cpp:625-667, including
+// its unit row for a target within roundoff of a node); mortar-to-element =
+// modal truncation V_face[:, :nf] Vinv_mortar[:nf, :] (Projection.cpp:97-136)
+// with V(i, k) = P_k(x_i) and the exact inverse of the LGL Vandermonde matrix,
+// Vinv(k, j) = w_j P_k(x_j) / gamma_k, gamma_k = 2 / (2k + 1) (k < n - 1),
+// gamma_{n-1} = 2 / (n - 1).
 #include "spectral.hpp"
 
 #include <cmath>
@@ -106,6 +114,83 @@ void lgl_differentiation_matrix(int n, double* d) {
     }
     d[i * n + i] = diag;
   }
+}
+
+namespace {
+double legendre_p(int k, double x) {
+  if (k == 0) return 1.0;
+  double prev = 1.0, cur = x;
+  for (int j = 2; j <= k; ++j) {
+    const double next = ((2.0 * j - 1.0) * x * cur - (j - 1.0) * prev) / j;
+    prev = cur;
+    cur = next;
+  }
+  return cur;
+}
+}  // namespace
+
+void mortar_interpolation_matrix(int nf, int nm, double* out) {
+  double xf[kMaxPoints], wf[kMaxPoints], xm[kMaxPoints], wm[kMaxPoints], b[kMaxPoints];
+  lgl_points_and_weights(nf, xf, wf);
+  lgl_points_and_weights(nm, xm, wm);
+  for (int k = 0; k < nf; ++k) b[k] = 1.0;
+  for (int j = 1; j < nf; ++j)
+    for (int k = 0; k < j; ++k) {
+      b[k] *= xf[k] - xf[j];
+      b[j] *= xf[j] - xf[k];
+    }
+  for (int k = 0; k < nf; ++k) b[k] = 1.0 / b[k];
+  const double eps = 100.0 * 2.220446049250313e-16;
+  for (int r = 0; r < nm; ++r) {
+    double* row = out + r * nf;
+    bool match = false;
+    for (int j = 0; j < nf; ++j) {
+      row[j] = 0.0;
+      if (std::fabs(xm[r] - xf[j]) < (std::fmax(std::fabs(xm[r]), std::fabs(xf[j])) + 1.0) * eps) {
+        row[j] = 1.0;
+        match = true;
+      }
+    }
+    if (match) continue;
+    double sum = 0.0;
+    for (int j = 0; j < nf; ++j) {
+      row[j] = b[j] / (xm[r] - xf[j]);
+      sum += row[j];
+    }
+    for (int j = 0; j < nf; ++j) row[j] /= sum;
+  }
+}
+
+void mortar_projection_matrix(int nf, int nm, double* out) {
+  double xf[kMaxPoints], wf[kMaxPoints], xm[kMaxPoints], wm[kMaxPoints];
+  lgl_points_and_weights(nf, xf, wf);
+  lgl_points_and_weights(nm, xm, wm);
+  double vinv[kMaxPoints * kMaxPoints];
+  for (int k = 0; k < nm; ++k) {
+    const double gamma = k < nm - 1 ? 2.0 / (2.0 * k + 1.0) : 2.0 / (nm - 1.0);
+    for (int j = 0; j < nm; ++j) vinv[k * nm + j] = wm[j] * legendre_p(k, xm[j]) / gamma;
+  }
+  for (int i = 0; i < nf; ++i)
+    for (int j = 0; j < nm; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < nf; ++k) s = std::fma(legendre_p(k, xf[i]), vinv[k * nm + j], s);
+      out[i * nm + j] = s;
+    }
+}
+
+const std::vector<double>& packed_mortar_matrices() {
+  static std::vector<double> table;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    table.assign(kPackedMortarSize, 0.0);
+    for (int nf = kMinPoints; nf <= kMaxPoints; ++nf)
+      for (int nm = nf; nm <= kMaxPoints; ++nm) {
+        double* blk = table.data() + packed_mortar_offset(nf, nm);
+        mortar_interpolation_matrix(nf, nm, blk);
+        mortar_projection_matrix(nf, nm, blk + nf * nm);
+      }
+  });
+  return table;
 }
 
 const std::vector<double>& packed_differentiation_matrices() {

@@ -16,9 +16,30 @@ constexpr int packed_d_offset(int n) {
 }
 constexpr int kPackedDSize = packed_d_offset(kMaxPoints + 1);
 
+// Mortar tables of mixed-p faces, for every (n_face, n_mortar) with
+// 2 <= n_face <= n_mortar <= 20: the element-to-mortar interpolation
+// [n_mortar][n_face] followed by the mortar-to-element projection
+// [n_face][n_mortar], both row-major, at packed_mortar_offset(n_face, n_mortar).
+constexpr int packed_mortar_offset(int nf, int nm) {
+  int off = 0;
+  for (int a = kMinPoints; a <= kMaxPoints; ++a)
+    for (int b = a; b <= kMaxPoints; ++b) {
+      if (a == nf && b == nm) return off;
+      off += 2 * a * b;
+    }
+  return off;
+}
+constexpr int kPackedMortarSize = packed_mortar_offset(kMaxPoints + 1, 0);
+
 void lgl_points_and_weights(int n, double* x, double* w);
 void lgl_differentiation_matrix(int n, double* d);  // row-major
 // All D matrices for n = 2..20, row-major, at packed_d_offset(n).
 const std::vector<double>& packed_differentiation_matrices();
+// Interpolation of a function given at the nf LGL points onto the nm LGL
+// points (row-major [nm][nf]).
+void mortar_interpolation_matrix(int nf, int nm, double* out);
+// Modal-truncation projection from nm LGL points onto nf (row-major [nf][nm]).
+void mortar_projection_matrix(int nf, int nm, double* out);
+const std::vector<double>& packed_mortar_matrices();
 
 }  // namespace sxd

@@ -198,9 +198,12 @@ class Domain:
     def _shapes(self, nc):
         return (self.num_elements, nc, self.num_points)
 
+    def _scalar_shape(self):
+        return (self.num_elements, self.num_points)
+
     def scalar_wave_rhs(self, u, gamma2, inv_jac, out=None):
         _require(u, "u", self._shapes(5))
-        _require(gamma2, "gamma2", (self.num_elements, self.num_points))
+        _require(gamma2, "gamma2", self._scalar_shape())
         _require(inv_jac, "inv_jac", self._shapes(9))
         if out is None:
             out = torch.empty_like(u)
@@ -238,7 +241,7 @@ class Domain:
         _require(out, "out", self._shapes(nc))
         _require(inv_jac, "inv_jac", self._shapes(9))
         if system == "scalar_wave":
-            _require(params, "gamma2", (self.num_elements, self.num_points))
+            _require(params, "gamma2", self._scalar_shape())
         else:
             _require(params, "gammas", self._shapes(3))
         k = ctypes.c_int()
@@ -252,3 +255,43 @@ class Domain:
 __all__ = ["SpectreAmdError", "partial_derivatives", "logical_partial_derivatives",
            "scalar_wave_time_derivative", "gh_time_derivative", "scalar_wave_upwind_penalty",
            "gh_upwind_penalty", "lift_flux", "rk3_substep", "Domain"]
+
+
+class MixedDomain(Domain):
+    """Mixed-p element set (sxd_domain_create_mixed): element e has its own
+    extents; fields are flat tensors in the ragged layout (element e's nc
+    blocks of Np_e points at nc * offset_e)."""
+
+    def __init__(self, extents, neighbors):
+        nb = np.ascontiguousarray(neighbors, dtype=np.int32)
+        ext = np.ascontiguousarray(extents, dtype=np.int32)
+        if nb.ndim != 2 or nb.shape[1] != 6 or ext.shape != (nb.shape[0], 3):
+            raise ValueError("need extents [E, 3] and neighbors [E, 6]")
+        self.extents = ext
+        self.num_elements = nb.shape[0]
+        handle = ctypes.c_void_p()
+        check(lib().sxd_domain_create_mixed(ctypes.byref(handle), self.num_elements,
+                                            ext.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                            nb.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        self._handle = handle
+        total = ctypes.c_int64()
+        check(lib().sxd_domain_num_points(self._handle, ctypes.byref(total)))
+        self.total_points = total.value
+
+    @classmethod
+    def from_brick(cls, brick):
+        return cls(brick.extents, brick.neighbors)
+
+    def _shapes(self, nc):
+        return (nc * self.total_points,)
+
+    def _scalar_shape(self):
+        return (self.total_points,)
+
+    def set_threads(self, system, threads):
+        check(lib().sxd_domain_set_threads(self._handle, SYSTEMS[system], threads))
+
+    def threads(self, system):
+        t = ctypes.c_int()
+        check(lib().sxd_domain_get_threads(self._handle, SYSTEMS[system], ctypes.byref(t)))
+        return t.value

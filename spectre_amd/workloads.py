@@ -143,6 +143,20 @@ PAIRS = [(a, b) for a in range(4) for b in range(a, 4)]
 
 
 # ---------------------------------------------------------------------------
+def periodic_neighbors(elements):
+    """[E][6] neighbour table of a periodic Ex x Ey x Ez brick (element index
+    ix + Ex (iy + Ey iz)); entry 2 d + side is the element across the lower
+    (side 0) or upper (side 1) face in dimension d."""
+    ex, ey, ez = elements
+    e = np.arange(ex * ey * ez)
+    ix, iy, iz = e % ex, (e // ex) % ey, e // (ex * ey)
+    idx = lambda a, b, c: (a % ex) + ex * ((b % ey) + ey * (c % ez))
+    return np.stack([
+        idx(ix - 1, iy, iz), idx(ix + 1, iy, iz),
+        idx(ix, iy - 1, iz), idx(ix, iy + 1, iz),
+        idx(ix, iy, iz - 1), idx(ix, iy, iz + 1)], axis=1).astype(np.int32)
+
+
 @dataclass
 class Brick:
     """A periodic brick of Ex x Ey x Ez affine elements with uniform extents."""
@@ -153,14 +167,7 @@ class Brick:
     neighbors: np.ndarray = field(init=False)
 
     def __post_init__(self):
-        ex, ey, ez = self.elements
-        e = np.arange(ex * ey * ez)
-        ix, iy, iz = e % ex, (e // ex) % ey, e // (ex * ey)
-        idx = lambda a, b, c: (a % ex) + ex * ((b % ey) + ey * (c % ez))
-        self.neighbors = np.stack([
-            idx(ix - 1, iy, iz), idx(ix + 1, iy, iz),
-            idx(ix, iy - 1, iz), idx(ix, iy + 1, iz),
-            idx(ix, iy, iz - 1), idx(ix, iy, iz + 1)], axis=1).astype(np.int32)
+        self.neighbors = periodic_neighbors(self.elements)
 
     @property
     def num_elements(self):
@@ -196,6 +203,71 @@ class Brick:
         for d in range(3):
             inv[:, d + 3 * d, :] = 2.0 / self.widths[d]
         return inv
+
+
+class MixedBrick:
+    """A periodic brick of affine elements whose per-dimension LGL extents
+    differ from element to element (p-refinement).  Fields use the ragged
+    Variables layout: element e's nc blocks of Np_e points start at
+    nc * offsets[e] of a flat array."""
+
+    def __init__(self, lower, upper, elements, extents):
+        self.lower, self.upper, self.elements = tuple(lower), tuple(upper), tuple(elements)
+        self.extents = np.ascontiguousarray(extents, dtype=np.int32).reshape(-1, 3)
+        if self.extents.shape[0] != int(np.prod(elements)):
+            raise ValueError("one extents triple per element")
+        self.neighbors = periodic_neighbors(self.elements)
+        self.points = np.prod(self.extents, axis=1).astype(np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(self.points)]).astype(np.int64)
+
+    @property
+    def num_elements(self):
+        return self.extents.shape[0]
+
+    @property
+    def total_points(self):
+        return int(self.offsets[-1])
+
+    @property
+    def widths(self):
+        return tuple((self.upper[d] - self.lower[d]) / self.elements[d] for d in range(3))
+
+    def element_coordinates(self, e):
+        """Inertial coordinates [3][Np_e] of element e."""
+        ex, ey, _ = self.elements
+        ie = (e % ex, (e // ex) % ey, e // (ex * ey))
+        n0, n1, n2 = (int(v) for v in self.extents[e])
+        xs = [spectral.lgl_points(n) for n in (n0, n1, n2)]
+        k, j, i = np.meshgrid(np.arange(n2), np.arange(n1), np.arange(n0), indexing="ij")
+        logical = [xs[0][i.ravel()], xs[1][j.ravel()], xs[2][k.ravel()]]
+        h = self.widths
+        return np.stack([self.lower[d] + h[d] * ie[d] + 0.5 * h[d] * (logical[d] + 1.0)
+                         for d in range(3)])
+
+    def ragged(self, per_element):
+        """Flatten a list of [nc][Np_e] arrays into the ragged layout."""
+        return np.ascontiguousarray(np.concatenate([np.asarray(a).ravel() for a in per_element]))
+
+    def split(self, flat, nc):
+        """The list of [nc][Np_e] blocks of a ragged array."""
+        return [flat[nc * self.offsets[e]:nc * self.offsets[e + 1]].reshape(nc, -1)
+                for e in range(self.num_elements)]
+
+    def inverse_jacobian(self):
+        out = []
+        for e in range(self.num_elements):
+            inv = np.zeros((9, int(self.points[e])))
+            for d in range(3):
+                inv[d + 3 * d] = 2.0 / self.widths[d]
+            out.append(inv)
+        return self.ragged(out)
+
+
+def random_extents(num_elements, lowest=4, highest=12, seed=0):
+    """Per-element, per-dimension LGL extents drawn i.i.d. uniformly from
+    {lowest .. highest}."""
+    rng = np.random.default_rng(seed)
+    return rng.integers(lowest, highest + 1, size=(num_elements, 3)).astype(np.int32)
 
 
 # ---------------------------------------------------------------------------
@@ -341,4 +413,34 @@ def gh_kerr_schild_brick(seed=0, elements=(8, 8, 8), extents=(8, 8, 8),
     gammas[:, 0] = 1.0
     gammas[:, 1] = -1.0
     gammas[:, 2] = 1.0
+    return brick, u, gammas, brick.inverse_jacobian()
+
+
+def scalar_wave_plane_wave_mixed(seed=0, elements=(10, 10, 10), lowest=4, highest=12):
+    """Mixed-p ScalarWave workload: the plane wave of scalar_wave_plane_wave_3d
+    on the periodic brick [-1,1]^3 of Ex x Ey x Ez elements with extents drawn
+    from {lowest..highest}; gamma2 = 1.  Returns (brick, u, gamma2, inv_jac)
+    in the ragged layout."""
+    ext = random_extents(int(np.prod(elements)), lowest, highest, seed)
+    brick = MixedBrick((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), elements, ext)
+    u = brick.ragged([plane_wave(brick.element_coordinates(e)[None], seed)[0][0]
+                      for e in range(brick.num_elements)])
+    gamma2 = np.ones(brick.total_points)
+    return brick, u, gamma2, brick.inverse_jacobian()
+
+
+def gh_kerr_schild_mixed(seed=0, elements=(10, 10, 10), lowest=4, highest=12, noise=1.0e-6,
+                         lower=(3.0, -4.0, -4.0), upper=(11.0, 4.0, 4.0)):
+    """Mixed-p GH workload: the Kerr-Schild state of gh_kerr_schild_brick on
+    Ex x Ey x Ez elements with extents drawn from {lowest..highest}.  Returns
+    (brick, u, gammas, inv_jac) in the ragged layout."""
+    ext = random_extents(int(np.prod(elements)), lowest, highest, seed)
+    brick = MixedBrick(lower, upper, elements, ext)
+    u = brick.ragged([gh_kerr_schild(brick.element_coordinates(e)[None])[0]
+                      for e in range(brick.num_elements)])
+    if noise:
+        rng = np.random.default_rng(seed)
+        u *= 1.0 + noise * rng.standard_normal(u.shape)
+    gammas = brick.ragged([np.stack([np.ones(n), -np.ones(n), np.ones(n)])
+                           for n in brick.points])
     return brick, u, gammas, brick.inverse_jacobian()

@@ -209,3 +209,43 @@ def rk3_substep(substep, dt_step, u0, uk, lk):
     _check(lib().sxo_rk3_substep(substep, ctypes.c_long(u0.size), ctypes.c_double(dt_step),
                                  _d(u0), _d(uk), _d(lk), _d(out)))
     return out
+
+
+def mortar_interpolation_matrix(n_face, n_mortar):
+    out = np.empty((n_mortar, n_face))
+    _check(lib().sxo_mortar_interpolation_matrix(n_face, n_mortar, _d(out)))
+    return out
+
+
+def mortar_projection_matrix(n_face, n_mortar):
+    out = np.empty((n_face, n_mortar))
+    _check(lib().sxo_mortar_projection_matrix(n_face, n_mortar, _d(out)))
+    return out
+
+
+def lgl_interpolation_matrix(n, targets):
+    targets = np.ascontiguousarray(targets, dtype=np.float64)
+    out = np.empty((targets.size, n))
+    _check(lib().sxo_lgl_interpolation_matrix(n, targets.size, _d(targets), _d(out)))
+    return out
+
+
+def sw_rhs_mixed(extents, neighbors, u, gamma2, inv_jac):
+    """Ragged layout (element e's blocks at nc * offset_e)."""
+    u, gamma2, inv_jac = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, gamma2, inv_jac))
+    dt = np.empty_like(u)
+    ext, ep = _i(np.asarray(extents).ravel())
+    nb, nbp = _i(np.asarray(neighbors).ravel())
+    _check(lib().sxo_sw_rhs_mixed(len(ext) // 3, ep, nbp, _d(u), _d(gamma2), _d(inv_jac),
+                                  _d(dt)))
+    return dt
+
+
+def gh_rhs_mixed(extents, neighbors, u, gammas, inv_jac):
+    u, gammas, inv_jac = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, gammas, inv_jac))
+    dt = np.empty_like(u)
+    ext, ep = _i(np.asarray(extents).ravel())
+    nb, nbp = _i(np.asarray(neighbors).ravel())
+    _check(lib().sxo_gh_rhs_mixed(len(ext) // 3, ep, nbp, _d(u), _d(gammas), _d(inv_jac),
+                                  _d(dt)))
+    return dt
