@@ -41,6 +41,105 @@ def algorithmic_bytes_per_point(nc, n_aux, n):
     return 8.0 * (nc + 9 + n_aux + nc) + 8.0 * nc * (6.0 * n * n / n ** 3)
 
 
+def mixed_algorithmic_bytes(brick, nc, n_aux):
+    """Compulsory HBM bytes of one mixed-p RHS: every element's fields,
+    inverse Jacobian and damping parameters read once, dt written once, plus
+    the neighbour face traces each face reads (nc values per neighbour face
+    point)."""
+    import numpy as np
+    total = 8.0 * (2 * nc + 9 + n_aux) * brick.total_points
+    for e in range(brick.num_elements):
+        for d in range(3):
+            for side in range(2):
+                en = brick.neighbors[e, 2 * d + side]
+                ext = brick.extents[en]
+                total += 8.0 * nc * int(np.prod(ext)) // int(ext[d])
+    return total
+
+
+def time_calls(fn, steps, warmup):
+    """Mean ms per call with HIP events on the launch stream."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        fn()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps
+
+
+def measure_mixed(system, steps, warmup, rk3_steps=5):
+    """Mixed-p workload (1000 elements of a 10x10x10 brick, extents drawn from
+    {4..12} per element and dimension): the batched work-list launch against
+    one launch per group of equal extents, with the autotuner's threads per
+    workgroup, and one RK3 step replayed from a HIP graph against the same
+    step enqueued call by call."""
+    import numpy as np
+    import torch
+    from spectre_amd import dg, workloads as wl
+
+    if system == "scalar_wave":
+        brick, u, params, inv = wl.scalar_wave_plane_wave_mixed(seed=0)
+        nc, n_aux = 5, 1
+        rhs_name = "scalar_wave_rhs"
+    else:
+        brick, u, params, inv = wl.gh_kerr_schild_mixed(seed=0)
+        nc, n_aux = 50, 3
+        rhs_name = "gh_rhs"
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    d_u, d_p, d_inv = dev(u), dev(params), dev(inv)
+    d_dt = torch.empty_like(d_u)
+    domain = dg.MixedDomain.from_brick(brick)
+    rhs = getattr(domain, rhs_name)
+    chosen, _ = domain.autotune(system, d_u, d_p, d_inv, d_dt, repetitions=5)
+    threads = domain.threads(system)
+    ms = {}
+    for kernel in ("mixed_batched", "mixed_per_extents"):
+        domain.set_kernel(system, kernel)
+        ms[kernel] = time_calls(lambda: rhs(d_u, d_p, d_inv, out=d_dt), steps, warmup)
+    domain.set_kernel(system, "mixed_batched")
+    # one RK3 step: graph replay against call-by-call enqueueing
+    dt_step = 1e-3
+    d_u2 = d_u.clone()
+    stepper = dg.Rk3Stepper(domain, system, d_u2, d_p, d_inv)
+    graph_ms = time_calls(lambda: stepper.step(dt_step, 1), rk3_steps, 1)
+    stepper.close()
+    stage, l = torch.empty_like(d_u), torch.empty_like(d_u)
+
+    def eager_step():
+        rhs(d_u2, d_p, d_inv, out=l)
+        dg.rk3_substep(0, dt_step, d_u2, d_u2, l, out=stage)
+        rhs(stage, d_p, d_inv, out=l)
+        dg.rk3_substep(1, dt_step, d_u2, stage, l, out=stage)
+        rhs(stage, d_p, d_inv, out=l)
+        dg.rk3_substep(2, dt_step, d_u2, stage, l, out=d_u2)
+
+    eager_ms = time_calls(eager_step, rk3_steps, 1)
+    points = brick.total_points
+    alg = mixed_algorithmic_bytes(brick, nc, n_aux)
+    b = ms["mixed_batched"]
+    return {
+        "workload": f"{system}_mixed_p_1000el_extents_4..12",
+        "grid_points": points,
+        "extents_groups": len({tuple(e) for e in brick.extents.tolist()}),
+        "autotuned": {"kernel": chosen, "threads_per_workgroup": threads},
+        "batched_ms": round(b, 6),
+        "per_extents_ms": round(ms["mixed_per_extents"], 6),
+        "batch_gain": round(ms["mixed_per_extents"] / b, 3),
+        "grid_points_per_s": round(points / (b * 1e-3), 1),
+        "hbm_frac": round(alg / (b * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "algorithmic_bytes": alg,
+        "rk3_step_graph_ms": round(graph_ms, 6),
+        "rk3_step_call_by_call_ms": round(eager_ms, 6),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +152,8 @@ def main():
                     choices=["tune", "auto", "fused", "fused_mfma", "composite"],
                     help="kernel organisation of the RHS; 'tune' runs the library's "
                          "autotuner on the workload first and keeps its choice")
+    ap.add_argument("--extensions", type=int, default=1,
+                    help="also measure the mixed-p batched form (1) or not (0)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (a plain copy "
                          "kernel of known size, to calibrate HBM byte counters) and exit")
@@ -171,6 +272,10 @@ def main():
                          f"{single:.4g} pts/s from one evaluation",
                "single_core_value": round(single, 1)}
 
+    extensions = None
+    if args.extensions:
+        extensions = {"mixed_p": [measure_mixed(s, 20, 3) for s in ("scalar_wave", "gh")]}
+
     result = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -190,6 +295,7 @@ def main():
                    "kernel": kernel_used, "kernel_choice": args.kernel},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "extensions": extensions,
         "wall_s_timed_region": round(wall, 4),
     }
     print(json.dumps(result))

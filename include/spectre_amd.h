@@ -221,6 +221,27 @@ int sxd_rk3_substep(int substep, int64_t n, double dt, const double* u0,
                     const double* uk, const double* lk, double* out,
                     void* stream);
 
+/* A full Rk3HesthavenSsp step (three right-hand sides and three update_u
+ * substeps, Time/TimeSteppers/Rk3HesthavenSsp.cpp:55-81) of `system` on
+ * `domain`, captured once into a HIP graph and replayed per step; replaces the
+ * per-substep action loop (ComputeTimeDerivative, RecordTimeStepperData,
+ * UpdateU: Evolution/Actions/..., Time/Actions/UpdateU.hpp:43-101) for a
+ * fixed-step evolution on one device.  u (the state, updated in place),
+ * params (gamma2 or gammas) and inv_jac are device arrays in the domain's
+ * layout and must stay allocated while the stepper lives; the stepper owns
+ * two field-sized work arrays.  The kernel organisation the domain has at
+ * creation time (sxd_domain_set_kernel / sxd_domain_autotune) is the one
+ * captured. */
+typedef struct sxd_rk3_stepper sxd_rk3_stepper;
+int sxd_rk3_stepper_create(sxd_rk3_stepper** stepper, sxd_domain* domain,
+                           int system, double* u, const double* params,
+                           const double* inv_jac);
+/* Enqueue num_steps steps of size dt on `stream` (after waiting for work
+ * already on it, to set the step size). */
+int sxd_rk3_stepper_step(sxd_rk3_stepper* stepper, double dt, int num_steps,
+                         void* stream);
+int sxd_rk3_stepper_destroy(sxd_rk3_stepper* stepper);
+
 #ifdef __cplusplus
 }
 #endif

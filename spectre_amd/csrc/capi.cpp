@@ -673,6 +673,133 @@ int sxd_domain_autotune(sxd_domain* domain, int system, const double* u, const d
   return ok();
 }
 
+// ---- RK3 step captured in a HIP graph ---------------------------------------
+
+struct sxd_rk3_stepper {
+  sxd_domain* domain = nullptr;
+  int system = 0;
+  int64_t n = 0;                 // doubles per field
+  double* u = nullptr;           // caller's state, updated in place
+  double* d_stage = nullptr;     // u1, then u2 (in place)
+  double* d_rhs = nullptr;       // L0, L1, L2
+  double* d_dt = nullptr;        // step size read by the update kernels
+  double* h_dt = nullptr;        // pinned host copy of the step size
+  hipStream_t capture = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+void destroy_stepper(sxd_rk3_stepper* s) {
+  if (s->exec != nullptr) (void)hipGraphExecDestroy(s->exec);
+  if (s->graph != nullptr) (void)hipGraphDestroy(s->graph);
+  if (s->capture != nullptr) (void)hipStreamDestroy(s->capture);
+  if (s->d_stage != nullptr) (void)hipFree(s->d_stage);
+  if (s->d_rhs != nullptr) (void)hipFree(s->d_rhs);
+  if (s->d_dt != nullptr) (void)hipFree(s->d_dt);
+  if (s->h_dt != nullptr) (void)hipHostFree(s->h_dt);
+  delete s;
+}
+
+// The three RHS evaluations and update_u substeps of one
+// Rk3HesthavenSsp step (Time/TimeSteppers/Rk3HesthavenSsp.cpp:55-81):
+//   L0 = RHS(u);  u1 = u + dt L0
+//   L1 = RHS(u1); u2 = (3 u + u1 + dt L1) / 4     (written over u1)
+//   L2 = RHS(u2); u  = (u + 2 u2 + 2 dt L2) / 3   (written over u)
+int enqueue_rk3_step(sxd_rk3_stepper* s, const double* params, const double* inv_jac,
+                     hipStream_t stream) {
+  const int kernel = resolve_kernel(s->domain, s->system);
+  const double* stage_in[3] = {s->u, s->d_stage, s->d_stage};
+  double* stage_out[3] = {s->d_stage, s->d_stage, s->u};
+  for (int k = 0; k < 3; ++k) {
+    int rc = run_system(s->domain, s->system, kernel, stage_in[k], params, inv_jac, s->d_rhs,
+                        stream);
+    if (rc != SXD_OK) return rc;
+    rc = launch_rk3_substep_dt(k, s->n, s->d_dt, s->u, s->d_stage, s->d_rhs, stage_out[k],
+                               stream);
+    if (rc != SXD_OK) return rc;
+  }
+  return SXD_OK;
+}
+}  // namespace
+
+int sxd_rk3_stepper_create(sxd_rk3_stepper** stepper, sxd_domain* domain, int system, double* u,
+                           const double* params, const double* inv_jac) {
+  if (stepper == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null stepper output");
+  *stepper = nullptr;
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (system != SXD_SYSTEM_SCALAR_WAVE && system != SXD_SYSTEM_GH)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "unknown system %d", system);
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(params);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  int64_t points = 0;
+  sxd_domain_num_points(domain, &points);
+  sxd_rk3_stepper* s = new (std::nothrow) sxd_rk3_stepper;
+  if (s == nullptr) return fail(SXD_ERR_OUT_OF_MEMORY, "host allocation failed");
+  s->domain = domain;
+  s->system = system;
+  s->n = points * (system == SXD_SYSTEM_GH ? 50 : 5);
+  s->u = u;
+  const size_t bytes = static_cast<size_t>(s->n) * sizeof(double);
+  if (hipMalloc(&s->d_stage, bytes) != hipSuccess || hipMalloc(&s->d_rhs, bytes) != hipSuccess ||
+      hipMalloc(&s->d_dt, sizeof(double)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&s->h_dt), sizeof(double)) != hipSuccess) {
+    destroy_stepper(s);
+    return fail(SXD_ERR_OUT_OF_MEMORY, "stepper buffers");
+  }
+  if (hipStreamCreateWithFlags(&s->capture, hipStreamNonBlocking) != hipSuccess) {
+    destroy_stepper(s);
+    return fail(SXD_ERR_DEVICE, "stream creation failed");
+  }
+  // One RHS outside the capture allocates any work arrays and sets kernel
+  // attributes, so the captured region only enqueues kernels.
+  int rc = run_system(domain, system, resolve_kernel(domain, system), u, params, inv_jac,
+                      s->d_rhs, s->capture);
+  if (rc == SXD_OK && hipStreamSynchronize(s->capture) != hipSuccess) rc = SXD_ERR_DEVICE;
+  if (rc != SXD_OK) {
+    destroy_stepper(s);
+    return finish(rc, __func__);
+  }
+  if (hipStreamBeginCapture(s->capture, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    destroy_stepper(s);
+    return fail(SXD_ERR_DEVICE, "hipStreamBeginCapture failed");
+  }
+  rc = enqueue_rk3_step(s, params, inv_jac, s->capture);
+  const hipError_t end = hipStreamEndCapture(s->capture, &s->graph);
+  if (rc != SXD_OK || end != hipSuccess ||
+      hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0) != hipSuccess) {
+    destroy_stepper(s);
+    return fail(rc != SXD_OK ? rc : SXD_ERR_DEVICE, "graph capture of the RK3 step failed");
+  }
+  *stepper = s;
+  return ok();
+}
+
+int sxd_rk3_stepper_step(sxd_rk3_stepper* stepper, double dt, int num_steps, void* stream) {
+  if (stepper == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null stepper");
+  if (num_steps < 0) return fail(SXD_ERR_INVALID_ARGUMENT, "negative step count");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // the update kernels read the step size from device memory
+  if (hipStreamSynchronize(st) != hipSuccess) return fail(SXD_ERR_DEVICE, "stream sync failed");
+  *stepper->h_dt = dt;
+  if (hipMemcpyAsync(stepper->d_dt, stepper->h_dt, sizeof(double), hipMemcpyHostToDevice, st) !=
+      hipSuccess)
+    return fail(SXD_ERR_DEVICE, "step size upload failed");
+  for (int k = 0; k < num_steps; ++k)
+    if (hipGraphLaunch(stepper->exec, st) != hipSuccess)
+      return fail(SXD_ERR_DEVICE, "hipGraphLaunch failed");
+  return ok();
+}
+
+int sxd_rk3_stepper_destroy(sxd_rk3_stepper* stepper) {
+  if (stepper != nullptr) {
+    (void)hipDeviceSynchronize();
+    destroy_stepper(stepper);
+  }
+  return ok();
+}
+
 int sxd_rk3_substep(int substep, int64_t n, double dt, const double* u0, const double* uk,
                     const double* lk, double* out, void* stream) {
   if (substep < 0 || substep > 2)

@@ -690,6 +690,32 @@ rk3_substep_kernel(int substep, int64_t n, double dt, const double* __restrict__
   }
 }
 
+// The same update with the step size read from device memory at run time
+// (for graphs that are replayed with changing step sizes).
+__global__ void __launch_bounds__(256)
+rk3_substep_dt_kernel(int substep, int64_t n, const double* __restrict__ dt_ptr,
+                      const double* __restrict__ u0, const double* __restrict__ uk,
+                      const double* __restrict__ lk, double* __restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dt = *dt_ptr;
+  if (substep == 0) {
+    out[i] = u0[i] + dt * lk[i];
+  } else if (substep == 1) {
+    out[i] = 0.25 * (3.0 * u0[i] + uk[i] + dt * lk[i]);
+  } else {
+    out[i] = (1.0 / 3.0) * (u0[i] + 2.0 * uk[i] + 2.0 * dt * lk[i]);
+  }
+}
+
+int launch_rk3_substep_dt(int substep, int64_t n, const double* dt, const double* u0,
+                          const double* uk, const double* lk, double* out, hipStream_t stream) {
+  if (n == 0) return SXD_OK;
+  hipLaunchKernelGGL(rk3_substep_dt_kernel, dim3(static_cast<unsigned>((n + 255) / 256)),
+                     dim3(256), 0, stream, substep, n, dt, u0, uk, lk, out);
+  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
+}
+
 int launch_rk3_substep(int substep, int64_t n, double dt, const double* u0, const double* uk,
                        const double* lk, double* out, hipStream_t stream) {
   if (n == 0) return SXD_OK;

@@ -61,6 +61,8 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
                      const int32_t* nbr, int np_max, int kf, int km, const double* u,
                      const double* params, const double* inv_jac, double* dt, double* scratch_du,
                      int threads, hipStream_t stream);
+int launch_rk3_substep_dt(int substep, int64_t n, const double* dt, const double* u0,
+                          const double* uk, const double* lk, double* out, hipStream_t stream);
 int launch_rk3_substep(int substep, int64_t n, double dt, const double* u0, const double* uk,
                        const double* lk, double* out, hipStream_t stream);
 

@@ -28,6 +28,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
+#include <vector>
 
 #include "device_common.hpp"
 #include "gh_point.hpp"
@@ -424,6 +426,28 @@ mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restrict
   }
 }
 
+// Dynamic LDS beyond the default 64 KiB has to be enabled per kernel; done
+// once per device (before any graph capture of these launches).
+bool enable_large_lds() {
+  static std::mutex mutex;
+  static std::vector<bool> done;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return false;
+  std::lock_guard<std::mutex> lock(mutex);
+  if (static_cast<int>(done.size()) <= dev) done.resize(dev + 1, false);
+  if (done[dev]) return true;
+  const int lds_max = 160 * 1024;
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&sw_mixed_volume_kernel),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_face_kernel<false>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_face_kernel<true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess)
+    return false;
+  done[dev] = true;
+  return true;
+}
+
 }  // namespace
 
 int mixed_face_lds_bytes(int kf, int km) { return FacePlan(kf, km, 13).total * 8; }
@@ -439,15 +463,7 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
   if (threads != 64 && threads != 128 && threads != 256) return SXD_ERR_INVALID_ARGUMENT;
   const int face_lds = mixed_face_lds_bytes(kf, km);
   if (face_lds > 160 * 1024) return SXD_ERR_UNSUPPORTED;
-  // dynamic LDS beyond the default 64 KiB has to be enabled per kernel
-  const int lds_max = 160 * 1024;
-  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&sw_mixed_volume_kernel),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
-      hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_face_kernel<false>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
-      hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_face_kernel<true>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess)
-    return SXD_ERR_DEVICE;
+  if (!enable_large_lds()) return SXD_ERR_DEVICE;
   if (!gh) {
     if (5 * np_max * 8 > 160 * 1024) return SXD_ERR_UNSUPPORTED;
     hipLaunchKernelGGL(sw_mixed_volume_kernel, dim3(count), dim3(threads), 5 * np_max * 8, stream,

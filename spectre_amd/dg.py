@@ -295,3 +295,34 @@ class MixedDomain(Domain):
         t = ctypes.c_int()
         check(lib().sxd_domain_get_threads(self._handle, SYSTEMS[system], ctypes.byref(t)))
         return t.value
+
+
+class Rk3Stepper:
+    """Rk3HesthavenSsp steps of a system on a domain, captured once into a HIP
+    graph (sxd_rk3_stepper_*).  `u` is updated in place; the tensors passed
+    here must outlive the stepper."""
+
+    def __init__(self, domain, system, u, params, inv_jac):
+        nc = 5 if system == "scalar_wave" else 50
+        _require(u, "u", domain._shapes(nc))
+        _require(inv_jac, "inv_jac", domain._shapes(9))
+        _require(params, "params", domain._scalar_shape() if nc == 5 else domain._shapes(3))
+        self._keep = (domain, u, params, inv_jac)
+        handle = ctypes.c_void_p()
+        check(lib().sxd_rk3_stepper_create(ctypes.byref(handle), domain._handle, SYSTEMS[system],
+                                           _ptr(u), _ptr(params), _ptr(inv_jac)))
+        self._handle = handle
+
+    def step(self, dt, num_steps=1):
+        check(lib().sxd_rk3_stepper_step(self._handle, float(dt), int(num_steps), _stream()))
+
+    def close(self):
+        if getattr(self, "_handle", None) is not None and self._handle.value:
+            lib().sxd_rk3_stepper_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

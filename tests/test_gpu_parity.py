@@ -262,3 +262,53 @@ def test_mixed_p_full_size_batched_equals_per_extents_and_oracle():
     assert np.array_equal(got, ref)
     domain.set_kernel("scalar_wave", "mixed_per_extents")
     assert np.array_equal(host(domain.scalar_wave_rhs(d_u, d_g2, d_inv)), ref)
+
+
+# ---- RK3 steps captured in a HIP graph ----------------------------------------
+def oracle_rk3(rhs, u, dt, steps):
+    for _ in range(steps):
+        l0 = rhs(u)
+        u1 = ol.rk3_substep(0, dt, u, u, l0)
+        l1 = rhs(u1)
+        u2 = ol.rk3_substep(1, dt, u, u1, l1)
+        l2 = rhs(u2)
+        u = ol.rk3_substep(2, dt, u, u2, l2)
+    return u
+
+
+def test_rk3_graph_scalar_wave_uniform_and_mixed():
+    brick, u, g2, inv = wl.scalar_wave_plane_wave_3d(seed=0)
+    ref = oracle_rk3(lambda v: ol.sw_rhs(brick.extents, brick.neighbors, v, g2, inv), u, 0.01, 3)
+    domain = dg.Domain.from_brick(brick)
+    d_u, d_g2, d_inv = dev(u), dev(g2), dev(inv)
+    stepper = dg.Rk3Stepper(domain, "scalar_wave", d_u, d_g2, d_inv)
+    stepper.step(0.01, 2)
+    # eager work between replays
+    scratch = domain.scalar_wave_rhs(d_u, d_g2, d_inv)
+    torch.cuda.synchronize()
+    stepper.step(0.01, 1)
+    assert np.array_equal(host(d_u), ref)
+    stepper.close()
+
+    mb, mu, mg2, minv = wl.scalar_wave_plane_wave_mixed(seed=3, elements=(2, 2, 2))
+    ref = oracle_rk3(lambda v: ol.sw_rhs_mixed(mb.extents, mb.neighbors, v, mg2, minv), mu,
+                     0.005, 2)
+    md = dg.MixedDomain.from_brick(mb)
+    d_mu = dev(mu)
+    stepper = dg.Rk3Stepper(md, "scalar_wave", d_mu, dev(mg2), dev(minv))
+    stepper.step(0.005, 1)
+    stepper.step(0.005, 1)
+    assert np.array_equal(host(d_mu), ref)
+    stepper.close()
+
+
+def test_rk3_graph_gh():
+    brick, u, gammas, inv = wl.gh_kerr_schild_brick(seed=5, elements=(2, 2, 2), extents=(6, 6, 6))
+    ref = oracle_rk3(lambda v: ol.gh_rhs(brick.extents, brick.neighbors, v, gammas, inv), u,
+                     0.002, 2)
+    domain = dg.Domain.from_brick(brick)
+    d_u = dev(u)
+    stepper = dg.Rk3Stepper(domain, "gh", d_u, dev(gammas), dev(inv))
+    stepper.step(0.002, 2)
+    assert np.array_equal(host(d_u), ref)
+    stepper.close()
