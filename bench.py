@@ -140,6 +140,48 @@ def measure_mixed(system, steps, warmup, rk3_steps=5):
     }
 
 
+def measure_adjoint(steps=40, reps=3):
+    """Adjoint ScalarWave workload (512 elements x 6^3, 40 RK3 steps at CFL
+    0.25, J = sum M psi^2): time of the forward evolution against the
+    backward (hand-written adjoint kernels), with and without recomputing
+    each 8-step segment from its checkpoint."""
+    import numpy as np
+    import torch
+    from spectre_amd import dg, evolution as ev, workloads as wl
+
+    brick, u, g2, inv, dt = wl.scalar_wave_adjoint_workload()
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    domain = dg.Domain.from_brick(brick)
+    d_g2, d_inv = dev(g2), dev(inv)
+    mass = ev.mass_weights(brick.extents, d_inv, brick.num_elements)
+
+    def run(recompute):
+        u0 = dev(u).requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t2 = torch.cuda.Event(enable_timing=True)
+        t0.record()
+        uf = ev.ScalarWaveEvolution.apply(u0, domain, d_g2, d_inv, dt, steps, 8, recompute)
+        t1.record()
+        J = ev.psi_energy(uf, mass)
+        J.backward()
+        t2.record()
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1), t1.elapsed_time(t2)
+
+    run(False)
+    fwd, bwd = zip(*[run(False) for _ in range(reps)])
+    _, bwd_rc = zip(*[run(True) for _ in range(reps)])
+    f = min(fwd)
+    return {"workload": "scalar_wave_adjoint_512el_6^3_40_rk3_steps",
+            "grid_points": brick.num_elements * brick.num_points, "steps": steps,
+            "dt": dt, "forward_ms": round(f, 4), "backward_ms": round(min(bwd), 4),
+            "backward_with_recompute_ms": round(min(bwd_rc), 4),
+            "adjoint_forward_cost": round(min(bwd) / f, 3),
+            "adjoint_forward_cost_with_recompute": round(min(bwd_rc) / f, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,7 +316,8 @@ def main():
 
     extensions = None
     if args.extensions:
-        extensions = {"mixed_p": [measure_mixed(s, 20, 3) for s in ("scalar_wave", "gh")]}
+        extensions = {"mixed_p": [measure_mixed(s, 20, 3) for s in ("scalar_wave", "gh")],
+                      "adjoint": measure_adjoint()}
 
     result = {
         "metric": METRIC,

@@ -221,6 +221,27 @@ int sxd_rk3_substep(int substep, int64_t n, double dt, const double* u0,
                     const double* uk, const double* lk, double* out,
                     void* stream);
 
+/* ---- Adjoints (no counterpart in the reference) ------------------------- */
+
+/* Transpose of the ScalarWave DG RHS of sxd_scalar_wave_rhs, which is linear
+ * in u for fixed gamma2 and inverse Jacobian: ubar = A^T gbar, i.e. the
+ * gradient of a functional J with respect to u given gbar = dJ/d(dt).
+ * Uniform domains with isotropic extents 2..10; gbar, ubar [E][5][Np] (must
+ * not alias). */
+int sxd_scalar_wave_rhs_adjoint(const sxd_domain* domain, const double* gbar,
+                                const double* gamma2, const double* inv_jac,
+                                double* ubar, void* stream);
+/* Transpose of one Rk3HesthavenSsp substep (sxd_rk3_substep), applied in
+ * reverse substep order 2, 1, 0 on n doubles.  lam is the adjoint of the
+ * substep's output, a = A^T lam (from the adjoint RHS), mu accumulates the
+ * adjoint of the step's input state:
+ *   substep 2: lam_out = 2/3 (lam + dt a),  mu  = lam / 3
+ *   substep 1: lam_out = (lam + dt a) / 4,  mu += 3/4 lam
+ *   substep 0:                              mu += lam + dt a   (lam_out unused) */
+int sxd_rk3_adjoint_substep(int substep, int64_t n, double dt, const double* lam,
+                            const double* a, double* mu, double* lam_out,
+                            void* stream);
+
 /* A full Rk3HesthavenSsp step (three right-hand sides and three update_u
  * substeps, Time/TimeSteppers/Rk3HesthavenSsp.cpp:55-81) of `system` on
  * `domain`, captured once into a HIP graph and replayed per step; replaces the

@@ -874,6 +874,116 @@ void gh_rhs(int num_elements, const int ext[3], const int* nbr, const T* u,
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint (transpose) of the ScalarWave DG RHS.  For fixed gamma2 and inverse
+// Jacobian the RHS u -> dt of sw_rhs is linear; this is its exact transpose
+// gbar (= dJ/d dt) -> ubar (= dJ/du), written by hand from the forward
+// operations (the reference has no adjoint):
+//  * volume (TimeDerivative.cpp:36-44): dt_psi = -Pi, dt_Pi = -d_i Phi_i,
+//    dt_Phi_i = -d_i Pi + gamma2 (d_i psi - Phi_i); derivatives
+//    d_i u = sum_j invJ(j, i) D_j u, whose transpose applies D_j^T along each
+//    pencil to sum_i invJ(j, i) du_bar_i;
+//  * faces: with the constant speeds (0, 1, -1) the UpwindPenalty correction
+//    (UpwindPenalty.cpp:35-205) reduces to
+//      bc_Pi = -(v+_ext + v-_int) / 2,  bc_Phi_d = (v-_int n_int,d - v+_ext n_ext,d) / 2,
+//      bc_psi = 0,  v-_int = -(Pi - n.Phi - gamma2 psi)_int,
+//      v+_ext = (Pi + n.Phi - gamma2 psi)_ext,
+//    lifted with lift_factor; its transpose sends s_in = (-a + b.n_int) / 2
+//    to the interior point and s_ext = -(a + b.n_ext) / 2 to the neighbour's
+//    point, where (a, b) = lift factor x (gbar_Pi, gbar_Phi).
+// Contributions are accumulated element by element, faces in direction order.
+template <class T>
+void sw_rhs_adjoint(int num_elements, const int ext[3], const int* nbr, const T* gbar,
+                    const T* gamma2, const T* inv_j, T* ubar) {
+  const auto& cache = SpectralCache::get();
+  const int n[3] = {ext[0], ext[1], ext[2]};
+  const size_t np = static_cast<size_t>(n[0]) * n[1] * n[2];
+  for (size_t x = 0; x < 5 * np * num_elements; ++x) ubar[x] = T(0.0);
+  std::vector<T> xbar(15 * np);  // logical-derivative adjoints [c][j][p]
+  for (int e = 0; e < num_elements; ++e) {
+    const T* g = gbar + e * 5 * np;
+    const T* ij = inv_j + e * 9 * np;
+    T* ub = ubar + e * 5 * np;
+    for (size_t p = 0; p < np; ++p) {
+      const T g2 = gamma2[e * np + p];
+      T dub[15];
+      for (int i = 0; i < 3; ++i) {
+        dub[i] = g2 * g[(2 + i) * np + p];   // d_i psi
+        dub[3 + i] = -g[(2 + i) * np + p];   // d_i Pi
+        for (int jj = 0; jj < 3; ++jj) dub[6 + 3 * jj + i] = i == jj ? -g[np + p] : T(0.0);
+      }
+      for (int c = 0; c < 5; ++c)
+        for (int j = 0; j < 3; ++j) {
+          T s = ij[ij3(j, 0) * np + p] * dub[3 * c + 0];
+          s = fma_(ij[ij3(j, 1) * np + p], dub[3 * c + 1], s);
+          s = fma_(ij[ij3(j, 2) * np + p], dub[3 * c + 2], s);
+          xbar[(3 * c + j) * np + p] = s;
+        }
+      ub[np + p] = ub[np + p] - g[p];                                   // dt_psi = -Pi
+      for (int i = 0; i < 3; ++i) ub[(2 + i) * np + p] = ub[(2 + i) * np + p] - g2 * g[(2 + i) * np + p];
+    }
+    // transposed sweeps: ubar_c(p) += sum_j sum_m D_j(m, idx_j(p)) xbar_cj(pencil m)
+    for (int c = 0; c < 5; ++c)
+      for (int k = 0; k < n[2]; ++k)
+        for (int jy = 0; jy < n[1]; ++jy)
+          for (int i = 0; i < n[0]; ++i) {
+            const size_t pt = i + n[0] * (jy + static_cast<size_t>(n[1]) * k);
+            const int idx[3] = {i, jy, k};
+            T acc = ub[c * np + pt];
+            for (int j = 0; j < 3; ++j) {
+              const double* D = cache.D[n[j]].data();
+              const int N = n[j];
+              for (int m = 0; m < N; ++m) {
+                int q[3] = {i, jy, k};
+                q[j] = m;
+                const size_t qt = q[0] + n[0] * (q[1] + static_cast<size_t>(n[1]) * q[2]);
+                acc = fma_(T(D[m * N + idx[j]]), xbar[(3 * c + j) * np + qt], acc);
+              }
+            }
+            ub[c * np + pt] = acc;
+          }
+  }
+  for (int e = 0; e < num_elements; ++e) {
+    for (int dir = 0; dir < 6; ++dir) {
+      const int d = dir / 2, side = dir % 2;
+      const int en = nbr[6 * e + dir];
+      const Face fi = make_face(ext, d, side);
+      const Face fe = make_face(ext, d, 1 - side);
+      for (size_t f = 0; f < fi.pts.size(); ++f) {
+        const size_t p = fi.pts[f], q = fe.pts[f];
+        T ni[3], ne[3], mag, magn;
+        for (int i = 0; i < 3; ++i) {
+          ni[i] = inv_j[(e * 9 + ij3(d, i)) * np + p];
+          if (side == 0) ni[i] = ni[i] * -1.0;
+          ne[i] = inv_j[(en * 9 + ij3(d, i)) * np + q];
+          if (side == 1) ne[i] = ne[i] * -1.0;
+        }
+        flat_normalize(ni, &mag);
+        flat_normalize(ne, &magn);
+        const T fac = lift_factor(ext[d], mag);
+        const T a = fac * gbar[(e * 5 + 1) * np + p];
+        T b[3];
+        for (int i = 0; i < 3; ++i) b[i] = fac * gbar[(e * 5 + 2 + i) * np + p];
+        T bn_i = b[0] * ni[0], bn_e = b[0] * ne[0];
+        for (int i = 1; i < 3; ++i) {
+          bn_i = fma_(b[i], ni[i], bn_i);
+          bn_e = fma_(b[i], ne[i], bn_e);
+        }
+        const T s_in = 0.5 * (bn_i - a);
+        const T s_ex = -0.5 * (a + bn_e);
+        T* ui = ubar + e * 5 * np;
+        T* ux = ubar + en * 5 * np;
+        ui[p] = ui[p] + gamma2[e * np + p] * s_in;
+        ui[np + p] = ui[np + p] - s_in;
+        for (int i = 0; i < 3; ++i) ui[(2 + i) * np + p] = ui[(2 + i) * np + p] + ni[i] * s_in;
+        ux[q] = ux[q] - gamma2[en * np + q] * s_ex;
+        ux[np + q] = ux[np + q] + s_ex;
+        for (int i = 0; i < 3; ++i) ux[(2 + i) * np + q] = ux[(2 + i) * np + q] + ne[i] * s_ex;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Mixed p: elements of different per-dimension LGL extents, coupled through
 // mortars (conforming faces, aligned orientation, ChildSize::Full).
 //

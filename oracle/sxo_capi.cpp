@@ -343,6 +343,13 @@ int sxo_gh_rhs(int num_elements, const int* ext, const int* nbr, const double* u
   return 0;
 }
 
+int sxo_sw_rhs_adjoint(int num_elements, const int* ext, const int* nbr, const double* gbar,
+                       const double* gamma2, const double* inv_j, double* ubar) {
+  if (bad_extents(ext) || num_elements < 1) return -1;
+  sxo::sw_rhs_adjoint(num_elements, ext, nbr, gbar, gamma2, inv_j, ubar);
+  return 0;
+}
+
 // ---- mixed p ---------------------------------------------------------------
 
 int sxo_mortar_interpolation_matrix(int n_face, int n_mortar, double* out) {

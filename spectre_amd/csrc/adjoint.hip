@@ -1,0 +1,247 @@
+// Adjoint (transpose) of the ScalarWave DG right-hand side and of the RK3
+// update, for gradients of functionals of the evolved state.
+//
+// For fixed gamma2 and inverse Jacobian the ScalarWave RHS u -> dt is linear;
+// these kernels apply its exact transpose gbar -> ubar, written by hand from
+// the forward operations (the reference has no adjoint):
+//  * volume: the transposed point-wise terms give the adjoints of the 15
+//    inertial derivatives, the transposed inverse-Jacobian contraction those
+//    of the logical derivatives (staged in LDS), and the transposed
+//    differentiation sweeps ubar_c(p) += sum_j sum_m D_j(m, idx_j(p)) x_cj(m);
+//  * faces (transposed UpwindPenalty + lift): with the constant speeds
+//    (0, 1, -1) the correction reduces to bc_Pi = -(v+_ext + v-_int) / 2,
+//    bc_Phi = (v-_int n_int - v+_ext n_ext) / 2; its transpose sends
+//    s_in = (b.n_int - a) / 2 to the interior point and
+//    s_ext = -(a + b.n_ext) / 2 to the exterior point, with (a, b) the lift
+//    factor times (gbar_Pi, gbar_Phi) at the interior point.
+// Each point gathers the face contributions that land on it (its own faces'
+// interior terms and the neighbours' exterior terms) and adds them in the
+// order (contributing element, direction), the order in which the oracle
+// scatters them, so results equal the oracle bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+#include "sw_point.hpp"
+
+namespace sxd {
+namespace {
+
+template <int N>
+__global__ void __launch_bounds__((N * N * N + 63) / 64 * 64)
+sw_rhs_adjoint_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ gbar,
+                      const double* __restrict__ gamma2, const double* __restrict__ inv_jac,
+                      double* __restrict__ ubar, const double* __restrict__ dtab) {
+  constexpr int NP = N * N * N;
+  __shared__ double s_x[15 * NP];  // logical-derivative adjoints [c][j][p]
+  const int64_t e = xcd_local_element(blockIdx.x, gridDim.x);
+  const int t = threadIdx.x;
+  const double* g = gbar + e * 5 * NP;
+  const double* ij = inv_jac + e * 9 * NP;
+  double gp[5], g2 = 0.0;
+  if (t < NP) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) gp[c] = g[c * NP + t];
+    g2 = gamma2[e * NP + t];
+    double dub[15];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      dub[i] = g2 * gp[2 + i];
+      dub[3 + i] = -gp[2 + i];
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) dub[6 + 3 * jj + i] = i == jj ? -gp[1] : 0.0;
+    }
+    double ijp[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) ijp[c] = ij[c * NP + t];
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double s = ijp[ij3(j, 0)] * dub[3 * c + 0];
+        s = fma(ijp[ij3(j, 1)], dub[3 * c + 1], s);
+        s = fma(ijp[ij3(j, 2)], dub[3 * c + 2], s);
+        s_x[(3 * c + j) * NP + t] = s;
+      }
+  }
+  __syncthreads();
+  if (t >= NP) return;
+  const int idx[3] = {t % N, (t / N) % N, t / (N * N)};
+  const double* D = diff_matrix(dtab, N);
+  double ub[5];
+  ub[0] = 0.0;
+  ub[1] = 0.0 - gp[0];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) ub[2 + i] = 0.0 - g2 * gp[2 + i];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    double acc = ub[c];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int stride = j == 0 ? 1 : j == 1 ? N : N * N;
+      const int base = t - idx[j] * stride;
+#pragma unroll
+      for (int m = 0; m < N; ++m)
+        acc = fma(D[m * N + idx[j]], s_x[(3 * c + j) * NP + base + m * stride], acc);
+    }
+    ub[c] = acc;
+  }
+  // face contributions: (element, direction, exterior?) sorted by element
+  // then direction
+  int64_t key[6];
+  int dim_of[6];
+  bool ext_of[6];
+  int nk = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (idx[d] != 0 && idx[d] != N - 1) continue;
+    const int s = idx[d] == 0 ? 0 : 1;
+    key[nk] = e * 8 + 2 * d + s;
+    dim_of[nk] = d;
+    ext_of[nk] = false;
+    ++nk;
+    key[nk] = static_cast<int64_t>(nbr[6 * e + 2 * d + s]) * 8 + 2 * d + (1 - s);
+    dim_of[nk] = d;
+    ext_of[nk] = true;
+    ++nk;
+  }
+  for (int a = 1; a < nk; ++a)
+    for (int b = a; b > 0 && key[b] < key[b - 1]; --b) {
+      const int64_t tk = key[b];
+      key[b] = key[b - 1];
+      key[b - 1] = tk;
+      const int td = dim_of[b];
+      dim_of[b] = dim_of[b - 1];
+      dim_of[b - 1] = td;
+      const bool tx = ext_of[b];
+      ext_of[b] = ext_of[b - 1];
+      ext_of[b - 1] = tx;
+    }
+  for (int x = 0; x < nk; ++x) {
+    const int d = dim_of[x];
+    const int s = idx[d] == 0 ? 0 : 1;
+    if (!ext_of[x]) {
+      // interior term of this element's face (d, s)
+      double n[3], mag;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) n[i] = s == 0 ? ij[ij3(d, i) * NP + t] * -1.0 : ij[ij3(d, i) * NP + t];
+      flat_normalize(n, &mag);
+      const double fac = lift_factor(N, mag);
+      const double a = fac * gp[1];
+      double bn = (fac * gp[2]) * n[0];
+      bn = fma(fac * gp[3], n[1], bn);
+      bn = fma(fac * gp[4], n[2], bn);
+      const double s_in = 0.5 * (bn - a);
+      ub[0] = ub[0] + g2 * s_in;
+      ub[1] = ub[1] - s_in;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) ub[2 + i] = ub[2 + i] + n[i] * s_in;
+    } else {
+      // exterior term of the neighbour's face (d, 1 - s): its interior point
+      // is the matching point q of the neighbour
+      const int64_t en = nbr[6 * e + 2 * d + s];
+      int qi[3] = {idx[0], idx[1], idx[2]};
+      qi[d] = s == 0 ? N - 1 : 0;
+      const int q = qi[0] + N * (qi[1] + N * qi[2]);
+      double nq[3], magq, ne[3], mage;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const double vq = inv_jac[(en * 9 + ij3(d, i)) * NP + q];
+        nq[i] = (1 - s) == 0 ? vq * -1.0 : vq;      // neighbour's own normal
+        const double vp = ij[ij3(d, i) * NP + t];
+        ne[i] = (1 - s) == 1 ? vp * -1.0 : vp;      // its exterior normal
+      }
+      flat_normalize(nq, &magq);
+      flat_normalize(ne, &mage);
+      const double fac = lift_factor(N, magq);
+      const double* gq = gbar + en * 5 * NP + q;
+      const double a = fac * gq[NP];
+      double bn = (fac * gq[2 * NP]) * ne[0];
+      bn = fma(fac * gq[3 * NP], ne[1], bn);
+      bn = fma(fac * gq[4 * NP], ne[2], bn);
+      const double s_ex = -0.5 * (a + bn);
+      ub[0] = ub[0] - g2 * s_ex;
+      ub[1] = ub[1] + s_ex;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) ub[2 + i] = ub[2 + i] + ne[i] * s_ex;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 5; ++c) ubar[(e * 5 + c) * NP + t] = ub[c];
+}
+
+template <int N>
+int launch_sw_adjoint_n(int num_elements, const int32_t* nbr, const double* gbar,
+                        const double* gamma2, const double* inv_jac, double* ubar,
+                        const double* dtab, hipStream_t stream) {
+  constexpr int threads = (N * N * N + 63) / 64 * 64;
+  hipLaunchKernelGGL(sw_rhs_adjoint_kernel<N>, dim3(num_elements), dim3(threads), 0, stream,
+                     nbr, gbar, gamma2, inv_jac, ubar, dtab);
+  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
+}
+
+// Transpose of one Rk3HesthavenSsp substep (Rk3HesthavenSsp.cpp:55-81),
+// substeps in reverse order.  lam: adjoint of the substep's output; a:
+// A^T lam (adjoint RHS); mu: accumulated adjoint of the step's input u.
+//   substep 2 (u3 = (u + 2 u2 + 2 dt L(u2)) / 3): lam_out = 2/3 (lam + dt a), mu = lam / 3
+//   substep 1 (u2 = (3 u + u1 + dt L(u1)) / 4):   lam_out = (lam + dt a) / 4, mu += 3/4 lam
+//   substep 0 (u1 = u + dt L(u)):                 mu += lam + dt a
+__global__ void __launch_bounds__(256)
+rk3_adjoint_substep_kernel(int substep, int64_t n, double dt, const double* lam,
+                           const double* __restrict__ a, double* __restrict__ mu,
+                           double* lam_out) {  // lam_out may be lam (element-wise in place)
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double l = lam[i];
+  if (substep == 2) {
+    lam_out[i] = (2.0 / 3.0) * (l + dt * a[i]);
+    mu[i] = (1.0 / 3.0) * l;
+  } else if (substep == 1) {
+    lam_out[i] = 0.25 * (l + dt * a[i]);
+    mu[i] = mu[i] + 0.75 * l;
+  } else {
+    mu[i] = mu[i] + (l + dt * a[i]);
+  }
+}
+
+}  // namespace
+
+bool sw_adjoint_supported(const int32_t ext[3]) {
+  return ext[0] == ext[1] && ext[1] == ext[2] && ext[0] >= 2 && ext[0] <= 10;
+}
+
+int launch_sw_rhs_adjoint(int num_elements, const int32_t ext[3], const int32_t* nbr,
+                          const double* gbar, const double* gamma2, const double* inv_jac,
+                          double* ubar, hipStream_t stream) {
+  const double* dtab = device_diff_matrices();
+  if (dtab == nullptr) return SXD_ERR_DEVICE;
+  if (!sw_adjoint_supported(ext)) return SXD_ERR_UNSUPPORTED;
+  switch (ext[0]) {
+#define SXD_ADJ_CASE(n) \
+  case n:               \
+    return launch_sw_adjoint_n<n>(num_elements, nbr, gbar, gamma2, inv_jac, ubar, dtab, stream);
+    SXD_ADJ_CASE(2)
+    SXD_ADJ_CASE(3)
+    SXD_ADJ_CASE(4)
+    SXD_ADJ_CASE(5)
+    SXD_ADJ_CASE(6)
+    SXD_ADJ_CASE(7)
+    SXD_ADJ_CASE(8)
+    SXD_ADJ_CASE(9)
+    SXD_ADJ_CASE(10)
+#undef SXD_ADJ_CASE
+    default: return SXD_ERR_UNSUPPORTED;
+  }
+}
+
+int launch_rk3_adjoint_substep(int substep, int64_t n, double dt, const double* lam,
+                               const double* a, double* mu, double* lam_out, hipStream_t stream) {
+  if (n == 0) return SXD_OK;
+  hipLaunchKernelGGL(rk3_adjoint_substep_kernel, dim3(static_cast<unsigned>((n + 255) / 256)),
+                     dim3(256), 0, stream, substep, n, dt, lam, a, mu, lam_out);
+  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
+}
+
+}  // namespace sxd

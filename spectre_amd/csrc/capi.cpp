@@ -673,6 +673,37 @@ int sxd_domain_autotune(sxd_domain* domain, int system, const double* u, const d
   return ok();
 }
 
+// ---- adjoints ------------------------------------------------------------------
+
+int sxd_scalar_wave_rhs_adjoint(const sxd_domain* domain, const double* gbar, const double* gamma2,
+                                const double* inv_jac, double* ubar, void* stream) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  SXD_REQUIRE_DEVICE(gbar);
+  SXD_REQUIRE_DEVICE(gamma2);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  SXD_REQUIRE_DEVICE(ubar);
+  if (domain->mixed || !sw_adjoint_supported(domain->ext))
+    return fail(SXD_ERR_UNSUPPORTED, "the ScalarWave adjoint needs a uniform domain with "
+                                     "isotropic extents 2..10");
+  if (gbar == ubar) return fail(SXD_ERR_INVALID_ARGUMENT, "gbar and ubar must not alias");
+  return finish(launch_sw_rhs_adjoint(domain->num_elements, domain->ext, domain->d_nbr, gbar,
+                                      gamma2, inv_jac, ubar, static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+int sxd_rk3_adjoint_substep(int substep, int64_t n, double dt, const double* lam,
+                            const double* a, double* mu, double* lam_out, void* stream) {
+  if (substep < 0 || substep > 2) return fail(SXD_ERR_INVALID_ARGUMENT, "substep must be 0..2");
+  if (n < 0) return fail(SXD_ERR_INVALID_ARGUMENT, "negative size");
+  SXD_REQUIRE_DEVICE(lam);
+  SXD_REQUIRE_DEVICE(a);
+  SXD_REQUIRE_DEVICE(mu);
+  if (substep > 0) SXD_REQUIRE_DEVICE(lam_out);
+  return finish(launch_rk3_adjoint_substep(substep, n, dt, lam, a, mu, lam_out,
+                                           static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
 // ---- RK3 step captured in a HIP graph ---------------------------------------
 
 struct sxd_rk3_stepper {

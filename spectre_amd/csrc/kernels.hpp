@@ -61,6 +61,14 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
                      const int32_t* nbr, int np_max, int kf, int km, const double* u,
                      const double* params, const double* inv_jac, double* dt, double* scratch_du,
                      int threads, hipStream_t stream);
+// Adjoint (transpose) of the ScalarWave RHS on a uniform domain (isotropic
+// extents 2..10) and of one RK3 substep.
+bool sw_adjoint_supported(const int32_t ext[3]);
+int launch_sw_rhs_adjoint(int num_elements, const int32_t ext[3], const int32_t* nbr,
+                          const double* gbar, const double* gamma2, const double* inv_jac,
+                          double* ubar, hipStream_t stream);
+int launch_rk3_adjoint_substep(int substep, int64_t n, double dt, const double* lam,
+                               const double* a, double* mu, double* lam_out, hipStream_t stream);
 int launch_rk3_substep_dt(int substep, int64_t n, const double* dt, const double* u0,
                           const double* uk, const double* lk, double* out, hipStream_t stream);
 int launch_rk3_substep(int substep, int64_t n, double dt, const double* u0, const double* uk,

@@ -223,6 +223,18 @@ class Domain:
                                _stream()))
         return out
 
+    def scalar_wave_rhs_adjoint(self, gbar, gamma2, inv_jac, out=None):
+        """Transpose of scalar_wave_rhs (linear in u): out = A^T gbar."""
+        _require(gbar, "gbar", self._shapes(5))
+        _require(gamma2, "gamma2", self._scalar_shape())
+        _require(inv_jac, "inv_jac", self._shapes(9))
+        if out is None:
+            out = torch.empty_like(gbar)
+        _require(out, "out", self._shapes(5))
+        check(lib().sxd_scalar_wave_rhs_adjoint(self._handle, _ptr(gbar), _ptr(gamma2),
+                                                _ptr(inv_jac), _ptr(out), _stream()))
+        return out
+
     def set_kernel(self, system, kernel):
         """Pin the kernel organisation ("auto", "composite", "fused",
         "fused_mfma") used for `system` ("scalar_wave" or "gh")."""

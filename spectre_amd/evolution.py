@@ -1,0 +1,113 @@
+"""Time evolution of a DG system with Rk3HesthavenSsp and its gradient.
+
+`ScalarWaveEvolution` is a torch.autograd.Function: forward runs `steps`
+steps of the ScalarWave DG RHS (the hand-written HIP kernels behind the C ABI)
+and returns the final state; backward applies the hand-written adjoint
+kernels (transposed RHS and transposed RK3 substeps) step by step in reverse.
+The forward keeps a checkpoint of the state every `checkpoint_every` steps;
+the backward walks the segments in reverse and, if `recompute` is set,
+recomputes each segment's step inputs from its checkpoint before reversing it
+(the schedule a system with a state-dependent linearisation needs; the
+ScalarWave RHS is linear, so its step adjoint does not read those states and
+`recompute=False` skips that work).
+
+Reference: Time/TimeSteppers/Rk3HesthavenSsp.cpp:55-81 (the forward step);
+the reference has no adjoint.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import dg
+from ._lib import check, lib
+
+
+def rk3_adjoint_substep(substep, dt, lam, a, mu, lam_out=None):
+    """Transpose of one RK3 substep (sxd_rk3_adjoint_substep)."""
+    check(lib().sxd_rk3_adjoint_substep(int(substep), lam.numel(), float(dt), dg._ptr(lam),
+                                        dg._ptr(a), dg._ptr(mu), dg._ptr(lam_out), dg._stream()))
+    return mu, lam_out
+
+
+def rk3_step(domain, rhs, u, params, inv_jac, dt, stage, l):
+    """One Rk3HesthavenSsp step of u in place (stage, l: work arrays)."""
+    rhs(u, params, inv_jac, out=l)
+    dg.rk3_substep(0, dt, u, u, l, out=stage)
+    rhs(stage, params, inv_jac, out=l)
+    dg.rk3_substep(1, dt, u, stage, l, out=stage)
+    rhs(stage, params, inv_jac, out=l)
+    dg.rk3_substep(2, dt, u, stage, l, out=u)
+
+
+def rk3_step_adjoint(adjoint_rhs, lam3, params, inv_jac, dt, a, lam, mu):
+    """Adjoint of one RK3 step of a linear RHS: mu = S^T lam3 (a, lam: work)."""
+    adjoint_rhs(lam3, params, inv_jac, out=a)
+    rk3_adjoint_substep(2, dt, lam3, a, mu, lam)        # lam = lam2, mu = lam3 / 3
+    adjoint_rhs(lam, params, inv_jac, out=a)
+    rk3_adjoint_substep(1, dt, lam, a, mu, lam)         # lam = lam1, mu += 3/4 lam2
+    adjoint_rhs(lam, params, inv_jac, out=a)
+    rk3_adjoint_substep(0, dt, lam, a, mu, None)        # mu += lam1 + dt A^T lam1
+    return mu
+
+
+class ScalarWaveEvolution(torch.autograd.Function):
+    """u_final = S^steps u0 for the ScalarWave DG RHS on `domain`."""
+
+    @staticmethod
+    def forward(ctx, u0, domain, gamma2, inv_jac, dt, steps, checkpoint_every=8,
+                recompute=False):
+        u = u0.detach().clone()
+        stage, l = torch.empty_like(u), torch.empty_like(u)
+        checkpoints = []
+        for n in range(steps):
+            if n % checkpoint_every == 0:
+                checkpoints.append(u.clone())
+            rk3_step(domain, domain.scalar_wave_rhs, u, gamma2, inv_jac, dt, stage, l)
+        ctx.domain, ctx.dt, ctx.steps = domain, dt, steps
+        ctx.checkpoint_every, ctx.recompute = checkpoint_every, recompute
+        ctx.checkpoints = checkpoints
+        ctx.save_for_backward(gamma2, inv_jac)
+        return u
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        gamma2, inv_jac = ctx.saved_tensors
+        domain, dt, k = ctx.domain, ctx.dt, ctx.checkpoint_every
+        lam3 = grad_out.detach().contiguous().clone()
+        a, lam, mu = (torch.empty_like(lam3) for _ in range(3))
+        stage, l = torch.empty_like(lam3), torch.empty_like(lam3)
+        for seg in reversed(range(len(ctx.checkpoints))):
+            first = seg * k
+            last = min(ctx.steps, first + k)
+            if ctx.recompute:
+                # inputs of the segment's steps, from its checkpoint
+                u = ctx.checkpoints[seg].clone()
+                states = []
+                for _ in range(first, last):
+                    states.append(u.clone())
+                    rk3_step(domain, domain.scalar_wave_rhs, u, gamma2, inv_jac, dt, stage, l)
+            for _ in range(last - first):
+                rk3_step_adjoint(domain.scalar_wave_rhs_adjoint, lam3, gamma2, inv_jac, dt, a,
+                                 lam, mu)
+                lam3, mu = mu, lam3
+        ctx.checkpoints = None
+        return lam3, None, None, None, None, None, None, None
+
+
+def mass_weights(domain_extents, inv_jac, num_elements):
+    """Diagonal LGL mass matrix per point, w_i w_j w_k det(J) [E][Np] (the
+    Jacobian determinant as 1 / det of the inverse Jacobian)."""
+    from . import spectral
+    n0, n1, n2 = domain_extents
+    w = [spectral.lgl_weights(n) for n in (n0, n1, n2)]
+    wk, wj, wi = np.meshgrid(w[2], w[1], w[0], indexing="ij")
+    wq = torch.from_numpy((wi * wj * wk).ravel()).to(inv_jac.device)
+    m = inv_jac.reshape(num_elements, 3, 3, -1)   # [E][i][j][p], block j + 3 i
+    det_inv = torch.linalg.det(m.permute(0, 3, 1, 2))  # [E][p]
+    return wq[None, :] / det_inv
+
+
+def psi_energy(u, mass):
+    """J = sum over points of M psi^2 (psi = component 0)."""
+    return torch.sum(mass * u[:, 0, :] ** 2)

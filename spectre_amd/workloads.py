@@ -444,3 +444,27 @@ def gh_kerr_schild_mixed(seed=0, elements=(10, 10, 10), lowest=4, highest=12, no
     gammas = brick.ragged([np.stack([np.ones(n), -np.ones(n), np.ones(n)])
                            for n in brick.points])
     return brick, u, gammas, brick.inverse_jacobian()
+
+
+def rk3_stable_step():
+    """Rk3HesthavenSsp::stable_step (Time/TimeSteppers/Rk3HesthavenSsp.cpp:23-26)."""
+    c = (4.0 + np.sqrt(17.0)) ** (1.0 / 3.0)
+    return 0.5 * (1.0 + c - 1.0 / c)
+
+
+def cfl_step(brick, safety, speed):
+    """Step size of the Cfl step chooser (Time/StepChoosers/Cfl.hpp:69-75):
+    safety * stable_step * minimum_grid_spacing / (speed * 3), with the
+    minimum grid spacing of an affine axis-aligned element the smallest LGL
+    node spacing times h_d / 2."""
+    spacing = min(0.5 * brick.widths[d] * float(np.min(np.diff(spectral.lgl_points(
+        brick.extents[d])))) for d in range(3))
+    return safety * rk3_stable_step() * spacing / (speed * 3.0)
+
+
+def scalar_wave_adjoint_workload(seed=0, elements=(8, 8, 8), extents=(6, 6, 6)):
+    """Adjoint ScalarWave workload: the plane wave of scalar_wave_plane_wave_3d
+    on 8x8x8 elements of 6^3 points, gamma2 = 1, step size at CFL 0.25.
+    Returns (brick, u, gamma2, inv_jac, dt)."""
+    brick, u, gamma2, inv = scalar_wave_plane_wave_3d(seed, elements, extents)
+    return brick, u, gamma2, inv, cfl_step(brick, 0.25, 1.0)

@@ -249,3 +249,15 @@ def gh_rhs_mixed(extents, neighbors, u, gammas, inv_jac):
     _check(lib().sxo_gh_rhs_mixed(len(ext) // 3, ep, nbp, _d(u), _d(gammas), _d(inv_jac),
                                   _d(dt)))
     return dt
+
+
+def sw_rhs_adjoint(extents, neighbors, gbar, gamma2, inv_jac):
+    """Transpose of the (linear) ScalarWave RHS: gbar [E][5][Np] -> ubar."""
+    gbar, gamma2, inv_jac = (np.ascontiguousarray(a, dtype=np.float64)
+                             for a in (gbar, gamma2, inv_jac))
+    ubar = np.empty_like(gbar)
+    ext, ep = _i(extents)
+    nb, nbp = _i(neighbors)
+    _check(lib().sxo_sw_rhs_adjoint(gbar.shape[0], ep, nbp, _d(gbar), _d(gamma2), _d(inv_jac),
+                                    _d(ubar)))
+    return ubar
