@@ -194,6 +194,8 @@ def main():
                     choices=["tune", "auto", "fused", "fused_mfma", "composite"],
                     help="kernel organisation of the RHS; 'tune' runs the library's "
                          "autotuner on the workload first and keeps its choice")
+    ap.add_argument("--threads", type=int, default=0,
+                    help="threads per workgroup with an explicit --kernel (0 = library default)")
     ap.add_argument("--extensions", type=int, default=1,
                     help="also measure the mixed-p batched form (1) or not (0)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -244,6 +246,8 @@ def main():
         domain.autotune(system, d_u, d_aux, d_inv, d_dt, repetitions=10)
     else:
         domain.set_kernel(system, args.kernel)
+        if args.threads:
+            domain.set_threads(system, args.threads)
     kernel_used = domain.kernel(system)
 
     for _ in range(args.warmup):
@@ -335,7 +339,8 @@ def main():
         "config": {"workload": name, "elements": brick.num_elements,
                    "extents": list(brick.extents), "grid_points": points,
                    "components": nc, "parallelism": "single GPU",
-                   "kernel": kernel_used, "kernel_choice": args.kernel},
+                   "kernel": kernel_used, "threads_per_workgroup": domain.threads(system),
+                   "kernel_choice": args.kernel},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "extensions": extensions,

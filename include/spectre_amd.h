@@ -197,7 +197,10 @@ enum { SXD_SYSTEM_SCALAR_WAVE = 0, SXD_SYSTEM_GH = 1 };
 int sxd_domain_set_kernel(sxd_domain* domain, int system, int kernel);
 /* The kernel the next RHS call on this domain will use. */
 int sxd_domain_get_kernel(const sxd_domain* domain, int system, int* kernel);
-/* Threads per workgroup of the mixed-p kernels (64, 128 or 256; default 128). */
+/* Threads per workgroup (tile shape): mixed-p kernels 64, 128 or 256
+ * (default 128); the fused GH kernel at 7^3 and 8^3 points 256 (one wave per
+ * SIMD, two points per thread) or 512 (two waves per SIMD, one point per
+ * thread).  Results do not depend on it. */
 int sxd_domain_set_threads(sxd_domain* domain, int system, int threads);
 int sxd_domain_get_threads(const sxd_domain* domain, int system, int* threads);
 /* Autotuner: time every kernel available for (domain, system) on the given

@@ -419,8 +419,10 @@ int sxd_domain_set_threads(sxd_domain* domain, int system, int threads) {
   if (domain == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null domain");
   if (system != SXD_SYSTEM_SCALAR_WAVE && system != SXD_SYSTEM_GH)
     return fail(SXD_ERR_INVALID_ARGUMENT, "unknown system %d", system);
-  if (threads != 64 && threads != 128 && threads != 256)
-    return fail(SXD_ERR_INVALID_ARGUMENT, "threads per workgroup must be 64, 128 or 256");
+  if (threads != 64 && threads != 128 && threads != 256 && threads != 512)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "threads per workgroup must be 64, 128, 256 or 512");
+  if (threads == 512 && domain->mixed)
+    return fail(SXD_ERR_UNSUPPORTED, "mixed-p kernels run 64, 128 or 256 threads");
   domain->threads[system] = threads;
   return ok();
 }
@@ -547,7 +549,7 @@ int run_gh(sxd_domain* d, int kernel, const double* u, const double* gammas,
   if (kernel == SXD_KERNEL_FUSED || kernel == SXD_KERNEL_FUSED_MFMA)
     return launch_gh_rhs_fused(d->num_elements, d->ext, d->d_nbr, u, gammas, inv_jac, dt,
                                kernel == SXD_KERNEL_FUSED ? kGhFusedValu : kGhFusedMfma,
-                               stream);
+                               d->threads[SXD_SYSTEM_GH] == 512 ? 512 : 256, stream);
   const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
   if (ensure_scratch(d, 150 * np * d->num_elements, 12 * np * d->num_elements) != SXD_OK)
     return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
@@ -638,12 +640,16 @@ int sxd_domain_autotune(sxd_domain* domain, int system, const double* u, const d
   const int ncand = candidate_kernels(domain, system, cand);
   const int saved_threads = domain->threads[system];
   int best_threads = saved_threads;
-  const int tile_options[3] = {64, 128, 256};
-  const int ntiles = domain->mixed ? 3 : 1;
+  // tile shapes: mixed p 64/128/256 threads; uniform GH fused 256/512
+  const int mixed_tiles[3] = {64, 128, 256};
+  const int fused_tiles[2] = {256, 512};
+  const bool gh_fused_tiles = !domain->mixed && system == SXD_SYSTEM_GH && domain->ext[0] >= 7;
+  const int ntiles = domain->mixed ? 3 : gh_fused_tiles ? 2 : 1;
   for (int cj = 0; cj < ncand * ntiles; ++cj) {
     const int ci = cj / ntiles;
     const int k = cand[ci];
-    if (domain->mixed) domain->threads[system] = tile_options[cj % ntiles];
+    if (domain->mixed) domain->threads[system] = mixed_tiles[cj % ntiles];
+    if (gh_fused_tiles) domain->threads[system] = fused_tiles[cj % ntiles];
     rc = run_system(domain, system, k, u, params, inv_jac, dt, s);  // warm-up
     if (rc != SXD_OK) break;
     if (hipEventRecord(ev[0], s) != hipSuccess) { rc = SXD_ERR_DEVICE; break; }

@@ -2,12 +2,15 @@
 // every term of ComputeTimeDerivative and ApplyBoundaryCorrections in a single
 // pass over HBM.
 //
-// Register budget.  The point-wise GH algebra keeps ~300 FP64 values live per
-// grid point.  A CDNA4 SIMD has 512 VGPRs per lane, so the kernel runs one
-// wave per SIMD (256 threads per element) and each thread owns P = ceil(Np /
-// 256) grid points (P = 2 at 8^3): with the vector-ALU derivative engine
-// nothing spills to scratch memory, and the two points give each thread two
-// independent instruction streams.
+// Register budget.  The point-wise GH algebra keeps ~235 FP64 registers live
+// per grid point.  A CDNA4 SIMD has 512 VGPRs per lane.  Two tile shapes:
+//  - 256 threads per element: one wave per SIMD, each thread owns P =
+//    ceil(Np / 256) grid points (P = 2 at 8^3); with the vector-ALU
+//    derivative engine nothing spills, and the two points give each thread
+//    two independent instruction streams;
+//  - 512 threads (N = 7, 8): one point per thread and two waves per SIMD in
+//    256 registers, with part of the phase-A state spilled; the second wave
+//    covers the other's LDS and barrier waits.
 //
 // Phases:
 //  A  point-wise setup for each owned point: load the 50 fields, run
@@ -62,14 +65,14 @@ namespace {
 
 typedef double mfma_acc_t __attribute__((ext_vector_type(4)));
 
-constexpr int kGhThreads = 256;
-
-template <int N, bool MFMA>
+// TPB: threads per workgroup (256: one wave per SIMD, two points per thread
+// at 8^3; 512: two waves per SIMD, one point per thread, 256 registers).
+template <int N, bool MFMA, int TPB>
 struct GhFusedLayout {
   static constexpr int NP = N * N * N;
   static constexpr int NF = N * N;                   // points per face
   static constexpr int SLOTS = 6 * NF;               // face points of the element
-  static constexpr int THREADS = NP < kGhThreads ? (NP + 63) / 64 * 64 : kGhThreads;
+  static constexpr int THREADS = NP < TPB ? (NP + 63) / 64 * 64 : TPB;
   static constexpr int WAVES = THREADS / 64;
   static constexpr int P = (NP + THREADS - 1) / THREADS;          // points per thread
   static constexpr int PF = (SLOTS + THREADS - 1) / THREADS;      // face slots per thread
@@ -113,11 +116,11 @@ __device__ __forceinline__ int pencil_point(int d, int l, int n) {
 // first + ncomp - 1) in all three directions, written to
 // deriv[(c * 3 + d) * NP + point] (c = 0 .. ncomp - 1).  The jobs
 // (direction, component group, pencil tile) are spread over the waves.
-template <int N, int NCOMP>
+template <int N, int NCOMP, int TPB>
 __device__ __forceinline__ void mfma_logical_derivatives(const double* stage, double* deriv,
                                                          int first, const double* afrag,
                                                          int wave, int lane) {
-  using L = GhFusedLayout<N, true>;
+  using L = GhFusedLayout<N, true, TPB>;
   constexpr int NP = L::NP;
   constexpr int GROUPS = (NCOMP + L::CPT - 1) / L::CPT;
   constexpr int JOBS = 3 * GROUPS * L::TILES;
@@ -172,12 +175,12 @@ __device__ __forceinline__ void mfma_logical_derivatives(const double* stage, do
   }
 }
 
-template <int N, bool MFMA>
-__global__ void __launch_bounds__((GhFusedLayout<N, MFMA>::THREADS), 1)
+template <int N, bool MFMA, int TPB>
+__global__ void __launch_bounds__((GhFusedLayout<N, MFMA, TPB>::THREADS), 1)
 gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ u,
                     const double* __restrict__ gammas, const double* __restrict__ inv_jac,
                     double* __restrict__ dt, const double* __restrict__ dtab) {
-  using L = GhFusedLayout<N, MFMA>;
+  using L = GhFusedLayout<N, MFMA, TPB>;
   constexpr int NP = L::NP, NF = L::NF, P = L::P, PF = L::PF, T = L::THREADS;
   __shared__ double smem[L::kDoubles];
   const int64_t e = xcd_local_element(blockIdx.x, gridDim.x);
@@ -427,7 +430,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
       lds_barrier();  // stage complete
       const int comp[5] = {p, 10 + p, 20 + 3 * p, 21 + 3 * p, 22 + 3 * p};
       double out[P][5];
-      mfma_logical_derivatives<N, 2>(stage, deriv, 0, afrag, wave, lane);  // g, Pi
+      mfma_logical_derivatives<N, 2, TPB>(stage, deriv, 0, afrag, wave, lane);  // g, Pi
       face_corrections(&ext_v[0][0]);
       lds_barrier();  // g, Pi derivatives complete
       double dg[P][3], dpi[P][3];
@@ -438,7 +441,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
           contract_lds(q, 1, dpi[q]);
         }
       lds_barrier();  // deriv free again
-      mfma_logical_derivatives<N, 3>(stage, deriv, 2, afrag, wave, lane);  // Phi_n
+      mfma_logical_derivatives<N, 3, TPB>(stage, deriv, 2, afrag, wave, lane);  // Phi_n
       lds_barrier();  // Phi derivatives and face corrections complete
 #pragma unroll
       for (int q = 0; q < P; ++q) {
@@ -586,23 +589,40 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
   }
 }
 
-template <int N, bool MFMA>
+template <int N, bool MFMA, int TPB>
 int launch_gh_fused_n(int num_elements, const int32_t* nbr, const double* u,
                       const double* gammas, const double* inv_jac, double* dt,
                       const double* dtab, hipStream_t stream) {
-  hipLaunchKernelGGL((gh_rhs_fused_kernel<N, MFMA>), dim3(num_elements),
-                     dim3(GhFusedLayout<N, MFMA>::THREADS), 0, stream, nbr, u, gammas, inv_jac,
-                     dt, dtab);
+  hipLaunchKernelGGL((gh_rhs_fused_kernel<N, MFMA, TPB>), dim3(num_elements),
+                     dim3(GhFusedLayout<N, MFMA, TPB>::THREADS), 0, stream, nbr, u, gammas,
+                     inv_jac, dt, dtab);
   return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
 }
 
-template <int N>
-int launch_gh_fused_variant(int variant, int num_elements, const int32_t* nbr, const double* u,
-                            const double* gammas, const double* inv_jac, double* dt,
-                            const double* dtab, hipStream_t stream) {
+template <int N, int TPB>
+int launch_gh_fused_engine(int variant, int num_elements, const int32_t* nbr, const double* u,
+                           const double* gammas, const double* inv_jac, double* dt,
+                           const double* dtab, hipStream_t stream) {
   if (variant == kGhFusedMfma)
-    return launch_gh_fused_n<N, true>(num_elements, nbr, u, gammas, inv_jac, dt, dtab, stream);
-  return launch_gh_fused_n<N, false>(num_elements, nbr, u, gammas, inv_jac, dt, dtab, stream);
+    return launch_gh_fused_n<N, true, TPB>(num_elements, nbr, u, gammas, inv_jac, dt, dtab,
+                                           stream);
+  return launch_gh_fused_n<N, false, TPB>(num_elements, nbr, u, gammas, inv_jac, dt, dtab,
+                                          stream);
+}
+
+// Below 7^3 points every tile has at most 256 threads; 512 only differs from
+// 256 at N = 7, 8.
+template <int N>
+int launch_gh_fused_variant(int variant, int threads, int num_elements, const int32_t* nbr,
+                            const double* u, const double* gammas, const double* inv_jac,
+                            double* dt, const double* dtab, hipStream_t stream) {
+  if constexpr (N >= 7) {
+    if (threads == 512)
+      return launch_gh_fused_engine<N, 512>(variant, num_elements, nbr, u, gammas, inv_jac, dt,
+                                            dtab, stream);
+  }
+  return launch_gh_fused_engine<N, 256>(variant, num_elements, nbr, u, gammas, inv_jac, dt, dtab,
+                                        stream);
 }
 
 }  // namespace
@@ -613,7 +633,7 @@ bool gh_fused_supported(const int32_t ext[3]) {
 
 int launch_gh_rhs_fused(int num_elements, const int32_t ext[3], const int32_t* nbr,
                         const double* u, const double* gammas, const double* inv_jac,
-                        double* dt, int variant, hipStream_t stream) {
+                        double* dt, int variant, int threads, hipStream_t stream) {
   const double* dtab = device_diff_matrices();
   if (dtab == nullptr) return SXD_ERR_DEVICE;
   if (!gh_fused_supported(ext)) return SXD_ERR_UNSUPPORTED;
@@ -621,7 +641,8 @@ int launch_gh_rhs_fused(int num_elements, const int32_t ext[3], const int32_t* n
   switch (ext[0]) {
 #define SXD_GH_FUSED_CASE(n) \
   case n:                    \
-    return launch_gh_fused_variant<n>(variant, num_elements, nbr, u, gammas, inv_jac, dt, dtab, stream);
+    return launch_gh_fused_variant<n>(variant, threads, num_elements, nbr, u, gammas, inv_jac, dt, \
+                                      dtab, stream);
     SXD_GH_FUSED_CASE(2)
     SXD_GH_FUSED_CASE(3)
     SXD_GH_FUSED_CASE(4)

@@ -41,9 +41,10 @@ int launch_gh_rhs_composite(int num_elements, const int32_t ext[3], const int32_
 constexpr int kGhFusedValu = 0;  // LDS pencils on the vector ALUs
 constexpr int kGhFusedMfma = 1;  // v_mfma_f64_16x16x4f64 tiles
 bool gh_fused_supported(const int32_t ext[3]);
+// threads: 256 or 512 per workgroup (512 applies to N = 7, 8).
 int launch_gh_rhs_fused(int num_elements, const int32_t ext[3], const int32_t* nbr,
                         const double* u, const double* gammas, const double* inv_jac,
-                        double* dt, int variant, hipStream_t stream);
+                        double* dt, int variant, int threads, hipStream_t stream);
 // Mixed p.  Per-element record of the ragged layout.
 struct MixedElement {
   int32_t n[3];  // LGL extents

@@ -235,6 +235,15 @@ class Domain:
                                                 _ptr(inv_jac), _ptr(out), _stream()))
         return out
 
+    def set_threads(self, system, threads):
+        """Threads per workgroup (tile shape) of the kernels for `system`."""
+        check(lib().sxd_domain_set_threads(self._handle, SYSTEMS[system], threads))
+
+    def threads(self, system):
+        t = ctypes.c_int()
+        check(lib().sxd_domain_get_threads(self._handle, SYSTEMS[system], ctypes.byref(t)))
+        return t.value
+
     def set_kernel(self, system, kernel):
         """Pin the kernel organisation ("auto", "composite", "fused",
         "fused_mfma") used for `system` ("scalar_wave" or "gh")."""
@@ -300,13 +309,6 @@ class MixedDomain(Domain):
     def _scalar_shape(self):
         return (self.total_points,)
 
-    def set_threads(self, system, threads):
-        check(lib().sxd_domain_set_threads(self._handle, SYSTEMS[system], threads))
-
-    def threads(self, system):
-        t = ctypes.c_int()
-        check(lib().sxd_domain_get_threads(self._handle, SYSTEMS[system], ctypes.byref(t)))
-        return t.value
 
 
 class Rk3Stepper:

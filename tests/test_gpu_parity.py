@@ -145,15 +145,18 @@ def test_gh_rhs_kerr_schild_512_elements():
     assert err <= TOL
 
 
+@pytest.mark.parametrize("threads", [256, 512])
 @pytest.mark.parametrize("kernel", ["composite", "fused", "fused_mfma"])
 @pytest.mark.parametrize("elements,extents", [((2, 2, 2), (8, 8, 8)), ((2, 1, 2), (5, 5, 5)),
                                               ((1, 2, 2), (7, 7, 7))])
-def test_gh_rhs_every_kernel_organisation(kernel, elements, extents):
-    """Each kernel organisation of the GH RHS gives the oracle's result bit for bit."""
+def test_gh_rhs_every_kernel_organisation(kernel, elements, extents, threads):
+    """Each kernel organisation and tile shape of the GH RHS gives the oracle's
+    result bit for bit."""
     brick, u, gammas, inv = wl.gh_kerr_schild_brick(seed=2, elements=elements, extents=extents)
     ref = ol.gh_rhs(brick.extents, brick.neighbors, u, gammas, inv)
     domain = dg.Domain.from_brick(brick)
     domain.set_kernel("gh", kernel)
+    domain.set_threads("gh", threads)
     assert domain.kernel("gh") == kernel
     got = host(domain.gh_rhs(dev(u), dev(gammas), dev(inv)))
     assert rel_err(got, ref) <= TOL
