@@ -524,44 +524,77 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
 #pragma unroll
         for (int c = 0; c < 5; ++c) corr[slot * 5 + c] = bc[c] * fac;
       }
-      // volume terms of the owned points
-      double out[P][5];
+      // volume terms of the owned points.  The logical derivatives of all
+      // owned points come first: when a thread's two points share their
+      // zeta pencil (T a multiple of N^2) it is read from LDS once, and xi
+      // pencils (contiguous) are read two doubles at a time for even N.
+      constexpr bool kShareZeta = P == 2 && T % (N * N) == 0;
+      double dlog[P][15];  // dlog[q][3 c + m] = d_m of staged component c
 #pragma unroll
-      for (int q = 0; q < P; ++q) {
-        if (!act[q]) continue;
-        const int pt = t + q * T;
-        const int i = pi_[q], j = pj[q], k = pk[q];
-        double dlog[15];
+      for (int c = 0; c < 5; ++c) {
+        const double* sc = stage + c * NP;
+        double zeta[N];
+        if constexpr (kShareZeta) {
 #pragma unroll
-        for (int c = 0; c < 5; ++c) {
-          const double* sc = stage + c * NP;
-          double d0 = Di[q][0] * sc[N * (j + N * k)];
+          for (int l = 0; l < N; ++l) zeta[l] = sc[pi_[0] + N * (pj[0] + N * l)];
+        }
 #pragma unroll
-          for (int l = 1; l < N; ++l) d0 = fma(Di[q][l], sc[l + N * (j + N * k)], d0);
+        for (int q = 0; q < P; ++q) {
+          if (!act[q]) continue;
+          const int i = pi_[q], j = pj[q], k = pk[q];
+          const double* row = sc + N * (j + N * k);
+          double xr[N];
+          if constexpr (N % 2 == 0) {
+#pragma unroll
+            for (int l = 0; l < N; l += 2) {
+              const double2 v = *reinterpret_cast<const double2*>(row + l);
+              xr[l] = v.x;
+              xr[l + 1] = v.y;
+            }
+          } else {
+#pragma unroll
+            for (int l = 0; l < N; ++l) xr[l] = row[l];
+          }
+          double d0 = Di[q][0] * xr[0];
+#pragma unroll
+          for (int l = 1; l < N; ++l) d0 = fma(Di[q][l], xr[l], d0);
           double d1 = Dj[q][0] * sc[i + N * N * k];
 #pragma unroll
           for (int l = 1; l < N; ++l) d1 = fma(Dj[q][l], sc[i + N * (l + N * k)], d1);
-          double d2 = Dk[q][0] * sc[i + N * j];
+          double d2;
+          if constexpr (kShareZeta) {
+            d2 = Dk[q][0] * zeta[0];
 #pragma unroll
-          for (int l = 1; l < N; ++l) d2 = fma(Dk[q][l], sc[i + N * (j + N * l)], d2);
+            for (int l = 1; l < N; ++l) d2 = fma(Dk[q][l], zeta[l], d2);
+          } else {
+            d2 = Dk[q][0] * sc[i + N * j];
+#pragma unroll
+            for (int l = 1; l < N; ++l) d2 = fma(Dk[q][l], sc[i + N * (j + N * l)], d2);
+          }
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
             double s = ij[q][ij3(0, a)] * d0;
             s = fma(ij[q][ij3(1, a)], d1, s);
             s = fma(ij[q][ij3(2, a)], d2, s);
-            dlog[3 * c + a] = s;
+            dlog[q][3 * c + a] = s;
           }
         }
-        // dlog[3 c + m] = d_m of component c; c = 0 g, 1 Pi, 2..4 Phi_n
+      }
+      double out[P][5];
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        if (!act[q]) continue;
+        const int pt = t + q * T;
+        // c = 0 g, 1 Pi, 2..4 Phi_n
         double phi[3], dphi[9];
 #pragma unroll
         for (int n = 0; n < 3; ++n) {
           phi[n] = stage[(2 + n) * NP + pt];
 #pragma unroll
-          for (int m = 0; m < 3; ++m) dphi[m + 3 * n] = dlog[3 * (2 + n) + m];
+          for (int m = 0; m < 3; ++m) dphi[m + 3 * n] = dlog[q][3 * (2 + n) + m];
         }
-        gh_point_pair(out[q], st[q], st[q].dt_pi_partial[0], stage[NP + pt], phi, dlog,
-                      dlog + 3, dphi);
+        gh_point_pair(out[q], st[q], st[q].dt_pi_partial[0], stage[NP + pt], phi, dlog[q],
+                      dlog[q] + 3, dphi);
         // rotate so that pass p + 1 finds its partial sum at index 0
 #pragma unroll
         for (int r = 0; r < 9; ++r) st[q].dt_pi_partial[r] = st[q].dt_pi_partial[r + 1];
