@@ -83,12 +83,13 @@ struct GhFusedLayout {
   static constexpr int TILES = (NF + 15) / 16;       // 16-pencil column tiles
   // LDS (doubles)
   static constexpr int kIntStride = 12;  // n[3] nv[3] lam[4] gamma2 lift
-  static constexpr int kExtStride = 11;  // n[3] nv[3] lam[4] gamma2
+  static constexpr int kExtStride = 12;  // n[3] nv[3] lam[4] gamma2, pad (16-byte rows)
   static constexpr int kFaceInt = 0;
   static constexpr int kFaceExt = kFaceInt + SLOTS * kIntStride;
   static constexpr int kStage = kFaceExt + SLOTS * kExtStride;
-  static constexpr int kCorr = kStage + 5 * NP;
-  static constexpr int kDeriv = kCorr + SLOTS * 5;   // MFMA: logical derivatives, 3 comps x 3 dirs
+  static constexpr int kCorrStride = 6;              // 5 corrections + pad (16-byte rows)
+  static constexpr int kCorr = kStage + (5 * NP + 1) / 2 * 2;  // 16-byte aligned
+  static constexpr int kDeriv = kCorr + SLOTS * kCorrStride;  // MFMA: logical derivatives
   static constexpr int kDoubles = kDeriv + (MFMA ? 9 * NP : 0);
 };
 
@@ -182,7 +183,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
                     double* __restrict__ dt, const double* __restrict__ dtab) {
   using L = GhFusedLayout<N, MFMA, TPB>;
   constexpr int NP = L::NP, NF = L::NF, P = L::P, PF = L::PF, T = L::THREADS;
-  __shared__ double smem[L::kDoubles];
+  __shared__ __align__(16) double smem[L::kDoubles];
   const int64_t e = xcd_local_element(blockIdx.x, gridDim.x);
   const int t = threadIdx.x;
   const int wave = t / 64, lane = t % 64;
@@ -391,7 +392,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
       gh_package_pair(pke, ext_v[5 * f], ext_v[5 * f + 1], phi_e, se);
       gh_boundary_terms_pair(bc, pki, si.lam, pke, se.lam);
 #pragma unroll
-      for (int c = 0; c < 5; ++c) corr[slot * 5 + c] = bc[c] * fac;
+      for (int c = 0; c < 5; ++c) corr[slot * L::kCorrStride + c] = bc[c] * fac;
     }
   };
   // add the corrections of the faces point q lies on (xi, eta, zeta order)
@@ -403,7 +404,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
       const int side = idx[d] == 0 ? 0 : 1;
       const int a = d == 0 ? idx[1] : idx[0];
       const int b = d == 2 ? idx[1] : idx[2];
-      const double* cr = corr + ((2 * d + side) * NF + a + N * b) * 5;
+      const double* cr = corr + ((2 * d + side) * NF + a + N * b) * L::kCorrStride;
 #pragma unroll
       for (int c = 0; c < 5; ++c) out[c] = out[c] + cr[c];
     }
@@ -494,8 +495,22 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
         const int dir = slot / NF;
         const int fp = slot % NF;
         const int pv = face_volume_index<N>(dir, fp % N, fp / N);
-        const double* fi = smem + L::kFaceInt + slot * L::kIntStride;
-        const double* fe = smem + L::kFaceExt + slot * L::kExtStride;
+        // both rows are 12 doubles on 16-byte boundaries: read in pairs
+        double fi[12], fe[12];
+        {
+          const double2* ri =
+              reinterpret_cast<const double2*>(smem + L::kFaceInt + slot * L::kIntStride);
+          const double2* re =
+              reinterpret_cast<const double2*>(smem + L::kFaceExt + slot * L::kExtStride);
+#pragma unroll
+          for (int x = 0; x < 6; ++x) {
+            const double2 a = ri[x], b = re[x];
+            fi[2 * x] = a.x;
+            fi[2 * x + 1] = a.y;
+            fe[2 * x] = b.x;
+            fe[2 * x + 1] = b.y;
+          }
+        }
         GhFaceSide si, se;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -522,7 +537,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
         gh_package_pair(pke, ext_v[f][0], ext_v[f][1], phi_e, se);
         gh_boundary_terms_pair(bc, pki, si.lam, pke, se.lam);
 #pragma unroll
-        for (int c = 0; c < 5; ++c) corr[slot * 5 + c] = bc[c] * fac;
+        for (int c = 0; c < 5; ++c) corr[slot * L::kCorrStride + c] = bc[c] * fac;
       }
       // volume terms of the owned points.  The logical derivatives of all
       // owned points come first: when a thread's two points share their
@@ -611,9 +626,14 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
           const int side = idx[d] == 0 ? 0 : 1;
           const int a = d == 0 ? idx[1] : idx[0];
           const int b = d == 2 ? idx[1] : idx[2];
-          const double* cr = corr + ((2 * d + side) * NF + a + N * b) * 5;
-#pragma unroll
-          for (int c = 0; c < 5; ++c) out[q][c] = out[q][c] + cr[c];
+          const double2* cr = reinterpret_cast<const double2*>(
+              corr + ((2 * d + side) * NF + a + N * b) * L::kCorrStride);
+          const double2 c01 = cr[0], c23 = cr[1], c4x = cr[2];
+          out[q][0] = out[q][0] + c01.x;
+          out[q][1] = out[q][1] + c01.y;
+          out[q][2] = out[q][2] + c23.x;
+          out[q][3] = out[q][3] + c23.y;
+          out[q][4] = out[q][4] + c4x.x;
         }
 #pragma unroll
         for (int c = 0; c < 5; ++c) dte[comp[c] * NP + t + q * T] = out[q][c];
