@@ -261,3 +261,49 @@ def sw_rhs_adjoint(extents, neighbors, gbar, gamma2, inv_jac):
     _check(lib().sxo_sw_rhs_adjoint(gbar.shape[0], ep, nbp, _d(gbar), _d(gamma2), _d(inv_jac),
                                     _d(ubar)))
     return ubar
+
+
+def _ad_call(fn, num_elements, extents, neighbors, *arrays):
+    arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in arrays]
+    out = np.empty_like(arrays[0])
+    ext, ep = _i(np.asarray(extents).ravel())
+    nb, nbp = _i(np.asarray(neighbors).ravel())
+    _check(fn(num_elements, ep, nbp, *[_d(a) for a in arrays], _d(out)))
+    return out
+
+
+def gh_rhs_vjp(extents, neighbors, u, gammas, inv_jac, gbar):
+    """(d dt / d u)^T gbar of the uniform GH RHS, by the oracle's tape."""
+    return _ad_call(lib().sxo_gh_rhs_vjp, u.shape[0], extents, neighbors, u, gammas, inv_jac,
+                    gbar)
+
+
+def gh_rhs_jvp(extents, neighbors, u, gammas, inv_jac, v):
+    return _ad_call(lib().sxo_gh_rhs_jvp, u.shape[0], extents, neighbors, u, gammas, inv_jac, v)
+
+
+def gh_rhs_mixed_vjp(extents, neighbors, u, gammas, inv_jac, gbar):
+    ext = np.asarray(extents).reshape(-1, 3)
+    return _ad_call(lib().sxo_gh_rhs_mixed_vjp, ext.shape[0], ext, neighbors, u, gammas,
+                    inv_jac, gbar)
+
+
+def gh_rhs_mixed_jvp(extents, neighbors, u, gammas, inv_jac, v):
+    ext = np.asarray(extents).reshape(-1, 3)
+    return _ad_call(lib().sxo_gh_rhs_mixed_jvp, ext.shape[0], ext, neighbors, u, gammas,
+                    inv_jac, v)
+
+
+def sw_rhs_vjp(extents, neighbors, u, gamma2, inv_jac, gbar):
+    return _ad_call(lib().sxo_sw_rhs_vjp, u.shape[0], extents, neighbors, u, gamma2, inv_jac,
+                    gbar)
+
+
+def gh_time_derivative_vjp(u, du, gammas, gbar):
+    """Point-wise (ubar [50][n], dubar [150][n]) by the oracle's tape."""
+    u, du, gammas, gbar = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, du, gammas, gbar))
+    n = u.shape[1]
+    ubar, dubar = np.empty((50, n)), np.empty((150, n))
+    _check(lib().sxo_gh_time_derivative_vjp(n, _d(u), _d(du), _d(gammas), _d(gbar), _d(ubar),
+                                            _d(dubar)))
+    return ubar, dubar
