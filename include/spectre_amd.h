@@ -234,6 +234,14 @@ int sxd_rk3_substep(int substep, int64_t n, double dt, const double* u0,
 int sxd_scalar_wave_rhs_adjoint(const sxd_domain* domain, const double* gbar,
                                 const double* gamma2, const double* inv_jac,
                                 double* ubar, void* stream);
+/* Adjoint of the GH DG RHS of sxd_gh_rhs linearised at the state u:
+ * ubar = (d dt / d u)^T gbar, the gradient with respect to u of a functional
+ * J given gbar = dJ/d(dt).  Uniform domains (Np up to 2730); u, gbar,
+ * ubar [E][50][Np], ubar must not alias u or gbar.  Uses the domain's
+ * derivative work array (150 x E x Np doubles, allocated on first use). */
+int sxd_gh_rhs_adjoint(const sxd_domain* domain, const double* u,
+                       const double* gammas, const double* inv_jac,
+                       const double* gbar, double* ubar, void* stream);
 /* Transpose of one Rk3HesthavenSsp substep (sxd_rk3_substep), applied in
  * reverse substep order 2, 1, 0 on n doubles.  lam is the adjoint of the
  * substep's output, a = A^T lam (from the adjoint RHS), mu accumulates the

@@ -48,6 +48,7 @@ _SIGNATURES = {
     "sxd_domain_get_threads": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "sxd_domain_set_kernel": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "sxd_scalar_wave_rhs_adjoint": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "sxd_gh_rhs_adjoint": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_rk3_adjoint_substep": (ctypes.c_int, [ctypes.c_int, _i64, ctypes.c_double, _vp, _vp, _vp,
                                                _vp, _vp]),
     "sxd_rk3_stepper_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, ctypes.c_int, _vp, _vp,

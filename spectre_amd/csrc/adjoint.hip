@@ -23,6 +23,7 @@
 #include <cstdint>
 
 #include "device_common.hpp"
+#include "gh_adjoint.hpp"
 #include "kernels.hpp"
 #include "sw_point.hpp"
 
@@ -206,6 +207,137 @@ rk3_adjoint_substep_kernel(int substep, int64_t n, double dt, const double* lam,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Generalized harmonic: adjoint of the RHS linearised at the state u (the RHS
+// is nonlinear).  Three launches, one thread per grid point each:
+//  1. point-wise: ubar = (df/du)^T gbar; the adjoint of the inertial
+//     derivatives contracted with the inverse Jacobian into logical-derivative
+//     adjoints xbar[3 c + j], written over du (read first, same point);
+//  2. transposed differentiation sweeps: ubar_c += sum_j D_j^T xbar_cj;
+//  3. faces: every face point gathers the transposed lifted UpwindPenalty
+//     terms of the faces it lies on, as interior point of its own face and as
+//     exterior point of the neighbour's face.
+__global__ void __launch_bounds__(128)
+gh_adjoint_pointwise_kernel(int64_t total, int np, const double* __restrict__ u,
+                            double* __restrict__ du_xbar, const double* __restrict__ gammas,
+                            const double* __restrict__ inv_jac, const double* __restrict__ gbar,
+                            double* __restrict__ ubar) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= total) return;
+  const int64_t e = gid / np;
+  const int p = static_cast<int>(gid % np);
+  double up[50], dup[150], gp[50], ub[50], dub[150];
+  for (int c = 0; c < 50; ++c) {
+    up[c] = u[(e * 50 + c) * np + p];
+    gp[c] = gbar[(e * 50 + c) * np + p];
+  }
+  for (int c = 0; c < 150; ++c) dup[c] = du_xbar[(e * 150 + c) * np + p];
+  const double* ga = gammas + e * 3 * np + p;
+  gh_time_derivative_vjp(up, dup, ga[0], ga[np], ga[2 * np], gp, ub, dub);
+  for (int c = 0; c < 50; ++c) ubar[(e * 50 + c) * np + p] = ub[c];
+  double ij[9];
+  for (int c = 0; c < 9; ++c) ij[c] = inv_jac[(e * 9 + c) * np + p];
+  for (int c = 0; c < 50; ++c)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int i = 0; i < 3; ++i) s += ij[ij3(j, i)] * dub[3 * c + i];
+      du_xbar[(e * 150 + 3 * c + j) * np + p] = s;
+    }
+}
+
+// Grid: (E, 50 / kSweepChunk); dynamic LDS: 3 Np doubles.
+constexpr int kSweepChunk = 5;
+__global__ void __launch_bounds__(256)
+transposed_sweeps_kernel(int n0, int n1, int n2, int nc, const double* __restrict__ xbar,
+                         double* __restrict__ ubar, const double* __restrict__ dtab) {
+  extern __shared__ double s_x[];
+  const int np = n0 * n1 * n2;
+  const int64_t e = blockIdx.x;
+  const int n[3] = {n0, n1, n2};
+  const int stride[3] = {1, n0, n0 * n1};
+  const double* D[3] = {diff_matrix(dtab, n0), diff_matrix(dtab, n1), diff_matrix(dtab, n2)};
+  const int c_end = min(nc, static_cast<int>(blockIdx.y + 1) * kSweepChunk);
+  for (int c = blockIdx.y * kSweepChunk; c < c_end; ++c) {
+    __syncthreads();
+    for (int x = threadIdx.x; x < 3 * np; x += blockDim.x)
+      s_x[x] = xbar[(e * 3 * nc + 3 * c) * np + x];
+    __syncthreads();
+    for (int p = threadIdx.x; p < np; p += blockDim.x) {
+      const int idx[3] = {p % n0, (p / n0) % n1, p / (n0 * n1)};
+      double acc = ubar[(e * nc + c) * np + p];
+      for (int j = 0; j < 3; ++j) {
+        const int base = p - idx[j] * stride[j];
+        for (int m = 0; m < n[j]; ++m)
+          acc = fma(D[j][m * n[j] + idx[j]], s_x[j * np + base + m * stride[j]], acc);
+      }
+      ubar[(e * nc + c) * np + p] = acc;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64)
+gh_adjoint_face_kernel(int64_t total, int n0, int n1, int n2, const int32_t* __restrict__ nbr,
+                       const double* __restrict__ u, const double* __restrict__ gammas,
+                       const double* __restrict__ inv_jac, const double* __restrict__ gbar,
+                       double* __restrict__ ubar) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= total) return;
+  const int np = n0 * n1 * n2;
+  const int n[3] = {n0, n1, n2};
+  const int64_t e = gid / np;
+  const int p = static_cast<int>(gid % np);
+  const int idx[3] = {p % n0, (p / n0) % n1, p / (n0 * n1)};
+  bool on_face = false;
+  for (int d = 0; d < 3; ++d) on_face = on_face || idx[d] == 0 || idx[d] == n[d] - 1;
+  if (!on_face) return;
+  double acc[50], up[50], uq[50], gq[50], gp[50], bi[50], bx[50];
+  for (int c = 0; c < 50; ++c) {
+    acc[c] = 0.0;
+    up[c] = u[(e * 50 + c) * np + p];
+    gp[c] = gbar[(e * 50 + c) * np + p];
+  }
+  const double* gap = gammas + e * 3 * np + p;
+  for (int d = 0; d < 3; ++d) {
+    for (int s = 0; s < 2; ++s) {
+      if (idx[d] != (s == 0 ? 0 : n[d] - 1)) continue;
+      if (n[d] == 1) continue;
+      const int64_t en = nbr[6 * e + 2 * d + s];
+      int qi[3] = {idx[0], idx[1], idx[2]};
+      qi[d] = s == 0 ? n[d] - 1 : 0;
+      const int q = qi[0] + n0 * (qi[1] + n1 * qi[2]);
+      for (int c = 0; c < 50; ++c) {
+        uq[c] = u[(en * 50 + c) * np + q];
+        gq[c] = gbar[(en * 50 + c) * np + q];
+      }
+      const double* gaq = gammas + en * 3 * np + q;
+      double np_row[3], nq_row[3];
+      for (int i = 0; i < 3; ++i) {
+        np_row[i] = inv_jac[(e * 9 + ij3(d, i)) * np + p];
+        nq_row[i] = inv_jac[(en * 9 + ij3(d, i)) * np + q];
+      }
+      // interior term of this element's face (d, s)
+      double ni[3], nx[3];
+      for (int i = 0; i < 3; ++i) {
+        ni[i] = s == 0 ? np_row[i] * -1.0 : np_row[i];
+        nx[i] = s == 1 ? nq_row[i] * -1.0 : nq_row[i];
+      }
+      for (int c = 0; c < 50; ++c) bi[c] = bx[c] = 0.0;
+      gh_face_vjp(up, gap[np], gap[2 * np], ni, uq, gaq[np], gaq[2 * np], nx, n[d], gp, bi, bx);
+      for (int c = 0; c < 50; ++c) acc[c] += bi[c];
+      // exterior term of the neighbour's face (d, 1 - s), whose interior
+      // point is q
+      for (int i = 0; i < 3; ++i) {
+        ni[i] = (1 - s) == 0 ? nq_row[i] * -1.0 : nq_row[i];
+        nx[i] = (1 - s) == 1 ? np_row[i] * -1.0 : np_row[i];
+      }
+      for (int c = 0; c < 50; ++c) bi[c] = bx[c] = 0.0;
+      gh_face_vjp(uq, gaq[np], gaq[2 * np], ni, up, gap[np], gap[2 * np], nx, n[d], gq, bi, bx);
+      for (int c = 0; c < 50; ++c) acc[c] += bx[c];
+    }
+  }
+  for (int c = 0; c < 50; ++c) ubar[(e * 50 + c) * np + p] += acc[c];
+}
+
 }  // namespace
 
 bool sw_adjoint_supported(const int32_t ext[3]) {
@@ -234,6 +366,32 @@ int launch_sw_rhs_adjoint(int num_elements, const int32_t ext[3], const int32_t*
 #undef SXD_ADJ_CASE
     default: return SXD_ERR_UNSUPPORTED;
   }
+}
+
+int launch_gh_rhs_adjoint(int num_elements, const int32_t ext[3], const int32_t* nbr,
+                          const double* u, const double* gammas, const double* inv_jac,
+                          const double* gbar, double* ubar, double* scratch_du,
+                          hipStream_t stream) {
+  const double* dtab = device_diff_matrices();
+  if (dtab == nullptr) return SXD_ERR_DEVICE;
+  const int np = ext[0] * ext[1] * ext[2];
+  if (3 * np * 8 > 64 * 1024) return SXD_ERR_UNSUPPORTED;
+  const int64_t total = static_cast<int64_t>(num_elements) * np;
+  int rc = launch_partial_derivatives(num_elements, ext, 50, u, inv_jac, scratch_du, nullptr,
+                                      stream);
+  if (rc != SXD_OK) return rc;
+  hipLaunchKernelGGL(gh_adjoint_pointwise_kernel, dim3(static_cast<unsigned>((total + 127) / 128)),
+                     dim3(128), 0, stream, total, np, u, scratch_du, gammas, inv_jac, gbar, ubar);
+  if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  hipLaunchKernelGGL(transposed_sweeps_kernel,
+                     dim3(num_elements, (50 + kSweepChunk - 1) / kSweepChunk), dim3(256),
+                     3 * np * sizeof(double), stream, ext[0], ext[1], ext[2], 50, scratch_du,
+                     ubar, dtab);
+  if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  hipLaunchKernelGGL(gh_adjoint_face_kernel, dim3(static_cast<unsigned>((total + 63) / 64)),
+                     dim3(64), 0, stream, total, ext[0], ext[1], ext[2], nbr, u, gammas, inv_jac,
+                     gbar, ubar);
+  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
 }
 
 int launch_rk3_adjoint_substep(int substep, int64_t n, double dt, const double* lam,

@@ -697,6 +697,26 @@ int sxd_scalar_wave_rhs_adjoint(const sxd_domain* domain, const double* gbar, co
                 __func__);
 }
 
+int sxd_gh_rhs_adjoint(const sxd_domain* domain, const double* u, const double* gammas,
+                       const double* inv_jac, const double* gbar, double* ubar, void* stream) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(gammas);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  SXD_REQUIRE_DEVICE(gbar);
+  SXD_REQUIRE_DEVICE(ubar);
+  if (domain->mixed)
+    return fail(SXD_ERR_UNSUPPORTED, "the GH adjoint needs a uniform domain");
+  if (gbar == ubar || u == ubar) return fail(SXD_ERR_INVALID_ARGUMENT, "ubar must not alias");
+  sxd_domain* d = const_cast<sxd_domain*>(domain);
+  const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
+  if (ensure_scratch(d, 150 * np * d->num_elements, 0) != SXD_OK)
+    return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
+  return finish(launch_gh_rhs_adjoint(d->num_elements, d->ext, d->d_nbr, u, gammas, inv_jac, gbar,
+                                      ubar, d->d_scratch_du, static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
 int sxd_rk3_adjoint_substep(int substep, int64_t n, double dt, const double* lam,
                             const double* a, double* mu, double* lam_out, void* stream) {
   if (substep < 0 || substep > 2) return fail(SXD_ERR_INVALID_ARGUMENT, "substep must be 0..2");

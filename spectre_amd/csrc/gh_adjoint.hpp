@@ -1,0 +1,681 @@
+// Hand-written adjoints (vector-Jacobian products) of the generalized-harmonic
+// DG right-hand side: the point-wise TimeDerivative and the UpwindPenalty face
+// correction with its lift.  Host and device code.
+//
+// The forward operations are those of gh::TimeDerivative<3>::apply
+// (Evolution/Systems/GeneralizedHarmonic/TimeDerivative.cpp:30-407, harmonic
+// gauge) and of the GH UpwindPenalty (GeneralizedHarmonic/BoundaryCorrections/
+// UpwindPenalty.cpp:35-275) with the normalisation of
+// NormalCovectorAndMagnitude.hpp:47-81 and dg::lift_flux.  Each adjoint first
+// recomputes the forward intermediates at the linearisation point, then runs
+// the reverse sweep: for every forward statement y = f(x1, x2, ...) it adds
+// (df/dxk) ybar to xkbar.  Sums are accumulated in the natural loop order
+// (the adjoints are compared with the oracle's tape to a tolerance, not bit
+// for bit).  The Heaviside weights of the upwind selection are piecewise
+// constant; their derivative is zero away from the switching points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+
+namespace sxd {
+
+#define SXD_HD __host__ __device__ inline
+
+namespace adj {
+SXD_HD int iaa(int i, int a, int b) { return i + 3 * sym4(a, b); }
+SXD_HD int ijaa(int i, int j, int a, int b) { return i + 3 * j + 9 * sym4(a, b); }
+SXD_HD int abb(int a, int b, int c) { return a + 4 * sym4(b, c); }
+SXD_HD double step(double x) { return x < 0.0 ? 0.0 : 1.0; }
+
+// Lapse, shift and inverse spatial metric from g (DeterminantAndInverse.hpp,
+// Shift.cpp, Lapse.cpp).
+struct LapseShift {
+  double isp[6], shift[3], lapse, gi0[3];  // gi0[i] = g(i+1, 0)
+};
+
+SXD_HD void lapse_shift(LapseShift& L, const double* g) {
+  const double t00 = g[sym4(1, 1)], t01 = g[sym4(1, 2)], t02 = g[sym4(1, 3)];
+  const double t11 = g[sym4(2, 2)], t12 = g[sym4(2, 3)], t22 = g[sym4(3, 3)];
+  const double a = t11 * t22 - t12 * t12;
+  const double b = t12 * t02 - t01 * t22;
+  const double c = t01 * t12 - t11 * t02;
+  const double od = 1.0 / (t00 * a + t01 * b + t02 * c);
+  L.isp[0] = a * od;
+  L.isp[1] = b * od;
+  L.isp[2] = c * od;
+  L.isp[3] = (t22 * t00 - t02 * t02) * od;
+  L.isp[4] = (t02 * t01 - t00 * t12) * od;
+  L.isp[5] = (t00 * t11 - t01 * t01) * od;
+  for (int i = 0; i < 3; ++i) L.gi0[i] = g[sym4(i + 1, 0)];
+  for (int i = 0; i < 3; ++i) {
+    double s = 0.0;
+    for (int j = 0; j < 3; ++j) s += L.isp[sym3(i, j)] * L.gi0[j];
+    L.shift[i] = s;
+  }
+  double l = -g[sym4(0, 0)];
+  for (int i = 0; i < 3; ++i) l += L.shift[i] * L.gi0[i];
+  L.lapse = sqrt(l);
+}
+
+// Reverse of lapse_shift: given isp_bar (per symmetric slot), shift_bar,
+// lapse_bar, add d/dg to gbar[10].
+SXD_HD void lapse_shift_vjp(const LapseShift& L, double isp_bar[6], double shift_bar[3],
+                            double lapse_bar, double* gbar) {
+  // lapse = sqrt(l), l = -g00 + shift . gi0
+  const double l_bar = lapse_bar / (2.0 * L.lapse);
+  gbar[sym4(0, 0)] -= l_bar;
+  for (int i = 0; i < 3; ++i) {
+    shift_bar[i] += l_bar * L.gi0[i];
+    gbar[sym4(i + 1, 0)] += l_bar * L.shift[i];
+  }
+  // shift_i = sum_j isp(i, j) gi0_j
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      isp_bar[sym3(i, j)] += shift_bar[i] * L.gi0[j];
+      gbar[sym4(j + 1, 0)] += shift_bar[i] * L.isp[sym3(i, j)];
+    }
+  // inverse of the spatial metric S: dInv = -Inv dS Inv.  Slot adjoints to
+  // the symmetric full-matrix gradient G (off-diagonal halves), then
+  // M = Inv G Inv; slot (i, j) of S gets -M(i,i) or -2 M(i, j).
+  double G[3][3], Inv[3][3], T[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      Inv[i][j] = L.isp[sym3(i, j)];
+      G[i][j] = i == j ? isp_bar[sym3(i, j)] : 0.5 * isp_bar[sym3(i, j)];
+    }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s += G[i][k] * Inv[k][j];
+      T[i][j] = s;
+    }
+  for (int i = 0; i < 3; ++i)
+    for (int j = i; j < 3; ++j) {
+      double m = 0.0;
+      for (int k = 0; k < 3; ++k) m += Inv[i][k] * T[k][j];
+      gbar[sym4(i + 1, j + 1)] -= i == j ? m : 2.0 * m;
+    }
+}
+}  // namespace adj
+
+// Adjoint of the point-wise GH time derivative dt = f(u, du; gamma0..2):
+// ubar[50] and dubar[150] are OVERWRITTEN with (df/du)^T gbar and
+// (df/d du)^T gbar.  Layouts as gh_time_derivative: u = (g, Pi, Phi), du =
+// (d g at 3 p + i, d Pi at 30 + 3 p + i, d Phi at 60 + i + 3 j + 9 p).
+SXD_HD void gh_time_derivative_vjp(const double* u, const double* du, double gamma0,
+                                   double gamma1, double gamma2, const double* gbar,
+                                   double* ubar, double* dubar) {
+  using namespace adj;
+  const double* g = u;
+  const double* pi = u + 10;
+  const double* phi = u + 20;
+  double* g_b = ubar;
+  double* pi_b = ubar + 10;
+  double* phi_b = ubar + 20;
+  for (int c = 0; c < 50; ++c) ubar[c] = 0.0;
+  for (int c = 0; c < 150; ++c) dubar[c] = 0.0;
+#define PI_(a, b) pi[sym4(a, b)]
+#define PHI_(i, a, b) phi[iaa(i, a, b)]
+#define DG_(i, a, b) du[3 * sym4(a, b) + (i)]
+#define DPI_(i, a, b) du[30 + 3 * sym4(a, b) + (i)]
+#define DPHI_(i, j, a, b) du[60 + ijaa(i, j, a, b)]
+  // ---- forward ------------------------------------------------------------
+  LapseShift L;
+  lapse_shift(L, g);
+  const double lapse = L.lapse;
+  const double* shift = L.shift;
+  const double* isp = L.isp;
+  double ig[10];
+  const double m1ol2 = -1.0 / (lapse * lapse);
+  ig[sym4(0, 0)] = m1ol2;
+  for (int i = 0; i < 3; ++i) ig[sym4(0, i + 1)] = -shift[i] * m1ol2;
+  for (int i = 0; i < 3; ++i)
+    for (int j = i; j < 3; ++j) ig[sym4(i + 1, j + 1)] = isp[sym3(i, j)] + shift[i] * shift[j] * m1ol2;
+  double dtg[10];
+  for (int a = 0; a < 4; ++a)
+    for (int b = a; b < 4; ++b) {
+      double s = -lapse * PI_(a, b);
+      for (int m = 0; m < 3; ++m) s += shift[m] * PHI_(m, a, b);
+      dtg[sym4(a, b)] = s;
+    }
+  auto DA = [&](int a, int b, int c) { return a == 0 ? dtg[sym4(b, c)] : PHI_(a - 1, b, c); };
+  double chr[40];
+  for (int k = 0; k < 4; ++k)
+    for (int i = 0; i < 4; ++i)
+      for (int j = i; j < 4; ++j) chr[abb(k, i, j)] = 0.5 * (DA(i, j, k) + DA(j, i, k) - DA(k, i, j));
+  double tc[4];
+  for (int i = 0; i < 4; ++i) {
+    double s = 0.0;
+    for (int j = 0; j < 4; ++j)
+      for (int k = j; k < 4; ++k) s += (j == k ? 1.0 : 2.0) * chr[abb(i, j, k)] * ig[sym4(j, k)];
+    tc[i] = s;
+  }
+  double nv[4];
+  nv[0] = 1.0 / lapse;
+  for (int i = 0; i < 3; ++i) nv[i + 1] = -shift[i] * nv[0];
+  const double gamma12 = gamma1 * gamma2;
+  double p1u[30];
+  for (int m = 0; m < 3; ++m)
+    for (int a = 0; a < 4; ++a)
+      for (int b = a; b < 4; ++b) {
+        double s = 0.0;
+        for (int n = 0; n < 3; ++n) s += isp[sym3(m, n)] * PHI_(n, a, b);
+        p1u[iaa(m, a, b)] = s;
+      }
+  double p3u[48];  // p3u[m + 3 (nu + 4 al)]
+  for (int m = 0; m < 3; ++m)
+    for (int nu = 0; nu < 4; ++nu)
+      for (int al = 0; al < 4; ++al) {
+        double s = 0.0;
+        for (int be = 0; be < 4; ++be) s += ig[sym4(al, be)] * PHI_(m, nu, be);
+        p3u[m + 3 * (nu + 4 * al)] = s;
+      }
+  double p2u[16];  // p2u[nu + 4 al]
+  for (int nu = 0; nu < 4; ++nu)
+    for (int al = 0; al < 4; ++al) {
+      double s = 0.0;
+      for (int be = 0; be < 4; ++be) s += ig[sym4(al, be)] * PI_(nu, be);
+      p2u[nu + 4 * al] = s;
+    }
+  double c3u[64];  // c3u[mu + 4 (nu + 4 al)] = sum_be ig(al, be) chr(mu, nu, be)
+  for (int mu = 0; mu < 4; ++mu)
+    for (int nu = 0; nu < 4; ++nu)
+      for (int al = 0; al < 4; ++al) {
+        double s = 0.0;
+        for (int be = 0; be < 4; ++be) s += ig[sym4(al, be)] * chr[abb(mu, nu, be)];
+        c3u[mu + 4 * (nu + 4 * al)] = s;
+      }
+  double pion[4];
+  for (int mu = 0; mu < 4; ++mu) {
+    double s = 0.0;
+    for (int nu = 0; nu < 4; ++nu) s += nv[nu] * PI_(nu, mu);
+    pion[mu] = s;
+  }
+  double hptn = 0.0;
+  for (int mu = 0; mu < 4; ++mu) hptn += nv[mu] * pion[mu];
+  hptn *= 0.5;
+  double pon[12];  // pon[n + 3 nu] = sum_mu nv[mu] Phi(n, mu, nu)
+  for (int n = 0; n < 3; ++n)
+    for (int nu = 0; nu < 4; ++nu) {
+      double s = 0.0;
+      for (int mu = 0; mu < 4; ++mu) s += nv[mu] * PHI_(n, mu, nu);
+      pon[n + 3 * nu] = s;
+    }
+  double hphi[3];
+  for (int n = 0; n < 3; ++n) {
+    double s = 0.0;
+    for (int mu = 0; mu < 4; ++mu) s += nv[mu] * pon[n + 3 * mu];
+    hphi[n] = 0.5 * s;
+  }
+  double c3[30];
+  for (int n = 0; n < 3; ++n)
+    for (int a = 0; a < 4; ++a)
+      for (int b = a; b < 4; ++b) c3[iaa(n, a, b)] = DG_(n, a, b) - PHI_(n, a, b);
+  double sdc3[10];
+  for (int a = 0; a < 4; ++a)
+    for (int b = a; b < 4; ++b) {
+      double s = 0.0;
+      for (int m = 0; m < 3; ++m) s += shift[m] * c3[iaa(m, a, b)];
+      sdc3[sym4(a, b)] = s;
+    }
+  double ndgc_raw = 0.0;
+  for (int mu = 0; mu < 4; ++mu) ndgc_raw += nv[mu] * tc[mu];
+  const double ndgc = ndgc_raw * gamma0;
+  const double tmp00 = -gamma0 * lapse;
+
+  // ---- reverse ------------------------------------------------------------
+  double lapse_b = 0.0, shift_b[3] = {0.0, 0.0, 0.0}, isp_b[6] = {0, 0, 0, 0, 0, 0};
+  double ig_b[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, dtg_b[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double chr_b[40], tc_b[4] = {0, 0, 0, 0}, nv_b[4] = {0, 0, 0, 0};
+  double p1u_b[30], p3u_b[48], p2u_b[16], c3u_b[64], pion_b[4] = {0, 0, 0, 0}, hptn_b = 0.0;
+  double pon_b[12], hphi_b[3] = {0, 0, 0}, c3_b[30], sdc3_b[10], ndgc_b = 0.0, tmp00_b = 0.0;
+  for (int x = 0; x < 40; ++x) chr_b[x] = 0.0;
+  for (int x = 0; x < 30; ++x) p1u_b[x] = c3_b[x] = 0.0;
+  for (int x = 0; x < 48; ++x) p3u_b[x] = 0.0;
+  for (int x = 0; x < 16; ++x) p2u_b[x] = 0.0;
+  for (int x = 0; x < 64; ++x) c3u_b[x] = 0.0;
+  for (int x = 0; x < 12; ++x) pon_b[x] = 0.0;
+  for (int x = 0; x < 10; ++x) sdc3_b[x] = 0.0;
+
+  // dt Phi(i, mu nu) = S lapse + sum_m shift_m dPhi(m, i, mu nu),
+  // S = Pi(mu nu) hphi_i - dPi(i, mu nu) + gamma2 C3(i, mu nu)
+  //     + sum_n pon[i + 3 (n + 1)] P1U(n, mu nu)
+  for (int i = 0; i < 3; ++i)
+    for (int a = 0; a < 4; ++a)
+      for (int b = a; b < 4; ++b) {
+        const double w = gbar[20 + iaa(i, a, b)];
+        double S = PI_(a, b) * hphi[i] - DPI_(i, a, b) + gamma2 * c3[iaa(i, a, b)];
+        for (int n = 0; n < 3; ++n) S += pon[i + 3 * (n + 1)] * p1u[iaa(n, a, b)];
+        lapse_b += w * S;
+        for (int m = 0; m < 3; ++m) {
+          shift_b[m] += w * DPHI_(m, i, a, b);
+          dubar[60 + ijaa(m, i, a, b)] += w * shift[m];
+        }
+        const double Sb = w * lapse;
+        pi_b[sym4(a, b)] += Sb * hphi[i];
+        hphi_b[i] += Sb * PI_(a, b);
+        dubar[30 + 3 * sym4(a, b) + i] -= Sb;
+        c3_b[iaa(i, a, b)] += gamma2 * Sb;
+        for (int n = 0; n < 3; ++n) {
+          pon_b[i + 3 * (n + 1)] += Sb * p1u[iaa(n, a, b)];
+          p1u_b[iaa(n, a, b)] += Sb * pon[i + 3 * (n + 1)];
+        }
+      }
+  // dt Pi(mu nu) = R lapse + gamma12 sdc3(mu nu) + sum_m shift_m dPi(m, mu nu)
+  for (int mu = 0; mu < 4; ++mu)
+    for (int nu = mu; nu < 4; ++nu) {
+      const double w = gbar[10 + sym4(mu, nu)];
+      double base;
+      if (mu == 0 && nu == 0)
+        base = 2.0 * tmp00 * tc[0] - ndgc * g[sym4(0, 0)];
+      else if (mu == 0)
+        base = tmp00 * tc[nu] - ndgc * g[sym4(0, nu)];
+      else
+        base = -ndgc * g[sym4(mu, nu)];
+      double R = base - hptn * PI_(mu, nu);
+      for (int de = 0; de < 4; ++de) {
+        R += -2.0 * PI_(mu, de) * p2u[nu + 4 * de];
+        for (int n = 0; n < 3; ++n) R += 2.0 * p1u[iaa(n, mu, de)] * p3u[n + 3 * (nu + 4 * de)];
+        for (int al = 0; al < 4; ++al)
+          R += -2.0 * c3u[mu + 4 * (al + 4 * de)] * c3u[nu + 4 * (de + 4 * al)];
+      }
+      for (int m = 0; m < 3; ++m) {
+        R += -pion[m + 1] * p1u[iaa(m, mu, nu)];
+        for (int n = 0; n < 3; ++n) R += -isp[sym3(m, n)] * DPHI_(m, n, mu, nu);
+      }
+      lapse_b += w * R;
+      sdc3_b[sym4(mu, nu)] += gamma12 * w;
+      for (int m = 0; m < 3; ++m) {
+        shift_b[m] += w * DPI_(m, mu, nu);
+        dubar[30 + 3 * sym4(mu, nu) + m] += w * shift[m];
+      }
+      const double Rb = w * lapse;
+      // base
+      if (mu == 0 && nu == 0) {
+        tmp00_b += 2.0 * Rb * tc[0];
+        tc_b[0] += 2.0 * tmp00 * Rb;
+      } else if (mu == 0) {
+        tmp00_b += Rb * tc[nu];
+        tc_b[nu] += tmp00 * Rb;
+      }
+      ndgc_b -= Rb * g[sym4(mu, nu)];
+      g_b[sym4(mu, nu)] -= Rb * ndgc;
+      hptn_b -= Rb * PI_(mu, nu);
+      pi_b[sym4(mu, nu)] -= Rb * hptn;
+      for (int de = 0; de < 4; ++de) {
+        pi_b[sym4(mu, de)] += -2.0 * Rb * p2u[nu + 4 * de];
+        p2u_b[nu + 4 * de] += -2.0 * Rb * PI_(mu, de);
+        for (int n = 0; n < 3; ++n) {
+          p1u_b[iaa(n, mu, de)] += 2.0 * Rb * p3u[n + 3 * (nu + 4 * de)];
+          p3u_b[n + 3 * (nu + 4 * de)] += 2.0 * Rb * p1u[iaa(n, mu, de)];
+        }
+        for (int al = 0; al < 4; ++al) {
+          c3u_b[mu + 4 * (al + 4 * de)] += -2.0 * Rb * c3u[nu + 4 * (de + 4 * al)];
+          c3u_b[nu + 4 * (de + 4 * al)] += -2.0 * Rb * c3u[mu + 4 * (al + 4 * de)];
+        }
+      }
+      for (int m = 0; m < 3; ++m) {
+        pion_b[m + 1] -= Rb * p1u[iaa(m, mu, nu)];
+        p1u_b[iaa(m, mu, nu)] -= Rb * pion[m + 1];
+        for (int n = 0; n < 3; ++n) {
+          isp_b[sym3(m, n)] -= Rb * DPHI_(m, n, mu, nu);
+          dubar[60 + ijaa(m, n, mu, nu)] -= Rb * isp[sym3(m, n)];
+        }
+      }
+    }
+  // tmp00 = -gamma0 lapse; ndgc = gamma0 ndgc_raw
+  lapse_b += -gamma0 * tmp00_b;
+  const double ndgc_raw_b = gamma0 * ndgc_b;
+  // dt g(mu nu) = dtg + (1 + gamma1) sdc3
+  for (int x = 0; x < 10; ++x) {
+    dtg_b[x] += gbar[x];
+    sdc3_b[x] += (1.0 + gamma1) * gbar[x];
+  }
+  // ndgc_raw = sum nv[mu] tc[mu]
+  for (int mu = 0; mu < 4; ++mu) {
+    nv_b[mu] += ndgc_raw_b * tc[mu];
+    tc_b[mu] += ndgc_raw_b * nv[mu];
+  }
+  // sdc3 = sum_m shift_m c3(m)
+  for (int a = 0; a < 4; ++a)
+    for (int b = a; b < 4; ++b)
+      for (int m = 0; m < 3; ++m) {
+        shift_b[m] += sdc3_b[sym4(a, b)] * c3[iaa(m, a, b)];
+        c3_b[iaa(m, a, b)] += sdc3_b[sym4(a, b)] * shift[m];
+      }
+  // c3 = dg - Phi
+  for (int n = 0; n < 3; ++n)
+    for (int a = 0; a < 4; ++a)
+      for (int b = a; b < 4; ++b) {
+        dubar[3 * sym4(a, b) + n] += c3_b[iaa(n, a, b)];
+        phi_b[iaa(n, a, b)] -= c3_b[iaa(n, a, b)];
+      }
+  // hphi[n] = 0.5 sum_mu nv[mu] pon[n + 3 mu]
+  for (int n = 0; n < 3; ++n)
+    for (int mu = 0; mu < 4; ++mu) {
+      nv_b[mu] += 0.5 * hphi_b[n] * pon[n + 3 * mu];
+      pon_b[n + 3 * mu] += 0.5 * hphi_b[n] * nv[mu];
+    }
+  // pon[n + 3 nu] = sum_mu nv[mu] Phi(n, mu, nu)
+  for (int n = 0; n < 3; ++n)
+    for (int nu = 0; nu < 4; ++nu)
+      for (int mu = 0; mu < 4; ++mu) {
+        nv_b[mu] += pon_b[n + 3 * nu] * PHI_(n, mu, nu);
+        phi_b[iaa(n, mu, nu)] += pon_b[n + 3 * nu] * nv[mu];
+      }
+  // hptn = 0.5 sum nv[mu] pion[mu]
+  for (int mu = 0; mu < 4; ++mu) {
+    nv_b[mu] += 0.5 * hptn_b * pion[mu];
+    pion_b[mu] += 0.5 * hptn_b * nv[mu];
+  }
+  // pion[mu] = sum_nu nv[nu] Pi(nu, mu)
+  for (int mu = 0; mu < 4; ++mu)
+    for (int nu = 0; nu < 4; ++nu) {
+      nv_b[nu] += pion_b[mu] * PI_(nu, mu);
+      pi_b[sym4(nu, mu)] += pion_b[mu] * nv[nu];
+    }
+  // c3u, p2u, p3u, p1u
+  for (int mu = 0; mu < 4; ++mu)
+    for (int nu = 0; nu < 4; ++nu)
+      for (int al = 0; al < 4; ++al) {
+        const double w = c3u_b[mu + 4 * (nu + 4 * al)];
+        for (int be = 0; be < 4; ++be) {
+          ig_b[sym4(al, be)] += w * chr[abb(mu, nu, be)];
+          chr_b[abb(mu, nu, be)] += w * ig[sym4(al, be)];
+        }
+      }
+  for (int nu = 0; nu < 4; ++nu)
+    for (int al = 0; al < 4; ++al) {
+      const double w = p2u_b[nu + 4 * al];
+      for (int be = 0; be < 4; ++be) {
+        ig_b[sym4(al, be)] += w * PI_(nu, be);
+        pi_b[sym4(nu, be)] += w * ig[sym4(al, be)];
+      }
+    }
+  for (int m = 0; m < 3; ++m)
+    for (int nu = 0; nu < 4; ++nu)
+      for (int al = 0; al < 4; ++al) {
+        const double w = p3u_b[m + 3 * (nu + 4 * al)];
+        for (int be = 0; be < 4; ++be) {
+          ig_b[sym4(al, be)] += w * PHI_(m, nu, be);
+          phi_b[iaa(m, nu, be)] += w * ig[sym4(al, be)];
+        }
+      }
+  for (int m = 0; m < 3; ++m)
+    for (int a = 0; a < 4; ++a)
+      for (int b = a; b < 4; ++b) {
+        const double w = p1u_b[iaa(m, a, b)];
+        for (int n = 0; n < 3; ++n) {
+          isp_b[sym3(m, n)] += w * PHI_(n, a, b);
+          phi_b[iaa(n, a, b)] += w * isp[sym3(m, n)];
+        }
+      }
+  // nv[0] = 1 / lapse, nv[i + 1] = -shift_i nv[0]
+  double nv0_b = nv_b[0];
+  for (int i = 0; i < 3; ++i) {
+    shift_b[i] -= nv_b[i + 1] * nv[0];
+    nv0_b -= nv_b[i + 1] * shift[i];
+  }
+  lapse_b -= nv0_b / (lapse * lapse);
+  // tc[i] = sum_{j <= k} w_jk chr(i, j, k) ig(j, k)
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j)
+      for (int k = j; k < 4; ++k) {
+        const double f = (j == k ? 1.0 : 2.0) * tc_b[i];
+        chr_b[abb(i, j, k)] += f * ig[sym4(j, k)];
+        ig_b[sym4(j, k)] += f * chr[abb(i, j, k)];
+      }
+  // chr(k, i, j) = 0.5 (DA(i, j, k) + DA(j, i, k) - DA(k, i, j))
+  auto add_da = [&](int a, int b, int c, double v) {
+    if (a == 0)
+      dtg_b[sym4(b, c)] += v;
+    else
+      phi_b[iaa(a - 1, b, c)] += v;
+  };
+  for (int k = 0; k < 4; ++k)
+    for (int i = 0; i < 4; ++i)
+      for (int j = i; j < 4; ++j) {
+        const double h = 0.5 * chr_b[abb(k, i, j)];
+        add_da(i, j, k, h);
+        add_da(j, i, k, h);
+        add_da(k, i, j, -h);
+      }
+  // dtg(a b) = -lapse Pi(a b) + sum_m shift_m Phi(m, a, b)
+  for (int a = 0; a < 4; ++a)
+    for (int b = a; b < 4; ++b) {
+      const double w = dtg_b[sym4(a, b)];
+      lapse_b -= w * PI_(a, b);
+      pi_b[sym4(a, b)] -= w * lapse;
+      for (int m = 0; m < 3; ++m) {
+        shift_b[m] += w * PHI_(m, a, b);
+        phi_b[iaa(m, a, b)] += w * shift[m];
+      }
+    }
+  // inverse spacetime metric
+  double m1ol2_b = ig_b[sym4(0, 0)];
+  for (int i = 0; i < 3; ++i) {
+    shift_b[i] -= ig_b[sym4(0, i + 1)] * m1ol2;
+    m1ol2_b -= ig_b[sym4(0, i + 1)] * shift[i];
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = i; j < 3; ++j) {
+      const double w = ig_b[sym4(i + 1, j + 1)];
+      isp_b[sym3(i, j)] += w;
+      shift_b[i] += w * shift[j] * m1ol2;
+      shift_b[j] += w * shift[i] * m1ol2;
+      m1ol2_b += w * shift[i] * shift[j];
+    }
+  // m1ol2 = -1 / lapse^2
+  lapse_b += m1ol2_b * 2.0 / (lapse * lapse * lapse);
+  lapse_shift_vjp(L, isp_b, shift_b, lapse_b, g_b);
+#undef PI_
+#undef PHI_
+#undef DG_
+#undef DPI_
+#undef DPHI_
+}
+
+// One side of a face point: unit normal covector/vector, magnitude and
+// speeds from g, gamma1 and the raw normal covector (the inverse-Jacobian row,
+// sign already applied).
+struct GhSideFwd {
+  adj::LapseShift L;
+  double nraw[3], nvraw[3], mag, n[3], nv[3], lam[4];
+};
+
+SXD_HD void gh_side_forward(GhSideFwd& s, const double* g, double gamma1, const double nraw[3]) {
+  adj::lapse_shift(s.L, g);
+  for (int i = 0; i < 3; ++i) s.nraw[i] = nraw[i];
+  for (int i = 0; i < 3; ++i) {
+    double v = 0.0;
+    for (int j = 0; j < 3; ++j) v += s.L.isp[sym3(i, j)] * nraw[j];
+    s.nvraw[i] = v;
+  }
+  double m = 0.0;
+  for (int i = 0; i < 3; ++i) m += s.nvraw[i] * nraw[i];
+  s.mag = sqrt(m);
+  for (int i = 0; i < 3; ++i) {
+    s.n[i] = nraw[i] / s.mag;
+    s.nv[i] = s.nvraw[i] / s.mag;
+  }
+  double sdn = 0.0;
+  for (int i = 0; i < 3; ++i) sdn += s.L.shift[i] * s.n[i];
+  sdn = -sdn;
+  s.lam[1] = sdn;
+  s.lam[0] = (1.0 + gamma1) * sdn;
+  s.lam[2] = s.L.lapse + sdn;
+  s.lam[3] = -s.L.lapse + sdn;
+}
+
+// Reverse of gh_side_forward for adjoints of n, nv, lam and mag: adds to gbar.
+SXD_HD void gh_side_vjp(const GhSideFwd& s, double gamma1, const double n_b_in[3],
+                        const double nv_b_in[3], const double lam_b[4], double mag_b,
+                        double* gbar) {
+  double n_b[3] = {n_b_in[0], n_b_in[1], n_b_in[2]};
+  double shift_b[3] = {0.0, 0.0, 0.0}, isp_b[6] = {0, 0, 0, 0, 0, 0};
+  double lapse_b = lam_b[2] - lam_b[3];
+  const double sdn_b = lam_b[1] + (1.0 + gamma1) * lam_b[0] + lam_b[2] + lam_b[3];
+  // sdn = -sum shift_i n_i
+  for (int i = 0; i < 3; ++i) {
+    shift_b[i] -= sdn_b * s.n[i];
+    n_b[i] -= sdn_b * s.L.shift[i];
+  }
+  // n = nraw / mag, nv = nvraw / mag
+  double nvraw_b[3];
+  for (int i = 0; i < 3; ++i) {
+    nvraw_b[i] = nv_b_in[i] / s.mag;
+    mag_b -= (n_b[i] * s.n[i] + nv_b_in[i] * s.nv[i]) / s.mag;
+  }
+  // mag = sqrt(nvraw . nraw)
+  const double m_b = mag_b / (2.0 * s.mag);
+  for (int i = 0; i < 3; ++i) nvraw_b[i] += m_b * s.nraw[i];
+  // nvraw_i = sum_j isp(i, j) nraw_j
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) isp_b[sym3(i, j)] += nvraw_b[i] * s.nraw[j];
+  adj::lapse_shift_vjp(s.L, isp_b, shift_b, lapse_b, gbar);
+}
+
+// Adjoint of the lifted GH UpwindPenalty correction at one face point,
+// dt_int += lift_factor(N_perp, |n_int|) bc(u_int, u_ext).  Given gbar = the
+// adjoint of dt at the interior point (50), ADDS the contributions to
+// ubar_int (50) and ubar_ext (50).  gamma1/gamma2 per side; nraw_* the
+// unnormalised normal covectors as used by the forward (interior: pointing
+// out of the interior element; exterior: the neighbour's own, sign as in the
+// forward).
+SXD_HD void gh_face_vjp(const double* u_int, double gamma1_int, double gamma2_int,
+                        const double nraw_int[3], const double* u_ext, double gamma1_ext,
+                        double gamma2_ext, const double nraw_ext[3], int n_perp,
+                        const double* gbar, double* ubar_int, double* ubar_ext) {
+  using namespace adj;
+  GhSideFwd si, se;
+  gh_side_forward(si, u_int, gamma1_int, nraw_int);
+  gh_side_forward(se, u_ext, gamma1_ext, nraw_ext);
+  const double fac = -0.5 * static_cast<double>(n_perp * (n_perp - 1)) * si.mag;
+  // weights of the upwind selection (piecewise constant)
+  const double wg_i = step(-si.lam[0]), wg_e = -step(se.lam[0]);
+  const double wz_i = step(-si.lam[1]), wz_e = -step(se.lam[1]);
+  const double wp_i = step(-si.lam[2]), wp_e = -step(se.lam[2]);
+  const double wm_i = step(-si.lam[3]), wm_e = -step(se.lam[3]);
+  double n_bi[3] = {0, 0, 0}, nv_bi[3] = {0, 0, 0}, lam_bi[4] = {0, 0, 0, 0};
+  double n_be[3] = {0, 0, 0}, nv_be[3] = {0, 0, 0}, lam_be[4] = {0, 0, 0, 0};
+  double fac_b = 0.0;
+  for (int pr = 0; pr < 10; ++pr) {
+    const int comp[5] = {pr, 10 + pr, 20 + 3 * pr, 21 + 3 * pr, 22 + 3 * pr};
+    // forward package of both sides (for bc and the lam adjoints)
+    const GhSideFwd* sides[2] = {&si, &se};
+    const double* us[2] = {u_int, u_ext};
+    const double g2s[2] = {gamma2_int, gamma2_ext};
+    double pk[2][13], ndphi[2], g2g[2];
+    for (int sd = 0; sd < 2; ++sd) {
+      const GhSideFwd& S = *sides[sd];
+      const double g = us[sd][pr], pi = us[sd][10 + pr];
+      const double* phi = us[sd] + 20 + 3 * pr;
+      g2g[sd] = g2s[sd] * g;
+      ndphi[sd] = 0.0;
+      for (int i = 0; i < 3; ++i) ndphi[sd] += S.nv[i] * phi[i];
+      pk[sd][4] = S.lam[2] * ((pi + ndphi[sd]) - g2g[sd]);
+      pk[sd][5] = S.lam[3] * ((pi - ndphi[sd]) - g2g[sd]);
+      for (int i = 0; i < 3; ++i) pk[sd][1 + i] = S.lam[1] * (phi[i] - S.n[i] * ndphi[sd]);
+      for (int d = 0; d < 3; ++d) {
+        pk[sd][6 + d] = pk[sd][4] * S.n[d];
+        pk[sd][9 + d] = pk[sd][5] * S.n[d];
+      }
+      pk[sd][0] = S.lam[0] * g;
+      pk[sd][12] = g2g[sd] * S.lam[0];
+    }
+    const double* in = pk[0];
+    const double* ex = pk[1];
+    double bc[5];
+    bc[0] = wg_e * ex[0] - wg_i * in[0];
+    bc[1] = 0.5 * (wp_e * ex[4] + wm_e * ex[5]) + wg_e * ex[12] - 0.5 * (wp_i * in[4] + wm_i * in[5]) -
+            wg_i * in[12];
+    for (int d = 0; d < 3; ++d)
+      bc[2 + d] = -0.5 * (wm_e * ex[9 + d] - wp_e * ex[6 + d]) + wz_e * ex[1 + d] -
+                  0.5 * (wp_i * in[6 + d] - wm_i * in[9 + d]) - wz_i * in[1 + d];
+    // dt += bc fac
+    double bc_b[5];
+    for (int c = 0; c < 5; ++c) {
+      bc_b[c] = gbar[comp[c]] * fac;
+      fac_b += gbar[comp[c]] * bc[c];
+    }
+    // boundary terms reverse (weights constant)
+    double pk_b[2][13];
+    for (int sd = 0; sd < 2; ++sd)
+      for (int c = 0; c < 13; ++c) pk_b[sd][c] = 0.0;
+    pk_b[1][0] += wg_e * bc_b[0];
+    pk_b[0][0] -= wg_i * bc_b[0];
+    pk_b[1][4] += 0.5 * wp_e * bc_b[1];
+    pk_b[1][5] += 0.5 * wm_e * bc_b[1];
+    pk_b[1][12] += wg_e * bc_b[1];
+    pk_b[0][4] -= 0.5 * wp_i * bc_b[1];
+    pk_b[0][5] -= 0.5 * wm_i * bc_b[1];
+    pk_b[0][12] -= wg_i * bc_b[1];
+    for (int d = 0; d < 3; ++d) {
+      const double w = bc_b[2 + d];
+      pk_b[1][9 + d] -= 0.5 * wm_e * w;
+      pk_b[1][6 + d] += 0.5 * wp_e * w;
+      pk_b[1][1 + d] += wz_e * w;
+      pk_b[0][6 + d] -= 0.5 * wp_i * w;
+      pk_b[0][9 + d] += 0.5 * wm_i * w;
+      pk_b[0][1 + d] -= wz_i * w;
+    }
+    // package reverse
+    double* ub[2] = {ubar_int, ubar_ext};
+    double* nbs[2] = {n_bi, n_be};
+    double* nvbs[2] = {nv_bi, nv_be};
+    double* lbs[2] = {lam_bi, lam_be};
+    for (int sd = 0; sd < 2; ++sd) {
+      const GhSideFwd& S = *sides[sd];
+      const double* P = pk[sd];
+      double* B = pk_b[sd];
+      const double g = us[sd][pr], pi = us[sd][10 + pr];
+      const double* phi = us[sd] + 20 + 3 * pr;
+      double g2g_b = 0.0, ndphi_b = 0.0, pi_b = 0.0, g_b = 0.0, phi_b[3] = {0, 0, 0};
+      // pk12 = g2g lam0; pk0 = lam0 g
+      g2g_b += B[12] * S.lam[0];
+      lbs[sd][0] += B[12] * g2g[sd] + B[0] * g;
+      g_b += B[0] * S.lam[0];
+      // pk6+d = pk4 n_d; pk9+d = pk5 n_d
+      for (int d = 0; d < 3; ++d) {
+        B[4] += B[6 + d] * S.n[d];
+        B[5] += B[9 + d] * S.n[d];
+        nbs[sd][d] += B[6 + d] * P[4] + B[9 + d] * P[5];
+      }
+      // pk1+i = lam1 (phi_i - n_i ndphi)
+      for (int i = 0; i < 3; ++i) {
+        const double inner = phi[i] - S.n[i] * ndphi[sd];
+        lbs[sd][1] += B[1 + i] * inner;
+        const double ib = B[1 + i] * S.lam[1];
+        phi_b[i] += ib;
+        nbs[sd][i] -= ib * ndphi[sd];
+        ndphi_b -= ib * S.n[i];
+      }
+      // pk4 = lam2 ((pi + ndphi) - g2g); pk5 = lam3 ((pi - ndphi) - g2g)
+      lbs[sd][2] += B[4] * ((pi + ndphi[sd]) - g2g[sd]);
+      lbs[sd][3] += B[5] * ((pi - ndphi[sd]) - g2g[sd]);
+      const double a4 = B[4] * S.lam[2], a5 = B[5] * S.lam[3];
+      pi_b += a4 + a5;
+      ndphi_b += a4 - a5;
+      g2g_b -= a4 + a5;
+      // ndphi = nv . phi; g2g = gamma2 g
+      for (int i = 0; i < 3; ++i) {
+        nvbs[sd][i] += ndphi_b * phi[i];
+        phi_b[i] += ndphi_b * S.nv[i];
+      }
+      g_b += g2g_b * g2s[sd];
+      ub[sd][pr] += g_b;
+      ub[sd][10 + pr] += pi_b;
+      for (int i = 0; i < 3; ++i) ub[sd][20 + 3 * pr + i] += phi_b[i];
+    }
+  }
+  // fac = -N(N-1)/2 |n_int|
+  const double mag_b = fac_b * (-0.5 * static_cast<double>(n_perp * (n_perp - 1)));
+  gh_side_vjp(si, gamma1_int, n_bi, nv_bi, lam_bi, mag_b, ubar_int);
+  gh_side_vjp(se, gamma1_ext, n_be, nv_be, lam_be, 0.0, ubar_ext);
+}
+
+#undef SXD_HD
+
+}  // namespace sxd

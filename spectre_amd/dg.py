@@ -235,6 +235,19 @@ class Domain:
                                                 _ptr(inv_jac), _ptr(out), _stream()))
         return out
 
+    def gh_rhs_adjoint(self, u, gammas, inv_jac, gbar, out=None):
+        """Adjoint of gh_rhs linearised at u: out = (d dt / d u)^T gbar."""
+        _require(u, "u", self._shapes(50))
+        _require(gammas, "gammas", self._shapes(3))
+        _require(inv_jac, "inv_jac", self._shapes(9))
+        _require(gbar, "gbar", self._shapes(50))
+        if out is None:
+            out = torch.empty_like(u)
+        _require(out, "out", self._shapes(50))
+        check(lib().sxd_gh_rhs_adjoint(self._handle, _ptr(u), _ptr(gammas), _ptr(inv_jac),
+                                       _ptr(gbar), _ptr(out), _stream()))
+        return out
+
     def set_threads(self, system, threads):
         """Threads per workgroup (tile shape) of the kernels for `system`."""
         check(lib().sxd_domain_set_threads(self._handle, SYSTEMS[system], threads))

@@ -424,3 +424,20 @@ def test_scalar_wave_gradient_matches_assembled_transpose():
         ref = S.T @ ref
     got = host(grad).ravel()
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+# ---- GH adjoint ----------------------------------------------------------------
+@pytest.mark.parametrize("elements,extents", [((2, 1, 1), (3, 4, 3)), ((1, 2, 2), (4, 4, 4))])
+def test_gh_rhs_adjoint_matches_oracle_tape(elements, extents):
+    """Hand-written GH adjoint kernels against the oracle's reverse-mode tape
+    (exact vector-Jacobian product of the oracle's GH RHS)."""
+    brick, u, gammas, inv = wl.gh_kerr_schild_brick(seed=3, elements=elements, extents=extents,
+                                                    noise=1e-3)
+    rng = np.random.default_rng(9)
+    gbar = rng.standard_normal(u.shape)
+    ref = ol.gh_rhs_vjp(brick.extents, brick.neighbors, u, gammas, inv, gbar)
+    domain = dg.Domain.from_brick(brick)
+    got = host(domain.gh_rhs_adjoint(dev(u), dev(gammas), dev(inv), dev(gbar)))
+    err = rel_err(got, ref)
+    print(f"GH adjoint vs tape: max-norm relative error {err:.3e}")
+    assert err <= 1e-12
