@@ -119,6 +119,32 @@ int sxo_gh_time_derivative_vjp(int npts, const double* u, const double* du, cons
   return 0;
 }
 
+// GH evolution through `steps` Rk3HesthavenSsp steps (uniform brick):
+// ubar0 = (d u_final / d u0)^T gbar_final, by the tape over the whole
+// evolution (the transpose of the assembled CPU operator, applied to one
+// vector).
+int sxo_gh_evolve_vjp(int num_elements, const int* ext, const int* nbr, const double* u0,
+                      const double* gammas, const double* inv_j, double dt, int steps,
+                      const double* gbar_final, double* ubar0) {
+  const size_t n = 50 * points(num_elements, ext);
+  auto ga = constants(gammas, 3 * points(num_elements, ext));
+  auto ij = constants(inv_j, 9 * points(num_elements, ext));
+  tape_vjp(n, u0, n, gbar_final, ubar0, [&](const Rev* x, Rev* y) {
+    std::vector<Rev> u(x, x + n), u1(n), u2(n), l(n);
+    for (int s = 0; s < steps; ++s) {
+      sxo::gh_rhs(num_elements, ext, nbr, u.data(), ga.data(), ij.data(), l.data());
+      sxo::rk3_substep(0, n, dt, u.data(), u.data(), l.data(), u1.data());
+      sxo::gh_rhs(num_elements, ext, nbr, u1.data(), ga.data(), ij.data(), l.data());
+      sxo::rk3_substep(1, n, dt, u.data(), u1.data(), l.data(), u2.data());
+      sxo::gh_rhs(num_elements, ext, nbr, u2.data(), ga.data(), ij.data(), l.data());
+      sxo::rk3_substep(2, n, dt, u.data(), u2.data(), l.data(), u1.data());
+      u.swap(u1);
+    }
+    for (size_t i = 0; i < n; ++i) y[i] = u[i];
+  });
+  return 0;
+}
+
 // ScalarWave DG RHS by the tape (cross-check of the hand-written adjoint).
 int sxo_sw_rhs_vjp(int num_elements, const int* ext, const int* nbr, const double* u,
                    const double* gamma2, const double* inv_j, const double* gbar, double* ubar) {

@@ -95,6 +95,114 @@ class ScalarWaveEvolution(torch.autograd.Function):
         return lam3, None, None, None, None, None, None, None
 
 
+def rk3_step_states(rhs, u, params, inv_jac, dt, l):
+    """One RK3 step of u (in place) returning its stage inputs (u, u1, u2)."""
+    u0 = u.clone()
+    rhs(u, params, inv_jac, out=l)
+    u1 = dg.rk3_substep(0, dt, u, u, l)
+    rhs(u1, params, inv_jac, out=l)
+    u2 = dg.rk3_substep(1, dt, u, u1, l)
+    rhs(u2, params, inv_jac, out=l)
+    dg.rk3_substep(2, dt, u, u2, l, out=u)
+    return u0, u1, u2
+
+
+class GhEvolution(torch.autograd.Function):
+    """u_final after `steps` RK3 steps of the GH DG RHS on `domain`.
+
+    The GH RHS is nonlinear, so each step's adjoint needs the step's stage
+    inputs u, u1, u2: the forward keeps the state every `checkpoint_every`
+    steps, and the backward recomputes the stage inputs of one segment at a
+    time from its checkpoint, then applies the transposed substeps with the
+    adjoint RHS linearised at those stages (sxd_gh_rhs_adjoint)."""
+
+    @staticmethod
+    def forward(ctx, u0, domain, gammas, inv_jac, dt, steps, checkpoint_every=8):
+        u = u0.detach().clone()
+        l = torch.empty_like(u)
+        checkpoints = []
+        for n in range(steps):
+            if n % checkpoint_every == 0:
+                checkpoints.append(u.clone())
+            rk3_step(domain, domain.gh_rhs, u, gammas, inv_jac, dt, torch.empty_like(u), l)
+        ctx.domain, ctx.dt, ctx.steps, ctx.k = domain, dt, steps, checkpoint_every
+        ctx.checkpoints = checkpoints
+        ctx.save_for_backward(gammas, inv_jac)
+        return u
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        gammas, inv_jac = ctx.saved_tensors
+        domain, dt, k = ctx.domain, ctx.dt, ctx.k
+        lam3 = grad_out.detach().contiguous().clone()
+        a, lam, mu, l = (torch.empty_like(lam3) for _ in range(4))
+        for seg in reversed(range(len(ctx.checkpoints))):
+            first = seg * k
+            last = min(ctx.steps, first + k)
+            u = ctx.checkpoints[seg].clone()
+            stages = [rk3_step_states(domain.gh_rhs, u, gammas, inv_jac, dt, l)
+                      for _ in range(first, last)]
+            for s0, s1, s2 in reversed(stages):
+                domain.gh_rhs_adjoint(s2, gammas, inv_jac, lam3, out=a)
+                rk3_adjoint_substep(2, dt, lam3, a, mu, lam)
+                domain.gh_rhs_adjoint(s1, gammas, inv_jac, lam, out=a)
+                rk3_adjoint_substep(1, dt, lam, a, mu, lam)
+                domain.gh_rhs_adjoint(s0, gammas, inv_jac, lam, out=a)
+                rk3_adjoint_substep(0, dt, lam, a, mu, None)
+                lam3, mu = mu, lam3
+        ctx.checkpoints = None
+        return lam3, None, None, None, None, None, None
+
+
+def gauge_constraint(u):
+    """Gauge constraint C_a of the first-order GH system with the harmonic gauge
+    source (H_a = 0): the trace of the Christoffel symbols written with Pi,
+    Phi and the normals, as gh::gauge_constraint / gh::trace_christoffel
+    (Evolution/Systems/GeneralizedHarmonic/Constraints.cpp:983-998,
+    PointwiseFunctions/GeneralRelativity/GeneralizedHarmonic/Christoffel.cpp:64-119).
+    u [E][50][Np] -> C [E][4][Np] (differentiable torch operations)."""
+    pairs = [(a, b) for a in range(4) for b in range(a, 4)]
+    idx = {}
+    for p, (a, b) in enumerate(pairs):
+        idx[(a, b)] = idx[(b, a)] = p
+    full = lambda base: torch.stack([torch.stack([u[:, base + idx[(a, b)]] for b in range(4)], 1)
+                                     for a in range(4)], 1)          # [E][4][4][Np]
+    g, pi = full(0), full(10)
+    phi = torch.stack([torch.stack([torch.stack([u[:, 20 + i + 3 * idx[(a, b)]] for b in range(4)],
+                                                1) for a in range(4)], 1) for i in range(3)], 1)
+    spatial = g[:, 1:, 1:].permute(0, 3, 1, 2)                      # [E][Np][3][3]
+    inv_sp = torch.linalg.inv(spatial)
+    g0i = g[:, 1:, 0].permute(0, 2, 1)                              # [E][Np][3]
+    shift = torch.einsum("epij,epj->epi", inv_sp, g0i)
+    lapse = torch.sqrt(-g[:, 0, 0] + torch.einsum("epi,epi->ep", shift, g0i))
+    n_up = torch.cat([(1.0 / lapse)[..., None], -shift / lapse[..., None]], -1)   # [E][Np][4]
+    m = -1.0 / lapse ** 2
+    top = torch.cat([m[..., None], (-shift * m[..., None])], -1)    # [E][Np][4]
+    lower = inv_sp + shift[..., :, None] * shift[..., None, :] * m[..., None, None]
+    lower = torch.cat([(-shift * m[..., None])[..., None], lower], -1)  # [E][Np][3][4]
+    inv_g = torch.cat([top[..., None, :], lower], -2)               # [E][Np][4][4]
+    pi_p = pi.permute(0, 3, 1, 2)                                   # [E][Np][4][4]
+    phi_p = phi.permute(0, 4, 1, 2, 3)                              # [E][Np][3][4][4]
+    k = -0.5 * (torch.einsum("epbc,epbc->ep", pi_p, inv_g)
+                + torch.einsum("epi,epibc,epbc->ep", n_up[..., 1:], phi_p, inv_g))
+    c = []
+    for a in range(4):
+        term = torch.einsum("epij,epij->ep", inv_sp, phi_p[:, :, :, 1:, a]) \
+            + torch.einsum("epb,epb->ep", n_up, pi_p[:, :, :, a])
+        if a == 0:
+            term = term - lapse * k
+        else:
+            term = term - 0.5 * torch.einsum("epbc,epbc->ep", phi_p[:, :, a - 1], inv_g)
+        c.append(term)
+    return torch.stack(c, 1)                                        # [E][4][Np]
+
+
+def gauge_constraint_norm(u, mass):
+    """J = ( sum over points of M sum_a C_a^2 )^(1/2)."""
+    c = gauge_constraint(u)
+    return torch.sqrt(torch.sum(mass[:, None, :] * c * c))
+
+
 def mass_weights(domain_extents, inv_jac, num_elements):
     """Diagonal LGL mass matrix per point, w_i w_j w_k det(J) [E][Np] (the
     Jacobian determinant as 1 / det of the inverse Jacobian)."""

@@ -468,3 +468,18 @@ def scalar_wave_adjoint_workload(seed=0, elements=(8, 8, 8), extents=(6, 6, 6)):
     Returns (brick, u, gamma2, inv_jac, dt)."""
     brick, u, gamma2, inv = scalar_wave_plane_wave_3d(seed, elements, extents)
     return brick, u, gamma2, inv, cfl_step(brick, 0.25, 1.0)
+
+
+def gh_adjoint_workload(seed=0, elements=(6, 5, 10), extents=(6, 6, 6), noise=1.0e-6,
+                        lower=(3.0, -4.0, 0.5), upper=(11.0, 4.0, 8.5)):
+    """Adjoint GH workload: the Kerr-Schild state of gh_kerr_schild_brick on a
+    uniform brick of Ex x Ey x Ez elements (300 by default), step size at CFL
+    0.25 with the largest characteristic speed of the state (lapse + |shift|
+    bounds it).  The default brick keeps element faces off the plane z = 0,
+    where the shift is tangent to the z faces and the upwind selection of the
+    zero-speed field switches (the RHS is not differentiable there).  Returns
+    (brick, u, gammas, inv_jac, dt)."""
+    brick, u, gammas, inv = gh_kerr_schild_brick(seed, elements, extents, noise, lower, upper)
+    lapse, shift, _ = _lapse_shift_inverse([u[:, p] for p in range(10)])
+    speed = float(np.max(np.abs(lapse) + np.sqrt(np.sum(shift ** 2, axis=0))))
+    return brick, u, gammas, inv, cfl_step(brick, 0.25, speed)

@@ -307,3 +307,15 @@ def gh_time_derivative_vjp(u, du, gammas, gbar):
     _check(lib().sxo_gh_time_derivative_vjp(n, _d(u), _d(du), _d(gammas), _d(gbar), _d(ubar),
                                             _d(dubar)))
     return ubar, dubar
+
+
+def gh_evolve_vjp(extents, neighbors, u0, gammas, inv_jac, dt, steps, gbar_final):
+    """(d u_final / d u0)^T gbar_final through `steps` RK3 steps, by the tape."""
+    u0, gammas, inv_jac, gbar_final = (np.ascontiguousarray(a, dtype=np.float64)
+                                       for a in (u0, gammas, inv_jac, gbar_final))
+    out = np.empty_like(u0)
+    ext, ep = _i(extents)
+    nb, nbp = _i(neighbors)
+    _check(lib().sxo_gh_evolve_vjp(u0.shape[0], ep, nbp, _d(u0), _d(gammas), _d(inv_jac),
+                                   ctypes.c_double(dt), int(steps), _d(gbar_final), _d(out)))
+    return out
