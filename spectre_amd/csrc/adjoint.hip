@@ -219,30 +219,34 @@ rk3_adjoint_substep_kernel(int substep, int64_t n, double dt, const double* lam,
 //     exterior point of the neighbour's face.
 __global__ void __launch_bounds__(128)
 gh_adjoint_pointwise_kernel(int64_t total, int np, const double* __restrict__ u,
-                            double* __restrict__ du_xbar, const double* __restrict__ gammas,
+                            double* du_xbar, const double* __restrict__ gammas,
                             const double* __restrict__ inv_jac, const double* __restrict__ gbar,
                             double* __restrict__ ubar) {
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= total) return;
   const int64_t e = gid / np;
   const int p = static_cast<int>(gid % np);
-  double up[50], dup[150], gp[50], ub[50], dub[150];
-  for (int c = 0; c < 50; ++c) {
-    up[c] = u[(e * 50 + c) * np + p];
-    gp[c] = gbar[(e * 50 + c) * np + p];
-  }
-  for (int c = 0; c < 150; ++c) dup[c] = du_xbar[(e * 150 + c) * np + p];
+  double ub[50];
+  const double* up = u + e * 50 * np + p;
+  const double* dup = du_xbar + e * 150 * np + p;
+  const double* gp = gbar + e * 50 * np + p;
   const double* ga = gammas + e * 3 * np + p;
-  gh_time_derivative_vjp(up, dup, ga[0], ga[np], ga[2 * np], gp, ub, dub);
+  const double gamma1 = ga[np], gamma2 = ga[2 * np];
+  GhVjpPoint pt;
+  gh_time_derivative_vjp(up, np, dup, np, ga[0], gamma1, gamma2, gp, np, ub, &pt);
   for (int c = 0; c < 50; ++c) ubar[(e * 50 + c) * np + p] = ub[c];
+  // every read of du is done: the logical-derivative adjoints go in its place
   double ij[9];
   for (int c = 0; c < 9; ++c) ij[c] = inv_jac[(e * 9 + c) * np + p];
-  for (int c = 0; c < 50; ++c)
+  for (int c = 0; c < 50; ++c) {
+    double dub[3];
+    for (int i = 0; i < 3; ++i) dub[i] = gh_time_derivative_dubar(pt, gamma1, gamma2, gp, np, 3 * c + i);
     for (int j = 0; j < 3; ++j) {
       double s = 0.0;
-      for (int i = 0; i < 3; ++i) s += ij[ij3(j, i)] * dub[3 * c + i];
+      for (int i = 0; i < 3; ++i) s += ij[ij3(j, i)] * dub[i];
       du_xbar[(e * 150 + 3 * c + j) * np + p] = s;
     }
+  }
 }
 
 // Grid: (E, 50 / kSweepChunk); dynamic LDS: 3 Np doubles.
@@ -290,12 +294,10 @@ gh_adjoint_face_kernel(int64_t total, int n0, int n1, int n2, const int32_t* __r
   bool on_face = false;
   for (int d = 0; d < 3; ++d) on_face = on_face || idx[d] == 0 || idx[d] == n[d] - 1;
   if (!on_face) return;
-  double acc[50], up[50], uq[50], gq[50], gp[50], bi[50], bx[50];
-  for (int c = 0; c < 50; ++c) {
-    acc[c] = 0.0;
-    up[c] = u[(e * 50 + c) * np + p];
-    gp[c] = gbar[(e * 50 + c) * np + p];
-  }
+  double acc[50];
+  for (int c = 0; c < 50; ++c) acc[c] = 0.0;
+  const double* up = u + e * 50 * np + p;
+  const double* gp = gbar + e * 50 * np + p;
   const double* gap = gammas + e * 3 * np + p;
   for (int d = 0; d < 3; ++d) {
     for (int s = 0; s < 2; ++s) {
@@ -305,10 +307,8 @@ gh_adjoint_face_kernel(int64_t total, int n0, int n1, int n2, const int32_t* __r
       int qi[3] = {idx[0], idx[1], idx[2]};
       qi[d] = s == 0 ? n[d] - 1 : 0;
       const int q = qi[0] + n0 * (qi[1] + n1 * qi[2]);
-      for (int c = 0; c < 50; ++c) {
-        uq[c] = u[(en * 50 + c) * np + q];
-        gq[c] = gbar[(en * 50 + c) * np + q];
-      }
+      const double* uq = u + en * 50 * np + q;
+      const double* gq = gbar + en * 50 * np + q;
       const double* gaq = gammas + en * 3 * np + q;
       double np_row[3], nq_row[3];
       for (int i = 0; i < 3; ++i) {
@@ -321,18 +321,16 @@ gh_adjoint_face_kernel(int64_t total, int n0, int n1, int n2, const int32_t* __r
         ni[i] = s == 0 ? np_row[i] * -1.0 : np_row[i];
         nx[i] = s == 1 ? nq_row[i] * -1.0 : nq_row[i];
       }
-      for (int c = 0; c < 50; ++c) bi[c] = bx[c] = 0.0;
-      gh_face_vjp(up, gap[np], gap[2 * np], ni, uq, gaq[np], gaq[2 * np], nx, n[d], gp, bi, bx);
-      for (int c = 0; c < 50; ++c) acc[c] += bi[c];
+      gh_face_vjp(up, gap[np], gap[2 * np], ni, uq, gaq[np], gaq[2 * np], nx, n[d], np, gp, np,
+                  acc, nullptr);
       // exterior term of the neighbour's face (d, 1 - s), whose interior
       // point is q
       for (int i = 0; i < 3; ++i) {
         ni[i] = (1 - s) == 0 ? nq_row[i] * -1.0 : nq_row[i];
         nx[i] = (1 - s) == 1 ? np_row[i] * -1.0 : np_row[i];
       }
-      for (int c = 0; c < 50; ++c) bi[c] = bx[c] = 0.0;
-      gh_face_vjp(uq, gaq[np], gaq[2 * np], ni, up, gap[np], gap[2 * np], nx, n[d], gq, bi, bx);
-      for (int c = 0; c < 50; ++c) acc[c] += bx[c];
+      gh_face_vjp(uq, gaq[np], gaq[2 * np], ni, up, gap[np], gap[2 * np], nx, n[d], np, gq, np,
+                  nullptr, acc);
     }
   }
   for (int c = 0; c < 50; ++c) ubar[(e * 50 + c) * np + p] += acc[c];

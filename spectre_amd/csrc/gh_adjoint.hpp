@@ -23,6 +23,16 @@ namespace sxd {
 
 #define SXD_HD __host__ __device__ inline
 
+// Makes a pointer's value opaque to the optimiser: loads through it after
+// this point are not merged with loads made before it.  A long point-wise
+// adjoint re-reads its inputs (from cache) in each phase instead of keeping
+// every value it read alive in registers across phases.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SXD_REREAD(p) asm volatile("" : "+v"(p))
+#else
+#define SXD_REREAD(p) asm volatile("" : "+r"(p))
+#endif
+
 namespace adj {
 SXD_HD int iaa(int i, int a, int b) { return i + 3 * sym4(a, b); }
 SXD_HD int ijaa(int i, int j, int a, int b) { return i + 3 * j + 9 * sym4(a, b); }
@@ -100,354 +110,503 @@ SXD_HD void lapse_shift_vjp(const LapseShift& L, double isp_bar[6], double shift
 }
 }  // namespace adj
 
-// Adjoint of the point-wise GH time derivative dt = f(u, du; gamma0..2):
-// ubar[50] and dubar[150] are OVERWRITTEN with (df/du)^T gbar and
-// (df/d du)^T gbar.  Layouts as gh_time_derivative: u = (g, Pi, Phi), du =
-// (d g at 3 p + i, d Pi at 30 + 3 p + i, d Phi at 60 + i + 3 j + 9 p).
-SXD_HD void gh_time_derivative_vjp(const double* u, const double* du, double gamma0,
-                                   double gamma1, double gamma2, const double* gbar,
-                                   double* ubar, double* dubar) {
+// Scalars of the linearisation point that the adjoint of the derivative
+// inputs needs (see gh_time_derivative_dubar).
+struct GhVjpPoint {
+  double lapse, shift[3], isp[6];
+};
+
+namespace adj {
+// 4x4 matrices: full row-major M[4 a + b]; symmetric ones in slots s[sym4(a, b)].
+SXD_HD void full4(const double* s, double* M) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) M[4 * a + b] = s[sym4(a, b)];
+}
+// C = A B
+SXD_HD void mul4(const double* A, const double* B, double* C) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) s += A[4 * a + k] * B[4 * k + b];
+      C[4 * a + b] = s;
+    }
+}
+// tr(A B)
+SXD_HD double trace4(const double* A, const double* B) {
+  double s = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) s += A[4 * a + b] * B[4 * b + a];
+  return s;
+}
+// Gradient G with respect to the full matrix of a symmetric tensor, mapped to
+// its slots: an off-diagonal slot stands for both (a, b) and (b, a).
+SXD_HD void add_slots(double* s_b, const double* G, double scale) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = a; b < 4; ++b)
+      s_b[sym4(a, b)] += scale * (a == b ? G[5 * a] : G[4 * a + b] + G[4 * b + a]);
+}
+}  // namespace adj
+
+// Adjoint of the point-wise GH time derivative dt = f(u, du; gamma0..2) with
+// respect to u: ubar[50] is OVERWRITTEN with (df/du)^T gbar.  u, du and gbar
+// are read through strides (su, sdu, sg: the distance between components, 1
+// for packed arrays or Np for a Variables block in memory).  Layouts as
+// gh_time_derivative: u = (g, Pi, Phi), du = (d g at 3 p + i, d Pi at
+// 30 + 3 p + i, d Phi at 60 + i + 3 j + 9 p).  The adjoint with respect to du
+// (f is affine in du) is gh_time_derivative_dubar(pt, ...), from the
+// scalars returned in *pt.
+//
+// The quadratic terms of dt Pi are written as traces of 4x4 products so that
+// their adjoints come in closed form and few intermediates are alive at once:
+// with P_n = Phi(n, ., .), Psi_n = sum_m isp(n, m) P_m, C_m = Gamma(m, ., .)
+// (Christoffel symbols of the first kind), ig the inverse spacetime metric,
+// and H the symmetric weight matrix of the dt Pi adjoint (diagonal = the slot
+// weight, off-diagonal entries = half of it, so that sum over slots of
+// w R = sum over all entries of H R):
+//   -2 sum_de Pi(mu, de) (Pi ig)(nu, de)           = -2 (Pi ig Pi)(mu, nu)
+//    2 sum_{de, n} Psi_n(mu, de) (P_n ig)(nu, de)  =  2 sum_n (Psi_n ig P_n)(mu, nu)
+//   -2 sum_{al, de} (C_mu ig)(al, de) (C_nu ig)(de, al) = -2 tr(C_mu ig C_nu ig)
+// each symmetric in (mu, nu).
+template <int kPart = 0>
+SXD_HD void gh_time_derivative_vjp(const double* u, int su, const double* du, int sdu,
+                                   double gamma0, double gamma1, double gamma2,
+                                   const double* gbar, int sg, double* ubar, GhVjpPoint* pt) {
   using namespace adj;
-  const double* g = u;
-  const double* pi = u + 10;
-  const double* phi = u + 20;
-  double* g_b = ubar;
-  double* pi_b = ubar + 10;
-  double* phi_b = ubar + 20;
-  for (int c = 0; c < 50; ++c) ubar[c] = 0.0;
-  for (int c = 0; c < 150; ++c) dubar[c] = 0.0;
-#define PI_(a, b) pi[sym4(a, b)]
-#define PHI_(i, a, b) phi[iaa(i, a, b)]
-#define DG_(i, a, b) du[3 * sym4(a, b) + (i)]
-#define DPI_(i, a, b) du[30 + 3 * sym4(a, b) + (i)]
-#define DPHI_(i, j, a, b) du[60 + ijaa(i, j, a, b)]
-  // ---- forward ------------------------------------------------------------
+  // kPart selects a share of the terms (0 = all; 1 = Christoffel terms;
+  // 2 = the remaining terms of dt Pi; 3 = dt Phi and the C3 terms): the
+  // adjoint is linear in gbar, so the shares add up to the whole.
+  constexpr bool kChr = kPart == 0 || kPart == 1;
+  constexpr bool kR = kPart == 0 || kPart == 2;
+  constexpr bool kS = kPart == 0 || kPart == 3;
+#define G_(a, b) u[sym4(a, b) * su]
+#define PI_(a, b) u[(10 + sym4(a, b)) * su]
+#define PHI_(i, a, b) u[(20 + iaa(i, a, b)) * su]
+#define DG_(i, a, b) du[(3 * sym4(a, b) + (i)) * sdu]
+#define DPI_(i, a, b) du[(30 + 3 * sym4(a, b) + (i)) * sdu]
+#define DPHI_(i, j, a, b) du[(60 + ijaa(i, j, a, b)) * sdu]
+#define GB_(c) gbar[(c) * sg]
+  // ---- metric quantities -------------------------------------------------
   LapseShift L;
-  lapse_shift(L, g);
+  {
+    double g[10];
+#pragma unroll
+    for (int c = 0; c < 10; ++c) g[c] = u[c * su];
+    lapse_shift(L, g);
+  }
   const double lapse = L.lapse;
   const double* shift = L.shift;
   const double* isp = L.isp;
+  pt->lapse = lapse;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) pt->shift[i] = shift[i];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) pt->isp[i] = isp[i];
   double ig[10];
   const double m1ol2 = -1.0 / (lapse * lapse);
   ig[sym4(0, 0)] = m1ol2;
+#pragma unroll
   for (int i = 0; i < 3; ++i) ig[sym4(0, i + 1)] = -shift[i] * m1ol2;
+#pragma unroll
   for (int i = 0; i < 3; ++i)
+#pragma unroll
     for (int j = i; j < 3; ++j) ig[sym4(i + 1, j + 1)] = isp[sym3(i, j)] + shift[i] * shift[j] * m1ol2;
-  double dtg[10];
-  for (int a = 0; a < 4; ++a)
-    for (int b = a; b < 4; ++b) {
-      double s = -lapse * PI_(a, b);
-      for (int m = 0; m < 3; ++m) s += shift[m] * PHI_(m, a, b);
-      dtg[sym4(a, b)] = s;
-    }
-  auto DA = [&](int a, int b, int c) { return a == 0 ? dtg[sym4(b, c)] : PHI_(a - 1, b, c); };
-  double chr[40];
-  for (int k = 0; k < 4; ++k)
-    for (int i = 0; i < 4; ++i)
-      for (int j = i; j < 4; ++j) chr[abb(k, i, j)] = 0.5 * (DA(i, j, k) + DA(j, i, k) - DA(k, i, j));
-  double tc[4];
-  for (int i = 0; i < 4; ++i) {
-    double s = 0.0;
-    for (int j = 0; j < 4; ++j)
-      for (int k = j; k < 4; ++k) s += (j == k ? 1.0 : 2.0) * chr[abb(i, j, k)] * ig[sym4(j, k)];
-    tc[i] = s;
-  }
   double nv[4];
   nv[0] = 1.0 / lapse;
+#pragma unroll
   for (int i = 0; i < 3; ++i) nv[i + 1] = -shift[i] * nv[0];
-  const double gamma12 = gamma1 * gamma2;
-  double p1u[30];
-  for (int m = 0; m < 3; ++m)
-    for (int a = 0; a < 4; ++a)
-      for (int b = a; b < 4; ++b) {
-        double s = 0.0;
-        for (int n = 0; n < 3; ++n) s += isp[sym3(m, n)] * PHI_(n, a, b);
-        p1u[iaa(m, a, b)] = s;
-      }
-  double p3u[48];  // p3u[m + 3 (nu + 4 al)]
-  for (int m = 0; m < 3; ++m)
-    for (int nu = 0; nu < 4; ++nu)
-      for (int al = 0; al < 4; ++al) {
-        double s = 0.0;
-        for (int be = 0; be < 4; ++be) s += ig[sym4(al, be)] * PHI_(m, nu, be);
-        p3u[m + 3 * (nu + 4 * al)] = s;
-      }
-  double p2u[16];  // p2u[nu + 4 al]
-  for (int nu = 0; nu < 4; ++nu)
-    for (int al = 0; al < 4; ++al) {
-      double s = 0.0;
-      for (int be = 0; be < 4; ++be) s += ig[sym4(al, be)] * PI_(nu, be);
-      p2u[nu + 4 * al] = s;
-    }
-  double c3u[64];  // c3u[mu + 4 (nu + 4 al)] = sum_be ig(al, be) chr(mu, nu, be)
-  for (int mu = 0; mu < 4; ++mu)
-    for (int nu = 0; nu < 4; ++nu)
-      for (int al = 0; al < 4; ++al) {
-        double s = 0.0;
-        for (int be = 0; be < 4; ++be) s += ig[sym4(al, be)] * chr[abb(mu, nu, be)];
-        c3u[mu + 4 * (nu + 4 * al)] = s;
-      }
-  double pion[4];
-  for (int mu = 0; mu < 4; ++mu) {
-    double s = 0.0;
-    for (int nu = 0; nu < 4; ++nu) s += nv[nu] * PI_(nu, mu);
-    pion[mu] = s;
-  }
-  double hptn = 0.0;
-  for (int mu = 0; mu < 4; ++mu) hptn += nv[mu] * pion[mu];
-  hptn *= 0.5;
-  double pon[12];  // pon[n + 3 nu] = sum_mu nv[mu] Phi(n, mu, nu)
-  for (int n = 0; n < 3; ++n)
-    for (int nu = 0; nu < 4; ++nu) {
-      double s = 0.0;
-      for (int mu = 0; mu < 4; ++mu) s += nv[mu] * PHI_(n, mu, nu);
-      pon[n + 3 * nu] = s;
-    }
-  double hphi[3];
-  for (int n = 0; n < 3; ++n) {
-    double s = 0.0;
-    for (int mu = 0; mu < 4; ++mu) s += nv[mu] * pon[n + 3 * mu];
-    hphi[n] = 0.5 * s;
-  }
-  double c3[30];
-  for (int n = 0; n < 3; ++n)
-    for (int a = 0; a < 4; ++a)
-      for (int b = a; b < 4; ++b) c3[iaa(n, a, b)] = DG_(n, a, b) - PHI_(n, a, b);
-  double sdc3[10];
-  for (int a = 0; a < 4; ++a)
-    for (int b = a; b < 4; ++b) {
-      double s = 0.0;
-      for (int m = 0; m < 3; ++m) s += shift[m] * c3[iaa(m, a, b)];
-      sdc3[sym4(a, b)] = s;
-    }
-  double ndgc_raw = 0.0;
-  for (int mu = 0; mu < 4; ++mu) ndgc_raw += nv[mu] * tc[mu];
-  const double ndgc = ndgc_raw * gamma0;
-  const double tmp00 = -gamma0 * lapse;
+  double igf[16];
+  full4(ig, igf);
 
-  // ---- reverse ------------------------------------------------------------
+  double* g_b = ubar;
+  double* pi_b = ubar + 10;
+  double* phi_b = ubar + 20;
+#pragma unroll
+  for (int c = 0; c < 50; ++c) ubar[c] = 0.0;
   double lapse_b = 0.0, shift_b[3] = {0.0, 0.0, 0.0}, isp_b[6] = {0, 0, 0, 0, 0, 0};
   double ig_b[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, dtg_b[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  double chr_b[40], tc_b[4] = {0, 0, 0, 0}, nv_b[4] = {0, 0, 0, 0};
-  double p1u_b[30], p3u_b[48], p2u_b[16], c3u_b[64], pion_b[4] = {0, 0, 0, 0}, hptn_b = 0.0;
-  double pon_b[12], hphi_b[3] = {0, 0, 0}, c3_b[30], sdc3_b[10], ndgc_b = 0.0, tmp00_b = 0.0;
-  for (int x = 0; x < 40; ++x) chr_b[x] = 0.0;
-  for (int x = 0; x < 30; ++x) p1u_b[x] = c3_b[x] = 0.0;
-  for (int x = 0; x < 48; ++x) p3u_b[x] = 0.0;
-  for (int x = 0; x < 16; ++x) p2u_b[x] = 0.0;
-  for (int x = 0; x < 64; ++x) c3u_b[x] = 0.0;
-  for (int x = 0; x < 12; ++x) pon_b[x] = 0.0;
-  for (int x = 0; x < 10; ++x) sdc3_b[x] = 0.0;
+  double nv_b[4] = {0, 0, 0, 0};
 
-  // dt Phi(i, mu nu) = S lapse + sum_m shift_m dPhi(m, i, mu nu),
-  // S = Pi(mu nu) hphi_i - dPi(i, mu nu) + gamma2 C3(i, mu nu)
-  //     + sum_n pon[i + 3 (n + 1)] P1U(n, mu nu)
-  for (int i = 0; i < 3; ++i)
-    for (int a = 0; a < 4; ++a)
-      for (int b = a; b < 4; ++b) {
-        const double w = gbar[20 + iaa(i, a, b)];
-        double S = PI_(a, b) * hphi[i] - DPI_(i, a, b) + gamma2 * c3[iaa(i, a, b)];
-        for (int n = 0; n < 3; ++n) S += pon[i + 3 * (n + 1)] * p1u[iaa(n, a, b)];
-        lapse_b += w * S;
-        for (int m = 0; m < 3; ++m) {
-          shift_b[m] += w * DPHI_(m, i, a, b);
-          dubar[60 + ijaa(m, i, a, b)] += w * shift[m];
-        }
-        const double Sb = w * lapse;
-        pi_b[sym4(a, b)] += Sb * hphi[i];
-        hphi_b[i] += Sb * PI_(a, b);
-        dubar[30 + 3 * sym4(a, b) + i] -= Sb;
-        c3_b[iaa(i, a, b)] += gamma2 * Sb;
-        for (int n = 0; n < 3; ++n) {
-          pon_b[i + 3 * (n + 1)] += Sb * p1u[iaa(n, a, b)];
-          p1u_b[iaa(n, a, b)] += Sb * pon[i + 3 * (n + 1)];
-        }
-      }
-  // dt Pi(mu nu) = R lapse + gamma12 sdc3(mu nu) + sum_m shift_m dPi(m, mu nu)
-  for (int mu = 0; mu < 4; ++mu)
-    for (int nu = mu; nu < 4; ++nu) {
-      const double w = gbar[10 + sym4(mu, nu)];
-      double base;
-      if (mu == 0 && nu == 0)
-        base = 2.0 * tmp00 * tc[0] - ndgc * g[sym4(0, 0)];
-      else if (mu == 0)
-        base = tmp00 * tc[nu] - ndgc * g[sym4(0, nu)];
-      else
-        base = -ndgc * g[sym4(mu, nu)];
-      double R = base - hptn * PI_(mu, nu);
-      for (int de = 0; de < 4; ++de) {
-        R += -2.0 * PI_(mu, de) * p2u[nu + 4 * de];
-        for (int n = 0; n < 3; ++n) R += 2.0 * p1u[iaa(n, mu, de)] * p3u[n + 3 * (nu + 4 * de)];
-        for (int al = 0; al < 4; ++al)
-          R += -2.0 * c3u[mu + 4 * (al + 4 * de)] * c3u[nu + 4 * (de + 4 * al)];
-      }
-      for (int m = 0; m < 3; ++m) {
-        R += -pion[m + 1] * p1u[iaa(m, mu, nu)];
-        for (int n = 0; n < 3; ++n) R += -isp[sym3(m, n)] * DPHI_(m, n, mu, nu);
-      }
-      lapse_b += w * R;
-      sdc3_b[sym4(mu, nu)] += gamma12 * w;
-      for (int m = 0; m < 3; ++m) {
-        shift_b[m] += w * DPI_(m, mu, nu);
-        dubar[30 + 3 * sym4(mu, nu) + m] += w * shift[m];
-      }
-      const double Rb = w * lapse;
-      // base
-      if (mu == 0 && nu == 0) {
-        tmp00_b += 2.0 * Rb * tc[0];
-        tc_b[0] += 2.0 * tmp00 * Rb;
-      } else if (mu == 0) {
-        tmp00_b += Rb * tc[nu];
-        tc_b[nu] += tmp00 * Rb;
-      }
-      ndgc_b -= Rb * g[sym4(mu, nu)];
-      g_b[sym4(mu, nu)] -= Rb * ndgc;
-      hptn_b -= Rb * PI_(mu, nu);
-      pi_b[sym4(mu, nu)] -= Rb * hptn;
-      for (int de = 0; de < 4; ++de) {
-        pi_b[sym4(mu, de)] += -2.0 * Rb * p2u[nu + 4 * de];
-        p2u_b[nu + 4 * de] += -2.0 * Rb * PI_(mu, de);
-        for (int n = 0; n < 3; ++n) {
-          p1u_b[iaa(n, mu, de)] += 2.0 * Rb * p3u[n + 3 * (nu + 4 * de)];
-          p3u_b[n + 3 * (nu + 4 * de)] += 2.0 * Rb * p1u[iaa(n, mu, de)];
-        }
-        for (int al = 0; al < 4; ++al) {
-          c3u_b[mu + 4 * (al + 4 * de)] += -2.0 * Rb * c3u[nu + 4 * (de + 4 * al)];
-          c3u_b[nu + 4 * (de + 4 * al)] += -2.0 * Rb * c3u[mu + 4 * (al + 4 * de)];
-        }
-      }
-      for (int m = 0; m < 3; ++m) {
-        pion_b[m + 1] -= Rb * p1u[iaa(m, mu, nu)];
-        p1u_b[iaa(m, mu, nu)] -= Rb * pion[m + 1];
-        for (int n = 0; n < 3; ++n) {
-          isp_b[sym3(m, n)] -= Rb * DPHI_(m, n, mu, nu);
-          dubar[60 + ijaa(m, n, mu, nu)] -= Rb * isp[sym3(m, n)];
-        }
-      }
-    }
-  // tmp00 = -gamma0 lapse; ndgc = gamma0 ndgc_raw
-  lapse_b += -gamma0 * tmp00_b;
-  const double ndgc_raw_b = gamma0 * ndgc_b;
-  // dt g(mu nu) = dtg + (1 + gamma1) sdc3
-  for (int x = 0; x < 10; ++x) {
-    dtg_b[x] += gbar[x];
-    sdc3_b[x] += (1.0 + gamma1) * gbar[x];
-  }
-  // ndgc_raw = sum nv[mu] tc[mu]
-  for (int mu = 0; mu < 4; ++mu) {
-    nv_b[mu] += ndgc_raw_b * tc[mu];
-    tc_b[mu] += ndgc_raw_b * nv[mu];
-  }
-  // sdc3 = sum_m shift_m c3(m)
+  // weight matrix H of the dt Pi adjoint (no lapse factor)
+  double H[16];
+#pragma unroll
   for (int a = 0; a < 4; ++a)
-    for (int b = a; b < 4; ++b)
-      for (int m = 0; m < 3; ++m) {
-        shift_b[m] += sdc3_b[sym4(a, b)] * c3[iaa(m, a, b)];
-        c3_b[iaa(m, a, b)] += sdc3_b[sym4(a, b)] * shift[m];
-      }
-  // c3 = dg - Phi
-  for (int n = 0; n < 3; ++n)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) H[4 * a + b] = (a == b ? 1.0 : 0.5) * GB_(10 + sym4(a, b));
+
+  SXD_REREAD(u);
+  SXD_REREAD(du);
+  SXD_REREAD(gbar);
+  // ---- Christoffel part: tc, the harmonic-gauge term, -2 tr(C ig C ig) -------
+  if (kChr) {
+    double dtg[10];
+#pragma unroll
     for (int a = 0; a < 4; ++a)
+#pragma unroll
       for (int b = a; b < 4; ++b) {
-        dubar[3 * sym4(a, b) + n] += c3_b[iaa(n, a, b)];
-        phi_b[iaa(n, a, b)] -= c3_b[iaa(n, a, b)];
+        double s = -lapse * PI_(a, b);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) s += shift[m] * PHI_(m, a, b);
+        dtg[sym4(a, b)] = s;
       }
-  // hphi[n] = 0.5 sum_mu nv[mu] pon[n + 3 mu]
-  for (int n = 0; n < 3; ++n)
+    auto DA = [&](int a, int b, int c) { return a == 0 ? dtg[sym4(b, c)] : PHI_(a - 1, b, c); };
+    double chr[40];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = i; j < 4; ++j) chr[abb(k, i, j)] = 0.5 * (DA(i, j, k) + DA(j, i, k) - DA(k, i, j));
+    double tc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int k = j; k < 4; ++k) s += (j == k ? 1.0 : 2.0) * chr[abb(i, j, k)] * ig[sym4(j, k)];
+      tc[i] = s;
+    }
+    double ndgc_raw = 0.0;
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) ndgc_raw += nv[mu] * tc[mu];
+    const double ndgc = ndgc_raw * gamma0;
+    const double tmp00 = -gamma0 * lapse;
+
+    double chr_b[40];
+#pragma unroll
+    for (int x = 0; x < 40; ++x) chr_b[x] = 0.0;
+    // -2 sum H(mu, nu) tr(C_mu ig C_nu ig): per mu, K = sum_nu H(mu, nu) C_nu,
+    // A = C_mu ig, B = K ig; d/dC_mu = -4 ig K ig, d/dig = -4 sum_mu A K.
+    double e3 = 0.0;
+#pragma unroll
     for (int mu = 0; mu < 4; ++mu) {
-      nv_b[mu] += 0.5 * hphi_b[n] * pon[n + 3 * mu];
-      pon_b[n + 3 * mu] += 0.5 * hphi_b[n] * nv[mu];
+      double Cf[16], K[16], A[16], B[16], T[16];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          Cf[4 * a + b] = chr[abb(mu, a, b)];
+          double s = 0.0;
+#pragma unroll
+          for (int nu = 0; nu < 4; ++nu) s += H[4 * mu + nu] * chr[abb(nu, a, b)];
+          K[4 * a + b] = s;
+        }
+      mul4(Cf, igf, A);
+      mul4(K, igf, B);
+      e3 += trace4(A, B);
+      mul4(igf, B, T);  // ig K ig
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = a; b < 4; ++b)
+          chr_b[abb(mu, a, b)] += -4.0 * lapse * (a == b ? T[5 * a] : T[4 * a + b] + T[4 * b + a]);
+      mul4(A, K, T);
+      add_slots(ig_b, T, -4.0 * lapse);
     }
-  // pon[n + 3 nu] = sum_mu nv[mu] Phi(n, mu, nu)
-  for (int n = 0; n < 3; ++n)
-    for (int nu = 0; nu < 4; ++nu)
-      for (int mu = 0; mu < 4; ++mu) {
-        nv_b[mu] += pon_b[n + 3 * nu] * PHI_(n, mu, nu);
-        phi_b[iaa(n, mu, nu)] += pon_b[n + 3 * nu] * nv[mu];
+    lapse_b += -2.0 * e3;
+    // base terms of dt Pi (upper slots) with weights Rb = lapse gbar_Pi
+    double tmp00_b = 0.0, ndgc_b = 0.0, tc_b[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+      for (int nu = mu; nu < 4; ++nu) {
+        const double w = GB_(10 + sym4(mu, nu));
+        double base;
+        if (mu == 0 && nu == 0)
+          base = 2.0 * tmp00 * tc[0] - ndgc * G_(0, 0);
+        else if (mu == 0)
+          base = tmp00 * tc[nu] - ndgc * G_(0, nu);
+        else
+          base = -ndgc * G_(mu, nu);
+        lapse_b += w * base;
+        const double Rb = w * lapse;
+        if (mu == 0 && nu == 0) {
+          tmp00_b += 2.0 * Rb * tc[0];
+          tc_b[0] += 2.0 * tmp00 * Rb;
+        } else if (mu == 0) {
+          tmp00_b += Rb * tc[nu];
+          tc_b[nu] += tmp00 * Rb;
+        }
+        ndgc_b -= Rb * G_(mu, nu);
+        g_b[sym4(mu, nu)] -= Rb * ndgc;
       }
-  // hptn = 0.5 sum nv[mu] pion[mu]
-  for (int mu = 0; mu < 4; ++mu) {
-    nv_b[mu] += 0.5 * hptn_b * pion[mu];
-    pion_b[mu] += 0.5 * hptn_b * nv[mu];
+    lapse_b += -gamma0 * tmp00_b;
+    const double ndgc_raw_b = gamma0 * ndgc_b;
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+      nv_b[mu] += ndgc_raw_b * tc[mu];
+      tc_b[mu] += ndgc_raw_b * nv[mu];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int k = j; k < 4; ++k) {
+          const double f = (j == k ? 1.0 : 2.0) * tc_b[i];
+          chr_b[abb(i, j, k)] += f * ig[sym4(j, k)];
+          ig_b[sym4(j, k)] += f * chr[abb(i, j, k)];
+        }
+    // chr(k, i, j) = 0.5 (DA(i, j, k) + DA(j, i, k) - DA(k, i, j))
+    auto add_da = [&](int a, int b, int c, double v) {
+      if (a == 0)
+        dtg_b[sym4(b, c)] += v;
+      else
+        phi_b[iaa(a - 1, b, c)] += v;
+    };
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = i; j < 4; ++j) {
+          const double h = 0.5 * chr_b[abb(k, i, j)];
+          add_da(i, j, k, h);
+          add_da(j, i, k, h);
+          add_da(k, i, j, -h);
+        }
   }
-  // pion[mu] = sum_nu nv[nu] Pi(nu, mu)
-  for (int mu = 0; mu < 4; ++mu)
-    for (int nu = 0; nu < 4; ++nu) {
-      nv_b[nu] += pion_b[mu] * PI_(nu, mu);
-      pi_b[sym4(nu, mu)] += pion_b[mu] * nv[nu];
+
+  // ---- Pi / Phi quadratic terms, and dt Phi -----------------------------------
+  if (kR || kS) {
+    SXD_REREAD(u);
+    SXD_REREAD(du);
+    SXD_REREAD(gbar);
+    double pion[4];
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+      double s = 0.0;
+#pragma unroll
+      for (int nu = 0; nu < 4; ++nu) s += nv[nu] * PI_(nu, mu);
+      pion[mu] = s;
     }
-  // c3u, p2u, p3u, p1u
-  for (int mu = 0; mu < 4; ++mu)
-    for (int nu = 0; nu < 4; ++nu)
-      for (int al = 0; al < 4; ++al) {
-        const double w = c3u_b[mu + 4 * (nu + 4 * al)];
-        for (int be = 0; be < 4; ++be) {
-          ig_b[sym4(al, be)] += w * chr[abb(mu, nu, be)];
-          chr_b[abb(mu, nu, be)] += w * ig[sym4(al, be)];
-        }
+    double hptn = 0.0;
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) hptn += nv[mu] * pion[mu];
+    hptn *= 0.5;
+    double pon[12];  // pon[n + 3 nu] = sum_mu nv[mu] Phi(n, mu, nu)
+#pragma unroll
+    for (int n = 0; n < 3; ++n)
+#pragma unroll
+      for (int nu = 0; nu < 4; ++nu) {
+        double s = 0.0;
+#pragma unroll
+        for (int mu = 0; mu < 4; ++mu) s += nv[mu] * PHI_(n, mu, nu);
+        pon[n + 3 * nu] = s;
       }
-  for (int nu = 0; nu < 4; ++nu)
-    for (int al = 0; al < 4; ++al) {
-      const double w = p2u_b[nu + 4 * al];
-      for (int be = 0; be < 4; ++be) {
-        ig_b[sym4(al, be)] += w * PI_(nu, be);
-        pi_b[sym4(nu, be)] += w * ig[sym4(al, be)];
+    double hphi[3];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      double s = 0.0;
+#pragma unroll
+      for (int mu = 0; mu < 4; ++mu) s += nv[mu] * pon[n + 3 * mu];
+      hphi[n] = 0.5 * s;
+    }
+    double pion_b[4] = {0, 0, 0, 0}, hptn_b = 0.0, pon_b[12], hphi_b[3] = {0, 0, 0};
+#pragma unroll
+    for (int x = 0; x < 12; ++x) pon_b[x] = 0.0;
+    // psi(n, ab) = sum_m isp(n, m) Phi(m, ab), formed where it is used, and
+    // its adjoint pushed to isp and Phi at once
+    auto psi_at = [&](int n, int a, int b) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) s += isp[sym3(n, m)] * PHI_(m, a, b);
+      return s;
+    };
+    auto psi_adjoint = [&](int n, int a, int b, double w) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        isp_b[sym3(n, m)] += w * PHI_(m, a, b);
+        phi_b[iaa(m, a, b)] += w * isp[sym3(n, m)];
+      }
+    };
+    if (kR) {
+      SXD_REREAD(u);
+      // -2 (Pi ig Pi): d/dPi = -4 H Pi ig, d/dig = -2 Pi H Pi
+      double Pf[16], Q[16], T[16], U[16];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) Pf[4 * a + b] = PI_(a, b);
+      mul4(Pf, igf, Q);  // Pi ig
+      mul4(H, Q, T);     // H Pi ig
+      lapse_b += -2.0 * trace4(T, Pf);
+      add_slots(pi_b, T, -4.0 * lapse);
+      mul4(Pf, H, T);
+      mul4(T, Pf, U);
+      add_slots(ig_b, U, -2.0 * lapse);
+#pragma unroll
+      for (int n = 0; n < 3; ++n) {
+        SXD_REREAD(u);
+        // 2 tr(H Psi_n ig P_n): d/dPsi_n = 2 H P_n ig, d/dig = 2 Psi_n H P_n,
+        // d/dP_n = 2 ig Psi_n H
+        double Sf[16], Pn[16];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = a; b < 4; ++b) {
+            Sf[4 * a + b] = Sf[4 * b + a] = psi_at(n, a, b);
+            Pn[4 * a + b] = Pn[4 * b + a] = PHI_(n, a, b);
+          }
+        mul4(H, Pn, T);   // H P_n
+        mul4(T, igf, U);  // H P_n ig
+        lapse_b += 2.0 * trace4(U, Sf);
+        double sb[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        add_slots(sb, U, 2.0 * lapse);
+        mul4(Sf, T, Q);  // Psi_n H P_n
+        add_slots(ig_b, Q, 2.0 * lapse);
+        mul4(igf, Sf, Q);  // ig Psi_n
+        mul4(Q, H, T);     // ig Psi_n H
+        double pb[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        add_slots(pb, T, 2.0 * lapse);
+        SXD_REREAD(u);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = a; b < 4; ++b) {
+            phi_b[iaa(n, a, b)] += pb[sym4(a, b)];
+            psi_adjoint(n, a, b, sb[sym4(a, b)]);
+          }
       }
     }
-  for (int m = 0; m < 3; ++m)
-    for (int nu = 0; nu < 4; ++nu)
-      for (int al = 0; al < 4; ++al) {
-        const double w = p3u_b[m + 3 * (nu + 4 * al)];
-        for (int be = 0; be < 4; ++be) {
-          ig_b[sym4(al, be)] += w * PHI_(m, nu, be);
-          phi_b[iaa(m, nu, be)] += w * ig[sym4(al, be)];
-        }
-      }
-  for (int m = 0; m < 3; ++m)
+    // slot by slot: the remaining terms of dt Pi (-hptn Pi
+    // - sum_m pion(m+1) Psi_m - sum_{m,n} isp(m, n) dPhi(m, n)
+    // + sum_m shift_m dPi(m)), dt Phi, and the C3 terms
+    //   dt Phi(i, ab) = S lapse + sum_m shift_m dPhi(m, i, ab),
+    //   S = Pi(ab) hphi_i - dPi(i, ab) + gamma2 C3(i, ab)
+    //       + sum_n pon[i + 3 (n + 1)] Psi(n, ab),  C3 = dg - Phi;
+    //   dt g = dtg + (1 + gamma1) sdc3, dt Pi += gamma12 sdc3,
+    //   sdc3 = sum_m shift_m C3(m) (the dg part of C3's adjoint: dubar)
+    const double gamma12 = gamma1 * gamma2;
+#pragma unroll
     for (int a = 0; a < 4; ++a)
+#pragma unroll
       for (int b = a; b < 4; ++b) {
-        const double w = p1u_b[iaa(m, a, b)];
-        for (int n = 0; n < 3; ++n) {
-          isp_b[sym3(m, n)] += w * PHI_(n, a, b);
-          phi_b[iaa(n, a, b)] += w * isp[sym3(m, n)];
+        SXD_REREAD(u);
+        SXD_REREAD(du);
+        SXD_REREAD(gbar);
+        double ps[3], psb[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int n = 0; n < 3; ++n) ps[n] = psi_at(n, a, b);
+        const double pi_ab = PI_(a, b);
+        if (kR) {
+          const double w = GB_(10 + sym4(a, b));
+          double R = -hptn * pi_ab;
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            R += -pion[m + 1] * ps[m];
+#pragma unroll
+            for (int n = 0; n < 3; ++n) R += -isp[sym3(m, n)] * DPHI_(m, n, a, b);
+          }
+          lapse_b += w * R;
+#pragma unroll
+          for (int m = 0; m < 3; ++m) shift_b[m] += w * DPI_(m, a, b);
+          const double Rb = w * lapse;
+          hptn_b -= Rb * pi_ab;
+          pi_b[sym4(a, b)] -= Rb * hptn;
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            pion_b[m + 1] -= Rb * ps[m];
+            psb[m] -= Rb * pion[m + 1];
+#pragma unroll
+            for (int n = 0; n < 3; ++n) isp_b[sym3(m, n)] -= Rb * DPHI_(m, n, a, b);
+          }
         }
+        if (kS) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const double w = GB_(20 + iaa(i, a, b));
+            double S = pi_ab * hphi[i] - DPI_(i, a, b) + gamma2 * (DG_(i, a, b) - PHI_(i, a, b));
+#pragma unroll
+            for (int n = 0; n < 3; ++n) S += pon[i + 3 * (n + 1)] * ps[n];
+            lapse_b += w * S;
+#pragma unroll
+            for (int m = 0; m < 3; ++m) shift_b[m] += w * DPHI_(m, i, a, b);
+            const double Sb = w * lapse;
+            pi_b[sym4(a, b)] += Sb * hphi[i];
+            hphi_b[i] += Sb * pi_ab;
+            phi_b[iaa(i, a, b)] -= gamma2 * Sb;
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+              pon_b[i + 3 * (n + 1)] += Sb * ps[n];
+              psb[n] += Sb * pon[i + 3 * (n + 1)];
+            }
+          }
+          const double wg = GB_(sym4(a, b));
+          dtg_b[sym4(a, b)] += wg;
+          const double sb = gamma12 * GB_(10 + sym4(a, b)) + (1.0 + gamma1) * wg;
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            shift_b[m] += sb * (DG_(m, a, b) - PHI_(m, a, b));
+            phi_b[iaa(m, a, b)] -= sb * shift[m];
+          }
+        }
+#pragma unroll
+        for (int n = 0; n < 3; ++n) psi_adjoint(n, a, b, psb[n]);
       }
+    SXD_REREAD(u);
+    // hphi[n] = 0.5 sum_mu nv[mu] pon[n + 3 mu]
+#pragma unroll
+    for (int n = 0; n < 3; ++n)
+#pragma unroll
+      for (int mu = 0; mu < 4; ++mu) {
+        nv_b[mu] += 0.5 * hphi_b[n] * pon[n + 3 * mu];
+        pon_b[n + 3 * mu] += 0.5 * hphi_b[n] * nv[mu];
+      }
+    // pon[n + 3 nu] = sum_mu nv[mu] Phi(n, mu, nu)
+#pragma unroll
+    for (int n = 0; n < 3; ++n)
+#pragma unroll
+      for (int nu = 0; nu < 4; ++nu)
+#pragma unroll
+        for (int mu = 0; mu < 4; ++mu) {
+          nv_b[mu] += pon_b[n + 3 * nu] * PHI_(n, mu, nu);
+          phi_b[iaa(n, mu, nu)] += pon_b[n + 3 * nu] * nv[mu];
+        }
+    // hptn = 0.5 sum nv[mu] pion[mu]
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+      nv_b[mu] += 0.5 * hptn_b * pion[mu];
+      pion_b[mu] += 0.5 * hptn_b * nv[mu];
+    }
+    // pion[mu] = sum_nu nv[nu] Pi(nu, mu)
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+      for (int nu = 0; nu < 4; ++nu) {
+        nv_b[nu] += pion_b[mu] * PI_(nu, mu);
+        pi_b[sym4(nu, mu)] += pion_b[mu] * nv[nu];
+      }
+  }
   // nv[0] = 1 / lapse, nv[i + 1] = -shift_i nv[0]
   double nv0_b = nv_b[0];
+#pragma unroll
   for (int i = 0; i < 3; ++i) {
     shift_b[i] -= nv_b[i + 1] * nv[0];
     nv0_b -= nv_b[i + 1] * shift[i];
   }
   lapse_b -= nv0_b / (lapse * lapse);
-  // tc[i] = sum_{j <= k} w_jk chr(i, j, k) ig(j, k)
-  for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j)
-      for (int k = j; k < 4; ++k) {
-        const double f = (j == k ? 1.0 : 2.0) * tc_b[i];
-        chr_b[abb(i, j, k)] += f * ig[sym4(j, k)];
-        ig_b[sym4(j, k)] += f * chr[abb(i, j, k)];
-      }
-  // chr(k, i, j) = 0.5 (DA(i, j, k) + DA(j, i, k) - DA(k, i, j))
-  auto add_da = [&](int a, int b, int c, double v) {
-    if (a == 0)
-      dtg_b[sym4(b, c)] += v;
-    else
-      phi_b[iaa(a - 1, b, c)] += v;
-  };
-  for (int k = 0; k < 4; ++k)
-    for (int i = 0; i < 4; ++i)
-      for (int j = i; j < 4; ++j) {
-        const double h = 0.5 * chr_b[abb(k, i, j)];
-        add_da(i, j, k, h);
-        add_da(j, i, k, h);
-        add_da(k, i, j, -h);
-      }
+  SXD_REREAD(u);
+  SXD_REREAD(du);
+  SXD_REREAD(gbar);
   // dtg(a b) = -lapse Pi(a b) + sum_m shift_m Phi(m, a, b)
+#pragma unroll
   for (int a = 0; a < 4; ++a)
+#pragma unroll
     for (int b = a; b < 4; ++b) {
       const double w = dtg_b[sym4(a, b)];
       lapse_b -= w * PI_(a, b);
       pi_b[sym4(a, b)] -= w * lapse;
+#pragma unroll
       for (int m = 0; m < 3; ++m) {
         shift_b[m] += w * PHI_(m, a, b);
         phi_b[iaa(m, a, b)] += w * shift[m];
@@ -455,11 +614,14 @@ SXD_HD void gh_time_derivative_vjp(const double* u, const double* du, double gam
     }
   // inverse spacetime metric
   double m1ol2_b = ig_b[sym4(0, 0)];
+#pragma unroll
   for (int i = 0; i < 3; ++i) {
     shift_b[i] -= ig_b[sym4(0, i + 1)] * m1ol2;
     m1ol2_b -= ig_b[sym4(0, i + 1)] * shift[i];
   }
+#pragma unroll
   for (int i = 0; i < 3; ++i)
+#pragma unroll
     for (int j = i; j < 3; ++j) {
       const double w = ig_b[sym4(i + 1, j + 1)];
       isp_b[sym3(i, j)] += w;
@@ -470,11 +632,36 @@ SXD_HD void gh_time_derivative_vjp(const double* u, const double* du, double gam
   // m1ol2 = -1 / lapse^2
   lapse_b += m1ol2_b * 2.0 / (lapse * lapse * lapse);
   lapse_shift_vjp(L, isp_b, shift_b, lapse_b, g_b);
+#undef G_
 #undef PI_
 #undef PHI_
 #undef DG_
 #undef DPI_
 #undef DPHI_
+#undef GB_
+}
+
+// Adjoint of the point-wise GH time derivative with respect to du component
+// c (layout of gh_time_derivative_vjp).  dt depends on du linearly:
+//   dt_g(ab)     += (1 + gamma1) sum_n shift_n dg(n, ab)
+//   dt_Pi(ab)    += gamma1 gamma2 sum_n shift_n dg(n, ab) + sum_m shift_m dPi(m, ab)
+//                   - lapse sum_{m,n} isp(m, n) dPhi(m, n, ab)
+//   dt_Phi(i,ab) += lapse (gamma2 dg(i, ab) - dPi(i, ab)) + sum_m shift_m dPhi(m, i, ab)
+SXD_HD double gh_time_derivative_dubar(const GhVjpPoint& pt, double gamma1, double gamma2,
+                                       const double* gbar, int sg, int c) {
+  if (c < 30) {  // dg(n, ab), c = 3 p + n
+    const int p = c / 3, n = c % 3;
+    const double ggb = gbar[p * sg], gpb = gbar[(10 + p) * sg], gfb = gbar[(20 + n + 3 * p) * sg];
+    return gamma2 * gfb * pt.lapse + (gamma1 * gamma2 * gpb + (1.0 + gamma1) * ggb) * pt.shift[n];
+  }
+  if (c < 60) {  // dPi(i, ab), c = 30 + 3 p + i
+    const int p = (c - 30) / 3, i = (c - 30) % 3;
+    return -gbar[(20 + i + 3 * p) * sg] * pt.lapse + gbar[(10 + p) * sg] * pt.shift[i];
+  }
+  // dPhi(m, n, ab), c = 60 + m + 3 n + 9 p
+  const int r = c - 60, m = r % 3, n = (r / 3) % 3, p = r / 9;
+  return gbar[(20 + n + 3 * p) * sg] * pt.shift[m] -
+         gbar[(10 + p) * sg] * pt.lapse * pt.isp[sym3(m, n)];
 }
 
 // One side of a face point: unit normal covector/vector, magnitude and
@@ -485,7 +672,10 @@ struct GhSideFwd {
   double nraw[3], nvraw[3], mag, n[3], nv[3], lam[4];
 };
 
-SXD_HD void gh_side_forward(GhSideFwd& s, const double* g, double gamma1, const double nraw[3]) {
+SXD_HD void gh_side_forward(GhSideFwd& s, const double* u, int su, double gamma1,
+                            const double nraw[3]) {
+  double g[10];
+  for (int c = 0; c < 10; ++c) g[c] = u[c * su];
   adj::lapse_shift(s.L, g);
   for (int i = 0; i < 3; ++i) s.nraw[i] = nraw[i];
   for (int i = 0; i < 3; ++i) {
@@ -539,19 +729,20 @@ SXD_HD void gh_side_vjp(const GhSideFwd& s, double gamma1, const double n_b_in[3
 
 // Adjoint of the lifted GH UpwindPenalty correction at one face point,
 // dt_int += lift_factor(N_perp, |n_int|) bc(u_int, u_ext).  Given gbar = the
-// adjoint of dt at the interior point (50), ADDS the contributions to
-// ubar_int (50) and ubar_ext (50).  gamma1/gamma2 per side; nraw_* the
-// unnormalised normal covectors as used by the forward (interior: pointing
-// out of the interior element; exterior: the neighbour's own, sign as in the
-// forward).
+// adjoint of dt at the interior point (50 components, stride sg), ADDS the
+// contributions to ubar_int (50) and ubar_ext (50); either may be null when
+// that side's adjoint is not wanted.  u_int and u_ext are read with component
+// stride su.  gamma1/gamma2 per side; nraw_* the unnormalised normal
+// covectors as used by the forward (interior: pointing out of the interior
+// element; exterior: the neighbour's own, sign as in the forward).
 SXD_HD void gh_face_vjp(const double* u_int, double gamma1_int, double gamma2_int,
                         const double nraw_int[3], const double* u_ext, double gamma1_ext,
-                        double gamma2_ext, const double nraw_ext[3], int n_perp,
-                        const double* gbar, double* ubar_int, double* ubar_ext) {
+                        double gamma2_ext, const double nraw_ext[3], int n_perp, int su,
+                        const double* gbar, int sg, double* ubar_int, double* ubar_ext) {
   using namespace adj;
   GhSideFwd si, se;
-  gh_side_forward(si, u_int, gamma1_int, nraw_int);
-  gh_side_forward(se, u_ext, gamma1_ext, nraw_ext);
+  gh_side_forward(si, u_int, su, gamma1_int, nraw_int);
+  gh_side_forward(se, u_ext, su, gamma1_ext, nraw_ext);
   const double fac = -0.5 * static_cast<double>(n_perp * (n_perp - 1)) * si.mag;
   // weights of the upwind selection (piecewise constant)
   const double wg_i = step(-si.lam[0]), wg_e = -step(se.lam[0]);
@@ -562,6 +753,9 @@ SXD_HD void gh_face_vjp(const double* u_int, double gamma1_int, double gamma2_in
   double n_be[3] = {0, 0, 0}, nv_be[3] = {0, 0, 0}, lam_be[4] = {0, 0, 0, 0};
   double fac_b = 0.0;
   for (int pr = 0; pr < 10; ++pr) {
+    SXD_REREAD(u_int);
+    SXD_REREAD(u_ext);
+    SXD_REREAD(gbar);
     const int comp[5] = {pr, 10 + pr, 20 + 3 * pr, 21 + 3 * pr, 22 + 3 * pr};
     // forward package of both sides (for bc and the lam adjoints)
     const GhSideFwd* sides[2] = {&si, &se};
@@ -570,14 +764,14 @@ SXD_HD void gh_face_vjp(const double* u_int, double gamma1_int, double gamma2_in
     double pk[2][13], ndphi[2], g2g[2];
     for (int sd = 0; sd < 2; ++sd) {
       const GhSideFwd& S = *sides[sd];
-      const double g = us[sd][pr], pi = us[sd][10 + pr];
-      const double* phi = us[sd] + 20 + 3 * pr;
+      const double g = us[sd][pr * su], pi = us[sd][(10 + pr) * su];
+      const double* phi = us[sd] + (20 + 3 * pr) * su;
       g2g[sd] = g2s[sd] * g;
       ndphi[sd] = 0.0;
-      for (int i = 0; i < 3; ++i) ndphi[sd] += S.nv[i] * phi[i];
+      for (int i = 0; i < 3; ++i) ndphi[sd] += S.nv[i] * phi[i * su];
       pk[sd][4] = S.lam[2] * ((pi + ndphi[sd]) - g2g[sd]);
       pk[sd][5] = S.lam[3] * ((pi - ndphi[sd]) - g2g[sd]);
-      for (int i = 0; i < 3; ++i) pk[sd][1 + i] = S.lam[1] * (phi[i] - S.n[i] * ndphi[sd]);
+      for (int i = 0; i < 3; ++i) pk[sd][1 + i] = S.lam[1] * (phi[i * su] - S.n[i] * ndphi[sd]);
       for (int d = 0; d < 3; ++d) {
         pk[sd][6 + d] = pk[sd][4] * S.n[d];
         pk[sd][9 + d] = pk[sd][5] * S.n[d];
@@ -597,8 +791,8 @@ SXD_HD void gh_face_vjp(const double* u_int, double gamma1_int, double gamma2_in
     // dt += bc fac
     double bc_b[5];
     for (int c = 0; c < 5; ++c) {
-      bc_b[c] = gbar[comp[c]] * fac;
-      fac_b += gbar[comp[c]] * bc[c];
+      bc_b[c] = gbar[comp[c] * sg] * fac;
+      fac_b += gbar[comp[c] * sg] * bc[c];
     }
     // boundary terms reverse (weights constant)
     double pk_b[2][13];
@@ -627,11 +821,12 @@ SXD_HD void gh_face_vjp(const double* u_int, double gamma1_int, double gamma2_in
     double* nvbs[2] = {nv_bi, nv_be};
     double* lbs[2] = {lam_bi, lam_be};
     for (int sd = 0; sd < 2; ++sd) {
+      if (ub[sd] == nullptr) continue;
       const GhSideFwd& S = *sides[sd];
       const double* P = pk[sd];
       double* B = pk_b[sd];
-      const double g = us[sd][pr], pi = us[sd][10 + pr];
-      const double* phi = us[sd] + 20 + 3 * pr;
+      const double g = us[sd][pr * su], pi = us[sd][(10 + pr) * su];
+      const double* phi = us[sd] + (20 + 3 * pr) * su;
       double g2g_b = 0.0, ndphi_b = 0.0, pi_b = 0.0, g_b = 0.0, phi_b[3] = {0, 0, 0};
       // pk12 = g2g lam0; pk0 = lam0 g
       g2g_b += B[12] * S.lam[0];
@@ -645,7 +840,7 @@ SXD_HD void gh_face_vjp(const double* u_int, double gamma1_int, double gamma2_in
       }
       // pk1+i = lam1 (phi_i - n_i ndphi)
       for (int i = 0; i < 3; ++i) {
-        const double inner = phi[i] - S.n[i] * ndphi[sd];
+        const double inner = phi[i * su] - S.n[i] * ndphi[sd];
         lbs[sd][1] += B[1 + i] * inner;
         const double ib = B[1 + i] * S.lam[1];
         phi_b[i] += ib;
@@ -661,7 +856,7 @@ SXD_HD void gh_face_vjp(const double* u_int, double gamma1_int, double gamma2_in
       g2g_b -= a4 + a5;
       // ndphi = nv . phi; g2g = gamma2 g
       for (int i = 0; i < 3; ++i) {
-        nvbs[sd][i] += ndphi_b * phi[i];
+        nvbs[sd][i] += ndphi_b * phi[i * su];
         phi_b[i] += ndphi_b * S.nv[i];
       }
       g_b += g2g_b * g2s[sd];
@@ -672,10 +867,11 @@ SXD_HD void gh_face_vjp(const double* u_int, double gamma1_int, double gamma2_in
   }
   // fac = -N(N-1)/2 |n_int|
   const double mag_b = fac_b * (-0.5 * static_cast<double>(n_perp * (n_perp - 1)));
-  gh_side_vjp(si, gamma1_int, n_bi, nv_bi, lam_bi, mag_b, ubar_int);
-  gh_side_vjp(se, gamma1_ext, n_be, nv_be, lam_be, 0.0, ubar_ext);
+  if (ubar_int != nullptr) gh_side_vjp(si, gamma1_int, n_bi, nv_bi, lam_bi, mag_b, ubar_int);
+  if (ubar_ext != nullptr) gh_side_vjp(se, gamma1_ext, n_be, nv_be, lam_be, 0.0, ubar_ext);
 }
 
+#undef SXD_REREAD
 #undef SXD_HD
 
 }  // namespace sxd
