@@ -209,14 +209,14 @@ rk3_adjoint_substep_kernel(int substep, int64_t n, double dt, const double* lam,
 
 // ---------------------------------------------------------------------------
 // Generalized harmonic: adjoint of the RHS linearised at the state u (the RHS
-// is nonlinear).  Three launches, one thread per grid point each:
+// is nonlinear), after the partial derivatives of u:
 //  1. point-wise: ubar = (df/du)^T gbar; the adjoint of the inertial
 //     derivatives contracted with the inverse Jacobian into logical-derivative
 //     adjoints xbar[3 c + j], written over du (read first, same point);
 //  2. transposed differentiation sweeps: ubar_c += sum_j D_j^T xbar_cj;
-//  3. faces: every face point gathers the transposed lifted UpwindPenalty
-//     terms of the faces it lies on, as interior point of its own face and as
-//     exterior point of the neighbour's face.
+//  3. faces, one launch per dimension: every face point adds the transposed
+//     lifted UpwindPenalty terms of its face, as interior point of its own
+//     face and as exterior point of the neighbour's face.
 __global__ void __launch_bounds__(128)
 gh_adjoint_pointwise_kernel(int64_t total, int np, const double* __restrict__ u,
                             double* du_xbar, const double* __restrict__ gammas,
@@ -279,61 +279,46 @@ transposed_sweeps_kernel(int n0, int n1, int n2, int nc, const double* __restric
   }
 }
 
+// Faces normal to dimension d, one thread per (element, side, face point):
+// the lower and upper faces hold different points, so no two threads of a
+// launch update the same point.  Grid: E * 2 * N_a * N_b threads.
 __global__ void __launch_bounds__(64)
-gh_adjoint_face_kernel(int64_t total, int n0, int n1, int n2, const int32_t* __restrict__ nbr,
-                       const double* __restrict__ u, const double* __restrict__ gammas,
-                       const double* __restrict__ inv_jac, const double* __restrict__ gbar,
-                       double* __restrict__ ubar) {
+gh_adjoint_face_kernel(int d, int64_t total, int n0, int n1, int n2,
+                       const int32_t* __restrict__ nbr, const double* __restrict__ u,
+                       const double* __restrict__ gammas, const double* __restrict__ inv_jac,
+                       const double* __restrict__ gbar, double* __restrict__ ubar) {
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= total) return;
   const int np = n0 * n1 * n2;
   const int n[3] = {n0, n1, n2};
-  const int64_t e = gid / np;
-  const int p = static_cast<int>(gid % np);
-  const int idx[3] = {p % n0, (p / n0) % n1, p / (n0 * n1)};
-  bool on_face = false;
-  for (int d = 0; d < 3; ++d) on_face = on_face || idx[d] == 0 || idx[d] == n[d] - 1;
-  if (!on_face) return;
-  double acc[50];
-  for (int c = 0; c < 50; ++c) acc[c] = 0.0;
-  const double* up = u + e * 50 * np + p;
-  const double* gp = gbar + e * 50 * np + p;
+  const int ad = d == 0 ? 1 : 0, bd = d == 2 ? 1 : 2;
+  const int per_face = n[ad] * n[bd];
+  const int64_t e = gid / (2 * per_face);
+  const int r = static_cast<int>(gid % (2 * per_face));
+  const int s = r / per_face, f = r % per_face;
+  int pi[3], qi[3];
+  pi[ad] = qi[ad] = f % n[ad];
+  pi[bd] = qi[bd] = f / n[ad];
+  pi[d] = s == 0 ? 0 : n[d] - 1;
+  qi[d] = s == 0 ? n[d] - 1 : 0;
+  const int p = pi[0] + n0 * (pi[1] + n1 * pi[2]);
+  const int q = qi[0] + n0 * (qi[1] + n1 * qi[2]);
+  const int64_t en = nbr[6 * e + 2 * d + s];
   const double* gap = gammas + e * 3 * np + p;
-  for (int d = 0; d < 3; ++d) {
-    for (int s = 0; s < 2; ++s) {
-      if (idx[d] != (s == 0 ? 0 : n[d] - 1)) continue;
-      if (n[d] == 1) continue;
-      const int64_t en = nbr[6 * e + 2 * d + s];
-      int qi[3] = {idx[0], idx[1], idx[2]};
-      qi[d] = s == 0 ? n[d] - 1 : 0;
-      const int q = qi[0] + n0 * (qi[1] + n1 * qi[2]);
-      const double* uq = u + en * 50 * np + q;
-      const double* gq = gbar + en * 50 * np + q;
-      const double* gaq = gammas + en * 3 * np + q;
-      double np_row[3], nq_row[3];
-      for (int i = 0; i < 3; ++i) {
-        np_row[i] = inv_jac[(e * 9 + ij3(d, i)) * np + p];
-        nq_row[i] = inv_jac[(en * 9 + ij3(d, i)) * np + q];
-      }
-      // interior term of this element's face (d, s)
-      double ni[3], nx[3];
-      for (int i = 0; i < 3; ++i) {
-        ni[i] = s == 0 ? np_row[i] * -1.0 : np_row[i];
-        nx[i] = s == 1 ? nq_row[i] * -1.0 : nq_row[i];
-      }
-      gh_face_vjp(up, gap[np], gap[2 * np], ni, uq, gaq[np], gaq[2 * np], nx, n[d], np, gp, np,
-                  acc, nullptr);
-      // exterior term of the neighbour's face (d, 1 - s), whose interior
-      // point is q
-      for (int i = 0; i < 3; ++i) {
-        ni[i] = (1 - s) == 0 ? nq_row[i] * -1.0 : nq_row[i];
-        nx[i] = (1 - s) == 1 ? np_row[i] * -1.0 : np_row[i];
-      }
-      gh_face_vjp(uq, gaq[np], gaq[2 * np], ni, up, gap[np], gap[2 * np], nx, n[d], np, gq, np,
-                  nullptr, acc);
-    }
+  const double* gaq = gammas + en * 3 * np + q;
+  // normal covectors pointing out of each point's own element
+  double np_row[3], nq_row[3];
+  for (int i = 0; i < 3; ++i) {
+    np_row[i] = inv_jac[(e * 9 + ij3(d, i)) * np + p];
+    nq_row[i] = inv_jac[(en * 9 + ij3(d, i)) * np + q];
+    if (s == 0) np_row[i] = np_row[i] * -1.0;
+    if (s == 1) nq_row[i] = nq_row[i] * -1.0;
   }
-  for (int c = 0; c < 50; ++c) ubar[(e * 50 + c) * np + p] += acc[c];
+  // this element's face (d, s) at p and the neighbour's face (d, 1 - s) at
+  // q: each point is the other's exterior
+  gh_face_pair_vjp(u + e * 50 * np + p, gap[np], gap[2 * np], np_row, u + en * 50 * np + q,
+                   gaq[np], gaq[2 * np], nq_row, n[d], np, gbar + e * 50 * np + p,
+                   gbar + en * 50 * np + q, np, ubar + e * 50 * np + p, np);
 }
 
 }  // namespace
@@ -386,10 +371,15 @@ int launch_gh_rhs_adjoint(int num_elements, const int32_t ext[3], const int32_t*
                      3 * np * sizeof(double), stream, ext[0], ext[1], ext[2], 50, scratch_du,
                      ubar, dtab);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
-  hipLaunchKernelGGL(gh_adjoint_face_kernel, dim3(static_cast<unsigned>((total + 63) / 64)),
-                     dim3(64), 0, stream, total, ext[0], ext[1], ext[2], nbr, u, gammas, inv_jac,
-                     gbar, ubar);
-  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
+  for (int d = 0; d < 3; ++d) {
+    if (ext[d] == 1) continue;
+    const int64_t faces = static_cast<int64_t>(num_elements) * 2 * (np / ext[d]);
+    hipLaunchKernelGGL(gh_adjoint_face_kernel, dim3(static_cast<unsigned>((faces + 63) / 64)),
+                       dim3(64), 0, stream, d, faces, ext[0], ext[1], ext[2], nbr, u, gammas,
+                       inv_jac, gbar, ubar);
+    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  }
+  return SXD_OK;
 }
 
 int launch_rk3_adjoint_substep(int substep, int64_t n, double dt, const double* lam,

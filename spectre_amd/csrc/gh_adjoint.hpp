@@ -727,148 +727,159 @@ SXD_HD void gh_side_vjp(const GhSideFwd& s, double gamma1, const double n_b_in[3
   adj::lapse_shift_vjp(s.L, isp_b, shift_b, lapse_b, gbar);
 }
 
-// Adjoint of the lifted GH UpwindPenalty correction at one face point,
-// dt_int += lift_factor(N_perp, |n_int|) bc(u_int, u_ext).  Given gbar = the
-// adjoint of dt at the interior point (50 components, stride sg), ADDS the
-// contributions to ubar_int (50) and ubar_ext (50); either may be null when
-// that side's adjoint is not wanted.  u_int and u_ext are read with component
-// stride su.  gamma1/gamma2 per side; nraw_* the unnormalised normal
-// covectors as used by the forward (interior: pointing out of the interior
-// element; exterior: the neighbour's own, sign as in the forward).
-SXD_HD void gh_face_vjp(const double* u_int, double gamma1_int, double gamma2_int,
-                        const double nraw_int[3], const double* u_ext, double gamma1_ext,
-                        double gamma2_ext, const double nraw_ext[3], int n_perp, int su,
-                        const double* gbar, int sg, double* ubar_int, double* ubar_ext) {
+// Adjoint of the lifted GH UpwindPenalty corrections of one face point pair,
+// with respect to the state at p.  p and q are the two LGL points that meet
+// on a face: the correction at p is
+//   dt(p) += lift_factor(N_perp, |n_p|) bc(p interior, q exterior)
+// and the correction at q (the neighbour's own face) is
+//   dt(q) += lift_factor(N_perp, |n_q|) bc(q interior, p exterior).
+// The two share the sides' normals, speeds and packaged fields: n_p is the
+// unnormalised normal covector at p pointing out of p's element (the
+// inverse-Jacobian row, sign applied) and n_q the one at q, as the forward
+// uses them.  Given gbar_p and gbar_q, the adjoints of dt at p and q (50
+// components, stride sg), ADDS d/du_p of <gbar_p, dt(p)> + <gbar_q, dt(q)>
+// to ubar_p (50 components, stride sb).  u_p and u_q are read with stride su.
+SXD_HD void gh_face_pair_vjp(const double* u_p, double gamma1_p, double gamma2_p,
+                             const double n_p[3], const double* u_q, double gamma1_q,
+                             double gamma2_q, const double n_q[3], int n_perp, int su,
+                             const double* gbar_p, const double* gbar_q, int sg, double* ubar_p,
+                             int sb) {
   using namespace adj;
-  GhSideFwd si, se;
-  gh_side_forward(si, u_int, su, gamma1_int, nraw_int);
-  gh_side_forward(se, u_ext, su, gamma1_ext, nraw_ext);
-  const double fac = -0.5 * static_cast<double>(n_perp * (n_perp - 1)) * si.mag;
-  // weights of the upwind selection (piecewise constant)
-  const double wg_i = step(-si.lam[0]), wg_e = -step(se.lam[0]);
-  const double wz_i = step(-si.lam[1]), wz_e = -step(se.lam[1]);
-  const double wp_i = step(-si.lam[2]), wp_e = -step(se.lam[2]);
-  const double wm_i = step(-si.lam[3]), wm_e = -step(se.lam[3]);
-  double n_bi[3] = {0, 0, 0}, nv_bi[3] = {0, 0, 0}, lam_bi[4] = {0, 0, 0, 0};
-  double n_be[3] = {0, 0, 0}, nv_be[3] = {0, 0, 0}, lam_be[4] = {0, 0, 0, 0};
-  double fac_b = 0.0;
-  for (int pr = 0; pr < 10; ++pr) {
-    SXD_REREAD(u_int);
-    SXD_REREAD(u_ext);
-    SXD_REREAD(gbar);
-    const int comp[5] = {pr, 10 + pr, 20 + 3 * pr, 21 + 3 * pr, 22 + 3 * pr};
-    // forward package of both sides (for bc and the lam adjoints)
-    const GhSideFwd* sides[2] = {&si, &se};
-    const double* us[2] = {u_int, u_ext};
-    const double g2s[2] = {gamma2_int, gamma2_ext};
-    double pk[2][13], ndphi[2], g2g[2];
-    for (int sd = 0; sd < 2; ++sd) {
-      const GhSideFwd& S = *sides[sd];
-      const double g = us[sd][pr * su], pi = us[sd][(10 + pr) * su];
-      const double* phi = us[sd] + (20 + 3 * pr) * su;
-      g2g[sd] = g2s[sd] * g;
-      ndphi[sd] = 0.0;
-      for (int i = 0; i < 3; ++i) ndphi[sd] += S.nv[i] * phi[i * su];
-      pk[sd][4] = S.lam[2] * ((pi + ndphi[sd]) - g2g[sd]);
-      pk[sd][5] = S.lam[3] * ((pi - ndphi[sd]) - g2g[sd]);
-      for (int i = 0; i < 3; ++i) pk[sd][1 + i] = S.lam[1] * (phi[i * su] - S.n[i] * ndphi[sd]);
-      for (int d = 0; d < 3; ++d) {
-        pk[sd][6 + d] = pk[sd][4] * S.n[d];
-        pk[sd][9 + d] = pk[sd][5] * S.n[d];
-      }
-      pk[sd][0] = S.lam[0] * g;
-      pk[sd][12] = g2g[sd] * S.lam[0];
-    }
-    const double* in = pk[0];
-    const double* ex = pk[1];
-    double bc[5];
-    bc[0] = wg_e * ex[0] - wg_i * in[0];
-    bc[1] = 0.5 * (wp_e * ex[4] + wm_e * ex[5]) + wg_e * ex[12] - 0.5 * (wp_i * in[4] + wm_i * in[5]) -
-            wg_i * in[12];
-    for (int d = 0; d < 3; ++d)
-      bc[2 + d] = -0.5 * (wm_e * ex[9 + d] - wp_e * ex[6 + d]) + wz_e * ex[1 + d] -
-                  0.5 * (wp_i * in[6 + d] - wm_i * in[9 + d]) - wz_i * in[1 + d];
-    // dt += bc fac
-    double bc_b[5];
-    for (int c = 0; c < 5; ++c) {
-      bc_b[c] = gbar[comp[c] * sg] * fac;
-      fac_b += gbar[comp[c] * sg] * bc[c];
-    }
-    // boundary terms reverse (weights constant)
-    double pk_b[2][13];
-    for (int sd = 0; sd < 2; ++sd)
-      for (int c = 0; c < 13; ++c) pk_b[sd][c] = 0.0;
-    pk_b[1][0] += wg_e * bc_b[0];
-    pk_b[0][0] -= wg_i * bc_b[0];
-    pk_b[1][4] += 0.5 * wp_e * bc_b[1];
-    pk_b[1][5] += 0.5 * wm_e * bc_b[1];
-    pk_b[1][12] += wg_e * bc_b[1];
-    pk_b[0][4] -= 0.5 * wp_i * bc_b[1];
-    pk_b[0][5] -= 0.5 * wm_i * bc_b[1];
-    pk_b[0][12] -= wg_i * bc_b[1];
-    for (int d = 0; d < 3; ++d) {
-      const double w = bc_b[2 + d];
-      pk_b[1][9 + d] -= 0.5 * wm_e * w;
-      pk_b[1][6 + d] += 0.5 * wp_e * w;
-      pk_b[1][1 + d] += wz_e * w;
-      pk_b[0][6 + d] -= 0.5 * wp_i * w;
-      pk_b[0][9 + d] += 0.5 * wm_i * w;
-      pk_b[0][1 + d] -= wz_i * w;
-    }
-    // package reverse
-    double* ub[2] = {ubar_int, ubar_ext};
-    double* nbs[2] = {n_bi, n_be};
-    double* nvbs[2] = {nv_bi, nv_be};
-    double* lbs[2] = {lam_bi, lam_be};
-    for (int sd = 0; sd < 2; ++sd) {
-      if (ub[sd] == nullptr) continue;
-      const GhSideFwd& S = *sides[sd];
-      const double* P = pk[sd];
-      double* B = pk_b[sd];
-      const double g = us[sd][pr * su], pi = us[sd][(10 + pr) * su];
-      const double* phi = us[sd] + (20 + 3 * pr) * su;
-      double g2g_b = 0.0, ndphi_b = 0.0, pi_b = 0.0, g_b = 0.0, phi_b[3] = {0, 0, 0};
-      // pk12 = g2g lam0; pk0 = lam0 g
-      g2g_b += B[12] * S.lam[0];
-      lbs[sd][0] += B[12] * g2g[sd] + B[0] * g;
-      g_b += B[0] * S.lam[0];
-      // pk6+d = pk4 n_d; pk9+d = pk5 n_d
-      for (int d = 0; d < 3; ++d) {
-        B[4] += B[6 + d] * S.n[d];
-        B[5] += B[9 + d] * S.n[d];
-        nbs[sd][d] += B[6 + d] * P[4] + B[9 + d] * P[5];
-      }
-      // pk1+i = lam1 (phi_i - n_i ndphi)
-      for (int i = 0; i < 3; ++i) {
-        const double inner = phi[i * su] - S.n[i] * ndphi[sd];
-        lbs[sd][1] += B[1 + i] * inner;
-        const double ib = B[1 + i] * S.lam[1];
-        phi_b[i] += ib;
-        nbs[sd][i] -= ib * ndphi[sd];
-        ndphi_b -= ib * S.n[i];
-      }
-      // pk4 = lam2 ((pi + ndphi) - g2g); pk5 = lam3 ((pi - ndphi) - g2g)
-      lbs[sd][2] += B[4] * ((pi + ndphi[sd]) - g2g[sd]);
-      lbs[sd][3] += B[5] * ((pi - ndphi[sd]) - g2g[sd]);
-      const double a4 = B[4] * S.lam[2], a5 = B[5] * S.lam[3];
-      pi_b += a4 + a5;
-      ndphi_b += a4 - a5;
-      g2g_b -= a4 + a5;
-      // ndphi = nv . phi; g2g = gamma2 g
-      for (int i = 0; i < 3; ++i) {
-        nvbs[sd][i] += ndphi_b * phi[i * su];
-        phi_b[i] += ndphi_b * S.nv[i];
-      }
-      g_b += g2g_b * g2s[sd];
-      ub[sd][pr] += g_b;
-      ub[sd][10 + pr] += pi_b;
-      for (int i = 0; i < 3; ++i) ub[sd][20 + 3 * pr + i] += phi_b[i];
-    }
+  GhSideFwd sp, sq;
+  gh_side_forward(sp, u_p, su, gamma1_p, n_p);
+  gh_side_forward(sq, u_q, su, gamma1_q, n_q);
+  const double lift = -0.5 * static_cast<double>(n_perp * (n_perp - 1));
+  const double fac_p = lift * sp.mag, fac_q = lift * sq.mag;
+  // upwind weights (piecewise constant) of p's side as the interior side
+  // (correction at p) and as the exterior side (correction at q)
+  double wi[4], we[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    wi[k] = step(-sp.lam[k]);
+    we[k] = -step(sp.lam[k]);
   }
-  // fac = -N(N-1)/2 |n_int|
-  const double mag_b = fac_b * (-0.5 * static_cast<double>(n_perp * (n_perp - 1)));
-  if (ubar_int != nullptr) gh_side_vjp(si, gamma1_int, n_bi, nv_bi, lam_bi, mag_b, ubar_int);
-  if (ubar_ext != nullptr) gh_side_vjp(se, gamma1_ext, n_be, nv_be, lam_be, 0.0, ubar_ext);
+  // the same for q's side: exterior at p, interior at q
+  double qe[4], qi[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    qe[k] = -step(sq.lam[k]);
+    qi[k] = step(-sq.lam[k]);
+  }
+  double n_b[3] = {0, 0, 0}, nv_b[3] = {0, 0, 0}, lam_b[4] = {0, 0, 0, 0}, fac_b = 0.0;
+  // packaged fields of one side for one symmetric pair pr (UpwindPenalty
+  // dg_package_data): 0 lam0 g, 1-3 lam1 (phi - n ndphi), 4 lam2 (pi + ndphi
+  // - g2 g), 5 lam3 (pi - ndphi - g2 g), 6-8 pk4 n, 9-11 pk5 n, 12 g2 g lam0
+  auto package = [&](const GhSideFwd& S, const double* u, double gamma2, int pr, double* pk) {
+    const double g = u[pr * su], pi = u[(10 + pr) * su];
+    const double* phi = u + (20 + 3 * pr) * su;
+    const double g2g = gamma2 * g;
+    double ndphi = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ndphi += S.nv[i] * phi[i * su];
+    pk[4] = S.lam[2] * ((pi + ndphi) - g2g);
+    pk[5] = S.lam[3] * ((pi - ndphi) - g2g);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) pk[1 + i] = S.lam[1] * (phi[i * su] - S.n[i] * ndphi);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      pk[6 + d] = pk[4] * S.n[d];
+      pk[9 + d] = pk[5] * S.n[d];
+    }
+    pk[0] = S.lam[0] * g;
+    pk[12] = g2g * S.lam[0];
+  };
+  // dg_boundary_terms for one pr: bc[0] (g), bc[1] (Pi), bc[2..4] (Phi);
+  // w*[k] are the weights of the speeds k = 0..3 (gauge, zero, plus, minus)
+  auto boundary = [&](const double* in, const double* ex, const double* w_i, const double* w_e,
+                      double* bc) {
+    bc[0] = w_e[0] * ex[0] - w_i[0] * in[0];
+    bc[1] = 0.5 * (w_e[2] * ex[4] + w_e[3] * ex[5]) + w_e[0] * ex[12] -
+            0.5 * (w_i[2] * in[4] + w_i[3] * in[5]) - w_i[0] * in[12];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      bc[2 + d] = -0.5 * (w_e[3] * ex[9 + d] - w_e[2] * ex[6 + d]) + w_e[1] * ex[1 + d] -
+                  0.5 * (w_i[2] * in[6 + d] - w_i[3] * in[9 + d]) - w_i[1] * in[1 + d];
+  };
+  for (int pr = 0; pr < 10; ++pr) {
+    const int comp[5] = {pr, 10 + pr, 20 + 3 * pr, 21 + 3 * pr, 22 + 3 * pr};
+    double pkp[13], pkq[13];
+    package(sp, u_p, gamma2_p, pr, pkp);
+    package(sq, u_q, gamma2_q, pr, pkq);
+    // correction at p (p interior): its value feeds fac_p's adjoint
+    double bc[5];
+    boundary(pkp, pkq, wi, qe, bc);
+    double bp_b[5], bq_b[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      bp_b[c] = gbar_p[comp[c] * sg] * fac_p;
+      fac_b += gbar_p[comp[c] * sg] * bc[c];
+      bq_b[c] = gbar_q[comp[c] * sg] * fac_q;
+    }
+    // adjoint of p's package: interior side of the correction at p,
+    // exterior side of the correction at q
+    double B[13];
+    B[0] = -wi[0] * bp_b[0] + we[0] * bq_b[0];
+    B[4] = -0.5 * wi[2] * bp_b[1] + 0.5 * we[2] * bq_b[1];
+    B[5] = -0.5 * wi[3] * bp_b[1] + 0.5 * we[3] * bq_b[1];
+    B[12] = -wi[0] * bp_b[1] + we[0] * bq_b[1];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      B[6 + d] = -0.5 * wi[2] * bp_b[2 + d] + 0.5 * we[2] * bq_b[2 + d];
+      B[9 + d] = 0.5 * wi[3] * bp_b[2 + d] - 0.5 * we[3] * bq_b[2 + d];
+      B[1 + d] = -wi[1] * bp_b[2 + d] + we[1] * bq_b[2 + d];
+    }
+    // package reverse for p
+    const double g = u_p[pr * su], pi = u_p[(10 + pr) * su];
+    const double* phi = u_p + (20 + 3 * pr) * su;
+    const double g2g = gamma2_p * g;
+    double ndphi = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ndphi += sp.nv[i] * phi[i * su];
+    double g2g_b = 0.0, ndphi_b = 0.0, pi_b = 0.0, g_b = 0.0, phi_b[3] = {0, 0, 0};
+    // pk12 = g2g lam0; pk0 = lam0 g
+    g2g_b += B[12] * sp.lam[0];
+    lam_b[0] += B[12] * g2g + B[0] * g;
+    g_b += B[0] * sp.lam[0];
+    // pk6+d = pk4 n_d; pk9+d = pk5 n_d
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      B[4] += B[6 + d] * sp.n[d];
+      B[5] += B[9 + d] * sp.n[d];
+      n_b[d] += B[6 + d] * pkp[4] + B[9 + d] * pkp[5];
+    }
+    // pk1+i = lam1 (phi_i - n_i ndphi)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      lam_b[1] += B[1 + i] * (phi[i * su] - sp.n[i] * ndphi);
+      const double ib = B[1 + i] * sp.lam[1];
+      phi_b[i] += ib;
+      n_b[i] -= ib * ndphi;
+      ndphi_b -= ib * sp.n[i];
+    }
+    // pk4 = lam2 ((pi + ndphi) - g2g); pk5 = lam3 ((pi - ndphi) - g2g)
+    lam_b[2] += B[4] * ((pi + ndphi) - g2g);
+    lam_b[3] += B[5] * ((pi - ndphi) - g2g);
+    const double a4 = B[4] * sp.lam[2], a5 = B[5] * sp.lam[3];
+    pi_b += a4 + a5;
+    ndphi_b += a4 - a5;
+    g2g_b -= a4 + a5;
+    // ndphi = nv . phi; g2g = gamma2 g
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      nv_b[i] += ndphi_b * phi[i * su];
+      phi_b[i] += ndphi_b * sp.nv[i];
+    }
+    g_b += g2g_b * gamma2_p;
+    ubar_p[pr * sb] += g_b;
+    ubar_p[(10 + pr) * sb] += pi_b;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ubar_p[(20 + 3 * pr + i) * sb] += phi_b[i];
+  }
+  // fac_p = lift |n_p|; the normals and speeds of p's side depend on g(p)
+  double gm_b[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  gh_side_vjp(sp, gamma1_p, n_b, nv_b, lam_b, fac_b * lift, gm_b);
+#pragma unroll
+  for (int c = 0; c < 10; ++c) ubar_p[c * sb] += gm_b[c];
 }
 
 #undef SXD_REREAD
