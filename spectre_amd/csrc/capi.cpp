@@ -705,10 +705,16 @@ int sxd_gh_rhs_adjoint(const sxd_domain* domain, const double* u, const double* 
   SXD_REQUIRE_DEVICE(inv_jac);
   SXD_REQUIRE_DEVICE(gbar);
   SXD_REQUIRE_DEVICE(ubar);
-  if (domain->mixed)
-    return fail(SXD_ERR_UNSUPPORTED, "the GH adjoint needs a uniform domain");
   if (gbar == ubar || u == ubar) return fail(SXD_ERR_INVALID_ARGUMENT, "ubar must not alias");
   sxd_domain* d = const_cast<sxd_domain*>(domain);
+  if (d->mixed) {
+    if (ensure_scratch(d, 150 * static_cast<size_t>(d->total_points), 0) != SXD_OK)
+      return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
+    return finish(launch_mixed_gh_adjoint(d->num_elements, d->d_list, d->d_el, d->d_nbr,
+                                          d->np_max, d->kf, d->km, u, gammas, inv_jac, gbar, ubar,
+                                          d->d_scratch_du, static_cast<hipStream_t>(stream)),
+                  __func__);
+  }
   const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
   if (ensure_scratch(d, 150 * np * d->num_elements, 0) != SXD_OK)
     return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");

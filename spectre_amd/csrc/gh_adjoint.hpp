@@ -727,6 +727,132 @@ SXD_HD void gh_side_vjp(const GhSideFwd& s, double gamma1, const double n_b_in[3
   adj::lapse_shift_vjp(s.L, isp_b, shift_b, lapse_b, gbar);
 }
 
+// Packaged fields of pair pr at one point (UpwindPenalty dg_package_data,
+// layout of gh_package_seed); u read with stride su.
+SXD_HD void gh_package_fwd(const double n[3], const double nv[3], const double lam[4],
+                           double gamma2, const double* u, int su, int pr, double pk[13]) {
+  const double g = u[pr * su], pi = u[(10 + pr) * su];
+  const double* phi = u + (20 + 3 * pr) * su;
+  const double g2g = gamma2 * g;
+  double ndphi = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) ndphi += nv[i] * phi[i * su];
+  pk[4] = lam[2] * ((pi + ndphi) - g2g);
+  pk[5] = lam[3] * ((pi - ndphi) - g2g);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) pk[1 + i] = lam[1] * (phi[i * su] - n[i] * ndphi);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    pk[6 + d] = pk[4] * n[d];
+    pk[9 + d] = pk[5] * n[d];
+  }
+  pk[0] = lam[0] * g;
+  pk[12] = g2g * lam[0];
+}
+
+// dg_boundary_terms of pair pr from the interior and exterior packages and
+// the speeds (upwind weights) of both sides.
+SXD_HD void gh_boundary_fwd(const double in[13], const double lam_in[4], const double ex[13],
+                            const double lam_ex[4], double bc[5]) {
+  double wi[4], we[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    wi[k] = adj::step(-lam_in[k]);
+    we[k] = -adj::step(lam_ex[k]);
+  }
+  bc[0] = we[0] * ex[0] - wi[0] * in[0];
+  bc[1] = 0.5 * (we[2] * ex[4] + we[3] * ex[5]) + we[0] * ex[12] -
+          0.5 * (wi[2] * in[4] + wi[3] * in[5]) - wi[0] * in[12];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    bc[2 + d] = -0.5 * (we[3] * ex[9 + d] - we[2] * ex[6 + d]) + we[1] * ex[1 + d] -
+                0.5 * (wi[2] * in[6 + d] - wi[3] * in[9 + d]) - wi[1] * in[1 + d];
+}
+
+// Adjoint of one side's 13 packaged fields of pair pr (UpwindPenalty
+// dg_package_data: 0 lam0 g, 1-3 lam1 (phi - n ndphi), 4 lam2 (pi + ndphi -
+// g2 g), 5 lam3 (pi - ndphi - g2 g), 6-8 pk4 n, 9-11 pk5 n, 12 g2 g lam0)
+// from the adjoints of two boundary corrections (dg_boundary_terms, 5
+// components g, Pi, Phi_0..2): bi of the correction where this side is the
+// interior and bx of the one where it is the exterior.  lam are this side's
+// speeds where the corrections are evaluated; the upwind weights are
+// piecewise constant.
+SXD_HD void gh_package_seed(const double lam[4], const double bi[5], const double bx[5],
+                            double B[13]) {
+  double wi[4], we[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    wi[k] = adj::step(-lam[k]);
+    we[k] = -adj::step(lam[k]);
+  }
+  B[0] = -wi[0] * bi[0] + we[0] * bx[0];
+  B[4] = -0.5 * wi[2] * bi[1] + 0.5 * we[2] * bx[1];
+  B[5] = -0.5 * wi[3] * bi[1] + 0.5 * we[3] * bx[1];
+  B[12] = -wi[0] * bi[1] + we[0] * bx[1];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    B[6 + d] = -0.5 * wi[2] * bi[2 + d] + 0.5 * we[2] * bx[2 + d];
+    B[9 + d] = 0.5 * wi[3] * bi[2 + d] - 0.5 * we[3] * bx[2 + d];
+    B[1 + d] = -wi[1] * bi[2 + d] + we[1] * bx[2 + d];
+  }
+}
+
+// Reverse of the package of pair pr at one point with unit normal n, normal
+// vector nv, speeds lam and gamma2: consumes the package adjoint B and ADDS
+// to the point's ubar (components g, Pi, Phi of pair pr, stride sb) and to
+// the adjoints of n, nv and lam.  u is read with stride su.
+SXD_HD void gh_package_vjp(const double n[3], const double nv[3], const double lam[4],
+                           double gamma2, const double* u, int su, int pr, double B[13],
+                           double* ubar, int sb, double n_b[3], double nv_b[3],
+                           double lam_b[4]) {
+  const double g = u[pr * su], pi = u[(10 + pr) * su];
+  const double* phi = u + (20 + 3 * pr) * su;
+  const double g2g = gamma2 * g;
+  double ndphi = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) ndphi += nv[i] * phi[i * su];
+  const double pk4 = lam[2] * ((pi + ndphi) - g2g), pk5 = lam[3] * ((pi - ndphi) - g2g);
+  double g2g_b = 0.0, ndphi_b = 0.0, pi_b = 0.0, g_b = 0.0, phi_b[3] = {0, 0, 0};
+  // pk12 = g2g lam0; pk0 = lam0 g
+  g2g_b += B[12] * lam[0];
+  lam_b[0] += B[12] * g2g + B[0] * g;
+  g_b += B[0] * lam[0];
+  // pk6+d = pk4 n_d; pk9+d = pk5 n_d
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    B[4] += B[6 + d] * n[d];
+    B[5] += B[9 + d] * n[d];
+    n_b[d] += B[6 + d] * pk4 + B[9 + d] * pk5;
+  }
+  // pk1+i = lam1 (phi_i - n_i ndphi)
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    lam_b[1] += B[1 + i] * (phi[i * su] - n[i] * ndphi);
+    const double ib = B[1 + i] * lam[1];
+    phi_b[i] += ib;
+    n_b[i] -= ib * ndphi;
+    ndphi_b -= ib * n[i];
+  }
+  // pk4 = lam2 ((pi + ndphi) - g2g); pk5 = lam3 ((pi - ndphi) - g2g)
+  lam_b[2] += B[4] * ((pi + ndphi) - g2g);
+  lam_b[3] += B[5] * ((pi - ndphi) - g2g);
+  const double a4 = B[4] * lam[2], a5 = B[5] * lam[3];
+  pi_b += a4 + a5;
+  ndphi_b += a4 - a5;
+  g2g_b -= a4 + a5;
+  // ndphi = nv . phi; g2g = gamma2 g
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    nv_b[i] += ndphi_b * phi[i * su];
+    phi_b[i] += ndphi_b * nv[i];
+  }
+  g_b += g2g_b * gamma2;
+  ubar[pr * sb] += g_b;
+  ubar[(10 + pr) * sb] += pi_b;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) ubar[(20 + 3 * pr + i) * sb] += phi_b[i];
+}
+
 // Adjoint of the lifted GH UpwindPenalty corrections of one face point pair,
 // with respect to the state at p.  p and q are the two LGL points that meet
 // on a face: the correction at p is
@@ -750,64 +876,15 @@ SXD_HD void gh_face_pair_vjp(const double* u_p, double gamma1_p, double gamma2_p
   gh_side_forward(sq, u_q, su, gamma1_q, n_q);
   const double lift = -0.5 * static_cast<double>(n_perp * (n_perp - 1));
   const double fac_p = lift * sp.mag, fac_q = lift * sq.mag;
-  // upwind weights (piecewise constant) of p's side as the interior side
-  // (correction at p) and as the exterior side (correction at q)
-  double wi[4], we[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    wi[k] = step(-sp.lam[k]);
-    we[k] = -step(sp.lam[k]);
-  }
-  // the same for q's side: exterior at p, interior at q
-  double qe[4], qi[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    qe[k] = -step(sq.lam[k]);
-    qi[k] = step(-sq.lam[k]);
-  }
   double n_b[3] = {0, 0, 0}, nv_b[3] = {0, 0, 0}, lam_b[4] = {0, 0, 0, 0}, fac_b = 0.0;
-  // packaged fields of one side for one symmetric pair pr (UpwindPenalty
-  // dg_package_data): 0 lam0 g, 1-3 lam1 (phi - n ndphi), 4 lam2 (pi + ndphi
-  // - g2 g), 5 lam3 (pi - ndphi - g2 g), 6-8 pk4 n, 9-11 pk5 n, 12 g2 g lam0
-  auto package = [&](const GhSideFwd& S, const double* u, double gamma2, int pr, double* pk) {
-    const double g = u[pr * su], pi = u[(10 + pr) * su];
-    const double* phi = u + (20 + 3 * pr) * su;
-    const double g2g = gamma2 * g;
-    double ndphi = 0.0;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) ndphi += S.nv[i] * phi[i * su];
-    pk[4] = S.lam[2] * ((pi + ndphi) - g2g);
-    pk[5] = S.lam[3] * ((pi - ndphi) - g2g);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) pk[1 + i] = S.lam[1] * (phi[i * su] - S.n[i] * ndphi);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      pk[6 + d] = pk[4] * S.n[d];
-      pk[9 + d] = pk[5] * S.n[d];
-    }
-    pk[0] = S.lam[0] * g;
-    pk[12] = g2g * S.lam[0];
-  };
-  // dg_boundary_terms for one pr: bc[0] (g), bc[1] (Pi), bc[2..4] (Phi);
-  // w*[k] are the weights of the speeds k = 0..3 (gauge, zero, plus, minus)
-  auto boundary = [&](const double* in, const double* ex, const double* w_i, const double* w_e,
-                      double* bc) {
-    bc[0] = w_e[0] * ex[0] - w_i[0] * in[0];
-    bc[1] = 0.5 * (w_e[2] * ex[4] + w_e[3] * ex[5]) + w_e[0] * ex[12] -
-            0.5 * (w_i[2] * in[4] + w_i[3] * in[5]) - w_i[0] * in[12];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-      bc[2 + d] = -0.5 * (w_e[3] * ex[9 + d] - w_e[2] * ex[6 + d]) + w_e[1] * ex[1 + d] -
-                  0.5 * (w_i[2] * in[6 + d] - w_i[3] * in[9 + d]) - w_i[1] * in[1 + d];
-  };
   for (int pr = 0; pr < 10; ++pr) {
     const int comp[5] = {pr, 10 + pr, 20 + 3 * pr, 21 + 3 * pr, 22 + 3 * pr};
     double pkp[13], pkq[13];
-    package(sp, u_p, gamma2_p, pr, pkp);
-    package(sq, u_q, gamma2_q, pr, pkq);
+    gh_package_fwd(sp.n, sp.nv, sp.lam, gamma2_p, u_p, su, pr, pkp);
+    gh_package_fwd(sq.n, sq.nv, sq.lam, gamma2_q, u_q, su, pr, pkq);
     // correction at p (p interior): its value feeds fac_p's adjoint
     double bc[5];
-    boundary(pkp, pkq, wi, qe, bc);
+    gh_boundary_fwd(pkp, sp.lam, pkq, sq.lam, bc);
     double bp_b[5], bq_b[5];
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
@@ -815,65 +892,10 @@ SXD_HD void gh_face_pair_vjp(const double* u_p, double gamma1_p, double gamma2_p
       fac_b += gbar_p[comp[c] * sg] * bc[c];
       bq_b[c] = gbar_q[comp[c] * sg] * fac_q;
     }
-    // adjoint of p's package: interior side of the correction at p,
-    // exterior side of the correction at q
+    // p's package: interior of the correction at p, exterior of the one at q
     double B[13];
-    B[0] = -wi[0] * bp_b[0] + we[0] * bq_b[0];
-    B[4] = -0.5 * wi[2] * bp_b[1] + 0.5 * we[2] * bq_b[1];
-    B[5] = -0.5 * wi[3] * bp_b[1] + 0.5 * we[3] * bq_b[1];
-    B[12] = -wi[0] * bp_b[1] + we[0] * bq_b[1];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      B[6 + d] = -0.5 * wi[2] * bp_b[2 + d] + 0.5 * we[2] * bq_b[2 + d];
-      B[9 + d] = 0.5 * wi[3] * bp_b[2 + d] - 0.5 * we[3] * bq_b[2 + d];
-      B[1 + d] = -wi[1] * bp_b[2 + d] + we[1] * bq_b[2 + d];
-    }
-    // package reverse for p
-    const double g = u_p[pr * su], pi = u_p[(10 + pr) * su];
-    const double* phi = u_p + (20 + 3 * pr) * su;
-    const double g2g = gamma2_p * g;
-    double ndphi = 0.0;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) ndphi += sp.nv[i] * phi[i * su];
-    double g2g_b = 0.0, ndphi_b = 0.0, pi_b = 0.0, g_b = 0.0, phi_b[3] = {0, 0, 0};
-    // pk12 = g2g lam0; pk0 = lam0 g
-    g2g_b += B[12] * sp.lam[0];
-    lam_b[0] += B[12] * g2g + B[0] * g;
-    g_b += B[0] * sp.lam[0];
-    // pk6+d = pk4 n_d; pk9+d = pk5 n_d
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      B[4] += B[6 + d] * sp.n[d];
-      B[5] += B[9 + d] * sp.n[d];
-      n_b[d] += B[6 + d] * pkp[4] + B[9 + d] * pkp[5];
-    }
-    // pk1+i = lam1 (phi_i - n_i ndphi)
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      lam_b[1] += B[1 + i] * (phi[i * su] - sp.n[i] * ndphi);
-      const double ib = B[1 + i] * sp.lam[1];
-      phi_b[i] += ib;
-      n_b[i] -= ib * ndphi;
-      ndphi_b -= ib * sp.n[i];
-    }
-    // pk4 = lam2 ((pi + ndphi) - g2g); pk5 = lam3 ((pi - ndphi) - g2g)
-    lam_b[2] += B[4] * ((pi + ndphi) - g2g);
-    lam_b[3] += B[5] * ((pi - ndphi) - g2g);
-    const double a4 = B[4] * sp.lam[2], a5 = B[5] * sp.lam[3];
-    pi_b += a4 + a5;
-    ndphi_b += a4 - a5;
-    g2g_b -= a4 + a5;
-    // ndphi = nv . phi; g2g = gamma2 g
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      nv_b[i] += ndphi_b * phi[i * su];
-      phi_b[i] += ndphi_b * sp.nv[i];
-    }
-    g_b += g2g_b * gamma2_p;
-    ubar_p[pr * sb] += g_b;
-    ubar_p[(10 + pr) * sb] += pi_b;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) ubar_p[(20 + 3 * pr + i) * sb] += phi_b[i];
+    gh_package_seed(sp.lam, bp_b, bq_b, B);
+    gh_package_vjp(sp.n, sp.nv, sp.lam, gamma2_p, u_p, su, pr, B, ubar_p, sb, n_b, nv_b, lam_b);
   }
   // fac_p = lift |n_p|; the normals and speeds of p's side depend on g(p)
   double gm_b[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};

@@ -62,6 +62,13 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
                      const int32_t* nbr, int np_max, int kf, int km, const double* u,
                      const double* params, const double* inv_jac, double* dt, double* scratch_du,
                      int threads, hipStream_t stream);
+// Adjoint of the GH RHS on a mixed-p domain, linearised at u: ubar =
+// (d dt / d u)^T gbar over the work list; scratch_du = 150 x total points.
+int mixed_face_adjoint_lds_bytes(int kf, int km);
+int launch_mixed_gh_adjoint(int count, const int32_t* list, const MixedElement* el,
+                            const int32_t* nbr, int np_max, int kf, int km, const double* u,
+                            const double* gammas, const double* inv_jac, const double* gbar,
+                            double* ubar, double* scratch_du, hipStream_t stream);
 // Adjoint (transpose) of the ScalarWave RHS on a uniform domain (isotropic
 // extents 2..10) and of one RK3 substep.
 bool sw_adjoint_supported(const int32_t ext[3]);

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "device_common.hpp"
+#include "gh_adjoint.hpp"
 #include "gh_point.hpp"
 #include "kernels.hpp"
 #include "sw_point.hpp"
@@ -426,6 +427,352 @@ mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restrict
   }
 }
 
+// ---------------------------------------------------------------------------
+// Generalized harmonic adjoint on mixed-p domains (the RHS linearised at u):
+// the kernels of the uniform adjoint with per-element extents, and a face
+// kernel that transposes the mortar coupling.
+//
+// Point-wise adjoint: ubar = (df/du)^T gbar, and the logical-derivative
+// adjoints xbar written over du (read first, same point).
+__global__ void __launch_bounds__(256)
+gh_mixed_adjoint_pointwise_kernel(const MixedElement* __restrict__ el,
+                                  const int32_t* __restrict__ list, const double* __restrict__ u,
+                                  double* du_xbar, const double* __restrict__ gammas,
+                                  const double* __restrict__ inv_jac,
+                                  const double* __restrict__ gbar, double* __restrict__ ubar) {
+  const MixedElement m = el[list[blockIdx.x]];
+  const int np = m.np;
+  for (int p = threadIdx.x; p < np; p += blockDim.x) {
+    const double* gp = gbar + 50 * m.off + p;
+    const double* ga = gammas + 3 * m.off + p;
+    const double gamma1 = ga[np], gamma2 = ga[2 * np];
+    double ub[50];
+    GhVjpPoint pt;
+    gh_time_derivative_vjp(u + 50 * m.off + p, np, du_xbar + 150 * m.off + p, np, ga[0], gamma1,
+                           gamma2, gp, np, ub, &pt);
+#pragma unroll
+    for (int c = 0; c < 50; ++c) ubar[50 * m.off + c * np + p] = ub[c];
+    double ij[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) ij[c] = inv_jac[9 * m.off + c * np + p];
+    for (int c = 0; c < 50; ++c) {
+      double dub[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        dub[i] = gh_time_derivative_dubar(pt, gamma1, gamma2, gp, np, 3 * c + i);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) s += ij[ij3(j, i)] * dub[i];
+        du_xbar[150 * m.off + (3 * c + j) * np + p] = s;
+      }
+    }
+  }
+}
+
+// ubar_c += sum_j D_j^T xbar_cj per element.  Grid: (list length, component
+// chunks of kRaggedChunk); dynamic LDS: 3 Np_max doubles.
+__global__ void __launch_bounds__(256)
+ragged_transposed_sweeps_kernel(const MixedElement* __restrict__ el,
+                                const int32_t* __restrict__ list, int nc,
+                                const double* __restrict__ xbar, double* __restrict__ ubar,
+                                const double* __restrict__ dtab) {
+  extern __shared__ double s_x[];
+  const MixedElement m = el[list[blockIdx.x]];
+  const int np = m.np;
+  const int stride[3] = {1, m.n[0], m.n[0] * m.n[1]};
+  const double* D[3] = {diff_matrix(dtab, m.n[0]), diff_matrix(dtab, m.n[1]),
+                        diff_matrix(dtab, m.n[2])};
+  const int c_end = min(nc, static_cast<int>(blockIdx.y + 1) * kRaggedChunk);
+  for (int c = blockIdx.y * kRaggedChunk; c < c_end; ++c) {
+    __syncthreads();
+    for (int x = threadIdx.x; x < 3 * np; x += blockDim.x)
+      s_x[x] = xbar[3 * nc * m.off + 3 * c * np + x];
+    __syncthreads();
+    for (int p = threadIdx.x; p < np; p += blockDim.x) {
+      const int idx[3] = {p % m.n[0], (p / m.n[0]) % m.n[1], p / (m.n[0] * m.n[1])};
+      double acc = ubar[nc * m.off + c * np + p];
+      for (int j = 0; j < 3; ++j) {
+        const int base = p - idx[j] * stride[j];
+        for (int l = 0; l < m.n[j]; ++l)
+          acc = fma(D[j][l * m.n[j] + idx[j]], s_x[j * np + base + l * stride[j]], acc);
+      }
+      ubar[nc * m.off + c * np + p] = acc;
+    }
+  }
+}
+
+// Transpose of apply_face_matrices: in [nc][oa ob] -> out [nc][ia ib] with
+// Ma [oa][ia] (skipped when ia == oa) and Mb [ob][ib] (skipped when
+// ib == ob), dimension b first, through tmp [nc][oa ib].  Includes the
+// barriers between the steps.
+__device__ void apply_face_matrices_t(int nc, int oa, int ob, int ia, int ib, const double* Ma,
+                                      const double* Mb, const double* in, double* tmp,
+                                      double* out) {
+  const int t = threadIdx.x, T = blockDim.x;
+  const double* src = in;
+  if (ib != ob) {
+    for (int x = t; x < nc * oa * ib; x += T) {
+      const int c = x / (oa * ib), rem = x % (oa * ib), lb = rem / oa, ra = rem % oa;
+      const double* v = in + c * oa * ob + ra;
+      double s = Mb[lb] * v[0];
+      for (int r = 1; r < ob; ++r) s = fma(Mb[r * ib + lb], v[oa * r], s);
+      tmp[x] = s;
+    }
+    __syncthreads();
+    src = tmp;
+  }
+  for (int x = t; x < nc * ia * ib; x += T) {
+    const int c = x / (ia * ib), rem = x % (ia * ib), lb = rem / ia, la = rem % ia;
+    const double* v = src + c * oa * ib + oa * lb;
+    double s;
+    if (ia == oa) {
+      s = v[la];
+    } else {
+      s = Ma[la] * v[0];
+      for (int r = 1; r < oa; ++r) s = fma(Ma[r * ia + la], v[r], s);
+    }
+    out[x] = s;
+  }
+  __syncthreads();
+}
+
+// LDS plan of the face adjoint kernel (doubles): per face point of either
+// side 12 side values (n, nv, lam, gamma2, |n|); per own face point 11
+// adjoint accumulators (n, nv, lam, |n|) and 5 corrections; 13-component
+// package buffers on the faces and on the mortar, reused once free.
+struct FaceAdjPlan {
+  int side_own, side_nb, acc, pk_own, pk_nb, bc, tmp, m_own, m_nb, lam_own, lam_nb, bc_m, total;
+  __host__ __device__ FaceAdjPlan(int kf, int km) {
+    side_own = 0;
+    side_nb = side_own + 12 * kf;
+    acc = side_nb + 12 * kf;
+    pk_own = acc + 11 * kf;   // then the package adjoint on the own face
+    pk_nb = pk_own + 13 * kf;  // then the seeds on both faces (5 + 5)
+    bc = pk_nb + 13 * kf;
+    tmp = bc + 5 * kf;
+    m_own = tmp + 13 * km;  // then the package adjoint on the mortar
+    m_nb = m_own + 13 * km;  // then the seeds on the mortar (5 + 5)
+    lam_own = m_nb + 13 * km;
+    lam_nb = lam_own + 4 * km;
+    bc_m = lam_nb + 4 * km;
+    total = bc_m + 5 * km;
+  }
+};
+
+// Side values of face point f of element m's face (d, side); the raw normal
+// is negated when `flip`.
+__device__ void face_side_adjoint(const MixedElement& m, int d, int side, bool flip, int f,
+                                  const double* __restrict__ u, const double* __restrict__ params,
+                                  const double* __restrict__ inv_jac, GhSideFwd& s, int* point) {
+  const int na = d == 0 ? m.n[1] : m.n[0];
+  const int p = face_point(m, d, side, f % na, f / na);
+  double n[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    n[i] = inv_jac[9 * m.off + ij3(d, i) * m.np + p];
+    if (flip) n[i] = n[i] * -1.0;
+  }
+  gh_side_forward(s, u + 50 * m.off + p, m.np, params[3 * m.off + m.np + p], n);
+  *point = p;
+}
+
+// Face adjoint: one workgroup per element e adds to e's face points the
+// adjoints of (A) e's own lifted corrections and (B) the neighbours'
+// corrections on the shared faces, in which e's points are the exterior
+// side.  Per face and pair pr the forward is recomputed up to the mortar
+// corrections, then
+//   seeds   gbar * lift_factor on both faces (the neighbour's with its own),
+//   P^T     each seed projected back onto the mortar (transposed projection),
+//   weights the adjoint of e's package on the mortar, interior in (A) and
+//           exterior in (B), from the upwind weights of the mortar speeds,
+//   I^T     transposed interpolation onto e's face,
+//   package reverse at e's face points into ubar and the side adjoints,
+// and after the ten pairs the adjoint of e's normals and speeds into g.
+// Dynamic LDS: FaceAdjPlan(kf, km).total doubles.
+__global__ void __launch_bounds__(256)
+gh_mixed_face_adjoint_kernel(const MixedElement* __restrict__ el,
+                             const int32_t* __restrict__ list, const int32_t* __restrict__ nbr,
+                             const double* __restrict__ u, const double* __restrict__ params,
+                             const double* __restrict__ inv_jac, const double* __restrict__ gbar,
+                             double* __restrict__ ubar, const double* __restrict__ mortar_tab,
+                             int kf, int km) {
+  extern __shared__ double sm[];
+  const FaceAdjPlan L(kf, km);
+  const int t = threadIdx.x, T = blockDim.x;
+  const int e = list[blockIdx.x];
+  const MixedElement m = el[e];
+  for (int dir = 0; dir < 6; ++dir) {
+    const int d = dir >> 1, side = dir & 1;
+    const MixedElement mn = el[nbr[6 * e + dir]];
+    const int na = d == 0 ? m.n[1] : m.n[0], nb = d == 2 ? m.n[1] : m.n[2];
+    const int ma = d == 0 ? mn.n[1] : mn.n[0], mb = d == 2 ? mn.n[1] : mn.n[2];
+    const int ka = max(na, ma), kb = max(nb, mb);
+    const int nf = na * nb, mf = ma * mb, kmf = ka * kb;
+    const double* Ia_own = mortar_tab + mortar_table_offset(na, ka);
+    const double* Ib_own = mortar_tab + mortar_table_offset(nb, kb);
+    const double* Ia_nb = mortar_tab + mortar_table_offset(ma, ka);
+    const double* Ib_nb = mortar_tab + mortar_table_offset(mb, kb);
+    const double* Pa = Ia_own + na * ka;  // projection block follows the interpolation
+    const double* Pb = Ib_own + nb * kb;
+    const double* Pa_nb = Ia_nb + ma * ka;
+    const double* Pb_nb = Ib_nb + mb * kb;
+    const double lift_own = lift_factor(m.n[d], 1.0), lift_nb = lift_factor(mn.n[d], 1.0);
+    __syncthreads();  // previous face done with the LDS
+    for (int f = t; f < nf; f += T) {
+      GhSideFwd s;
+      int p;
+      face_side_adjoint(m, d, side, side == 0, f, u, params, inv_jac, s, &p);
+      double* o = sm + L.side_own + 12 * f;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        o[i] = s.n[i];
+        o[3 + i] = s.nv[i];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) o[6 + k] = s.lam[k];
+      o[10] = params[3 * m.off + 2 * m.np + p];
+      o[11] = s.mag;
+#pragma unroll
+      for (int c = 0; c < 11; ++c) sm[L.acc + 11 * f + c] = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sm[L.pk_own + k * nf + f] = s.lam[k];
+    }
+    for (int f = t; f < mf; f += T) {
+      GhSideFwd s;
+      int q;
+      face_side_adjoint(mn, d, 1 - side, side == 1, f, u, params, inv_jac, s, &q);
+      double* o = sm + L.side_nb + 12 * f;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        o[i] = s.n[i];
+        o[3 + i] = s.nv[i];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) o[6 + k] = s.lam[k];
+      o[10] = params[3 * mn.off + 2 * mn.np + q];
+      o[11] = s.mag;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) sm[L.pk_nb + k * mf + f] = s.lam[k];
+    }
+    __syncthreads();
+    // speeds on the mortar (their upwind weights)
+    apply_face_matrices(4, na, nb, ka, kb, Ia_own, Ib_own, sm + L.pk_own, sm + L.tmp,
+                        sm + L.lam_own);
+    apply_face_matrices(4, ma, mb, ka, kb, Ia_nb, Ib_nb, sm + L.pk_nb, sm + L.tmp,
+                        sm + L.lam_nb);
+    for (int pr = 0; pr < 10; ++pr) {
+      const int comp[5] = {pr, 10 + pr, 20 + 3 * pr, 21 + 3 * pr, 22 + 3 * pr};
+      // forward: packages, mortar, corrections of e's face
+      for (int f = t; f < nf; f += T) {
+        const double* o = sm + L.side_own + 12 * f;
+        const int p = face_point(m, d, side, f % na, f / na);
+        double pk[13];
+        gh_package_fwd(o, o + 3, o + 6, o[10], u + 50 * m.off + p, m.np, pr, pk);
+#pragma unroll
+        for (int c = 0; c < 13; ++c) sm[L.pk_own + c * nf + f] = pk[c];
+      }
+      for (int f = t; f < mf; f += T) {
+        const double* o = sm + L.side_nb + 12 * f;
+        const int q = face_point(mn, d, 1 - side, f % ma, f / ma);
+        double pk[13];
+        gh_package_fwd(o, o + 3, o + 6, o[10], u + 50 * mn.off + q, mn.np, pr, pk);
+#pragma unroll
+        for (int c = 0; c < 13; ++c) sm[L.pk_nb + c * mf + f] = pk[c];
+      }
+      __syncthreads();
+      apply_face_matrices(13, na, nb, ka, kb, Ia_own, Ib_own, sm + L.pk_own, sm + L.tmp,
+                          sm + L.m_own);
+      apply_face_matrices(13, ma, mb, ka, kb, Ia_nb, Ib_nb, sm + L.pk_nb, sm + L.tmp,
+                          sm + L.m_nb);
+      for (int k = t; k < kmf; k += T) {
+        double in[13], ex[13], li[4], le[4], bc[5];
+#pragma unroll
+        for (int c = 0; c < 13; ++c) {
+          in[c] = sm[L.m_own + c * kmf + k];
+          ex[c] = sm[L.m_nb + c * kmf + k];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          li[c] = sm[L.lam_own + c * kmf + k];
+          le[c] = sm[L.lam_nb + c * kmf + k];
+        }
+        gh_boundary_fwd(in, li, ex, le, bc);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) sm[L.bc_m + c * kmf + k] = bc[c];
+      }
+      __syncthreads();
+      apply_face_matrices(5, ka, kb, na, nb, Pa, Pb, sm + L.bc_m, sm + L.tmp, sm + L.bc);
+      // seeds on both faces; |n| adjoint of e's lift factor
+      double* seed_own = sm + L.pk_nb;
+      double* seed_nb = sm + L.pk_nb + 5 * kf;
+      for (int f = t; f < nf; f += T) {
+        const int p = face_point(m, d, side, f % na, f / na);
+        const double fac = lift_own * sm[L.side_own + 12 * f + 11];
+        double fac_b = 0.0;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+          const double gb = gbar[50 * m.off + comp[c] * m.np + p];
+          fac_b += gb * sm[L.bc + c * nf + f];
+          seed_own[c * nf + f] = gb * fac;
+        }
+        sm[L.acc + 11 * f + 10] += lift_own * fac_b;
+      }
+      for (int f = t; f < mf; f += T) {
+        const int q = face_point(mn, d, 1 - side, f % ma, f / ma);
+        const double fac = lift_nb * sm[L.side_nb + 12 * f + 11];
+#pragma unroll
+        for (int c = 0; c < 5; ++c)
+          seed_nb[c * mf + f] = gbar[50 * mn.off + comp[c] * mn.np + q] * fac;
+      }
+      __syncthreads();
+      double* a_m = sm + L.m_nb;
+      double* b_m = sm + L.m_nb + 5 * km;
+      apply_face_matrices_t(5, na, nb, ka, kb, Pa, Pb, seed_own, sm + L.tmp, a_m);
+      apply_face_matrices_t(5, ma, mb, ka, kb, Pa_nb, Pb_nb, seed_nb, sm + L.tmp, b_m);
+      for (int k = t; k < kmf; k += T) {
+        double li[4], bi[5], bx[5], B[13];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) li[c] = sm[L.lam_own + c * kmf + k];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+          bi[c] = a_m[c * kmf + k];
+          bx[c] = b_m[c * kmf + k];
+        }
+        gh_package_seed(li, bi, bx, B);
+#pragma unroll
+        for (int c = 0; c < 13; ++c) sm[L.m_own + c * kmf + k] = B[c];
+      }
+      __syncthreads();
+      apply_face_matrices_t(13, ka, kb, na, nb, Ia_own, Ib_own, sm + L.m_own, sm + L.tmp,
+                            sm + L.pk_own);
+      for (int f = t; f < nf; f += T) {
+        const double* o = sm + L.side_own + 12 * f;
+        double* a = sm + L.acc + 11 * f;
+        const int p = face_point(m, d, side, f % na, f / na);
+        double B[13];
+#pragma unroll
+        for (int c = 0; c < 13; ++c) B[c] = sm[L.pk_own + c * nf + f];
+        gh_package_vjp(o, o + 3, o + 6, o[10], u + 50 * m.off + p, m.np, pr, B,
+                       ubar + 50 * m.off + p, m.np, a, a + 3, a + 6);
+      }
+      __syncthreads();
+    }
+    // normals, speeds and |n| of e's side depend on g
+    for (int f = t; f < nf; f += T) {
+      GhSideFwd s;
+      int p;
+      face_side_adjoint(m, d, side, side == 0, f, u, params, inv_jac, s, &p);
+      const double* a = sm + L.acc + 11 * f;
+      double gm_b[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      gh_side_vjp(s, params[3 * m.off + m.np + p], a, a + 3, a + 6, a[10], gm_b);
+#pragma unroll
+      for (int c = 0; c < 10; ++c) ubar[50 * m.off + c * m.np + p] += gm_b[c];
+    }
+  }
+}
+
 // Dynamic LDS beyond the default 64 KiB has to be enabled per kernel; done
 // once per device (before any graph capture of these launches).
 bool enable_large_lds() {
@@ -442,6 +789,10 @@ bool enable_large_lds() {
       hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_face_kernel<false>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
       hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_face_kernel<true>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&ragged_transposed_sweeps_kernel),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&gh_mixed_face_adjoint_kernel),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess)
     return false;
   done[dev] = true;
@@ -483,6 +834,35 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   hipLaunchKernelGGL(mixed_face_kernel<true>, dim3(count), dim3(threads), face_lds, stream, el,
                      list, nbr, u, params, inv_jac, dt, mtab, kf, km);
+  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
+}
+
+int mixed_face_adjoint_lds_bytes(int kf, int km) { return FaceAdjPlan(kf, km).total * 8; }
+
+int launch_mixed_gh_adjoint(int count, const int32_t* list, const MixedElement* el,
+                            const int32_t* nbr, int np_max, int kf, int km, const double* u,
+                            const double* gammas, const double* inv_jac, const double* gbar,
+                            double* ubar, double* scratch_du, hipStream_t stream) {
+  if (count == 0) return SXD_OK;
+  const double* dtab = device_diff_matrices();
+  const double* mtab = device_mortar_matrices();
+  if (dtab == nullptr || mtab == nullptr) return SXD_ERR_DEVICE;
+  const int face_lds = mixed_face_adjoint_lds_bytes(kf, km);
+  if (face_lds > 160 * 1024 || 3 * np_max * 8 > 160 * 1024) return SXD_ERR_UNSUPPORTED;
+  if (scratch_du == nullptr) return SXD_ERR_INVALID_ARGUMENT;
+  if (!enable_large_lds()) return SXD_ERR_DEVICE;
+  const dim3 chunks(count, (50 + kRaggedChunk - 1) / kRaggedChunk);
+  hipLaunchKernelGGL(ragged_partial_derivatives_kernel, chunks, dim3(256), np_max * 8, stream, el,
+                     list, 50, u, inv_jac, scratch_du, dtab);
+  if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  hipLaunchKernelGGL(gh_mixed_adjoint_pointwise_kernel, dim3(count), dim3(256), 0, stream, el,
+                     list, u, scratch_du, gammas, inv_jac, gbar, ubar);
+  if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  hipLaunchKernelGGL(ragged_transposed_sweeps_kernel, chunks, dim3(256), 3 * np_max * 8, stream,
+                     el, list, 50, scratch_du, ubar, dtab);
+  if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  hipLaunchKernelGGL(gh_mixed_face_adjoint_kernel, dim3(count), dim3(256), face_lds, stream, el,
+                     list, nbr, u, gammas, inv_jac, gbar, ubar, mtab, kf, km);
   return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
 }
 

@@ -443,6 +443,26 @@ def test_gh_rhs_adjoint_matches_oracle_tape(elements, extents):
     assert err <= 1e-12
 
 
+@pytest.mark.parametrize("elements,lowest,highest", [((2, 2, 1), 3, 5), ((1, 2, 2), 4, 6)])
+def test_gh_rhs_adjoint_mixed_p_matches_oracle_tape(elements, lowest, highest):
+    """Mixed p: the GH adjoint kernels (transposed mortar interpolation and
+    projection included) against the oracle's reverse-mode tape of its mixed
+    GH RHS."""
+    brick, u, gammas, inv = wl.gh_kerr_schild_mixed(seed=4, elements=elements, lowest=lowest,
+                                                    highest=highest, noise=1e-3,
+                                                    lower=(3.0, -4.0, 0.5),
+                                                    upper=(11.0, 4.0, 8.5))
+    assert len({tuple(x) for x in brick.extents.tolist()}) > 1
+    rng = np.random.default_rng(10)
+    gbar = rng.standard_normal(u.shape)
+    ref = ol.gh_rhs_mixed_vjp(brick.extents, brick.neighbors, u, gammas, inv, gbar)
+    domain = dg.MixedDomain.from_brick(brick)
+    got = host(domain.gh_rhs_adjoint(dev(u), dev(gammas), dev(inv), dev(gbar)))
+    err = rel_err(got, ref)
+    print(f"mixed-p GH adjoint vs tape: max-norm relative error {err:.3e}")
+    assert err <= 1e-12
+
+
 def _gh_gradient(brick, u, gammas, inv, dt, steps, k):
     from spectre_amd import evolution as ev
     domain = dg.Domain.from_brick(brick)
