@@ -182,20 +182,33 @@ def measure_adjoint(steps=40, reps=3):
             "adjoint_forward_cost_with_recompute": round(min(bwd_rc) / f, 3)}
 
 
-def measure_gh_adjoint(steps=20, reps=2):
-    """Adjoint GH workload (300 elements x 6^3, uniform, 20 RK3 steps, J = L2
-    norm of the gauge constraint): forward evolution against the backward
-    (recompute of each 8-step segment from its checkpoint + hand-written
-    adjoint kernels)."""
+def measure_gh_adjoint(steps=20, reps=2, mixed=False):
+    """Adjoint GH workload (300 elements, 20 RK3 steps, J = L2 norm of the
+    gauge constraint): forward evolution against the backward (recompute of
+    each 8-step segment from its checkpoint + hand-written adjoint kernels).
+    Uniform: 6^3 points per element; mixed: extents drawn from {6..10} per
+    element and dimension."""
     import numpy as np
     import torch
     from spectre_amd import dg, evolution as ev, workloads as wl
 
-    brick, u, gammas, inv, dt = wl.gh_adjoint_workload()
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
-    domain = dg.Domain.from_brick(brick)
-    d_g, d_inv = dev(gammas), dev(inv)
-    mass = ev.mass_weights(brick.extents, d_inv, brick.num_elements)
+    if mixed:
+        brick, u, gammas, inv, dt = wl.gh_adjoint_workload_mixed()
+        domain = dg.MixedDomain.from_brick(brick)
+        d_g, d_inv = dev(gammas), dev(inv)
+        index = ev.ragged_point_index(brick.extents, 50).to(d_inv.device)
+        mass = ev.ragged_mass_weights(brick.extents, d_inv)
+        functional = lambda uf: ev.ragged_gauge_constraint_norm(uf, index, mass)
+        points, name = brick.total_points, "gh_adjoint_300el_mixed_6..10_20_rk3_steps_gauge_constraint"
+    else:
+        brick, u, gammas, inv, dt = wl.gh_adjoint_workload()
+        domain = dg.Domain.from_brick(brick)
+        d_g, d_inv = dev(gammas), dev(inv)
+        mass = ev.mass_weights(brick.extents, d_inv, brick.num_elements)
+        functional = lambda uf: ev.gauge_constraint_norm(uf, mass)
+        points, name = brick.num_elements * brick.num_points, \
+            "gh_adjoint_300el_6^3_20_rk3_steps_gauge_constraint"
 
     def run():
         u0 = dev(u).requires_grad_(True)
@@ -204,7 +217,7 @@ def measure_gh_adjoint(steps=20, reps=2):
         t0.record()
         uf = ev.GhEvolution.apply(u0, domain, d_g, d_inv, dt, steps, 8)
         t1.record()
-        J = ev.gauge_constraint_norm(uf, mass)
+        J = functional(uf)
         J.backward()
         t2.record()
         torch.cuda.synchronize()
@@ -217,8 +230,7 @@ def measure_gh_adjoint(steps=20, reps=2):
     rhs_ms = time_calls(lambda: domain.gh_rhs(u_d, d_g, d_inv, out=out), 5, 1)
     adj_ms = time_calls(lambda: domain.gh_rhs_adjoint(u_d, d_g, d_inv, gb, out=out), 3, 1)
     f = min(fwd)
-    return {"workload": "gh_adjoint_300el_6^3_20_rk3_steps_gauge_constraint",
-            "grid_points": brick.num_elements * brick.num_points, "steps": steps, "dt": dt,
+    return {"workload": name, "grid_points": int(points), "steps": steps, "dt": dt,
             "forward_ms": round(f, 3), "backward_with_recompute_ms": round(min(bwd), 3),
             "adjoint_forward_cost": round(min(bwd) / f, 3),
             "rhs_ms": round(rhs_ms, 4), "adjoint_rhs_ms": round(adj_ms, 4)}
@@ -364,7 +376,8 @@ def main():
     if args.extensions:
         extensions = {"mixed_p": [measure_mixed(s, 20, 3) for s in ("scalar_wave", "gh")],
                       "adjoint": measure_adjoint(),
-                      "gh_adjoint": measure_gh_adjoint()}
+                      "gh_adjoint": measure_gh_adjoint(),
+                      "gh_adjoint_mixed": measure_gh_adjoint(mixed=True)}
 
     result = {
         "metric": METRIC,

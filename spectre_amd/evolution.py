@@ -216,6 +216,43 @@ def mass_weights(domain_extents, inv_jac, num_elements):
     return wq[None, :] / det_inv
 
 
+def ragged_point_index(extents, nc):
+    """Gather index of a ragged field (element e's nc blocks of Np_e points at
+    nc * offset_e) as [nc][total points], points element by element:
+    u_flat[index] is the field point-major, for point-wise functionals."""
+    ext = np.asarray(extents, dtype=np.int64).reshape(-1, 3)
+    points = np.prod(ext, axis=1)
+    offsets = np.concatenate([[0], np.cumsum(points)])
+    index = np.empty((nc, int(offsets[-1])), dtype=np.int64)
+    comps = np.arange(nc)[:, None]
+    for e, n in enumerate(points):
+        index[:, offsets[e]:offsets[e + 1]] = nc * offsets[e] + comps * n + np.arange(n)[None, :]
+    return torch.from_numpy(index)
+
+
+def ragged_mass_weights(extents, inv_jac):
+    """mass_weights of a mixed-p domain: [total points] in the ragged point
+    order (inv_jac: flat ragged field of 9 components)."""
+    from . import spectral
+    ext = np.asarray(extents, dtype=np.int64).reshape(-1, 3)
+    out, off = [], 0
+    for n0, n1, n2 in ext:
+        n = int(n0 * n1 * n2)
+        w = [spectral.lgl_weights(int(k)) for k in (n0, n1, n2)]
+        wk, wj, wi = np.meshgrid(w[2], w[1], w[0], indexing="ij")
+        m = inv_jac[9 * off:9 * (off + n)].reshape(3, 3, n)
+        det_inv = torch.linalg.det(m.permute(2, 0, 1))
+        out.append(torch.from_numpy((wi * wj * wk).ravel()).to(inv_jac.device) / det_inv)
+        off += n
+    return torch.cat(out)
+
+
+def ragged_gauge_constraint_norm(u, index, mass):
+    """gauge_constraint_norm of a ragged GH state u (flat), with the gather
+    index of ragged_point_index(extents, 50) and ragged_mass_weights."""
+    return gauge_constraint_norm(u[index][None], mass[None])
+
+
 def psi_energy(u, mass):
     """J = sum over points of M psi^2 (psi = component 0)."""
     return torch.sum(mass * u[:, 0, :] ** 2)

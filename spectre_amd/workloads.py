@@ -456,9 +456,10 @@ def cfl_step(brick, safety, speed):
     """Step size of the Cfl step chooser (Time/StepChoosers/Cfl.hpp:69-75):
     safety * stable_step * minimum_grid_spacing / (speed * 3), with the
     minimum grid spacing of an affine axis-aligned element the smallest LGL
-    node spacing times h_d / 2."""
+    node spacing times h_d / 2 (on a mixed-p brick, over all elements)."""
+    ext = np.asarray(brick.extents).reshape(-1, 3)
     spacing = min(0.5 * brick.widths[d] * float(np.min(np.diff(spectral.lgl_points(
-        brick.extents[d])))) for d in range(3))
+        int(np.max(ext[:, d])))))) for d in range(3))
     return safety * rk3_stable_step() * spacing / (speed * 3.0)
 
 
@@ -468,6 +469,21 @@ def scalar_wave_adjoint_workload(seed=0, elements=(8, 8, 8), extents=(6, 6, 6)):
     Returns (brick, u, gamma2, inv_jac, dt)."""
     brick, u, gamma2, inv = scalar_wave_plane_wave_3d(seed, elements, extents)
     return brick, u, gamma2, inv, cfl_step(brick, 0.25, 1.0)
+
+
+def gh_adjoint_workload_mixed(seed=0, elements=(6, 5, 10), lowest=6, highest=10, noise=1.0e-6,
+                              lower=(3.0, -4.0, 0.5), upper=(11.0, 4.0, 8.5)):
+    """Mixed-p adjoint GH workload: the Kerr-Schild state of
+    gh_kerr_schild_mixed on Ex x Ey x Ez elements (300 by default) with
+    extents drawn from {lowest..highest}, faces off the plane z = 0 as in
+    gh_adjoint_workload, step size at CFL 0.25.  Returns (brick, u, gammas,
+    inv_jac, dt) in the ragged layout."""
+    brick, u, gammas, inv = gh_kerr_schild_mixed(seed, elements, lowest, highest, noise, lower,
+                                                 upper)
+    comps = np.concatenate(brick.split(u, 50), axis=1)
+    lapse, shift, _ = _lapse_shift_inverse([comps[p] for p in range(10)])
+    speed = float(np.max(np.abs(lapse) + np.sqrt(np.sum(shift ** 2, axis=0))))
+    return brick, u, gammas, inv, cfl_step(brick, 0.25, speed)
 
 
 def gh_adjoint_workload(seed=0, elements=(6, 5, 10), extents=(6, 6, 6), noise=1.0e-6,

@@ -45,3 +45,26 @@ def test_gauge_constraint_is_the_christoffel_trace():
     got = ev.gauge_constraint(torch.from_numpy(u)).numpy()
     ref = direct_trace(u)
     assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def test_ragged_functional_equals_uniform_on_equal_extents():
+    """On a brick whose elements all have the same extents the ragged layout
+    is the uniform one: the ragged gather, mass weights and functional agree
+    with the uniform ones; the mixed-p mass weights integrate 1 to the volume."""
+    brick, u, gammas, inv, _ = wl.gh_adjoint_workload(seed=2, elements=(2, 1, 2),
+                                                      extents=(3, 4, 5), noise=1e-3)
+    tu, tinv = torch.from_numpy(u), torch.from_numpy(inv)
+    ext = np.tile(np.array(brick.extents, dtype=np.int32), (brick.num_elements, 1))
+    index = ev.ragged_point_index(ext, 50)
+    mass_r = ev.ragged_mass_weights(ext, tinv.reshape(-1))
+    mass_u = ev.mass_weights(brick.extents, tinv, brick.num_elements)
+    assert torch.equal(tu.reshape(-1)[index], tu.permute(1, 0, 2).reshape(50, -1))
+    assert torch.allclose(mass_r, mass_u.reshape(-1), rtol=1e-15, atol=0)
+    j_r = float(ev.ragged_gauge_constraint_norm(tu.reshape(-1), index, mass_r))
+    j_u = float(ev.gauge_constraint_norm(tu, mass_u))
+    assert abs(j_r - j_u) <= 1e-14 * j_u
+    mb, _, _, minv, _ = wl.gh_adjoint_workload_mixed(seed=1, elements=(2, 2, 1), lowest=3,
+                                                     highest=5, noise=1e-3)
+    volume = np.prod(np.subtract(mb.upper, mb.lower))
+    total = float(ev.ragged_mass_weights(mb.extents, torch.from_numpy(minv)).sum())
+    assert abs(total - volume) <= 1e-12 * volume
