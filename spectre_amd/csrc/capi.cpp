@@ -39,6 +39,7 @@ struct sxd_domain {
   sxd::MixedElement* d_el = nullptr;
   int32_t* d_list = nullptr;          // all elements, decreasing cost
   int32_t* d_buckets = nullptr;       // elements grouped by extents
+  int32_t* d_point_el = nullptr;      // element of each grid point
   std::vector<std::pair<int, int>> buckets;  // (start, count) in d_buckets
 };
 
@@ -395,6 +396,12 @@ int sxd_domain_create_mixed(sxd_domain** domain, int num_elements, const int32_t
   };
   upload(&d->d_nbr, neighbors, 6 * static_cast<size_t>(num_elements));
   upload(&d->d_el, el.data(), el.size());
+  {
+    std::vector<int32_t> point_el(static_cast<size_t>(off));
+    for (int e = 0; e < num_elements; ++e)
+      std::fill(point_el.begin() + el[e].off, point_el.begin() + el[e].off + el[e].np, e);
+    upload(&d->d_point_el, point_el.data(), point_el.size());
+  }
   upload(&d->d_list, list.data(), list.size());
   upload(&d->d_buckets, grouped.data(), grouped.size());
   if (rc == SXD_OK && (device_diff_matrices() == nullptr || device_mortar_matrices() == nullptr))
@@ -438,6 +445,7 @@ int sxd_domain_get_threads(const sxd_domain* domain, int system, int* threads) {
 int sxd_domain_destroy(sxd_domain* domain) {
   if (domain == nullptr) return ok();
   if (domain->d_el != nullptr) (void)hipFree(domain->d_el);
+  if (domain->d_point_el != nullptr) (void)hipFree(domain->d_point_el);
   if (domain->d_list != nullptr) (void)hipFree(domain->d_list);
   if (domain->d_buckets != nullptr) (void)hipFree(domain->d_buckets);
   if (domain->d_nbr != nullptr) (void)hipFree(domain->d_nbr);
@@ -710,7 +718,8 @@ int sxd_gh_rhs_adjoint(const sxd_domain* domain, const double* u, const double* 
   if (d->mixed) {
     if (ensure_scratch(d, 150 * static_cast<size_t>(d->total_points), 0) != SXD_OK)
       return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
-    return finish(launch_mixed_gh_adjoint(d->num_elements, d->d_list, d->d_el, d->d_nbr,
+    return finish(launch_mixed_gh_adjoint(d->num_elements, d->d_list, d->d_el, d->d_point_el,
+                                          d->total_points, d->d_nbr,
                                           d->np_max, d->kf, d->km, u, gammas, inv_jac, gbar, ubar,
                                           d->d_scratch_du, static_cast<hipStream_t>(stream)),
                   __func__);

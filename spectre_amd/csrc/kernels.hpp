@@ -65,7 +65,9 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
 // Adjoint of the GH RHS on a mixed-p domain, linearised at u: ubar =
 // (d dt / d u)^T gbar over the work list; scratch_du = 150 x total points.
 int mixed_face_adjoint_lds_bytes(int kf, int km);
+// point_el: the element of each of the total_points grid points.
 int launch_mixed_gh_adjoint(int count, const int32_t* list, const MixedElement* el,
+                            const int32_t* point_el, int64_t total_points,
                             const int32_t* nbr, int np_max, int kf, int km, const double* u,
                             const double* gammas, const double* inv_jac, const double* gbar,
                             double* ubar, double* scratch_du, hipStream_t stream);

@@ -12,8 +12,10 @@
 // launches the same kernels once per group of equal extents.
 //
 // Face coupling (ApplyBoundaryCorrections.hpp:797-907 with
-// InternalMortarDataImpl.hpp:230-318): for each face of an element, in the
-// order lower/upper xi, eta, zeta, one workgroup
+// InternalMortarDataImpl.hpp:230-318), each element's faces added into dt in
+// the order lower/upper xi, eta, zeta (ScalarWave: one workgroup per element
+// runs its six faces; GH: one launch per dimension, one workgroup per face
+// and symmetric pair):
 //   1. packages both sides on their own face meshes (UpwindPenalty
 //      dg_package_data; the neighbour's side from the neighbour's fields),
 //   2. interpolates both packages to the mortar mesh (max extents per
@@ -324,17 +326,15 @@ __device__ void face_package(const MixedElement& m, int d, int side, int f, int 
   for (int c = 0; c < 13; ++c) pk[c * nf + f] = out[c];
 }
 
-// Dynamic LDS: FacePlan(kf, km, 13).total doubles.
-template <bool GH>
+// ScalarWave faces: one workgroup per element, its six faces in direction
+// order.  Dynamic LDS: FacePlan(kf, km, 13).total doubles.
 __global__ void __launch_bounds__(256)
-mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restrict__ list,
-                  const int32_t* __restrict__ nbr, const double* __restrict__ u,
-                  const double* __restrict__ params, const double* __restrict__ inv_jac,
-                  double* __restrict__ dt, const double* __restrict__ mortar_tab, int kf,
-                  int km) {
+sw_mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restrict__ list,
+                     const int32_t* __restrict__ nbr, const double* __restrict__ u,
+                     const double* __restrict__ params, const double* __restrict__ inv_jac,
+                     double* __restrict__ dt, const double* __restrict__ mortar_tab, int kf,
+                     int km) {
   extern __shared__ double sm[];
-  constexpr int NC = GH ? 50 : 5;
-  constexpr int SIDE_OWN = GH ? 12 : 4, SIDE_NB = GH ? 11 : 4;
   const FacePlan L(kf, km, 13);
   const int t = threadIdx.x, T = blockDim.x;
   const int e = list[blockIdx.x];
@@ -354,75 +354,45 @@ mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restrict
     const double* Pb = Ib_own + nb * kb;
     __syncthreads();  // previous face done with the LDS
     for (int f = t; f < nf; f += T)
-      face_side<GH>(m, d, side, side == 0, f, u, params, inv_jac, sm + L.side_own + SIDE_OWN * f,
-                    true);
+      face_side<false>(m, d, side, side == 0, f, u, params, inv_jac, sm + L.side_own + 4 * f,
+                       true);
     for (int f = t; f < mf; f += T)
-      face_side<GH>(mn, d, 1 - side, side == 1, f, u, params, inv_jac,
-                    sm + L.side_nb + SIDE_NB * f, false);
+      face_side<false>(mn, d, 1 - side, side == 1, f, u, params, inv_jac,
+                       sm + L.side_nb + 4 * f, false);
     __syncthreads();
-    if constexpr (GH) {
-      // characteristic speeds on the mortar (part of the packaged data)
-      for (int f = t; f < nf; f += T)
+    for (int f = t; f < nf; f += T)
+      face_package<false>(m, d, side, f, nf, 0, u, params, sm + L.side_own + 4 * f,
+                          sm + L.pk_own);
+    for (int f = t; f < mf; f += T)
+      face_package<false>(mn, d, 1 - side, f, mf, 0, u, params, sm + L.side_nb + 4 * f,
+                          sm + L.pk_nb);
+    __syncthreads();
+    apply_face_matrices(13, na, nb, ka, kb, Ia_own, Ib_own, sm + L.pk_own, sm + L.tmp,
+                        sm + L.m_own);
+    apply_face_matrices(13, ma, mb, ka, kb, Ia_nb, Ib_nb, sm + L.pk_nb, sm + L.tmp,
+                        sm + L.m_nb);
+    for (int k = t; k < kmf; k += T) {
+      double in[13], ex[13], bc[5];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sm[L.pk_own + c * nf + f] = sm[L.side_own + 12 * f + 6 + c];
-      for (int f = t; f < mf; f += T)
+      for (int c = 0; c < 13; ++c) {
+        in[c] = sm[L.m_own + c * kmf + k];
+        ex[c] = sm[L.m_nb + c * kmf + k];
+      }
+      sw_boundary_terms(bc, in, ex);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sm[L.pk_nb + c * mf + f] = sm[L.side_nb + 11 * f + 6 + c];
-      __syncthreads();
-      apply_face_matrices(4, na, nb, ka, kb, Ia_own, Ib_own, sm + L.pk_own, sm + L.tmp,
-                          sm + L.lam_own);
-      apply_face_matrices(4, ma, mb, ka, kb, Ia_nb, Ib_nb, sm + L.pk_nb, sm + L.tmp,
-                          sm + L.lam_nb);
+      for (int c = 0; c < 5; ++c) sm[L.bc_m + c * kmf + k] = bc[c];
     }
-    constexpr int PASSES = GH ? 10 : 1;
-    for (int pr = 0; pr < PASSES; ++pr) {
-      for (int f = t; f < nf; f += T)
-        face_package<GH>(m, d, side, f, nf, pr, u, params, sm + L.side_own + SIDE_OWN * f,
-                         sm + L.pk_own);
-      for (int f = t; f < mf; f += T)
-        face_package<GH>(mn, d, 1 - side, f, mf, pr, u, params, sm + L.side_nb + SIDE_NB * f,
-                         sm + L.pk_nb);
-      __syncthreads();
-      apply_face_matrices(13, na, nb, ka, kb, Ia_own, Ib_own, sm + L.pk_own, sm + L.tmp,
-                          sm + L.m_own);
-      apply_face_matrices(13, ma, mb, ka, kb, Ia_nb, Ib_nb, sm + L.pk_nb, sm + L.tmp,
-                          sm + L.m_nb);
-      for (int k = t; k < kmf; k += T) {
-        double in[13], ex[13], bc[5];
+    __syncthreads();
+    apply_face_matrices(5, ka, kb, na, nb, Pa, Pb, sm + L.bc_m, sm + L.tmp, sm + L.bc);
+    // lift into dt
+    for (int f = t; f < nf; f += T) {
+      const int p = face_point(m, d, side, f % na, f / na);
+      const double fac = lift_factor(m.n[d], sm[L.side_own + 4 * f + 3]);
 #pragma unroll
-        for (int c = 0; c < 13; ++c) {
-          in[c] = sm[L.m_own + c * kmf + k];
-          ex[c] = sm[L.m_nb + c * kmf + k];
-        }
-        if constexpr (GH) {
-          double li[4], le[4];
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            li[c] = sm[L.lam_own + c * kmf + k];
-            le[c] = sm[L.lam_nb + c * kmf + k];
-          }
-          gh_boundary_terms_pair(bc, in, li, ex, le);
-        } else {
-          sw_boundary_terms(bc, in, ex);
-        }
-#pragma unroll
-        for (int c = 0; c < 5; ++c) sm[L.bc_m + c * kmf + k] = bc[c];
+      for (int c = 0; c < 5; ++c) {
+        double& target = dt[5 * m.off + c * m.np + p];
+        target = target + sm[L.bc + c * nf + f] * fac;
       }
-      __syncthreads();
-      apply_face_matrices(5, ka, kb, na, nb, Pa, Pb, sm + L.bc_m, sm + L.tmp, sm + L.bc);
-      // lift into dt
-      for (int f = t; f < nf; f += T) {
-        const int p = face_point(m, d, side, f % na, f / na);
-        const double fac = lift_factor(m.n[d], sm[L.side_own + SIDE_OWN * f + SIDE_OWN - 1]);
-#pragma unroll
-        for (int c = 0; c < 5; ++c) {
-          // component c of the pass: g/Pi/Phi_{0..2} of pair pr (GH); c (SW)
-          const int comp = !GH ? c : c == 0 ? pr : c == 1 ? 10 + pr : 18 + c + 3 * pr;
-          double& target = dt[NC * m.off + comp * m.np + p];
-          target = target + sm[L.bc + c * nf + f] * fac;
-        }
-      }
-      __syncthreads();
     }
   }
 }
@@ -432,41 +402,44 @@ mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restrict
 // the kernels of the uniform adjoint with per-element extents, and a face
 // kernel that transposes the mortar coupling.
 //
-// Point-wise adjoint: ubar = (df/du)^T gbar, and the logical-derivative
-// adjoints xbar written over du (read first, same point).
-__global__ void __launch_bounds__(256)
-gh_mixed_adjoint_pointwise_kernel(const MixedElement* __restrict__ el,
-                                  const int32_t* __restrict__ list, const double* __restrict__ u,
-                                  double* du_xbar, const double* __restrict__ gammas,
+// Point-wise adjoint, one thread per grid point (point_el gives its
+// element): ubar = (df/du)^T gbar, and the logical-derivative adjoints xbar
+// written over du (read first, same point).
+__global__ void __launch_bounds__(128)
+gh_mixed_adjoint_pointwise_kernel(int64_t total, const MixedElement* __restrict__ el,
+                                  const int32_t* __restrict__ point_el,
+                                  const double* __restrict__ u, double* du_xbar,
+                                  const double* __restrict__ gammas,
                                   const double* __restrict__ inv_jac,
                                   const double* __restrict__ gbar, double* __restrict__ ubar) {
-  const MixedElement m = el[list[blockIdx.x]];
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= total) return;
+  const MixedElement m = el[point_el[gid]];
   const int np = m.np;
-  for (int p = threadIdx.x; p < np; p += blockDim.x) {
-    const double* gp = gbar + 50 * m.off + p;
-    const double* ga = gammas + 3 * m.off + p;
-    const double gamma1 = ga[np], gamma2 = ga[2 * np];
-    double ub[50];
-    GhVjpPoint pt;
-    gh_time_derivative_vjp(u + 50 * m.off + p, np, du_xbar + 150 * m.off + p, np, ga[0], gamma1,
-                           gamma2, gp, np, ub, &pt);
+  const int p = static_cast<int>(gid - m.off);
+  const double* gp = gbar + 50 * m.off + p;
+  const double* ga = gammas + 3 * m.off + p;
+  const double gamma1 = ga[np], gamma2 = ga[2 * np];
+  double ub[50];
+  GhVjpPoint pt;
+  gh_time_derivative_vjp(u + 50 * m.off + p, np, du_xbar + 150 * m.off + p, np, ga[0], gamma1,
+                         gamma2, gp, np, ub, &pt);
 #pragma unroll
-    for (int c = 0; c < 50; ++c) ubar[50 * m.off + c * np + p] = ub[c];
-    double ij[9];
+  for (int c = 0; c < 50; ++c) ubar[50 * m.off + c * np + p] = ub[c];
+  double ij[9];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) ij[c] = inv_jac[9 * m.off + c * np + p];
-    for (int c = 0; c < 50; ++c) {
-      double dub[3];
+  for (int c = 0; c < 9; ++c) ij[c] = inv_jac[9 * m.off + c * np + p];
+  for (int c = 0; c < 50; ++c) {
+    double dub[3];
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
-        dub[i] = gh_time_derivative_dubar(pt, gamma1, gamma2, gp, np, 3 * c + i);
+    for (int i = 0; i < 3; ++i)
+      dub[i] = gh_time_derivative_dubar(pt, gamma1, gamma2, gp, np, 3 * c + i);
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        double s = 0.0;
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) s += ij[ij3(j, i)] * dub[i];
-        du_xbar[150 * m.off + (3 * c + j) * np + p] = s;
-      }
+      for (int i = 0; i < 3; ++i) s += ij[ij3(j, i)] * dub[i];
+      du_xbar[150 * m.off + (3 * c + j) * np + p] = s;
     }
   }
 }
@@ -538,29 +511,6 @@ __device__ void apply_face_matrices_t(int nc, int oa, int ob, int ia, int ib, co
   __syncthreads();
 }
 
-// LDS plan of the face adjoint kernel (doubles): per face point of either
-// side 12 side values (n, nv, lam, gamma2, |n|); per own face point 11
-// adjoint accumulators (n, nv, lam, |n|) and 5 corrections; 13-component
-// package buffers on the faces and on the mortar, reused once free.
-struct FaceAdjPlan {
-  int side_own, side_nb, acc, pk_own, pk_nb, bc, tmp, m_own, m_nb, lam_own, lam_nb, bc_m, total;
-  __host__ __device__ FaceAdjPlan(int kf, int km) {
-    side_own = 0;
-    side_nb = side_own + 12 * kf;
-    acc = side_nb + 12 * kf;
-    pk_own = acc + 11 * kf;   // then the package adjoint on the own face
-    pk_nb = pk_own + 13 * kf;  // then the seeds on both faces (5 + 5)
-    bc = pk_nb + 13 * kf;
-    tmp = bc + 5 * kf;
-    m_own = tmp + 13 * km;  // then the package adjoint on the mortar
-    m_nb = m_own + 13 * km;  // then the seeds on the mortar (5 + 5)
-    lam_own = m_nb + 13 * km;
-    lam_nb = lam_own + 4 * km;
-    bc_m = lam_nb + 4 * km;
-    total = bc_m + 5 * km;
-  }
-};
-
 // Side values of face point f of element m's face (d, side); the raw normal
 // is negated when `flip`.
 __device__ void face_side_adjoint(const MixedElement& m, int d, int side, bool flip, int f,
@@ -578,198 +528,308 @@ __device__ void face_side_adjoint(const MixedElement& m, int d, int side, bool f
   *point = p;
 }
 
-// Face adjoint: one workgroup per element e adds to e's face points the
-// adjoints of (A) e's own lifted corrections and (B) the neighbours'
-// corrections on the shared faces, in which e's points are the exterior
-// side.  Per face and pair pr the forward is recomputed up to the mortar
-// corrections, then
-//   seeds   gbar * lift_factor on both faces (the neighbour's with its own),
-//   P^T     each seed projected back onto the mortar (transposed projection),
-//   weights the adjoint of e's package on the mortar, interior in (A) and
-//           exterior in (B), from the upwind weights of the mortar speeds,
-//   I^T     transposed interpolation onto e's face,
-//   package reverse at e's face points into ubar and the side adjoints,
-// and after the ten pairs the adjoint of e's normals and speeds into g.
-// Dynamic LDS: FaceAdjPlan(kf, km).total doubles.
+// GH faces, one pass per workgroup: grid (list length, 2 sides, 10 pairs) for
+// the faces normal to dimension d, so every (face, pair) runs concurrently.
+// Each workgroup packages its pair on both face meshes together with the
+// speeds (17 components), interpolates them to the mortar, evaluates the
+// boundary terms there, projects back and lifts into dt.  The lower and
+// upper faces of one dimension hold different points and the pairs write
+// different components, so no two workgroups of a launch touch the same dt
+// value; launching d = 0, 1, 2 in order adds each point's corrections in
+// direction order.  Every value is computed with the arithmetic of the
+// oracle's mortar coupling.
+struct FacePassPlan {
+  int face, mag, tmp, m_own, m_nb, total;
+  __host__ __device__ FacePassPlan(int kf, int km) {
+    face = 0;               // 17 per face point: speeds, package; then 5 corrections
+    mag = face + 17 * kf;   // |n| per own face point
+    tmp = mag + kf;         // 17 per mortar point
+    m_own = tmp + 17 * km;  // own side on the mortar; then the mortar corrections
+    m_nb = m_own + 17 * km;
+    total = m_nb + 17 * km;
+  }
+};
+
 __global__ void __launch_bounds__(256)
-gh_mixed_face_adjoint_kernel(const MixedElement* __restrict__ el,
-                             const int32_t* __restrict__ list, const int32_t* __restrict__ nbr,
-                             const double* __restrict__ u, const double* __restrict__ params,
-                             const double* __restrict__ inv_jac, const double* __restrict__ gbar,
-                             double* __restrict__ ubar, const double* __restrict__ mortar_tab,
-                             int kf, int km) {
+gh_mixed_face_pass_kernel(int d, const MixedElement* __restrict__ el,
+                          const int32_t* __restrict__ list, const int32_t* __restrict__ nbr,
+                          const double* __restrict__ u, const double* __restrict__ params,
+                          const double* __restrict__ inv_jac, double* __restrict__ dt,
+                          const double* __restrict__ mortar_tab, int kf, int km) {
   extern __shared__ double sm[];
-  const FaceAdjPlan L(kf, km);
+  const FacePassPlan L(kf, km);
   const int t = threadIdx.x, T = blockDim.x;
+  const int side = blockIdx.y, pr = blockIdx.z;
   const int e = list[blockIdx.x];
   const MixedElement m = el[e];
-  for (int dir = 0; dir < 6; ++dir) {
-    const int d = dir >> 1, side = dir & 1;
-    const MixedElement mn = el[nbr[6 * e + dir]];
-    const int na = d == 0 ? m.n[1] : m.n[0], nb = d == 2 ? m.n[1] : m.n[2];
-    const int ma = d == 0 ? mn.n[1] : mn.n[0], mb = d == 2 ? mn.n[1] : mn.n[2];
-    const int ka = max(na, ma), kb = max(nb, mb);
-    const int nf = na * nb, mf = ma * mb, kmf = ka * kb;
-    const double* Ia_own = mortar_tab + mortar_table_offset(na, ka);
-    const double* Ib_own = mortar_tab + mortar_table_offset(nb, kb);
-    const double* Ia_nb = mortar_tab + mortar_table_offset(ma, ka);
-    const double* Ib_nb = mortar_tab + mortar_table_offset(mb, kb);
-    const double* Pa = Ia_own + na * ka;  // projection block follows the interpolation
-    const double* Pb = Ib_own + nb * kb;
-    const double* Pa_nb = Ia_nb + ma * ka;
-    const double* Pb_nb = Ib_nb + mb * kb;
-    const double lift_own = lift_factor(m.n[d], 1.0), lift_nb = lift_factor(mn.n[d], 1.0);
-    __syncthreads();  // previous face done with the LDS
-    for (int f = t; f < nf; f += T) {
-      GhSideFwd s;
-      int p;
-      face_side_adjoint(m, d, side, side == 0, f, u, params, inv_jac, s, &p);
-      double* o = sm + L.side_own + 12 * f;
+  const MixedElement mn = el[nbr[6 * e + 2 * d + side]];
+  const int na = d == 0 ? m.n[1] : m.n[0], nb = d == 2 ? m.n[1] : m.n[2];
+  const int ma = d == 0 ? mn.n[1] : mn.n[0], mb = d == 2 ? mn.n[1] : mn.n[2];
+  const int ka = max(na, ma), kb = max(nb, mb);
+  const int nf = na * nb, mf = ma * mb, kmf = ka * kb;
+  const double* Ia_own = mortar_tab + mortar_table_offset(na, ka);
+  const double* Ib_own = mortar_tab + mortar_table_offset(nb, kb);
+  const double* Ia_nb = mortar_tab + mortar_table_offset(ma, ka);
+  const double* Ib_nb = mortar_tab + mortar_table_offset(mb, kb);
+  const double* Pa = Ia_own + na * ka;  // projection block follows the interpolation
+  const double* Pb = Ib_own + nb * kb;
+  double* face = sm + L.face;
+  // own side: speeds and package of the pair on the face mesh
+  for (int f = t; f < nf; f += T) {
+    double sd[12];
+    face_side<true>(m, d, side, side == 0, f, u, params, inv_jac, sd, true);
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        o[i] = s.n[i];
-        o[3 + i] = s.nv[i];
-      }
+    for (int c = 0; c < 4; ++c) face[c * nf + f] = sd[6 + c];
+    sm[L.mag + f] = sd[11];
+    face_package<true>(m, d, side, f, nf, pr, u, params, sd, face + 4 * nf);
+  }
+  __syncthreads();
+  apply_face_matrices(17, na, nb, ka, kb, Ia_own, Ib_own, face, sm + L.tmp, sm + L.m_own);
+  // neighbour side
+  for (int f = t; f < mf; f += T) {
+    double sd[12];
+    face_side<true>(mn, d, 1 - side, side == 1, f, u, params, inv_jac, sd, false);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) o[6 + k] = s.lam[k];
-      o[10] = params[3 * m.off + 2 * m.np + p];
-      o[11] = s.mag;
+    for (int c = 0; c < 4; ++c) face[c * mf + f] = sd[6 + c];
+    face_package<true>(mn, d, 1 - side, f, mf, pr, u, params, sd, face + 4 * mf);
+  }
+  __syncthreads();
+  apply_face_matrices(17, ma, mb, ka, kb, Ia_nb, Ib_nb, face, sm + L.tmp, sm + L.m_nb);
+  // boundary terms at the mortar points, written over the own side's slots
+  for (int k = t; k < kmf; k += T) {
+    double in[13], ex[13], li[4], le[4], bc[5];
 #pragma unroll
-      for (int c = 0; c < 11; ++c) sm[L.acc + 11 * f + c] = 0.0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) sm[L.pk_own + k * nf + f] = s.lam[k];
+    for (int c = 0; c < 4; ++c) {
+      li[c] = sm[L.m_own + c * kmf + k];
+      le[c] = sm[L.m_nb + c * kmf + k];
     }
-    for (int f = t; f < mf; f += T) {
-      GhSideFwd s;
-      int q;
-      face_side_adjoint(mn, d, 1 - side, side == 1, f, u, params, inv_jac, s, &q);
-      double* o = sm + L.side_nb + 12 * f;
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        o[i] = s.n[i];
-        o[3 + i] = s.nv[i];
-      }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) o[6 + k] = s.lam[k];
-      o[10] = params[3 * mn.off + 2 * mn.np + q];
-      o[11] = s.mag;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) sm[L.pk_nb + k * mf + f] = s.lam[k];
+    for (int c = 0; c < 13; ++c) {
+      in[c] = sm[L.m_own + (4 + c) * kmf + k];
+      ex[c] = sm[L.m_nb + (4 + c) * kmf + k];
     }
-    __syncthreads();
-    // speeds on the mortar (their upwind weights)
-    apply_face_matrices(4, na, nb, ka, kb, Ia_own, Ib_own, sm + L.pk_own, sm + L.tmp,
-                        sm + L.lam_own);
-    apply_face_matrices(4, ma, mb, ka, kb, Ia_nb, Ib_nb, sm + L.pk_nb, sm + L.tmp,
-                        sm + L.lam_nb);
-    for (int pr = 0; pr < 10; ++pr) {
-      const int comp[5] = {pr, 10 + pr, 20 + 3 * pr, 21 + 3 * pr, 22 + 3 * pr};
-      // forward: packages, mortar, corrections of e's face
-      for (int f = t; f < nf; f += T) {
-        const double* o = sm + L.side_own + 12 * f;
-        const int p = face_point(m, d, side, f % na, f / na);
-        double pk[13];
-        gh_package_fwd(o, o + 3, o + 6, o[10], u + 50 * m.off + p, m.np, pr, pk);
+    gh_boundary_terms_pair(bc, in, li, ex, le);
 #pragma unroll
-        for (int c = 0; c < 13; ++c) sm[L.pk_own + c * nf + f] = pk[c];
-      }
-      for (int f = t; f < mf; f += T) {
-        const double* o = sm + L.side_nb + 12 * f;
-        const int q = face_point(mn, d, 1 - side, f % ma, f / ma);
-        double pk[13];
-        gh_package_fwd(o, o + 3, o + 6, o[10], u + 50 * mn.off + q, mn.np, pr, pk);
+    for (int c = 0; c < 5; ++c) sm[L.m_own + c * kmf + k] = bc[c];
+  }
+  __syncthreads();
+  apply_face_matrices(5, ka, kb, na, nb, Pa, Pb, sm + L.m_own, sm + L.tmp, face);
+  // lift into dt: components g, Pi, Phi_0..2 of the pair
+  for (int f = t; f < nf; f += T) {
+    const int p = face_point(m, d, side, f % na, f / na);
+    const double fac = lift_factor(m.n[d], sm[L.mag + f]);
 #pragma unroll
-        for (int c = 0; c < 13; ++c) sm[L.pk_nb + c * mf + f] = pk[c];
-      }
-      __syncthreads();
-      apply_face_matrices(13, na, nb, ka, kb, Ia_own, Ib_own, sm + L.pk_own, sm + L.tmp,
-                          sm + L.m_own);
-      apply_face_matrices(13, ma, mb, ka, kb, Ia_nb, Ib_nb, sm + L.pk_nb, sm + L.tmp,
-                          sm + L.m_nb);
-      for (int k = t; k < kmf; k += T) {
-        double in[13], ex[13], li[4], le[4], bc[5];
-#pragma unroll
-        for (int c = 0; c < 13; ++c) {
-          in[c] = sm[L.m_own + c * kmf + k];
-          ex[c] = sm[L.m_nb + c * kmf + k];
-        }
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          li[c] = sm[L.lam_own + c * kmf + k];
-          le[c] = sm[L.lam_nb + c * kmf + k];
-        }
-        gh_boundary_fwd(in, li, ex, le, bc);
-#pragma unroll
-        for (int c = 0; c < 5; ++c) sm[L.bc_m + c * kmf + k] = bc[c];
-      }
-      __syncthreads();
-      apply_face_matrices(5, ka, kb, na, nb, Pa, Pb, sm + L.bc_m, sm + L.tmp, sm + L.bc);
-      // seeds on both faces; |n| adjoint of e's lift factor
-      double* seed_own = sm + L.pk_nb;
-      double* seed_nb = sm + L.pk_nb + 5 * kf;
-      for (int f = t; f < nf; f += T) {
-        const int p = face_point(m, d, side, f % na, f / na);
-        const double fac = lift_own * sm[L.side_own + 12 * f + 11];
-        double fac_b = 0.0;
-#pragma unroll
-        for (int c = 0; c < 5; ++c) {
-          const double gb = gbar[50 * m.off + comp[c] * m.np + p];
-          fac_b += gb * sm[L.bc + c * nf + f];
-          seed_own[c * nf + f] = gb * fac;
-        }
-        sm[L.acc + 11 * f + 10] += lift_own * fac_b;
-      }
-      for (int f = t; f < mf; f += T) {
-        const int q = face_point(mn, d, 1 - side, f % ma, f / ma);
-        const double fac = lift_nb * sm[L.side_nb + 12 * f + 11];
-#pragma unroll
-        for (int c = 0; c < 5; ++c)
-          seed_nb[c * mf + f] = gbar[50 * mn.off + comp[c] * mn.np + q] * fac;
-      }
-      __syncthreads();
-      double* a_m = sm + L.m_nb;
-      double* b_m = sm + L.m_nb + 5 * km;
-      apply_face_matrices_t(5, na, nb, ka, kb, Pa, Pb, seed_own, sm + L.tmp, a_m);
-      apply_face_matrices_t(5, ma, mb, ka, kb, Pa_nb, Pb_nb, seed_nb, sm + L.tmp, b_m);
-      for (int k = t; k < kmf; k += T) {
-        double li[4], bi[5], bx[5], B[13];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) li[c] = sm[L.lam_own + c * kmf + k];
-#pragma unroll
-        for (int c = 0; c < 5; ++c) {
-          bi[c] = a_m[c * kmf + k];
-          bx[c] = b_m[c * kmf + k];
-        }
-        gh_package_seed(li, bi, bx, B);
-#pragma unroll
-        for (int c = 0; c < 13; ++c) sm[L.m_own + c * kmf + k] = B[c];
-      }
-      __syncthreads();
-      apply_face_matrices_t(13, ka, kb, na, nb, Ia_own, Ib_own, sm + L.m_own, sm + L.tmp,
-                            sm + L.pk_own);
-      for (int f = t; f < nf; f += T) {
-        const double* o = sm + L.side_own + 12 * f;
-        double* a = sm + L.acc + 11 * f;
-        const int p = face_point(m, d, side, f % na, f / na);
-        double B[13];
-#pragma unroll
-        for (int c = 0; c < 13; ++c) B[c] = sm[L.pk_own + c * nf + f];
-        gh_package_vjp(o, o + 3, o + 6, o[10], u + 50 * m.off + p, m.np, pr, B,
-                       ubar + 50 * m.off + p, m.np, a, a + 3, a + 6);
-      }
-      __syncthreads();
+    for (int c = 0; c < 5; ++c) {
+      const int comp = c == 0 ? pr : c == 1 ? 10 + pr : 18 + c + 3 * pr;
+      double& target = dt[50 * m.off + comp * m.np + p];
+      target = target + face[c * nf + f] * fac;
     }
-    // normals, speeds and |n| of e's side depend on g
-    for (int f = t; f < nf; f += T) {
-      GhSideFwd s;
-      int p;
-      face_side_adjoint(m, d, side, side == 0, f, u, params, inv_jac, s, &p);
-      const double* a = sm + L.acc + 11 * f;
-      double gm_b[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-      gh_side_vjp(s, params[3 * m.off + m.np + p], a, a + 3, a + 6, a[10], gm_b);
+  }
+}
+
+// GH face adjoint, one pass per workgroup: grid (list length, 2 sides, 10
+// pairs) for the faces normal to dimension d, as gh_mixed_face_pass_kernel.
+// Workgroup (e, side, pr) adds to e's points on face (d, side) the adjoint,
+// for pair pr, of e's own lifted correction and of the neighbour's (in which
+// e is the exterior side): the forward is recomputed up to the mortar
+// corrections, then
+//   seeds   gbar * lift factor on both faces (the neighbour's with its own),
+//   P^T     each seed projected back onto the mortar (transposed projection),
+//   weights the adjoint of e's package on the mortar, interior for its own
+//           correction and exterior for the neighbour's, from the upwind
+//           weights of the mortar speeds,
+//   I^T     transposed interpolation onto e's face,
+//   package reverse at e's face points into ubar (the pair's components).
+// The adjoints of e's normals, speeds and |n| from this pair go to side_adj
+// (11 per pair at each face point, indexed by the point's volume index);
+// gh_mixed_side_adjoint_kernel sums them over the pairs and takes them to g.
+// Dynamic LDS: FaceAdjPassPlan(kf, km).total doubles.
+struct FaceAdjPassPlan {
+  int side, mag_nb, face, tmp, m_own, m_nb, total;
+  __host__ __device__ FaceAdjPassPlan(int kf, int km) {
+    side = 0;                // own face point: n, nv, lam, gamma2, |n|
+    mag_nb = side + 12 * kf;  // |n| of the neighbour's face points
+    face = mag_nb + kf;       // 17 per face point (speeds, package); then
+                              // corrections, seeds, package adjoint
+    tmp = face + 17 * kf;
+    m_own = tmp + 17 * km;   // own speeds, package; then seeds on the mortar
+    m_nb = m_own + 17 * km;  // neighbour's; then corrections, package adjoint
+    total = m_nb + 17 * km;
+  }
+};
+
+__global__ void __launch_bounds__(256)
+gh_mixed_face_adjoint_pass_kernel(int d, const MixedElement* __restrict__ el,
+                                  const int32_t* __restrict__ list,
+                                  const int32_t* __restrict__ nbr, const double* __restrict__ u,
+                                  const double* __restrict__ params,
+                                  const double* __restrict__ inv_jac,
+                                  const double* __restrict__ gbar, double* __restrict__ ubar,
+                                  double* __restrict__ side_adj,
+                                  const double* __restrict__ mortar_tab, int kf, int km) {
+  extern __shared__ double sm[];
+  const FaceAdjPassPlan L(kf, km);
+  const int t = threadIdx.x, T = blockDim.x;
+  const int side = blockIdx.y, pr = blockIdx.z;
+  const int e = list[blockIdx.x];
+  const MixedElement m = el[e];
+  const MixedElement mn = el[nbr[6 * e + 2 * d + side]];
+  const int na = d == 0 ? m.n[1] : m.n[0], nb = d == 2 ? m.n[1] : m.n[2];
+  const int ma = d == 0 ? mn.n[1] : mn.n[0], mb = d == 2 ? mn.n[1] : mn.n[2];
+  const int ka = max(na, ma), kb = max(nb, mb);
+  const int nf = na * nb, mf = ma * mb, kmf = ka * kb;
+  const double* Ia_own = mortar_tab + mortar_table_offset(na, ka);
+  const double* Ib_own = mortar_tab + mortar_table_offset(nb, kb);
+  const double* Ia_nb = mortar_tab + mortar_table_offset(ma, ka);
+  const double* Ib_nb = mortar_tab + mortar_table_offset(mb, kb);
+  const double* Pa = Ia_own + na * ka;  // projection block follows the interpolation
+  const double* Pb = Ib_own + nb * kb;
+  const double* Pa_nb = Ia_nb + ma * ka;
+  const double* Pb_nb = Ib_nb + mb * kb;
+  const double lift_own = lift_factor(m.n[d], 1.0), lift_nb = lift_factor(mn.n[d], 1.0);
+  const int comp[5] = {pr, 10 + pr, 20 + 3 * pr, 21 + 3 * pr, 22 + 3 * pr};
+  double* face = sm + L.face;
+  // own side: side values, speeds and package of the pair
+  for (int f = t; f < nf; f += T) {
+    GhSideFwd s;
+    int p;
+    face_side_adjoint(m, d, side, side == 0, f, u, params, inv_jac, s, &p);
+    double* o = sm + L.side + 12 * f;
 #pragma unroll
-      for (int c = 0; c < 10; ++c) ubar[50 * m.off + c * m.np + p] += gm_b[c];
+    for (int i = 0; i < 3; ++i) {
+      o[i] = s.n[i];
+      o[3 + i] = s.nv[i];
     }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      o[6 + k] = s.lam[k];
+      face[k * nf + f] = s.lam[k];
+    }
+    o[10] = params[3 * m.off + 2 * m.np + p];
+    o[11] = s.mag;
+    double pk[13];
+    gh_package_fwd(s.n, s.nv, s.lam, o[10], u + 50 * m.off + p, m.np, pr, pk);
+#pragma unroll
+    for (int c = 0; c < 13; ++c) face[(4 + c) * nf + f] = pk[c];
+  }
+  __syncthreads();
+  apply_face_matrices(17, na, nb, ka, kb, Ia_own, Ib_own, face, sm + L.tmp, sm + L.m_own);
+  for (int f = t; f < mf; f += T) {
+    GhSideFwd s;
+    int q;
+    face_side_adjoint(mn, d, 1 - side, side == 1, f, u, params, inv_jac, s, &q);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) face[k * mf + f] = s.lam[k];
+    sm[L.mag_nb + f] = s.mag;
+    double pk[13];
+    gh_package_fwd(s.n, s.nv, s.lam, params[3 * mn.off + 2 * mn.np + q], u + 50 * mn.off + q,
+                   mn.np, pr, pk);
+#pragma unroll
+    for (int c = 0; c < 13; ++c) face[(4 + c) * mf + f] = pk[c];
+  }
+  __syncthreads();
+  apply_face_matrices(17, ma, mb, ka, kb, Ia_nb, Ib_nb, face, sm + L.tmp, sm + L.m_nb);
+  // e's corrections on the mortar, over the neighbour's package slots
+  for (int k = t; k < kmf; k += T) {
+    double in[13], ex[13], li[4], le[4], bc[5];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      li[c] = sm[L.m_own + c * kmf + k];
+      le[c] = sm[L.m_nb + c * kmf + k];
+    }
+#pragma unroll
+    for (int c = 0; c < 13; ++c) {
+      in[c] = sm[L.m_own + (4 + c) * kmf + k];
+      ex[c] = sm[L.m_nb + (4 + c) * kmf + k];
+    }
+    gh_boundary_fwd(in, li, ex, le, bc);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) sm[L.m_nb + (4 + c) * kmf + k] = bc[c];
+  }
+  __syncthreads();
+  apply_face_matrices(5, ka, kb, na, nb, Pa, Pb, sm + L.m_nb + 4 * kmf, sm + L.tmp, face);
+  // seeds on both faces; the pair's part of the |n| adjoint of e's lift factor
+  double* seed_own = face + 5 * kf;
+  double* seed_nb = face + 10 * kf;
+  for (int f = t; f < nf; f += T) {
+    const int p = face_point(m, d, side, f % na, f / na);
+    const double fac = lift_own * sm[L.side + 12 * f + 11];
+    double fac_b = 0.0;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      const double gb = gbar[50 * m.off + comp[c] * m.np + p];
+      fac_b += gb * face[c * nf + f];
+      seed_own[c * nf + f] = gb * fac;
+    }
+    side_adj[110 * (m.off + p) + 11 * pr + 10] = lift_own * fac_b;
+  }
+  for (int f = t; f < mf; f += T) {
+    const int q = face_point(mn, d, 1 - side, f % ma, f / ma);
+    const double fac = lift_nb * sm[L.mag_nb + f];
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+      seed_nb[c * mf + f] = gbar[50 * mn.off + comp[c] * mn.np + q] * fac;
+  }
+  __syncthreads();
+  apply_face_matrices_t(5, na, nb, ka, kb, Pa, Pb, seed_own, sm + L.tmp, sm + L.m_own + 4 * kmf);
+  apply_face_matrices_t(5, ma, mb, ka, kb, Pa_nb, Pb_nb, seed_nb, sm + L.tmp,
+                        sm + L.m_own + 9 * kmf);
+  for (int k = t; k < kmf; k += T) {
+    double li[4], bi[5], bx[5], B[13];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) li[c] = sm[L.m_own + c * kmf + k];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      bi[c] = sm[L.m_own + (4 + c) * kmf + k];
+      bx[c] = sm[L.m_own + (9 + c) * kmf + k];
+    }
+    gh_package_seed(li, bi, bx, B);
+#pragma unroll
+    for (int c = 0; c < 13; ++c) sm[L.m_nb + c * kmf + k] = B[c];
+  }
+  __syncthreads();
+  apply_face_matrices_t(13, ka, kb, na, nb, Ia_own, Ib_own, sm + L.m_nb, sm + L.tmp, face);
+  for (int f = t; f < nf; f += T) {
+    const double* o = sm + L.side + 12 * f;
+    const int p = face_point(m, d, side, f % na, f / na);
+    double B[13], a[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 13; ++c) B[c] = face[c * nf + f];
+    gh_package_vjp(o, o + 3, o + 6, o[10], u + 50 * m.off + p, m.np, pr, B,
+                   ubar + 50 * m.off + p, m.np, a, a + 3, a + 6);
+    double* out = side_adj + 110 * (m.off + p) + 11 * pr;
+#pragma unroll
+    for (int c = 0; c < 10; ++c) out[c] = a[c];
+  }
+}
+
+// The adjoints of e's normals, speeds and |n| on the faces normal to
+// dimension d, summed over the ten pairs in order, into g.  Grid: (list
+// length, 2 sides).
+__global__ void __launch_bounds__(256)
+gh_mixed_side_adjoint_kernel(int d, const MixedElement* __restrict__ el,
+                             const int32_t* __restrict__ list, const double* __restrict__ u,
+                             const double* __restrict__ params,
+                             const double* __restrict__ inv_jac,
+                             const double* __restrict__ side_adj, double* __restrict__ ubar) {
+  const int side = blockIdx.y;
+  const MixedElement m = el[list[blockIdx.x]];
+  const int na = d == 0 ? m.n[1] : m.n[0], nb = d == 2 ? m.n[1] : m.n[2];
+  for (int f = threadIdx.x; f < na * nb; f += blockDim.x) {
+    GhSideFwd s;
+    int p;
+    face_side_adjoint(m, d, side, side == 0, f, u, params, inv_jac, s, &p);
+    const double* in = side_adj + 110 * (m.off + p);
+    double a[11];
+#pragma unroll
+    for (int c = 0; c < 11; ++c) a[c] = in[c];
+    for (int pr = 1; pr < 10; ++pr)
+#pragma unroll
+      for (int c = 0; c < 11; ++c) a[c] += in[11 * pr + c];
+    double gm_b[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    gh_side_vjp(s, params[3 * m.off + m.np + p], a, a + 3, a + 6, a[10], gm_b);
+#pragma unroll
+    for (int c = 0; c < 10; ++c) ubar[50 * m.off + c * m.np + p] += gm_b[c];
   }
 }
 
@@ -786,13 +846,13 @@ bool enable_large_lds() {
   const int lds_max = 160 * 1024;
   if (hipFuncSetAttribute(reinterpret_cast<const void*>(&sw_mixed_volume_kernel),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
-      hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_face_kernel<false>),
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&sw_mixed_face_kernel),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
-      hipFuncSetAttribute(reinterpret_cast<const void*>(&mixed_face_kernel<true>),
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&gh_mixed_face_pass_kernel),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&gh_mixed_face_adjoint_pass_kernel),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
       hipFuncSetAttribute(reinterpret_cast<const void*>(&ragged_transposed_sweeps_kernel),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
-      hipFuncSetAttribute(reinterpret_cast<const void*>(&gh_mixed_face_adjoint_kernel),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess)
     return false;
   done[dev] = true;
@@ -820,7 +880,7 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
     hipLaunchKernelGGL(sw_mixed_volume_kernel, dim3(count), dim3(threads), 5 * np_max * 8, stream,
                        el, list, u, params, inv_jac, dt, dtab);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
-    hipLaunchKernelGGL(mixed_face_kernel<false>, dim3(count), dim3(threads), face_lds, stream, el,
+    hipLaunchKernelGGL(sw_mixed_face_kernel, dim3(count), dim3(threads), face_lds, stream, el,
                        list, nbr, u, params, inv_jac, dt, mtab, kf, km);
     return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
   }
@@ -832,14 +892,19 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
   hipLaunchKernelGGL(gh_mixed_pointwise_kernel, dim3(count), dim3(threads), 0, stream, el, list,
                      u, scratch_du, params, dt);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
-  hipLaunchKernelGGL(mixed_face_kernel<true>, dim3(count), dim3(threads), face_lds, stream, el,
-                     list, nbr, u, params, inv_jac, dt, mtab, kf, km);
-  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
+  const int pass_lds = FacePassPlan(kf, km).total * 8;
+  for (int d = 0; d < 3; ++d) {
+    hipLaunchKernelGGL(gh_mixed_face_pass_kernel, dim3(count, 2, 10), dim3(threads), pass_lds,
+                       stream, d, el, list, nbr, u, params, inv_jac, dt, mtab, kf, km);
+    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  }
+  return SXD_OK;
 }
 
-int mixed_face_adjoint_lds_bytes(int kf, int km) { return FaceAdjPlan(kf, km).total * 8; }
+int mixed_face_adjoint_lds_bytes(int kf, int km) { return FaceAdjPassPlan(kf, km).total * 8; }
 
 int launch_mixed_gh_adjoint(int count, const int32_t* list, const MixedElement* el,
+                            const int32_t* point_el, int64_t total_points,
                             const int32_t* nbr, int np_max, int kf, int km, const double* u,
                             const double* gammas, const double* inv_jac, const double* gbar,
                             double* ubar, double* scratch_du, hipStream_t stream) {
@@ -847,23 +912,34 @@ int launch_mixed_gh_adjoint(int count, const int32_t* list, const MixedElement* 
   const double* dtab = device_diff_matrices();
   const double* mtab = device_mortar_matrices();
   if (dtab == nullptr || mtab == nullptr) return SXD_ERR_DEVICE;
-  const int face_lds = mixed_face_adjoint_lds_bytes(kf, km);
-  if (face_lds > 160 * 1024 || 3 * np_max * 8 > 160 * 1024) return SXD_ERR_UNSUPPORTED;
+  if (mixed_face_adjoint_lds_bytes(kf, km) > 160 * 1024 || 3 * np_max * 8 > 160 * 1024)
+    return SXD_ERR_UNSUPPORTED;
   if (scratch_du == nullptr) return SXD_ERR_INVALID_ARGUMENT;
   if (!enable_large_lds()) return SXD_ERR_DEVICE;
   const dim3 chunks(count, (50 + kRaggedChunk - 1) / kRaggedChunk);
   hipLaunchKernelGGL(ragged_partial_derivatives_kernel, chunks, dim3(256), np_max * 8, stream, el,
                      list, 50, u, inv_jac, scratch_du, dtab);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
-  hipLaunchKernelGGL(gh_mixed_adjoint_pointwise_kernel, dim3(count), dim3(256), 0, stream, el,
-                     list, u, scratch_du, gammas, inv_jac, gbar, ubar);
+  hipLaunchKernelGGL(gh_mixed_adjoint_pointwise_kernel,
+                     dim3(static_cast<unsigned>((total_points + 127) / 128)), dim3(128), 0, stream,
+                     total_points, el, point_el, u, scratch_du, gammas, inv_jac, gbar, ubar);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   hipLaunchKernelGGL(ragged_transposed_sweeps_kernel, chunks, dim3(256), 3 * np_max * 8, stream,
                      el, list, 50, scratch_du, ubar, dtab);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
-  hipLaunchKernelGGL(gh_mixed_face_adjoint_kernel, dim3(count), dim3(256), face_lds, stream, el,
-                     list, nbr, u, gammas, inv_jac, gbar, ubar, mtab, kf, km);
-  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
+  // faces: the derivative work array is free again and holds the side
+  // adjoints (110 doubles per point, under the 150 it has)
+  const int pass_lds = FaceAdjPassPlan(kf, km).total * 8;
+  for (int d = 0; d < 3; ++d) {
+    hipLaunchKernelGGL(gh_mixed_face_adjoint_pass_kernel, dim3(count, 2, 10), dim3(256),
+                       pass_lds, stream, d, el, list, nbr, u, gammas, inv_jac, gbar, ubar,
+                       scratch_du, mtab, kf, km);
+    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+    hipLaunchKernelGGL(gh_mixed_side_adjoint_kernel, dim3(count, 2), dim3(128), 0, stream, d, el,
+                       list, u, gammas, inv_jac, scratch_du, ubar);
+    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  }
+  return SXD_OK;
 }
 
 }  // namespace sxd
