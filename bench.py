@@ -30,7 +30,7 @@ FP64_PEAK_TFLOPS = 78.6
 # FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), for the workload's default kernel;
 # the summaries are profiles/gh_rhs_fused_kernel_gh_kerr_schild_512el_8cubed.md
 # and profiles/sw_rhs_fused_kernel_sw_plane_wave_64el_5cubed.md.
-MEASURED_TRAFFIC = {("gh", "fused"): 636803264.0, ("scalar_wave", "fused"): 3229690.0}
+MEASURED_TRAFFIC = {("gh", "fused"): 404157952.0, ("scalar_wave", "fused"): 3229690.0}
 
 
 def algorithmic_bytes_per_point(nc, n_aux, n):
@@ -252,13 +252,19 @@ def main():
                     help="threads per workgroup with an explicit --kernel (0 = library default)")
     ap.add_argument("--extensions", type=int, default=1,
                     help="also measure the mixed-p batched form (1) or not (0)")
+    ap.add_argument("--only-extension", default="",
+                    choices=["", "mixed_sw", "mixed_gh", "adjoint", "gh_adjoint",
+                             "gh_adjoint_mixed"],
+                    help="run only this extension measurement (for profiling) and print "
+                         "its JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (a plain copy "
                          "kernel of known size, to calibrate HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not "
-              "supported (the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+              "supported (across GPUs the path shards by elements with one face-trace "
+              "exchange per RHS; that is not built here)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -278,6 +284,15 @@ def main():
     root = os.path.dirname(os.path.abspath(__file__))
     sys.path.insert(0, root)
     from spectre_amd import dg, workloads as wl
+
+    if args.only_extension:
+        only = {"mixed_sw": lambda: measure_mixed("scalar_wave", 20, 3),
+                "mixed_gh": lambda: measure_mixed("gh", 20, 3),
+                "adjoint": measure_adjoint,
+                "gh_adjoint": measure_gh_adjoint,
+                "gh_adjoint_mixed": lambda: measure_gh_adjoint(mixed=True)}
+        print(json.dumps(only[args.only_extension]()))
+        return
 
     if args.workload == "gh":
         brick, u, aux, inv = wl.gh_kerr_schild_brick(seed=0)
