@@ -463,6 +463,33 @@ def test_gh_rhs_adjoint_mixed_p_matches_oracle_tape(elements, lowest, highest):
     assert err <= 1e-12
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_gh_rhs_adjoint_full_size_dot_product(mixed):
+    """Full adjoint workloads (300 elements: uniform 6^3, and mixed extents
+    6..10): <f(u + eps v) - f(u - eps v), w> / (2 eps) = <v, J^T w> for seeded
+    v, w (eps = 1e-6), relative difference <= 1e-7."""
+    if mixed:
+        brick, u, gammas, inv, _ = wl.gh_adjoint_workload_mixed()
+        domain = dg.MixedDomain.from_brick(brick)
+    else:
+        brick, u, gammas, inv, _ = wl.gh_adjoint_workload()
+        domain = dg.Domain.from_brick(brick)
+    rng = np.random.default_rng(12)
+    v = rng.standard_normal(u.shape) * np.abs(u)
+    w = rng.standard_normal(u.shape)
+    d_u, d_g, d_inv, d_v, d_w = dev(u), dev(gammas), dev(inv), dev(v), dev(w)
+    jtw = domain.gh_rhs_adjoint(d_u, d_g, d_inv, d_w)
+    eps = 1e-6
+    fp = domain.gh_rhs(d_u + eps * d_v, d_g, d_inv)
+    fm = domain.gh_rhs(d_u - eps * d_v, d_g, d_inv)
+    lhs = float(torch.sum((fp - fm) * d_w)) / (2 * eps)
+    rhs = float(torch.sum(d_v * jtw))
+    rel = abs(lhs - rhs) / abs(rhs)
+    print(f"GH adjoint dot product ({'mixed' if mixed else 'uniform'}): FD {lhs:.12e}, "
+          f"adjoint {rhs:.12e}, relative difference {rel:.3e}")
+    assert rel <= 1e-7
+
+
 def _gh_gradient(brick, u, gammas, inv, dt, steps, k):
     from spectre_amd import evolution as ev
     domain = dg.Domain.from_brick(brick)
