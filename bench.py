@@ -74,6 +74,39 @@ def time_calls(fn, steps, warmup):
     return start.elapsed_time(stop) / steps
 
 
+def measure_sw_graph(calls=1000, reps=5):
+    """ScalarWave plane wave on 64 elements of 5^3 points (8000 points): one
+    launch is far too small to fill the GPU, so the RHS is also timed inside a
+    HIP graph of `calls` RHS evaluations (the library's launches captured by
+    torch.cuda.graph on the capture stream).  The per-call difference between
+    call-by-call enqueueing and graph replay is the launch overhead."""
+    import numpy as np
+    import torch
+    from spectre_amd import dg, workloads as wl
+
+    brick, u, g2, inv = wl.scalar_wave_plane_wave_3d(seed=0)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    d_u, d_g, d_inv = dev(u), dev(g2), dev(inv)
+    d_dt = torch.empty_like(d_u)
+    domain = dg.Domain.from_brick(brick)
+    domain.set_kernel("scalar_wave", "fused")
+    rhs = lambda: domain.scalar_wave_rhs(d_u, d_g, d_inv, out=d_dt)
+    direct_ms = time_calls(rhs, calls, 50)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        for _ in range(calls):
+            rhs()
+    graph.replay()
+    torch.cuda.synchronize()
+    graph_ms = time_calls(graph.replay, reps, 1) / calls
+    points = brick.num_elements * brick.num_points
+    return {"workload": "scalar_wave_plane_wave_64el_5^3", "grid_points": points,
+            "graph_calls": calls, "rhs_us_graph": round(graph_ms * 1e3, 3),
+            "rhs_us_call_by_call": round(direct_ms * 1e3, 3),
+            "launch_overhead_us": round((direct_ms - graph_ms) * 1e3, 3),
+            "grid_points_per_s_graph": round(points / (graph_ms * 1e-3), 1)}
+
+
 def measure_mixed(system, steps, warmup, rk3_steps=5):
     """Mixed-p workload (1000 elements of a 10x10x10 brick, extents drawn from
     {4..12} per element and dimension): the batched work-list launch against
@@ -253,7 +286,7 @@ def main():
     ap.add_argument("--extensions", type=int, default=1,
                     help="also measure the mixed-p batched form (1) or not (0)")
     ap.add_argument("--only-extension", default="",
-                    choices=["", "mixed_sw", "mixed_gh", "adjoint", "gh_adjoint",
+                    choices=["", "sw_graph", "mixed_sw", "mixed_gh", "adjoint", "gh_adjoint",
                              "gh_adjoint_mixed"],
                     help="run only this extension measurement (for profiling) and print "
                          "its JSON")
@@ -286,7 +319,8 @@ def main():
     from spectre_amd import dg, workloads as wl
 
     if args.only_extension:
-        only = {"mixed_sw": lambda: measure_mixed("scalar_wave", 20, 3),
+        only = {"sw_graph": measure_sw_graph,
+                "mixed_sw": lambda: measure_mixed("scalar_wave", 20, 3),
                 "mixed_gh": lambda: measure_mixed("gh", 20, 3),
                 "adjoint": measure_adjoint,
                 "gh_adjoint": measure_gh_adjoint,
@@ -389,7 +423,8 @@ def main():
 
     extensions = None
     if args.extensions:
-        extensions = {"mixed_p": [measure_mixed(s, 20, 3) for s in ("scalar_wave", "gh")],
+        extensions = {"sw_64el_graph": measure_sw_graph(),
+                      "mixed_p": [measure_mixed(s, 20, 3) for s in ("scalar_wave", "gh")],
                       "adjoint": measure_adjoint(),
                       "gh_adjoint": measure_gh_adjoint(),
                       "gh_adjoint_mixed": measure_gh_adjoint(mixed=True)}

@@ -305,6 +305,34 @@ def test_rk3_graph_scalar_wave_uniform_and_mixed():
     stepper.close()
 
 
+def test_scalar_wave_rhs_in_a_torch_graph_replays():
+    """The library's RHS launches captured in a torch.cuda graph (ten
+    evaluations, the last one writing a second output): every replay, with
+    eager work and synchronisations between replays, equals the eager result."""
+    brick, u, g2, inv = wl.scalar_wave_plane_wave_3d(seed=3)
+    domain = dg.Domain.from_brick(brick)
+    domain.set_kernel("scalar_wave", "fused")
+    d_u, d_g, d_inv = dev(u), dev(g2), dev(inv)
+    eager = host(domain.scalar_wave_rhs(d_u, d_g, d_inv))
+    out, out2 = torch.empty_like(d_u), torch.empty_like(d_u)
+    domain.scalar_wave_rhs(d_u, d_g, d_inv, out=out)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        for _ in range(9):
+            domain.scalar_wave_rhs(d_u, d_g, d_inv, out=out)
+        domain.scalar_wave_rhs(d_u, d_g, d_inv, out=out2)
+    for r in range(4):
+        out.zero_()
+        out2.fill_(float(r + 1))
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(host(out), eager)
+        assert np.array_equal(host(out2), eager)
+        domain.scalar_wave_rhs(d_u * 2.0, d_g, d_inv)  # eager work between replays
+
+
 def test_rk3_graph_gh():
     brick, u, gammas, inv = wl.gh_kerr_schild_brick(seed=5, elements=(2, 2, 2), extents=(6, 6, 6))
     ref = oracle_rk3(lambda v: ol.gh_rhs(brick.extents, brick.neighbors, v, gammas, inv), u,
