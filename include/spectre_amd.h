@@ -238,9 +238,10 @@ int sxd_scalar_wave_rhs_adjoint(const sxd_domain* domain, const double* gbar,
  * ubar = (d dt / d u)^T gbar, the gradient with respect to u of a functional
  * J given gbar = dJ/d(dt).  Uniform domains (Np up to 2730): u, gbar,
  * ubar [E][50][Np]; mixed-p domains (sxd_domain_create_mixed): the ragged
- * layout.  ubar must not alias u or gbar.  Uses the domain's derivative work
- * array (150 doubles per grid point, allocated on first use).  No
- * counterpart in the reference (which has no adjoints). */
+ * layout.  ubar must not alias u or gbar.  Uses the domain's work arrays,
+ * allocated on first use: 150 doubles per grid point, and on uniform domains
+ * 300 doubles per element and point of its largest face.  No counterpart in
+ * the reference (which has no adjoints). */
 int sxd_gh_rhs_adjoint(const sxd_domain* domain, const double* u,
                        const double* gammas, const double* inv_jac,
                        const double* gbar, double* ubar, void* stream);

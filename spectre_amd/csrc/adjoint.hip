@@ -20,6 +20,8 @@
 // scatters them, so results equal the oracle bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cstdint>
 
 #include "device_common.hpp"
@@ -213,10 +215,11 @@ rk3_adjoint_substep_kernel(int substep, int64_t n, double dt, const double* lam,
 //  1. point-wise: ubar = (df/du)^T gbar; the adjoint of the inertial
 //     derivatives contracted with the inverse Jacobian into logical-derivative
 //     adjoints xbar[3 c + j], written over du (read first, same point);
-//  2. transposed differentiation sweeps: ubar_c += sum_j D_j^T xbar_cj;
-//  3. faces, one launch per dimension: every face point adds the transposed
-//     lifted UpwindPenalty terms of its face, as interior point of its own
-//     face and as exterior point of the neighbour's face.
+//  2. faces: every face point's transposed lifted UpwindPenalty terms of each
+//     face it lies on, as interior point of its own face and as exterior
+//     point of the neighbour's face, into a per-face work array;
+//  3. transposed differentiation sweeps ubar_c += sum_j D_j^T xbar_cj, then
+//     the face terms.
 __global__ void __launch_bounds__(128)
 gh_adjoint_pointwise_kernel(int64_t total, int np, const double* __restrict__ u,
                             double* du_xbar, const double* __restrict__ gammas,
@@ -249,11 +252,14 @@ gh_adjoint_pointwise_kernel(int64_t total, int np, const double* __restrict__ u,
   }
 }
 
-// Grid: (E, 50 / kSweepChunk); dynamic LDS: 3 Np doubles.
+// ubar_c += sum_j D_j^T xbar_cj, then the face terms of the points on the
+// element's faces (contrib: [E][6][50][F], F = largest face, directions in
+// order).  Grid: (E, 50 / kSweepChunk); dynamic LDS: 3 Np doubles.
 constexpr int kSweepChunk = 5;
 __global__ void __launch_bounds__(256)
 transposed_sweeps_kernel(int n0, int n1, int n2, int nc, const double* __restrict__ xbar,
-                         double* __restrict__ ubar, const double* __restrict__ dtab) {
+                         double* __restrict__ ubar, const double* __restrict__ dtab,
+                         const double* __restrict__ contrib, int F) {
   extern __shared__ double s_x[];
   const int np = n0 * n1 * n2;
   const int64_t e = blockIdx.x;
@@ -274,28 +280,38 @@ transposed_sweeps_kernel(int n0, int n1, int n2, int nc, const double* __restric
         for (int m = 0; m < n[j]; ++m)
           acc = fma(D[j][m * n[j] + idx[j]], s_x[j * np + base + m * stride[j]], acc);
       }
+      for (int d = 0; d < 3; ++d) {
+        if (n[d] == 1) continue;
+        const int side = idx[d] == 0 ? 0 : idx[d] == n[d] - 1 ? 1 : -1;
+        if (side < 0) continue;
+        const int a = d == 0 ? idx[1] : idx[0], b = d == 2 ? idx[1] : idx[2];
+        const int f = a + (d == 0 ? n1 : n0) * b;
+        acc += contrib[((e * 6 + 2 * d + side) * nc + c) * F + f];
+      }
       ubar[(e * nc + c) * np + p] = acc;
     }
   }
 }
 
-// Faces normal to dimension d, one thread per (element, side, face point):
-// the lower and upper faces hold different points, so no two threads of a
-// launch update the same point.  Grid: E * 2 * N_a * N_b threads.
+// Faces, one thread per (element, face, face point): the point's face terms
+// (its own correction's and the neighbour's, gh_face_pair_vjp) go to its
+// slot of contrib [E][6][50][F] (F = largest face), which the transposed
+// sweeps add; no two threads write the same value.  Grid: E * 6 * F threads.
 __global__ void __launch_bounds__(64)
-gh_adjoint_face_kernel(int d, int64_t total, int n0, int n1, int n2,
+gh_adjoint_face_kernel(int64_t total, int F, int n0, int n1, int n2,
                        const int32_t* __restrict__ nbr, const double* __restrict__ u,
                        const double* __restrict__ gammas, const double* __restrict__ inv_jac,
-                       const double* __restrict__ gbar, double* __restrict__ ubar) {
+                       const double* __restrict__ gbar, double* __restrict__ contrib) {
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= total) return;
   const int np = n0 * n1 * n2;
   const int n[3] = {n0, n1, n2};
+  const int64_t e = gid / (6 * F);
+  const int r = static_cast<int>(gid % (6 * F));
+  const int dir = r / F, f = r % F;
+  const int d = dir >> 1, s = dir & 1;
   const int ad = d == 0 ? 1 : 0, bd = d == 2 ? 1 : 2;
-  const int per_face = n[ad] * n[bd];
-  const int64_t e = gid / (2 * per_face);
-  const int r = static_cast<int>(gid % (2 * per_face));
-  const int s = r / per_face, f = r % per_face;
+  if (n[d] == 1 || f >= n[ad] * n[bd]) return;
   int pi[3], qi[3];
   pi[ad] = qi[ad] = f % n[ad];
   pi[bd] = qi[bd] = f / n[ad];
@@ -303,7 +319,7 @@ gh_adjoint_face_kernel(int d, int64_t total, int n0, int n1, int n2,
   qi[d] = s == 0 ? n[d] - 1 : 0;
   const int p = pi[0] + n0 * (pi[1] + n1 * pi[2]);
   const int q = qi[0] + n0 * (qi[1] + n1 * qi[2]);
-  const int64_t en = nbr[6 * e + 2 * d + s];
+  const int64_t en = nbr[6 * e + dir];
   const double* gap = gammas + e * 3 * np + p;
   const double* gaq = gammas + en * 3 * np + q;
   // normal covectors pointing out of each point's own element
@@ -314,11 +330,13 @@ gh_adjoint_face_kernel(int d, int64_t total, int n0, int n1, int n2,
     if (s == 0) np_row[i] = np_row[i] * -1.0;
     if (s == 1) nq_row[i] = nq_row[i] * -1.0;
   }
+  double* out = contrib + (e * 6 + dir) * 50 * static_cast<int64_t>(F) + f;
+  for (int c = 0; c < 50; ++c) out[c * F] = 0.0;
   // this element's face (d, s) at p and the neighbour's face (d, 1 - s) at
   // q: each point is the other's exterior
   gh_face_pair_vjp(u + e * 50 * np + p, gap[np], gap[2 * np], np_row, u + en * 50 * np + q,
                    gaq[np], gaq[2 * np], nq_row, n[d], np, gbar + e * 50 * np + p,
-                   gbar + en * 50 * np + q, np, ubar + e * 50 * np + p, np);
+                   gbar + en * 50 * np + q, np, out, F);
 }
 
 }  // namespace
@@ -351,10 +369,14 @@ int launch_sw_rhs_adjoint(int num_elements, const int32_t ext[3], const int32_t*
   }
 }
 
+int gh_adjoint_face_slots(const int32_t ext[3]) {
+  return std::max({ext[1] * ext[2], ext[0] * ext[2], ext[0] * ext[1]});
+}
+
 int launch_gh_rhs_adjoint(int num_elements, const int32_t ext[3], const int32_t* nbr,
                           const double* u, const double* gammas, const double* inv_jac,
                           const double* gbar, double* ubar, double* scratch_du,
-                          hipStream_t stream) {
+                          double* scratch_faces, hipStream_t stream) {
   const double* dtab = device_diff_matrices();
   if (dtab == nullptr) return SXD_ERR_DEVICE;
   const int np = ext[0] * ext[1] * ext[2];
@@ -366,20 +388,17 @@ int launch_gh_rhs_adjoint(int num_elements, const int32_t ext[3], const int32_t*
   hipLaunchKernelGGL(gh_adjoint_pointwise_kernel, dim3(static_cast<unsigned>((total + 127) / 128)),
                      dim3(128), 0, stream, total, np, u, scratch_du, gammas, inv_jac, gbar, ubar);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  const int F = gh_adjoint_face_slots(ext);
+  const int64_t faces = static_cast<int64_t>(num_elements) * 6 * F;
+  hipLaunchKernelGGL(gh_adjoint_face_kernel, dim3(static_cast<unsigned>((faces + 63) / 64)),
+                     dim3(64), 0, stream, faces, F, ext[0], ext[1], ext[2], nbr, u, gammas,
+                     inv_jac, gbar, scratch_faces);
+  if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   hipLaunchKernelGGL(transposed_sweeps_kernel,
                      dim3(num_elements, (50 + kSweepChunk - 1) / kSweepChunk), dim3(256),
                      3 * np * sizeof(double), stream, ext[0], ext[1], ext[2], 50, scratch_du,
-                     ubar, dtab);
-  if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
-  for (int d = 0; d < 3; ++d) {
-    if (ext[d] == 1) continue;
-    const int64_t faces = static_cast<int64_t>(num_elements) * 2 * (np / ext[d]);
-    hipLaunchKernelGGL(gh_adjoint_face_kernel, dim3(static_cast<unsigned>((faces + 63) / 64)),
-                       dim3(64), 0, stream, d, faces, ext[0], ext[1], ext[2], nbr, u, gammas,
-                       inv_jac, gbar, ubar);
-    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
-  }
-  return SXD_OK;
+                     ubar, dtab, scratch_faces, F);
+  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
 }
 
 int launch_rk3_adjoint_substep(int substep, int64_t n, double dt, const double* lam,

@@ -725,10 +725,12 @@ int sxd_gh_rhs_adjoint(const sxd_domain* domain, const double* u, const double* 
                   __func__);
   }
   const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
-  if (ensure_scratch(d, 150 * np * d->num_elements, 0) != SXD_OK)
+  const size_t face_doubles = 300 * static_cast<size_t>(gh_adjoint_face_slots(d->ext));
+  if (ensure_scratch(d, 150 * np * d->num_elements, face_doubles * d->num_elements) != SXD_OK)
     return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
   return finish(launch_gh_rhs_adjoint(d->num_elements, d->ext, d->d_nbr, u, gammas, inv_jac, gbar,
-                                      ubar, d->d_scratch_du, static_cast<hipStream_t>(stream)),
+                                      ubar, d->d_scratch_du, d->d_scratch_temps,
+                                      static_cast<hipStream_t>(stream)),
                 __func__);
 }
 

@@ -78,11 +78,13 @@ int launch_sw_rhs_adjoint(int num_elements, const int32_t ext[3], const int32_t*
                           const double* gbar, const double* gamma2, const double* inv_jac,
                           double* ubar, hipStream_t stream);
 // Adjoint of the GH RHS linearised at u (uniform extents, Np <= 2730):
-// ubar = (d dt / d u)^T gbar; scratch_du: 150 x E x Np doubles.
+// ubar = (d dt / d u)^T gbar; scratch_du: 150 x E x Np doubles; scratch_faces:
+// 300 x E x gh_adjoint_face_slots(ext) doubles.
+int gh_adjoint_face_slots(const int32_t ext[3]);
 int launch_gh_rhs_adjoint(int num_elements, const int32_t ext[3], const int32_t* nbr,
                           const double* u, const double* gammas, const double* inv_jac,
                           const double* gbar, double* ubar, double* scratch_du,
-                          hipStream_t stream);
+                          double* scratch_faces, hipStream_t stream);
 int launch_rk3_adjoint_substep(int substep, int64_t n, double dt, const double* lam,
                                const double* a, double* mu, double* lam_out, hipStream_t stream);
 int launch_rk3_substep_dt(int substep, int64_t n, const double* dt, const double* u0,
