@@ -538,6 +538,11 @@ __device__ void face_side_adjoint(const MixedElement& m, int d, int side, bool f
 // value; launching d = 0, 1, 2 in order adds each point's corrections in
 // direction order.  Every value is computed with the arithmetic of the
 // oracle's mortar coupling.
+// kFusedNb: the neighbour side's dimension-b interpolation is done per mortar
+// point where its boundary terms are formed, without a mortar buffer for that
+// side (less LDS, so more workgroups per CU, but one thread per mortar point
+// in that phase); otherwise as apply_face_matrices into its own buffer.
+template <bool kFusedNb>
 struct FacePassPlan {
   int face, mag, tmp, m_own, m_nb, total;
   __host__ __device__ FacePassPlan(int kf, int km) {
@@ -546,18 +551,19 @@ struct FacePassPlan {
     tmp = mag + kf;         // 17 per mortar point
     m_own = tmp + 17 * km;  // own side on the mortar; then the mortar corrections
     m_nb = m_own + 17 * km;
-    total = m_nb + 17 * km;
+    total = m_nb + (kFusedNb ? 0 : 17 * km);
   }
 };
 
-__global__ void __launch_bounds__(256)
+template <bool kFusedNb>
+__global__ void __launch_bounds__(256, 3)
 gh_mixed_face_pass_kernel(int d, const MixedElement* __restrict__ el,
                           const int32_t* __restrict__ list, const int32_t* __restrict__ nbr,
                           const double* __restrict__ u, const double* __restrict__ params,
                           const double* __restrict__ inv_jac, double* __restrict__ dt,
                           const double* __restrict__ mortar_tab, int kf, int km) {
   extern __shared__ double sm[];
-  const FacePassPlan L(kf, km);
+  const FacePassPlan<kFusedNb> L(kf, km);
   const int t = threadIdx.x, T = blockDim.x;
   const int side = blockIdx.y, pr = blockIdx.z;
   const int e = list[blockIdx.x];
@@ -594,25 +600,69 @@ gh_mixed_face_pass_kernel(int d, const MixedElement* __restrict__ el,
     face_package<true>(mn, d, 1 - side, f, mf, pr, u, params, sd, face + 4 * mf);
   }
   __syncthreads();
-  apply_face_matrices(17, ma, mb, ka, kb, Ia_nb, Ib_nb, face, sm + L.tmp, sm + L.m_nb);
-  // boundary terms at the mortar points, written over the own side's slots
-  for (int k = t; k < kmf; k += T) {
-    double in[13], ex[13], li[4], le[4], bc[5];
+  if constexpr (!kFusedNb) {
+    apply_face_matrices(17, ma, mb, ka, kb, Ia_nb, Ib_nb, face, sm + L.tmp, sm + L.m_nb);
+    // boundary terms at the mortar points, written over the own side's slots
+    for (int k = t; k < kmf; k += T) {
+      double in[13], ex[13], li[4], le[4], bc[5];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      li[c] = sm[L.m_own + c * kmf + k];
-      le[c] = sm[L.m_nb + c * kmf + k];
+      for (int c = 0; c < 4; ++c) {
+        li[c] = sm[L.m_own + c * kmf + k];
+        le[c] = sm[L.m_nb + c * kmf + k];
+      }
+#pragma unroll
+      for (int c = 0; c < 13; ++c) {
+        in[c] = sm[L.m_own + (4 + c) * kmf + k];
+        ex[c] = sm[L.m_nb + (4 + c) * kmf + k];
+      }
+      gh_boundary_terms_pair(bc, in, li, ex, le);
+#pragma unroll
+      for (int c = 0; c < 5; ++c) sm[L.m_own + c * kmf + k] = bc[c];
     }
-#pragma unroll
-    for (int c = 0; c < 13; ++c) {
-      in[c] = sm[L.m_own + (4 + c) * kmf + k];
-      ex[c] = sm[L.m_nb + (4 + c) * kmf + k];
+    __syncthreads();
+  } else {
+    // the neighbour side on the mortar: dimension a through tmp, then
+    // dimension b for each mortar point where its boundary terms are formed
+    // (apply_face_matrices' sums, without a mortar buffer for this side)
+    const double* src_nb = face;
+    if (ma != ka) {
+      for (int x = t; x < 17 * ka * mb; x += T) {
+        const int c = x / (ka * mb), rem = x % (ka * mb), b = rem / ka, r = rem % ka;
+        const double* v = face + c * mf + ma * b;
+        double sv = Ia_nb[r * ma] * v[0];
+        for (int l = 1; l < ma; ++l) sv = fma(Ia_nb[r * ma + l], v[l], sv);
+        sm[L.tmp + x] = sv;
+      }
+      __syncthreads();
+      src_nb = sm + L.tmp;
     }
-    gh_boundary_terms_pair(bc, in, li, ex, le);
+    // boundary terms at the mortar points, written over the own side's slots
+    for (int k = t; k < kmf; k += T) {
+      const int ra = k % ka, rb = k / ka;
+      double nbv[17];
 #pragma unroll
-    for (int c = 0; c < 5; ++c) sm[L.m_own + c * kmf + k] = bc[c];
+      for (int c = 0; c < 17; ++c) {
+        const double* v = src_nb + c * ka * mb + ra;
+        double sv;
+        if (mb == kb) {
+          sv = v[ka * rb];
+        } else {
+          sv = Ib_nb[rb * mb] * v[0];
+          for (int l = 1; l < mb; ++l) sv = fma(Ib_nb[rb * mb + l], v[ka * l], sv);
+        }
+        nbv[c] = sv;
+      }
+      double in[13], li[4], bc[5];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) li[c] = sm[L.m_own + c * kmf + k];
+#pragma unroll
+      for (int c = 0; c < 13; ++c) in[c] = sm[L.m_own + (4 + c) * kmf + k];
+      gh_boundary_terms_pair(bc, in, li, nbv + 4, nbv);
+#pragma unroll
+      for (int c = 0; c < 5; ++c) sm[L.m_own + c * kmf + k] = bc[c];
+    }
+    __syncthreads();
   }
-  __syncthreads();
   apply_face_matrices(5, ka, kb, na, nb, Pa, Pb, sm + L.m_own, sm + L.tmp, face);
   // lift into dt: components g, Pi, Phi_0..2 of the pair
   for (int f = t; f < nf; f += T) {
@@ -848,7 +898,9 @@ bool enable_large_lds() {
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
       hipFuncSetAttribute(reinterpret_cast<const void*>(&sw_mixed_face_kernel),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
-      hipFuncSetAttribute(reinterpret_cast<const void*>(&gh_mixed_face_pass_kernel),
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&gh_mixed_face_pass_kernel<false>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&gh_mixed_face_pass_kernel<true>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
       hipFuncSetAttribute(reinterpret_cast<const void*>(&gh_mixed_face_adjoint_pass_kernel),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
@@ -892,10 +944,20 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
   hipLaunchKernelGGL(gh_mixed_pointwise_kernel, dim3(count), dim3(threads), 0, stream, el, list,
                      u, scratch_du, params, dt);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
-  const int pass_lds = FacePassPlan(kf, km).total * 8;
+  // the fused neighbour interpolation only where its smaller LDS plan fits
+  // more workgroups on a CU
+  const int lds_split = FacePassPlan<false>(kf, km).total * 8;
+  const int lds_fused = FacePassPlan<true>(kf, km).total * 8;
+  const bool fused_nb = 160 * 1024 / lds_fused > 160 * 1024 / lds_split;
   for (int d = 0; d < 3; ++d) {
-    hipLaunchKernelGGL(gh_mixed_face_pass_kernel, dim3(count, 2, 10), dim3(threads), pass_lds,
-                       stream, d, el, list, nbr, u, params, inv_jac, dt, mtab, kf, km);
+    if (fused_nb)
+      hipLaunchKernelGGL(gh_mixed_face_pass_kernel<true>, dim3(count, 2, 10), dim3(threads),
+                         lds_fused, stream, d, el, list, nbr, u, params, inv_jac, dt, mtab, kf,
+                         km);
+    else
+      hipLaunchKernelGGL(gh_mixed_face_pass_kernel<false>, dim3(count, 2, 10), dim3(threads),
+                         lds_split, stream, d, el, list, nbr, u, params, inv_jac, dt, mtab, kf,
+                         km);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   }
   return SXD_OK;
