@@ -694,6 +694,8 @@ gh_mixed_face_pass_kernel(int d, const MixedElement* __restrict__ el,
 // (11 per pair at each face point, indexed by the point's volume index);
 // gh_mixed_side_adjoint_kernel sums them over the pairs and takes them to g.
 // Dynamic LDS: FaceAdjPassPlan(kf, km).total doubles.
+// kFusedNb as in FacePassPlan: without a mortar buffer for the neighbour side.
+template <bool kFusedNb>
 struct FaceAdjPassPlan {
   int side, mag_nb, face, tmp, m_own, m_nb, total;
   __host__ __device__ FaceAdjPassPlan(int kf, int km) {
@@ -702,13 +704,15 @@ struct FaceAdjPassPlan {
     face = mag_nb + kf;       // 17 per face point (speeds, package); then
                               // corrections, seeds, package adjoint
     tmp = face + 17 * kf;
-    m_own = tmp + 17 * km;   // own speeds, package; then seeds on the mortar
-    m_nb = m_own + 17 * km;  // neighbour's; then corrections, package adjoint
-    total = m_nb + 17 * km;
+    m_own = tmp + 17 * km;   // own speeds, package; then corrections, seeds on
+                             // the mortar, package adjoint (all in place)
+    m_nb = m_own + 17 * km;  // the neighbour side on the mortar
+    total = m_nb + (kFusedNb ? 0 : 17 * km);
   }
 };
 
-__global__ void __launch_bounds__(256)
+template <bool kFusedNb>
+__global__ void __launch_bounds__(256, 3)
 gh_mixed_face_adjoint_pass_kernel(int d, const MixedElement* __restrict__ el,
                                   const int32_t* __restrict__ list,
                                   const int32_t* __restrict__ nbr, const double* __restrict__ u,
@@ -718,7 +722,7 @@ gh_mixed_face_adjoint_pass_kernel(int d, const MixedElement* __restrict__ el,
                                   double* __restrict__ side_adj,
                                   const double* __restrict__ mortar_tab, int kf, int km) {
   extern __shared__ double sm[];
-  const FaceAdjPassPlan L(kf, km);
+  const FaceAdjPassPlan<kFusedNb> L(kf, km);
   const int t = threadIdx.x, T = blockDim.x;
   const int side = blockIdx.y, pr = blockIdx.z;
   const int e = list[blockIdx.x];
@@ -778,26 +782,53 @@ gh_mixed_face_adjoint_pass_kernel(int d, const MixedElement* __restrict__ el,
     for (int c = 0; c < 13; ++c) face[(4 + c) * mf + f] = pk[c];
   }
   __syncthreads();
-  apply_face_matrices(17, ma, mb, ka, kb, Ia_nb, Ib_nb, face, sm + L.tmp, sm + L.m_nb);
-  // e's corrections on the mortar, over the neighbour's package slots
+  // the neighbour side on the mortar (as apply_face_matrices; with kFusedNb
+  // its dimension-b sums are formed per mortar point below), and e's
+  // corrections there, over e's package slots
+  const double* src_nb = face;
+  if constexpr (!kFusedNb) {
+    apply_face_matrices(17, ma, mb, ka, kb, Ia_nb, Ib_nb, face, sm + L.tmp, sm + L.m_nb);
+  } else if (ma != ka) {
+    for (int x = t; x < 17 * ka * mb; x += T) {
+      const int c = x / (ka * mb), rem = x % (ka * mb), b = rem / ka, r = rem % ka;
+      const double* v = face + c * mf + ma * b;
+      double sv = Ia_nb[r * ma] * v[0];
+      for (int l = 1; l < ma; ++l) sv = fma(Ia_nb[r * ma + l], v[l], sv);
+      sm[L.tmp + x] = sv;
+    }
+    __syncthreads();
+    src_nb = sm + L.tmp;
+  }
   for (int k = t; k < kmf; k += T) {
-    double in[13], ex[13], li[4], le[4], bc[5];
+    double in[13], li[4], nbv[17], bc[5];  // nbv: neighbour speeds, package
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      li[c] = sm[L.m_own + c * kmf + k];
-      le[c] = sm[L.m_nb + c * kmf + k];
+    for (int c = 0; c < 4; ++c) li[c] = sm[L.m_own + c * kmf + k];
+#pragma unroll
+    for (int c = 0; c < 13; ++c) in[c] = sm[L.m_own + (4 + c) * kmf + k];
+    if constexpr (!kFusedNb) {
+#pragma unroll
+      for (int c = 0; c < 17; ++c) nbv[c] = sm[L.m_nb + c * kmf + k];
+    } else {
+      const int ra = k % ka, rb = k / ka;
+#pragma unroll
+      for (int c = 0; c < 17; ++c) {
+        const double* v = src_nb + c * ka * mb + ra;
+        double sv;
+        if (mb == kb) {
+          sv = v[ka * rb];
+        } else {
+          sv = Ib_nb[rb * mb] * v[0];
+          for (int l = 1; l < mb; ++l) sv = fma(Ib_nb[rb * mb + l], v[ka * l], sv);
+        }
+        nbv[c] = sv;
+      }
     }
+    gh_boundary_fwd(in, li, nbv + 4, nbv, bc);
 #pragma unroll
-    for (int c = 0; c < 13; ++c) {
-      in[c] = sm[L.m_own + (4 + c) * kmf + k];
-      ex[c] = sm[L.m_nb + (4 + c) * kmf + k];
-    }
-    gh_boundary_fwd(in, li, ex, le, bc);
-#pragma unroll
-    for (int c = 0; c < 5; ++c) sm[L.m_nb + (4 + c) * kmf + k] = bc[c];
+    for (int c = 0; c < 5; ++c) sm[L.m_own + (4 + c) * kmf + k] = bc[c];
   }
   __syncthreads();
-  apply_face_matrices(5, ka, kb, na, nb, Pa, Pb, sm + L.m_nb + 4 * kmf, sm + L.tmp, face);
+  apply_face_matrices(5, ka, kb, na, nb, Pa, Pb, sm + L.m_own + 4 * kmf, sm + L.tmp, face);
   // seeds on both faces; the pair's part of the |n| adjoint of e's lift factor
   double* seed_own = face + 5 * kf;
   double* seed_nb = face + 10 * kf;
@@ -835,10 +866,10 @@ gh_mixed_face_adjoint_pass_kernel(int d, const MixedElement* __restrict__ el,
     }
     gh_package_seed(li, bi, bx, B);
 #pragma unroll
-    for (int c = 0; c < 13; ++c) sm[L.m_nb + c * kmf + k] = B[c];
+    for (int c = 0; c < 13; ++c) sm[L.m_own + c * kmf + k] = B[c];
   }
   __syncthreads();
-  apply_face_matrices_t(13, ka, kb, na, nb, Ia_own, Ib_own, sm + L.m_nb, sm + L.tmp, face);
+  apply_face_matrices_t(13, ka, kb, na, nb, Ia_own, Ib_own, sm + L.m_own, sm + L.tmp, face);
   for (int f = t; f < nf; f += T) {
     const double* o = sm + L.side + 12 * f;
     const int p = face_point(m, d, side, f % na, f / na);
@@ -902,7 +933,9 @@ bool enable_large_lds() {
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
       hipFuncSetAttribute(reinterpret_cast<const void*>(&gh_mixed_face_pass_kernel<true>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
-      hipFuncSetAttribute(reinterpret_cast<const void*>(&gh_mixed_face_adjoint_pass_kernel),
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&gh_mixed_face_adjoint_pass_kernel<false>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&gh_mixed_face_adjoint_pass_kernel<true>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
       hipFuncSetAttribute(reinterpret_cast<const void*>(&ragged_transposed_sweeps_kernel),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess)
@@ -963,7 +996,9 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
   return SXD_OK;
 }
 
-int mixed_face_adjoint_lds_bytes(int kf, int km) { return FaceAdjPassPlan(kf, km).total * 8; }
+int mixed_face_adjoint_lds_bytes(int kf, int km) {
+  return FaceAdjPassPlan<false>(kf, km).total * 8;
+}
 
 int launch_mixed_gh_adjoint(int count, const int32_t* list, const MixedElement* el,
                             const int32_t* point_el, int64_t total_points,
@@ -991,11 +1026,18 @@ int launch_mixed_gh_adjoint(int count, const int32_t* list, const MixedElement* 
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   // faces: the derivative work array is free again and holds the side
   // adjoints (110 doubles per point, under the 150 it has)
-  const int pass_lds = FaceAdjPassPlan(kf, km).total * 8;
+  const int lds_split = FaceAdjPassPlan<false>(kf, km).total * 8;
+  const int lds_fused = FaceAdjPassPlan<true>(kf, km).total * 8;
+  const bool fused_nb = 160 * 1024 / lds_fused > 160 * 1024 / lds_split;
   for (int d = 0; d < 3; ++d) {
-    hipLaunchKernelGGL(gh_mixed_face_adjoint_pass_kernel, dim3(count, 2, 10), dim3(256),
-                       pass_lds, stream, d, el, list, nbr, u, gammas, inv_jac, gbar, ubar,
-                       scratch_du, mtab, kf, km);
+    if (fused_nb)
+      hipLaunchKernelGGL(gh_mixed_face_adjoint_pass_kernel<true>, dim3(count, 2, 10), dim3(256),
+                         lds_fused, stream, d, el, list, nbr, u, gammas, inv_jac, gbar, ubar,
+                         scratch_du, mtab, kf, km);
+    else
+      hipLaunchKernelGGL(gh_mixed_face_adjoint_pass_kernel<false>, dim3(count, 2, 10),
+                         dim3(256), lds_split, stream, d, el, list, nbr, u, gammas, inv_jac,
+                         gbar, ubar, scratch_du, mtab, kf, km);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
     hipLaunchKernelGGL(gh_mixed_side_adjoint_kernel, dim3(count, 2), dim3(128), 0, stream, d, el,
                        list, u, gammas, inv_jac, scratch_du, ubar);
