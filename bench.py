@@ -217,8 +217,9 @@ def measure_adjoint(steps=40, reps=3):
 
 def measure_gh_adjoint(steps=20, reps=2, mixed=False):
     """Adjoint GH workload (300 elements, 20 RK3 steps, J = L2 norm of the
-    gauge constraint): forward evolution against the backward (recompute of
-    each 8-step segment from its checkpoint + hand-written adjoint kernels).
+    gauge constraint): forward evolution against the backward (hand-written
+    adjoint kernels), with a checkpoint every 8 steps (earlier segments
+    recomputed) and with every stage input kept by the forward.
     Uniform: 6^3 points per element; mixed: extents drawn from {6..10} per
     element and dimension."""
     import numpy as np
@@ -243,12 +244,12 @@ def measure_gh_adjoint(steps=20, reps=2, mixed=False):
         points, name = brick.num_elements * brick.num_points, \
             "gh_adjoint_300el_6^3_20_rk3_steps_gauge_constraint"
 
-    def run():
+    def run(k):
         u0 = dev(u).requires_grad_(True)
         torch.cuda.synchronize()
         t0, t1, t2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         t0.record()
-        uf = ev.GhEvolution.apply(u0, domain, d_g, d_inv, dt, steps, 8)
+        uf = ev.GhEvolution.apply(u0, domain, d_g, d_inv, dt, steps, k)
         t1.record()
         J = functional(uf)
         J.backward()
@@ -256,16 +257,26 @@ def measure_gh_adjoint(steps=20, reps=2, mixed=False):
         torch.cuda.synchronize()
         return t0.elapsed_time(t1), t1.elapsed_time(t2)
 
-    run()
-    fwd, bwd = zip(*[run() for _ in range(reps)])
+    # checkpoint every 8 steps (earlier segments recomputed in the backward),
+    # and one segment (the forward keeps every stage input: 3 states per step,
+    # a few GB at these sizes, no recomputation)
+    timings = {}
+    for k in (8, steps):
+        run(k)
+        fwd, bwd = zip(*[run(k) for _ in range(reps)])
+        timings[k] = (min(fwd), min(bwd))
     u_d, gb = dev(u), dev(u)
     out = torch.empty_like(u_d)
     rhs_ms = time_calls(lambda: domain.gh_rhs(u_d, d_g, d_inv, out=out), 5, 1)
     adj_ms = time_calls(lambda: domain.gh_rhs_adjoint(u_d, d_g, d_inv, gb, out=out), 3, 1)
-    f = min(fwd)
+    f8, b8 = timings[8]
+    fa, ba = timings[steps]
     return {"workload": name, "grid_points": int(points), "steps": steps, "dt": dt,
-            "forward_ms": round(f, 3), "backward_with_recompute_ms": round(min(bwd), 3),
-            "adjoint_forward_cost": round(min(bwd) / f, 3),
+            "forward_ms": round(f8, 3), "backward_with_recompute_ms": round(b8, 3),
+            "adjoint_forward_cost": round(b8 / f8, 3),
+            "forward_ms_all_stages_kept": round(fa, 3),
+            "backward_ms_all_stages_kept": round(ba, 3),
+            "adjoint_forward_cost_all_stages_kept": round(ba / fa, 3),
             "rhs_ms": round(rhs_ms, 4), "adjoint_rhs_ms": round(adj_ms, 4)}
 
 

@@ -114,19 +114,25 @@ class GhEvolution(torch.autograd.Function):
     inputs u, u1, u2: the forward keeps the state every `checkpoint_every`
     steps, and the backward recomputes the stage inputs of one segment at a
     time from its checkpoint, then applies the transposed substeps with the
-    adjoint RHS linearised at those stages (sxd_gh_rhs_adjoint)."""
+    adjoint RHS linearised at those stages (sxd_gh_rhs_adjoint).  The last
+    segment's stage inputs are kept by the forward itself (the backward
+    starts there), so only the earlier segments are recomputed."""
 
     @staticmethod
     def forward(ctx, u0, domain, gammas, inv_jac, dt, steps, checkpoint_every=8):
         u = u0.detach().clone()
         l = torch.empty_like(u)
-        checkpoints = []
+        checkpoints, last_stages = [], []
+        last_first = (max(steps - 1, 0) // checkpoint_every) * checkpoint_every
         for n in range(steps):
             if n % checkpoint_every == 0:
                 checkpoints.append(u.clone())
-            rk3_step(domain, domain.gh_rhs, u, gammas, inv_jac, dt, torch.empty_like(u), l)
+            if n >= last_first:
+                last_stages.append(rk3_step_states(domain.gh_rhs, u, gammas, inv_jac, dt, l))
+            else:
+                rk3_step(domain, domain.gh_rhs, u, gammas, inv_jac, dt, torch.empty_like(u), l)
         ctx.domain, ctx.dt, ctx.steps, ctx.k = domain, dt, steps, checkpoint_every
-        ctx.checkpoints = checkpoints
+        ctx.checkpoints, ctx.last_stages = checkpoints, last_stages
         ctx.save_for_backward(gammas, inv_jac)
         return u
 
@@ -139,9 +145,12 @@ class GhEvolution(torch.autograd.Function):
         for seg in reversed(range(len(ctx.checkpoints))):
             first = seg * k
             last = min(ctx.steps, first + k)
-            u = ctx.checkpoints[seg].clone()
-            stages = [rk3_step_states(domain.gh_rhs, u, gammas, inv_jac, dt, l)
-                      for _ in range(first, last)]
+            if seg == len(ctx.checkpoints) - 1:
+                stages = ctx.last_stages
+            else:
+                u = ctx.checkpoints[seg].clone()
+                stages = [rk3_step_states(domain.gh_rhs, u, gammas, inv_jac, dt, l)
+                          for _ in range(first, last)]
             for s0, s1, s2 in reversed(stages):
                 domain.gh_rhs_adjoint(s2, gammas, inv_jac, lam3, out=a)
                 rk3_adjoint_substep(2, dt, lam3, a, mu, lam)
@@ -150,7 +159,7 @@ class GhEvolution(torch.autograd.Function):
                 domain.gh_rhs_adjoint(s0, gammas, inv_jac, lam, out=a)
                 rk3_adjoint_substep(0, dt, lam, a, mu, None)
                 lam3, mu = mu, lam3
-        ctx.checkpoints = None
+        ctx.checkpoints = ctx.last_stages = None
         return lam3, None, None, None, None, None, None
 
 
@@ -160,39 +169,46 @@ def gauge_constraint(u):
     Phi and the normals, as gh::gauge_constraint / gh::trace_christoffel
     (Evolution/Systems/GeneralizedHarmonic/Constraints.cpp:983-998,
     PointwiseFunctions/GeneralRelativity/GeneralizedHarmonic/Christoffel.cpp:64-119).
-    u [E][50][Np] -> C [E][4][Np] (differentiable torch operations)."""
+    u [E][50][Np] -> C [E][4][Np] (differentiable torch operations: products
+    and sums over the small tensor indices, the inverse spatial metric by
+    cofactors)."""
     pairs = [(a, b) for a in range(4) for b in range(a, 4)]
     idx = {}
     for p, (a, b) in enumerate(pairs):
         idx[(a, b)] = idx[(b, a)] = p
-    full = lambda base: torch.stack([torch.stack([u[:, base + idx[(a, b)]] for b in range(4)], 1)
-                                     for a in range(4)], 1)          # [E][4][4][Np]
-    g, pi = full(0), full(10)
-    phi = torch.stack([torch.stack([torch.stack([u[:, 20 + i + 3 * idx[(a, b)]] for b in range(4)],
-                                                1) for a in range(4)], 1) for i in range(3)], 1)
-    spatial = g[:, 1:, 1:].permute(0, 3, 1, 2)                      # [E][Np][3][3]
-    inv_sp = torch.linalg.inv(spatial)
-    g0i = g[:, 1:, 0].permute(0, 2, 1)                              # [E][Np][3]
-    shift = torch.einsum("epij,epj->epi", inv_sp, g0i)
-    lapse = torch.sqrt(-g[:, 0, 0] + torch.einsum("epi,epi->ep", shift, g0i))
+    perm = torch.tensor([idx[(a, b)] for a in range(4) for b in range(4)], device=u.device)
+    x = u.permute(0, 2, 1)                                          # [E][Np][50]
+    g = x[..., perm].reshape(x.shape[0], x.shape[1], 4, 4)          # [E][Np][4][4]
+    pi = x[..., 10 + perm].reshape(g.shape)
+    phi = torch.stack([x[..., 20 + i + 3 * perm].reshape(g.shape) for i in range(3)], 2)
+    s = g[..., 1:, 1:]                                              # spatial metric
+    c00 = s[..., 1, 1] * s[..., 2, 2] - s[..., 1, 2] * s[..., 1, 2]
+    c01 = s[..., 1, 2] * s[..., 0, 2] - s[..., 0, 1] * s[..., 2, 2]
+    c02 = s[..., 0, 1] * s[..., 1, 2] - s[..., 1, 1] * s[..., 0, 2]
+    c11 = s[..., 2, 2] * s[..., 0, 0] - s[..., 0, 2] * s[..., 0, 2]
+    c12 = s[..., 0, 2] * s[..., 0, 1] - s[..., 0, 0] * s[..., 1, 2]
+    c22 = s[..., 0, 0] * s[..., 1, 1] - s[..., 0, 1] * s[..., 0, 1]
+    det = s[..., 0, 0] * c00 + s[..., 0, 1] * c01 + s[..., 0, 2] * c02
+    inv_sp = torch.stack([torch.stack([c00, c01, c02], -1), torch.stack([c01, c11, c12], -1),
+                          torch.stack([c02, c12, c22], -1)], -2) / det[..., None, None]
+    g0i = g[..., 1:, 0]                                             # [E][Np][3]
+    shift = (inv_sp * g0i[..., None, :]).sum(-1)
+    lapse = torch.sqrt(-g[..., 0, 0] + (shift * g0i).sum(-1))
     n_up = torch.cat([(1.0 / lapse)[..., None], -shift / lapse[..., None]], -1)   # [E][Np][4]
     m = -1.0 / lapse ** 2
     top = torch.cat([m[..., None], (-shift * m[..., None])], -1)    # [E][Np][4]
     lower = inv_sp + shift[..., :, None] * shift[..., None, :] * m[..., None, None]
     lower = torch.cat([(-shift * m[..., None])[..., None], lower], -1)  # [E][Np][3][4]
     inv_g = torch.cat([top[..., None, :], lower], -2)               # [E][Np][4][4]
-    pi_p = pi.permute(0, 3, 1, 2)                                   # [E][Np][4][4]
-    phi_p = phi.permute(0, 4, 1, 2, 3)                              # [E][Np][3][4][4]
-    k = -0.5 * (torch.einsum("epbc,epbc->ep", pi_p, inv_g)
-                + torch.einsum("epi,epibc,epbc->ep", n_up[..., 1:], phi_p, inv_g))
+    phi_ig = (phi * inv_g[..., None, :, :]).sum((-2, -1))           # [E][Np][3]
+    k = -0.5 * ((pi * inv_g).sum((-2, -1)) + (n_up[..., 1:] * phi_ig).sum(-1))
     c = []
     for a in range(4):
-        term = torch.einsum("epij,epij->ep", inv_sp, phi_p[:, :, :, 1:, a]) \
-            + torch.einsum("epb,epb->ep", n_up, pi_p[:, :, :, a])
+        term = (inv_sp * phi[..., :, 1:, a]).sum((-2, -1)) + (n_up * pi[..., :, a]).sum(-1)
         if a == 0:
             term = term - lapse * k
         else:
-            term = term - 0.5 * torch.einsum("epbc,epbc->ep", phi_p[:, :, a - 1], inv_g)
+            term = term - 0.5 * phi_ig[..., a - 1]
         c.append(term)
     return torch.stack(c, 1)                                        # [E][4][Np]
 
