@@ -236,7 +236,8 @@ int sxd_scalar_wave_rhs_adjoint(const sxd_domain* domain, const double* gbar,
                                 double* ubar, void* stream);
 /* Adjoint of the GH DG RHS of sxd_gh_rhs linearised at the state u:
  * ubar = (d dt / d u)^T gbar, the gradient with respect to u of a functional
- * J given gbar = dJ/d(dt).  Uniform domains (Np up to 2730): u, gbar,
+ * J given gbar = dJ/d(dt).  Uniform domains (3 Np + N0^2 + N1^2 + N2^2 <=
+ * 20480, e.g. 18^3): u, gbar,
  * ubar [E][50][Np]; mixed-p domains (sxd_domain_create_mixed): the ragged
  * layout.  ubar must not alias u or gbar.  Uses the domain's work arrays,
  * allocated on first use: 150 doubles per grid point, and on uniform domains

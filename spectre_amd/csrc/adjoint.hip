@@ -254,7 +254,8 @@ gh_adjoint_pointwise_kernel(int64_t total, int np, const double* __restrict__ u,
 
 // ubar_c += sum_j D_j^T xbar_cj, then the face terms of the points on the
 // element's faces (contrib: [E][6][50][F], F = largest face, directions in
-// order).  Grid: (E, 50 / kSweepChunk); dynamic LDS: 3 Np doubles.
+// order).  Grid: (E, 50 / kSweepChunk); dynamic LDS: 3 Np doubles, then
+// the three D matrices.
 constexpr int kSweepChunk = 5;
 __global__ void __launch_bounds__(256)
 transposed_sweeps_kernel(int n0, int n1, int n2, int nc, const double* __restrict__ xbar,
@@ -265,7 +266,8 @@ transposed_sweeps_kernel(int n0, int n1, int n2, int nc, const double* __restric
   const int64_t e = blockIdx.x;
   const int n[3] = {n0, n1, n2};
   const int stride[3] = {1, n0, n0 * n1};
-  const double* D[3] = {diff_matrix(dtab, n0), diff_matrix(dtab, n1), diff_matrix(dtab, n2)};
+  const double* D[3];
+  stage_diff_matrices(dtab, n, s_x + 3 * np, D);  // synchronised by the loop's first barrier
   const int c_end = min(nc, static_cast<int>(blockIdx.y + 1) * kSweepChunk);
   for (int c = blockIdx.y * kSweepChunk; c < c_end; ++c) {
     __syncthreads();
@@ -380,7 +382,11 @@ int launch_gh_rhs_adjoint(int num_elements, const int32_t ext[3], const int32_t*
   const double* dtab = device_diff_matrices();
   if (dtab == nullptr) return SXD_ERR_DEVICE;
   const int np = ext[0] * ext[1] * ext[2];
-  if (3 * np * 8 > 64 * 1024) return SXD_ERR_UNSUPPORTED;
+  const int d_doubles = ext[0] * ext[0] + ext[1] * ext[1] + ext[2] * ext[2];
+  const int sweep_lds = (3 * np + d_doubles) * static_cast<int>(sizeof(double));
+  if (sweep_lds > 160 * 1024) return SXD_ERR_UNSUPPORTED;
+  if (!allow_lds(reinterpret_cast<const void*>(&transposed_sweeps_kernel), sweep_lds))
+    return SXD_ERR_DEVICE;
   const int64_t total = static_cast<int64_t>(num_elements) * np;
   int rc = launch_partial_derivatives(num_elements, ext, 50, u, inv_jac, scratch_du, nullptr,
                                       stream);
@@ -396,8 +402,8 @@ int launch_gh_rhs_adjoint(int num_elements, const int32_t ext[3], const int32_t*
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   hipLaunchKernelGGL(transposed_sweeps_kernel,
                      dim3(num_elements, (50 + kSweepChunk - 1) / kSweepChunk), dim3(256),
-                     3 * np * sizeof(double), stream, ext[0], ext[1], ext[2], 50, scratch_du,
-                     ubar, dtab, scratch_faces, F);
+                     sweep_lds, stream, ext[0], ext[1], ext[2], 50, scratch_du, ubar, dtab,
+                     scratch_faces, F);
   return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
 }
 

@@ -21,6 +21,23 @@ __device__ __forceinline__ const double* diff_matrix(const double* table, int n)
   return table + packed_d_offset(n);
 }
 
+// Largest LDS copy of the three D matrices of an element (extents <= 20).
+constexpr int kStagedDiffDoubles = 3 * 20 * 20;
+
+// Copies the D matrices of extents n[0..2] from the device table into LDS at
+// dst (n0^2 + n1^2 + n2^2 doubles, at most kStagedDiffDoubles) and points
+// D[0..2] at the copies.  The caller synchronises the workgroup before use.
+__device__ __forceinline__ void stage_diff_matrices(const double* table, const int n[3],
+                                                    double* dst, const double* D[3]) {
+  int off = 0;
+  for (int d = 0; d < 3; ++d) {
+    const double* src = diff_matrix(table, n[d]);
+    for (int x = threadIdx.x; x < n[d] * n[d]; x += blockDim.x) dst[off + x] = src[x];
+    D[d] = dst + off;
+    off += n[d] * n[d];
+  }
+}
+
 // Offset of the (nf, nm) block pair in the packed mortar table (the same
 // value as packed_mortar_offset, in O(nf) steps).
 __host__ __device__ inline int mortar_table_offset(int nf, int nm) {

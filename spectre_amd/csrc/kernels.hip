@@ -9,6 +9,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
+#include <set>
+#include <utility>
 
 #include "device_common.hpp"
 #include "gh_point.hpp"
@@ -20,7 +23,7 @@ namespace sxd {
 // ---------------------------------------------------------------------------
 // Generic partial derivatives (any extents 2..20, any nc).
 // Grid: (E, number of component chunks); block: 256 threads; dynamic LDS:
-// Np doubles.  For each component of the chunk the block stages u_c in LDS,
+// Np doubles, then the three D matrices.  For each component of the chunk the block stages u_c in LDS,
 // then each thread forms the three logical derivatives of its points
 // (sum k = 0..N-1, PartialDerivatives.cpp:28-44) and, if inv_jac is given, the
 // inertial derivatives sum_j (d xi^j / d x^i) d_j u
@@ -36,9 +39,12 @@ partial_derivatives_kernel(int n0, int n1, int n2, int nc, int comps_per_block,
   const int64_t e = blockIdx.x;
   const int c_begin = blockIdx.y * comps_per_block;
   const int c_end = min(nc, c_begin + comps_per_block);
-  const double* D0 = diff_matrix(dtab, n0);
-  const double* D1 = diff_matrix(dtab, n1);
-  const double* D2 = diff_matrix(dtab, n2);
+  const int n[3] = {n0, n1, n2};
+  const double* Dm[3];
+  stage_diff_matrices(dtab, n, s_u + np, Dm);  // synchronised by the loop's first barrier
+  const double* D0 = Dm[0];
+  const double* D1 = Dm[1];
+  const double* D2 = Dm[2];
   for (int c = c_begin; c < c_end; ++c) {
     const double* uc = u + (e * nc + c) * np;
     __syncthreads();
@@ -74,6 +80,22 @@ partial_derivatives_kernel(int n0, int n1, int n2, int nc, int comps_per_block,
   }
 }
 
+bool allow_lds(const void* kernel, int bytes) {
+  if (bytes <= 64 * 1024) return true;
+  if (bytes > 160 * 1024) return false;
+  static std::mutex mutex;
+  static std::set<std::pair<int, const void*>> done;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return false;
+  std::lock_guard<std::mutex> lock(mutex);
+  if (done.count({dev, kernel}) != 0) return true;
+  if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) !=
+      hipSuccess)
+    return false;
+  done.insert({dev, kernel});
+  return true;
+}
+
 int launch_partial_derivatives(int num_elements, const int32_t ext[3], int nc,
                                const double* u, const double* inv_jac, double* du,
                                double* dlogical, hipStream_t stream) {
@@ -83,8 +105,12 @@ int launch_partial_derivatives(int num_elements, const int32_t ext[3], int nc,
   const int comps_per_block = nc < 4 ? nc : 4;
   dim3 grid(num_elements, (nc + comps_per_block - 1) / comps_per_block);
   const int64_t logical_stride = static_cast<int64_t>(num_elements) * nc * np;
-  hipLaunchKernelGGL(partial_derivatives_kernel, grid, dim3(256),
-                     np * sizeof(double), stream, ext[0], ext[1], ext[2], nc,
+  const int d_doubles = ext[0] * ext[0] + ext[1] * ext[1] + ext[2] * ext[2];
+  const int lds = (np + d_doubles) * static_cast<int>(sizeof(double));
+  if (!allow_lds(reinterpret_cast<const void*>(&partial_derivatives_kernel), lds))
+    return SXD_ERR_DEVICE;
+  hipLaunchKernelGGL(partial_derivatives_kernel, grid, dim3(256), lds, stream, ext[0], ext[1],
+                     ext[2], nc,
                      comps_per_block, u, inv_jac, du, dlogical, logical_stride, dtab);
   return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
 }

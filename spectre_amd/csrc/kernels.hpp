@@ -10,6 +10,9 @@
 
 namespace sxd {
 
+// Enables `bytes` of dynamic LDS for `kernel` on the current device (needed
+// beyond 64 KiB; done once per device and kernel).  False if it cannot be.
+bool allow_lds(const void* kernel, int bytes);
 int launch_partial_derivatives(int num_elements, const int32_t ext[3], int nc,
                                const double* u, const double* inv_jac, double* du,
                                double* dlogical, hipStream_t stream);
@@ -77,7 +80,8 @@ bool sw_adjoint_supported(const int32_t ext[3]);
 int launch_sw_rhs_adjoint(int num_elements, const int32_t ext[3], const int32_t* nbr,
                           const double* gbar, const double* gamma2, const double* inv_jac,
                           double* ubar, hipStream_t stream);
-// Adjoint of the GH RHS linearised at u (uniform extents, Np <= 2730):
+// Adjoint of the GH RHS linearised at u (uniform extents with
+// 3 Np + N0^2 + N1^2 + N2^2 <= 20480):
 // ubar = (d dt / d u)^T gbar; scratch_du: 150 x E x Np doubles; scratch_faces:
 // 300 x E x gh_adjoint_face_slots(ext) doubles.
 int gh_adjoint_face_slots(const int32_t ext[3]);

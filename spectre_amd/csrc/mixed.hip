@@ -114,7 +114,8 @@ sw_mixed_volume_kernel(const MixedElement* __restrict__ el, const int32_t* __res
 
 // ---------------------------------------------------------------------------
 // Partial derivatives of nc components (GH volume terms).  Grid: (list
-// length, component chunks of kRaggedChunk); dynamic LDS: Np_max doubles.
+// length, component chunks of kRaggedChunk); dynamic LDS: the element's D
+// matrices (kStagedDiffDoubles), then Np_max doubles.
 constexpr int kRaggedChunk = 5;
 
 __global__ void __launch_bounds__(256)
@@ -123,12 +124,15 @@ ragged_partial_derivatives_kernel(const MixedElement* __restrict__ el,
                                   const double* __restrict__ u,
                                   const double* __restrict__ inv_jac, double* __restrict__ du,
                                   const double* __restrict__ dtab) {
-  extern __shared__ double s_c[];
+  extern __shared__ double s_lds[];
+  double* s_c = s_lds + kStagedDiffDoubles;
   const MixedElement m = el[list[blockIdx.x]];
   const int np = m.np;
-  const double* D0 = diff_matrix(dtab, m.n[0]);
-  const double* D1 = diff_matrix(dtab, m.n[1]);
-  const double* D2 = diff_matrix(dtab, m.n[2]);
+  const double* Dm[3];
+  stage_diff_matrices(dtab, m.n, s_lds, Dm);  // synchronised by the loop's first barrier
+  const double* D0 = Dm[0];
+  const double* D1 = Dm[1];
+  const double* D2 = Dm[2];
   const double* ij = inv_jac + 9 * m.off;
   const int c_end = min(nc, static_cast<int>(blockIdx.y + 1) * kRaggedChunk);
   for (int c = blockIdx.y * kRaggedChunk; c < c_end; ++c) {
@@ -445,18 +449,20 @@ gh_mixed_adjoint_pointwise_kernel(int64_t total, const MixedElement* __restrict_
 }
 
 // ubar_c += sum_j D_j^T xbar_cj per element.  Grid: (list length, component
-// chunks of kRaggedChunk); dynamic LDS: 3 Np_max doubles.
+// chunks of kRaggedChunk); dynamic LDS: the element's D matrices
+// (kStagedDiffDoubles), then 3 Np_max doubles.
 __global__ void __launch_bounds__(256)
 ragged_transposed_sweeps_kernel(const MixedElement* __restrict__ el,
                                 const int32_t* __restrict__ list, int nc,
                                 const double* __restrict__ xbar, double* __restrict__ ubar,
                                 const double* __restrict__ dtab) {
-  extern __shared__ double s_x[];
+  extern __shared__ double s_lds[];
+  double* s_x = s_lds + kStagedDiffDoubles;
   const MixedElement m = el[list[blockIdx.x]];
   const int np = m.np;
   const int stride[3] = {1, m.n[0], m.n[0] * m.n[1]};
-  const double* D[3] = {diff_matrix(dtab, m.n[0]), diff_matrix(dtab, m.n[1]),
-                        diff_matrix(dtab, m.n[2])};
+  const double* D[3];
+  stage_diff_matrices(dtab, m.n, s_lds, D);  // synchronised by the loop's first barrier
   const int c_end = min(nc, static_cast<int>(blockIdx.y + 1) * kRaggedChunk);
   for (int c = blockIdx.y * kRaggedChunk; c < c_end; ++c) {
     __syncthreads();
@@ -972,7 +978,8 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
   if (scratch_du == nullptr) return SXD_ERR_INVALID_ARGUMENT;
   hipLaunchKernelGGL(ragged_partial_derivatives_kernel,
                      dim3(count, (50 + kRaggedChunk - 1) / kRaggedChunk), dim3(threads),
-                     np_max * 8, stream, el, list, 50, u, inv_jac, scratch_du, dtab);
+                     (kStagedDiffDoubles + np_max) * 8, stream, el, list, 50, u, inv_jac,
+                     scratch_du, dtab);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   hipLaunchKernelGGL(gh_mixed_pointwise_kernel, dim3(count), dim3(threads), 0, stream, el, list,
                      u, scratch_du, params, dt);
@@ -1009,20 +1016,23 @@ int launch_mixed_gh_adjoint(int count, const int32_t* list, const MixedElement* 
   const double* dtab = device_diff_matrices();
   const double* mtab = device_mortar_matrices();
   if (dtab == nullptr || mtab == nullptr) return SXD_ERR_DEVICE;
-  if (mixed_face_adjoint_lds_bytes(kf, km) > 160 * 1024 || 3 * np_max * 8 > 160 * 1024)
+  if (mixed_face_adjoint_lds_bytes(kf, km) > 160 * 1024 ||
+      (kStagedDiffDoubles + 3 * np_max) * 8 > 160 * 1024)
     return SXD_ERR_UNSUPPORTED;
   if (scratch_du == nullptr) return SXD_ERR_INVALID_ARGUMENT;
   if (!enable_large_lds()) return SXD_ERR_DEVICE;
   const dim3 chunks(count, (50 + kRaggedChunk - 1) / kRaggedChunk);
-  hipLaunchKernelGGL(ragged_partial_derivatives_kernel, chunks, dim3(256), np_max * 8, stream, el,
-                     list, 50, u, inv_jac, scratch_du, dtab);
+  hipLaunchKernelGGL(ragged_partial_derivatives_kernel, chunks, dim3(256),
+                     (kStagedDiffDoubles + np_max) * 8, stream, el, list, 50, u, inv_jac,
+                     scratch_du, dtab);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   hipLaunchKernelGGL(gh_mixed_adjoint_pointwise_kernel,
                      dim3(static_cast<unsigned>((total_points + 127) / 128)), dim3(128), 0, stream,
                      total_points, el, point_el, u, scratch_du, gammas, inv_jac, gbar, ubar);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
-  hipLaunchKernelGGL(ragged_transposed_sweeps_kernel, chunks, dim3(256), 3 * np_max * 8, stream,
-                     el, list, 50, scratch_du, ubar, dtab);
+  hipLaunchKernelGGL(ragged_transposed_sweeps_kernel, chunks, dim3(256),
+                     (kStagedDiffDoubles + 3 * np_max) * 8, stream, el, list, 50, scratch_du, ubar,
+                     dtab);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   // faces: the derivative work array is free again and holds the side
   // adjoints (110 doubles per point, under the 150 it has)
