@@ -7,12 +7,14 @@
 // gauge) and of the GH UpwindPenalty (GeneralizedHarmonic/BoundaryCorrections/
 // UpwindPenalty.cpp:35-275) with the normalisation of
 // NormalCovectorAndMagnitude.hpp:47-81 and dg::lift_flux.  Each adjoint first
-// recomputes the forward intermediates at the linearisation point, then runs
-// the reverse sweep: for every forward statement y = f(x1, x2, ...) it adds
-// (df/dxk) ybar to xkbar.  Sums are accumulated in the natural loop order
-// (the adjoints are compared with the oracle's tape to a tolerance, not bit
-// for bit).  The Heaviside weights of the upwind selection are piecewise
-// constant; their derivative is zero away from the switching points.
+// recomputes the forward intermediates it needs at the linearisation point,
+// then runs the reverse: statement by statement (for y = f(x1, x2, ...) it
+// adds (df/dxk) ybar to xkbar) for most terms, and in closed form, as
+// products of 4x4 matrices, for the quadratic terms of dt Pi.  Sums are
+// accumulated in an order of their own (the adjoints are compared with the
+// oracle's tape to a tolerance, not bit for bit).  The Heaviside weights of
+// the upwind selection are piecewise constant; their derivative is zero away
+// from the switching points.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -176,17 +178,10 @@ SXD_HD void add_slots(double* s_b, const double* G, double scale) {
 //    2 sum_{de, n} Psi_n(mu, de) (P_n ig)(nu, de)  =  2 sum_n (Psi_n ig P_n)(mu, nu)
 //   -2 sum_{al, de} (C_mu ig)(al, de) (C_nu ig)(de, al) = -2 tr(C_mu ig C_nu ig)
 // each symmetric in (mu, nu).
-template <int kPart = 0>
 SXD_HD void gh_time_derivative_vjp(const double* u, int su, const double* du, int sdu,
                                    double gamma0, double gamma1, double gamma2,
                                    const double* gbar, int sg, double* ubar, GhVjpPoint* pt) {
   using namespace adj;
-  // kPart selects a share of the terms (0 = all; 1 = Christoffel terms;
-  // 2 = the remaining terms of dt Pi; 3 = dt Phi and the C3 terms): the
-  // adjoint is linear in gbar, so the shares add up to the whole.
-  constexpr bool kChr = kPart == 0 || kPart == 1;
-  constexpr bool kR = kPart == 0 || kPart == 2;
-  constexpr bool kS = kPart == 0 || kPart == 3;
 #define G_(a, b) u[sym4(a, b) * su]
 #define PI_(a, b) u[(10 + sym4(a, b)) * su]
 #define PHI_(i, a, b) u[(20 + iaa(i, a, b)) * su]
@@ -246,7 +241,7 @@ SXD_HD void gh_time_derivative_vjp(const double* u, int su, const double* du, in
   SXD_REREAD(du);
   SXD_REREAD(gbar);
   // ---- Christoffel part: tc, the harmonic-gauge term, -2 tr(C ig C ig) -------
-  if (kChr) {
+  {
     double dtg[10];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -377,7 +372,7 @@ SXD_HD void gh_time_derivative_vjp(const double* u, int su, const double* du, in
   }
 
   // ---- Pi / Phi quadratic terms, and dt Phi -----------------------------------
-  if (kR || kS) {
+  {
     SXD_REREAD(u);
     SXD_REREAD(du);
     SXD_REREAD(gbar);
@@ -429,7 +424,7 @@ SXD_HD void gh_time_derivative_vjp(const double* u, int su, const double* du, in
         phi_b[iaa(m, a, b)] += w * isp[sym3(n, m)];
       }
     };
-    if (kR) {
+    {
       SXD_REREAD(u);
       // -2 (Pi ig Pi): d/dPi = -4 H Pi ig, d/dig = -2 Pi H Pi
       double Pf[16], Q[16], T[16], U[16];
@@ -498,7 +493,7 @@ SXD_HD void gh_time_derivative_vjp(const double* u, int su, const double* du, in
 #pragma unroll
         for (int n = 0; n < 3; ++n) ps[n] = psi_at(n, a, b);
         const double pi_ab = PI_(a, b);
-        if (kR) {
+        {
           const double w = GB_(10 + sym4(a, b));
           double R = -hptn * pi_ab;
 #pragma unroll
@@ -521,7 +516,7 @@ SXD_HD void gh_time_derivative_vjp(const double* u, int su, const double* du, in
             for (int n = 0; n < 3; ++n) isp_b[sym3(m, n)] -= Rb * DPHI_(m, n, a, b);
           }
         }
-        if (kS) {
+        {
 #pragma unroll
           for (int i = 0; i < 3; ++i) {
             const double w = GB_(20 + iaa(i, a, b));
