@@ -196,37 +196,94 @@ gh_mixed_pointwise_kernel(const MixedElement* __restrict__ el, const int32_t* __
 // ---------------------------------------------------------------------------
 // Face coupling.
 //
-// Apply per-dimension matrices to `nc` components on a 2-d mesh, as the
-// oracle's apply_face_matrices: in [nc][ia ib] -> out [nc][oa ob] through
-// tmp [nc][oa ib]; Ma [oa][ia] (skipped when ia == oa), Mb [ob][ib]
-// (skipped when ib == ob).  Includes the barriers between the steps.
-__device__ void apply_face_matrices(int nc, int ia, int ib, int oa, int ob, const double* Ma,
+// Apply per-dimension matrices to NC components on a 2-d mesh, as the
+// oracle's apply_face_matrices: in [NC][ia ib] -> out [NC][oa ob] through
+// tmp [NC][oa ib]; Ma [oa][ia] (skipped when ia == oa), Mb [ob][ib]
+// (skipped when ib == ob).  Each work item is one output point for a group
+// of NC / G components (G groups): the matrix row is read once for the group
+// and the index arithmetic done once.  Every output is the same sum over
+// l = 0 .. n-1 (first term a product, then fused multiply-adds).  Includes
+// the barriers between the steps.
+// Dimension a of apply_face_matrices alone (no barrier): out [NC][oa ib] =
+// Ma applied to in [NC][ia ib].
+template <int NC, int G>
+__device__ void face_matrix_a(int ia, int ib, int oa, const double* Ma, const double* in,
+                              double* out) {
+  constexpr int NG = (NC + G - 1) / G;
+  for (int x = threadIdx.x; x < G * oa * ib; x += blockDim.x) {
+    const int g = x / (oa * ib), y = x % (oa * ib), b = y / oa, r = y % oa;
+    const double* v = in + g * NG * ia * ib + ia * b;
+    const double* row = Ma + r * ia;
+    double acc[NG];
+#pragma unroll
+    for (int c = 0; c < NG; ++c) acc[c] = g * NG + c < NC ? row[0] * v[c * ia * ib] : 0.0;
+    for (int l = 1; l < ia; ++l) {
+      const double m = row[l];
+#pragma unroll
+      for (int c = 0; c < NG; ++c)
+        if (g * NG + c < NC) acc[c] = fma(m, v[c * ia * ib + l], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NG; ++c)
+      if (g * NG + c < NC) out[(g * NG + c) * oa * ib + y] = acc[c];
+  }
+}
+
+// Dimension b of apply_face_matrices at one output point (ra, rb): vals[c] =
+// Mb applied to src [NC][oa ib] (src itself where ib == ob).
+template <int NC>
+__device__ __forceinline__ void face_matrix_b_point(int ib, int oa, int ob, const double* Mb,
+                                                    const double* src, int ra, int rb,
+                                                    double* vals) {
+  const double* v = src + ra;
+  if (ib == ob) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) vals[c] = v[c * oa * ib + oa * rb];
+    return;
+  }
+  const double* row = Mb + rb * ib;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) vals[c] = row[0] * v[c * oa * ib];
+  for (int l = 1; l < ib; ++l) {
+    const double m = row[l];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) vals[c] = fma(m, v[c * oa * ib + oa * l], vals[c]);
+  }
+}
+
+template <int NC, int G>
+__device__ void apply_face_matrices(int ia, int ib, int oa, int ob, const double* Ma,
                                     const double* Mb, const double* in, double* tmp,
                                     double* out) {
+  constexpr int NG = (NC + G - 1) / G;
   const int t = threadIdx.x, T = blockDim.x;
   const double* src = in;
   if (ia != oa) {
-    for (int x = t; x < nc * oa * ib; x += T) {
-      const int c = x / (oa * ib), rem = x % (oa * ib), b = rem / oa, r = rem % oa;
-      const double* v = in + c * ia * ib + ia * b;
-      double s = Ma[r * ia] * v[0];
-      for (int l = 1; l < ia; ++l) s = fma(Ma[r * ia + l], v[l], s);
-      tmp[x] = s;
-    }
+    face_matrix_a<NC, G>(ia, ib, oa, Ma, in, tmp);
     __syncthreads();
     src = tmp;
   }
-  for (int x = t; x < nc * oa * ob; x += T) {
-    const int c = x / (oa * ob), rem = x % (oa * ob), r = rem / oa, a = rem % oa;
-    const double* v = src + c * oa * ib + a;
-    double s;
+  for (int x = t; x < G * oa * ob; x += T) {
+    const int g = x / (oa * ob), y = x % (oa * ob), r = y / oa, a = y % oa;
+    const double* v = src + g * NG * oa * ib + a;
+    double acc[NG];
     if (ib == ob) {
-      s = v[oa * r];
+#pragma unroll
+      for (int c = 0; c < NG; ++c) acc[c] = g * NG + c < NC ? v[c * oa * ib + oa * r] : 0.0;
     } else {
-      s = Mb[r * ib] * v[0];
-      for (int l = 1; l < ib; ++l) s = fma(Mb[r * ib + l], v[oa * l], s);
+      const double* row = Mb + r * ib;
+#pragma unroll
+      for (int c = 0; c < NG; ++c) acc[c] = g * NG + c < NC ? row[0] * v[c * oa * ib] : 0.0;
+      for (int l = 1; l < ib; ++l) {
+        const double m = row[l];
+#pragma unroll
+        for (int c = 0; c < NG; ++c)
+          if (g * NG + c < NC) acc[c] = fma(m, v[c * oa * ib + oa * l], acc[c]);
+      }
     }
-    out[x] = s;
+#pragma unroll
+    for (int c = 0; c < NG; ++c)
+      if (g * NG + c < NC) out[(g * NG + c) * oa * ob + y] = acc[c];
   }
   __syncthreads();
 }
@@ -371,9 +428,9 @@ sw_mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restr
       face_package<false>(mn, d, 1 - side, f, mf, 0, u, params, sm + L.side_nb + 4 * f,
                           sm + L.pk_nb);
     __syncthreads();
-    apply_face_matrices(13, na, nb, ka, kb, Ia_own, Ib_own, sm + L.pk_own, sm + L.tmp,
+    apply_face_matrices<13, 2>(na, nb, ka, kb, Ia_own, Ib_own, sm + L.pk_own, sm + L.tmp,
                         sm + L.m_own);
-    apply_face_matrices(13, ma, mb, ka, kb, Ia_nb, Ib_nb, sm + L.pk_nb, sm + L.tmp,
+    apply_face_matrices<13, 2>(ma, mb, ka, kb, Ia_nb, Ib_nb, sm + L.pk_nb, sm + L.tmp,
                         sm + L.m_nb);
     for (int k = t; k < kmf; k += T) {
       double in[13], ex[13], bc[5];
@@ -387,7 +444,7 @@ sw_mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restr
       for (int c = 0; c < 5; ++c) sm[L.bc_m + c * kmf + k] = bc[c];
     }
     __syncthreads();
-    apply_face_matrices(5, ka, kb, na, nb, Pa, Pb, sm + L.bc_m, sm + L.tmp, sm + L.bc);
+    apply_face_matrices<5, 1>(ka, kb, na, nb, Pa, Pb, sm + L.bc_m, sm + L.tmp, sm + L.bc);
     // lift into dt
     for (int f = t; f < nf; f += T) {
       const int p = face_point(m, d, side, f % na, f / na);
@@ -482,37 +539,57 @@ ragged_transposed_sweeps_kernel(const MixedElement* __restrict__ el,
   }
 }
 
-// Transpose of apply_face_matrices: in [nc][oa ob] -> out [nc][ia ib] with
+// Transpose of apply_face_matrices: in [NC][oa ob] -> out [NC][ia ib] with
 // Ma [oa][ia] (skipped when ia == oa) and Mb [ob][ib] (skipped when
-// ib == ob), dimension b first, through tmp [nc][oa ib].  Includes the
-// barriers between the steps.
-__device__ void apply_face_matrices_t(int nc, int oa, int ob, int ia, int ib, const double* Ma,
+// ib == ob), dimension b first, through tmp [NC][oa ib]; work items as in
+// apply_face_matrices.  Includes the barriers between the steps.
+template <int NC, int G>
+__device__ void apply_face_matrices_t(int oa, int ob, int ia, int ib, const double* Ma,
                                       const double* Mb, const double* in, double* tmp,
                                       double* out) {
+  constexpr int NG = (NC + G - 1) / G;
   const int t = threadIdx.x, T = blockDim.x;
   const double* src = in;
   if (ib != ob) {
-    for (int x = t; x < nc * oa * ib; x += T) {
-      const int c = x / (oa * ib), rem = x % (oa * ib), lb = rem / oa, ra = rem % oa;
-      const double* v = in + c * oa * ob + ra;
-      double s = Mb[lb] * v[0];
-      for (int r = 1; r < ob; ++r) s = fma(Mb[r * ib + lb], v[oa * r], s);
-      tmp[x] = s;
+    for (int x = t; x < G * oa * ib; x += T) {
+      const int g = x / (oa * ib), y = x % (oa * ib), lb = y / oa, ra = y % oa;
+      const double* v = in + g * NG * oa * ob + ra;
+      double acc[NG];
+#pragma unroll
+      for (int c = 0; c < NG; ++c) acc[c] = g * NG + c < NC ? Mb[lb] * v[c * oa * ob] : 0.0;
+      for (int r = 1; r < ob; ++r) {
+        const double m = Mb[r * ib + lb];
+#pragma unroll
+        for (int c = 0; c < NG; ++c)
+          if (g * NG + c < NC) acc[c] = fma(m, v[c * oa * ob + oa * r], acc[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < NG; ++c)
+        if (g * NG + c < NC) tmp[(g * NG + c) * oa * ib + y] = acc[c];
     }
     __syncthreads();
     src = tmp;
   }
-  for (int x = t; x < nc * ia * ib; x += T) {
-    const int c = x / (ia * ib), rem = x % (ia * ib), lb = rem / ia, la = rem % ia;
-    const double* v = src + c * oa * ib + oa * lb;
-    double s;
+  for (int x = t; x < G * ia * ib; x += T) {
+    const int g = x / (ia * ib), y = x % (ia * ib), lb = y / ia, la = y % ia;
+    const double* v = src + g * NG * oa * ib + oa * lb;
+    double acc[NG];
     if (ia == oa) {
-      s = v[la];
+#pragma unroll
+      for (int c = 0; c < NG; ++c) acc[c] = g * NG + c < NC ? v[c * oa * ib + la] : 0.0;
     } else {
-      s = Ma[la] * v[0];
-      for (int r = 1; r < oa; ++r) s = fma(Ma[r * ia + la], v[r], s);
+#pragma unroll
+      for (int c = 0; c < NG; ++c) acc[c] = g * NG + c < NC ? Ma[la] * v[c * oa * ib] : 0.0;
+      for (int r = 1; r < oa; ++r) {
+        const double m = Ma[r * ia + la];
+#pragma unroll
+        for (int c = 0; c < NG; ++c)
+          if (g * NG + c < NC) acc[c] = fma(m, v[c * oa * ib + r], acc[c]);
+      }
     }
-    out[x] = s;
+#pragma unroll
+    for (int c = 0; c < NG; ++c)
+      if (g * NG + c < NC) out[(g * NG + c) * ia * ib + y] = acc[c];
   }
   __syncthreads();
 }
@@ -596,7 +673,7 @@ gh_mixed_face_pass_kernel(int d, const MixedElement* __restrict__ el,
     face_package<true>(m, d, side, f, nf, pr, u, params, sd, face + 4 * nf);
   }
   __syncthreads();
-  apply_face_matrices(17, na, nb, ka, kb, Ia_own, Ib_own, face, sm + L.tmp, sm + L.m_own);
+  apply_face_matrices<17, 2>(na, nb, ka, kb, Ia_own, Ib_own, face, sm + L.tmp, sm + L.m_own);
   // neighbour side
   for (int f = t; f < mf; f += T) {
     double sd[12];
@@ -607,7 +684,7 @@ gh_mixed_face_pass_kernel(int d, const MixedElement* __restrict__ el,
   }
   __syncthreads();
   if constexpr (!kFusedNb) {
-    apply_face_matrices(17, ma, mb, ka, kb, Ia_nb, Ib_nb, face, sm + L.tmp, sm + L.m_nb);
+    apply_face_matrices<17, 2>(ma, mb, ka, kb, Ia_nb, Ib_nb, face, sm + L.tmp, sm + L.m_nb);
     // boundary terms at the mortar points, written over the own side's slots
     for (int k = t; k < kmf; k += T) {
       double in[13], ex[13], li[4], le[4], bc[5];
@@ -632,32 +709,14 @@ gh_mixed_face_pass_kernel(int d, const MixedElement* __restrict__ el,
     // (apply_face_matrices' sums, without a mortar buffer for this side)
     const double* src_nb = face;
     if (ma != ka) {
-      for (int x = t; x < 17 * ka * mb; x += T) {
-        const int c = x / (ka * mb), rem = x % (ka * mb), b = rem / ka, r = rem % ka;
-        const double* v = face + c * mf + ma * b;
-        double sv = Ia_nb[r * ma] * v[0];
-        for (int l = 1; l < ma; ++l) sv = fma(Ia_nb[r * ma + l], v[l], sv);
-        sm[L.tmp + x] = sv;
-      }
+      face_matrix_a<17, 2>(ma, mb, ka, Ia_nb, face, sm + L.tmp);
       __syncthreads();
       src_nb = sm + L.tmp;
     }
     // boundary terms at the mortar points, written over the own side's slots
     for (int k = t; k < kmf; k += T) {
-      const int ra = k % ka, rb = k / ka;
       double nbv[17];
-#pragma unroll
-      for (int c = 0; c < 17; ++c) {
-        const double* v = src_nb + c * ka * mb + ra;
-        double sv;
-        if (mb == kb) {
-          sv = v[ka * rb];
-        } else {
-          sv = Ib_nb[rb * mb] * v[0];
-          for (int l = 1; l < mb; ++l) sv = fma(Ib_nb[rb * mb + l], v[ka * l], sv);
-        }
-        nbv[c] = sv;
-      }
+      face_matrix_b_point<17>(mb, ka, kb, Ib_nb, src_nb, k % ka, k / ka, nbv);
       double in[13], li[4], bc[5];
 #pragma unroll
       for (int c = 0; c < 4; ++c) li[c] = sm[L.m_own + c * kmf + k];
@@ -669,7 +728,7 @@ gh_mixed_face_pass_kernel(int d, const MixedElement* __restrict__ el,
     }
     __syncthreads();
   }
-  apply_face_matrices(5, ka, kb, na, nb, Pa, Pb, sm + L.m_own, sm + L.tmp, face);
+  apply_face_matrices<5, 1>(ka, kb, na, nb, Pa, Pb, sm + L.m_own, sm + L.tmp, face);
   // lift into dt: components g, Pi, Phi_0..2 of the pair
   for (int f = t; f < nf; f += T) {
     const int p = face_point(m, d, side, f % na, f / na);
@@ -773,7 +832,7 @@ gh_mixed_face_adjoint_pass_kernel(int d, const MixedElement* __restrict__ el,
     for (int c = 0; c < 13; ++c) face[(4 + c) * nf + f] = pk[c];
   }
   __syncthreads();
-  apply_face_matrices(17, na, nb, ka, kb, Ia_own, Ib_own, face, sm + L.tmp, sm + L.m_own);
+  apply_face_matrices<17, 2>(na, nb, ka, kb, Ia_own, Ib_own, face, sm + L.tmp, sm + L.m_own);
   for (int f = t; f < mf; f += T) {
     GhSideFwd s;
     int q;
@@ -793,15 +852,9 @@ gh_mixed_face_adjoint_pass_kernel(int d, const MixedElement* __restrict__ el,
   // corrections there, over e's package slots
   const double* src_nb = face;
   if constexpr (!kFusedNb) {
-    apply_face_matrices(17, ma, mb, ka, kb, Ia_nb, Ib_nb, face, sm + L.tmp, sm + L.m_nb);
+    apply_face_matrices<17, 2>(ma, mb, ka, kb, Ia_nb, Ib_nb, face, sm + L.tmp, sm + L.m_nb);
   } else if (ma != ka) {
-    for (int x = t; x < 17 * ka * mb; x += T) {
-      const int c = x / (ka * mb), rem = x % (ka * mb), b = rem / ka, r = rem % ka;
-      const double* v = face + c * mf + ma * b;
-      double sv = Ia_nb[r * ma] * v[0];
-      for (int l = 1; l < ma; ++l) sv = fma(Ia_nb[r * ma + l], v[l], sv);
-      sm[L.tmp + x] = sv;
-    }
+    face_matrix_a<17, 2>(ma, mb, ka, Ia_nb, face, sm + L.tmp);
     __syncthreads();
     src_nb = sm + L.tmp;
   }
@@ -815,26 +868,14 @@ gh_mixed_face_adjoint_pass_kernel(int d, const MixedElement* __restrict__ el,
 #pragma unroll
       for (int c = 0; c < 17; ++c) nbv[c] = sm[L.m_nb + c * kmf + k];
     } else {
-      const int ra = k % ka, rb = k / ka;
-#pragma unroll
-      for (int c = 0; c < 17; ++c) {
-        const double* v = src_nb + c * ka * mb + ra;
-        double sv;
-        if (mb == kb) {
-          sv = v[ka * rb];
-        } else {
-          sv = Ib_nb[rb * mb] * v[0];
-          for (int l = 1; l < mb; ++l) sv = fma(Ib_nb[rb * mb + l], v[ka * l], sv);
-        }
-        nbv[c] = sv;
-      }
+      face_matrix_b_point<17>(mb, ka, kb, Ib_nb, src_nb, k % ka, k / ka, nbv);
     }
     gh_boundary_fwd(in, li, nbv + 4, nbv, bc);
 #pragma unroll
     for (int c = 0; c < 5; ++c) sm[L.m_own + (4 + c) * kmf + k] = bc[c];
   }
   __syncthreads();
-  apply_face_matrices(5, ka, kb, na, nb, Pa, Pb, sm + L.m_own + 4 * kmf, sm + L.tmp, face);
+  apply_face_matrices<5, 1>(ka, kb, na, nb, Pa, Pb, sm + L.m_own + 4 * kmf, sm + L.tmp, face);
   // seeds on both faces; the pair's part of the |n| adjoint of e's lift factor
   double* seed_own = face + 5 * kf;
   double* seed_nb = face + 10 * kf;
@@ -858,8 +899,9 @@ gh_mixed_face_adjoint_pass_kernel(int d, const MixedElement* __restrict__ el,
       seed_nb[c * mf + f] = gbar[50 * mn.off + comp[c] * mn.np + q] * fac;
   }
   __syncthreads();
-  apply_face_matrices_t(5, na, nb, ka, kb, Pa, Pb, seed_own, sm + L.tmp, sm + L.m_own + 4 * kmf);
-  apply_face_matrices_t(5, ma, mb, ka, kb, Pa_nb, Pb_nb, seed_nb, sm + L.tmp,
+  apply_face_matrices_t<5, 1>(na, nb, ka, kb, Pa, Pb, seed_own, sm + L.tmp,
+                              sm + L.m_own + 4 * kmf);
+  apply_face_matrices_t<5, 1>(ma, mb, ka, kb, Pa_nb, Pb_nb, seed_nb, sm + L.tmp,
                         sm + L.m_own + 9 * kmf);
   for (int k = t; k < kmf; k += T) {
     double li[4], bi[5], bx[5], B[13];
@@ -875,7 +917,7 @@ gh_mixed_face_adjoint_pass_kernel(int d, const MixedElement* __restrict__ el,
     for (int c = 0; c < 13; ++c) sm[L.m_own + c * kmf + k] = B[c];
   }
   __syncthreads();
-  apply_face_matrices_t(13, ka, kb, na, nb, Ia_own, Ib_own, sm + L.m_own, sm + L.tmp, face);
+  apply_face_matrices_t<13, 2>(ka, kb, na, nb, Ia_own, Ib_own, sm + L.m_own, sm + L.tmp, face);
   for (int f = t; f < nf; f += T) {
     const double* o = sm + L.side + 12 * f;
     const int p = face_point(m, d, side, f % na, f / na);
