@@ -136,8 +136,10 @@ int sxd_lift_flux(int64_t num_points, int nc, double* bc,
  * (Evolution/DiscontinuousGalerkin/Actions/ComputeTimeDerivative.hpp:383-650,
  * ApplyBoundaryCorrections.hpp:722-1050) for a single device.
  * neighbors: host array [6 E], entry 6 e + 2 d + side (side 0 = lower,
- * 1 = upper) is the element across that face.  The table is copied to the
- * device; this call allocates device memory. */
+ * 1 = upper) is the element across that face, or -1 for an external face
+ * (closed by a ghost state, see sxd_scalar_wave_rhs_dirichlet; every other
+ * call on such a domain returns SXD_ERR_UNSUPPORTED).  The table is copied to
+ * the device; this call allocates device memory. */
 typedef struct sxd_domain sxd_domain;
 int sxd_domain_create(sxd_domain** domain, int num_elements,
                       const int32_t extents[3], const int32_t* neighbors);
@@ -168,6 +170,19 @@ int sxd_domain_num_points(const sxd_domain* domain, int64_t* points);
 int sxd_scalar_wave_rhs(const sxd_domain* domain, const double* u,
                         const double* gamma2, const double* inv_jac,
                         double* dt, void* stream);
+
+/* sxd_scalar_wave_rhs on a domain with external faces (neighbour -1), closed
+ * by the Dirichlet analytic ghost boundary condition: on an external face the
+ * exterior UpwindPenalty state is ghost's fields at the face's own points,
+ * with the interior gamma2 and the reversed interior unit normal, and the
+ * correction is lifted with the interior normal magnitude
+ * (ScalarWave/BoundaryConditions/DirichletAnalytic.cpp:72-100 dg_ghost, applied
+ * by Evolution/DiscontinuousGalerkin/Actions/BoundaryConditionsImpl.hpp:427-586).
+ * ghost [E][5][Np] (psi, pi, phi_i of the analytic solution at every point;
+ * only face points are read).  Internal faces are as in sxd_scalar_wave_rhs. */
+int sxd_scalar_wave_rhs_dirichlet(const sxd_domain* domain, const double* u,
+                                  const double* gamma2, const double* inv_jac,
+                                  const double* ghost, double* dt, void* stream);
 
 /* Full generalized-harmonic DG time derivative (harmonic gauge).
  * u, dt [E][50][Np]; gammas [E][3][Np]; inv_jac [E][9][Np]. */

@@ -736,9 +736,16 @@ void sw_rhs_volume(int e_begin, int e_end, const int ext[3], const T* u,
   }
 }
 
+// External faces (neighbour index < 0) take the exterior state from `ghost`
+// ([E][5][Np], read at the face's own points), as a ghost boundary condition
+// does (ScalarWave DirichletAnalytic::dg_ghost,
+// ScalarWave/BoundaryConditions/DirichletAnalytic.cpp:72-100, applied by
+// Evolution/DiscontinuousGalerkin/Actions/BoundaryConditionsImpl.hpp:427-586):
+// ghost fields, the interior gamma2, and the negated interior unit normal.
 template <class T>
 void sw_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
-                  const T* u, const T* gamma2, const T* inv_j, T* dt) {
+                  const T* u, const T* gamma2, const T* inv_j, T* dt,
+                  const T* ghost = nullptr) {
   const size_t np = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
   // boundary corrections
   for (int e = e_begin; e < e_end; ++e) {
@@ -759,16 +766,24 @@ void sw_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
           }
           flat_normalize(n, &mag);
           sw_package_point(pk_in, up, gamma2[e * np + p], n);
-          // neighbour side
-          const size_t q = fe.pts[f];
-          T un[5], nn[3], magn;
-          for (int c = 0; c < 5; ++c) un[c] = u[(en * 5 + c) * np + q];
-          for (int i = 0; i < 3; ++i) {
-            nn[i] = inv_j[(en * 9 + ij3(d, i)) * np + q];
-            if (side == 1) nn[i] = nn[i] * -1.0;  // neighbour's face is lower
+          if (en < 0) {
+            // ghost exterior state at the boundary point
+            T ug[5], ng[3];
+            for (int c = 0; c < 5; ++c) ug[c] = ghost[(e * 5 + c) * np + p];
+            for (int i = 0; i < 3; ++i) ng[i] = n[i] * -1.0;
+            sw_package_point(pk_ex, ug, gamma2[e * np + p], ng);
+          } else {
+            // neighbour side
+            const size_t q = fe.pts[f];
+            T un[5], nn[3], magn;
+            for (int c = 0; c < 5; ++c) un[c] = u[(en * 5 + c) * np + q];
+            for (int i = 0; i < 3; ++i) {
+              nn[i] = inv_j[(en * 9 + ij3(d, i)) * np + q];
+              if (side == 1) nn[i] = nn[i] * -1.0;  // neighbour's face is lower
+            }
+            flat_normalize(nn, &magn);
+            sw_package_point(pk_ex, un, gamma2[en * np + q], nn);
           }
-          flat_normalize(nn, &magn);
-          sw_package_point(pk_ex, un, gamma2[en * np + q], nn);
           sw_boundary_terms_point(bc, pk_in, pk_ex);
           const T fac = lift_factor(ext[d], mag);
           for (int c = 0; c < 5; ++c) {
@@ -783,9 +798,9 @@ void sw_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
 
 template <class T>
 void sw_rhs(int num_elements, const int ext[3], const int* nbr, const T* u,
-            const T* gamma2, const T* inv_j, T* dt) {
+            const T* gamma2, const T* inv_j, T* dt, const T* ghost = nullptr) {
   sw_rhs_volume(0, num_elements, ext, u, gamma2, inv_j, dt);
-  sw_rhs_faces(0, num_elements, ext, nbr, u, gamma2, inv_j, dt);
+  sw_rhs_faces(0, num_elements, ext, nbr, u, gamma2, inv_j, dt, ghost);
 }
 
 // Full GH DG RHS on a uniform-extent periodic brick.

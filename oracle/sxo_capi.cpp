@@ -336,6 +336,16 @@ int sxo_sw_rhs(int num_elements, const int* ext, const int* nbr, const double* u
   return 0;
 }
 
+// ScalarWave RHS with external faces (neighbour < 0) closed by the ghost
+// state `ghost` [E][5][Np] (Dirichlet analytic boundary condition).
+int sxo_sw_rhs_dirichlet(int num_elements, const int* ext, const int* nbr, const double* u,
+                         const double* gamma2, const double* inv_j, const double* ghost,
+                         double* dt) {
+  if (bad_extents(ext) || num_elements < 1 || ghost == nullptr) return -1;
+  sxo::sw_rhs(num_elements, ext, nbr, u, gamma2, inv_j, dt, ghost);
+  return 0;
+}
+
 int sxo_gh_rhs(int num_elements, const int* ext, const int* nbr, const double* u,
                const double* gammas, const double* inv_j, double* dt) {
   if (bad_extents(ext) || num_elements < 1) return -1;

@@ -22,6 +22,8 @@ struct sxd_domain {
   int num_elements = 0;
   int32_t ext[3] = {0, 0, 0};
   int32_t* d_nbr = nullptr;
+  // Some neighbour is -1 (an external face closed by a ghost state).
+  bool has_boundary = false;
   // Work arrays of the composite (non-fused) paths, allocated on first use.
   double* d_scratch_du = nullptr;
   double* d_scratch_temps = nullptr;
@@ -289,10 +291,13 @@ int sxd_domain_create(sxd_domain** domain, int num_elements, const int32_t exten
   if (!valid_extents(extents))
     return fail(SXD_ERR_INVALID_ARGUMENT, "extents must be 2..20 in each dimension");
   if (neighbors == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null neighbour table");
-  for (int64_t i = 0; i < 6LL * num_elements; ++i)
-    if (neighbors[i] < 0 || neighbors[i] >= num_elements)
+  bool has_boundary = false;
+  for (int64_t i = 0; i < 6LL * num_elements; ++i) {
+    if (neighbors[i] < -1 || neighbors[i] >= num_elements)
       return fail(SXD_ERR_INVALID_ARGUMENT, "neighbour %d of element %d out of range",
                   neighbors[i], static_cast<int>(i / 6));
+    has_boundary = has_boundary || neighbors[i] == -1;
+  }
   sxd_domain* d = new (std::nothrow) sxd_domain;
   if (d == nullptr) return fail(SXD_ERR_OUT_OF_MEMORY, "host allocation failed");
   if (hipGetDevice(&d->device) != hipSuccess) {
@@ -300,6 +305,7 @@ int sxd_domain_create(sxd_domain** domain, int num_elements, const int32_t exten
     return fail(SXD_ERR_DEVICE, "no HIP device");
   }
   d->num_elements = num_elements;
+  d->has_boundary = has_boundary;
   for (int i = 0; i < 3; ++i) d->ext[i] = extents[i];
   const size_t bytes = sizeof(int32_t) * 6 * static_cast<size_t>(num_elements);
   if (hipMalloc(&d->d_nbr, bytes) != hipSuccess) {
@@ -485,6 +491,16 @@ int check_domain(const sxd_domain* d) {
                 d->device, dev);
   return SXD_OK;
 }
+
+// Paths that read every face's neighbour (all but the ScalarWave Dirichlet RHS)
+// need a domain without external faces.
+int check_periodic(const sxd_domain* d) {
+  if (int rc = check_domain(d); rc != SXD_OK) return rc;
+  if (d->has_boundary)
+    return fail(SXD_ERR_UNSUPPORTED, "the domain has external faces (neighbour -1); only "
+                                     "sxd_scalar_wave_rhs_dirichlet evaluates it");
+  return SXD_OK;
+}
 }  // namespace
 
 namespace {
@@ -538,7 +554,8 @@ int run_mixed(sxd_domain* d, bool gh, int kernel, const double* u, const double*
 }
 
 int run_sw(sxd_domain* d, int kernel, const double* u, const double* gamma2,
-           const double* inv_jac, double* dt, hipStream_t stream) {
+           const double* inv_jac, double* dt, hipStream_t stream,
+           const double* ghost = nullptr) {
   if (d->mixed) return run_mixed(d, false, kernel, u, gamma2, inv_jac, dt, stream);
   double* scratch = nullptr;
   if (kernel == SXD_KERNEL_COMPOSITE) {
@@ -548,7 +565,7 @@ int run_sw(sxd_domain* d, int kernel, const double* u, const double* gamma2,
     scratch = d->d_scratch_du;
   }
   return launch_sw_rhs(d->num_elements, d->ext, d->d_nbr, u, gamma2, inv_jac, dt,
-                       kernel == SXD_KERNEL_FUSED, scratch, stream);
+                       kernel == SXD_KERNEL_FUSED, scratch, ghost, stream);
 }
 
 int run_gh(sxd_domain* d, int kernel, const double* u, const double* gammas,
@@ -574,7 +591,7 @@ int run_system(sxd_domain* d, int system, int kernel, const double* u, const dou
 
 int sxd_scalar_wave_rhs(const sxd_domain* domain, const double* u, const double* gamma2,
                         const double* inv_jac, double* dt, void* stream) {
-  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (int rc = check_periodic(domain); rc != SXD_OK) return rc;
   SXD_REQUIRE_DEVICE(u);
   SXD_REQUIRE_DEVICE(gamma2);
   SXD_REQUIRE_DEVICE(inv_jac);
@@ -585,9 +602,26 @@ int sxd_scalar_wave_rhs(const sxd_domain* domain, const double* u, const double*
                 __func__);
 }
 
+int sxd_scalar_wave_rhs_dirichlet(const sxd_domain* domain, const double* u,
+                                  const double* gamma2, const double* inv_jac,
+                                  const double* ghost, double* dt, void* stream) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(gamma2);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  SXD_REQUIRE_DEVICE(ghost);
+  SXD_REQUIRE_DEVICE(dt);
+  if (domain->mixed)
+    return fail(SXD_ERR_UNSUPPORTED, "external faces need a uniform-extent domain");
+  sxd_domain* d = const_cast<sxd_domain*>(domain);
+  return finish(run_sw(d, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,
+                       static_cast<hipStream_t>(stream), ghost),
+                __func__);
+}
+
 int sxd_gh_rhs(const sxd_domain* domain, const double* u, const double* gammas,
                const double* inv_jac, double* dt, void* stream) {
-  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (int rc = check_periodic(domain); rc != SXD_OK) return rc;
   SXD_REQUIRE_DEVICE(u);
   SXD_REQUIRE_DEVICE(gammas);
   SXD_REQUIRE_DEVICE(inv_jac);
@@ -626,7 +660,7 @@ int sxd_domain_get_kernel(const sxd_domain* domain, int system, int* kernel) {
 int sxd_domain_autotune(sxd_domain* domain, int system, const double* u, const double* params,
                         const double* inv_jac, double* dt, int repetitions, void* stream,
                         int* chosen, float* milliseconds) {
-  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (int rc = check_periodic(domain); rc != SXD_OK) return rc;
   if (system != SXD_SYSTEM_SCALAR_WAVE && system != SXD_SYSTEM_GH)
     return fail(SXD_ERR_INVALID_ARGUMENT, "unknown system %d", system);
   if (repetitions < 1) return fail(SXD_ERR_INVALID_ARGUMENT, "repetitions must be positive");
@@ -691,7 +725,7 @@ int sxd_domain_autotune(sxd_domain* domain, int system, const double* u, const d
 
 int sxd_scalar_wave_rhs_adjoint(const sxd_domain* domain, const double* gbar, const double* gamma2,
                                 const double* inv_jac, double* ubar, void* stream) {
-  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (int rc = check_periodic(domain); rc != SXD_OK) return rc;
   SXD_REQUIRE_DEVICE(gbar);
   SXD_REQUIRE_DEVICE(gamma2);
   SXD_REQUIRE_DEVICE(inv_jac);
@@ -707,7 +741,7 @@ int sxd_scalar_wave_rhs_adjoint(const sxd_domain* domain, const double* gbar, co
 
 int sxd_gh_rhs_adjoint(const sxd_domain* domain, const double* u, const double* gammas,
                        const double* inv_jac, const double* gbar, double* ubar, void* stream) {
-  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (int rc = check_periodic(domain); rc != SXD_OK) return rc;
   SXD_REQUIRE_DEVICE(u);
   SXD_REQUIRE_DEVICE(gammas);
   SXD_REQUIRE_DEVICE(inv_jac);
@@ -801,7 +835,7 @@ int sxd_rk3_stepper_create(sxd_rk3_stepper** stepper, sxd_domain* domain, int sy
                            const double* params, const double* inv_jac) {
   if (stepper == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null stepper output");
   *stepper = nullptr;
-  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (int rc = check_periodic(domain); rc != SXD_OK) return rc;
   if (system != SXD_SYSTEM_SCALAR_WAVE && system != SXD_SYSTEM_GH)
     return fail(SXD_ERR_INVALID_ARGUMENT, "unknown system %d", system);
   SXD_REQUIRE_DEVICE(u);

@@ -409,7 +409,7 @@ __global__ void __launch_bounds__(256)
 sw_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
                     const int32_t* __restrict__ nbr, const double* __restrict__ u,
                     const double* __restrict__ gamma2, const double* __restrict__ inv_jac,
-                    double* __restrict__ dt) {
+                    double* __restrict__ dt, const double* __restrict__ ghost) {
   const int ext[3] = {n0, n1, n2};
   const int na = d == 0 ? n1 : n0;
   const int nb = d == 2 ? n1 : n2;
@@ -430,21 +430,33 @@ sw_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
   const int q = opposite_point(idx[0], idx[1], idx[2], d, n0, n1, n2);
   double ui[5], ue[5], ni[3], ne[3], mi, me, pki[13], pke[13], bc[5];
 #pragma unroll
-  for (int c = 0; c < 5; ++c) {
-    ui[c] = u[(e * 5 + c) * np + p];
-    ue[c] = u[(en * 5 + c) * np + q];
-  }
+  for (int c = 0; c < 5; ++c) ui[c] = u[(e * 5 + c) * np + p];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     ni[i] = inv_jac[(e * 9 + ij3(d, i)) * np + p];
-    ne[i] = inv_jac[(en * 9 + ij3(d, i)) * np + q];
     if (side == 0) ni[i] = ni[i] * -1.0;
-    if (side == 1) ne[i] = ne[i] * -1.0;
   }
   flat_normalize(ni, &mi);
-  flat_normalize(ne, &me);
-  sw_package(pki, ui, gamma2[e * np + p], ni);
-  sw_package(pke, ue, gamma2[en * np + q], ne);
+  const double g2 = gamma2[e * np + p];
+  sw_package(pki, ui, g2, ni);
+  if (en < 0) {
+    // external face: ghost state at the own point, own gamma2, reversed unit normal
+#pragma unroll
+    for (int c = 0; c < 5; ++c) ue[c] = ghost[(e * 5 + c) * np + p];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ne[i] = ni[i] * -1.0;
+    sw_package(pke, ue, g2, ne);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) ue[c] = u[(en * 5 + c) * np + q];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      ne[i] = inv_jac[(en * 9 + ij3(d, i)) * np + q];
+      if (side == 1) ne[i] = ne[i] * -1.0;
+    }
+    flat_normalize(ne, &me);
+    sw_package(pke, ue, gamma2[en * np + q], ne);
+  }
   sw_boundary_terms(bc, pki, pke);
   const double fac = lift_factor(ext[d], mi);
 #pragma unroll
@@ -456,7 +468,7 @@ sw_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
 
 int launch_face_passes(bool gh, int num_elements, const int32_t ext[3], const int32_t* nbr,
                        const double* u, const double* aux, const double* inv_jac,
-                       double* dt, hipStream_t stream) {
+                       double* dt, const double* ghost, hipStream_t stream) {
   for (int d = 0; d < 3; ++d) {
     const int na = d == 0 ? ext[1] : ext[0];
     const int nb = d == 2 ? ext[1] : ext[2];
@@ -468,7 +480,7 @@ int launch_face_passes(bool gh, int num_elements, const int32_t ext[3], const in
                          ext[1], ext[2], num_elements, nbr, u, aux, inv_jac, dt);
     } else {
       hipLaunchKernelGGL(sw_face_pass_kernel, dim3(blocks), dim3(bs), 0, stream, d, ext[0],
-                         ext[1], ext[2], num_elements, nbr, u, aux, inv_jac, dt);
+                         ext[1], ext[2], num_elements, nbr, u, aux, inv_jac, dt, ghost);
     }
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   }
@@ -485,7 +497,8 @@ template <int N>
 __global__ void __launch_bounds__((N * N * N + 63) / 64 * 64)
 sw_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ u,
                     const double* __restrict__ gamma2, const double* __restrict__ inv_jac,
-                    double* __restrict__ dt, const double* __restrict__ dtab) {
+                    double* __restrict__ dt, const double* __restrict__ dtab,
+                    const double* __restrict__ ghost) {
   constexpr int NP = N * N * N;
   __shared__ double s_u[5 * NP];
   const int64_t e = xcd_local_element(blockIdx.x, gridDim.x);
@@ -545,17 +558,26 @@ sw_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
     flat_normalize(ni, &mi);
     sw_package(pki, up, g2, ni);
     const int64_t en = nbr[6 * e + 2 * d + side];
-    const int q = opposite_point(i, j, k, d, N, N, N);
     double uo[5], no[3], mo, pko[13], bc[5];
+    if (en < 0) {
+      // external face: ghost state at the own point, own gamma2, reversed unit normal
 #pragma unroll
-    for (int c = 0; c < 5; ++c) uo[c] = u[(en * 5 + c) * NP + q];
+      for (int c = 0; c < 5; ++c) uo[c] = ghost[(e * 5 + c) * NP + t];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      no[a] = inv_jac[(en * 9 + ij3(d, a)) * NP + q];
-      if (side == 1) no[a] = no[a] * -1.0;
+      for (int a = 0; a < 3; ++a) no[a] = ni[a] * -1.0;
+      sw_package(pko, uo, g2, no);
+    } else {
+      const int q = opposite_point(i, j, k, d, N, N, N);
+#pragma unroll
+      for (int c = 0; c < 5; ++c) uo[c] = u[(en * 5 + c) * NP + q];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        no[a] = inv_jac[(en * 9 + ij3(d, a)) * NP + q];
+        if (side == 1) no[a] = no[a] * -1.0;
+      }
+      flat_normalize(no, &mo);
+      sw_package(pko, uo, gamma2[en * NP + q], no);
     }
-    flat_normalize(no, &mo);
-    sw_package(pko, uo, gamma2[en * NP + q], no);
     sw_boundary_terms(bc, pki, pko);
     const double fac = lift_factor(N, mi);
 #pragma unroll
@@ -587,10 +609,10 @@ sw_volume_batched_kernel(int np, int64_t total, const double* __restrict__ u,
 template <int N>
 static int launch_sw_fused_n(int num_elements, const int32_t* nbr, const double* u,
                              const double* gamma2, const double* inv_jac, double* dt,
-                             const double* dtab, hipStream_t stream) {
+                             const double* dtab, const double* ghost, hipStream_t stream) {
   constexpr int threads = (N * N * N + 63) / 64 * 64;
   hipLaunchKernelGGL(sw_rhs_fused_kernel<N>, dim3(num_elements), dim3(threads), 0, stream,
-                     nbr, u, gamma2, inv_jac, dt, dtab);
+                     nbr, u, gamma2, inv_jac, dt, dtab, ghost);
   return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
 }
 
@@ -600,21 +622,21 @@ bool sw_fused_supported(const int32_t ext[3]) {
 
 int launch_sw_rhs(int num_elements, const int32_t ext[3], const int32_t* nbr,
                   const double* u, const double* gamma2, const double* inv_jac, double* dt,
-                  bool fused, double* scratch_du, hipStream_t stream) {
+                  bool fused, double* scratch_du, const double* ghost, hipStream_t stream) {
   const double* dtab = device_diff_matrices();
   if (dtab == nullptr) return SXD_ERR_DEVICE;
   if (fused) {
     if (!sw_fused_supported(ext)) return SXD_ERR_UNSUPPORTED;
     switch (ext[0]) {
-      case 2: return launch_sw_fused_n<2>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, stream);
-      case 3: return launch_sw_fused_n<3>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, stream);
-      case 4: return launch_sw_fused_n<4>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, stream);
-      case 5: return launch_sw_fused_n<5>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, stream);
-      case 6: return launch_sw_fused_n<6>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, stream);
-      case 7: return launch_sw_fused_n<7>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, stream);
-      case 8: return launch_sw_fused_n<8>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, stream);
-      case 9: return launch_sw_fused_n<9>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, stream);
-      case 10: return launch_sw_fused_n<10>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, stream);
+      case 2: return launch_sw_fused_n<2>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
+      case 3: return launch_sw_fused_n<3>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
+      case 4: return launch_sw_fused_n<4>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
+      case 5: return launch_sw_fused_n<5>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
+      case 6: return launch_sw_fused_n<6>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
+      case 7: return launch_sw_fused_n<7>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
+      case 8: return launch_sw_fused_n<8>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
+      case 9: return launch_sw_fused_n<9>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
+      case 10: return launch_sw_fused_n<10>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
       default: return SXD_ERR_UNSUPPORTED;
     }
   }
@@ -629,7 +651,8 @@ int launch_sw_rhs(int num_elements, const int32_t ext[3], const int32_t* nbr,
                      dim3(256), 0, stream, static_cast<int>(np), total, u, scratch_du, gamma2,
                      dt);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
-  return launch_face_passes(false, num_elements, ext, nbr, u, gamma2, inv_jac, dt, stream);
+  return launch_face_passes(false, num_elements, ext, nbr, u, gamma2, inv_jac, dt, ghost,
+                            stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -696,7 +719,7 @@ int launch_gh_rhs_composite(int num_elements, const int32_t ext[3], const int32_
                      scratch_temps);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   return launch_face_passes(true, num_elements, ext, nbr, u, scratch_temps, inv_jac, dt,
-                            stream);
+                            nullptr, stream);
 }
 
 // ---------------------------------------------------------------------------

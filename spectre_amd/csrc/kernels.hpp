@@ -32,10 +32,12 @@ int launch_lift_flux(int64_t n, int nc, double* bc, int n_perp, const double* ma
                      hipStream_t stream);
 bool sw_fused_supported(const int32_t ext[3]);
 // fused: one launch per RHS (requires sw_fused_supported); otherwise the
-// composite path, which needs scratch_du [E][15][Np].
+// composite path, which needs scratch_du [E][15][Np].  Faces whose neighbour
+// index is negative take their exterior state from ghost ([E][5][Np], read at
+// the face's own points); ghost may be null when no neighbour is negative.
 int launch_sw_rhs(int num_elements, const int32_t ext[3], const int32_t* nbr,
                   const double* u, const double* gamma2, const double* inv_jac, double* dt,
-                  bool fused, double* scratch_du, hipStream_t stream);
+                  bool fused, double* scratch_du, const double* ghost, hipStream_t stream);
 int launch_gh_rhs_composite(int num_elements, const int32_t ext[3], const int32_t* nbr,
                             const double* u, const double* gammas, const double* inv_jac,
                             double* dt, double* scratch_du, double* scratch_temps,

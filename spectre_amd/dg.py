@@ -161,11 +161,12 @@ def rk3_substep(substep, dt, u0, uk, lk, out=None):
 
 
 class Domain:
-    """Device-side element set of a periodic brick (conforming, aligned).
+    """Device-side element set of a brick (conforming, aligned).
 
     Wraps sxd_domain: the neighbour table lives on the device; scalar_wave_rhs
     and gh_rhs run ComputeTimeDerivative + ApplyBoundaryCorrections for all
-    elements in one call."""
+    elements in one call.  A neighbour of -1 marks an external face; such a
+    domain is evaluated by scalar_wave_rhs_dirichlet only."""
 
     def __init__(self, extents, neighbors):
         nb = np.ascontiguousarray(neighbors, dtype=np.int32)
@@ -210,6 +211,23 @@ class Domain:
         _require(out, "out", self._shapes(5))
         check(lib().sxd_scalar_wave_rhs(self._handle, _ptr(u), _ptr(gamma2), _ptr(inv_jac),
                                         _ptr(out), _stream()))
+        return out
+
+    def scalar_wave_rhs_dirichlet(self, u, gamma2, inv_jac, ghost, out=None):
+        """scalar_wave_rhs with external faces (neighbour -1) closed by the
+        Dirichlet analytic ghost state: ghost [E, 5, Np] holds the analytic
+        psi, pi, phi_i at every point (only face points are read)
+        (ScalarWave/BoundaryConditions/DirichletAnalytic.cpp:72-100)."""
+        _require(u, "u", self._shapes(5))
+        _require(gamma2, "gamma2", self._scalar_shape())
+        _require(inv_jac, "inv_jac", self._shapes(9))
+        _require(ghost, "ghost", self._shapes(5))
+        if out is None:
+            out = torch.empty_like(u)
+        _require(out, "out", self._shapes(5))
+        check(lib().sxd_scalar_wave_rhs_dirichlet(self._handle, _ptr(u), _ptr(gamma2),
+                                                  _ptr(inv_jac), _ptr(ghost), _ptr(out),
+                                                  _stream()))
         return out
 
     def gh_rhs(self, u, gammas, inv_jac, out=None):

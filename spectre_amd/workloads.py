@@ -157,17 +157,32 @@ def periodic_neighbors(elements):
         idx(ix, iy, iz - 1), idx(ix, iy, iz + 1)], axis=1).astype(np.int32)
 
 
+def bounded_neighbors(elements):
+    """periodic_neighbors with -1 on the faces of the brick's outer boundary
+    (external faces, closed by a boundary condition)."""
+    nb = periodic_neighbors(elements)
+    e = np.arange(nb.shape[0])
+    ie = (e % elements[0], (e // elements[0]) % elements[1], e // (elements[0] * elements[1]))
+    for d in range(3):
+        nb[ie[d] == 0, 2 * d] = -1
+        nb[ie[d] == elements[d] - 1, 2 * d + 1] = -1
+    return nb
+
+
 @dataclass
 class Brick:
-    """A periodic brick of Ex x Ey x Ez affine elements with uniform extents."""
+    """A brick of Ex x Ey x Ez affine elements with uniform extents, periodic
+    or (periodic=False) with external faces on its outer boundary."""
     lower: tuple
     upper: tuple
     elements: tuple
     extents: tuple
+    periodic: bool = True
     neighbors: np.ndarray = field(init=False)
 
     def __post_init__(self):
-        self.neighbors = periodic_neighbors(self.elements)
+        self.neighbors = (periodic_neighbors(self.elements) if self.periodic
+                          else bounded_neighbors(self.elements))
 
     @property
     def num_elements(self):
@@ -396,6 +411,23 @@ def scalar_wave_plane_wave_3d(seed=0, elements=(4, 4, 4), extents=(5, 5, 5)):
     u, _ = plane_wave(brick.coordinates(), seed)
     gamma2 = np.ones((brick.num_elements, brick.num_points))
     return brick, u, gamma2, brick.inverse_jacobian()
+
+
+def scalar_wave_plane_wave_dirichlet(seed=0, elements=(4, 4, 4), extents=(5, 5, 5), t=0.0,
+                                     noise=0.0, lower=(-1.0, -1.0, -1.0),
+                                     upper=(1.0, 1.0, 1.0)):
+    """Plane wave of plane_wave(seed, t) on a non-periodic brick whose outer
+    faces are closed by the Dirichlet analytic condition: the ghost state is
+    the plane wave itself at every point.  u is the plane wave with seeded
+    Gaussian noise of absolute size `noise` (0: exact); gamma2 = 1.  Returns
+    (brick, u, gamma2, inv_jac, ghost), fields [E][5][Np]."""
+    brick = Brick(lower, upper, elements, extents, periodic=False)
+    ghost, _ = plane_wave(brick.coordinates(), seed, t)
+    u = ghost.copy()
+    if noise:
+        u += noise * np.random.default_rng(seed + 1).standard_normal(u.shape)
+    gamma2 = np.ones((brick.num_elements, brick.num_points))
+    return brick, u, gamma2, brick.inverse_jacobian(), ghost
 
 
 def gh_kerr_schild_brick(seed=0, elements=(8, 8, 8), extents=(8, 8, 8),

@@ -155,6 +155,19 @@ def gh_upwind_direct(u_int, aux_int, nrm_int, u_ext, aux_ext, nrm_ext):
     return bc
 
 
+def sw_rhs_dirichlet(extents, neighbors, u, gamma2, inv_jac, ghost):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    gamma2 = np.ascontiguousarray(gamma2, dtype=np.float64)
+    inv_jac = np.ascontiguousarray(inv_jac, dtype=np.float64)
+    ghost = np.ascontiguousarray(ghost, dtype=np.float64)
+    dt = np.empty_like(u)
+    ext, ep = _i(extents)
+    nb, nbp = _i(neighbors)
+    _check(lib().sxo_sw_rhs_dirichlet(u.shape[0], ep, nbp, _d(u), _d(gamma2), _d(inv_jac),
+                                      _d(ghost), _d(dt)))
+    return dt
+
+
 def sw_rhs(extents, neighbors, u, gamma2, inv_jac):
     u = np.ascontiguousarray(u, dtype=np.float64)
     gamma2 = np.ascontiguousarray(gamma2, dtype=np.float64)

@@ -124,6 +124,44 @@ def test_scalar_wave_rhs_other_extents(extents):
     assert rel_err(got, ref) <= TOL
 
 
+@pytest.mark.parametrize("kernel,elements,extents", [
+    ("fused", (3, 2, 2), (5, 5, 5)), ("composite", (3, 2, 2), (5, 5, 5)),
+    ("composite", (2, 3, 2), (4, 6, 5)), ("fused", (1, 1, 2), (10, 10, 10)),
+    ("composite", (1, 1, 1), (7, 3, 12))])
+def test_scalar_wave_rhs_dirichlet_matches_oracle(kernel, elements, extents):
+    """External faces closed by the Dirichlet analytic ghost state: noisy
+    plane-wave data, the exact plane wave as ghost, point-varying gamma2."""
+    brick, u, _, inv, ghost = wl.scalar_wave_plane_wave_dirichlet(
+        seed=4, elements=elements, extents=extents, t=0.3, noise=1e-2)
+    gamma2 = np.random.default_rng(8).uniform(0, 2, (brick.num_elements, brick.num_points))
+    ref = ol.sw_rhs_dirichlet(brick.extents, brick.neighbors, u, gamma2, inv, ghost)
+    domain = dg.Domain.from_brick(brick)
+    domain.set_kernel("scalar_wave", kernel)
+    got = host(domain.scalar_wave_rhs_dirichlet(dev(u), dev(gamma2), dev(inv), dev(ghost)))
+    assert rel_err(got, ref) <= TOL
+    assert np.array_equal(got, ref)
+
+
+def test_domain_with_external_faces_rejects_the_periodic_calls():
+    brick, u, gamma2, inv, ghost = wl.scalar_wave_plane_wave_dirichlet(
+        seed=1, elements=(2, 1, 1), extents=(4, 4, 4))
+    domain = dg.Domain.from_brick(brick)
+    du, dg2, dinv = dev(u), dev(gamma2), dev(inv)
+    with pytest.raises(dg.SpectreAmdError, match="external faces"):
+        domain.scalar_wave_rhs(du, dg2, dinv)
+    with pytest.raises(dg.SpectreAmdError, match="external faces"):
+        domain.scalar_wave_rhs_adjoint(du, dg2, dinv)
+    gh_u = torch.zeros((2, 50, 64), dtype=torch.float64, device="cuda")
+    gh_g = torch.ones((2, 3, 64), dtype=torch.float64, device="cuda")
+    with pytest.raises(dg.SpectreAmdError, match="external faces"):
+        domain.gh_rhs(gh_u, gh_g, dinv)
+    # the Dirichlet call on a periodic domain is the periodic RHS
+    per = wl.Brick(brick.lower, brick.upper, brick.elements, brick.extents)
+    pdom = dg.Domain.from_brick(per)
+    got = host(pdom.scalar_wave_rhs_dirichlet(du, dg2, dinv, dev(ghost)))
+    np.testing.assert_array_equal(got, host(pdom.scalar_wave_rhs(du, dg2, dinv)))
+
+
 @pytest.mark.parametrize("elements,extents", [((2, 2, 2), (5, 5, 5)), ((2, 3, 2), (4, 6, 5))])
 def test_gh_rhs_small(elements, extents):
     brick, u, gammas, inv = wl.gh_kerr_schild_brick(seed=1, elements=elements, extents=extents)

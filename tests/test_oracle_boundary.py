@@ -1,0 +1,103 @@
+"""Pins of the oracle's external faces (ScalarWave Dirichlet analytic ghost
+condition, ScalarWave/BoundaryConditions/DirichletAnalytic.cpp:72-100 as
+applied by Evolution/DiscontinuousGalerkin/Actions/BoundaryConditionsImpl.hpp:
+427-586).
+
+* Equivalence with an internal face: cutting the face between two elements of a
+  periodic brick and handing each side the other's trace as its ghost state
+  reproduces the periodic RHS (same gamma2 on both sides, affine elements of
+  equal size, so the reversed interior unit normal equals the neighbour's
+  normal).  A ghost read at the wrong point or component, or a wrong normal,
+  fails this.
+* With the exact plane wave as the ghost state the RHS converges spectrally to
+  the exact time derivative, while the periodic closure of the same
+  non-periodic data does not.
+* A ghost state equal to the interior trace adds no correction.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from spectre_amd import workloads as wl
+
+
+def _face_points(extents, d, side):
+    n0, n1, n2 = extents
+    k, j, i = np.meshgrid(np.arange(n2), np.arange(n1), np.arange(n0), indexing="ij")
+    idx = (i.ravel(), j.ravel(), k.ravel())
+    return np.nonzero(idx[d] == (0 if side == 0 else extents[d] - 1))[0]
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_cut_internal_face_with_neighbour_ghost_reproduces_periodic_rhs(d):
+    elements = [1, 1, 1]
+    elements[d] = 2
+    extents = (4, 5, 6)
+    brick = wl.Brick((-1.0, 0.0, 0.5), (1.0, 1.5, 2.0), tuple(elements), extents)
+    rng = np.random.default_rng(11 + d)
+    u = rng.uniform(-1, 1, size=(2, 5, brick.num_points))
+    gamma2 = np.full((2, brick.num_points), 0.7)
+    inv = brick.inverse_jacobian()
+    periodic = ol.sw_rhs(extents, brick.neighbors, u, gamma2, inv)
+    # cut both faces between element 0 and element 1 along d
+    nb = brick.neighbors.copy()
+    assert nb[0, 2 * d + 1] == 1 and nb[1, 2 * d] == 0
+    assert nb[0, 2 * d] == 1 and nb[1, 2 * d + 1] == 0
+    nb[0, 2 * d + 1] = -1
+    nb[1, 2 * d] = -1
+    upper, lower = _face_points(extents, d, 1), _face_points(extents, d, 0)
+    ghost = rng.uniform(-5, 5, size=u.shape)  # only face points may be read
+    ghost[0][:, upper] = u[1][:, lower]
+    ghost[1][:, lower] = u[0][:, upper]
+    cut = ol.sw_rhs_dirichlet(extents, nb, u, gamma2, inv, ghost)
+    np.testing.assert_array_equal(cut, periodic)
+
+
+def _exact_dt(brick, seed, t):
+    _, k = wl.plane_wave(brick.coordinates(), seed, t)
+    w = np.linalg.norm(k)
+    ph = np.einsum("i,eip->ep", k, brick.coordinates()) - w * t
+    return np.stack([-w * np.cos(ph), w * w * np.sin(ph)]
+                    + [w * k[i] * np.sin(ph) for i in range(3)], axis=1)
+
+
+def test_dirichlet_rhs_converges_to_exact_time_derivative():
+    errors, periodic_errors = [], []
+    for n in (5, 7, 9, 11):
+        brick, u, gamma2, inv, ghost = wl.scalar_wave_plane_wave_dirichlet(
+            seed=3, elements=(2, 2, 2), extents=(n, n, n), t=0.4)
+        exact = _exact_dt(brick, 3, 0.4)
+        dt = ol.sw_rhs_dirichlet(brick.extents, brick.neighbors, u, gamma2, inv, ghost)
+        errors.append(np.max(np.abs(dt - exact)))
+        per = wl.periodic_neighbors(brick.elements)
+        periodic_errors.append(np.max(np.abs(ol.sw_rhs(brick.extents, per, u, gamma2, inv)
+                                             - exact)))
+    assert errors[-1] < 1e-5
+    assert all(b < a * 0.1 for a, b in zip(errors, errors[1:]))
+    # the plane wave is not periodic on the brick: a periodic closure is O(1) wrong
+    assert min(periodic_errors) > 0.1
+
+
+def test_ghost_equal_to_interior_trace_adds_no_correction():
+    brick = wl.Brick((-1.0, 0.0, 0.5), (1.0, 1.5, 2.0), (1, 1, 1), (5, 4, 6), periodic=False)
+    assert (brick.neighbors == -1).all()
+    rng = np.random.default_rng(5)
+    u = rng.uniform(-1, 1, size=(1, 5, brick.num_points))
+    gamma2 = rng.uniform(0.0, 2.0, size=(1, brick.num_points))
+    inv = brick.inverse_jacobian()
+    dt = ol.sw_rhs_dirichlet(brick.extents, brick.neighbors, u, gamma2, inv, u)
+    du = ol.partial_derivatives(u[0], brick.extents, inv[0])
+    vol = ol.sw_time_derivative(u[0], du, gamma2[0])
+    np.testing.assert_allclose(dt[0], vol, rtol=0, atol=1e-12 * np.max(np.abs(vol)))
+
+
+def test_bounded_neighbour_table():
+    nb = wl.bounded_neighbors((3, 2, 1))
+    per = wl.periodic_neighbors((3, 2, 1))
+    assert nb.shape == (6, 6)
+    # x: element 0 has no lower, element 2 no upper neighbour
+    assert nb[0, 0] == -1 and nb[0, 1] == 1 and nb[2, 1] == -1 and nb[1, 0] == 0
+    # one element along z: both z faces external everywhere
+    assert (nb[:, 4] == -1).all() and (nb[:, 5] == -1).all()
+    keep = nb >= 0
+    np.testing.assert_array_equal(nb[keep], per[keep])
