@@ -137,8 +137,9 @@ int sxd_lift_flux(int64_t num_points, int nc, double* bc,
  * ApplyBoundaryCorrections.hpp:722-1050) for a single device.
  * neighbors: host array [6 E], entry 6 e + 2 d + side (side 0 = lower,
  * 1 = upper) is the element across that face, or -1 for an external face
- * (closed by a ghost state, see sxd_scalar_wave_rhs_dirichlet; every other
- * call on such a domain returns SXD_ERR_UNSUPPORTED).  The table is copied to
+ * (closed by a ghost state, see sxd_scalar_wave_rhs_dirichlet and
+ * sxd_gh_rhs_dirichlet; every other call on such a domain returns
+ * SXD_ERR_UNSUPPORTED).  The table is copied to
  * the device; this call allocates device memory. */
 typedef struct sxd_domain sxd_domain;
 int sxd_domain_create(sxd_domain** domain, int num_elements,
@@ -188,6 +189,23 @@ int sxd_scalar_wave_rhs_dirichlet(const sxd_domain* domain, const double* u,
  * u, dt [E][50][Np]; gammas [E][3][Np]; inv_jac [E][9][Np]. */
 int sxd_gh_rhs(const sxd_domain* domain, const double* u, const double* gammas,
                const double* inv_jac, double* dt, void* stream);
+
+/* sxd_gh_rhs on a domain with external faces (neighbour -1), closed by the
+ * generalized-harmonic Dirichlet analytic ghost condition: on an external face
+ * the exterior UpwindPenalty state is ghost's g_ab, Pi_ab, Phi_iab at the
+ * face's own points, with the interior gamma1 and gamma2, the lapse, shift and
+ * inverse spatial metric of the ghost metric, and the reversed interior unit
+ * normal renormalised with the ghost inverse spatial metric; the correction is
+ * lifted with the interior normal magnitude
+ * (GeneralizedHarmonic/BoundaryConditions/DirichletAnalytic.cpp:70-115
+ * dg_ghost, applied by
+ * Evolution/DiscontinuousGalerkin/Actions/BoundaryConditionsImpl.hpp:427-586).
+ * ghost [E][50][Np] (only face points are read).  Runs the domain's fused
+ * (vector-ALU) or composite kernel; SXD_ERR_UNSUPPORTED if the fused MFMA
+ * kernel is selected. */
+int sxd_gh_rhs_dirichlet(const sxd_domain* domain, const double* u,
+                         const double* gammas, const double* inv_jac,
+                         const double* ghost, double* dt, void* stream);
 
 /* ---- Kernel selection (no counterpart in the reference) ----------------- */
 

@@ -826,9 +826,37 @@ void gh_rhs_volume(int e_begin, int e_end, const int ext[3], const T* u,
   }
 }
 
+// Exterior normal of a ghost face (BoundaryConditionsImpl.hpp:524-560): the
+// reversed interior unit covector n, raised with the exterior inverse spatial
+// metric, and both divided by sqrt(n_i n^i).
+template <class T>
+inline void gh_ghost_normal(T n[3], T nv[3], const T inv_sp[6]) {
+  for (int i = 0; i < 3; ++i) {
+    T s = inv_sp[sym3(i, 0)] * n[0];
+    for (int j = 1; j < 3; ++j) s = fma_(inv_sp[sym3(i, j)], n[j], s);
+    nv[i] = s;
+  }
+  T m = n[0] * nv[0];
+  for (int i = 1; i < 3; ++i) m = fma_(n[i], nv[i], m);
+  const T mag = sqrt_(m);
+  for (int i = 0; i < 3; ++i) {
+    n[i] = n[i] / mag;
+    nv[i] = nv[i] / mag;
+  }
+}
+
+// External faces (neighbour index < 0) take the exterior state from `ghost`
+// ([E][50][Np], read at the face's own points), as the generalized-harmonic
+// DirichletAnalytic::dg_ghost does (GeneralizedHarmonic/BoundaryConditions/
+// DirichletAnalytic.cpp:70-115): ghost fields, the interior gamma1 and gamma2,
+// and lapse, shift and inverse spatial metric of the ghost spacetime metric;
+// the exterior normal is gh_ghost_normal of the reversed interior unit normal
+// and the correction is lifted with the interior magnitude
+// (Evolution/DiscontinuousGalerkin/Actions/BoundaryConditionsImpl.hpp:427-586).
 template <class T>
 void gh_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
-                  const T* u, const T* inv_j, T* dt, const GhFaceTemps<T>* temps) {
+                  const T* u, const T* inv_j, T* dt, const GhFaceTemps<T>* temps,
+                  const T* ghost = nullptr) {
   const size_t np = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
   for (int e = e_begin; e < e_end; ++e) {
     for (int dir = 0; dir < 6; ++dir) {
@@ -838,18 +866,37 @@ void gh_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
       const Face fe = make_face(ext, d, 1 - side);
       for (size_t f = 0; f < fi.pts.size(); ++f) {
         const size_t p = fi.pts[f];
-        const size_t q = fe.pts[f];
         const GhFaceTemps<T>& ti = temps[e * np + p];
-        const GhFaceTemps<T>& te = temps[en * np + q];
         T n_in[3], nv_in[3], mag_in, n_ex[3], nv_ex[3], mag_ex;
         for (int i = 0; i < 3; ++i) {
           n_in[i] = inv_j[(e * 9 + ij3(d, i)) * np + p];
           if (side == 0) n_in[i] = n_in[i] * -1.0;
-          n_ex[i] = inv_j[(en * 9 + ij3(d, i)) * np + q];
-          if (side == 1) n_ex[i] = n_ex[i] * -1.0;
         }
         gh_normalize(n_in, nv_in, &mag_in, ti.inv_spatial);
-        gh_normalize(n_ex, nv_ex, &mag_ex, te.inv_spatial);
+        // exterior side: the neighbour's point q, or the ghost state at p
+        const T* ux;
+        size_t q;
+        GhFaceTemps<T> te;
+        if (en < 0) {
+          ux = ghost + e * 50 * np;
+          q = p;
+          T g[10], det;
+          for (int c = 0; c < 10; ++c) g[c] = ux[c * np + q];
+          te.gamma1 = ti.gamma1;
+          te.gamma2 = ti.gamma2;
+          gh_lapse_shift(&te.lapse, te.shift, &det, te.inv_spatial, g);
+          for (int i = 0; i < 3; ++i) n_ex[i] = n_in[i] * -1.0;
+          gh_ghost_normal(n_ex, nv_ex, te.inv_spatial);
+        } else {
+          ux = u + en * 50 * np;
+          q = fe.pts[f];
+          te = temps[en * np + q];
+          for (int i = 0; i < 3; ++i) {
+            n_ex[i] = inv_j[(en * 9 + ij3(d, i)) * np + q];
+            if (side == 1) n_ex[i] = n_ex[i] * -1.0;
+          }
+          gh_normalize(n_ex, nv_ex, &mag_ex, te.inv_spatial);
+        }
         T lam_in[4], lam_ex[4];
         gh_char_speeds(lam_in, ti.gamma1, ti.lapse, ti.shift, n_in);
         gh_char_speeds(lam_ex, te.gamma1, te.lapse, te.shift, n_ex);
@@ -858,14 +905,13 @@ void gh_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
           T phi_i[3], phi_e[3];
           for (int i = 0; i < 3; ++i) {
             phi_i[i] = u[(e * 50 + 20 + i + 3 * pr) * np + p];
-            phi_e[i] = u[(en * 50 + 20 + i + 3 * pr) * np + q];
+            phi_e[i] = ux[(20 + i + 3 * pr) * np + q];
           }
           T pk_in[13], pk_ex[13], bc[5];
           gh_package_pair(pk_in, u[(e * 50 + pr) * np + p],
                           u[(e * 50 + 10 + pr) * np + p], phi_i, ti.gamma2,
                           lam_in, n_in, nv_in);
-          gh_package_pair(pk_ex, u[(en * 50 + pr) * np + q],
-                          u[(en * 50 + 10 + pr) * np + q], phi_e, te.gamma2,
+          gh_package_pair(pk_ex, ux[pr * np + q], ux[(10 + pr) * np + q], phi_e, te.gamma2,
                           lam_ex, n_ex, nv_ex);
           gh_boundary_terms_pair(bc, pk_in, lam_in, pk_ex, lam_ex);
           const int comp[5] = {pr, 10 + pr, 20 + 3 * pr, 21 + 3 * pr, 22 + 3 * pr};
@@ -881,11 +927,11 @@ void gh_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
 
 template <class T>
 void gh_rhs(int num_elements, const int ext[3], const int* nbr, const T* u,
-            const T* gammas, const T* inv_j, T* dt) {
+            const T* gammas, const T* inv_j, T* dt, const T* ghost = nullptr) {
   const size_t np = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
   std::vector<GhFaceTemps<T>> temps(static_cast<size_t>(num_elements) * np);
   gh_rhs_volume(0, num_elements, ext, u, gammas, inv_j, dt, temps.data());
-  gh_rhs_faces(0, num_elements, ext, nbr, u, inv_j, dt, temps.data());
+  gh_rhs_faces(0, num_elements, ext, nbr, u, inv_j, dt, temps.data(), ghost);
 }
 
 // ---------------------------------------------------------------------------

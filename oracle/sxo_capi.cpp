@@ -346,6 +346,16 @@ int sxo_sw_rhs_dirichlet(int num_elements, const int* ext, const int* nbr, const
   return 0;
 }
 
+// Generalized-harmonic RHS with external faces (neighbour < 0) closed by the
+// ghost state `ghost` [E][50][Np] (Dirichlet analytic boundary condition).
+int sxo_gh_rhs_dirichlet(int num_elements, const int* ext, const int* nbr, const double* u,
+                         const double* gammas, const double* inv_j, const double* ghost,
+                         double* dt) {
+  if (bad_extents(ext) || num_elements < 1 || ghost == nullptr) return -1;
+  sxo::gh_rhs(num_elements, ext, nbr, u, gammas, inv_j, dt, ghost);
+  return 0;
+}
+
 int sxo_gh_rhs(int num_elements, const int* ext, const int* nbr, const double* u,
                const double* gammas, const double* inv_j, double* dt) {
   if (bad_extents(ext) || num_elements < 1) return -1;

@@ -492,13 +492,13 @@ int check_domain(const sxd_domain* d) {
   return SXD_OK;
 }
 
-// Paths that read every face's neighbour (all but the ScalarWave Dirichlet RHS)
+// Paths that read every face's neighbour (all but the Dirichlet RHS calls)
 // need a domain without external faces.
 int check_periodic(const sxd_domain* d) {
   if (int rc = check_domain(d); rc != SXD_OK) return rc;
   if (d->has_boundary)
     return fail(SXD_ERR_UNSUPPORTED, "the domain has external faces (neighbour -1); only "
-                                     "sxd_scalar_wave_rhs_dirichlet evaluates it");
+                                     "the *_rhs_dirichlet calls evaluate it");
   return SXD_OK;
 }
 }  // namespace
@@ -569,17 +569,18 @@ int run_sw(sxd_domain* d, int kernel, const double* u, const double* gamma2,
 }
 
 int run_gh(sxd_domain* d, int kernel, const double* u, const double* gammas,
-           const double* inv_jac, double* dt, hipStream_t stream) {
+           const double* inv_jac, double* dt, hipStream_t stream,
+           const double* ghost = nullptr) {
   if (d->mixed) return run_mixed(d, true, kernel, u, gammas, inv_jac, dt, stream);
   if (kernel == SXD_KERNEL_FUSED || kernel == SXD_KERNEL_FUSED_MFMA)
     return launch_gh_rhs_fused(d->num_elements, d->ext, d->d_nbr, u, gammas, inv_jac, dt,
                                kernel == SXD_KERNEL_FUSED ? kGhFusedValu : kGhFusedMfma,
-                               d->threads[SXD_SYSTEM_GH] == 512 ? 512 : 256, stream);
+                               d->threads[SXD_SYSTEM_GH] == 512 ? 512 : 256, ghost, stream);
   const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
   if (ensure_scratch(d, 150 * np * d->num_elements, 12 * np * d->num_elements) != SXD_OK)
     return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
   return launch_gh_rhs_composite(d->num_elements, d->ext, d->d_nbr, u, gammas, inv_jac, dt,
-                                 d->d_scratch_du, d->d_scratch_temps, stream);
+                                 d->d_scratch_du, d->d_scratch_temps, ghost, stream);
 }
 
 int run_system(sxd_domain* d, int system, int kernel, const double* u, const double* params,
@@ -629,6 +630,27 @@ int sxd_gh_rhs(const sxd_domain* domain, const double* u, const double* gammas,
   sxd_domain* d = const_cast<sxd_domain*>(domain);
   return finish(run_gh(d, resolve_kernel(d, SXD_SYSTEM_GH), u, gammas, inv_jac, dt,
                        static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+int sxd_gh_rhs_dirichlet(const sxd_domain* domain, const double* u, const double* gammas,
+                         const double* inv_jac, const double* ghost, double* dt,
+                         void* stream) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(gammas);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  SXD_REQUIRE_DEVICE(ghost);
+  SXD_REQUIRE_DEVICE(dt);
+  if (domain->mixed)
+    return fail(SXD_ERR_UNSUPPORTED, "external faces need a uniform-extent domain");
+  sxd_domain* d = const_cast<sxd_domain*>(domain);
+  const int kernel = resolve_kernel(d, SXD_SYSTEM_GH);
+  if (kernel == SXD_KERNEL_FUSED_MFMA)
+    return fail(SXD_ERR_UNSUPPORTED, "external faces are not built for the MFMA engine; "
+                                     "select the fused or composite kernel");
+  return finish(run_gh(d, kernel, u, gammas, inv_jac, dt, static_cast<hipStream_t>(stream),
+                       ghost),
                 __func__);
 }
 

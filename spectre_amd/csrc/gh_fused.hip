@@ -21,6 +21,11 @@
 //     of the element (6 N^2 slots, strided over the threads) gets the
 //     neighbour side computed from the neighbour's g_ab, gamma1, gamma2 and
 //     inverse Jacobian at the matching point (HBM reads of face traces).
+//     GHOST instantiations also take external faces (neighbour -1): their
+//     slots read the ghost state at the element's own face point and use the
+//     element's gamma1, gamma2 and reversed unit normal (Dirichlet analytic
+//     condition, GeneralizedHarmonic/BoundaryConditions/DirichletAnalytic.cpp:
+//     70-115, Actions/BoundaryConditionsImpl.hpp:427-586).
 //  B  for each symmetric pair (a,b) (10 passes): stage g_ab, Pi_ab, Phi_{i,ab}
 //     of the element in LDS (the next pass's values are prefetched into
 //     registers while the current pass computes); face slots form the lifted
@@ -176,11 +181,12 @@ __device__ __forceinline__ void mfma_logical_derivatives(const double* stage, do
   }
 }
 
-template <int N, bool MFMA, int TPB>
+template <int N, bool MFMA, int TPB, bool GHOST>
 __global__ void __launch_bounds__((GhFusedLayout<N, MFMA, TPB>::THREADS), 1)
 gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ u,
                     const double* __restrict__ gammas, const double* __restrict__ inv_jac,
-                    double* __restrict__ dt, const double* __restrict__ dtab) {
+                    double* __restrict__ dt, const double* __restrict__ dtab,
+                    const double* __restrict__ ghost) {
   using L = GhFusedLayout<N, MFMA, TPB>;
   constexpr int NP = L::NP, NF = L::NF, P = L::P, PF = L::PF, T = L::THREADS;
   __shared__ __align__(16) double smem[L::kDoubles];
@@ -254,16 +260,59 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
   // neighbour side of the face slots s = t + f T
   bool fact[PF];
   int64_t nb_base[PF];  // neighbour element offset + neighbour volume index
+  bool nb_ghost[PF];    // GHOST: nb_base indexes the ghost state
 #pragma unroll
   for (int f = 0; f < PF; ++f) {
     const int slot = t + f * T;
     fact[f] = slot < L::SLOTS;
     nb_base[f] = 0;
+    nb_ghost[f] = false;
     if (!fact[f]) continue;
     const int dir = slot / NF;
     const int fp = slot % NF;
     const int d = dir >> 1, side = dir & 1;
     const int64_t en = nbr[6 * e + dir];
+    if constexpr (GHOST) {
+      if (en < 0) {
+        // the element's own face point and its unit normal, formed as the
+        // owner of the point forms it in the loop above
+        const int po = face_volume_index<N>(dir, fp % N, fp / N);
+        nb_ghost[f] = true;
+        nb_base[f] = e * 50 * NP + po;
+        double go[10], lo, so[3], iso[6], mag;
+#pragma unroll
+        for (int c = 0; c < 10; ++c) go[c] = ue[c * NP + po];
+        gh_lapse_shift(&lo, so, iso, go);
+        double n_in[3], nv_in[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double r = inv_jac[(e * 9 + ij3(d, c)) * NP + po];
+          n_in[c] = side == 0 ? r * -1.0 : r;
+        }
+        gh_normalize(n_in, nv_in, &mag, iso);
+        const double* gx = ghost + nb_base[f];
+        double g[10], lapse, shift[3], inv_sp[6];
+#pragma unroll
+        for (int c = 0; c < 10; ++c) g[c] = gx[c * NP];
+        gh_lapse_shift(&lapse, shift, inv_sp, g);
+        GhFaceSide s;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s.n[c] = n_in[c] * -1.0;
+        gh_ghost_normal(s.n, s.nv, inv_sp);
+        const double* go_ = gammas + e * 3 * NP + po;
+        gh_char_speeds(s.lam, go_[NP], lapse, shift, s.n);
+        double* fe = smem + L::kFaceExt + slot * L::kExtStride;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          fe[c] = s.n[c];
+          fe[3 + c] = s.nv[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) fe[6 + c] = s.lam[c];
+        fe[10] = go_[2 * NP];
+        continue;
+      }
+    }
     const int q = face_volume_index<N>(dir ^ 1, fp % N, fp / N);
     nb_base[f] = en * 50 * NP + q;
     const double* un = u + nb_base[f];
@@ -337,7 +386,14 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
 #pragma unroll
     for (int f = 0; f < PF; ++f)
 #pragma unroll
-      for (int c = 0; c < 5; ++c) ext_next[f][c] = fact[f] ? u[nb_base[f] + comp[c] * NP] : 0.0;
+      for (int c = 0; c < 5; ++c) {
+        if constexpr (GHOST) {
+          ext_next[f][c] =
+              fact[f] ? (nb_ghost[f] ? ghost : u)[nb_base[f] + comp[c] * NP] : 0.0;
+        } else {
+          ext_next[f][c] = fact[f] ? u[nb_base[f] + comp[c] * NP] : 0.0;
+        }
+      }
   };
   // inertial derivatives of an owned point from its three logical ones
   auto contract = [&](int q, double d0, double d1, double d2, double* out3) {
@@ -642,25 +698,31 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
   }
 }
 
-template <int N, bool MFMA, int TPB>
+template <int N, bool MFMA, int TPB, bool GHOST>
 int launch_gh_fused_n(int num_elements, const int32_t* nbr, const double* u,
                       const double* gammas, const double* inv_jac, double* dt,
-                      const double* dtab, hipStream_t stream) {
-  hipLaunchKernelGGL((gh_rhs_fused_kernel<N, MFMA, TPB>), dim3(num_elements),
+                      const double* dtab, const double* ghost, hipStream_t stream) {
+  hipLaunchKernelGGL((gh_rhs_fused_kernel<N, MFMA, TPB, GHOST>), dim3(num_elements),
                      dim3(GhFusedLayout<N, MFMA, TPB>::THREADS), 0, stream, nbr, u, gammas,
-                     inv_jac, dt, dtab);
+                     inv_jac, dt, dtab, ghost);
   return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
 }
 
+// External faces (ghost != null) are instantiated for the vector-ALU engine.
 template <int N, int TPB>
 int launch_gh_fused_engine(int variant, int num_elements, const int32_t* nbr, const double* u,
                            const double* gammas, const double* inv_jac, double* dt,
-                           const double* dtab, hipStream_t stream) {
+                           const double* dtab, const double* ghost, hipStream_t stream) {
+  if (ghost != nullptr) {
+    if (variant == kGhFusedMfma) return SXD_ERR_UNSUPPORTED;
+    return launch_gh_fused_n<N, false, TPB, true>(num_elements, nbr, u, gammas, inv_jac, dt,
+                                                  dtab, ghost, stream);
+  }
   if (variant == kGhFusedMfma)
-    return launch_gh_fused_n<N, true, TPB>(num_elements, nbr, u, gammas, inv_jac, dt, dtab,
-                                           stream);
-  return launch_gh_fused_n<N, false, TPB>(num_elements, nbr, u, gammas, inv_jac, dt, dtab,
-                                          stream);
+    return launch_gh_fused_n<N, true, TPB, false>(num_elements, nbr, u, gammas, inv_jac, dt,
+                                                  dtab, nullptr, stream);
+  return launch_gh_fused_n<N, false, TPB, false>(num_elements, nbr, u, gammas, inv_jac, dt,
+                                                 dtab, nullptr, stream);
 }
 
 // Below 7^3 points every tile has at most 256 threads; 512 only differs from
@@ -668,14 +730,15 @@ int launch_gh_fused_engine(int variant, int num_elements, const int32_t* nbr, co
 template <int N>
 int launch_gh_fused_variant(int variant, int threads, int num_elements, const int32_t* nbr,
                             const double* u, const double* gammas, const double* inv_jac,
-                            double* dt, const double* dtab, hipStream_t stream) {
+                            double* dt, const double* dtab, const double* ghost,
+                            hipStream_t stream) {
   if constexpr (N >= 7) {
     if (threads == 512)
       return launch_gh_fused_engine<N, 512>(variant, num_elements, nbr, u, gammas, inv_jac, dt,
-                                            dtab, stream);
+                                            dtab, ghost, stream);
   }
   return launch_gh_fused_engine<N, 256>(variant, num_elements, nbr, u, gammas, inv_jac, dt, dtab,
-                                        stream);
+                                        ghost, stream);
 }
 
 }  // namespace
@@ -686,7 +749,8 @@ bool gh_fused_supported(const int32_t ext[3]) {
 
 int launch_gh_rhs_fused(int num_elements, const int32_t ext[3], const int32_t* nbr,
                         const double* u, const double* gammas, const double* inv_jac,
-                        double* dt, int variant, int threads, hipStream_t stream) {
+                        double* dt, int variant, int threads, const double* ghost,
+                        hipStream_t stream) {
   const double* dtab = device_diff_matrices();
   if (dtab == nullptr) return SXD_ERR_DEVICE;
   if (!gh_fused_supported(ext)) return SXD_ERR_UNSUPPORTED;
@@ -695,7 +759,7 @@ int launch_gh_rhs_fused(int num_elements, const int32_t ext[3], const int32_t* n
 #define SXD_GH_FUSED_CASE(n) \
   case n:                    \
     return launch_gh_fused_variant<n>(variant, threads, num_elements, nbr, u, gammas, inv_jac, dt, \
-                                      dtab, stream);
+                                      dtab, ghost, stream);
     SXD_GH_FUSED_CASE(2)
     SXD_GH_FUSED_CASE(3)
     SXD_GH_FUSED_CASE(4)

@@ -374,6 +374,30 @@ __device__ __forceinline__ void gh_char_speeds(double lam[4], double gamma1, dou
 
 // Per-face-point quantities of one side: unit normal covector and vector,
 // characteristic speeds and gamma2.
+// Exterior normal of a ghost face (Evolution/DiscontinuousGalerkin/Actions/
+// BoundaryConditionsImpl.hpp:524-560): n holds the reversed interior unit
+// covector on entry; it is raised with the exterior inverse spatial metric and
+// both are divided by sqrt(n_i n^i).
+__device__ __forceinline__ void gh_ghost_normal(double n[3], double nv[3],
+                                                const double inv_sp[6]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double s = inv_sp[sym3(i, 0)] * n[0];
+    s = fma(inv_sp[sym3(i, 1)], n[1], s);
+    s = fma(inv_sp[sym3(i, 2)], n[2], s);
+    nv[i] = s;
+  }
+  double m = n[0] * nv[0];
+  m = fma(n[1], nv[1], m);
+  m = fma(n[2], nv[2], m);
+  const double mag = sqrt(m);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    n[i] = n[i] / mag;
+    nv[i] = nv[i] / mag;
+  }
+}
+
 struct GhFaceSide {
   double n[3], nv[3], lam[4], gamma2;
 };

@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(128)
 gh_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
                     const int32_t* __restrict__ nbr, const double* __restrict__ u,
                     const double* __restrict__ temps, const double* __restrict__ inv_jac,
-                    double* __restrict__ dt) {
+                    double* __restrict__ dt, const double* __restrict__ ghost) {
   const int ext[3] = {n0, n1, n2};
   const int na = d == 0 ? n1 : n0;
   const int nb = d == 2 ? n1 : n2;
@@ -347,9 +347,7 @@ gh_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
   const int np = n0 * n1 * n2;
   const int p = idx[0] + n0 * (idx[1] + n1 * idx[2]);
   const int64_t en = nbr[6 * e + 2 * d + side];
-  const int q = opposite_point(idx[0], idx[1], idx[2], d, n0, n1, n2);
   const double* tp = temps + e * 12 * np;
-  const double* tq = temps + en * 12 * np;
   GhFaceSide fi, fe;
   double mag_i, mag_e;
   {
@@ -366,7 +364,26 @@ gh_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
     gh_char_speeds(fi.lam, tp[p], tp[2 * np + p], shift, fi.n);
     fi.gamma2 = tp[np + p];
   }
-  {
+  // exterior side: the neighbour's point q, or (external face) the ghost state
+  // at p with the interior damping parameters and the reversed unit normal
+  const double* un;
+  int q;
+  if (en < 0) {
+    un = ghost + e * 50 * np;
+    q = p;
+    double g[10], lapse, shift[3], inv_sp[6];
+#pragma unroll
+    for (int c = 0; c < 10; ++c) g[c] = un[c * np + q];
+    gh_lapse_shift(&lapse, shift, inv_sp, g);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) fe.n[i] = fi.n[i] * -1.0;
+    gh_ghost_normal(fe.n, fe.nv, inv_sp);
+    gh_char_speeds(fe.lam, tp[p], lapse, shift, fe.n);
+    fe.gamma2 = fi.gamma2;
+  } else {
+    un = u + en * 50 * np;
+    q = opposite_point(idx[0], idx[1], idx[2], d, n0, n1, n2);
+    const double* tq = temps + en * 12 * np;
     double inv_sp[6], shift[3];
 #pragma unroll
     for (int i = 0; i < 6; ++i) inv_sp[i] = tq[(6 + i) * np + q];
@@ -382,7 +399,6 @@ gh_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
   }
   const double fac = lift_factor(ext[d], mag_i);
   const double* ue = u + e * 50 * np;
-  const double* un = u + en * 50 * np;
   double* dte = dt + e * 50 * np;
 #pragma unroll
   for (int pr = 0; pr < 10; ++pr) {
@@ -477,7 +493,7 @@ int launch_face_passes(bool gh, int num_elements, const int32_t ext[3], const in
     const unsigned blocks = static_cast<unsigned>((total + bs - 1) / bs);
     if (gh) {
       hipLaunchKernelGGL(gh_face_pass_kernel, dim3(blocks), dim3(bs), 0, stream, d, ext[0],
-                         ext[1], ext[2], num_elements, nbr, u, aux, inv_jac, dt);
+                         ext[1], ext[2], num_elements, nbr, u, aux, inv_jac, dt, ghost);
     } else {
       hipLaunchKernelGGL(sw_face_pass_kernel, dim3(blocks), dim3(bs), 0, stream, d, ext[0],
                          ext[1], ext[2], num_elements, nbr, u, aux, inv_jac, dt, ghost);
@@ -708,7 +724,7 @@ gh_volume_batched_kernel(int np, int64_t total, const double* __restrict__ u,
 int launch_gh_rhs_composite(int num_elements, const int32_t ext[3], const int32_t* nbr,
                             const double* u, const double* gammas, const double* inv_jac,
                             double* dt, double* scratch_du, double* scratch_temps,
-                            hipStream_t stream) {
+                            const double* ghost, hipStream_t stream) {
   const int np = ext[0] * ext[1] * ext[2];
   int rc = launch_partial_derivatives(num_elements, ext, 50, u, inv_jac, scratch_du, nullptr,
                                       stream);
@@ -719,7 +735,7 @@ int launch_gh_rhs_composite(int num_elements, const int32_t ext[3], const int32_
                      scratch_temps);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   return launch_face_passes(true, num_elements, ext, nbr, u, scratch_temps, inv_jac, dt,
-                            nullptr, stream);
+                            ghost, stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -38,18 +38,21 @@ bool sw_fused_supported(const int32_t ext[3]);
 int launch_sw_rhs(int num_elements, const int32_t ext[3], const int32_t* nbr,
                   const double* u, const double* gamma2, const double* inv_jac, double* dt,
                   bool fused, double* scratch_du, const double* ghost, hipStream_t stream);
+// ghost: as for launch_sw_rhs, with [E][50][Np].
 int launch_gh_rhs_composite(int num_elements, const int32_t ext[3], const int32_t* nbr,
                             const double* u, const double* gammas, const double* inv_jac,
                             double* dt, double* scratch_du, double* scratch_temps,
-                            hipStream_t stream);
+                            const double* ghost, hipStream_t stream);
 // Derivative engines of the fused GH kernel.
 constexpr int kGhFusedValu = 0;  // LDS pencils on the vector ALUs
 constexpr int kGhFusedMfma = 1;  // v_mfma_f64_16x16x4f64 tiles
 bool gh_fused_supported(const int32_t ext[3]);
-// threads: 256 or 512 per workgroup (512 applies to N = 7, 8).
+// threads: 256 or 512 per workgroup (512 applies to N = 7, 8).  ghost: as for
+// launch_gh_rhs_composite; the MFMA engine takes none (SXD_ERR_UNSUPPORTED).
 int launch_gh_rhs_fused(int num_elements, const int32_t ext[3], const int32_t* nbr,
                         const double* u, const double* gammas, const double* inv_jac,
-                        double* dt, int variant, int threads, hipStream_t stream);
+                        double* dt, int variant, int threads, const double* ghost,
+                        hipStream_t stream);
 // Mixed p.  Per-element record of the ragged layout.
 struct MixedElement {
   int32_t n[3];  // LGL extents

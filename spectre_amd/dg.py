@@ -166,7 +166,8 @@ class Domain:
     Wraps sxd_domain: the neighbour table lives on the device; scalar_wave_rhs
     and gh_rhs run ComputeTimeDerivative + ApplyBoundaryCorrections for all
     elements in one call.  A neighbour of -1 marks an external face; such a
-    domain is evaluated by scalar_wave_rhs_dirichlet only."""
+    domain is evaluated by scalar_wave_rhs_dirichlet and gh_rhs_dirichlet
+    only."""
 
     def __init__(self, extents, neighbors):
         nb = np.ascontiguousarray(neighbors, dtype=np.int32)
@@ -239,6 +240,22 @@ class Domain:
         _require(out, "out", self._shapes(50))
         check(lib().sxd_gh_rhs(self._handle, _ptr(u), _ptr(gammas), _ptr(inv_jac), _ptr(out),
                                _stream()))
+        return out
+
+    def gh_rhs_dirichlet(self, u, gammas, inv_jac, ghost, out=None):
+        """gh_rhs with external faces (neighbour -1) closed by the Dirichlet
+        analytic ghost state: ghost [E, 50, Np] holds the analytic g, Pi, Phi
+        at every point (only face points are read)
+        (GeneralizedHarmonic/BoundaryConditions/DirichletAnalytic.cpp:70-115)."""
+        _require(u, "u", self._shapes(50))
+        _require(gammas, "gammas", self._shapes(3))
+        _require(inv_jac, "inv_jac", self._shapes(9))
+        _require(ghost, "ghost", self._shapes(50))
+        if out is None:
+            out = torch.empty_like(u)
+        _require(out, "out", self._shapes(50))
+        check(lib().sxd_gh_rhs_dirichlet(self._handle, _ptr(u), _ptr(gammas), _ptr(inv_jac),
+                                         _ptr(ghost), _ptr(out), _stream()))
         return out
 
     def scalar_wave_rhs_adjoint(self, gbar, gamma2, inv_jac, out=None):

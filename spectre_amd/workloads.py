@@ -1,10 +1,11 @@
-"""Synthetic workloads: periodic affine bricks and analytic initial data.
+"""Synthetic workloads: affine bricks and analytic initial data.
 
 These build the inputs of the DG right-hand side in SpECTRE's layouts:
 
-* element ordering e = ex + Ex (ey + Ey ez); neighbours wrap periodically and
-  are listed per element in the direction order lower_xi, upper_xi,
-  lower_eta, upper_eta, lower_zeta, upper_zeta;
+* element ordering e = ex + Ex (ey + Ey ez); neighbours wrap periodically (or
+  are -1 on the outer faces of a non-periodic brick, whose boundary condition
+  takes a ghost state) and are listed per element in the direction order
+  lower_xi, upper_xi, lower_eta, upper_eta, lower_zeta, upper_zeta;
 * per-element fields are Variables buffers [nc][Np] with xi fastest
   (DataStructures/Index.hpp:177-183);
 * the inverse Jacobian is 9 blocks, block j + 3 i = d xi^j / d x^i
@@ -446,6 +447,24 @@ def gh_kerr_schild_brick(seed=0, elements=(8, 8, 8), extents=(8, 8, 8),
     gammas[:, 1] = -1.0
     gammas[:, 2] = 1.0
     return brick, u, gammas, brick.inverse_jacobian()
+
+
+def gh_kerr_schild_dirichlet(seed=0, elements=(2, 2, 2), extents=(6, 6, 6), noise=1.0e-4,
+                             lower=(3.0, -4.0, -4.0), upper=(11.0, 4.0, 4.0)):
+    """gh_kerr_schild_brick on a non-periodic brick whose outer faces are
+    closed by the Dirichlet analytic condition: the ghost state is the exact
+    Kerr-Schild solution at every point, u carries the seeded noise.  Returns
+    (brick, u, gammas, inv_jac, ghost), fields [E][50][Np]."""
+    brick = Brick(lower, upper, elements, extents, periodic=False)
+    ghost = gh_kerr_schild(brick.coordinates())
+    u = ghost.copy()
+    if noise:
+        u *= 1.0 + noise * np.random.default_rng(seed).standard_normal(u.shape)
+    gammas = np.empty((brick.num_elements, 3, brick.num_points))
+    gammas[:, 0] = 1.0
+    gammas[:, 1] = -1.0
+    gammas[:, 2] = 1.0
+    return brick, u, gammas, brick.inverse_jacobian(), ghost
 
 
 def scalar_wave_plane_wave_mixed(seed=0, elements=(10, 10, 10), lowest=4, highest=12):

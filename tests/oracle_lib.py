@@ -168,6 +168,19 @@ def sw_rhs_dirichlet(extents, neighbors, u, gamma2, inv_jac, ghost):
     return dt
 
 
+def gh_rhs_dirichlet(extents, neighbors, u, gammas, inv_jac, ghost):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    gammas = np.ascontiguousarray(gammas, dtype=np.float64)
+    inv_jac = np.ascontiguousarray(inv_jac, dtype=np.float64)
+    ghost = np.ascontiguousarray(ghost, dtype=np.float64)
+    dt = np.empty_like(u)
+    ext, ep = _i(extents)
+    nb, nbp = _i(neighbors)
+    _check(lib().sxo_gh_rhs_dirichlet(u.shape[0], ep, nbp, _d(u), _d(gammas), _d(inv_jac),
+                                      _d(ghost), _d(dt)))
+    return dt
+
+
 def sw_rhs(extents, neighbors, u, gamma2, inv_jac):
     u = np.ascontiguousarray(u, dtype=np.float64)
     gamma2 = np.ascontiguousarray(gamma2, dtype=np.float64)

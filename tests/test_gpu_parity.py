@@ -142,6 +142,28 @@ def test_scalar_wave_rhs_dirichlet_matches_oracle(kernel, elements, extents):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("kernel,threads,elements,extents", [
+    ("fused", 256, (2, 2, 2), (8, 8, 8)), ("fused", 512, (2, 1, 2), (8, 8, 8)),
+    ("fused", 256, (3, 2, 1), (5, 5, 5)), ("composite", 256, (2, 2, 2), (8, 8, 8)),
+    ("composite", 256, (2, 1, 2), (4, 6, 5))])
+def test_gh_rhs_dirichlet_matches_oracle(kernel, threads, elements, extents):
+    """Kerr-Schild + 1e-4 noise with the exact solution as ghost on every
+    outer face; point-varying damping parameters."""
+    brick, u, gammas, inv, ghost = wl.gh_kerr_schild_dirichlet(
+        seed=6, elements=elements, extents=extents)
+    rng = np.random.default_rng(2)
+    gammas[:, 0] = rng.uniform(0.5, 1.5, gammas[:, 0].shape)
+    gammas[:, 1] = rng.uniform(-1.5, -0.5, gammas[:, 1].shape)
+    gammas[:, 2] = rng.uniform(0.5, 1.5, gammas[:, 2].shape)
+    ref = ol.gh_rhs_dirichlet(brick.extents, brick.neighbors, u, gammas, inv, ghost)
+    domain = dg.Domain.from_brick(brick)
+    domain.set_kernel("gh", kernel)
+    domain.set_threads("gh", threads)
+    got = host(domain.gh_rhs_dirichlet(dev(u), dev(gammas), dev(inv), dev(ghost)))
+    assert rel_err(got, ref) <= TOL
+    assert np.array_equal(got, ref)
+
+
 def test_domain_with_external_faces_rejects_the_periodic_calls():
     brick, u, gamma2, inv, ghost = wl.scalar_wave_plane_wave_dirichlet(
         seed=1, elements=(2, 1, 1), extents=(4, 4, 4))
@@ -155,6 +177,9 @@ def test_domain_with_external_faces_rejects_the_periodic_calls():
     gh_g = torch.ones((2, 3, 64), dtype=torch.float64, device="cuda")
     with pytest.raises(dg.SpectreAmdError, match="external faces"):
         domain.gh_rhs(gh_u, gh_g, dinv)
+    domain.set_kernel("gh", "fused_mfma")
+    with pytest.raises(dg.SpectreAmdError, match="MFMA"):
+        domain.gh_rhs_dirichlet(gh_u, gh_g, dinv, gh_u.clone())
     # the Dirichlet call on a periodic domain is the periodic RHS
     per = wl.Brick(brick.lower, brick.upper, brick.elements, brick.extents)
     pdom = dg.Domain.from_brick(per)

@@ -1,7 +1,7 @@
-"""Pins of the oracle's external faces (ScalarWave Dirichlet analytic ghost
-condition, ScalarWave/BoundaryConditions/DirichletAnalytic.cpp:72-100 as
-applied by Evolution/DiscontinuousGalerkin/Actions/BoundaryConditionsImpl.hpp:
-427-586).
+"""Pins of the oracle's external faces (Dirichlet analytic ghost conditions,
+ScalarWave/BoundaryConditions/DirichletAnalytic.cpp:72-100 and
+GeneralizedHarmonic/BoundaryConditions/DirichletAnalytic.cpp:70-115, as applied
+by Evolution/DiscontinuousGalerkin/Actions/BoundaryConditionsImpl.hpp:427-586).
 
 * Equivalence with an internal face: cutting the face between two elements of a
   periodic brick and handing each side the other's trace as its ghost state
@@ -9,9 +9,9 @@ applied by Evolution/DiscontinuousGalerkin/Actions/BoundaryConditionsImpl.hpp:
   equal size, so the reversed interior unit normal equals the neighbour's
   normal).  A ghost read at the wrong point or component, or a wrong normal,
   fails this.
-* With the exact plane wave as the ghost state the RHS converges spectrally to
-  the exact time derivative, while the periodic closure of the same
-  non-periodic data does not.
+* With the exact solution as the ghost state (plane wave; Kerr-Schild) the RHS
+  converges spectrally to the exact time derivative, while the periodic
+  closure of the same non-periodic data does not.
 * A ghost state equal to the interior trace adds no correction.
 """
 import numpy as np
@@ -101,3 +101,83 @@ def test_bounded_neighbour_table():
     assert (nb[:, 4] == -1).all() and (nb[:, 5] == -1).all()
     keep = nb >= 0
     np.testing.assert_array_equal(nb[keep], per[keep])
+
+
+# ---- generalized harmonic ---------------------------------------------------
+# GeneralizedHarmonic/BoundaryConditions/DirichletAnalytic.cpp:70-115: ghost
+# fields, interior gamma1 and gamma2, lapse/shift/inverse spatial metric of the
+# ghost metric; the reversed interior unit normal renormalised with the ghost
+# inverse spatial metric (BoundaryConditionsImpl.hpp:524-560).
+
+
+def _gh_state(brick, noise, seed):
+    u = wl.gh_kerr_schild(brick.coordinates())
+    if noise:
+        u *= 1.0 + noise * np.random.default_rng(seed).standard_normal(u.shape)
+    gammas = np.empty((brick.num_elements, 3, brick.num_points))
+    gammas[:, 0], gammas[:, 1], gammas[:, 2] = 0.9, -1.0, 1.3
+    return u, gammas
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_gh_cut_internal_face_with_neighbour_ghost_reproduces_periodic_rhs(d):
+    """Equal to the periodic RHS up to the rounding of the exterior normal,
+    which the ghost path divides by the magnitude where the neighbour's own
+    normalisation multiplies by its reciprocal."""
+    elements = [1, 1, 1]
+    elements[d] = 2
+    extents = (4, 5, 3)
+    brick = wl.Brick((3.0, -2.0, 0.5), (6.0, 1.0, 3.5), tuple(elements), extents)
+    u, gammas = _gh_state(brick, 1e-3, 21 + d)
+    inv = brick.inverse_jacobian()
+    periodic = ol.gh_rhs(extents, brick.neighbors, u, gammas, inv)
+    nb = brick.neighbors.copy()
+    nb[0, 2 * d + 1] = -1
+    nb[1, 2 * d] = -1
+    upper, lower = _face_points(extents, d, 1), _face_points(extents, d, 0)
+    ghost = np.full_like(u, np.nan)  # only face points may be read
+    ghost[0][:, upper] = u[1][:, lower]
+    ghost[1][:, lower] = u[0][:, upper]
+    ghost[0][:, lower] = u[1][:, upper]   # periodic seam faces stay internal;
+    ghost[1][:, upper] = u[0][:, lower]   # these slots are never read
+    cut = ol.gh_rhs_dirichlet(extents, nb, u, gammas, inv, ghost)
+    assert np.isfinite(cut).all()
+    scale = np.max(np.abs(periodic))
+    np.testing.assert_allclose(cut, periodic, rtol=0, atol=1e-13 * scale)
+
+
+def test_gh_dirichlet_kerr_schild_converges_to_exact_time_derivative():
+    """Kerr-Schild data with the exact solution as ghost: the DG RHS tends to
+    the point-wise time derivative with exact derivatives (zero for g and Phi;
+    dt Pi carries the non-harmonic gauge source)."""
+    errors, periodic_errors = [], []
+    for n in (4, 6, 8, 10):
+        brick = wl.Brick((3.0, -2.0, 0.5), (7.0, 2.0, 4.5), (2, 1, 1), (n, n, n),
+                         periodic=False)
+        u, gammas = _gh_state(brick, 0.0, 0)
+        x = brick.coordinates()
+        exact = np.empty_like(u)
+        for e in range(brick.num_elements):
+            ue, due = wl.gh_kerr_schild_with_derivatives(x[e:e + 1])
+            exact[e] = ol.gh_time_derivative(ue[0], due[0], gammas[e])
+        inv = brick.inverse_jacobian()
+        dt = ol.gh_rhs_dirichlet(brick.extents, brick.neighbors, u, gammas, inv, u)
+        errors.append(np.max(np.abs(dt - exact)))
+        per = wl.periodic_neighbors(brick.elements)
+        periodic_errors.append(np.max(np.abs(ol.gh_rhs(brick.extents, per, u, gammas, inv)
+                                             - exact)))
+    # spectral: 0.11, 0.025, 5.3e-3, 9.3e-4 for n = 4, 6, 8, 10
+    assert errors[-1] < 2e-3
+    assert all(b < a * 0.3 for a, b in zip(errors, errors[1:]))
+    # Kerr-Schild is not periodic on the brick: a periodic closure stays O(1) off
+    assert min(periodic_errors) > 100 * errors[-1]
+
+
+def test_gh_ghost_equal_to_interior_trace_adds_no_correction():
+    brick = wl.Brick((3.0, -2.0, 0.5), (5.0, 0.0, 2.5), (1, 1, 1), (5, 4, 6), periodic=False)
+    u, gammas = _gh_state(brick, 1e-2, 4)
+    inv = brick.inverse_jacobian()
+    dt = ol.gh_rhs_dirichlet(brick.extents, brick.neighbors, u, gammas, inv, u)
+    du = ol.partial_derivatives(u[0], brick.extents, inv[0])
+    vol = ol.gh_time_derivative(u[0], du, gammas[0])
+    np.testing.assert_allclose(dt[0], vol, rtol=0, atol=1e-12 * np.max(np.abs(vol)))
