@@ -137,9 +137,9 @@ int sxd_lift_flux(int64_t num_points, int nc, double* bc,
  * ApplyBoundaryCorrections.hpp:722-1050) for a single device.
  * neighbors: host array [6 E], entry 6 e + 2 d + side (side 0 = lower,
  * 1 = upper) is the element across that face, or -1 for an external face
- * (closed by a ghost state, see sxd_scalar_wave_rhs_dirichlet and
- * sxd_gh_rhs_dirichlet; every other call on such a domain returns
- * SXD_ERR_UNSUPPORTED).  The table is copied to
+ * (closed by a ghost state, see sxd_scalar_wave_rhs_dirichlet,
+ * sxd_scalar_wave_rhs_spherical_radiation and sxd_gh_rhs_dirichlet; every
+ * other call on such a domain returns SXD_ERR_UNSUPPORTED).  The table is copied to
  * the device; this call allocates device memory. */
 typedef struct sxd_domain sxd_domain;
 int sxd_domain_create(sxd_domain** domain, int num_elements,
@@ -184,6 +184,25 @@ int sxd_scalar_wave_rhs(const sxd_domain* domain, const double* u,
 int sxd_scalar_wave_rhs_dirichlet(const sxd_domain* domain, const double* u,
                                   const double* gamma2, const double* inv_jac,
                                   const double* ghost, double* dt, void* stream);
+
+/* sxd_scalar_wave_rhs on a domain with external faces closed by the
+ * SphericalRadiation ghost condition
+ * (ScalarWave/BoundaryConditions/SphericalRadiation.cpp:62-100 dg_ghost): the
+ * exterior state keeps the interior psi and phi_i and sets
+ * pi = n^i phi_i (outward interior unit normal), plus psi / |x| for the
+ * Bayliss-Turkel type; interior gamma2 and reversed unit normal as in
+ * sxd_scalar_wave_rhs_dirichlet.  coords [E][3][Np]: inertial coordinates
+ * (Bayliss-Turkel only; may be null for Sommerfeld). */
+enum {
+  SXD_SPHERICAL_RADIATION_SOMMERFELD = 1,
+  SXD_SPHERICAL_RADIATION_BAYLISS_TURKEL = 2
+};
+int sxd_scalar_wave_rhs_spherical_radiation(const sxd_domain* domain,
+                                            const double* u,
+                                            const double* gamma2,
+                                            const double* inv_jac,
+                                            const double* coords, int type,
+                                            double* dt, void* stream);
 
 /* Full generalized-harmonic DG time derivative (harmonic gauge).
  * u, dt [E][50][Np]; gammas [E][3][Np]; inv_jac [E][9][Np]. */

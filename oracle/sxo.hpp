@@ -742,10 +742,20 @@ void sw_rhs_volume(int e_begin, int e_end, const int ext[3], const T* u,
 // ScalarWave/BoundaryConditions/DirichletAnalytic.cpp:72-100, applied by
 // Evolution/DiscontinuousGalerkin/Actions/BoundaryConditionsImpl.hpp:427-586):
 // ghost fields, the interior gamma2, and the negated interior unit normal.
+// bc_kind selects how the exterior state is formed:
+//   kSwGhostArray     ghost holds psi, pi, phi_i ([E][5][Np]);
+//   kSwSommerfeld     SphericalRadiation::dg_ghost, Sommerfeld type
+//                     (ScalarWave/BoundaryConditions/SphericalRadiation.cpp:
+//                     62-100): psi and phi of the interior trace, pi = n^i phi_i
+//                     with the outward interior unit normal;
+//   kSwBaylissTurkel  the same plus psi / |x|, ghost holding the inertial
+//                     coordinates ([E][3][Np]).
+constexpr int kSwGhostArray = 0, kSwSommerfeld = 1, kSwBaylissTurkel = 2;
+
 template <class T>
 void sw_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
                   const T* u, const T* gamma2, const T* inv_j, T* dt,
-                  const T* ghost = nullptr) {
+                  const T* ghost = nullptr, int bc_kind = kSwGhostArray) {
   const size_t np = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
   // boundary corrections
   for (int e = e_begin; e < e_end; ++e) {
@@ -769,7 +779,22 @@ void sw_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
           if (en < 0) {
             // ghost exterior state at the boundary point
             T ug[5], ng[3];
-            for (int c = 0; c < 5; ++c) ug[c] = ghost[(e * 5 + c) * np + p];
+            if (bc_kind == kSwGhostArray) {
+              for (int c = 0; c < 5; ++c) ug[c] = ghost[(e * 5 + c) * np + p];
+            } else {
+              ug[0] = up[0];
+              T pe = n[0] * up[2];
+              for (int i = 1; i < 3; ++i) pe = fma_(n[i], up[2 + i], pe);
+              if (bc_kind == kSwBaylissTurkel) {
+                const T* x = ghost + e * 3 * np + p;
+                T r = x[0] * x[0];
+                for (int i = 1; i < 3; ++i) r = fma_(x[i * np], x[i * np], r);
+                r = sqrt_(r);
+                pe = pe + up[0] / r;
+              }
+              ug[1] = pe;
+              for (int i = 0; i < 3; ++i) ug[2 + i] = up[2 + i];
+            }
             for (int i = 0; i < 3; ++i) ng[i] = n[i] * -1.0;
             sw_package_point(pk_ex, ug, gamma2[e * np + p], ng);
           } else {
@@ -798,9 +823,10 @@ void sw_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
 
 template <class T>
 void sw_rhs(int num_elements, const int ext[3], const int* nbr, const T* u,
-            const T* gamma2, const T* inv_j, T* dt, const T* ghost = nullptr) {
+            const T* gamma2, const T* inv_j, T* dt, const T* ghost = nullptr,
+            int bc = kSwGhostArray) {
   sw_rhs_volume(0, num_elements, ext, u, gamma2, inv_j, dt);
-  sw_rhs_faces(0, num_elements, ext, nbr, u, gamma2, inv_j, dt, ghost);
+  sw_rhs_faces(0, num_elements, ext, nbr, u, gamma2, inv_j, dt, ghost, bc);
 }
 
 // Full GH DG RHS on a uniform-extent periodic brick.

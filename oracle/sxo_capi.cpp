@@ -348,6 +348,18 @@ int sxo_sw_rhs_dirichlet(int num_elements, const int* ext, const int* nbr, const
 
 // Generalized-harmonic RHS with external faces (neighbour < 0) closed by the
 // ghost state `ghost` [E][50][Np] (Dirichlet analytic boundary condition).
+// ScalarWave RHS with external faces closed by the SphericalRadiation ghost
+// condition: kind 1 Sommerfeld, 2 Bayliss-Turkel (coords [E][3][Np]).
+int sxo_sw_rhs_spherical_radiation(int num_elements, const int* ext, const int* nbr,
+                                   const double* u, const double* gamma2, const double* inv_j,
+                                   const double* coords, int kind, double* dt) {
+  if (bad_extents(ext) || num_elements < 1) return -1;
+  if (kind != sxo::kSwSommerfeld && kind != sxo::kSwBaylissTurkel) return -1;
+  if (kind == sxo::kSwBaylissTurkel && coords == nullptr) return -1;
+  sxo::sw_rhs(num_elements, ext, nbr, u, gamma2, inv_j, dt, coords, kind);
+  return 0;
+}
+
 int sxo_gh_rhs_dirichlet(int num_elements, const int* ext, const int* nbr, const double* u,
                          const double* gammas, const double* inv_j, const double* ghost,
                          double* dt) {

@@ -555,7 +555,7 @@ int run_mixed(sxd_domain* d, bool gh, int kernel, const double* u, const double*
 
 int run_sw(sxd_domain* d, int kernel, const double* u, const double* gamma2,
            const double* inv_jac, double* dt, hipStream_t stream,
-           const double* ghost = nullptr) {
+           const double* ghost = nullptr, int bc_kind = kSwBcGhostArray) {
   if (d->mixed) return run_mixed(d, false, kernel, u, gamma2, inv_jac, dt, stream);
   double* scratch = nullptr;
   if (kernel == SXD_KERNEL_COMPOSITE) {
@@ -565,7 +565,7 @@ int run_sw(sxd_domain* d, int kernel, const double* u, const double* gamma2,
     scratch = d->d_scratch_du;
   }
   return launch_sw_rhs(d->num_elements, d->ext, d->d_nbr, u, gamma2, inv_jac, dt,
-                       kernel == SXD_KERNEL_FUSED, scratch, ghost, stream);
+                       kernel == SXD_KERNEL_FUSED, scratch, ghost, bc_kind, stream);
 }
 
 int run_gh(sxd_domain* d, int kernel, const double* u, const double* gammas,
@@ -617,6 +617,30 @@ int sxd_scalar_wave_rhs_dirichlet(const sxd_domain* domain, const double* u,
   sxd_domain* d = const_cast<sxd_domain*>(domain);
   return finish(run_sw(d, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,
                        static_cast<hipStream_t>(stream), ghost),
+                __func__);
+}
+
+int sxd_scalar_wave_rhs_spherical_radiation(const sxd_domain* domain, const double* u,
+                                            const double* gamma2, const double* inv_jac,
+                                            const double* coords, int type, double* dt,
+                                            void* stream) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(gamma2);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  SXD_REQUIRE_DEVICE(dt);
+  if (type != SXD_SPHERICAL_RADIATION_SOMMERFELD &&
+      type != SXD_SPHERICAL_RADIATION_BAYLISS_TURKEL)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "unknown spherical radiation type %d", type);
+  if (type == SXD_SPHERICAL_RADIATION_BAYLISS_TURKEL) SXD_REQUIRE_DEVICE(coords);
+  if (domain->mixed)
+    return fail(SXD_ERR_UNSUPPORTED, "external faces need a uniform-extent domain");
+  sxd_domain* d = const_cast<sxd_domain*>(domain);
+  const int kind = type == SXD_SPHERICAL_RADIATION_SOMMERFELD ? kSwBcSommerfeld
+                                                              : kSwBcBaylissTurkel;
+  return finish(run_sw(d, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,
+                       static_cast<hipStream_t>(stream),
+                       kind == kSwBcBaylissTurkel ? coords : nullptr, kind),
                 __func__);
 }
 

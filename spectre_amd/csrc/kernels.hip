@@ -420,12 +420,42 @@ gh_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
   }
 }
 
+// Exterior state of an external ScalarWave face from the interior trace ui and
+// the outward interior unit normal n (SphericalRadiation::dg_ghost,
+// ScalarWave/BoundaryConditions/SphericalRadiation.cpp:62-100): psi and phi of
+// the interior, pi = n^i phi_i, plus psi / |x| for Bayliss-Turkel (x: the
+// point's inertial coordinates, blocks of stride np).  kSwBcGhostArray reads
+// the caller's ghost state instead (DirichletAnalytic).
+__device__ __forceinline__ void sw_ghost_state(double ue[5], int bc_kind, const double ui[5],
+                                               const double n[3], const double* ghost_at,
+                                               int64_t stride) {
+  if (bc_kind == kSwBcGhostArray) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) ue[c] = ghost_at[c * stride];
+    return;
+  }
+  ue[0] = ui[0];
+  double pe = n[0] * ui[2];
+  pe = fma(n[1], ui[3], pe);
+  pe = fma(n[2], ui[4], pe);
+  if (bc_kind == kSwBcBaylissTurkel) {
+    double r = ghost_at[0] * ghost_at[0];
+    r = fma(ghost_at[stride], ghost_at[stride], r);
+    r = fma(ghost_at[2 * stride], ghost_at[2 * stride], r);
+    r = sqrt(r);
+    pe = pe + ui[0] / r;
+  }
+  ue[1] = pe;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) ue[2 + i] = ui[2 + i];
+}
+
 // Composite SW face pass (any extents), same scheme as the GH one.
 __global__ void __launch_bounds__(256)
 sw_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
                     const int32_t* __restrict__ nbr, const double* __restrict__ u,
                     const double* __restrict__ gamma2, const double* __restrict__ inv_jac,
-                    double* __restrict__ dt, const double* __restrict__ ghost) {
+                    double* __restrict__ dt, const double* __restrict__ ghost, int bc_kind) {
   const int ext[3] = {n0, n1, n2};
   const int na = d == 0 ? n1 : n0;
   const int nb = d == 2 ? n1 : n2;
@@ -457,8 +487,8 @@ sw_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
   sw_package(pki, ui, g2, ni);
   if (en < 0) {
     // external face: ghost state at the own point, own gamma2, reversed unit normal
-#pragma unroll
-    for (int c = 0; c < 5; ++c) ue[c] = ghost[(e * 5 + c) * np + p];
+    const int nblk = bc_kind == kSwBcGhostArray ? 5 : 3;
+    sw_ghost_state(ue, bc_kind, ui, ni, ghost ? ghost + e * nblk * np + p : nullptr, np);
 #pragma unroll
     for (int i = 0; i < 3; ++i) ne[i] = ni[i] * -1.0;
     sw_package(pke, ue, g2, ne);
@@ -484,7 +514,7 @@ sw_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
 
 int launch_face_passes(bool gh, int num_elements, const int32_t ext[3], const int32_t* nbr,
                        const double* u, const double* aux, const double* inv_jac,
-                       double* dt, const double* ghost, hipStream_t stream) {
+                       double* dt, const double* ghost, int bc_kind, hipStream_t stream) {
   for (int d = 0; d < 3; ++d) {
     const int na = d == 0 ? ext[1] : ext[0];
     const int nb = d == 2 ? ext[1] : ext[2];
@@ -496,7 +526,8 @@ int launch_face_passes(bool gh, int num_elements, const int32_t ext[3], const in
                          ext[1], ext[2], num_elements, nbr, u, aux, inv_jac, dt, ghost);
     } else {
       hipLaunchKernelGGL(sw_face_pass_kernel, dim3(blocks), dim3(bs), 0, stream, d, ext[0],
-                         ext[1], ext[2], num_elements, nbr, u, aux, inv_jac, dt, ghost);
+                         ext[1], ext[2], num_elements, nbr, u, aux, inv_jac, dt, ghost,
+                         bc_kind);
     }
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   }
@@ -514,7 +545,7 @@ __global__ void __launch_bounds__((N * N * N + 63) / 64 * 64)
 sw_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ u,
                     const double* __restrict__ gamma2, const double* __restrict__ inv_jac,
                     double* __restrict__ dt, const double* __restrict__ dtab,
-                    const double* __restrict__ ghost) {
+                    const double* __restrict__ ghost, int bc_kind) {
   constexpr int NP = N * N * N;
   __shared__ double s_u[5 * NP];
   const int64_t e = xcd_local_element(blockIdx.x, gridDim.x);
@@ -577,8 +608,8 @@ sw_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
     double uo[5], no[3], mo, pko[13], bc[5];
     if (en < 0) {
       // external face: ghost state at the own point, own gamma2, reversed unit normal
-#pragma unroll
-      for (int c = 0; c < 5; ++c) uo[c] = ghost[(e * 5 + c) * NP + t];
+      const int nblk = bc_kind == kSwBcGhostArray ? 5 : 3;
+      sw_ghost_state(uo, bc_kind, up, ni, ghost ? ghost + e * nblk * NP + t : nullptr, NP);
 #pragma unroll
       for (int a = 0; a < 3; ++a) no[a] = ni[a] * -1.0;
       sw_package(pko, uo, g2, no);
@@ -625,10 +656,11 @@ sw_volume_batched_kernel(int np, int64_t total, const double* __restrict__ u,
 template <int N>
 static int launch_sw_fused_n(int num_elements, const int32_t* nbr, const double* u,
                              const double* gamma2, const double* inv_jac, double* dt,
-                             const double* dtab, const double* ghost, hipStream_t stream) {
+                             const double* dtab, const double* ghost, int bc_kind,
+                             hipStream_t stream) {
   constexpr int threads = (N * N * N + 63) / 64 * 64;
   hipLaunchKernelGGL(sw_rhs_fused_kernel<N>, dim3(num_elements), dim3(threads), 0, stream,
-                     nbr, u, gamma2, inv_jac, dt, dtab, ghost);
+                     nbr, u, gamma2, inv_jac, dt, dtab, ghost, bc_kind);
   return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
 }
 
@@ -638,21 +670,22 @@ bool sw_fused_supported(const int32_t ext[3]) {
 
 int launch_sw_rhs(int num_elements, const int32_t ext[3], const int32_t* nbr,
                   const double* u, const double* gamma2, const double* inv_jac, double* dt,
-                  bool fused, double* scratch_du, const double* ghost, hipStream_t stream) {
+                  bool fused, double* scratch_du, const double* ghost, int bc_kind,
+                  hipStream_t stream) {
   const double* dtab = device_diff_matrices();
   if (dtab == nullptr) return SXD_ERR_DEVICE;
   if (fused) {
     if (!sw_fused_supported(ext)) return SXD_ERR_UNSUPPORTED;
     switch (ext[0]) {
-      case 2: return launch_sw_fused_n<2>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
-      case 3: return launch_sw_fused_n<3>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
-      case 4: return launch_sw_fused_n<4>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
-      case 5: return launch_sw_fused_n<5>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
-      case 6: return launch_sw_fused_n<6>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
-      case 7: return launch_sw_fused_n<7>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
-      case 8: return launch_sw_fused_n<8>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
-      case 9: return launch_sw_fused_n<9>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
-      case 10: return launch_sw_fused_n<10>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, stream);
+      case 2: return launch_sw_fused_n<2>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, bc_kind, stream);
+      case 3: return launch_sw_fused_n<3>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, bc_kind, stream);
+      case 4: return launch_sw_fused_n<4>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, bc_kind, stream);
+      case 5: return launch_sw_fused_n<5>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, bc_kind, stream);
+      case 6: return launch_sw_fused_n<6>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, bc_kind, stream);
+      case 7: return launch_sw_fused_n<7>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, bc_kind, stream);
+      case 8: return launch_sw_fused_n<8>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, bc_kind, stream);
+      case 9: return launch_sw_fused_n<9>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, bc_kind, stream);
+      case 10: return launch_sw_fused_n<10>(num_elements, nbr, u, gamma2, inv_jac, dt, dtab, ghost, bc_kind, stream);
       default: return SXD_ERR_UNSUPPORTED;
     }
   }
@@ -668,7 +701,7 @@ int launch_sw_rhs(int num_elements, const int32_t ext[3], const int32_t* nbr,
                      dt);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   return launch_face_passes(false, num_elements, ext, nbr, u, gamma2, inv_jac, dt, ghost,
-                            stream);
+                            bc_kind, stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -735,7 +768,7 @@ int launch_gh_rhs_composite(int num_elements, const int32_t ext[3], const int32_
                      scratch_temps);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   return launch_face_passes(true, num_elements, ext, nbr, u, scratch_temps, inv_jac, dt,
-                            ghost, stream);
+                            ghost, kSwBcGhostArray, stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -31,13 +31,19 @@ int launch_gh_upwind(int64_t n, const double* u_int, const double* t_int,
 int launch_lift_flux(int64_t n, int nc, double* bc, int n_perp, const double* mag,
                      hipStream_t stream);
 bool sw_fused_supported(const int32_t ext[3]);
+// Exterior state of external ScalarWave faces: the caller's ghost array
+// (DirichletAnalytic), or SphericalRadiation of the interior trace.
+constexpr int kSwBcGhostArray = 0, kSwBcSommerfeld = 1, kSwBcBaylissTurkel = 2;
 // fused: one launch per RHS (requires sw_fused_supported); otherwise the
 // composite path, which needs scratch_du [E][15][Np].  Faces whose neighbour
-// index is negative take their exterior state from ghost ([E][5][Np], read at
-// the face's own points); ghost may be null when no neighbour is negative.
+// index is negative take their exterior state by bc_kind: kSwBcGhostArray
+// ghost = [E][5][Np] state read at the face's own points; kSwBcSommerfeld
+// ghost unused; kSwBcBaylissTurkel ghost = inertial coordinates [E][3][Np].
+// ghost may be null when no neighbour is negative.
 int launch_sw_rhs(int num_elements, const int32_t ext[3], const int32_t* nbr,
                   const double* u, const double* gamma2, const double* inv_jac, double* dt,
-                  bool fused, double* scratch_du, const double* ghost, hipStream_t stream);
+                  bool fused, double* scratch_du, const double* ghost, int bc_kind,
+                  hipStream_t stream);
 // ghost: as for launch_sw_rhs, with [E][50][Np].
 int launch_gh_rhs_composite(int num_elements, const int32_t ext[3], const int32_t* nbr,
                             const double* u, const double* gammas, const double* inv_jac,

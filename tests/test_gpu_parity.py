@@ -164,6 +164,29 @@ def test_gh_rhs_dirichlet_matches_oracle(kernel, threads, elements, extents):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("kind", ["sommerfeld", "bayliss_turkel"])
+@pytest.mark.parametrize("kernel,elements,extents", [
+    ("fused", (3, 2, 2), (5, 5, 5)), ("composite", (3, 2, 2), (5, 5, 5)),
+    ("composite", (2, 2, 1), (4, 6, 5))])
+def test_scalar_wave_rhs_spherical_radiation_matches_oracle(kind, kernel, elements, extents):
+    """Outer faces closed by SphericalRadiation of the interior trace: noisy
+    plane wave on a brick off the origin, point-varying gamma2."""
+    brick, u, _, inv, _ = wl.scalar_wave_plane_wave_dirichlet(
+        seed=5, elements=elements, extents=extents, t=0.2, noise=1e-2,
+        lower=(1.0, -1.0, 0.5), upper=(3.0, 1.0, 2.5))
+    gamma2 = np.random.default_rng(3).uniform(0, 2, (brick.num_elements, brick.num_points))
+    coords = brick.coordinates()
+    ref = ol.sw_rhs_spherical_radiation(brick.extents, brick.neighbors, u, gamma2, inv, coords,
+                                        1 if kind == "sommerfeld" else 2)
+    domain = dg.Domain.from_brick(brick)
+    domain.set_kernel("scalar_wave", kernel)
+    got = host(domain.scalar_wave_rhs_spherical_radiation(
+        dev(u), dev(gamma2), dev(inv), kind,
+        dev(coords) if kind == "bayliss_turkel" else None))
+    assert rel_err(got, ref) <= TOL
+    assert np.array_equal(got, ref)
+
+
 def test_domain_with_external_faces_rejects_the_periodic_calls():
     brick, u, gamma2, inv, ghost = wl.scalar_wave_plane_wave_dirichlet(
         seed=1, elements=(2, 1, 1), extents=(4, 4, 4))

@@ -181,3 +181,76 @@ def test_gh_ghost_equal_to_interior_trace_adds_no_correction():
     du = ol.partial_derivatives(u[0], brick.extents, inv[0])
     vol = ol.gh_time_derivative(u[0], du, gammas[0])
     np.testing.assert_allclose(dt[0], vol, rtol=0, atol=1e-12 * np.max(np.abs(vol)))
+
+
+# ---- ScalarWave SphericalRadiation (ScalarWave/BoundaryConditions/
+# SphericalRadiation.cpp:62-100): ghost psi, phi = interior trace, ghost
+# pi = n^i phi_i (Sommerfeld) plus psi / |x| (Bayliss-Turkel).
+SOMMERFELD, BAYLISS_TURKEL = 1, 2
+
+
+def _plane_wave_along_x(brick, direction):
+    """psi = sin(x - direction t) at t = 0: outgoing through the upper x faces
+    for direction = +1, incoming for -1."""
+    x = brick.coordinates()[:, 0]
+    u = np.zeros((brick.num_elements, 5, brick.num_points))
+    u[:, 0] = np.sin(x)
+    u[:, 1] = direction * np.cos(x)    # pi = -dt psi
+    u[:, 2] = np.cos(x)                # phi_x
+    return u
+
+
+def _cut_upper_x(brick):
+    nb = brick.neighbors.copy()
+    ex = brick.elements[0]
+    e = np.arange(brick.num_elements)
+    nb[e % ex == ex - 1, 1] = -1
+    return nb
+
+
+def test_sommerfeld_lets_an_outgoing_plane_wave_leave_without_correction():
+    """A wave leaving along the face normal has pi = n.phi, so the ghost state
+    equals the interior trace and the RHS equals the periodic one (the data
+    are periodic: wavelength = brick length); an incoming wave is reflected
+    (an O(1) correction)."""
+    brick = wl.Brick((0.0, 0.0, 0.0), (2 * np.pi,) * 3, (2, 1, 1), (7, 4, 4))
+    nb = _cut_upper_x(brick)
+    gamma2 = np.full((brick.num_elements, brick.num_points), 0.4)
+    inv = brick.inverse_jacobian()
+    coords = brick.coordinates()
+    out = _plane_wave_along_x(brick, +1)
+    per = ol.sw_rhs(brick.extents, brick.neighbors, out, gamma2, inv)
+    som = ol.sw_rhs_spherical_radiation(brick.extents, nb, out, gamma2, inv, coords, SOMMERFELD)
+    np.testing.assert_allclose(som, per, rtol=0, atol=1e-12)
+    inc = _plane_wave_along_x(brick, -1)
+    per = ol.sw_rhs(brick.extents, brick.neighbors, inc, gamma2, inv)
+    som = ol.sw_rhs_spherical_radiation(brick.extents, nb, inc, gamma2, inv, coords, SOMMERFELD)
+    assert np.max(np.abs(som - per)) > 0.5
+
+
+def test_bayliss_turkel_is_exact_for_an_outgoing_spherical_wave_on_the_axis():
+    """psi = sin(r - t) / r: pi = cos(r - t) / r, phi = (cos/r - sin/r^2) x/r.
+    Where the face normal is radial (the centre of the upper x face, on the x
+    axis) pi = n.phi + psi / r, so Bayliss-Turkel adds no correction there;
+    Sommerfeld (no psi / r) does."""
+    brick = wl.Brick((3.0, -1.0, -1.0), (5.0, 1.0, 1.0), (1, 1, 1), (5, 5, 5), periodic=False)
+    x = brick.coordinates()
+    r = np.sqrt(np.sum(x * x, axis=1))
+    u = np.zeros((1, 5, brick.num_points))
+    u[:, 0] = np.sin(r) / r
+    u[:, 1] = np.cos(r) / r
+    for i in range(3):
+        u[:, 2 + i] = (np.cos(r) / r - np.sin(r) / r ** 2) * x[:, i] / r
+    gamma2 = np.full((1, brick.num_points), 0.4)
+    inv = brick.inverse_jacobian()
+    no_corr = ol.sw_rhs_dirichlet(brick.extents, brick.neighbors, u, gamma2, inv, u)
+    bt = ol.sw_rhs_spherical_radiation(brick.extents, brick.neighbors, u, gamma2, inv, x,
+                                       BAYLISS_TURKEL)
+    som = ol.sw_rhs_spherical_radiation(brick.extents, brick.neighbors, u, gamma2, inv, x,
+                                        SOMMERFELD)
+    centre = 4 + 5 * (2 + 5 * 2)   # i = N-1, j = k = 2: (5, 0, 0)
+    assert abs(x[0, 1, centre]) < 1e-15 and abs(x[0, 2, centre]) < 1e-15
+    np.testing.assert_allclose(bt[0, :, centre], no_corr[0, :, centre], rtol=0, atol=1e-13)
+    assert np.max(np.abs(som[0, :, centre] - no_corr[0, :, centre])) > 1e-2
+    # off the axis the planar face is not radial and Bayliss-Turkel corrects
+    assert np.max(np.abs(bt - no_corr)) > 1e-3
