@@ -757,11 +757,16 @@ void sw_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
                   const T* u, const T* gamma2, const T* inv_j, T* dt,
                   const T* ghost = nullptr, int bc_kind = kSwGhostArray) {
   const size_t np = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
-  // boundary corrections
+  // boundary corrections: external faces first (the reference applies them in
+  // ComputeTimeDerivative, ComputeTimeDerivative.hpp:633-637, before the
+  // internal mortars' corrections of ApplyBoundaryCorrections), each group in
+  // direction order
   for (int e = e_begin; e < e_end; ++e) {
-    for (int dir = 0; dir < 6; ++dir) {
-      const int d = dir / 2, side = dir % 2;
+    for (int step = 0; step < 12; ++step) {
+      const int dir = step % 6;
       const int en = nbr[6 * e + dir];
+      if ((en < 0) != (step < 6)) continue;
+      const int d = dir / 2, side = dir % 2;
       const Face fi = make_face(ext, d, side);
       const Face fe = make_face(ext, d, 1 - side);
       for (size_t f = 0; f < fi.pts.size(); ++f) {
@@ -884,10 +889,14 @@ void gh_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
                   const T* u, const T* inv_j, T* dt, const GhFaceTemps<T>* temps,
                   const T* ghost = nullptr) {
   const size_t np = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
+  // external faces first, then internal ones, each in direction order (as in
+  // sw_rhs_faces)
   for (int e = e_begin; e < e_end; ++e) {
-    for (int dir = 0; dir < 6; ++dir) {
-      const int d = dir / 2, side = dir % 2;
+    for (int step = 0; step < 12; ++step) {
+      const int dir = step % 6;
       const int en = nbr[6 * e + dir];
+      if ((en < 0) != (step < 6)) continue;
+      const int d = dir / 2, side = dir % 2;
       const Face fi = make_face(ext, d, side);
       const Face fe = make_face(ext, d, 1 - side);
       for (size_t f = 0; f < fi.pts.size(); ++f) {

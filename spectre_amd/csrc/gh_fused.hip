@@ -451,13 +451,23 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
       for (int c = 0; c < 5; ++c) corr[slot * L::kCorrStride + c] = bc[c] * fac;
     }
   };
-  // add the corrections of the faces point q lies on (xi, eta, zeta order)
+  // add the corrections of the faces point q lies on (xi, eta, zeta order;
+  // GHOST: external faces first, then internal ones, each in that order)
+  int ext_mask = 0;
+  if constexpr (GHOST) {
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) ext_mask |= (nbr[6 * e + dir] < 0 ? 1 : 0) << dir;
+  }
   auto add_corrections = [&](int q, double* out) {
     const int idx[3] = {pi_[q], pj[q], pk[q]};
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
+    for (int step = 0; step < (GHOST ? 6 : 3); ++step) {
+      const int d = step % 3;
       if (idx[d] != 0 && idx[d] != N - 1) continue;
       const int side = idx[d] == 0 ? 0 : 1;
+      if constexpr (GHOST) {
+        if ((((ext_mask >> (2 * d + side)) & 1) != 0) != (step < 3)) continue;
+      }
       const int a = d == 0 ? idx[1] : idx[0];
       const int b = d == 2 ? idx[1] : idx[2];
       const double* cr = corr + ((2 * d + side) * NF + a + N * b) * L::kCorrStride;
@@ -677,9 +687,13 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
         if (!act[q]) continue;
         const int idx[3] = {pi_[q], pj[q], pk[q]};
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
+        for (int step = 0; step < (GHOST ? 6 : 3); ++step) {
+          const int d = step % 3;
           if (idx[d] != 0 && idx[d] != N - 1) continue;
           const int side = idx[d] == 0 ? 0 : 1;
+          if constexpr (GHOST) {
+            if ((((ext_mask >> (2 * d + side)) & 1) != 0) != (step < 3)) continue;
+          }
           const int a = d == 0 ? idx[1] : idx[0];
           const int b = d == 2 ? idx[1] : idx[2];
           const double2* cr = reinterpret_cast<const double2*>(

@@ -320,16 +320,21 @@ __device__ __forceinline__ int opposite_point(int i, int j, int k, int d, int n0
   return i + n0 * (j + n1 * k);
 }
 
+// Which faces a face-pass launch handles: all of them (no external faces), or
+// the external or internal ones of a domain with external faces.
+constexpr int kFacesAll = 0, kFacesExternal = 1, kFacesInternal = 2;
+
 // ---------------------------------------------------------------------------
 // Composite GH face pass for one dimension d: every (element, side, face
 // point) adds its lifted UpwindPenalty correction into dt.  Run for d = 0, 1, 2
 // in order, so a point on several faces receives the corrections in the
-// order lower/upper xi, eta, zeta.
+// order lower/upper xi, eta, zeta; with external faces the external sweep
+// runs first, then the internal one.
 __global__ void __launch_bounds__(128)
 gh_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
                     const int32_t* __restrict__ nbr, const double* __restrict__ u,
                     const double* __restrict__ temps, const double* __restrict__ inv_jac,
-                    double* __restrict__ dt, const double* __restrict__ ghost) {
+                    double* __restrict__ dt, const double* __restrict__ ghost, int phase) {
   const int ext[3] = {n0, n1, n2};
   const int na = d == 0 ? n1 : n0;
   const int nb = d == 2 ? n1 : n2;
@@ -347,6 +352,7 @@ gh_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
   const int np = n0 * n1 * n2;
   const int p = idx[0] + n0 * (idx[1] + n1 * idx[2]);
   const int64_t en = nbr[6 * e + 2 * d + side];
+  if ((phase == kFacesExternal && en >= 0) || (phase == kFacesInternal && en < 0)) return;
   const double* tp = temps + e * 12 * np;
   GhFaceSide fi, fe;
   double mag_i, mag_e;
@@ -455,7 +461,8 @@ __global__ void __launch_bounds__(256)
 sw_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
                     const int32_t* __restrict__ nbr, const double* __restrict__ u,
                     const double* __restrict__ gamma2, const double* __restrict__ inv_jac,
-                    double* __restrict__ dt, const double* __restrict__ ghost, int bc_kind) {
+                    double* __restrict__ dt, const double* __restrict__ ghost, int bc_kind,
+                    int phase) {
   const int ext[3] = {n0, n1, n2};
   const int na = d == 0 ? n1 : n0;
   const int nb = d == 2 ? n1 : n2;
@@ -473,6 +480,7 @@ sw_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
   const int np = n0 * n1 * n2;
   const int p = idx[0] + n0 * (idx[1] + n1 * idx[2]);
   const int64_t en = nbr[6 * e + 2 * d + side];
+  if ((phase == kFacesExternal && en >= 0) || (phase == kFacesInternal && en < 0)) return;
   const int q = opposite_point(idx[0], idx[1], idx[2], d, n0, n1, n2);
   double ui[5], ue[5], ni[3], ne[3], mi, me, pki[13], pke[13], bc[5];
 #pragma unroll
@@ -515,21 +523,28 @@ sw_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
 int launch_face_passes(bool gh, int num_elements, const int32_t ext[3], const int32_t* nbr,
                        const double* u, const double* aux, const double* inv_jac,
                        double* dt, const double* ghost, int bc_kind, hipStream_t stream) {
-  for (int d = 0; d < 3; ++d) {
-    const int na = d == 0 ? ext[1] : ext[0];
-    const int nb = d == 2 ? ext[1] : ext[2];
-    const int64_t total = static_cast<int64_t>(num_elements) * 2 * na * nb;
-    const int bs = gh ? 128 : 256;
-    const unsigned blocks = static_cast<unsigned>((total + bs - 1) / bs);
-    if (gh) {
-      hipLaunchKernelGGL(gh_face_pass_kernel, dim3(blocks), dim3(bs), 0, stream, d, ext[0],
-                         ext[1], ext[2], num_elements, nbr, u, aux, inv_jac, dt, ghost);
-    } else {
-      hipLaunchKernelGGL(sw_face_pass_kernel, dim3(blocks), dim3(bs), 0, stream, d, ext[0],
-                         ext[1], ext[2], num_elements, nbr, u, aux, inv_jac, dt, ghost,
-                         bc_kind);
+  // With external faces: one sweep over the external faces, then one over the
+  // internal ones (the order in which the corrections reach dt).
+  const bool external = ghost != nullptr || bc_kind != kSwBcGhostArray;
+  const int phases[2] = {external ? kFacesExternal : kFacesAll, kFacesInternal};
+  for (int ph = 0; ph < (external ? 2 : 1); ++ph) {
+    for (int d = 0; d < 3; ++d) {
+      const int na = d == 0 ? ext[1] : ext[0];
+      const int nb = d == 2 ? ext[1] : ext[2];
+      const int64_t total = static_cast<int64_t>(num_elements) * 2 * na * nb;
+      const int bs = gh ? 128 : 256;
+      const unsigned blocks = static_cast<unsigned>((total + bs - 1) / bs);
+      if (gh) {
+        hipLaunchKernelGGL(gh_face_pass_kernel, dim3(blocks), dim3(bs), 0, stream, d, ext[0],
+                           ext[1], ext[2], num_elements, nbr, u, aux, inv_jac, dt, ghost,
+                           phases[ph]);
+      } else {
+        hipLaunchKernelGGL(sw_face_pass_kernel, dim3(blocks), dim3(bs), 0, stream, d, ext[0],
+                           ext[1], ext[2], num_elements, nbr, u, aux, inv_jac, dt, ghost,
+                           bc_kind, phases[ph]);
+      }
+      if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
     }
-    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   }
   return SXD_OK;
 }
@@ -540,7 +555,7 @@ int launch_face_passes(bool gh, int num_elements, const int32_t ext[3], const in
 // its 15 derivatives from the LDS pencils, the volume terms, then the lifted
 // UpwindPenalty corrections of the faces it lies on (xi, eta, zeta order),
 // reading the neighbour's trace directly from HBM, and writes dt once.
-template <int N>
+template <int N, bool GHOST>
 __global__ void __launch_bounds__((N * N * N + 63) / 64 * 64)
 sw_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ u,
                     const double* __restrict__ gamma2, const double* __restrict__ inv_jac,
@@ -595,18 +610,24 @@ sw_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
   double dtp[5];
   sw_time_derivative(dtp, up, dup, g2);
   const int idx[3] = {i, j, k};
+  // faces in xi, eta, zeta order (GHOST: external faces first, then
+  // internal ones, each in that order)
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
+  for (int step = 0; step < (GHOST ? 6 : 3); ++step) {
+    const int d = step % 3;
     if (idx[d] != 0 && idx[d] != N - 1) continue;
     const int side = idx[d] == 0 ? 0 : 1;
+    const int64_t en = nbr[6 * e + 2 * d + side];
+    if constexpr (GHOST) {
+      if ((en < 0) != (step < 3)) continue;
+    }
     double ni[3], mi, pki[13];
 #pragma unroll
     for (int a = 0; a < 3; ++a) ni[a] = side == 0 ? ij[ij3(d, a)] * -1.0 : ij[ij3(d, a)];
     flat_normalize(ni, &mi);
     sw_package(pki, up, g2, ni);
-    const int64_t en = nbr[6 * e + 2 * d + side];
     double uo[5], no[3], mo, pko[13], bc[5];
-    if (en < 0) {
+    if (GHOST && en < 0) {
       // external face: ghost state at the own point, own gamma2, reversed unit normal
       const int nblk = bc_kind == kSwBcGhostArray ? 5 : 3;
       sw_ghost_state(uo, bc_kind, up, ni, ghost ? ghost + e * nblk * NP + t : nullptr, NP);
@@ -659,8 +680,13 @@ static int launch_sw_fused_n(int num_elements, const int32_t* nbr, const double*
                              const double* dtab, const double* ghost, int bc_kind,
                              hipStream_t stream) {
   constexpr int threads = (N * N * N + 63) / 64 * 64;
-  hipLaunchKernelGGL(sw_rhs_fused_kernel<N>, dim3(num_elements), dim3(threads), 0, stream,
-                     nbr, u, gamma2, inv_jac, dt, dtab, ghost, bc_kind);
+  if (ghost != nullptr || bc_kind != kSwBcGhostArray) {
+    hipLaunchKernelGGL((sw_rhs_fused_kernel<N, true>), dim3(num_elements), dim3(threads), 0,
+                       stream, nbr, u, gamma2, inv_jac, dt, dtab, ghost, bc_kind);
+  } else {
+    hipLaunchKernelGGL((sw_rhs_fused_kernel<N, false>), dim3(num_elements), dim3(threads), 0,
+                       stream, nbr, u, gamma2, inv_jac, dt, dtab, ghost, bc_kind);
+  }
   return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
 }
 

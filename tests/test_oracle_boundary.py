@@ -28,6 +28,13 @@ def _face_points(extents, d, side):
     return np.nonzero(idx[d] == (0 if side == 0 else extents[d] - 1))[0]
 
 
+def _faces_per_point(extents):
+    n0, n1, n2 = extents
+    k, j, i = np.meshgrid(np.arange(n2), np.arange(n1), np.arange(n0), indexing="ij")
+    return sum(((a.ravel() == 0) | (a.ravel() == n - 1)).astype(int)
+               for a, n in ((i, n0), (j, n1), (k, n2)))
+
+
 @pytest.mark.parametrize("d", [0, 1, 2])
 def test_cut_internal_face_with_neighbour_ghost_reproduces_periodic_rhs(d):
     elements = [1, 1, 1]
@@ -50,7 +57,12 @@ def test_cut_internal_face_with_neighbour_ghost_reproduces_periodic_rhs(d):
     ghost[0][:, upper] = u[1][:, lower]
     ghost[1][:, lower] = u[0][:, upper]
     cut = ol.sw_rhs_dirichlet(extents, nb, u, gamma2, inv, ghost)
-    np.testing.assert_array_equal(cut, periodic)
+    # external corrections are added before internal ones, so points on more
+    # than one face round differently; points on one face are bit-identical
+    scale = np.max(np.abs(periodic))
+    np.testing.assert_allclose(cut, periodic, rtol=0, atol=4e-16 * scale)
+    one_face = _faces_per_point(extents) <= 1
+    np.testing.assert_array_equal(cut[:, :, one_face], periodic[:, :, one_face])
 
 
 def _exact_dt(brick, seed, t):
