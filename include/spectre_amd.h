@@ -137,9 +137,11 @@ int sxd_lift_flux(int64_t num_points, int nc, double* bc,
  * ApplyBoundaryCorrections.hpp:722-1050) for a single device.
  * neighbors: host array [6 E], entry 6 e + 2 d + side (side 0 = lower,
  * 1 = upper) is the element across that face, or -1 for an external face
- * (closed by a ghost state, see sxd_scalar_wave_rhs_dirichlet,
- * sxd_scalar_wave_rhs_spherical_radiation and sxd_gh_rhs_dirichlet; every
- * other call on such a domain returns SXD_ERR_UNSUPPORTED).  The table is copied to
+ * (closed by a boundary condition, see sxd_scalar_wave_rhs_dirichlet,
+ * sxd_scalar_wave_rhs_spherical_radiation,
+ * sxd_scalar_wave_rhs_constraint_preserving_spherical_radiation and
+ * sxd_gh_rhs_dirichlet; every other call on such a domain returns
+ * SXD_ERR_UNSUPPORTED).  The table is copied to
  * the device; this call allocates device memory. */
 typedef struct sxd_domain sxd_domain;
 int sxd_domain_create(sxd_domain** domain, int num_elements,
@@ -203,6 +205,25 @@ int sxd_scalar_wave_rhs_spherical_radiation(const sxd_domain* domain,
                                             const double* inv_jac,
                                             const double* coords, int type,
                                             double* dt, void* stream);
+
+/* sxd_scalar_wave_rhs on a domain with external faces closed by the
+ * ConstraintPreservingSphericalRadiation time-derivative condition
+ * (ScalarWave/BoundaryConditions/ConstraintPreservingSphericalRadiation.cpp:
+ * 59-132 dg_time_derivative, static mesh; added to dt on the face as in
+ * Actions/BoundaryConditionsImpl.hpp:636-648): external faces get no upwind
+ * term; at their points dt Pi gains the correction built from the interior
+ * state, its derivatives, the outward unit normal and gamma2 (Sommerfeld,
+ * first- or second-order Bayliss-Turkel, the latter two with 1/|x|).
+ * coords [E][3][Np] (Bayliss-Turkel types; may be null for Sommerfeld). */
+enum {
+  SXD_CPSR_SOMMERFELD = 1,
+  SXD_CPSR_FIRST_ORDER_BAYLISS_TURKEL = 2,
+  SXD_CPSR_SECOND_ORDER_BAYLISS_TURKEL = 3
+};
+int sxd_scalar_wave_rhs_constraint_preserving_spherical_radiation(
+    const sxd_domain* domain, const double* u, const double* gamma2,
+    const double* inv_jac, const double* coords, int type, double* dt,
+    void* stream);
 
 /* Full generalized-harmonic DG time derivative (harmonic gauge).
  * u, dt [E][50][Np]; gammas [E][3][Np]; inv_jac [E][9][Np]. */

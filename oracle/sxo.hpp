@@ -749,8 +749,49 @@ void sw_rhs_volume(int e_begin, int e_end, const int ext[3], const T* u,
 //                     62-100): psi and phi of the interior trace, pi = n^i phi_i
 //                     with the outward interior unit normal;
 //   kSwBaylissTurkel  the same plus psi / |x|, ghost holding the inertial
-//                     coordinates ([E][3][Np]).
+//                     coordinates ([E][3][Np]);
+//   kSwCpsr*          ConstraintPreservingSphericalRadiation, a time
+//                     derivative condition: no upwind term on the face; the
+//                     point's dt gets sw_cpsr_correction_point (ghost holds
+//                     the coordinates for the Bayliss-Turkel types).
 constexpr int kSwGhostArray = 0, kSwSommerfeld = 1, kSwBaylissTurkel = 2;
+constexpr int kSwCpsrSommerfeld = 3, kSwCpsrFirstOrderBt = 4, kSwCpsrSecondOrderBt = 5;
+
+// ConstraintPreservingSphericalRadiation::dg_time_derivative for a static mesh
+// (ScalarWave/BoundaryConditions/ConstraintPreservingSphericalRadiation.cpp:
+// 59-132): the dt Pi correction from the interior state u, its derivatives du
+// (du[3 c + i] = d_i of component c), the outward unit normal n, gamma2 and
+// the coordinates x (Bayliss-Turkel types; x[i * stride]).  dt psi and dt phi
+// get zero corrections.
+template <class T>
+inline T sw_cpsr_correction_point(int kind, const T u[5], const T du[15], const T n[3],
+                                  const T& gamma2, const T* x, size_t stride) {
+  auto dphi = [&](int i, int j) -> const T& { return du[3 * (2 + j) + i]; };
+  T c = dphi(0, 0);
+  for (int i = 1; i < 3; ++i) c = c + dphi(i, i);
+  if (kind == kSwCpsrSommerfeld) {
+    for (int i = 0; i < 3; ++i)
+      c = fma_(n[i], gamma2 * (du[i] - u[2 + i]) - du[3 + i], c);
+    return c;
+  }
+  T r2 = x[0] * x[0];
+  for (int i = 1; i < 3; ++i) r2 = fma_(x[i * stride], x[i * stride], r2);
+  const T inv_r = 1.0 / sqrt_(r2);
+  if (kind == kSwCpsrFirstOrderBt) {
+    c = fma_(-inv_r, u[1], c);
+    for (int i = 0; i < 3; ++i)
+      c = fma_(n[i], gamma2 * (du[i] - u[2 + i]) - du[3 + i], c);
+    return c;
+  }
+  c = fma_(-(2.0 * inv_r), fma_(-inv_r, u[0], 2.0 * u[1]), c);
+  for (int i = 0; i < 3; ++i) {
+    T w = fma_(-2.0, du[3 + i], gamma2 * (du[i] - u[2 + i]));
+    w = fma_(4.0 * inv_r, u[2 + i], w);
+    c = fma_(n[i], w, c);
+    for (int j = 0; j < 3; ++j) c = fma_(n[i] * n[j], dphi(i, j), c);
+  }
+  return c;
+}
 
 template <class T>
 void sw_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
@@ -761,7 +802,10 @@ void sw_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
   // ComputeTimeDerivative, ComputeTimeDerivative.hpp:633-637, before the
   // internal mortars' corrections of ApplyBoundaryCorrections), each group in
   // direction order
+  const bool cpsr = bc_kind >= kSwCpsrSommerfeld;
+  std::vector<T> du(cpsr ? 15 * np : 0);
   for (int e = e_begin; e < e_end; ++e) {
+    if (cpsr) partial_derivatives(u + e * 5 * np, 5, ext, inv_j + e * 9 * np, du.data());
     for (int step = 0; step < 12; ++step) {
       const int dir = step % 6;
       const int en = nbr[6 * e + dir];
@@ -769,6 +813,28 @@ void sw_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
       const int d = dir / 2, side = dir % 2;
       const Face fi = make_face(ext, d, side);
       const Face fe = make_face(ext, d, 1 - side);
+      if (en < 0 && cpsr) {
+        // time derivative condition: dt += correction at the face points
+        for (size_t f = 0; f < fi.pts.size(); ++f) {
+          const size_t p = fi.pts[f];
+          T up[5], dup[15], n[3], mag;
+          for (int c = 0; c < 5; ++c) up[c] = u[(e * 5 + c) * np + p];
+          for (int c = 0; c < 15; ++c) dup[c] = du[c * np + p];
+          for (int i = 0; i < 3; ++i) {
+            n[i] = inv_j[(e * 9 + ij3(d, i)) * np + p];
+            if (side == 0) n[i] = n[i] * -1.0;
+          }
+          flat_normalize(n, &mag);
+          const T corr = sw_cpsr_correction_point(
+              bc_kind, up, dup, n, gamma2[e * np + p],
+              ghost == nullptr ? nullptr : ghost + e * 3 * np + p, np);
+          for (int c = 0; c < 5; ++c) {
+            T& target = dt[(e * 5 + c) * np + p];
+            target = target + (c == 1 ? corr : T(0.0));
+          }
+        }
+        continue;
+      }
       for (size_t f = 0; f < fi.pts.size(); ++f) {
         T pk_in[16], pk_ex[16], bc[5];
         {

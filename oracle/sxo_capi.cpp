@@ -360,6 +360,36 @@ int sxo_sw_rhs_spherical_radiation(int num_elements, const int* ext, const int* 
   return 0;
 }
 
+// ScalarWave RHS with external faces closed by the constraint-preserving
+// spherical radiation condition: kind 1 Sommerfeld, 2 first-order and
+// 3 second-order Bayliss-Turkel (coords [E][3][Np], not read for Sommerfeld).
+int sxo_sw_rhs_cpsr(int num_elements, const int* ext, const int* nbr, const double* u,
+                    const double* gamma2, const double* inv_j, const double* coords, int kind,
+                    double* dt) {
+  if (bad_extents(ext) || num_elements < 1 || kind < 1 || kind > 3) return -1;
+  if (kind > 1 && coords == nullptr) return -1;
+  sxo::sw_rhs(num_elements, ext, nbr, u, gamma2, inv_j, dt, coords,
+              sxo::kSwCpsrSommerfeld + kind - 1);
+  return 0;
+}
+
+// dt Pi correction of the constraint-preserving spherical radiation condition
+// at n points: u [5][n], du [15][n], normal [3][n], gamma2 [n], coords [3][n].
+int sxo_sw_cpsr_correction(int n, int kind, const double* u, const double* du,
+                           const double* normal, const double* gamma2, const double* coords,
+                           double* out) {
+  if (n < 0 || kind < 1 || kind > 3) return -1;
+  for (int p = 0; p < n; ++p) {
+    double up[5], dup[15], nn[3];
+    for (int c = 0; c < 5; ++c) up[c] = u[c * n + p];
+    for (int c = 0; c < 15; ++c) dup[c] = du[c * n + p];
+    for (int i = 0; i < 3; ++i) nn[i] = normal[i * n + p];
+    out[p] = sxo::sw_cpsr_correction_point(sxo::kSwCpsrSommerfeld + kind - 1, up, dup, nn,
+                                           gamma2[p], coords + p, static_cast<size_t>(n));
+  }
+  return 0;
+}
+
 int sxo_gh_rhs_dirichlet(int num_elements, const int* ext, const int* nbr, const double* u,
                          const double* gammas, const double* inv_j, const double* ghost,
                          double* dt) {

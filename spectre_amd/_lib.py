@@ -43,6 +43,8 @@ _SIGNATURES = {
     "sxd_scalar_wave_rhs_dirichlet": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_scalar_wave_rhs_spherical_radiation": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp,
                                                                 ctypes.c_int, _vp, _vp]),
+    "sxd_scalar_wave_rhs_constraint_preserving_spherical_radiation": (
+        ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]),
     "sxd_gh_rhs": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_gh_rhs_dirichlet": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_rk3_substep": (ctypes.c_int, [ctypes.c_int, _i64, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]),

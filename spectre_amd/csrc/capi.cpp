@@ -644,6 +644,29 @@ int sxd_scalar_wave_rhs_spherical_radiation(const sxd_domain* domain, const doub
                 __func__);
 }
 
+int sxd_scalar_wave_rhs_constraint_preserving_spherical_radiation(
+    const sxd_domain* domain, const double* u, const double* gamma2, const double* inv_jac,
+    const double* coords, int type, double* dt, void* stream) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(gamma2);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  SXD_REQUIRE_DEVICE(dt);
+  if (type != SXD_CPSR_SOMMERFELD && type != SXD_CPSR_FIRST_ORDER_BAYLISS_TURKEL &&
+      type != SXD_CPSR_SECOND_ORDER_BAYLISS_TURKEL)
+    return fail(SXD_ERR_INVALID_ARGUMENT,
+                "unknown constraint-preserving spherical radiation type %d", type);
+  if (type != SXD_CPSR_SOMMERFELD) SXD_REQUIRE_DEVICE(coords);
+  if (domain->mixed)
+    return fail(SXD_ERR_UNSUPPORTED, "external faces need a uniform-extent domain");
+  sxd_domain* d = const_cast<sxd_domain*>(domain);
+  const int kind = kSwBcCpsrSommerfeld + (type - SXD_CPSR_SOMMERFELD);
+  return finish(run_sw(d, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,
+                       static_cast<hipStream_t>(stream),
+                       type == SXD_CPSR_SOMMERFELD ? nullptr : coords, kind),
+                __func__);
+}
+
 int sxd_gh_rhs(const sxd_domain* domain, const double* u, const double* gammas,
                const double* inv_jac, double* dt, void* stream) {
   if (int rc = check_periodic(domain); rc != SXD_OK) return rc;

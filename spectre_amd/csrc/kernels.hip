@@ -456,13 +456,54 @@ __device__ __forceinline__ void sw_ghost_state(double ue[5], int bc_kind, const 
   for (int i = 0; i < 3; ++i) ue[2 + i] = ui[2 + i];
 }
 
+// dt Pi correction of ConstraintPreservingSphericalRadiation::dg_time_derivative
+// for a static mesh (ScalarWave/BoundaryConditions/
+// ConstraintPreservingSphericalRadiation.cpp:59-132): from the interior state
+// u, its inertial derivatives du (du[3 c + i] = d_i of component c), the
+// outward unit normal n, gamma2 and, for the Bayliss-Turkel types, the
+// coordinates x (blocks of stride np).
+__device__ __forceinline__ double sw_cpsr_correction(int kind, const double u[5],
+                                                     const double du[15], const double n[3],
+                                                     double gamma2, const double* x,
+                                                     int64_t stride) {
+  // d_i Phi_j = du[3 (2 + j) + i]
+  double c = du[6];
+  c = c + du[10];
+  c = c + du[14];
+  if (kind == kSwBcCpsrSommerfeld) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) c = fma(n[i], gamma2 * (du[i] - u[2 + i]) - du[3 + i], c);
+    return c;
+  }
+  double r2 = x[0] * x[0];
+  r2 = fma(x[stride], x[stride], r2);
+  r2 = fma(x[2 * stride], x[2 * stride], r2);
+  const double inv_r = 1.0 / sqrt(r2);
+  if (kind == kSwBcCpsrFirstOrderBt) {
+    c = fma(-inv_r, u[1], c);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) c = fma(n[i], gamma2 * (du[i] - u[2 + i]) - du[3 + i], c);
+    return c;
+  }
+  c = fma(-(2.0 * inv_r), fma(-inv_r, u[0], 2.0 * u[1]), c);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double w = fma(-2.0, du[3 + i], gamma2 * (du[i] - u[2 + i]));
+    w = fma(4.0 * inv_r, u[2 + i], w);
+    c = fma(n[i], w, c);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) c = fma(n[i] * n[j], du[3 * (2 + j) + i], c);
+  }
+  return c;
+}
+
 // Composite SW face pass (any extents), same scheme as the GH one.
 __global__ void __launch_bounds__(256)
 sw_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
                     const int32_t* __restrict__ nbr, const double* __restrict__ u,
                     const double* __restrict__ gamma2, const double* __restrict__ inv_jac,
                     double* __restrict__ dt, const double* __restrict__ ghost, int bc_kind,
-                    int phase) {
+                    int phase, const double* __restrict__ du) {
   const int ext[3] = {n0, n1, n2};
   const int na = d == 0 ? n1 : n0;
   const int nb = d == 2 ? n1 : n2;
@@ -492,6 +533,20 @@ sw_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
   }
   flat_normalize(ni, &mi);
   const double g2 = gamma2[e * np + p];
+  if (en < 0 && bc_kind >= kSwBcCpsrSommerfeld) {
+    // time derivative condition: dt Pi += correction, no upwind term
+    double dup[15];
+#pragma unroll
+    for (int c = 0; c < 15; ++c) dup[c] = du[(e * 15 + c) * np + p];
+    const double corr = sw_cpsr_correction(bc_kind, ui, dup, ni, g2,
+                                           ghost ? ghost + e * 3 * np + p : nullptr, np);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      double& t = dt[(e * 5 + c) * np + p];
+      t = t + (c == 1 ? corr : 0.0);
+    }
+    return;
+  }
   sw_package(pki, ui, g2, ni);
   if (en < 0) {
     // external face: ghost state at the own point, own gamma2, reversed unit normal
@@ -522,7 +577,8 @@ sw_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
 
 int launch_face_passes(bool gh, int num_elements, const int32_t ext[3], const int32_t* nbr,
                        const double* u, const double* aux, const double* inv_jac,
-                       double* dt, const double* ghost, int bc_kind, hipStream_t stream) {
+                       double* dt, const double* ghost, int bc_kind, const double* du,
+                       hipStream_t stream) {
   // With external faces: one sweep over the external faces, then one over the
   // internal ones (the order in which the corrections reach dt).
   const bool external = ghost != nullptr || bc_kind != kSwBcGhostArray;
@@ -541,7 +597,7 @@ int launch_face_passes(bool gh, int num_elements, const int32_t ext[3], const in
       } else {
         hipLaunchKernelGGL(sw_face_pass_kernel, dim3(blocks), dim3(bs), 0, stream, d, ext[0],
                            ext[1], ext[2], num_elements, nbr, u, aux, inv_jac, dt, ghost,
-                           bc_kind, phases[ph]);
+                           bc_kind, phases[ph], du);
       }
       if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
     }
@@ -627,6 +683,14 @@ sw_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
     flat_normalize(ni, &mi);
     sw_package(pki, up, g2, ni);
     double uo[5], no[3], mo, pko[13], bc[5];
+    if (GHOST && en < 0 && bc_kind >= kSwBcCpsrSommerfeld) {
+      // time derivative condition: dt Pi += correction, no upwind term
+      const double corr = sw_cpsr_correction(bc_kind, up, dup, ni, g2,
+                                             ghost ? ghost + e * 3 * NP + t : nullptr, NP);
+#pragma unroll
+      for (int c = 0; c < 5; ++c) dtp[c] = dtp[c] + (c == 1 ? corr : 0.0);
+      continue;
+    }
     if (GHOST && en < 0) {
       // external face: ghost state at the own point, own gamma2, reversed unit normal
       const int nblk = bc_kind == kSwBcGhostArray ? 5 : 3;
@@ -727,7 +791,7 @@ int launch_sw_rhs(int num_elements, const int32_t ext[3], const int32_t* nbr,
                      dt);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   return launch_face_passes(false, num_elements, ext, nbr, u, gamma2, inv_jac, dt, ghost,
-                            bc_kind, stream);
+                            bc_kind, scratch_du, stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -794,7 +858,7 @@ int launch_gh_rhs_composite(int num_elements, const int32_t ext[3], const int32_
                      scratch_temps);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   return launch_face_passes(true, num_elements, ext, nbr, u, scratch_temps, inv_jac, dt,
-                            ghost, kSwBcGhostArray, stream);
+                            ghost, kSwBcGhostArray, nullptr, stream);
 }
 
 // ---------------------------------------------------------------------------

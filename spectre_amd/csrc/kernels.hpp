@@ -34,6 +34,10 @@ bool sw_fused_supported(const int32_t ext[3]);
 // Exterior state of external ScalarWave faces: the caller's ghost array
 // (DirichletAnalytic), or SphericalRadiation of the interior trace.
 constexpr int kSwBcGhostArray = 0, kSwBcSommerfeld = 1, kSwBcBaylissTurkel = 2;
+// ConstraintPreservingSphericalRadiation (a time-derivative condition: no
+// upwind term on the face, dt Pi += correction); ghost = coordinates for the
+// Bayliss-Turkel types.
+constexpr int kSwBcCpsrSommerfeld = 3, kSwBcCpsrFirstOrderBt = 4, kSwBcCpsrSecondOrderBt = 5;
 // fused: one launch per RHS (requires sw_fused_supported); otherwise the
 // composite path, which needs scratch_du [E][15][Np].  Faces whose neighbour
 // index is negative take their exterior state by bc_kind: kSwBcGhostArray

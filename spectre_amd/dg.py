@@ -254,6 +254,32 @@ class Domain:
             _ptr(coords), kinds[kind], _ptr(out), _stream()))
         return out
 
+    def scalar_wave_rhs_constraint_preserving_spherical_radiation(
+            self, u, gamma2, inv_jac, kind="sommerfeld", coords=None, out=None):
+        """scalar_wave_rhs with external faces closed by the
+        ConstraintPreservingSphericalRadiation time-derivative condition
+        (ConstraintPreservingSphericalRadiation.cpp:59-132): kind
+        "sommerfeld", "first_order_bayliss_turkel" or
+        "second_order_bayliss_turkel" (the latter two need coords [E, 3, Np])."""
+        kinds = {"sommerfeld": 1, "first_order_bayliss_turkel": 2,
+                 "second_order_bayliss_turkel": 3}
+        if kind not in kinds:
+            raise ValueError(f"kind must be one of {sorted(kinds)}")
+        _require(u, "u", self._shapes(5))
+        _require(gamma2, "gamma2", self._scalar_shape())
+        _require(inv_jac, "inv_jac", self._shapes(9))
+        if kind != "sommerfeld":
+            if coords is None:
+                raise ValueError(f"{kind} needs coords")
+            _require(coords, "coords", self._shapes(3))
+        if out is None:
+            out = torch.empty_like(u)
+        _require(out, "out", self._shapes(5))
+        check(lib().sxd_scalar_wave_rhs_constraint_preserving_spherical_radiation(
+            self._handle, _ptr(u), _ptr(gamma2), _ptr(inv_jac), _ptr(coords), kinds[kind],
+            _ptr(out), _stream()))
+        return out
+
     def gh_rhs(self, u, gammas, inv_jac, out=None):
         _require(u, "u", self._shapes(50))
         _require(gammas, "gammas", self._shapes(3))

@@ -182,6 +182,28 @@ def sw_rhs_spherical_radiation(extents, neighbors, u, gamma2, inv_jac, coords, k
     return dt
 
 
+def sw_rhs_cpsr(extents, neighbors, u, gamma2, inv_jac, coords, kind):
+    """kind: 1 Sommerfeld, 2 first-order, 3 second-order Bayliss-Turkel."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    gamma2 = np.ascontiguousarray(gamma2, dtype=np.float64)
+    inv_jac = np.ascontiguousarray(inv_jac, dtype=np.float64)
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    dt = np.empty_like(u)
+    ext, ep = _i(extents)
+    nb, nbp = _i(neighbors)
+    _check(lib().sxo_sw_rhs_cpsr(u.shape[0], ep, nbp, _d(u), _d(gamma2), _d(inv_jac),
+                                 _d(coords), int(kind), _d(dt)))
+    return dt
+
+
+def sw_cpsr_correction(kind, u, du, normal, gamma2, coords):
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, du, normal, gamma2, coords)]
+    n = args[0].shape[1]
+    out = np.empty(n)
+    _check(lib().sxo_sw_cpsr_correction(n, int(kind), *[_d(a) for a in args], _d(out)))
+    return out
+
+
 def gh_rhs_dirichlet(extents, neighbors, u, gammas, inv_jac, ghost):
     u = np.ascontiguousarray(u, dtype=np.float64)
     gammas = np.ascontiguousarray(gammas, dtype=np.float64)

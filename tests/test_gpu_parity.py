@@ -187,6 +187,30 @@ def test_scalar_wave_rhs_spherical_radiation_matches_oracle(kind, kernel, elemen
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("kind", ["sommerfeld", "first_order_bayliss_turkel",
+                                  "second_order_bayliss_turkel"])
+@pytest.mark.parametrize("kernel,elements,extents", [
+    ("fused", (3, 2, 2), (5, 5, 5)), ("composite", (3, 2, 2), (5, 5, 5)),
+    ("composite", (2, 2, 1), (4, 6, 5))])
+def test_scalar_wave_rhs_cpsr_matches_oracle(kind, kernel, elements, extents):
+    """Outer faces closed by the constraint-preserving spherical radiation
+    condition: noisy plane wave off the origin, point-varying gamma2."""
+    brick, u, _, inv, _ = wl.scalar_wave_plane_wave_dirichlet(
+        seed=7, elements=elements, extents=extents, t=0.1, noise=1e-2,
+        lower=(1.0, -1.0, 0.5), upper=(3.0, 1.0, 2.5))
+    gamma2 = np.random.default_rng(4).uniform(0, 2, (brick.num_elements, brick.num_points))
+    coords = brick.coordinates()
+    code = {"sommerfeld": 1, "first_order_bayliss_turkel": 2,
+            "second_order_bayliss_turkel": 3}[kind]
+    ref = ol.sw_rhs_cpsr(brick.extents, brick.neighbors, u, gamma2, inv, coords, code)
+    domain = dg.Domain.from_brick(brick)
+    domain.set_kernel("scalar_wave", kernel)
+    got = host(domain.scalar_wave_rhs_constraint_preserving_spherical_radiation(
+        dev(u), dev(gamma2), dev(inv), kind, dev(coords) if code > 1 else None))
+    assert rel_err(got, ref) <= TOL
+    assert np.array_equal(got, ref)
+
+
 def test_domain_with_external_faces_rejects_the_periodic_calls():
     brick, u, gamma2, inv, ghost = wl.scalar_wave_plane_wave_dirichlet(
         seed=1, elements=(2, 1, 1), extents=(4, 4, 4))

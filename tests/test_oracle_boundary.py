@@ -266,3 +266,114 @@ def test_bayliss_turkel_is_exact_for_an_outgoing_spherical_wave_on_the_axis():
     assert np.max(np.abs(som[0, :, centre] - no_corr[0, :, centre])) > 1e-2
     # off the axis the planar face is not radial and Bayliss-Turkel corrects
     assert np.max(np.abs(bt - no_corr)) > 1e-3
+
+
+# ---- ScalarWave ConstraintPreservingSphericalRadiation (a time-derivative
+# condition, ConstraintPreservingSphericalRadiation.cpp:59-132): the dt Pi
+# correction vanishes for outgoing solutions of the wave equation it is built
+# for: Sommerfeld for plane waves along n; first-order Bayliss-Turkel for the
+# monopole f(r - t)/r; second-order Bayliss-Turkel also for the dipole
+# d_z [f(r - t)/r], whose expansion has a 1/r and a 1/r^2 term.
+CPSR_SOMMERFELD, CPSR_BT1, CPSR_BT2 = 1, 2, 3
+
+
+def _radial_wave(x, dipole):
+    """psi = f(r - t)/r (dipole: its z derivative), f = sin, at t = 0, with
+    exact derivatives from nested dual numbers.  pi = -dt psi: since
+    dt f(r - t) = -f'(r - t), pi is the same expression with f' = cos.
+    Returns u [5, n], du [15, n] (du[3 c + i] = d_i of c), radial unit n."""
+    X = wl.seed_coordinates(x, 3)
+    r = wl.dsqrt(X[0] * X[0] + X[1] * X[1] + X[2] * X[2])
+    psi3, pi3 = wl.dsin(r) / r, wl.dcos(r) / r    # third-order duals
+    if dipole:
+        psi2, pi2 = psi3.d[2], pi3.d[2]
+    else:
+        psi2, pi2 = psi3.v, pi3.v
+    n = x.shape[1]
+    u = np.empty((5, n))
+    du = np.empty((15, n))
+    u[0], u[1] = psi2.v.v, pi2.v.v
+    for i in range(3):
+        u[2 + i] = psi2.d[i].v
+        du[i] = psi2.d[i].v
+        du[3 + i] = pi2.d[i].v
+        for j in range(3):
+            du[3 * (2 + j) + i] = psi2.d[j].d[i]
+    rr = np.sqrt(np.sum(x * x, axis=0))
+    return u, du, x / rr
+
+
+def _sample_points(seed, n=40):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(2.0, 5.0, (3, n)) * rng.choice([-1.0, 1.0], (3, n))
+
+
+def test_cpsr_annihilates_the_outgoing_waves_of_each_order():
+    x = _sample_points(1)
+    g2 = np.full(x.shape[1], 0.7)
+    u1, du1, nr = _radial_wave(x, dipole=False)
+    u2, du2, _ = _radial_wave(x, dipole=True)
+    corr = lambda k, u, du, n: ol.sw_cpsr_correction(k, u, du, n, g2, x)
+    assert np.max(np.abs(corr(CPSR_BT1, u1, du1, nr))) < 1e-14
+    assert np.max(np.abs(corr(CPSR_BT2, u1, du1, nr))) < 1e-14
+    assert np.max(np.abs(corr(CPSR_BT2, u2, du2, nr))) < 1e-14
+    assert np.max(np.abs(corr(CPSR_BT1, u2, du2, nr))) > 1e-3    # 1/r^2 term left
+    assert np.max(np.abs(corr(CPSR_SOMMERFELD, u1, du1, nr))) > 1e-3
+    # Sommerfeld: a plane wave leaving along n
+    k = np.array([0.6, -1.1, 0.8])
+    w = np.linalg.norm(k)
+    ph = k @ x
+    u = np.vstack([np.sin(ph), w * np.cos(ph), np.outer(k, np.cos(ph))])
+    du = np.zeros((15, x.shape[1]))
+    for i in range(3):
+        du[i] = k[i] * np.cos(ph)
+        du[3 + i] = -w * k[i] * np.sin(ph)
+        for j in range(3):
+            du[3 * (2 + j) + i] = -k[i] * k[j] * np.sin(ph)
+    n_out = np.repeat((k / w)[:, None], x.shape[1], axis=1)
+    assert np.max(np.abs(corr(CPSR_SOMMERFELD, u, du, n_out))) < 1e-14
+    assert np.max(np.abs(corr(CPSR_SOMMERFELD, u, du, -n_out))) > 1e-1
+
+
+def test_cpsr_constraint_damping_term():
+    """A constraint violation Phi = d psi - delta changes the correction by
+    gamma2 n.delta (all three types)."""
+    x = _sample_points(2)
+    u, du, nr = _radial_wave(x, dipole=True)
+    g2 = np.full(x.shape[1], 0.7)
+    delta = np.random.default_rng(3).uniform(-0.1, 0.1, (3, x.shape[1]))
+    bad = u.copy()
+    bad[2:] -= delta
+    for k in (CPSR_SOMMERFELD, CPSR_BT1):
+        diff = (ol.sw_cpsr_correction(k, bad, du, nr, g2, x)
+                - ol.sw_cpsr_correction(k, u, du, nr, g2, x))
+        np.testing.assert_allclose(diff, 0.7 * np.sum(nr * delta, axis=0), atol=1e-14)
+
+
+def test_cpsr_rhs_adds_the_point_correction_on_external_faces_only():
+    """One element, all faces external: face points that lie on one face get
+    the volume terms plus the point correction from the DG derivatives, with
+    no upwind term; interior points get the volume terms."""
+    brick = wl.Brick((2.0, -1.0, 0.5), (4.0, 1.0, 2.5), (1, 1, 1), (5, 6, 4), periodic=False)
+    x = brick.coordinates()
+    rng = np.random.default_rng(7)
+    u = rng.uniform(-1, 1, (1, 5, brick.num_points))
+    gamma2 = rng.uniform(0, 2, (1, brick.num_points))
+    inv = brick.inverse_jacobian()
+    du = ol.partial_derivatives(u[0], brick.extents, inv[0])
+    vol = ol.sw_time_derivative(u[0], du, gamma2[0])
+    count = _faces_per_point(brick.extents)
+    for kind in (CPSR_SOMMERFELD, CPSR_BT1, CPSR_BT2):
+        dt = ol.sw_rhs_cpsr(brick.extents, brick.neighbors, u, gamma2, inv, x, kind)[0]
+        np.testing.assert_array_equal(dt[:, count == 0], vol[:, count == 0])
+        for d in range(3):
+            for side in (0, 1):
+                pts = _face_points(brick.extents, d, side)
+                pts = pts[count[pts] == 1]
+                n = np.zeros((3, pts.size))
+                n[d] = 1.0 if side == 1 else -1.0
+                corr = ol.sw_cpsr_correction(kind, u[0][:, pts], du[:, pts], n,
+                                             gamma2[0, pts], x[0][:, pts])
+                np.testing.assert_array_equal(dt[1, pts], vol[1, pts] + corr)
+                for c in (0, 2, 3, 4):
+                    np.testing.assert_array_equal(dt[c, pts], vol[c, pts])
