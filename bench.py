@@ -218,8 +218,9 @@ def measure_adjoint(steps=40, reps=3):
 def measure_gh_adjoint(steps=20, reps=2, mixed=False):
     """Adjoint GH workload (300 elements, 20 RK3 steps, J = L2 norm of the
     gauge constraint): forward evolution against the backward (hand-written
-    adjoint kernels), with a checkpoint every 8 steps (earlier segments
-    recomputed) and with every stage input kept by the forward.
+    adjoint kernels), with a checkpoint every 8 and every steps / 2 steps
+    (earlier segments recomputed) and with every stage input kept by the
+    forward.
     Uniform: 6^3 points per element; mixed: extents drawn from {6..10} per
     element and dimension."""
     import numpy as np
@@ -257,11 +258,13 @@ def measure_gh_adjoint(steps=20, reps=2, mixed=False):
         torch.cuda.synchronize()
         return t0.elapsed_time(t1), t1.elapsed_time(t2)
 
-    # checkpoint every 8 steps (earlier segments recomputed in the backward),
-    # and one segment (the forward keeps every stage input: 3 states per step,
-    # a few GB at these sizes, no recomputation)
+    # checkpoint every 8 steps and every steps / 2 steps (earlier segments
+    # recomputed in the backward; the forward keeps the last segment's stage
+    # inputs), and one segment (the forward keeps every stage input: 3 states
+    # per step, a few GB at these sizes, no recomputation)
+    half = max(steps // 2, 1)
     timings = {}
-    for k in (8, steps):
+    for k in sorted({8, half, steps}):
         run(k)
         fwd, bwd = zip(*[run(k) for _ in range(reps)])
         timings[k] = (min(fwd), min(bwd))
@@ -270,10 +273,14 @@ def measure_gh_adjoint(steps=20, reps=2, mixed=False):
     rhs_ms = time_calls(lambda: domain.gh_rhs(u_d, d_g, d_inv, out=out), 5, 1)
     adj_ms = time_calls(lambda: domain.gh_rhs_adjoint(u_d, d_g, d_inv, gb, out=out), 3, 1)
     f8, b8 = timings[8]
+    fh, bh = timings[half]
     fa, ba = timings[steps]
     return {"workload": name, "grid_points": int(points), "steps": steps, "dt": dt,
             "forward_ms": round(f8, 3), "backward_with_recompute_ms": round(b8, 3),
             "adjoint_forward_cost": round(b8 / f8, 3),
+            f"forward_ms_checkpoint_{half}": round(fh, 3),
+            f"backward_ms_checkpoint_{half}": round(bh, 3),
+            f"adjoint_forward_cost_checkpoint_{half}": round(bh / fh, 3),
             "forward_ms_all_stages_kept": round(fa, 3),
             "backward_ms_all_stages_kept": round(ba, 3),
             "adjoint_forward_cost_all_stages_kept": round(ba / fa, 3),

@@ -611,6 +611,35 @@ __device__ void face_side_adjoint(const MixedElement& m, int d, int side, bool f
   *point = p;
 }
 
+// Side values of every face point of the faces normal to dimension d, both
+// sides of every element of the list, as face_side_adjoint forms them (the
+// normal taken outward): n, nv, lam[4], gamma2, |n| at sides[12 (off + p)].
+// Grid: (list length, 2 sides).
+__global__ void __launch_bounds__(128)
+gh_mixed_face_sides_kernel(int d, const MixedElement* __restrict__ el,
+                           const int32_t* __restrict__ list, const double* __restrict__ u,
+                           const double* __restrict__ params,
+                           const double* __restrict__ inv_jac, double* __restrict__ sides) {
+  const int side = blockIdx.y;
+  const MixedElement m = el[list[blockIdx.x]];
+  const int na = d == 0 ? m.n[1] : m.n[0], nb = d == 2 ? m.n[1] : m.n[2];
+  for (int f = threadIdx.x; f < na * nb; f += blockDim.x) {
+    GhSideFwd s;
+    int p;
+    face_side_adjoint(m, d, side, side == 0, f, u, params, inv_jac, s, &p);
+    double* o = sides + 12 * (m.off + p);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      o[i] = s.n[i];
+      o[3 + i] = s.nv[i];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[6 + k] = s.lam[k];
+    o[10] = params[3 * m.off + 2 * m.np + p];
+    o[11] = s.mag;
+  }
+}
+
 // GH faces, one pass per workgroup: grid (list length, 2 sides, 10 pairs) for
 // the faces normal to dimension d, so every (face, pair) runs concurrently.
 // Each workgroup packages its pair on both face meshes together with the
@@ -785,6 +814,7 @@ gh_mixed_face_adjoint_pass_kernel(int d, const MixedElement* __restrict__ el,
                                   const double* __restrict__ inv_jac,
                                   const double* __restrict__ gbar, double* __restrict__ ubar,
                                   double* __restrict__ side_adj,
+                                  const double* __restrict__ sides,
                                   const double* __restrict__ mortar_tab, int kf, int km) {
   extern __shared__ double sm[];
   const FaceAdjPassPlan<kFusedNb> L(kf, km);
@@ -808,41 +838,33 @@ gh_mixed_face_adjoint_pass_kernel(int d, const MixedElement* __restrict__ el,
   const double lift_own = lift_factor(m.n[d], 1.0), lift_nb = lift_factor(mn.n[d], 1.0);
   const int comp[5] = {pr, 10 + pr, 20 + 3 * pr, 21 + 3 * pr, 22 + 3 * pr};
   double* face = sm + L.face;
-  // own side: side values, speeds and package of the pair
+  // own side: side values (gh_mixed_face_sides_kernel), speeds and package
+  // of the pair
   for (int f = t; f < nf; f += T) {
-    GhSideFwd s;
-    int p;
-    face_side_adjoint(m, d, side, side == 0, f, u, params, inv_jac, s, &p);
+    const int p = face_point(m, d, side, f % na, f / na);
+    const double* sv = sides + 12 * (m.off + p);
     double* o = sm + L.side + 12 * f;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      o[i] = s.n[i];
-      o[3 + i] = s.nv[i];
-    }
+    for (int c = 0; c < 12; ++c) o[c] = sv[c];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      o[6 + k] = s.lam[k];
-      face[k * nf + f] = s.lam[k];
-    }
-    o[10] = params[3 * m.off + 2 * m.np + p];
-    o[11] = s.mag;
+    for (int k = 0; k < 4; ++k) face[k * nf + f] = o[6 + k];
     double pk[13];
-    gh_package_fwd(s.n, s.nv, s.lam, o[10], u + 50 * m.off + p, m.np, pr, pk);
+    gh_package_fwd(o, o + 3, o + 6, o[10], u + 50 * m.off + p, m.np, pr, pk);
 #pragma unroll
     for (int c = 0; c < 13; ++c) face[(4 + c) * nf + f] = pk[c];
   }
   __syncthreads();
   apply_face_matrices<17, 2>(na, nb, ka, kb, Ia_own, Ib_own, face, sm + L.tmp, sm + L.m_own);
   for (int f = t; f < mf; f += T) {
-    GhSideFwd s;
-    int q;
-    face_side_adjoint(mn, d, 1 - side, side == 1, f, u, params, inv_jac, s, &q);
+    const int q = face_point(mn, d, 1 - side, f % ma, f / ma);
+    double sv[12];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) face[k * mf + f] = s.lam[k];
-    sm[L.mag_nb + f] = s.mag;
+    for (int c = 0; c < 12; ++c) sv[c] = sides[12 * (mn.off + q) + c];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) face[k * mf + f] = sv[6 + k];
+    sm[L.mag_nb + f] = sv[11];
     double pk[13];
-    gh_package_fwd(s.n, s.nv, s.lam, params[3 * mn.off + 2 * mn.np + q], u + 50 * mn.off + q,
-                   mn.np, pr, pk);
+    gh_package_fwd(sv, sv + 3, sv + 6, sv[10], u + 50 * mn.off + q, mn.np, pr, pk);
 #pragma unroll
     for (int c = 0; c < 13; ++c) face[(4 + c) * mf + f] = pk[c];
   }
@@ -1077,19 +1099,24 @@ int launch_mixed_gh_adjoint(int count, const int32_t* list, const MixedElement* 
                      dtab);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   // faces: the derivative work array is free again and holds the side
-  // adjoints (110 doubles per point, under the 150 it has)
+  // adjoints (110 doubles per point) and the side values of the current
+  // dimension's face points (12 per point), under the 150 it has
   const int lds_split = FaceAdjPassPlan<false>(kf, km).total * 8;
   const int lds_fused = FaceAdjPassPlan<true>(kf, km).total * 8;
   const bool fused_nb = 160 * 1024 / lds_fused > 160 * 1024 / lds_split;
+  double* sides = scratch_du + 110 * total_points;
   for (int d = 0; d < 3; ++d) {
+    hipLaunchKernelGGL(gh_mixed_face_sides_kernel, dim3(count, 2), dim3(128), 0, stream, d, el,
+                       list, u, gammas, inv_jac, sides);
+    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
     if (fused_nb)
       hipLaunchKernelGGL(gh_mixed_face_adjoint_pass_kernel<true>, dim3(count, 2, 10), dim3(256),
                          lds_fused, stream, d, el, list, nbr, u, gammas, inv_jac, gbar, ubar,
-                         scratch_du, mtab, kf, km);
+                         scratch_du, sides, mtab, kf, km);
     else
       hipLaunchKernelGGL(gh_mixed_face_adjoint_pass_kernel<false>, dim3(count, 2, 10),
                          dim3(256), lds_split, stream, d, el, list, nbr, u, gammas, inv_jac,
-                         gbar, ubar, scratch_du, mtab, kf, km);
+                         gbar, ubar, scratch_du, sides, mtab, kf, km);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
     hipLaunchKernelGGL(gh_mixed_side_adjoint_kernel, dim3(count, 2), dim3(128), 0, stream, d, el,
                        list, u, gammas, inv_jac, scratch_du, ubar);

@@ -178,9 +178,15 @@ def gauge_constraint(u):
         idx[(a, b)] = idx[(b, a)] = p
     perm = torch.tensor([idx[(a, b)] for a in range(4) for b in range(4)], device=u.device)
     x = u.permute(0, 2, 1)                                          # [E][Np][50]
-    g = x[..., perm].reshape(x.shape[0], x.shape[1], 4, 4)          # [E][Np][4][4]
-    pi = x[..., 10 + perm].reshape(g.shape)
-    phi = torch.stack([x[..., 20 + i + 3 * perm].reshape(g.shape) for i in range(3)], 2)
+    # index_select rather than advanced indexing: its backward is a scatter-add
+    # (each storage slot receives at most two terms, so the sum is exact in any
+    # order) instead of torch's sort-based index_put backward
+    gather = lambda first: torch.index_select(x, -1, first + perm).reshape(
+        x.shape[0], x.shape[1], 4, 4)
+    g = gather(0)                                                   # [E][Np][4][4]
+    pi = gather(10)
+    phi = torch.stack([torch.index_select(x, -1, 20 + i + 3 * perm).reshape(g.shape)
+                       for i in range(3)], 2)
     s = g[..., 1:, 1:]                                              # spatial metric
     c00 = s[..., 1, 1] * s[..., 2, 2] - s[..., 1, 2] * s[..., 1, 2]
     c01 = s[..., 1, 2] * s[..., 0, 2] - s[..., 0, 1] * s[..., 2, 2]
@@ -266,7 +272,8 @@ def ragged_mass_weights(extents, inv_jac):
 def ragged_gauge_constraint_norm(u, index, mass):
     """gauge_constraint_norm of a ragged GH state u (flat), with the gather
     index of ragged_point_index(extents, 50) and ragged_mass_weights."""
-    return gauge_constraint_norm(u[index][None], mass[None])
+    pts = torch.index_select(u, 0, index.reshape(-1)).reshape(index.shape)
+    return gauge_constraint_norm(pts[None], mass[None])
 
 
 def psi_energy(u, mass):
