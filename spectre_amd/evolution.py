@@ -95,15 +95,18 @@ class ScalarWaveEvolution(torch.autograd.Function):
         return lam3, None, None, None, None, None, None, None
 
 
-def rk3_step_states(rhs, u, params, inv_jac, dt, l):
-    """One RK3 step of u (in place) returning its stage inputs (u, u1, u2)."""
-    u0 = u.clone()
+def rk3_step_states(rhs, u, params, inv_jac, dt, l, complete=True):
+    """One RK3 step of u (in place) returning its stage inputs (u, u1, u2).
+    complete=False stops after the stage inputs (u is left unchanged): the
+    step's result is not needed."""
+    u0 = u.clone() if complete else u
     rhs(u, params, inv_jac, out=l)
     u1 = dg.rk3_substep(0, dt, u, u, l)
     rhs(u1, params, inv_jac, out=l)
     u2 = dg.rk3_substep(1, dt, u, u1, l)
-    rhs(u2, params, inv_jac, out=l)
-    dg.rk3_substep(2, dt, u, u2, l, out=u)
+    if complete:
+        rhs(u2, params, inv_jac, out=l)
+        dg.rk3_substep(2, dt, u, u2, l, out=u)
     return u0, u1, u2
 
 
@@ -148,9 +151,12 @@ class GhEvolution(torch.autograd.Function):
             if seg == len(ctx.checkpoints) - 1:
                 stages = ctx.last_stages
             else:
+                # the segment's last step is not completed: its result is
+                # the next checkpoint
                 u = ctx.checkpoints[seg].clone()
-                stages = [rk3_step_states(domain.gh_rhs, u, gammas, inv_jac, dt, l)
-                          for _ in range(first, last)]
+                stages = [rk3_step_states(domain.gh_rhs, u, gammas, inv_jac, dt, l,
+                                          complete=n < last - 1)
+                          for n in range(first, last)]
             for s0, s1, s2 in reversed(stages):
                 domain.gh_rhs_adjoint(s2, gammas, inv_jac, lam3, out=a)
                 rk3_adjoint_substep(2, dt, lam3, a, mu, lam)
