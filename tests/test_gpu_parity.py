@@ -124,6 +124,27 @@ def test_scalar_wave_rhs_other_extents(extents):
     assert rel_err(got, ref) <= TOL
 
 
+@pytest.mark.parametrize("extents", [(20, 20, 20), (20, 2, 19), (2, 2, 2)])
+def test_whole_rhs_at_the_extremes_of_the_extents(extents):
+    """Largest (20 points) and smallest (2 points) extents per dimension, both
+    systems, composite organisation where the fused kernels do not apply."""
+    brick = wl.Brick((3.0, -1.0, 0.5), (5.0, 1.0, 1.5), (2, 1, 1), extents)
+    rng = np.random.default_rng(sum(extents))
+    u = rng.uniform(-1, 1, (brick.num_elements, 5, brick.num_points))
+    gamma2 = rng.uniform(0, 2, (brick.num_elements, brick.num_points))
+    inv = brick.inverse_jacobian()
+    domain = dg.Domain.from_brick(brick)
+    ref = ol.sw_rhs(brick.extents, brick.neighbors, u, gamma2, inv)
+    got = host(domain.scalar_wave_rhs(dev(u), dev(gamma2), dev(inv)))
+    assert np.array_equal(got, ref)
+    _, ug, gammas, _ = wl.gh_kerr_schild_brick(seed=3, elements=(2, 1, 1), extents=extents,
+                                               lower=(3.0, -1.0, 0.5), upper=(5.0, 1.0, 1.5))
+    ref = ol.gh_rhs(brick.extents, brick.neighbors, ug, gammas, inv)
+    got = host(domain.gh_rhs(dev(ug), dev(gammas), dev(inv)))
+    assert rel_err(got, ref) <= TOL
+    assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("kernel,elements,extents", [
     ("fused", (3, 2, 2), (5, 5, 5)), ("composite", (3, 2, 2), (5, 5, 5)),
     ("composite", (2, 3, 2), (4, 6, 5)), ("fused", (1, 1, 2), (10, 10, 10)),
