@@ -95,7 +95,11 @@ struct GhFusedLayout {
   static constexpr int kCorrStride = 6;              // 5 corrections + pad (16-byte rows)
   static constexpr int kCorr = kStage + (5 * NP + 1) / 2 * 2;  // 16-byte aligned
   static constexpr int kDeriv = kCorr + SLOTS * kCorrStride;  // MFMA: logical derivatives
-  static constexpr int kDoubles = kDeriv + (MFMA ? 9 * NP : 0);
+  // vector ALUs: second stage and correction buffers (odd passes), so a pass
+  // can stage its fields while slower waves still read the previous pass's
+  static constexpr int kStage2 = kDeriv + (MFMA ? 9 * NP : 0);
+  static constexpr int kCorr2 = kStage2 + (MFMA ? 0 : (5 * NP + 1) / 2 * 2);
+  static constexpr int kDoubles = kCorr2 + (MFMA ? 0 : SLOTS * kCorrStride);
 };
 
 // Volume index of face point (a, b) of face `dir` (= 2 d + side).
@@ -539,7 +543,12 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
     load_pass(0);
 #pragma unroll 1
     for (int p = 0; p < 10; ++p) {
-      lds_barrier();  // previous pass done with stage and corr
+      // Pass p stages into buffer p & 1.  A wave gets here only after the
+      // barrier that ends pass p - 1's reads of its stage, so buffer p & 1
+      // (last read in pass p - 2) is free, and pass p - 1's correction
+      // buffer is only read until the next barrier.
+      stage = smem + ((p & 1) ? L::kStage2 : L::kStage);
+      corr = smem + ((p & 1) ? L::kCorr2 : L::kCorr);
 #pragma unroll
       for (int q = 0; q < P; ++q)
         if (act[q]) {
