@@ -540,13 +540,49 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
       }
     }
   } else {
+    // pass pp's point-wise outputs plus the corrections of the faces each
+    // point lies on (xi, eta, zeta order; GHOST: external faces first), then
+    // the dt write
+    double prev[P][5];
+    auto finish_pass = [&](int pp) {
+      const double* cpp = smem + ((pp & 1) ? L::kCorr2 : L::kCorr);
+      const int comp[5] = {pp, 10 + pp, 20 + 3 * pp, 21 + 3 * pp, 22 + 3 * pp};
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        if (!act[q]) continue;
+        const int idx[3] = {pi_[q], pj[q], pk[q]};
+#pragma unroll
+        for (int step = 0; step < (GHOST ? 6 : 3); ++step) {
+          const int d = step % 3;
+          if (idx[d] != 0 && idx[d] != N - 1) continue;
+          const int side = idx[d] == 0 ? 0 : 1;
+          if constexpr (GHOST) {
+            if ((((ext_mask >> (2 * d + side)) & 1) != 0) != (step < 3)) continue;
+          }
+          const int a = d == 0 ? idx[1] : idx[0];
+          const int b = d == 2 ? idx[1] : idx[2];
+          const double2* cr = reinterpret_cast<const double2*>(
+              cpp + ((2 * d + side) * NF + a + N * b) * L::kCorrStride);
+          const double2 c01 = cr[0], c23 = cr[1], c4x = cr[2];
+          prev[q][0] = prev[q][0] + c01.x;
+          prev[q][1] = prev[q][1] + c01.y;
+          prev[q][2] = prev[q][2] + c23.x;
+          prev[q][3] = prev[q][3] + c23.y;
+          prev[q][4] = prev[q][4] + c4x.x;
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) dte[comp[c] * NP + t + q * T] = prev[q][c];
+      }
+    };
     load_pass(0);
 #pragma unroll 1
     for (int p = 0; p < 10; ++p) {
       // Pass p stages into buffer p & 1.  A wave gets here only after the
       // barrier that ends pass p - 1's reads of its stage, so buffer p & 1
-      // (last read in pass p - 2) is free, and pass p - 1's correction
-      // buffer is only read until the next barrier.
+      // (last read in pass p - 2) is free.  Pass p's corrections go to
+      // correction buffer p & 1; pass p - 1's are read after this pass's
+      // staging barrier (which ends their writes), before pass p + 1's
+      // barrier lets anyone write that buffer again.
       stage = smem + ((p & 1) ? L::kStage2 : L::kStage);
       corr = smem + ((p & 1) ? L::kCorr2 : L::kCorr);
 #pragma unroll
@@ -562,6 +598,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
         for (int c = 0; c < 5; ++c) ext_v[f][c] = ext_next[f][c];
       if (p + 1 < 10) load_pass(p + 1);
       lds_barrier();
+      if (p > 0) finish_pass(p - 1);
       // lifted boundary corrections of the face slots
 #pragma unroll
       for (int f = 0; f < PF; ++f) {
@@ -689,35 +726,15 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
 #pragma unroll
         for (int r = 0; r < 9; ++r) st[q].dt_pi_partial[r] = st[q].dt_pi_partial[r + 1];
       }
-      lds_barrier();  // corrections complete
-      const int comp[5] = {p, 10 + p, 20 + 3 * p, 21 + 3 * p, 22 + 3 * p};
+      // this pass's corrections are added after the next pass's staging
+      // barrier, which also ends their writes
 #pragma unroll
-      for (int q = 0; q < P; ++q) {
-        if (!act[q]) continue;
-        const int idx[3] = {pi_[q], pj[q], pk[q]};
+      for (int q = 0; q < P; ++q)
 #pragma unroll
-        for (int step = 0; step < (GHOST ? 6 : 3); ++step) {
-          const int d = step % 3;
-          if (idx[d] != 0 && idx[d] != N - 1) continue;
-          const int side = idx[d] == 0 ? 0 : 1;
-          if constexpr (GHOST) {
-            if ((((ext_mask >> (2 * d + side)) & 1) != 0) != (step < 3)) continue;
-          }
-          const int a = d == 0 ? idx[1] : idx[0];
-          const int b = d == 2 ? idx[1] : idx[2];
-          const double2* cr = reinterpret_cast<const double2*>(
-              corr + ((2 * d + side) * NF + a + N * b) * L::kCorrStride);
-          const double2 c01 = cr[0], c23 = cr[1], c4x = cr[2];
-          out[q][0] = out[q][0] + c01.x;
-          out[q][1] = out[q][1] + c01.y;
-          out[q][2] = out[q][2] + c23.x;
-          out[q][3] = out[q][3] + c23.y;
-          out[q][4] = out[q][4] + c4x.x;
-        }
-#pragma unroll
-        for (int c = 0; c < 5; ++c) dte[comp[c] * NP + t + q * T] = out[q][c];
-      }
+        for (int c = 0; c < 5; ++c) prev[q][c] = out[q][c];
     }
+    lds_barrier();  // the last pass's corrections complete
+    finish_pass(9);
   }
 }
 
