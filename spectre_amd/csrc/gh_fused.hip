@@ -36,7 +36,9 @@
 //
 // Two derivative engines (template parameter MFMA):
 //  - vector ALUs: each point sums its three LDS pencils, l = 0 .. N-1 in
-//    order (3 barriers per pass);
+//    order.  One barrier per pass: the stage and correction tiles are
+//    double-buffered, and a pass's face corrections are added (and its dt
+//    written) after the next pass's staging barrier;
 //  - matrix cores, described next (5 barriers per pass, logical derivatives
 //    go through LDS).
 // At 8^3 the vector-ALU engine is faster: the MFMA engine needs the extra
