@@ -232,6 +232,47 @@ def test_scalar_wave_rhs_cpsr_matches_oracle(kind, kernel, elements, extents):
     assert np.array_equal(got, ref)
 
 
+def test_boundary_calls_validate_their_arguments():
+    """Unknown condition types, missing coordinates, host ghost arrays and
+    mixed-p domains are rejected before any launch."""
+    import ctypes
+    from spectre_amd import _lib
+    brick, u, gamma2, inv, ghost = wl.scalar_wave_plane_wave_dirichlet(
+        seed=1, elements=(2, 1, 1), extents=(4, 4, 4))
+    domain = dg.Domain.from_brick(brick)
+    du, dg2, dinv, out = dev(u), dev(gamma2), dev(inv), dev(u)
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr() if t is not None else 0)
+    lib = _lib.lib()
+    h = domain._handle
+    sph = lib.sxd_scalar_wave_rhs_spherical_radiation
+    cpsr = lib.sxd_scalar_wave_rhs_constraint_preserving_spherical_radiation
+    assert sph(h, ptr(du), ptr(dg2), ptr(dinv), ptr(None), 7, ptr(out), None) == -1
+    assert sph(h, ptr(du), ptr(dg2), ptr(dinv), ptr(None), 2, ptr(out), None) == -1
+    assert cpsr(h, ptr(du), ptr(dg2), ptr(dinv), ptr(None), 0, ptr(out), None) == -1
+    assert cpsr(h, ptr(du), ptr(dg2), ptr(dinv), ptr(None), 3, ptr(out), None) == -1
+    host_ghost = np.ascontiguousarray(ghost)
+    assert lib.sxd_scalar_wave_rhs_dirichlet(
+        h, ptr(du), ptr(dg2), ptr(dinv), ctypes.c_void_p(host_ghost.ctypes.data), ptr(out),
+        None) == -1
+    with pytest.raises(ValueError):
+        domain.scalar_wave_rhs_spherical_radiation(du, dg2, dinv, "bayliss_turkel")
+    with pytest.raises(ValueError):
+        domain.scalar_wave_rhs_constraint_preserving_spherical_radiation(du, dg2, dinv, "robin")
+    # Sommerfeld needs no coordinates
+    torch.cuda.synchronize()
+    assert sph(h, ptr(du), ptr(dg2), ptr(dinv), ptr(None), 1, ptr(out), None) == 0
+    assert cpsr(h, ptr(du), ptr(dg2), ptr(dinv), ptr(None), 1, ptr(out), None) == 0
+    torch.cuda.synchronize()
+    # mixed-p domains have no external faces
+    mbrick, mu, mg2, minv = wl.scalar_wave_plane_wave_mixed(seed=2, elements=(2, 1, 1),
+                                                             lowest=3, highest=4)
+    mdom = dg.MixedDomain.from_brick(mbrick)
+    dmu = dev(mu)
+    assert lib.sxd_scalar_wave_rhs_dirichlet(
+        mdom._handle, ptr(dmu), ptr(dev(mg2)), ptr(dev(minv)), ptr(dmu), ptr(torch.empty_like(dmu)),
+        None) == -2
+
+
 def test_domain_with_external_faces_rejects_the_periodic_calls():
     brick, u, gamma2, inv, ghost = wl.scalar_wave_plane_wave_dirichlet(
         seed=1, elements=(2, 1, 1), extents=(4, 4, 4))
