@@ -177,6 +177,7 @@ class Domain:
         self.extents = tuple(int(x) for x in ext)
         self.num_elements = nb.shape[0]
         self.num_points = _np(self.extents)
+        self._args = (self.extents, nb.copy())
         handle = ctypes.c_void_p()
         check(lib().sxd_domain_create(ctypes.byref(handle), self.num_elements, ep,
                                       nb.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
@@ -332,6 +333,16 @@ class Domain:
                                        _ptr(gbar), _ptr(out), _stream()))
         return out
 
+    def twin(self):
+        """A second domain over the same elements, with this one's kernel and
+        tile choices and work arrays of its own, for RHS calls on another
+        stream at the same time as this domain's."""
+        t = type(self)(*self._args)
+        for system in SYSTEMS:
+            t.set_kernel(system, self.kernel(system))
+            t.set_threads(system, self.threads(system))
+        return t
+
     def set_threads(self, system, threads):
         """Threads per workgroup (tile shape) of the kernels for `system`."""
         check(lib().sxd_domain_set_threads(self._handle, SYSTEMS[system], threads))
@@ -387,6 +398,7 @@ class MixedDomain(Domain):
             raise ValueError("need extents [E, 3] and neighbors [E, 6]")
         self.extents = ext
         self.num_elements = nb.shape[0]
+        self._args = (ext.copy(), nb.copy())
         handle = ctypes.c_void_p()
         check(lib().sxd_domain_create_mixed(ctypes.byref(handle), self.num_elements,
                                             ext.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),

@@ -119,10 +119,11 @@ class GhEvolution(torch.autograd.Function):
     time from its checkpoint, then applies the transposed substeps with the
     adjoint RHS linearised at those stages (sxd_gh_rhs_adjoint).  The last
     segment's stage inputs are kept by the forward itself (the backward
-    starts there), so only the earlier segments are recomputed."""
+    starts there), so only the earlier segments are recomputed, each on a
+    second stream while the adjoint of the segment after it runs."""
 
     @staticmethod
-    def forward(ctx, u0, domain, gammas, inv_jac, dt, steps, checkpoint_every=8):
+    def forward(ctx, u0, domain, gammas, inv_jac, dt, steps, checkpoint_every=8, overlap=None):
         u = u0.detach().clone()
         l = torch.empty_like(u)
         checkpoints, last_stages = [], []
@@ -135,6 +136,10 @@ class GhEvolution(torch.autograd.Function):
             else:
                 rk3_step(domain, domain.gh_rhs, u, gammas, inv_jac, dt, torch.empty_like(u), l)
         ctx.domain, ctx.dt, ctx.steps, ctx.k = domain, dt, steps, checkpoint_every
+        # overlap the recompute with the adjoint (see backward): on by default
+        # for mixed-p domains; the uniform fused RHS kernel fills whole CUs
+        # and only competes with the adjoint kernels
+        ctx.overlap = isinstance(domain, dg.MixedDomain) if overlap is None else bool(overlap)
         ctx.checkpoints, ctx.last_stages = checkpoints, last_stages
         ctx.save_for_backward(gammas, inv_jac)
         return u
@@ -144,19 +149,50 @@ class GhEvolution(torch.autograd.Function):
         gammas, inv_jac = ctx.saved_tensors
         domain, dt, k = ctx.domain, ctx.dt, ctx.k
         lam3 = grad_out.detach().contiguous().clone()
-        a, lam, mu, l = (torch.empty_like(lam3) for _ in range(4))
-        for seg in reversed(range(len(ctx.checkpoints))):
+        a, lam, mu = (torch.empty_like(lam3) for _ in range(3))
+        nseg = len(ctx.checkpoints)
+        main = torch.cuda.current_stream(lam3.device)
+        side, twin = None, domain
+        if nseg > 1 and ctx.overlap:
+            # The stage inputs of segment seg - 1 do not depend on segment
+            # seg's adjoint: they are recomputed on a second stream (default
+            # priority) by a twin domain with work arrays of its own, while
+            # this stream runs the adjoint kernels.
+            side = torch.cuda.Stream(device=lam3.device)
+            side.wait_stream(main)
+            twin = getattr(domain, "_recompute_twin", None)
+            if twin is None:
+                twin = domain._recompute_twin = domain.twin()
+
+        def recompute(seg):
             first = seg * k
             last = min(ctx.steps, first + k)
-            if seg == len(ctx.checkpoints) - 1:
-                stages = ctx.last_stages
-            else:
+            with torch.cuda.stream(side if side is not None else main):
+                u = ctx.checkpoints[seg].clone()
+                l2 = torch.empty_like(u)
                 # the segment's last step is not completed: its result is
                 # the next checkpoint
-                u = ctx.checkpoints[seg].clone()
-                stages = [rk3_step_states(domain.gh_rhs, u, gammas, inv_jac, dt, l,
+                stages = [rk3_step_states(twin.gh_rhs, u, gammas, inv_jac, dt, l2,
                                           complete=n < last - 1)
                           for n in range(first, last)]
+                done = None
+                if side is not None:
+                    done = torch.cuda.Event()
+                    done.record(side)
+            return stages, done
+
+        pending = recompute(nseg - 2) if nseg > 1 and side is not None else None
+        for seg in reversed(range(nseg)):
+            if seg == nseg - 1:
+                stages = ctx.last_stages
+            else:
+                stages, done = pending if pending is not None else recompute(seg)
+                if done is not None:
+                    main.wait_event(done)
+                    for st in stages:
+                        for t in st:
+                            t.record_stream(main)
+                pending = recompute(seg - 1) if seg > 0 and side is not None else None
             for s0, s1, s2 in reversed(stages):
                 domain.gh_rhs_adjoint(s2, gammas, inv_jac, lam3, out=a)
                 rk3_adjoint_substep(2, dt, lam3, a, mu, lam)
@@ -166,7 +202,7 @@ class GhEvolution(torch.autograd.Function):
                 rk3_adjoint_substep(0, dt, lam, a, mu, None)
                 lam3, mu = mu, lam3
         ctx.checkpoints = ctx.last_stages = None
-        return lam3, None, None, None, None, None, None
+        return lam3, None, None, None, None, None, None, None
 
 
 def gauge_constraint(u):
