@@ -387,8 +387,27 @@ __device__ void face_package(const MixedElement& m, int d, int side, int f, int 
   for (int c = 0; c < 13; ++c) pk[c * nf + f] = out[c];
 }
 
+// LDS plan of the ScalarWave face kernel (doubles): small enough for two
+// workgroups per CU at 12 x 12 faces.  The neighbour side gets no mortar
+// buffer (its dimension-b interpolation is formed per mortar point with the
+// boundary terms), the mortar corrections overwrite the own side's mortar
+// package, and the projected corrections reuse the own face package buffer.
+struct SwFacePlan {
+  int side_own, side_nb, pk_own, pk_nb, tmp, m_own, bc, total;
+  __host__ __device__ SwFacePlan(int kf, int km) {
+    side_own = 0;               // n[3], |n| per own face point
+    side_nb = side_own + 4 * kf;  // the same per neighbour face point
+    pk_own = side_nb + 4 * kf;  // 13 per own face point; later 5 corrections
+    pk_nb = pk_own + 13 * kf;   // 13 per neighbour face point
+    tmp = pk_nb + 13 * kf;      // 13 per mortar point
+    m_own = tmp + 13 * km;      // own package on the mortar; then 5 corrections
+    bc = pk_own;
+    total = m_own + 13 * km;
+  }
+};
+
 // ScalarWave faces: one workgroup per element, its six faces in direction
-// order.  Dynamic LDS: FacePlan(kf, km, 13).total doubles.
+// order.  Dynamic LDS: SwFacePlan(kf, km).total doubles.
 __global__ void __launch_bounds__(256)
 sw_mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restrict__ list,
                      const int32_t* __restrict__ nbr, const double* __restrict__ u,
@@ -396,7 +415,7 @@ sw_mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restr
                      double* __restrict__ dt, const double* __restrict__ mortar_tab, int kf,
                      int km) {
   extern __shared__ double sm[];
-  const FacePlan L(kf, km, 13);
+  const SwFacePlan L(kf, km);
   const int t = threadIdx.x, T = blockDim.x;
   const int e = list[blockIdx.x];
   const MixedElement m = el[e];
@@ -430,21 +449,27 @@ sw_mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restr
     __syncthreads();
     apply_face_matrices<13, 2>(na, nb, ka, kb, Ia_own, Ib_own, sm + L.pk_own, sm + L.tmp,
                         sm + L.m_own);
-    apply_face_matrices<13, 2>(ma, mb, ka, kb, Ia_nb, Ib_nb, sm + L.pk_nb, sm + L.tmp,
-                        sm + L.m_nb);
+    // the neighbour side on the mortar: dimension a through tmp, then
+    // dimension b for each mortar point where its boundary terms are formed
+    // (apply_face_matrices' sums, without a mortar buffer for this side)
+    const double* src_nb = sm + L.pk_nb;
+    if (ma != ka) {
+      face_matrix_a<13, 2>(ma, mb, ka, Ia_nb, sm + L.pk_nb, sm + L.tmp);
+      __syncthreads();
+      src_nb = sm + L.tmp;
+    }
+    // boundary terms at the mortar points, written over the own side's slots
     for (int k = t; k < kmf; k += T) {
       double in[13], ex[13], bc[5];
+      face_matrix_b_point<13>(mb, ka, kb, Ib_nb, src_nb, k % ka, k / ka, ex);
 #pragma unroll
-      for (int c = 0; c < 13; ++c) {
-        in[c] = sm[L.m_own + c * kmf + k];
-        ex[c] = sm[L.m_nb + c * kmf + k];
-      }
+      for (int c = 0; c < 13; ++c) in[c] = sm[L.m_own + c * kmf + k];
       sw_boundary_terms(bc, in, ex);
 #pragma unroll
-      for (int c = 0; c < 5; ++c) sm[L.bc_m + c * kmf + k] = bc[c];
+      for (int c = 0; c < 5; ++c) sm[L.m_own + c * kmf + k] = bc[c];
     }
     __syncthreads();
-    apply_face_matrices<5, 1>(ka, kb, na, nb, Pa, Pb, sm + L.bc_m, sm + L.tmp, sm + L.bc);
+    apply_face_matrices<5, 1>(ka, kb, na, nb, Pa, Pb, sm + L.m_own, sm + L.tmp, sm + L.bc);
     // lift into dt
     for (int f = t; f < nf; f += T) {
       const int p = face_point(m, d, side, f % na, f / na);
@@ -1031,12 +1056,13 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
   if (face_lds > 160 * 1024) return SXD_ERR_UNSUPPORTED;
   if (!enable_large_lds()) return SXD_ERR_DEVICE;
   if (!gh) {
+    const int sw_face_lds = SwFacePlan(kf, km).total * 8;
     if (5 * np_max * 8 > 160 * 1024) return SXD_ERR_UNSUPPORTED;
     hipLaunchKernelGGL(sw_mixed_volume_kernel, dim3(count), dim3(threads), 5 * np_max * 8, stream,
                        el, list, u, params, inv_jac, dt, dtab);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
-    hipLaunchKernelGGL(sw_mixed_face_kernel, dim3(count), dim3(threads), face_lds, stream, el,
-                       list, nbr, u, params, inv_jac, dt, mtab, kf, km);
+    hipLaunchKernelGGL(sw_mixed_face_kernel, dim3(count), dim3(threads), sw_face_lds, stream,
+                       el, list, nbr, u, params, inv_jac, dt, mtab, kf, km);
     return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
   }
   if (scratch_du == nullptr) return SXD_ERR_INVALID_ARGUMENT;
