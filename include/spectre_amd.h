@@ -160,7 +160,9 @@ int sxd_domain_destroy(sxd_domain* domain);
  * Np_e points start at nc * offset_e, offset_e = sum_{e' < e} Np_{e'}
  * (damping parameters and inverse Jacobian likewise, with their component
  * counts).  SXD_ERR_UNSUPPORTED if an element or face is too large for the
- * kernels' LDS plan (element Np up to 4096 and faces up to 12 x 12 fit). */
+ * kernels' LDS plan (element Np up to 4096 and faces up to 12 x 12 fit).
+ * A neighbour of -1 marks an external face, evaluated by
+ * sxd_scalar_wave_rhs_dirichlet only. */
 int sxd_domain_create_mixed(sxd_domain** domain, int num_elements,
                             const int32_t* extents, const int32_t* neighbors);
 /* Total grid points of the domain (sum of Np_e). */
@@ -182,7 +184,8 @@ int sxd_scalar_wave_rhs(const sxd_domain* domain, const double* u,
  * (ScalarWave/BoundaryConditions/DirichletAnalytic.cpp:72-100 dg_ghost, applied
  * by Evolution/DiscontinuousGalerkin/Actions/BoundaryConditionsImpl.hpp:427-586).
  * ghost [E][5][Np] (psi, pi, phi_i of the analytic solution at every point;
- * only face points are read).  Internal faces are as in sxd_scalar_wave_rhs. */
+ * only face points are read).  Internal faces are as in sxd_scalar_wave_rhs.
+ * Uniform and mixed-p domains (ghost then in the ragged layout). */
 int sxd_scalar_wave_rhs_dirichlet(const sxd_domain* domain, const double* u,
                                   const double* gamma2, const double* inv_jac,
                                   const double* ghost, double* dt, void* stream);

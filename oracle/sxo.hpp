@@ -1323,9 +1323,12 @@ inline void to_mortar(int npk, int na, int nb, int ma, int mb, const T* pk_own, 
 }
 
 // Full ScalarWave DG RHS on a mixed-p set of elements (ragged layout).
+// External faces (neighbour < 0) take the exterior state from `ghost`
+// (ragged [5][Np_e] per element, read at the face's own points) as in
+// sw_rhs_faces, with no mortar; they are corrected before the internal faces.
 template <class T>
 void sw_rhs_mixed(int num_elements, const int* extents, const int* nbr, const T* u,
-                  const T* gamma2, const T* inv_j, T* dt) {
+                  const T* gamma2, const T* inv_j, T* dt, const T* ghost = nullptr) {
   const RaggedLayout L(num_elements, extents);
   for (int e = 0; e < num_elements; ++e) {
     const size_t np = L.np[e], off = L.off[e];
@@ -1341,9 +1344,37 @@ void sw_rhs_mixed(int num_elements, const int* extents, const int* nbr, const T*
   }
   for (int e = 0; e < num_elements; ++e) {
     const size_t np = L.np[e], off = L.off[e];
-    for (int dir = 0; dir < 6; ++dir) {
-      const int d = dir / 2, side = dir % 2;
+    for (int step = 0; step < 12; ++step) {
+      const int dir = step % 6;
       const int en = nbr[6 * e + dir];
+      if ((en < 0) != (step < 6)) continue;
+      const int d = dir / 2, side = dir % 2;
+      if (en < 0) {
+        const Face fi = make_face(L.ext[e].data(), d, side);
+        for (size_t f = 0; f < fi.pts.size(); ++f) {
+          const size_t p = fi.pts[f];
+          T up[5], ug[5], n[3], ng[3], mag, pk_in[16], pk_ex[16], b[5];
+          for (int c = 0; c < 5; ++c) {
+            up[c] = u[5 * off + c * np + p];
+            ug[c] = ghost[5 * off + c * np + p];
+          }
+          for (int i = 0; i < 3; ++i) {
+            n[i] = inv_j[9 * off + ij3(d, i) * np + p];
+            if (side == 0) n[i] = n[i] * -1.0;
+          }
+          flat_normalize(n, &mag);
+          sw_package_point(pk_in, up, gamma2[off + p], n);
+          for (int i = 0; i < 3; ++i) ng[i] = n[i] * -1.0;
+          sw_package_point(pk_ex, ug, gamma2[off + p], ng);
+          sw_boundary_terms_point(b, pk_in, pk_ex);
+          const T fac = lift_factor(L.ext[e][d], mag);
+          for (int c = 0; c < 5; ++c) {
+            T& target = dt[5 * off + c * np + p];
+            target = target + b[c] * fac;
+          }
+        }
+        continue;
+      }
       const size_t npn = L.np[en], offn = L.off[en];
       const Face fi = make_face(L.ext[e].data(), d, side);
       const Face fe = make_face(L.ext[en].data(), d, 1 - side);

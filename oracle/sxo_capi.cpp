@@ -435,14 +435,25 @@ int sxo_lgl_interpolation_matrix(int n, int num_targets, const double* targets, 
   return 0;
 }
 
-static bool bad_ragged(int num_elements, const int* extents, const int* nbr) {
+static bool bad_ragged(int num_elements, const int* extents, const int* nbr,
+                       bool external = false) {
   if (num_elements < 1) return true;
   for (int e = 0; e < num_elements; ++e) {
     if (bad_extents(extents + 3 * e)) return true;
     for (int k = 0; k < 6; ++k)
-      if (nbr[6 * e + k] < 0 || nbr[6 * e + k] >= num_elements) return true;
+      if (nbr[6 * e + k] < (external ? -1 : 0) || nbr[6 * e + k] >= num_elements) return true;
   }
   return false;
+}
+
+// Mixed-p ScalarWave RHS with external faces (neighbour -1) closed by the
+// ghost state `ghost` (ragged, 5 components).
+int sxo_sw_rhs_mixed_dirichlet(int num_elements, const int* extents, const int* nbr,
+                               const double* u, const double* gamma2, const double* inv_j,
+                               const double* ghost, double* dt) {
+  if (bad_ragged(num_elements, extents, nbr, true) || ghost == nullptr) return -1;
+  sxo::sw_rhs_mixed(num_elements, extents, nbr, u, gamma2, inv_j, dt, ghost);
+  return 0;
 }
 
 int sxo_sw_rhs_mixed(int num_elements, const int* extents, const int* nbr, const double* u,

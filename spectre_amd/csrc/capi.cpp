@@ -336,10 +336,13 @@ int sxd_domain_create_mixed(sxd_domain** domain, int num_elements, const int32_t
   for (int e = 0; e < num_elements; ++e)
     if (!valid_extents(extents + 3 * e))
       return fail(SXD_ERR_INVALID_ARGUMENT, "extents of element %d must be 2..20", e);
-  for (int64_t i = 0; i < 6LL * num_elements; ++i)
-    if (neighbors[i] < 0 || neighbors[i] >= num_elements)
+  bool has_boundary = false;
+  for (int64_t i = 0; i < 6LL * num_elements; ++i) {
+    if (neighbors[i] < -1 || neighbors[i] >= num_elements)
       return fail(SXD_ERR_INVALID_ARGUMENT, "neighbour %d of element %d out of range",
                   neighbors[i], static_cast<int>(i / 6));
+    has_boundary = has_boundary || neighbors[i] == -1;
+  }
   std::vector<MixedElement> el(num_elements);
   int64_t off = 0;
   int np_max = 0, kf = 0, km = 0;
@@ -355,7 +358,8 @@ int sxd_domain_create_mixed(sxd_domain** domain, int num_elements, const int32_t
     for (int dir = 0; dir < 6; ++dir) {
       const int d = dir / 2;
       const int32_t* a = extents + 3 * e;
-      const int32_t* b = extents + 3 * neighbors[6 * e + dir];
+      // an external face has no mortar: it sizes like a face onto itself
+      const int32_t* b = neighbors[6 * e + dir] < 0 ? a : extents + 3 * neighbors[6 * e + dir];
       const int fa = (d == 0 ? a[1] : a[0]), fb = (d == 2 ? a[1] : a[2]);
       const int ga = (d == 0 ? b[1] : b[0]), gb = (d == 2 ? b[1] : b[2]);
       kf = std::max({kf, fa * fb, ga * gb});
@@ -384,6 +388,7 @@ int sxd_domain_create_mixed(sxd_domain** domain, int num_elements, const int32_t
     i = j;
   }
   d->mixed = true;
+  d->has_boundary = has_boundary;
   d->num_elements = num_elements;
   d->total_points = off;
   d->np_max = np_max;
@@ -533,7 +538,8 @@ int resolve_kernel(const sxd_domain* d, int system) {
 }
 
 int run_mixed(sxd_domain* d, bool gh, int kernel, const double* u, const double* params,
-              const double* inv_jac, double* dt, hipStream_t stream) {
+              const double* inv_jac, double* dt, hipStream_t stream,
+              const double* ghost = nullptr) {
   double* scratch = nullptr;
   if (gh) {
     if (ensure_scratch(d, 150 * static_cast<size_t>(d->total_points), 0) != SXD_OK)
@@ -543,11 +549,11 @@ int run_mixed(sxd_domain* d, bool gh, int kernel, const double* u, const double*
   const int threads = d->threads[gh ? SXD_SYSTEM_GH : SXD_SYSTEM_SCALAR_WAVE];
   if (kernel == SXD_KERNEL_MIXED_BATCHED)
     return launch_mixed_rhs(gh, d->num_elements, d->d_list, d->d_el, d->d_nbr, d->np_max, d->kf,
-                            d->km, u, params, inv_jac, dt, scratch, threads, stream);
+                            d->km, u, params, inv_jac, dt, scratch, threads, stream, ghost);
   for (const auto& b : d->buckets) {
     const int rc = launch_mixed_rhs(gh, b.second, d->d_buckets + b.first, d->d_el, d->d_nbr,
                                     d->np_max, d->kf, d->km, u, params, inv_jac, dt, scratch,
-                                    threads, stream);
+                                    threads, stream, ghost);
     if (rc != SXD_OK) return rc;
   }
   return SXD_OK;
@@ -556,7 +562,10 @@ int run_mixed(sxd_domain* d, bool gh, int kernel, const double* u, const double*
 int run_sw(sxd_domain* d, int kernel, const double* u, const double* gamma2,
            const double* inv_jac, double* dt, hipStream_t stream,
            const double* ghost = nullptr, int bc_kind = kSwBcGhostArray) {
-  if (d->mixed) return run_mixed(d, false, kernel, u, gamma2, inv_jac, dt, stream);
+  if (d->mixed) {
+    if (bc_kind != kSwBcGhostArray) return SXD_ERR_UNSUPPORTED;
+    return run_mixed(d, false, kernel, u, gamma2, inv_jac, dt, stream, ghost);
+  }
   double* scratch = nullptr;
   if (kernel == SXD_KERNEL_COMPOSITE) {
     const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
@@ -612,8 +621,6 @@ int sxd_scalar_wave_rhs_dirichlet(const sxd_domain* domain, const double* u,
   SXD_REQUIRE_DEVICE(inv_jac);
   SXD_REQUIRE_DEVICE(ghost);
   SXD_REQUIRE_DEVICE(dt);
-  if (domain->mixed)
-    return fail(SXD_ERR_UNSUPPORTED, "external faces need a uniform-extent domain");
   sxd_domain* d = const_cast<sxd_domain*>(domain);
   return finish(run_sw(d, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,
                        static_cast<hipStream_t>(stream), ghost),

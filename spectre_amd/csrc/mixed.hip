@@ -407,21 +407,59 @@ struct SwFacePlan {
 };
 
 // ScalarWave faces: one workgroup per element, its six faces in direction
-// order.  Dynamic LDS: SwFacePlan(kf, km).total doubles.
+// order, external faces (neighbour -1) first.  An external face takes its
+// exterior state from ghost at its own points (own gamma2, reversed unit
+// normal; Dirichlet analytic condition) and needs no mortar.  Dynamic LDS:
+// SwFacePlan(kf, km).total doubles.
 __global__ void __launch_bounds__(256)
 sw_mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restrict__ list,
                      const int32_t* __restrict__ nbr, const double* __restrict__ u,
                      const double* __restrict__ params, const double* __restrict__ inv_jac,
                      double* __restrict__ dt, const double* __restrict__ mortar_tab, int kf,
-                     int km) {
+                     int km, const double* __restrict__ ghost) {
   extern __shared__ double sm[];
   const SwFacePlan L(kf, km);
   const int t = threadIdx.x, T = blockDim.x;
   const int e = list[blockIdx.x];
   const MixedElement m = el[e];
-  for (int dir = 0; dir < 6; ++dir) {
+  for (int step = 0; step < 12; ++step) {
+    const int dir = step % 6;
+    const int en = nbr[6 * e + dir];
+    if ((en < 0) != (step < 6)) continue;  // the same for the whole workgroup
     const int d = dir >> 1, side = dir & 1;
-    const MixedElement mn = el[nbr[6 * e + dir]];
+    if (en < 0) {
+      __syncthreads();  // the previous face's dt updates are done
+      const int na = d == 0 ? m.n[1] : m.n[0], nb = d == 2 ? m.n[1] : m.n[2];
+      for (int f = t; f < na * nb; f += T) {
+        const int p = face_point(m, d, side, f % na, f / na);
+        double ui[5], ug[5], n[3], ng[3], mag, pki[13], pke[13], bc[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+          ui[c] = u[5 * m.off + c * m.np + p];
+          ug[c] = ghost[5 * m.off + c * m.np + p];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          n[i] = inv_jac[9 * m.off + ij3(d, i) * m.np + p];
+          if (side == 0) n[i] = n[i] * -1.0;
+        }
+        flat_normalize(n, &mag);
+        const double g2 = params[m.off + p];
+        sw_package(pki, ui, g2, n);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ng[i] = n[i] * -1.0;
+        sw_package(pke, ug, g2, ng);
+        sw_boundary_terms(bc, pki, pke);
+        const double fac = lift_factor(m.n[d], mag);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+          double& target = dt[5 * m.off + c * m.np + p];
+          target = target + bc[c] * fac;
+        }
+      }
+      continue;
+    }
+    const MixedElement mn = el[en];
     const int na = d == 0 ? m.n[1] : m.n[0], nb = d == 2 ? m.n[1] : m.n[2];
     const int ma = d == 0 ? mn.n[1] : mn.n[0], mb = d == 2 ? mn.n[1] : mn.n[2];
     const int ka = max(na, ma), kb = max(nb, mb);
@@ -1046,7 +1084,7 @@ int mixed_face_lds_bytes(int kf, int km) { return FacePlan(kf, km, 13).total * 8
 int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement* el,
                      const int32_t* nbr, int np_max, int kf, int km, const double* u,
                      const double* params, const double* inv_jac, double* dt, double* scratch_du,
-                     int threads, hipStream_t stream) {
+                     int threads, hipStream_t stream, const double* ghost) {
   if (count == 0) return SXD_OK;
   const double* dtab = device_diff_matrices();
   const double* mtab = device_mortar_matrices();
@@ -1062,7 +1100,7 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
                        el, list, u, params, inv_jac, dt, dtab);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
     hipLaunchKernelGGL(sw_mixed_face_kernel, dim3(count), dim3(threads), sw_face_lds, stream,
-                       el, list, nbr, u, params, inv_jac, dt, mtab, kf, km);
+                       el, list, nbr, u, params, inv_jac, dt, mtab, kf, km, ghost);
     return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
   }
   if (scratch_du == nullptr) return SXD_ERR_INVALID_ARGUMENT;

@@ -222,17 +222,20 @@ class Brick:
 
 
 class MixedBrick:
-    """A periodic brick of affine elements whose per-dimension LGL extents
-    differ from element to element (p-refinement).  Fields use the ragged
-    Variables layout: element e's nc blocks of Np_e points start at
-    nc * offsets[e] of a flat array."""
+    """A brick of affine elements whose per-dimension LGL extents differ from
+    element to element (p-refinement), periodic or (periodic=False) with
+    external faces on its outer boundary.  Fields use the ragged Variables
+    layout: element e's nc blocks of Np_e points start at nc * offsets[e] of a
+    flat array."""
 
-    def __init__(self, lower, upper, elements, extents):
+    def __init__(self, lower, upper, elements, extents, periodic=True):
         self.lower, self.upper, self.elements = tuple(lower), tuple(upper), tuple(elements)
         self.extents = np.ascontiguousarray(extents, dtype=np.int32).reshape(-1, 3)
         if self.extents.shape[0] != int(np.prod(elements)):
             raise ValueError("one extents triple per element")
-        self.neighbors = periodic_neighbors(self.elements)
+        self.periodic = periodic
+        self.neighbors = (periodic_neighbors(self.elements) if periodic
+                          else bounded_neighbors(self.elements))
         self.points = np.prod(self.extents, axis=1).astype(np.int64)
         self.offsets = np.concatenate([[0], np.cumsum(self.points)]).astype(np.int64)
 
@@ -478,6 +481,24 @@ def scalar_wave_plane_wave_mixed(seed=0, elements=(10, 10, 10), lowest=4, highes
                       for e in range(brick.num_elements)])
     gamma2 = np.ones(brick.total_points)
     return brick, u, gamma2, brick.inverse_jacobian()
+
+
+def scalar_wave_plane_wave_mixed_dirichlet(seed=0, elements=(4, 4, 4), lowest=3, highest=8,
+                                           t=0.0, noise=0.0):
+    """Mixed-p ScalarWave workload on the non-periodic brick [-1,1]^3 whose
+    outer faces are closed by the Dirichlet analytic condition: the ghost
+    state is the plane wave of plane_wave(seed, t) at every point; u is that
+    wave plus seeded Gaussian noise of absolute size `noise`; gamma2 = 1.
+    Returns (brick, u, gamma2, inv_jac, ghost) in the ragged layout."""
+    ext = random_extents(int(np.prod(elements)), lowest, highest, seed)
+    brick = MixedBrick((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), elements, ext, periodic=False)
+    ghost = brick.ragged([plane_wave(brick.element_coordinates(e)[None], seed, t)[0][0]
+                          for e in range(brick.num_elements)])
+    u = ghost.copy()
+    if noise:
+        u += noise * np.random.default_rng(seed + 1).standard_normal(u.shape)
+    gamma2 = np.ones(brick.total_points)
+    return brick, u, gamma2, brick.inverse_jacobian(), ghost
 
 
 def gh_kerr_schild_mixed(seed=0, elements=(10, 10, 10), lowest=4, highest=12, noise=1.0e-6,

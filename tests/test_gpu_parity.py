@@ -232,6 +232,28 @@ def test_scalar_wave_rhs_cpsr_matches_oracle(kind, kernel, elements, extents):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("kernel", ["mixed_batched", "mixed_per_extents"])
+def test_scalar_wave_rhs_dirichlet_mixed_p_matches_oracle(kernel):
+    """Mixed p (extents 3..7) with the outer faces closed by the exact plane
+    wave as ghost; noisy data, point-varying gamma2."""
+    brick, u, _, inv, ghost = wl.scalar_wave_plane_wave_mixed_dirichlet(
+        seed=6, elements=(3, 2, 2), lowest=3, highest=7, t=0.3, noise=1e-2)
+    gamma2 = np.random.default_rng(9).uniform(0, 2, brick.total_points)
+    ref = ol.sw_rhs_mixed_dirichlet(brick.extents, brick.neighbors, u, gamma2, inv, ghost)
+    domain = dg.MixedDomain.from_brick(brick)
+    domain.set_kernel("scalar_wave", kernel)
+    got = host(domain.scalar_wave_rhs_dirichlet(dev(u), dev(gamma2), dev(inv), dev(ghost)))
+    assert rel_err(got, ref) <= TOL
+    assert np.array_equal(got, ref)
+    with pytest.raises(dg.SpectreAmdError, match="external faces"):
+        domain.scalar_wave_rhs(dev(u), dev(gamma2), dev(inv))
+    # on a periodic mixed domain the Dirichlet call is the periodic RHS
+    per = wl.MixedBrick(brick.lower, brick.upper, brick.elements, brick.extents)
+    pdom = dg.MixedDomain.from_brick(per)
+    a = host(pdom.scalar_wave_rhs_dirichlet(dev(u), dev(gamma2), dev(inv), dev(ghost)))
+    np.testing.assert_array_equal(a, host(pdom.scalar_wave_rhs(dev(u), dev(gamma2), dev(inv))))
+
+
 def test_boundary_calls_validate_their_arguments():
     """Unknown condition types, missing coordinates, host ghost arrays and
     mixed-p domains are rejected before any launch."""
@@ -263,14 +285,13 @@ def test_boundary_calls_validate_their_arguments():
     assert sph(h, ptr(du), ptr(dg2), ptr(dinv), ptr(None), 1, ptr(out), None) == 0
     assert cpsr(h, ptr(du), ptr(dg2), ptr(dinv), ptr(None), 1, ptr(out), None) == 0
     torch.cuda.synchronize()
-    # mixed-p domains have no external faces
+    # mixed-p domains take only the Dirichlet condition
     mbrick, mu, mg2, minv = wl.scalar_wave_plane_wave_mixed(seed=2, elements=(2, 1, 1),
                                                              lowest=3, highest=4)
     mdom = dg.MixedDomain.from_brick(mbrick)
     dmu = dev(mu)
-    assert lib.sxd_scalar_wave_rhs_dirichlet(
-        mdom._handle, ptr(dmu), ptr(dev(mg2)), ptr(dev(minv)), ptr(dmu), ptr(torch.empty_like(dmu)),
-        None) == -2
+    assert sph(mdom._handle, ptr(dmu), ptr(dev(mg2)), ptr(dev(minv)), ptr(None), 1,
+               ptr(torch.empty_like(dmu)), None) == -2
 
 
 def test_domain_with_external_faces_rejects_the_periodic_calls():

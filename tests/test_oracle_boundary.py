@@ -377,3 +377,38 @@ def test_cpsr_rhs_adds_the_point_correction_on_external_faces_only():
                 np.testing.assert_array_equal(dt[1, pts], vol[1, pts] + corr)
                 for c in (0, 2, 3, 4):
                     np.testing.assert_array_equal(dt[c, pts], vol[c, pts])
+
+
+# ---- external faces on mixed-p meshes ----------------------------------------
+
+
+def test_mixed_dirichlet_with_equal_extents_is_the_uniform_dirichlet_rhs():
+    brick, u, gamma2, inv, ghost = wl.scalar_wave_plane_wave_dirichlet(
+        seed=2, elements=(2, 2, 1), extents=(5, 4, 6), t=0.2, noise=1e-2)
+    uni = ol.sw_rhs_dirichlet(brick.extents, brick.neighbors, u, gamma2, inv, ghost)
+    ext = np.tile(np.asarray(brick.extents, dtype=np.int32), (brick.num_elements, 1))
+    mixed = ol.sw_rhs_mixed_dirichlet(ext, brick.neighbors, u.reshape(-1),
+                                      gamma2.reshape(-1), inv.reshape(-1), ghost.reshape(-1))
+    np.testing.assert_array_equal(mixed, uni.reshape(-1))
+
+
+def test_mixed_dirichlet_converges_to_exact_time_derivative():
+    """Extents drawn per element from {n - 2 .. n}: the RHS with the exact
+    plane wave as ghost tends to the exact dt as n grows."""
+    errors = []
+    for n in (6, 8, 10):
+        brick, u, gamma2, inv, ghost = wl.scalar_wave_plane_wave_mixed_dirichlet(
+            seed=3, elements=(2, 2, 2), lowest=n - 2, highest=n, t=0.4)
+        dt = ol.sw_rhs_mixed_dirichlet(brick.extents, brick.neighbors, u, gamma2, inv, ghost)
+        err = 0.0
+        for e, blk in enumerate(brick.split(dt, 5)):
+            x = brick.element_coordinates(e)
+            _, k = wl.plane_wave(x[None], 3, 0.4)
+            w = np.linalg.norm(k)
+            ph = k @ x - w * 0.4
+            exact = np.stack([-w * np.cos(ph), w * w * np.sin(ph)]
+                             + [w * k[i] * np.sin(ph) for i in range(3)])
+            err = max(err, np.max(np.abs(blk - exact)))
+        errors.append(err)
+    assert all(b < a * 0.2 for a, b in zip(errors, errors[1:]))
+    assert errors[-1] < 1e-4
