@@ -162,7 +162,7 @@ int sxd_domain_destroy(sxd_domain* domain);
  * counts).  SXD_ERR_UNSUPPORTED if an element or face is too large for the
  * kernels' LDS plan (element Np up to 4096 and faces up to 12 x 12 fit).
  * A neighbour of -1 marks an external face, evaluated by
- * sxd_scalar_wave_rhs_dirichlet only. */
+ * sxd_scalar_wave_rhs_dirichlet and sxd_gh_rhs_dirichlet only. */
 int sxd_domain_create_mixed(sxd_domain** domain, int num_elements,
                             const int32_t* extents, const int32_t* neighbors);
 /* Total grid points of the domain (sum of Np_e). */
@@ -243,9 +243,9 @@ int sxd_gh_rhs(const sxd_domain* domain, const double* u, const double* gammas,
  * (GeneralizedHarmonic/BoundaryConditions/DirichletAnalytic.cpp:70-115
  * dg_ghost, applied by
  * Evolution/DiscontinuousGalerkin/Actions/BoundaryConditionsImpl.hpp:427-586).
- * ghost [E][50][Np] (only face points are read).  Runs the domain's fused
- * (vector-ALU) or composite kernel; SXD_ERR_UNSUPPORTED if the fused MFMA
- * kernel is selected. */
+ * ghost [E][50][Np] (only face points are read; ragged on mixed-p domains).
+ * Runs the domain's fused (vector-ALU), composite or mixed-p kernels;
+ * SXD_ERR_UNSUPPORTED if the fused MFMA kernel is selected. */
 int sxd_gh_rhs_dirichlet(const sxd_domain* domain, const double* u,
                          const double* gammas, const double* inv_jac,
                          const double* ghost, double* dt, void* stream);

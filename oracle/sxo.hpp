@@ -1435,10 +1435,12 @@ void sw_rhs_mixed(int num_elements, const int* extents, const int* nbr, const T*
 }
 
 // Full GH DG RHS on a mixed-p set of elements (ragged layout); gammas
-// [3][Np_e] per element.
+// [3][Np_e] per element.  External faces (neighbour < 0) take the exterior
+// state from `ghost` (ragged [50][Np_e]) at the face's own points, as in
+// gh_rhs_faces, with no mortar; they are corrected before the internal faces.
 template <class T>
 void gh_rhs_mixed(int num_elements, const int* extents, const int* nbr, const T* u,
-                  const T* gammas, const T* inv_j, T* dt) {
+                  const T* gammas, const T* inv_j, T* dt, const T* ghost = nullptr) {
   const RaggedLayout L(num_elements, extents);
   std::vector<GhFaceTemps<T>> temps(L.total);
   for (int e = 0; e < num_elements; ++e) {
@@ -1456,9 +1458,54 @@ void gh_rhs_mixed(int num_elements, const int* extents, const int* nbr, const T*
   }
   for (int e = 0; e < num_elements; ++e) {
     const size_t np = L.np[e], off = L.off[e];
-    for (int dir = 0; dir < 6; ++dir) {
-      const int d = dir / 2, side = dir % 2;
+    for (int step = 0; step < 12; ++step) {
+      const int dir = step % 6;
       const int en = nbr[6 * e + dir];
+      if ((en < 0) != (step < 6)) continue;
+      const int d = dir / 2, side = dir % 2;
+      if (en < 0) {
+        const Face fi = make_face(L.ext[e].data(), d, side);
+        const T* ux = ghost + 50 * off;
+        for (size_t f = 0; f < fi.pts.size(); ++f) {
+          const size_t p = fi.pts[f];
+          const GhFaceTemps<T>& ti = temps[off + p];
+          T n_in[3], nv_in[3], mag_in, n_ex[3], nv_ex[3], g[10], det;
+          for (int i = 0; i < 3; ++i) {
+            n_in[i] = inv_j[9 * off + ij3(d, i) * np + p];
+            if (side == 0) n_in[i] = n_in[i] * -1.0;
+          }
+          gh_normalize(n_in, nv_in, &mag_in, ti.inv_spatial);
+          for (int c = 0; c < 10; ++c) g[c] = ux[c * np + p];
+          GhFaceTemps<T> te;
+          te.gamma1 = ti.gamma1;
+          te.gamma2 = ti.gamma2;
+          gh_lapse_shift(&te.lapse, te.shift, &det, te.inv_spatial, g);
+          for (int i = 0; i < 3; ++i) n_ex[i] = n_in[i] * -1.0;
+          gh_ghost_normal(n_ex, nv_ex, te.inv_spatial);
+          T lam_in[4], lam_ex[4];
+          gh_char_speeds(lam_in, ti.gamma1, ti.lapse, ti.shift, n_in);
+          gh_char_speeds(lam_ex, te.gamma1, te.lapse, te.shift, n_ex);
+          const T fac = lift_factor(L.ext[e][d], mag_in);
+          for (int pr = 0; pr < 10; ++pr) {
+            T phi_i[3], phi_e[3], pk_in[13], pk_ex[13], b[5];
+            for (int i = 0; i < 3; ++i) {
+              phi_i[i] = u[50 * off + (20 + i + 3 * pr) * np + p];
+              phi_e[i] = ux[(20 + i + 3 * pr) * np + p];
+            }
+            gh_package_pair(pk_in, u[50 * off + pr * np + p], u[50 * off + (10 + pr) * np + p],
+                            phi_i, ti.gamma2, lam_in, n_in, nv_in);
+            gh_package_pair(pk_ex, ux[pr * np + p], ux[(10 + pr) * np + p], phi_e, te.gamma2,
+                            lam_ex, n_ex, nv_ex);
+            gh_boundary_terms_pair(b, pk_in, lam_in, pk_ex, lam_ex);
+            const int comp[5] = {pr, 10 + pr, 20 + 3 * pr, 21 + 3 * pr, 22 + 3 * pr};
+            for (int c = 0; c < 5; ++c) {
+              T& target = dt[50 * off + comp[c] * np + p];
+              target = target + b[c] * fac;
+            }
+          }
+        }
+        continue;
+      }
       const size_t npn = L.np[en], offn = L.off[en];
       const Face fi = make_face(L.ext[e].data(), d, side);
       const Face fe = make_face(L.ext[en].data(), d, 1 - side);

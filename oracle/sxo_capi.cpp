@@ -463,6 +463,16 @@ int sxo_sw_rhs_mixed(int num_elements, const int* extents, const int* nbr, const
   return 0;
 }
 
+// Mixed-p GH RHS with external faces (neighbour -1) closed by the ghost state
+// `ghost` (ragged, 50 components).
+int sxo_gh_rhs_mixed_dirichlet(int num_elements, const int* extents, const int* nbr,
+                               const double* u, const double* gammas, const double* inv_j,
+                               const double* ghost, double* dt) {
+  if (bad_ragged(num_elements, extents, nbr, true) || ghost == nullptr) return -1;
+  sxo::gh_rhs_mixed(num_elements, extents, nbr, u, gammas, inv_j, dt, ghost);
+  return 0;
+}
+
 int sxo_gh_rhs_mixed(int num_elements, const int* extents, const int* nbr, const double* u,
                      const double* gammas, const double* inv_j, double* dt) {
   if (bad_ragged(num_elements, extents, nbr)) return -1;

@@ -580,7 +580,7 @@ int run_sw(sxd_domain* d, int kernel, const double* u, const double* gamma2,
 int run_gh(sxd_domain* d, int kernel, const double* u, const double* gammas,
            const double* inv_jac, double* dt, hipStream_t stream,
            const double* ghost = nullptr) {
-  if (d->mixed) return run_mixed(d, true, kernel, u, gammas, inv_jac, dt, stream);
+  if (d->mixed) return run_mixed(d, true, kernel, u, gammas, inv_jac, dt, stream, ghost);
   if (kernel == SXD_KERNEL_FUSED || kernel == SXD_KERNEL_FUSED_MFMA)
     return launch_gh_rhs_fused(d->num_elements, d->ext, d->d_nbr, u, gammas, inv_jac, dt,
                                kernel == SXD_KERNEL_FUSED ? kGhFusedValu : kGhFusedMfma,
@@ -696,8 +696,6 @@ int sxd_gh_rhs_dirichlet(const sxd_domain* domain, const double* u, const double
   SXD_REQUIRE_DEVICE(inv_jac);
   SXD_REQUIRE_DEVICE(ghost);
   SXD_REQUIRE_DEVICE(dt);
-  if (domain->mixed)
-    return fail(SXD_ERR_UNSUPPORTED, "external faces need a uniform-extent domain");
   sxd_domain* d = const_cast<sxd_domain*>(domain);
   const int kernel = resolve_kernel(d, SXD_SYSTEM_GH);
   if (kernel == SXD_KERNEL_FUSED_MFMA)

@@ -320,9 +320,6 @@ __device__ __forceinline__ int opposite_point(int i, int j, int k, int d, int n0
   return i + n0 * (j + n1 * k);
 }
 
-// Which faces a face-pass launch handles: all of them (no external faces), or
-// the external or internal ones of a domain with external faces.
-constexpr int kFacesAll = 0, kFacesExternal = 1, kFacesInternal = 2;
 
 // ---------------------------------------------------------------------------
 // Composite GH face pass for one dimension d: every (element, side, face

@@ -31,6 +31,10 @@ int launch_gh_upwind(int64_t n, const double* u_int, const double* t_int,
 int launch_lift_flux(int64_t n, int nc, double* bc, int n_perp, const double* mag,
                      hipStream_t stream);
 bool sw_fused_supported(const int32_t ext[3]);
+// Which faces a face-pass launch handles: all of them (no external faces), or
+// the external or internal ones of a domain with external faces (external
+// faces are corrected first).
+constexpr int kFacesAll = 0, kFacesExternal = 1, kFacesInternal = 2;
 // Exterior state of external ScalarWave faces: the caller's ghost array
 // (DirichletAnalytic), or SphericalRadiation of the interior trace.
 constexpr int kSwBcGhostArray = 0, kSwBcSommerfeld = 1, kSwBcBaylissTurkel = 2;
@@ -75,8 +79,8 @@ int mixed_face_lds_bytes(int kf, int km);
 // Volume terms and mortar-coupled boundary corrections of the `count`
 // elements in `list` (one workgroup each, `threads` threads: 64, 128, 256).
 // gh: params = gammas [3][Np_e], scratch_du = 150 x total points; otherwise
-// params = gamma2 [Np_e].  ScalarWave: faces whose neighbour is -1 take their
-// exterior state from ghost (ragged [5][Np_e], read at the face's own points).
+// params = gamma2 [Np_e].  Faces whose neighbour is -1 take their exterior
+// state from ghost (ragged [nc][Np_e], read at the face's own points).
 int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement* el,
                      const int32_t* nbr, int np_max, int kf, int km, const double* u,
                      const double* params, const double* inv_jac, double* dt, double* scratch_du,

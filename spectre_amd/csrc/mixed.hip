@@ -736,14 +736,67 @@ gh_mixed_face_pass_kernel(int d, const MixedElement* __restrict__ el,
                           const int32_t* __restrict__ list, const int32_t* __restrict__ nbr,
                           const double* __restrict__ u, const double* __restrict__ params,
                           const double* __restrict__ inv_jac, double* __restrict__ dt,
-                          const double* __restrict__ mortar_tab, int kf, int km) {
+                          const double* __restrict__ mortar_tab, int kf, int km,
+                          const double* __restrict__ ghost, int phase) {
   extern __shared__ double sm[];
   const FacePassPlan<kFusedNb> L(kf, km);
   const int t = threadIdx.x, T = blockDim.x;
   const int side = blockIdx.y, pr = blockIdx.z;
   const int e = list[blockIdx.x];
   const MixedElement m = el[e];
-  const MixedElement mn = el[nbr[6 * e + 2 * d + side]];
+  const int en = nbr[6 * e + 2 * d + side];
+  // the same for the whole workgroup
+  if ((phase == kFacesExternal && en >= 0) || (phase == kFacesInternal && en < 0)) return;
+  if (en < 0) {
+    // external face (Dirichlet analytic): the ghost state at the face's own
+    // points, the element's damping parameters and the reversed unit normal
+    // renormalised with the ghost inverse spatial metric; no mortar
+    const int na = d == 0 ? m.n[1] : m.n[0], nb = d == 2 ? m.n[1] : m.n[2];
+    const int np = m.np;
+    for (int f = t; f < na * nb; f += T) {
+      const int p = face_point(m, d, side, f % na, f / na);
+      double sd[12];
+      face_side<true>(m, d, side, side == 0, f, u, params, inv_jac, sd, true);
+      GhFaceSide si, se;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        si.n[i] = sd[i];
+        si.nv[i] = sd[3 + i];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) si.lam[i] = sd[6 + i];
+      si.gamma2 = sd[10];
+      const double* ux = ghost + 50 * m.off + p;
+      double g[10], lapse, shift[3], inv_sp[6];
+#pragma unroll
+      for (int c = 0; c < 10; ++c) g[c] = ux[c * np];
+      gh_lapse_shift(&lapse, shift, inv_sp, g);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) se.n[i] = si.n[i] * -1.0;
+      gh_ghost_normal(se.n, se.nv, inv_sp);
+      gh_char_speeds(se.lam, params[3 * m.off + np + p], lapse, shift, se.n);
+      se.gamma2 = si.gamma2;
+      const double* ue = u + 50 * m.off + p;
+      double phi_i[3], phi_e[3], pki[13], pke[13], bc[5];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        phi_i[i] = ue[(20 + i + 3 * pr) * np];
+        phi_e[i] = ux[(20 + i + 3 * pr) * np];
+      }
+      gh_package_pair(pki, ue[pr * np], ue[(10 + pr) * np], phi_i, si);
+      gh_package_pair(pke, ux[pr * np], ux[(10 + pr) * np], phi_e, se);
+      gh_boundary_terms_pair(bc, pki, si.lam, pke, se.lam);
+      const double fac = lift_factor(m.n[d], sd[11]);
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        const int comp = c == 0 ? pr : c == 1 ? 10 + pr : 18 + c + 3 * pr;
+        double& target = dt[50 * m.off + comp * np + p];
+        target = target + bc[c] * fac;
+      }
+    }
+    return;
+  }
+  const MixedElement mn = el[en];
   const int na = d == 0 ? m.n[1] : m.n[0], nb = d == 2 ? m.n[1] : m.n[2];
   const int ma = d == 0 ? mn.n[1] : mn.n[0], mb = d == 2 ? mn.n[1] : mn.n[2];
   const int ka = max(na, ma), kb = max(nb, mb);
@@ -1117,16 +1170,21 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
   const int lds_split = FacePassPlan<false>(kf, km).total * 8;
   const int lds_fused = FacePassPlan<true>(kf, km).total * 8;
   const bool fused_nb = 160 * 1024 / lds_fused > 160 * 1024 / lds_split;
-  for (int d = 0; d < 3; ++d) {
-    if (fused_nb)
-      hipLaunchKernelGGL(gh_mixed_face_pass_kernel<true>, dim3(count, 2, 10), dim3(threads),
-                         lds_fused, stream, d, el, list, nbr, u, params, inv_jac, dt, mtab, kf,
-                         km);
-    else
-      hipLaunchKernelGGL(gh_mixed_face_pass_kernel<false>, dim3(count, 2, 10), dim3(threads),
-                         lds_split, stream, d, el, list, nbr, u, params, inv_jac, dt, mtab, kf,
-                         km);
-    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  // with external faces: a sweep over the external faces, then one over the
+  // internal ones (the order in which the corrections reach dt)
+  const int phases[2] = {ghost != nullptr ? kFacesExternal : kFacesAll, kFacesInternal};
+  for (int ph = 0; ph < (ghost != nullptr ? 2 : 1); ++ph) {
+    for (int d = 0; d < 3; ++d) {
+      if (fused_nb)
+        hipLaunchKernelGGL(gh_mixed_face_pass_kernel<true>, dim3(count, 2, 10), dim3(threads),
+                           lds_fused, stream, d, el, list, nbr, u, params, inv_jac, dt, mtab, kf,
+                           km, ghost, phases[ph]);
+      else
+        hipLaunchKernelGGL(gh_mixed_face_pass_kernel<false>, dim3(count, 2, 10), dim3(threads),
+                           lds_split, stream, d, el, list, nbr, u, params, inv_jac, dt, mtab, kf,
+                           km, ghost, phases[ph]);
+      if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+    }
   }
   return SXD_OK;
 }

@@ -501,6 +501,26 @@ def scalar_wave_plane_wave_mixed_dirichlet(seed=0, elements=(4, 4, 4), lowest=3,
     return brick, u, gamma2, brick.inverse_jacobian(), ghost
 
 
+def gh_kerr_schild_mixed_dirichlet(seed=0, elements=(2, 2, 2), lowest=3, highest=6,
+                                   noise=1.0e-4, lower=(3.0, -4.0, -4.0),
+                                   upper=(11.0, 4.0, 4.0)):
+    """Mixed-p GH workload on a non-periodic brick whose outer faces are
+    closed by the Dirichlet analytic condition: the ghost state is the exact
+    Kerr-Schild solution at every point, u carries the seeded noise; gamma0 =
+    gamma2 = 1, gamma1 = -1.  Returns (brick, u, gammas, inv_jac, ghost) in the
+    ragged layout."""
+    ext = random_extents(int(np.prod(elements)), lowest, highest, seed)
+    brick = MixedBrick(lower, upper, elements, ext, periodic=False)
+    ghost = brick.ragged([gh_kerr_schild(brick.element_coordinates(e)[None])[0]
+                          for e in range(brick.num_elements)])
+    u = ghost.copy()
+    if noise:
+        u *= 1.0 + noise * np.random.default_rng(seed).standard_normal(u.shape)
+    gammas = brick.ragged([np.stack([np.ones(int(n)), -np.ones(int(n)), np.ones(int(n))])
+                           for n in brick.points])
+    return brick, u, gammas, brick.inverse_jacobian(), ghost
+
+
 def gh_kerr_schild_mixed(seed=0, elements=(10, 10, 10), lowest=4, highest=12, noise=1.0e-6,
                          lower=(3.0, -4.0, -4.0), upper=(11.0, 4.0, 4.0)):
     """Mixed-p GH workload: the Kerr-Schild state of gh_kerr_schild_brick on

@@ -304,6 +304,18 @@ def sw_rhs_mixed_dirichlet(extents, neighbors, u, gamma2, inv_jac, ghost):
     return dt
 
 
+def gh_rhs_mixed_dirichlet(extents, neighbors, u, gammas, inv_jac, ghost):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    dt = np.empty_like(u)
+    ext, ep = _i(extents)
+    nb, nbp = _i(neighbors)
+    _check(lib().sxo_gh_rhs_mixed_dirichlet(
+        nb.shape[0], ep, nbp, _d(u), _d(np.ascontiguousarray(gammas, dtype=np.float64)),
+        _d(np.ascontiguousarray(inv_jac, dtype=np.float64)),
+        _d(np.ascontiguousarray(ghost, dtype=np.float64)), _d(dt)))
+    return dt
+
+
 def sw_rhs_mixed(extents, neighbors, u, gamma2, inv_jac):
     """Ragged layout (element e's blocks at nc * offset_e)."""
     u, gamma2, inv_jac = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, gamma2, inv_jac))

@@ -254,6 +254,25 @@ def test_scalar_wave_rhs_dirichlet_mixed_p_matches_oracle(kernel):
     np.testing.assert_array_equal(a, host(pdom.scalar_wave_rhs(dev(u), dev(gamma2), dev(inv))))
 
 
+@pytest.mark.parametrize("kernel", ["mixed_batched", "mixed_per_extents"])
+def test_gh_rhs_dirichlet_mixed_p_matches_oracle(kernel):
+    """Mixed p (extents 3..6), Kerr-Schild + 1e-4 noise, the exact solution
+    as ghost on the outer faces, point-varying damping parameters."""
+    brick, u, gammas, inv, ghost = wl.gh_kerr_schild_mixed_dirichlet(
+        seed=7, elements=(2, 2, 2), lowest=3, highest=6, lower=(3.0, -2.0, 0.5),
+        upper=(7.0, 2.0, 4.5))
+    rng = np.random.default_rng(12)
+    gammas = gammas * rng.uniform(0.5, 1.5, gammas.shape)
+    ref = ol.gh_rhs_mixed_dirichlet(brick.extents, brick.neighbors, u, gammas, inv, ghost)
+    domain = dg.MixedDomain.from_brick(brick)
+    domain.set_kernel("gh", kernel)
+    got = host(domain.gh_rhs_dirichlet(dev(u), dev(gammas), dev(inv), dev(ghost)))
+    assert rel_err(got, ref) <= TOL
+    assert np.array_equal(got, ref)
+    with pytest.raises(dg.SpectreAmdError, match="external faces"):
+        domain.gh_rhs(dev(u), dev(gammas), dev(inv))
+
+
 def test_boundary_calls_validate_their_arguments():
     """Unknown condition types, missing coordinates, host ghost arrays and
     mixed-p domains are rejected before any launch."""

@@ -412,3 +412,14 @@ def test_mixed_dirichlet_converges_to_exact_time_derivative():
         errors.append(err)
     assert all(b < a * 0.2 for a, b in zip(errors, errors[1:]))
     assert errors[-1] < 1e-4
+
+
+def test_gh_mixed_dirichlet_with_equal_extents_is_the_uniform_dirichlet_rhs():
+    brick, u, gammas, inv, ghost = wl.gh_kerr_schild_dirichlet(
+        seed=4, elements=(2, 1, 2), extents=(4, 5, 3), lower=(3.0, -2.0, 0.5),
+        upper=(6.0, 1.0, 3.5), noise=1e-3)
+    uni = ol.gh_rhs_dirichlet(brick.extents, brick.neighbors, u, gammas, inv, ghost)
+    ext = np.tile(np.asarray(brick.extents, dtype=np.int32), (brick.num_elements, 1))
+    mixed = ol.gh_rhs_mixed_dirichlet(ext, brick.neighbors, u.reshape(-1), gammas.reshape(-1),
+                                      inv.reshape(-1), ghost.reshape(-1))
+    np.testing.assert_array_equal(mixed, uni.reshape(-1))
