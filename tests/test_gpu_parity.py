@@ -165,8 +165,10 @@ def test_scalar_wave_rhs_dirichlet_matches_oracle(kernel, elements, extents):
 
 @pytest.mark.parametrize("kernel,threads,elements,extents", [
     ("fused", 256, (2, 2, 2), (8, 8, 8)), ("fused", 512, (2, 1, 2), (8, 8, 8)),
-    ("fused", 256, (3, 2, 1), (5, 5, 5)), ("composite", 256, (2, 2, 2), (8, 8, 8)),
-    ("composite", 256, (2, 1, 2), (4, 6, 5))])
+    ("fused", 256, (3, 2, 1), (5, 5, 5)), ("fused", 512, (2, 2, 1), (7, 7, 7)),
+    ("fused", 256, (2, 1, 1), (6, 6, 6)), ("fused", 256, (3, 2, 2), (3, 3, 3)),
+    ("fused", 256, (2, 2, 2), (2, 2, 2)), ("composite", 256, (2, 2, 2), (8, 8, 8)),
+    ("composite", 256, (2, 1, 2), (4, 6, 5)), ("composite", 256, (1, 1, 1), (12, 3, 9))])
 def test_gh_rhs_dirichlet_matches_oracle(kernel, threads, elements, extents):
     """Kerr-Schild + 1e-4 noise with the exact solution as ghost on every
     outer face; point-varying damping parameters."""
@@ -188,7 +190,8 @@ def test_gh_rhs_dirichlet_matches_oracle(kernel, threads, elements, extents):
 @pytest.mark.parametrize("kind", ["sommerfeld", "bayliss_turkel"])
 @pytest.mark.parametrize("kernel,elements,extents", [
     ("fused", (3, 2, 2), (5, 5, 5)), ("composite", (3, 2, 2), (5, 5, 5)),
-    ("composite", (2, 2, 1), (4, 6, 5))])
+    ("composite", (2, 2, 1), (4, 6, 5)), ("fused", (1, 2, 1), (10, 10, 10)),
+    ("fused", (2, 2, 2), (2, 2, 2))])
 def test_scalar_wave_rhs_spherical_radiation_matches_oracle(kind, kernel, elements, extents):
     """Outer faces closed by SphericalRadiation of the interior trace: noisy
     plane wave on a brick off the origin, point-varying gamma2."""
@@ -212,7 +215,8 @@ def test_scalar_wave_rhs_spherical_radiation_matches_oracle(kind, kernel, elemen
                                   "second_order_bayliss_turkel"])
 @pytest.mark.parametrize("kernel,elements,extents", [
     ("fused", (3, 2, 2), (5, 5, 5)), ("composite", (3, 2, 2), (5, 5, 5)),
-    ("composite", (2, 2, 1), (4, 6, 5))])
+    ("composite", (2, 2, 1), (4, 6, 5)), ("fused", (1, 2, 1), (10, 10, 10)),
+    ("fused", (2, 2, 2), (2, 2, 2))])
 def test_scalar_wave_rhs_cpsr_matches_oracle(kind, kernel, elements, extents):
     """Outer faces closed by the constraint-preserving spherical radiation
     condition: noisy plane wave off the origin, point-varying gamma2."""
