@@ -162,7 +162,8 @@ int sxd_domain_destroy(sxd_domain* domain);
  * counts).  SXD_ERR_UNSUPPORTED if an element or face is too large for the
  * kernels' LDS plan (element Np up to 4096 and faces up to 12 x 12 fit).
  * A neighbour of -1 marks an external face, evaluated by
- * sxd_scalar_wave_rhs_dirichlet and sxd_gh_rhs_dirichlet only. */
+ * sxd_scalar_wave_rhs_dirichlet, sxd_scalar_wave_rhs_spherical_radiation and
+ * sxd_gh_rhs_dirichlet only. */
 int sxd_domain_create_mixed(sxd_domain** domain, int num_elements,
                             const int32_t* extents, const int32_t* neighbors);
 /* Total grid points of the domain (sum of Np_e). */
@@ -197,7 +198,8 @@ int sxd_scalar_wave_rhs_dirichlet(const sxd_domain* domain, const double* u,
  * pi = n^i phi_i (outward interior unit normal), plus psi / |x| for the
  * Bayliss-Turkel type; interior gamma2 and reversed unit normal as in
  * sxd_scalar_wave_rhs_dirichlet.  coords [E][3][Np]: inertial coordinates
- * (Bayliss-Turkel only; may be null for Sommerfeld). */
+ * (Bayliss-Turkel only; may be null for Sommerfeld).  Uniform and mixed-p
+ * domains (coords then in the ragged layout). */
 enum {
   SXD_SPHERICAL_RADIATION_SOMMERFELD = 1,
   SXD_SPHERICAL_RADIATION_BAYLISS_TURKEL = 2

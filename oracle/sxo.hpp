@@ -757,6 +757,30 @@ void sw_rhs_volume(int e_begin, int e_end, const int ext[3], const T* u,
 constexpr int kSwGhostArray = 0, kSwSommerfeld = 1, kSwBaylissTurkel = 2;
 constexpr int kSwCpsrSommerfeld = 3, kSwCpsrFirstOrderBt = 4, kSwCpsrSecondOrderBt = 5;
 
+// Exterior state ug of an external face point for the ghost kinds
+// (kSwGhostArray, kSwSommerfeld, kSwBaylissTurkel) from the interior trace up
+// and the outward interior unit normal n; ghost_at points at the point's first
+// ghost value (ghost array) or x coordinate (Bayliss-Turkel), blocks of stride.
+template <class T>
+inline void sw_ghost_state_point(T ug[5], int bc_kind, const T up[5], const T n[3],
+                                 const T* ghost_at, size_t stride) {
+  if (bc_kind == kSwGhostArray) {
+    for (int c = 0; c < 5; ++c) ug[c] = ghost_at[c * stride];
+    return;
+  }
+  ug[0] = up[0];
+  T pe = n[0] * up[2];
+  for (int i = 1; i < 3; ++i) pe = fma_(n[i], up[2 + i], pe);
+  if (bc_kind == kSwBaylissTurkel) {
+    T r = ghost_at[0] * ghost_at[0];
+    for (int i = 1; i < 3; ++i) r = fma_(ghost_at[i * stride], ghost_at[i * stride], r);
+    r = sqrt_(r);
+    pe = pe + up[0] / r;
+  }
+  ug[1] = pe;
+  for (int i = 0; i < 3; ++i) ug[2 + i] = up[2 + i];
+}
+
 // ConstraintPreservingSphericalRadiation::dg_time_derivative for a static mesh
 // (ScalarWave/BoundaryConditions/ConstraintPreservingSphericalRadiation.cpp:
 // 59-132): the dt Pi correction from the interior state u, its derivatives du
@@ -850,22 +874,9 @@ void sw_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
           if (en < 0) {
             // ghost exterior state at the boundary point
             T ug[5], ng[3];
-            if (bc_kind == kSwGhostArray) {
-              for (int c = 0; c < 5; ++c) ug[c] = ghost[(e * 5 + c) * np + p];
-            } else {
-              ug[0] = up[0];
-              T pe = n[0] * up[2];
-              for (int i = 1; i < 3; ++i) pe = fma_(n[i], up[2 + i], pe);
-              if (bc_kind == kSwBaylissTurkel) {
-                const T* x = ghost + e * 3 * np + p;
-                T r = x[0] * x[0];
-                for (int i = 1; i < 3; ++i) r = fma_(x[i * np], x[i * np], r);
-                r = sqrt_(r);
-                pe = pe + up[0] / r;
-              }
-              ug[1] = pe;
-              for (int i = 0; i < 3; ++i) ug[2 + i] = up[2 + i];
-            }
+            const size_t nblk = bc_kind == kSwGhostArray ? 5 : 3;
+            sw_ghost_state_point(ug, bc_kind, up, n,
+                                 ghost == nullptr ? nullptr : ghost + e * nblk * np + p, np);
             for (int i = 0; i < 3; ++i) ng[i] = n[i] * -1.0;
             sw_package_point(pk_ex, ug, gamma2[e * np + p], ng);
           } else {
@@ -1323,12 +1334,15 @@ inline void to_mortar(int npk, int na, int nb, int ma, int mb, const T* pk_own, 
 }
 
 // Full ScalarWave DG RHS on a mixed-p set of elements (ragged layout).
-// External faces (neighbour < 0) take the exterior state from `ghost`
-// (ragged [5][Np_e] per element, read at the face's own points) as in
-// sw_rhs_faces, with no mortar; they are corrected before the internal faces.
+// External faces (neighbour < 0) take the exterior state as in sw_rhs_faces
+// (bc_kind kSwGhostArray: ghost ragged [5][Np_e] per element, read at the
+// face's own points; kSwSommerfeld: from the interior trace; kSwBaylissTurkel:
+// ghost holds the ragged [3][Np_e] coordinates), with no mortar; they are
+// corrected before the internal faces.
 template <class T>
 void sw_rhs_mixed(int num_elements, const int* extents, const int* nbr, const T* u,
-                  const T* gamma2, const T* inv_j, T* dt, const T* ghost = nullptr) {
+                  const T* gamma2, const T* inv_j, T* dt, const T* ghost = nullptr,
+                  int bc_kind = kSwGhostArray) {
   const RaggedLayout L(num_elements, extents);
   for (int e = 0; e < num_elements; ++e) {
     const size_t np = L.np[e], off = L.off[e];
@@ -1354,15 +1368,15 @@ void sw_rhs_mixed(int num_elements, const int* extents, const int* nbr, const T*
         for (size_t f = 0; f < fi.pts.size(); ++f) {
           const size_t p = fi.pts[f];
           T up[5], ug[5], n[3], ng[3], mag, pk_in[16], pk_ex[16], b[5];
-          for (int c = 0; c < 5; ++c) {
-            up[c] = u[5 * off + c * np + p];
-            ug[c] = ghost[5 * off + c * np + p];
-          }
+          for (int c = 0; c < 5; ++c) up[c] = u[5 * off + c * np + p];
           for (int i = 0; i < 3; ++i) {
             n[i] = inv_j[9 * off + ij3(d, i) * np + p];
             if (side == 0) n[i] = n[i] * -1.0;
           }
           flat_normalize(n, &mag);
+          const size_t nblk = bc_kind == kSwGhostArray ? 5 : 3;
+          sw_ghost_state_point(ug, bc_kind, up, n,
+                               ghost == nullptr ? nullptr : ghost + nblk * off + p, np);
           sw_package_point(pk_in, up, gamma2[off + p], n);
           for (int i = 0; i < 3; ++i) ng[i] = n[i] * -1.0;
           sw_package_point(pk_ex, ug, gamma2[off + p], ng);

@@ -456,6 +456,20 @@ int sxo_sw_rhs_mixed_dirichlet(int num_elements, const int* extents, const int* 
   return 0;
 }
 
+// Mixed-p ScalarWave RHS with external faces closed by SphericalRadiation:
+// kind 1 Sommerfeld, 2 Bayliss-Turkel (coords ragged [3][Np_e], not read for
+// Sommerfeld).
+int sxo_sw_rhs_mixed_spherical_radiation(int num_elements, const int* extents, const int* nbr,
+                                         const double* u, const double* gamma2,
+                                         const double* inv_j, const double* coords, int kind,
+                                         double* dt) {
+  if (bad_ragged(num_elements, extents, nbr, true)) return -1;
+  if (kind != sxo::kSwSommerfeld && kind != sxo::kSwBaylissTurkel) return -1;
+  if (kind == sxo::kSwBaylissTurkel && coords == nullptr) return -1;
+  sxo::sw_rhs_mixed(num_elements, extents, nbr, u, gamma2, inv_j, dt, coords, kind);
+  return 0;
+}
+
 int sxo_sw_rhs_mixed(int num_elements, const int* extents, const int* nbr, const double* u,
                      const double* gamma2, const double* inv_j, double* dt) {
   if (bad_ragged(num_elements, extents, nbr)) return -1;

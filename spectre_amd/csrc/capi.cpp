@@ -539,7 +539,7 @@ int resolve_kernel(const sxd_domain* d, int system) {
 
 int run_mixed(sxd_domain* d, bool gh, int kernel, const double* u, const double* params,
               const double* inv_jac, double* dt, hipStream_t stream,
-              const double* ghost = nullptr) {
+              const double* ghost = nullptr, int bc_kind = kSwBcGhostArray) {
   double* scratch = nullptr;
   if (gh) {
     if (ensure_scratch(d, 150 * static_cast<size_t>(d->total_points), 0) != SXD_OK)
@@ -549,11 +549,12 @@ int run_mixed(sxd_domain* d, bool gh, int kernel, const double* u, const double*
   const int threads = d->threads[gh ? SXD_SYSTEM_GH : SXD_SYSTEM_SCALAR_WAVE];
   if (kernel == SXD_KERNEL_MIXED_BATCHED)
     return launch_mixed_rhs(gh, d->num_elements, d->d_list, d->d_el, d->d_nbr, d->np_max, d->kf,
-                            d->km, u, params, inv_jac, dt, scratch, threads, stream, ghost);
+                            d->km, u, params, inv_jac, dt, scratch, threads, stream, ghost,
+                            bc_kind);
   for (const auto& b : d->buckets) {
     const int rc = launch_mixed_rhs(gh, b.second, d->d_buckets + b.first, d->d_el, d->d_nbr,
                                     d->np_max, d->kf, d->km, u, params, inv_jac, dt, scratch,
-                                    threads, stream, ghost);
+                                    threads, stream, ghost, bc_kind);
     if (rc != SXD_OK) return rc;
   }
   return SXD_OK;
@@ -563,8 +564,8 @@ int run_sw(sxd_domain* d, int kernel, const double* u, const double* gamma2,
            const double* inv_jac, double* dt, hipStream_t stream,
            const double* ghost = nullptr, int bc_kind = kSwBcGhostArray) {
   if (d->mixed) {
-    if (bc_kind != kSwBcGhostArray) return SXD_ERR_UNSUPPORTED;
-    return run_mixed(d, false, kernel, u, gamma2, inv_jac, dt, stream, ghost);
+    if (bc_kind > kSwBcBaylissTurkel) return SXD_ERR_UNSUPPORTED;
+    return run_mixed(d, false, kernel, u, gamma2, inv_jac, dt, stream, ghost, bc_kind);
   }
   double* scratch = nullptr;
   if (kernel == SXD_KERNEL_COMPOSITE) {
@@ -640,8 +641,6 @@ int sxd_scalar_wave_rhs_spherical_radiation(const sxd_domain* domain, const doub
       type != SXD_SPHERICAL_RADIATION_BAYLISS_TURKEL)
     return fail(SXD_ERR_INVALID_ARGUMENT, "unknown spherical radiation type %d", type);
   if (type == SXD_SPHERICAL_RADIATION_BAYLISS_TURKEL) SXD_REQUIRE_DEVICE(coords);
-  if (domain->mixed)
-    return fail(SXD_ERR_UNSUPPORTED, "external faces need a uniform-extent domain");
   sxd_domain* d = const_cast<sxd_domain*>(domain);
   const int kind = type == SXD_SPHERICAL_RADIATION_SOMMERFELD ? kSwBcSommerfeld
                                                               : kSwBcBaylissTurkel;
@@ -665,7 +664,8 @@ int sxd_scalar_wave_rhs_constraint_preserving_spherical_radiation(
                 "unknown constraint-preserving spherical radiation type %d", type);
   if (type != SXD_CPSR_SOMMERFELD) SXD_REQUIRE_DEVICE(coords);
   if (domain->mixed)
-    return fail(SXD_ERR_UNSUPPORTED, "external faces need a uniform-extent domain");
+    return fail(SXD_ERR_UNSUPPORTED,
+                "the constraint-preserving condition needs a uniform-extent domain");
   sxd_domain* d = const_cast<sxd_domain*>(domain);
   const int kind = kSwBcCpsrSommerfeld + (type - SXD_CPSR_SOMMERFELD);
   return finish(run_sw(d, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,

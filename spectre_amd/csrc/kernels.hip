@@ -423,36 +423,6 @@ gh_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
   }
 }
 
-// Exterior state of an external ScalarWave face from the interior trace ui and
-// the outward interior unit normal n (SphericalRadiation::dg_ghost,
-// ScalarWave/BoundaryConditions/SphericalRadiation.cpp:62-100): psi and phi of
-// the interior, pi = n^i phi_i, plus psi / |x| for Bayliss-Turkel (x: the
-// point's inertial coordinates, blocks of stride np).  kSwBcGhostArray reads
-// the caller's ghost state instead (DirichletAnalytic).
-__device__ __forceinline__ void sw_ghost_state(double ue[5], int bc_kind, const double ui[5],
-                                               const double n[3], const double* ghost_at,
-                                               int64_t stride) {
-  if (bc_kind == kSwBcGhostArray) {
-#pragma unroll
-    for (int c = 0; c < 5; ++c) ue[c] = ghost_at[c * stride];
-    return;
-  }
-  ue[0] = ui[0];
-  double pe = n[0] * ui[2];
-  pe = fma(n[1], ui[3], pe);
-  pe = fma(n[2], ui[4], pe);
-  if (bc_kind == kSwBcBaylissTurkel) {
-    double r = ghost_at[0] * ghost_at[0];
-    r = fma(ghost_at[stride], ghost_at[stride], r);
-    r = fma(ghost_at[2 * stride], ghost_at[2 * stride], r);
-    r = sqrt(r);
-    pe = pe + ui[0] / r;
-  }
-  ue[1] = pe;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) ue[2 + i] = ui[2 + i];
-}
-
 // dt Pi correction of ConstraintPreservingSphericalRadiation::dg_time_derivative
 // for a static mesh (ScalarWave/BoundaryConditions/
 // ConstraintPreservingSphericalRadiation.cpp:59-132): from the interior state

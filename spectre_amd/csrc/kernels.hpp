@@ -80,11 +80,14 @@ int mixed_face_lds_bytes(int kf, int km);
 // elements in `list` (one workgroup each, `threads` threads: 64, 128, 256).
 // gh: params = gammas [3][Np_e], scratch_du = 150 x total points; otherwise
 // params = gamma2 [Np_e].  Faces whose neighbour is -1 take their exterior
-// state from ghost (ragged [nc][Np_e], read at the face's own points).
+// state by bc_kind: kSwBcGhostArray from ghost (ragged [nc][Np_e], read at the
+// face's own points); for ScalarWave also kSwBcSommerfeld (ghost unused) and
+// kSwBcBaylissTurkel (ghost = ragged [3][Np_e] coordinates).
 int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement* el,
                      const int32_t* nbr, int np_max, int kf, int km, const double* u,
                      const double* params, const double* inv_jac, double* dt, double* scratch_du,
-                     int threads, hipStream_t stream, const double* ghost = nullptr);
+                     int threads, hipStream_t stream, const double* ghost = nullptr,
+                     int bc_kind = kSwBcGhostArray);
 // Adjoint of the GH RHS on a mixed-p domain, linearised at u: ubar =
 // (d dt / d u)^T gbar over the work list; scratch_du = 150 x total points.
 int mixed_face_adjoint_lds_bytes(int kf, int km);

@@ -407,18 +407,21 @@ struct SwFacePlan {
 };
 
 // ScalarWave faces: one workgroup per element, its six faces in direction
-// order, external faces (neighbour -1) first.  An external face takes its
-// exterior state from ghost at its own points (own gamma2, reversed unit
-// normal; Dirichlet analytic condition) and needs no mortar.  Dynamic LDS:
-// SwFacePlan(kf, km).total doubles.
+// order, external faces (neighbour -1) first.  An external face forms its
+// exterior state at its own points by bc_kind (sw_ghost_state: the ghost
+// array for the Dirichlet analytic condition, ragged [5][Np_e]; the interior
+// trace for SphericalRadiation, ghost = ragged [3][Np_e] coordinates for
+// Bayliss-Turkel) with the own gamma2 and the reversed unit normal, and
+// needs no mortar.  Dynamic LDS: SwFacePlan(kf, km).total doubles.
 __global__ void __launch_bounds__(256)
 sw_mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restrict__ list,
                      const int32_t* __restrict__ nbr, const double* __restrict__ u,
                      const double* __restrict__ params, const double* __restrict__ inv_jac,
                      double* __restrict__ dt, const double* __restrict__ mortar_tab, int kf,
-                     int km, const double* __restrict__ ghost) {
+                     int km, const double* __restrict__ ghost, int bc_kind) {
   extern __shared__ double sm[];
   const SwFacePlan L(kf, km);
+  const int ghost_blocks = bc_kind == kSwBcGhostArray ? 5 : 3;
   const int t = threadIdx.x, T = blockDim.x;
   const int e = list[blockIdx.x];
   const MixedElement m = el[e];
@@ -434,16 +437,15 @@ sw_mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restr
         const int p = face_point(m, d, side, f % na, f / na);
         double ui[5], ug[5], n[3], ng[3], mag, pki[13], pke[13], bc[5];
 #pragma unroll
-        for (int c = 0; c < 5; ++c) {
-          ui[c] = u[5 * m.off + c * m.np + p];
-          ug[c] = ghost[5 * m.off + c * m.np + p];
-        }
+        for (int c = 0; c < 5; ++c) ui[c] = u[5 * m.off + c * m.np + p];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
           n[i] = inv_jac[9 * m.off + ij3(d, i) * m.np + p];
           if (side == 0) n[i] = n[i] * -1.0;
         }
         flat_normalize(n, &mag);
+        sw_ghost_state(ug, bc_kind, ui, n,
+                       ghost != nullptr ? ghost + ghost_blocks * m.off + p : nullptr, m.np);
         const double g2 = params[m.off + p];
         sw_package(pki, ui, g2, n);
 #pragma unroll
@@ -1137,8 +1139,10 @@ int mixed_face_lds_bytes(int kf, int km) { return FacePlan(kf, km, 13).total * 8
 int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement* el,
                      const int32_t* nbr, int np_max, int kf, int km, const double* u,
                      const double* params, const double* inv_jac, double* dt, double* scratch_du,
-                     int threads, hipStream_t stream, const double* ghost) {
+                     int threads, hipStream_t stream, const double* ghost, int bc_kind) {
   if (count == 0) return SXD_OK;
+  if (gh && bc_kind != kSwBcGhostArray) return SXD_ERR_INVALID_ARGUMENT;
+  if (bc_kind < kSwBcGhostArray || bc_kind > kSwBcBaylissTurkel) return SXD_ERR_UNSUPPORTED;
   const double* dtab = device_diff_matrices();
   const double* mtab = device_mortar_matrices();
   if (dtab == nullptr || mtab == nullptr) return SXD_ERR_DEVICE;
@@ -1153,7 +1157,7 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
                        el, list, u, params, inv_jac, dt, dtab);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
     hipLaunchKernelGGL(sw_mixed_face_kernel, dim3(count), dim3(threads), sw_face_lds, stream,
-                       el, list, nbr, u, params, inv_jac, dt, mtab, kf, km, ghost);
+                       el, list, nbr, u, params, inv_jac, dt, mtab, kf, km, ghost, bc_kind);
     return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
   }
   if (scratch_du == nullptr) return SXD_ERR_INVALID_ARGUMENT;

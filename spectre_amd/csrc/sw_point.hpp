@@ -3,6 +3,7 @@
 #pragma once
 
 #include "device_common.hpp"
+#include "kernels.hpp"
 
 namespace sxd {
 
@@ -70,6 +71,36 @@ __device__ __forceinline__ void sw_boundary_terms(double bc[5], const double in[
   for (int d = 0; d < 3; ++d)
     bc[2 + d] = 0.5 * (wplus_e * ex[6 + d] - wminus_e * ex[9 + d]) + wzero_e * ex[1 + d] -
                 0.5 * (wplus_i * in[6 + d] - wminus_i * in[9 + d]) - wzero_i * in[1 + d];
+}
+
+// Exterior state of an external ScalarWave face from the interior trace ui and
+// the outward interior unit normal n (SphericalRadiation::dg_ghost,
+// ScalarWave/BoundaryConditions/SphericalRadiation.cpp:62-100): psi and phi of
+// the interior, pi = n^i phi_i, plus psi / |x| for Bayliss-Turkel (x: the
+// point's inertial coordinates, blocks of stride np).  kSwBcGhostArray reads
+// the caller's ghost state instead (DirichletAnalytic).
+__device__ __forceinline__ void sw_ghost_state(double ue[5], int bc_kind, const double ui[5],
+                                               const double n[3], const double* ghost_at,
+                                               int64_t stride) {
+  if (bc_kind == kSwBcGhostArray) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) ue[c] = ghost_at[c * stride];
+    return;
+  }
+  ue[0] = ui[0];
+  double pe = n[0] * ui[2];
+  pe = fma(n[1], ui[3], pe);
+  pe = fma(n[2], ui[4], pe);
+  if (bc_kind == kSwBcBaylissTurkel) {
+    double r = ghost_at[0] * ghost_at[0];
+    r = fma(ghost_at[stride], ghost_at[stride], r);
+    r = fma(ghost_at[2 * stride], ghost_at[2 * stride], r);
+    r = sqrt(r);
+    pe = pe + ui[0] / r;
+  }
+  ue[1] = pe;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) ue[2 + i] = ui[2 + i];
 }
 
 }  // namespace sxd

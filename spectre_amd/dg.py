@@ -236,7 +236,8 @@ class Domain:
                                             coords=None, out=None):
         """scalar_wave_rhs with external faces closed by SphericalRadiation
         (ScalarWave/BoundaryConditions/SphericalRadiation.cpp:62-100): kind
-        "sommerfeld" or "bayliss_turkel" (needs coords [E, 3, Np])."""
+        "sommerfeld" or "bayliss_turkel" (needs coords [E, 3, Np]; ragged
+        [3 * total points] on a MixedDomain)."""
         kinds = {"sommerfeld": 1, "bayliss_turkel": 2}
         if kind not in kinds:
             raise ValueError(f"kind must be one of {sorted(kinds)}")

@@ -267,6 +267,10 @@ class MixedBrick:
         """Flatten a list of [nc][Np_e] arrays into the ragged layout."""
         return np.ascontiguousarray(np.concatenate([np.asarray(a).ravel() for a in per_element]))
 
+    def coordinates(self):
+        """Inertial coordinates in the ragged layout ([3][Np_e] per element)."""
+        return self.ragged([self.element_coordinates(e) for e in range(self.num_elements)])
+
     def split(self, flat, nc):
         """The list of [nc][Np_e] blocks of a ragged array."""
         return [flat[nc * self.offsets[e]:nc * self.offsets[e + 1]].reshape(nc, -1)
@@ -484,14 +488,16 @@ def scalar_wave_plane_wave_mixed(seed=0, elements=(10, 10, 10), lowest=4, highes
 
 
 def scalar_wave_plane_wave_mixed_dirichlet(seed=0, elements=(4, 4, 4), lowest=3, highest=8,
-                                           t=0.0, noise=0.0):
-    """Mixed-p ScalarWave workload on the non-periodic brick [-1,1]^3 whose
-    outer faces are closed by the Dirichlet analytic condition: the ghost
+                                           t=0.0, noise=0.0, lower=(-1.0, -1.0, -1.0),
+                                           upper=(1.0, 1.0, 1.0)):
+    """Mixed-p ScalarWave workload on the non-periodic brick [lower, upper]
+    (default [-1,1]^3) whose outer faces are closed by the Dirichlet analytic
+    condition: the ghost
     state is the plane wave of plane_wave(seed, t) at every point; u is that
     wave plus seeded Gaussian noise of absolute size `noise`; gamma2 = 1.
     Returns (brick, u, gamma2, inv_jac, ghost) in the ragged layout."""
     ext = random_extents(int(np.prod(elements)), lowest, highest, seed)
-    brick = MixedBrick((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), elements, ext, periodic=False)
+    brick = MixedBrick(lower, upper, elements, ext, periodic=False)
     ghost = brick.ragged([plane_wave(brick.element_coordinates(e)[None], seed, t)[0][0]
                           for e in range(brick.num_elements)])
     u = ghost.copy()

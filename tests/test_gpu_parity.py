@@ -258,6 +258,31 @@ def test_scalar_wave_rhs_dirichlet_mixed_p_matches_oracle(kernel):
     np.testing.assert_array_equal(a, host(pdom.scalar_wave_rhs(dev(u), dev(gamma2), dev(inv))))
 
 
+@pytest.mark.parametrize("kind", ["sommerfeld", "bayliss_turkel"])
+@pytest.mark.parametrize("kernel", ["mixed_batched", "mixed_per_extents"])
+def test_scalar_wave_rhs_spherical_radiation_mixed_p_matches_oracle(kernel, kind):
+    """Mixed p (extents 3..8) on a brick off the origin, outer faces closed by
+    SphericalRadiation of the interior trace; noisy data, point-varying
+    gamma2."""
+    brick, u, _, inv, _ = wl.scalar_wave_plane_wave_mixed_dirichlet(
+        seed=8, elements=(3, 2, 2), lowest=3, highest=8, t=0.2, noise=1e-2,
+        lower=(1.0, -1.0, 0.5), upper=(4.0, 1.0, 2.5))
+    gamma2 = np.random.default_rng(10).uniform(0, 2, brick.total_points)
+    coords = brick.coordinates()
+    code = 1 if kind == "sommerfeld" else 2
+    ref = ol.sw_rhs_mixed_spherical_radiation(brick.extents, brick.neighbors, u, gamma2, inv,
+                                              coords, code)
+    domain = dg.MixedDomain.from_brick(brick)
+    domain.set_kernel("scalar_wave", kernel)
+    got = host(domain.scalar_wave_rhs_spherical_radiation(
+        dev(u), dev(gamma2), dev(inv), kind, dev(coords) if code == 2 else None))
+    assert rel_err(got, ref) <= TOL
+    assert np.array_equal(got, ref)
+    with pytest.raises(dg.SpectreAmdError, match="uniform-extent"):
+        domain.scalar_wave_rhs_constraint_preserving_spherical_radiation(
+            dev(u), dev(gamma2), dev(inv), "sommerfeld")
+
+
 @pytest.mark.parametrize("kernel", ["mixed_batched", "mixed_per_extents"])
 def test_gh_rhs_dirichlet_mixed_p_matches_oracle(kernel):
     """Mixed p (extents 3..6), Kerr-Schild + 1e-4 noise, the exact solution

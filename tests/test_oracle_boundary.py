@@ -414,6 +414,40 @@ def test_mixed_dirichlet_converges_to_exact_time_derivative():
     assert errors[-1] < 1e-4
 
 
+@pytest.mark.parametrize("kind", [1, 2])
+def test_mixed_spherical_radiation_with_equal_extents_is_the_uniform_rhs(kind):
+    """kind 1 Sommerfeld, 2 Bayliss-Turkel: the mixed-p path forms the same
+    exterior state at the same points as the uniform one."""
+    brick, u, _, inv, _ = wl.scalar_wave_plane_wave_dirichlet(
+        seed=6, elements=(2, 1, 2), extents=(4, 6, 5), t=0.1, noise=1e-2,
+        lower=(1.0, -1.0, 0.5), upper=(3.0, 1.0, 2.5))
+    gamma2 = np.random.default_rng(5).uniform(0, 2, (brick.num_elements, brick.num_points))
+    coords = brick.coordinates()
+    uni = ol.sw_rhs_spherical_radiation(brick.extents, brick.neighbors, u, gamma2, inv, coords,
+                                        kind)
+    ext = np.tile(np.asarray(brick.extents, dtype=np.int32), (brick.num_elements, 1))
+    mixed = ol.sw_rhs_mixed_spherical_radiation(ext, brick.neighbors, u.reshape(-1),
+                                                gamma2.reshape(-1), inv.reshape(-1),
+                                                coords.reshape(-1), kind)
+    np.testing.assert_array_equal(mixed, uni.reshape(-1))
+
+
+def test_mixed_spherical_radiation_differs_from_dirichlet_only_on_external_faces():
+    """Sommerfeld and Dirichlet closures of the same mixed-p state agree at
+    every point off the outer faces of the brick; on them they differ."""
+    brick, u, gamma2, inv, ghost = wl.scalar_wave_plane_wave_mixed_dirichlet(
+        seed=5, elements=(2, 2, 2), lowest=3, highest=6, t=0.2, noise=1e-2)
+    sr = ol.sw_rhs_mixed_spherical_radiation(brick.extents, brick.neighbors, u, gamma2, inv,
+                                             brick.coordinates(), 1)
+    dd = ol.sw_rhs_mixed_dirichlet(brick.extents, brick.neighbors, u, gamma2, inv, ghost)
+    x = brick.coordinates()
+    for e, (a, b) in enumerate(zip(brick.split(sr, 5), brick.split(dd, 5))):
+        xe = brick.split(x, 3)[e]
+        outer = np.any(np.isclose(np.abs(xe), 1.0, atol=1e-14), axis=0)
+        np.testing.assert_array_equal(a[:, ~outer], b[:, ~outer])
+        assert np.any(a[:, outer] != b[:, outer])
+
+
 def test_gh_mixed_dirichlet_with_equal_extents_is_the_uniform_dirichlet_rhs():
     brick, u, gammas, inv, ghost = wl.gh_kerr_schild_dirichlet(
         seed=4, elements=(2, 1, 2), extents=(4, 5, 3), lower=(3.0, -2.0, 0.5),
