@@ -162,7 +162,8 @@ int sxd_domain_destroy(sxd_domain* domain);
  * counts).  SXD_ERR_UNSUPPORTED if an element or face is too large for the
  * kernels' LDS plan (element Np up to 4096 and faces up to 12 x 12 fit).
  * A neighbour of -1 marks an external face, evaluated by
- * sxd_scalar_wave_rhs_dirichlet, sxd_scalar_wave_rhs_spherical_radiation and
+ * sxd_scalar_wave_rhs_dirichlet, sxd_scalar_wave_rhs_spherical_radiation,
+ * sxd_scalar_wave_rhs_constraint_preserving_spherical_radiation and
  * sxd_gh_rhs_dirichlet only. */
 int sxd_domain_create_mixed(sxd_domain** domain, int num_elements,
                             const int32_t* extents, const int32_t* neighbors);
@@ -219,7 +220,8 @@ int sxd_scalar_wave_rhs_spherical_radiation(const sxd_domain* domain,
  * term; at their points dt Pi gains the correction built from the interior
  * state, its derivatives, the outward unit normal and gamma2 (Sommerfeld,
  * first- or second-order Bayliss-Turkel, the latter two with 1/|x|).
- * coords [E][3][Np] (Bayliss-Turkel types; may be null for Sommerfeld). */
+ * coords [E][3][Np] (Bayliss-Turkel types; may be null for Sommerfeld).
+ * Uniform and mixed-p domains (coords then in the ragged layout). */
 enum {
   SXD_CPSR_SOMMERFELD = 1,
   SXD_CPSR_FIRST_ORDER_BAYLISS_TURKEL = 2,

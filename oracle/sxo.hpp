@@ -1337,8 +1337,10 @@ inline void to_mortar(int npk, int na, int nb, int ma, int mb, const T* pk_own, 
 // External faces (neighbour < 0) take the exterior state as in sw_rhs_faces
 // (bc_kind kSwGhostArray: ghost ragged [5][Np_e] per element, read at the
 // face's own points; kSwSommerfeld: from the interior trace; kSwBaylissTurkel:
-// ghost holds the ragged [3][Np_e] coordinates), with no mortar; they are
-// corrected before the internal faces.
+// ghost holds the ragged [3][Np_e] coordinates), with no mortar, or, for the
+// kSwCpsr* kinds, no upwind term and the dt Pi correction of
+// sw_cpsr_correction_point (ghost: the coordinates); they are corrected before
+// the internal faces.
 template <class T>
 void sw_rhs_mixed(int num_elements, const int* extents, const int* nbr, const T* u,
                   const T* gamma2, const T* inv_j, T* dt, const T* ghost = nullptr,
@@ -1356,13 +1358,39 @@ void sw_rhs_mixed(int num_elements, const int* extents, const int* nbr, const T*
       for (int c = 0; c < 5; ++c) dt[5 * off + c * np + p] = dtp[c];
     }
   }
+  const bool cpsr = bc_kind >= kSwCpsrSommerfeld;
+  std::vector<T> du_e;
   for (int e = 0; e < num_elements; ++e) {
     const size_t np = L.np[e], off = L.off[e];
+    if (cpsr) {
+      du_e.assign(15 * np, T(0.0));
+      partial_derivatives(u + 5 * off, 5, L.ext[e].data(), inv_j + 9 * off, du_e.data());
+    }
     for (int step = 0; step < 12; ++step) {
       const int dir = step % 6;
       const int en = nbr[6 * e + dir];
       if ((en < 0) != (step < 6)) continue;
       const int d = dir / 2, side = dir % 2;
+      if (en < 0 && cpsr) {
+        const Face fi = make_face(L.ext[e].data(), d, side);
+        for (size_t f = 0; f < fi.pts.size(); ++f) {
+          const size_t p = fi.pts[f];
+          T up[5], dup[15], n[3], mag;
+          for (int c = 0; c < 5; ++c) up[c] = u[5 * off + c * np + p];
+          for (int c = 0; c < 15; ++c) dup[c] = du_e[c * np + p];
+          for (int i = 0; i < 3; ++i) {
+            n[i] = inv_j[9 * off + ij3(d, i) * np + p];
+            if (side == 0) n[i] = n[i] * -1.0;
+          }
+          flat_normalize(n, &mag);
+          const T corr = sw_cpsr_correction_point(
+              bc_kind, up, dup, n, gamma2[off + p],
+              ghost == nullptr ? nullptr : ghost + 3 * off + p, np);
+          T& target = dt[5 * off + np + p];
+          target = target + corr;
+        }
+        continue;
+      }
       if (en < 0) {
         const Face fi = make_face(L.ext[e].data(), d, side);
         for (size_t f = 0; f < fi.pts.size(); ++f) {

@@ -470,6 +470,20 @@ int sxo_sw_rhs_mixed_spherical_radiation(int num_elements, const int* extents, c
   return 0;
 }
 
+// Mixed-p ScalarWave RHS with external faces closed by the constraint-
+// preserving spherical radiation condition: kind 1 Sommerfeld, 2 first-order
+// and 3 second-order Bayliss-Turkel (coords ragged [3][Np_e], not read for
+// Sommerfeld).
+int sxo_sw_rhs_mixed_cpsr(int num_elements, const int* extents, const int* nbr,
+                          const double* u, const double* gamma2, const double* inv_j,
+                          const double* coords, int kind, double* dt) {
+  if (bad_ragged(num_elements, extents, nbr, true)) return -1;
+  if (kind < 1 || kind > 3 || (kind > 1 && coords == nullptr)) return -1;
+  sxo::sw_rhs_mixed(num_elements, extents, nbr, u, gamma2, inv_j, dt, coords,
+                    sxo::kSwCpsrSommerfeld + kind - 1);
+  return 0;
+}
+
 int sxo_sw_rhs_mixed(int num_elements, const int* extents, const int* nbr, const double* u,
                      const double* gamma2, const double* inv_j, double* dt) {
   if (bad_ragged(num_elements, extents, nbr)) return -1;

@@ -564,7 +564,6 @@ int run_sw(sxd_domain* d, int kernel, const double* u, const double* gamma2,
            const double* inv_jac, double* dt, hipStream_t stream,
            const double* ghost = nullptr, int bc_kind = kSwBcGhostArray) {
   if (d->mixed) {
-    if (bc_kind > kSwBcBaylissTurkel) return SXD_ERR_UNSUPPORTED;
     return run_mixed(d, false, kernel, u, gamma2, inv_jac, dt, stream, ghost, bc_kind);
   }
   double* scratch = nullptr;
@@ -663,9 +662,6 @@ int sxd_scalar_wave_rhs_constraint_preserving_spherical_radiation(
     return fail(SXD_ERR_INVALID_ARGUMENT,
                 "unknown constraint-preserving spherical radiation type %d", type);
   if (type != SXD_CPSR_SOMMERFELD) SXD_REQUIRE_DEVICE(coords);
-  if (domain->mixed)
-    return fail(SXD_ERR_UNSUPPORTED,
-                "the constraint-preserving condition needs a uniform-extent domain");
   sxd_domain* d = const_cast<sxd_domain*>(domain);
   const int kind = kSwBcCpsrSommerfeld + (type - SXD_CPSR_SOMMERFELD);
   return finish(run_sw(d, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,

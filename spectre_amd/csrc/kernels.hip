@@ -423,47 +423,6 @@ gh_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,
   }
 }
 
-// dt Pi correction of ConstraintPreservingSphericalRadiation::dg_time_derivative
-// for a static mesh (ScalarWave/BoundaryConditions/
-// ConstraintPreservingSphericalRadiation.cpp:59-132): from the interior state
-// u, its inertial derivatives du (du[3 c + i] = d_i of component c), the
-// outward unit normal n, gamma2 and, for the Bayliss-Turkel types, the
-// coordinates x (blocks of stride np).
-__device__ __forceinline__ double sw_cpsr_correction(int kind, const double u[5],
-                                                     const double du[15], const double n[3],
-                                                     double gamma2, const double* x,
-                                                     int64_t stride) {
-  // d_i Phi_j = du[3 (2 + j) + i]
-  double c = du[6];
-  c = c + du[10];
-  c = c + du[14];
-  if (kind == kSwBcCpsrSommerfeld) {
-#pragma unroll
-    for (int i = 0; i < 3; ++i) c = fma(n[i], gamma2 * (du[i] - u[2 + i]) - du[3 + i], c);
-    return c;
-  }
-  double r2 = x[0] * x[0];
-  r2 = fma(x[stride], x[stride], r2);
-  r2 = fma(x[2 * stride], x[2 * stride], r2);
-  const double inv_r = 1.0 / sqrt(r2);
-  if (kind == kSwBcCpsrFirstOrderBt) {
-    c = fma(-inv_r, u[1], c);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) c = fma(n[i], gamma2 * (du[i] - u[2 + i]) - du[3 + i], c);
-    return c;
-  }
-  c = fma(-(2.0 * inv_r), fma(-inv_r, u[0], 2.0 * u[1]), c);
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    double w = fma(-2.0, du[3 + i], gamma2 * (du[i] - u[2 + i]));
-    w = fma(4.0 * inv_r, u[2 + i], w);
-    c = fma(n[i], w, c);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) c = fma(n[i] * n[j], du[3 * (2 + j) + i], c);
-  }
-  return c;
-}
-
 // Composite SW face pass (any extents), same scheme as the GH one.
 __global__ void __launch_bounds__(256)
 sw_face_pass_kernel(int d, int n0, int n1, int n2, int num_elements,

@@ -73,14 +73,22 @@ __device__ __forceinline__ void logical_derivs(const double* sc, const MixedElem
 }
 
 // ---------------------------------------------------------------------------
-// ScalarWave volume terms.  Dynamic LDS: 5 Np_max doubles.
+// ScalarWave volume terms.  Dynamic LDS: 5 Np_max doubles.  CPSR: external
+// faces (neighbour -1) are closed by the constraint-preserving spherical
+// radiation condition of type bc_kind (kSwBcCpsr*; coords = ragged [3][Np_e]
+// for the Bayliss-Turkel types), a time-derivative condition: a point on such
+// faces adds their dt Pi corrections, in direction order, to its volume terms
+// (the face kernel then skips those faces).
+template <bool CPSR>
 __global__ void __launch_bounds__(256)
 sw_mixed_volume_kernel(const MixedElement* __restrict__ el, const int32_t* __restrict__ list,
                        const double* __restrict__ u, const double* __restrict__ gamma2,
                        const double* __restrict__ inv_jac, double* __restrict__ dt,
-                       const double* __restrict__ dtab) {
+                       const double* __restrict__ dtab, const int32_t* __restrict__ nbr,
+                       int bc_kind, const double* __restrict__ coords) {
   extern __shared__ double s_u[];
-  const MixedElement m = el[list[blockIdx.x]];
+  const int e = list[blockIdx.x];
+  const MixedElement m = el[e];
   const int np = m.np;
   const double* ue = u + 5 * m.off;
   for (int p = threadIdx.x; p < 5 * np; p += blockDim.x) s_u[p] = ue[p];
@@ -107,6 +115,24 @@ sw_mixed_volume_kernel(const MixedElement* __restrict__ el, const int32_t* __res
     }
     double dtp[5];
     sw_time_derivative(dtp, up, dup, gamma2[m.off + p]);
+    if constexpr (CPSR) {
+      const int at[3] = {i, j, k};
+      for (int dir = 0; dir < 6; ++dir) {
+        const int d = dir >> 1, side = dir & 1;
+        if (at[d] != (side == 0 ? 0 : m.n[d] - 1) || nbr[6 * e + dir] >= 0) continue;
+        double n[3], mag;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          n[a] = ij[ij3(d, a) * np + p];
+          if (side == 0) n[a] = n[a] * -1.0;
+        }
+        flat_normalize(n, &mag);
+        dtp[1] = dtp[1] + sw_cpsr_correction(bc_kind, up, dup, n, gamma2[m.off + p],
+                                             coords != nullptr ? coords + 3 * m.off + p
+                                                               : nullptr,
+                                             np);
+      }
+    }
 #pragma unroll
     for (int c = 0; c < 5; ++c) dt[5 * m.off + c * np + p] = dtp[c];
   }
@@ -412,7 +438,8 @@ struct SwFacePlan {
 // array for the Dirichlet analytic condition, ragged [5][Np_e]; the interior
 // trace for SphericalRadiation, ghost = ragged [3][Np_e] coordinates for
 // Bayliss-Turkel) with the own gamma2 and the reversed unit normal, and
-// needs no mortar.  Dynamic LDS: SwFacePlan(kf, km).total doubles.
+// needs no mortar; for the kSwBcCpsr* kinds it is skipped (the volume kernel
+// applied the condition).  Dynamic LDS: SwFacePlan(kf, km).total doubles.
 __global__ void __launch_bounds__(256)
 sw_mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restrict__ list,
                      const int32_t* __restrict__ nbr, const double* __restrict__ u,
@@ -431,6 +458,8 @@ sw_mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restr
     if ((en < 0) != (step < 6)) continue;  // the same for the whole workgroup
     const int d = dir >> 1, side = dir & 1;
     if (en < 0) {
+      // the constraint-preserving condition was applied with the volume terms
+      if (bc_kind >= kSwBcCpsrSommerfeld) continue;  // uniform over the workgroup
       __syncthreads();  // the previous face's dt updates are done
       const int na = d == 0 ? m.n[1] : m.n[0], nb = d == 2 ? m.n[1] : m.n[2];
       for (int f = t; f < na * nb; f += T) {
@@ -1113,7 +1142,9 @@ bool enable_large_lds() {
   if (static_cast<int>(done.size()) <= dev) done.resize(dev + 1, false);
   if (done[dev]) return true;
   const int lds_max = 160 * 1024;
-  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&sw_mixed_volume_kernel),
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&sw_mixed_volume_kernel<false>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&sw_mixed_volume_kernel<true>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
       hipFuncSetAttribute(reinterpret_cast<const void*>(&sw_mixed_face_kernel),
                           hipFuncAttributeMaxDynamicSharedMemorySize, lds_max) != hipSuccess ||
@@ -1142,7 +1173,7 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
                      int threads, hipStream_t stream, const double* ghost, int bc_kind) {
   if (count == 0) return SXD_OK;
   if (gh && bc_kind != kSwBcGhostArray) return SXD_ERR_INVALID_ARGUMENT;
-  if (bc_kind < kSwBcGhostArray || bc_kind > kSwBcBaylissTurkel) return SXD_ERR_UNSUPPORTED;
+  if (bc_kind < kSwBcGhostArray || bc_kind > kSwBcCpsrSecondOrderBt) return SXD_ERR_UNSUPPORTED;
   const double* dtab = device_diff_matrices();
   const double* mtab = device_mortar_matrices();
   if (dtab == nullptr || mtab == nullptr) return SXD_ERR_DEVICE;
@@ -1153,8 +1184,14 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
   if (!gh) {
     const int sw_face_lds = SwFacePlan(kf, km).total * 8;
     if (5 * np_max * 8 > 160 * 1024) return SXD_ERR_UNSUPPORTED;
-    hipLaunchKernelGGL(sw_mixed_volume_kernel, dim3(count), dim3(threads), 5 * np_max * 8, stream,
-                       el, list, u, params, inv_jac, dt, dtab);
+    if (bc_kind >= kSwBcCpsrSommerfeld)
+      hipLaunchKernelGGL(sw_mixed_volume_kernel<true>, dim3(count), dim3(threads),
+                         5 * np_max * 8, stream, el, list, u, params, inv_jac, dt, dtab, nbr,
+                         bc_kind, ghost);
+    else
+      hipLaunchKernelGGL(sw_mixed_volume_kernel<false>, dim3(count), dim3(threads),
+                         5 * np_max * 8, stream, el, list, u, params, inv_jac, dt, dtab, nbr,
+                         bc_kind, nullptr);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
     hipLaunchKernelGGL(sw_mixed_face_kernel, dim3(count), dim3(threads), sw_face_lds, stream,
                        el, list, nbr, u, params, inv_jac, dt, mtab, kf, km, ghost, bc_kind);

@@ -262,7 +262,8 @@ class Domain:
         ConstraintPreservingSphericalRadiation time-derivative condition
         (ConstraintPreservingSphericalRadiation.cpp:59-132): kind
         "sommerfeld", "first_order_bayliss_turkel" or
-        "second_order_bayliss_turkel" (the latter two need coords [E, 3, Np])."""
+        "second_order_bayliss_turkel" (the latter two need coords [E, 3, Np];
+        ragged [3 * total points] on a MixedDomain)."""
         kinds = {"sommerfeld": 1, "first_order_bayliss_turkel": 2,
                  "second_order_bayliss_turkel": 3}
         if kind not in kinds:

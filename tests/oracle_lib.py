@@ -317,6 +317,20 @@ def sw_rhs_mixed_spherical_radiation(extents, neighbors, u, gamma2, inv_jac, coo
     return dt
 
 
+def sw_rhs_mixed_cpsr(extents, neighbors, u, gamma2, inv_jac, coords, kind):
+    """Ragged layout; kind 1 Sommerfeld, 2 first-order, 3 second-order
+    Bayliss-Turkel (coords ragged [3][Np_e])."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    dt = np.empty_like(u)
+    ext, ep = _i(extents)
+    nb, nbp = _i(neighbors)
+    _check(lib().sxo_sw_rhs_mixed_cpsr(
+        nb.shape[0], ep, nbp, _d(u), _d(np.ascontiguousarray(gamma2, dtype=np.float64)),
+        _d(np.ascontiguousarray(inv_jac, dtype=np.float64)),
+        _d(np.ascontiguousarray(coords, dtype=np.float64)), int(kind), _d(dt)))
+    return dt
+
+
 def gh_rhs_mixed_dirichlet(extents, neighbors, u, gammas, inv_jac, ghost):
     u = np.ascontiguousarray(u, dtype=np.float64)
     dt = np.empty_like(u)

@@ -278,9 +278,29 @@ def test_scalar_wave_rhs_spherical_radiation_mixed_p_matches_oracle(kernel, kind
         dev(u), dev(gamma2), dev(inv), kind, dev(coords) if code == 2 else None))
     assert rel_err(got, ref) <= TOL
     assert np.array_equal(got, ref)
-    with pytest.raises(dg.SpectreAmdError, match="uniform-extent"):
-        domain.scalar_wave_rhs_constraint_preserving_spherical_radiation(
-            dev(u), dev(gamma2), dev(inv), "sommerfeld")
+
+
+@pytest.mark.parametrize("kind", ["sommerfeld", "first_order_bayliss_turkel",
+                                  "second_order_bayliss_turkel"])
+@pytest.mark.parametrize("kernel", ["mixed_batched", "mixed_per_extents"])
+def test_scalar_wave_rhs_cpsr_mixed_p_matches_oracle(kernel, kind):
+    """Mixed p (extents 3..8) off the origin, outer faces closed by the
+    constraint-preserving spherical radiation condition; noisy data,
+    point-varying gamma2."""
+    brick, u, _, inv, _ = wl.scalar_wave_plane_wave_mixed_dirichlet(
+        seed=9, elements=(2, 3, 2), lowest=3, highest=8, t=0.1, noise=1e-2,
+        lower=(1.0, -1.5, 0.5), upper=(3.0, 1.5, 2.5))
+    gamma2 = np.random.default_rng(11).uniform(0, 2, brick.total_points)
+    coords = brick.coordinates()
+    code = {"sommerfeld": 1, "first_order_bayliss_turkel": 2,
+            "second_order_bayliss_turkel": 3}[kind]
+    ref = ol.sw_rhs_mixed_cpsr(brick.extents, brick.neighbors, u, gamma2, inv, coords, code)
+    domain = dg.MixedDomain.from_brick(brick)
+    domain.set_kernel("scalar_wave", kernel)
+    got = host(domain.scalar_wave_rhs_constraint_preserving_spherical_radiation(
+        dev(u), dev(gamma2), dev(inv), kind, dev(coords) if code > 1 else None))
+    assert rel_err(got, ref) <= TOL
+    assert np.array_equal(got, ref)
 
 
 @pytest.mark.parametrize("kernel", ["mixed_batched", "mixed_per_extents"])

@@ -432,6 +432,21 @@ def test_mixed_spherical_radiation_with_equal_extents_is_the_uniform_rhs(kind):
     np.testing.assert_array_equal(mixed, uni.reshape(-1))
 
 
+@pytest.mark.parametrize("kind", [1, 2, 3])
+def test_mixed_cpsr_with_equal_extents_is_the_uniform_rhs(kind):
+    """kind 1 Sommerfeld, 2 first-order, 3 second-order Bayliss-Turkel."""
+    brick, u, _, inv, _ = wl.scalar_wave_plane_wave_dirichlet(
+        seed=7, elements=(1, 2, 2), extents=(5, 3, 6), t=0.1, noise=1e-2,
+        lower=(1.0, -1.0, 0.5), upper=(3.0, 1.0, 2.5))
+    gamma2 = np.random.default_rng(6).uniform(0, 2, (brick.num_elements, brick.num_points))
+    coords = brick.coordinates()
+    uni = ol.sw_rhs_cpsr(brick.extents, brick.neighbors, u, gamma2, inv, coords, kind)
+    ext = np.tile(np.asarray(brick.extents, dtype=np.int32), (brick.num_elements, 1))
+    mixed = ol.sw_rhs_mixed_cpsr(ext, brick.neighbors, u.reshape(-1), gamma2.reshape(-1),
+                                 inv.reshape(-1), coords.reshape(-1), kind)
+    np.testing.assert_array_equal(mixed, uni.reshape(-1))
+
+
 def test_mixed_spherical_radiation_differs_from_dirichlet_only_on_external_faces():
     """Sommerfeld and Dirichlet closures of the same mixed-p state agree at
     every point off the outer faces of the brick; on them they differ."""
@@ -442,6 +457,14 @@ def test_mixed_spherical_radiation_differs_from_dirichlet_only_on_external_faces
     dd = ol.sw_rhs_mixed_dirichlet(brick.extents, brick.neighbors, u, gamma2, inv, ghost)
     x = brick.coordinates()
     for e, (a, b) in enumerate(zip(brick.split(sr, 5), brick.split(dd, 5))):
+        xe = brick.split(x, 3)[e]
+        outer = np.any(np.isclose(np.abs(xe), 1.0, atol=1e-14), axis=0)
+        np.testing.assert_array_equal(a[:, ~outer], b[:, ~outer])
+        assert np.any(a[:, outer] != b[:, outer])
+    # the constraint-preserving closure leaves the same points alone
+    cp = ol.sw_rhs_mixed_cpsr(brick.extents, brick.neighbors, u, gamma2, inv,
+                              brick.coordinates(), 3)
+    for e, (a, b) in enumerate(zip(brick.split(cp, 5), brick.split(dd, 5))):
         xe = brick.split(x, 3)[e]
         outer = np.any(np.isclose(np.abs(xe), 1.0, atol=1e-14), axis=0)
         np.testing.assert_array_equal(a[:, ~outer], b[:, ~outer])
