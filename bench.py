@@ -173,6 +173,78 @@ def measure_mixed(system, steps, warmup, rk3_steps=5):
     }
 
 
+def measure_boundaries(steps=50, warmup=5):
+    """Cost of the external boundary conditions: the RHS of a non-periodic
+    brick whose outer faces are closed by a condition, against the periodic
+    RHS of the same brick and data (one launch per RHS for the fused kernels).
+    GH: 512 elements of 8^3 (Kerr-Schild, Dirichlet analytic ghost), fused
+    vector-ALU kernel, 256 threads.  ScalarWave: 512 elements of 8^3 (plane
+    wave; Dirichlet, Sommerfeld spherical radiation, constraint-preserving
+    Sommerfeld), fused kernel; and the mixed-p brick of 1000 elements with
+    extents 4..12 (Dirichlet, constraint-preserving Sommerfeld), batched."""
+    import numpy as np
+    import torch
+    from spectre_amd import dg, workloads as wl
+
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    out = {}
+    brick, u, gammas, inv, ghost = wl.gh_kerr_schild_dirichlet(seed=0, elements=(8, 8, 8),
+                                                               extents=(8, 8, 8))
+    periodic = wl.Brick(brick.lower, brick.upper, brick.elements, brick.extents)
+    d_u, d_g, d_inv, d_gh = dev(u), dev(gammas), dev(inv), dev(ghost)
+    d_dt = torch.empty_like(d_u)
+    ms = {}
+    for name, b in (("periodic", periodic), ("dirichlet", brick)):
+        domain = dg.Domain.from_brick(b)
+        domain.set_kernel("gh", "fused")
+        domain.set_threads("gh", 256)
+        if name == "periodic":
+            fn = lambda: domain.gh_rhs(d_u, d_g, d_inv, out=d_dt)
+        else:
+            fn = lambda: domain.gh_rhs_dirichlet(d_u, d_g, d_inv, d_gh, out=d_dt)
+        ms[name] = round(time_calls(fn, steps, warmup), 6)
+    out["gh_512el_8^3_fused_ms"] = ms
+    del d_u, d_g, d_inv, d_gh, d_dt
+
+    brick, u, g2, inv, ghost = wl.scalar_wave_plane_wave_dirichlet(
+        seed=0, elements=(8, 8, 8), extents=(8, 8, 8), t=0.1, noise=1e-2,
+        lower=(1.0, -4.0, -4.0), upper=(9.0, 4.0, 4.0))
+    periodic = wl.Brick(brick.lower, brick.upper, brick.elements, brick.extents)
+    d_u, d_g, d_inv, d_gh = dev(u), dev(g2), dev(inv), dev(ghost)
+    d_dt = torch.empty_like(d_u)
+    ms = {}
+    for name in ("periodic", "dirichlet", "spherical_radiation", "cpsr"):
+        domain = dg.Domain.from_brick(periodic if name == "periodic" else brick)
+        domain.set_kernel("scalar_wave", "fused")
+        fn = {"periodic": lambda: domain.scalar_wave_rhs(d_u, d_g, d_inv, out=d_dt),
+              "dirichlet": lambda: domain.scalar_wave_rhs_dirichlet(d_u, d_g, d_inv, d_gh,
+                                                                    out=d_dt),
+              "spherical_radiation": lambda: domain.scalar_wave_rhs_spherical_radiation(
+                  d_u, d_g, d_inv, "sommerfeld", out=d_dt),
+              "cpsr": lambda: domain.scalar_wave_rhs_constraint_preserving_spherical_radiation(
+                  d_u, d_g, d_inv, "sommerfeld", out=d_dt)}[name]
+        ms[name] = round(time_calls(fn, steps, warmup), 6)
+    out["scalar_wave_512el_8^3_fused_ms"] = ms
+
+    brick, u, g2, inv, ghost = wl.scalar_wave_plane_wave_mixed_dirichlet(
+        seed=0, elements=(10, 10, 10), lowest=4, highest=12, t=0.1, noise=1e-2)
+    periodic = wl.MixedBrick(brick.lower, brick.upper, brick.elements, brick.extents)
+    d_u, d_g, d_inv, d_gh = dev(u), dev(g2), dev(inv), dev(ghost)
+    d_dt = torch.empty_like(d_u)
+    ms = {}
+    for name in ("periodic", "dirichlet", "cpsr"):
+        domain = dg.MixedDomain.from_brick(periodic if name == "periodic" else brick)
+        domain.set_kernel("scalar_wave", "mixed_batched")
+        fn = {"periodic": lambda: domain.scalar_wave_rhs(d_u, d_g, d_inv, out=d_dt),
+              "dirichlet": lambda: domain.scalar_wave_rhs_dirichlet(d_u, d_g, d_inv, d_gh,
+                                                                    out=d_dt),
+              "cpsr": lambda: domain.scalar_wave_rhs_constraint_preserving_spherical_radiation(
+                  d_u, d_g, d_inv, "sommerfeld", out=d_dt)}[name]
+        ms[name] = round(time_calls(fn, steps, warmup), 6)
+    out["scalar_wave_mixed_p_1000el_extents_4..12_batched_ms"] = ms
+    return out
+
+
 def measure_adjoint(steps=40, reps=3):
     """Adjoint ScalarWave workload (512 elements x 6^3, 40 RK3 steps at CFL
     0.25, J = sum M psi^2): time of the forward evolution against the
@@ -305,7 +377,7 @@ def main():
                     help="also measure the mixed-p batched form (1) or not (0)")
     ap.add_argument("--only-extension", default="",
                     choices=["", "sw_graph", "mixed_sw", "mixed_gh", "adjoint", "gh_adjoint",
-                             "gh_adjoint_mixed"],
+                             "gh_adjoint_mixed", "boundaries"],
                     help="run only this extension measurement (for profiling) and print "
                          "its JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -342,7 +414,8 @@ def main():
                 "mixed_gh": lambda: measure_mixed("gh", 20, 3),
                 "adjoint": measure_adjoint,
                 "gh_adjoint": measure_gh_adjoint,
-                "gh_adjoint_mixed": lambda: measure_gh_adjoint(mixed=True)}
+                "gh_adjoint_mixed": lambda: measure_gh_adjoint(mixed=True),
+                "boundaries": measure_boundaries}
         print(json.dumps(only[args.only_extension]()))
         return
 
@@ -445,7 +518,8 @@ def main():
                       "mixed_p": [measure_mixed(s, 20, 3) for s in ("scalar_wave", "gh")],
                       "adjoint": measure_adjoint(),
                       "gh_adjoint": measure_gh_adjoint(),
-                      "gh_adjoint_mixed": measure_gh_adjoint(mixed=True)}
+                      "gh_adjoint_mixed": measure_gh_adjoint(mixed=True),
+                      "boundaries": measure_boundaries()}
 
     result = {
         "metric": METRIC,

@@ -47,8 +47,14 @@ __device__ __forceinline__ int point_index(const MixedElement& m, int i, int j, 
 }
 
 // Volume index of point (a, b) of the face in dimension d on `side`.
+// ext_at(m, d) for a runtime d, by selects: a runtime index into the element's
+// extents would place the element record in scratch memory.
+__device__ __forceinline__ int ext_at(const MixedElement& m, int d) {
+  return d == 0 ? m.n[0] : (d == 1 ? m.n[1] : m.n[2]);
+}
+
 __device__ __forceinline__ int face_point(const MixedElement& m, int d, int side, int a, int b) {
-  const int fixed = side == 0 ? 0 : m.n[d] - 1;
+  const int fixed = side == 0 ? 0 : ext_at(m, d) - 1;
   if (d == 0) return point_index(m, fixed, a, b);
   if (d == 1) return point_index(m, a, fixed, b);
   return point_index(m, a, b, fixed);
@@ -119,7 +125,7 @@ sw_mixed_volume_kernel(const MixedElement* __restrict__ el, const int32_t* __res
       const int at[3] = {i, j, k};
       for (int dir = 0; dir < 6; ++dir) {
         const int d = dir >> 1, side = dir & 1;
-        if (at[d] != (side == 0 ? 0 : m.n[d] - 1) || nbr[6 * e + dir] >= 0) continue;
+        if (at[d] != (side == 0 ? 0 : ext_at(m, d) - 1) || nbr[6 * e + dir] >= 0) continue;
         double n[3], mag;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -481,7 +487,7 @@ sw_mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restr
         for (int i = 0; i < 3; ++i) ng[i] = n[i] * -1.0;
         sw_package(pke, ug, g2, ng);
         sw_boundary_terms(bc, pki, pke);
-        const double fac = lift_factor(m.n[d], mag);
+        const double fac = lift_factor(ext_at(m, d), mag);
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
           double& target = dt[5 * m.off + c * m.np + p];
@@ -542,7 +548,7 @@ sw_mixed_face_kernel(const MixedElement* __restrict__ el, const int32_t* __restr
     // lift into dt
     for (int f = t; f < nf; f += T) {
       const int p = face_point(m, d, side, f % na, f / na);
-      const double fac = lift_factor(m.n[d], sm[L.side_own + 4 * f + 3]);
+      const double fac = lift_factor(ext_at(m, d), sm[L.side_own + 4 * f + 3]);
 #pragma unroll
       for (int c = 0; c < 5; ++c) {
         double& target = dt[5 * m.off + c * m.np + p];
@@ -817,7 +823,7 @@ gh_mixed_face_pass_kernel(int d, const MixedElement* __restrict__ el,
       gh_package_pair(pki, ue[pr * np], ue[(10 + pr) * np], phi_i, si);
       gh_package_pair(pke, ux[pr * np], ux[(10 + pr) * np], phi_e, se);
       gh_boundary_terms_pair(bc, pki, si.lam, pke, se.lam);
-      const double fac = lift_factor(m.n[d], sd[11]);
+      const double fac = lift_factor(ext_at(m, d), sd[11]);
 #pragma unroll
       for (int c = 0; c < 5; ++c) {
         const int comp = c == 0 ? pr : c == 1 ? 10 + pr : 18 + c + 3 * pr;
@@ -908,7 +914,7 @@ gh_mixed_face_pass_kernel(int d, const MixedElement* __restrict__ el,
   // lift into dt: components g, Pi, Phi_0..2 of the pair
   for (int f = t; f < nf; f += T) {
     const int p = face_point(m, d, side, f % na, f / na);
-    const double fac = lift_factor(m.n[d], sm[L.mag + f]);
+    const double fac = lift_factor(ext_at(m, d), sm[L.mag + f]);
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
       const int comp = c == 0 ? pr : c == 1 ? 10 + pr : 18 + c + 3 * pr;
@@ -982,7 +988,7 @@ gh_mixed_face_adjoint_pass_kernel(int d, const MixedElement* __restrict__ el,
   const double* Pb = Ib_own + nb * kb;
   const double* Pa_nb = Ia_nb + ma * ka;
   const double* Pb_nb = Ib_nb + mb * kb;
-  const double lift_own = lift_factor(m.n[d], 1.0), lift_nb = lift_factor(mn.n[d], 1.0);
+  const double lift_own = lift_factor(ext_at(m, d), 1.0), lift_nb = lift_factor(ext_at(mn, d), 1.0);
   const int comp[5] = {pr, 10 + pr, 20 + 3 * pr, 21 + 3 * pr, 22 + 3 * pr};
   double* face = sm + L.face;
   // own side: side values (gh_mixed_face_sides_kernel), speeds and package
