@@ -92,39 +92,46 @@ sw_rhs_adjoint_kernel(const int32_t* __restrict__ nbr, const double* __restrict_
     ub[c] = acc;
   }
   // face contributions: (element, direction, exterior?) sorted by element
-  // then direction
+  // then direction.  Six fixed slots (slots of faces the point is not on are
+  // keyed last) and a fully unrolled exchange sort keep the lists in
+  // registers; a runtime index into them would place them in scratch memory.
+  constexpr int64_t kNoFace = INT64_MAX;
   int64_t key[6];
   int dim_of[6];
   bool ext_of[6];
-  int nk = 0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    if (idx[d] != 0 && idx[d] != N - 1) continue;
+    const bool on = idx[d] == 0 || idx[d] == N - 1;
     const int s = idx[d] == 0 ? 0 : 1;
-    key[nk] = e * 8 + 2 * d + s;
-    dim_of[nk] = d;
-    ext_of[nk] = false;
-    ++nk;
-    key[nk] = static_cast<int64_t>(nbr[6 * e + 2 * d + s]) * 8 + 2 * d + (1 - s);
-    dim_of[nk] = d;
-    ext_of[nk] = true;
-    ++nk;
+    key[2 * d] = on ? static_cast<int64_t>(e) * 8 + 2 * d + s : kNoFace;
+    dim_of[2 * d] = d;
+    ext_of[2 * d] = false;
+    key[2 * d + 1] =
+        on ? static_cast<int64_t>(nbr[6 * e + 2 * d + s]) * 8 + 2 * d + (1 - s) : kNoFace;
+    dim_of[2 * d + 1] = d;
+    ext_of[2 * d + 1] = true;
   }
-  for (int a = 1; a < nk; ++a)
-    for (int b = a; b > 0 && key[b] < key[b - 1]; --b) {
-      const int64_t tk = key[b];
-      key[b] = key[b - 1];
-      key[b - 1] = tk;
-      const int td = dim_of[b];
-      dim_of[b] = dim_of[b - 1];
-      dim_of[b - 1] = td;
-      const bool tx = ext_of[b];
-      ext_of[b] = ext_of[b - 1];
-      ext_of[b - 1] = tx;
+#pragma unroll
+  for (int a = 0; a < 5; ++a)
+#pragma unroll
+    for (int b = 0; b < 5 - a; ++b) {
+      const bool swap = key[b + 1] < key[b];
+      const int64_t k0 = key[b], k1 = key[b + 1];
+      key[b] = swap ? k1 : k0;
+      key[b + 1] = swap ? k0 : k1;
+      const int d0 = dim_of[b], d1 = dim_of[b + 1];
+      dim_of[b] = swap ? d1 : d0;
+      dim_of[b + 1] = swap ? d0 : d1;
+      const bool x0 = ext_of[b], x1 = ext_of[b + 1];
+      ext_of[b] = swap ? x1 : x0;
+      ext_of[b + 1] = swap ? x0 : x1;
     }
-  for (int x = 0; x < nk; ++x) {
+#pragma unroll
+  for (int x = 0; x < 6; ++x) {
+    if (key[x] == kNoFace) continue;
     const int d = dim_of[x];
-    const int s = idx[d] == 0 ? 0 : 1;
+    const int id = d == 0 ? idx[0] : (d == 1 ? idx[1] : idx[2]);
+    const int s = id == 0 ? 0 : 1;
     if (!ext_of[x]) {
       // interior term of this element's face (d, s)
       double n[3], mag;
@@ -145,9 +152,9 @@ sw_rhs_adjoint_kernel(const int32_t* __restrict__ nbr, const double* __restrict_
       // exterior term of the neighbour's face (d, 1 - s): its interior point
       // is the matching point q of the neighbour
       const int64_t en = nbr[6 * e + 2 * d + s];
-      int qi[3] = {idx[0], idx[1], idx[2]};
-      qi[d] = s == 0 ? N - 1 : 0;
-      const int q = qi[0] + N * (qi[1] + N * qi[2]);
+      const int qd = s == 0 ? N - 1 : 0;
+      const int q = (d == 0 ? qd : idx[0]) + N * ((d == 1 ? qd : idx[1]) +
+                                                  N * (d == 2 ? qd : idx[2]));
       double nq[3], magq, ne[3], mage;
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
