@@ -353,13 +353,20 @@ def test_boundary_calls_validate_their_arguments():
     assert sph(h, ptr(du), ptr(dg2), ptr(dinv), ptr(None), 1, ptr(out), None) == 0
     assert cpsr(h, ptr(du), ptr(dg2), ptr(dinv), ptr(None), 1, ptr(out), None) == 0
     torch.cuda.synchronize()
-    # mixed-p domains take only the Dirichlet condition
+    # on a periodic mixed-p domain (no external face) both conditions give the
+    # periodic RHS; an unknown type is still rejected
     mbrick, mu, mg2, minv = wl.scalar_wave_plane_wave_mixed(seed=2, elements=(2, 1, 1),
                                                              lowest=3, highest=4)
     mdom = dg.MixedDomain.from_brick(mbrick)
-    dmu = dev(mu)
-    assert sph(mdom._handle, ptr(dmu), ptr(dev(mg2)), ptr(dev(minv)), ptr(None), 1,
-               ptr(torch.empty_like(dmu)), None) == -2
+    dmu, dmg2, dminv = dev(mu), dev(mg2), dev(minv)
+    ref = host(mdom.scalar_wave_rhs(dmu, dmg2, dminv))
+    for call in (sph, cpsr):
+        mout = torch.empty_like(dmu)
+        assert call(mdom._handle, ptr(dmu), ptr(dmg2), ptr(dminv), ptr(None), 1, ptr(mout),
+                    None) == 0
+        np.testing.assert_array_equal(host(mout), ref)
+        assert call(mdom._handle, ptr(dmu), ptr(dmg2), ptr(dminv), ptr(None), 7, ptr(mout),
+                    None) == -1
 
 
 def test_domain_with_external_faces_rejects_the_periodic_calls():
