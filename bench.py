@@ -23,6 +23,7 @@ import time
 
 METRIC = "DG RHS grid-pts/s FP64 on 1 GPU, % roofline; mixed-p batch gain; adjoint/forward cost"
 HBM_PEAK_GBS = 8000.0
+LLC_BYTES = 256 << 20  # MI355X Infinity Cache
 FP64_PEAK_TFLOPS = 78.6
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
@@ -72,6 +73,40 @@ def time_calls(fn, steps, warmup):
     stop.record(stream)
     torch.cuda.synchronize()
     return start.elapsed_time(stop) / steps
+
+
+def hbm_probe(mib, reps=5):
+    """Probe kernels of known access pattern (spectre_amd/csrc/hbm_probe.hip):
+    a streaming read, a streaming write and a copy of `mib` MiB each, `reps`
+    launches each on the current stream, timed with HIP events.  Profiled with
+    rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE), the counters' KiB per
+    dispatch against the known bytes give the factor that converts the
+    counters into bytes; the copy rate is a measured HBM bandwidth."""
+    import ctypes
+    import torch
+    root = os.path.dirname(os.path.abspath(__file__))
+    probe = ctypes.CDLL(os.path.join(root, "spectre_amd", "libsxd_hbm_probe.so"))
+    probe.sxd_probe_run.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    partial = torch.empty(probe.sxd_probe_workgroups(), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    out = {"probe_doubles": n}
+    for mode, name, read, written in ((0, "read", n * 8, partial.numel() * 8),
+                                      (1, "write", 0, n * 8), (2, "copy", n * 8, n * 8)):
+        call = lambda: probe.sxd_probe_run(mode, ctypes.c_void_p(src.data_ptr()),
+                                           ctypes.c_void_p(dst.data_ptr()),
+                                           ctypes.c_void_p(partial.data_ptr()), n,
+                                           ctypes.c_void_p(stream.cuda_stream))
+        if call() != 0:
+            raise RuntimeError(f"probe {name} failed")
+        ms = time_calls(lambda: call(), reps, 1)
+        out[name] = {"bytes_read": read, "bytes_written": written, "ms": round(ms, 5),
+                     "gb_per_s": round((read + written) / (ms * 1e-3) / 1e9, 1)}
+    torch.cuda.synchronize()
+    return out
 
 
 def measure_sw_graph(calls=1000, reps=5):
@@ -373,6 +408,9 @@ def main():
                          "autotuner on the workload first and keeps its choice")
     ap.add_argument("--threads", type=int, default=0,
                     help="threads per workgroup with an explicit --kernel (0 = library default)")
+    ap.add_argument("--rotate", type=int, default=0,
+                    help="number of input/output buffer sets the timed steps cycle through "
+                         "(0 = enough to exceed 3x the 256 MiB Infinity Cache, at most 8)")
     ap.add_argument("--extensions", type=int, default=1,
                     help="also measure the mixed-p batched form (1) or not (0)")
     ap.add_argument("--only-extension", default="",
@@ -394,14 +432,7 @@ def main():
     import torch
 
     if args.copy_calibration:
-        n = args.copy_calibration * (1 << 20) // 8
-        src = torch.ones(n, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes_read_per_copy": n * 8,
-                          "copy_calibration_bytes_written_per_copy": n * 8}))
+        print(json.dumps(hbm_probe(args.copy_calibration)))
         return
 
     root = os.path.dirname(os.path.abspath(__file__))
@@ -444,21 +475,37 @@ def main():
             domain.set_threads(system, args.threads)
     kernel_used = domain.kernel(system)
 
-    for _ in range(args.warmup):
-        rhs(d_u, d_aux, d_inv, out=d_dt)
+    # The timed steps cycle through `rotate` copies of the inputs and outputs
+    # whose total size is at least three times the 256 MiB last-level
+    # (Infinity) cache, so every step reads its inputs from HBM rather than
+    # from what the previous step left in that cache.
+    footprint = sum(t.numel() * 8 for t in (d_u, d_aux, d_inv, d_dt))
+    rotate = args.rotate if args.rotate > 0 else \
+        max(1, min(8, -(-3 * LLC_BYTES // footprint)))
+    sets = [(d_u, d_aux, d_inv, d_dt)] + [
+        (d_u.clone(), d_aux.clone(), d_inv.clone(), torch.empty_like(d_dt))
+        for _ in range(rotate - 1)]
+    for k in range(args.warmup):
+        a, p, j, o = sets[k % rotate]
+        rhs(a, p, j, out=o)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     start.record(stream)
-    for _ in range(args.steps):
-        rhs(d_u, d_aux, d_inv, out=d_dt)
+    for k in range(args.steps):
+        a, p, j, o = sets[k % rotate]
+        rhs(a, p, j, out=o)
     stop.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = start.elapsed_time(stop) / args.steps
     value = points / (ms_per_step * 1e-3)
+    # the same launches on one set of buffers (inputs partly served by the
+    # last-level cache), for comparison
+    ms_same_buffers = time_calls(lambda: rhs(d_u, d_aux, d_inv, out=d_dt), args.steps, 2)
+    del sets
 
     bpp = algorithmic_bytes_per_point(nc, n_aux, n)
     bytes_per_launch = bpp * points
@@ -543,6 +590,8 @@ def main():
         "cpu_baseline": cpu,
         "extensions": extensions,
         "wall_s_timed_region": round(wall, 4),
+        "buffer_sets_rotated": rotate,
+        "ms_per_step_same_buffers": round(ms_same_buffers, 6),
     }
     print(json.dumps(result))
 

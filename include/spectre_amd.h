@@ -17,6 +17,12 @@
  *    sxd_last_error() gives a message for the calling thread's last failure.
  *  - Meshes are Legendre / Gauss-Lobatto in every dimension with 2..20
  *    points per dimension (Spectral.hpp:100-102).
+ *  - A domain (sxd_domain) keeps one set of device work arrays per stream
+ *    that calls it, so calls on different streams may overlap.  Calls that
+ *    change its kernel choice must not overlap other calls on it.
+ *  - The whole-domain RHS calls reject a dt that overlaps u
+ *    (SXD_ERR_INVALID_ARGUMENT): neighbours' face values of u are read while
+ *    dt is written.
  */
 #ifndef SPECTRE_AMD_H
 #define SPECTRE_AMD_H
@@ -73,6 +79,19 @@ int sxd_logical_partial_derivatives(int num_elements, const int32_t extents[3],
 int sxd_partial_derivatives(int num_elements, const int32_t extents[3], int nc,
                             const double* u, const double* inv_jac, double* du,
                             void* stream);
+
+/* Transpose of sxd_partial_derivatives for a fixed inverse Jacobian (the
+ * vector-Jacobian product of the linear map u -> du): dubar [E][3 nc][Np]
+ * (block 3 c + i) -> ubar [E][nc][Np],
+ *   ubar_c = sum_j D_j^T (sum_i d xi^j / d x^i dubar_{3c+i})
+ * with D_j the LGL differentiation matrix applied along dimension j.  No
+ * counterpart in the reference (which has no adjoints); it is the backward of
+ * the torch.autograd path of the Python partial_derivative binding
+ * (NumericalAlgorithms/LinearOperators/Python/PartialDerivatives.cpp:20-40).
+ * SXD_ERR_UNSUPPORTED if 3 Np doubles exceed the LDS (Np above ~6500). */
+int sxd_partial_derivatives_adjoint(int num_elements, const int32_t extents[3], int nc,
+                                    const double* dubar, const double* inv_jac,
+                                    double* ubar, void* stream);
 
 /* ---- Point-wise system terms ------------------------------------------ */
 
@@ -347,7 +366,9 @@ int sxd_rk3_adjoint_substep(int substep, int64_t n, double dt, const double* lam
  * layout and must stay allocated while the stepper lives; the stepper owns
  * two field-sized work arrays.  The kernel organisation the domain has at
  * creation time (sxd_domain_set_kernel / sxd_domain_autotune) is the one
- * captured. */
+ * captured.  The stepper owns the work arrays its graph uses (sized at
+ * creation), so later calls on the domain cannot move them; the domain must
+ * outlive the stepper. */
 typedef struct sxd_rk3_stepper sxd_rk3_stepper;
 int sxd_rk3_stepper_create(sxd_rk3_stepper** stepper, sxd_domain* domain,
                            int system, double* u, const double* params,

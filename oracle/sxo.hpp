@@ -265,6 +265,48 @@ void partial_derivatives(const T* u, int nc, const int ext[3], const T* inv_j,
       }
 }
 
+// Transpose of partial_derivatives for a fixed inverse Jacobian (the
+// reference has no adjoints; this is the exact transpose of the linear map
+// u -> du above, PartialDerivatives.tpp:56-110,315-363):
+//   xbar_j = sum_i inv_j(j, i) dubar_{3c+i}          (product, then fmas)
+//   ubar_c(p) = sum_{j=0..2} sum_{l} D_j[l][p_j] xbar_j(p with p_j -> l)
+// accumulated as one chain per point: j = 0, 1, 2, and l = 0 .. n_j - 1
+// (the first term a product, then fused multiply-adds).
+template <class T>
+void partial_derivatives_adjoint(const T* dubar, int nc, const int ext[3],
+                                 const T* inv_j, T* ubar) {
+  const auto& cache = SpectralCache::get();
+  const int n0 = ext[0], n1 = ext[1], n2 = ext[2];
+  const size_t np = static_cast<size_t>(n0) * n1 * n2;
+  const double* D0 = cache.D[n0].data();
+  const double* D1 = cache.D[n1].data();
+  const double* D2 = cache.D[n2].data();
+  std::vector<T> x(3 * np);
+  for (int c = 0; c < nc; ++c) {
+    const T* g = dubar + 3 * c * np;
+    for (int j = 0; j < 3; ++j)
+      for (size_t p = 0; p < np; ++p) {
+        T s = inv_j[ij3(j, 0) * np + p] * g[p];
+        s = fma_(inv_j[ij3(j, 1) * np + p], g[np + p], s);
+        s = fma_(inv_j[ij3(j, 2) * np + p], g[2 * np + p], s);
+        x[j * np + p] = s;
+      }
+    for (int k = 0; k < n2; ++k)
+      for (int j = 0; j < n1; ++j)
+        for (int i = 0; i < n0; ++i) {
+          const size_t pt = i + n0 * (j + static_cast<size_t>(n1) * k);
+          T acc = D0[0 * n0 + i] * x[0 + n0 * (j + n1 * k)];
+          for (int l = 1; l < n0; ++l)
+            acc = fma_(T(D0[l * n0 + i]), x[l + n0 * (j + n1 * k)], acc);
+          for (int l = 0; l < n1; ++l)
+            acc = fma_(T(D1[l * n1 + j]), x[np + i + n0 * (l + n1 * k)], acc);
+          for (int l = 0; l < n2; ++l)
+            acc = fma_(T(D2[l * n2 + k]), x[2 * np + i + n0 * (j + n1 * l)], acc);
+          ubar[c * np + pt] = acc;
+        }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // ScalarWave.  Variables order Psi, Pi, Phi_i (ScalarWave/System.hpp:48-50).
 

@@ -65,6 +65,15 @@ int sxo_partial_derivatives(const double* u, int nc, const int* ext,
   return 0;
 }
 
+// Transpose of the partial derivatives (one element): dubar [3 nc][Np] ->
+// ubar [nc][Np].
+int sxo_partial_derivatives_adjoint(const double* dubar, int nc, const int* ext,
+                                    const double* inv_j, double* ubar) {
+  if (bad_extents(ext) || nc < 1) return -1;
+  sxo::partial_derivatives_adjoint(dubar, nc, ext, inv_j, ubar);
+  return 0;
+}
+
 // Batched partial derivatives over num_elements elements of equal extents.
 int sxo_partial_derivatives_batch(int num_elements, const double* u, int nc,
                                   const int* ext, const double* inv_j,

@@ -32,6 +32,8 @@ _SIGNATURES = {
     "sxd_lgl_differentiation_matrix": (ctypes.c_int, [ctypes.c_int, _dp]),
     "sxd_logical_partial_derivatives": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _vp, _vp, _vp]),
     "sxd_partial_derivatives": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _vp, _vp, _vp, _vp]),
+    "sxd_partial_derivatives_adjoint": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _vp, _vp,
+                                                       _vp, _vp]),
     "sxd_scalar_wave_time_derivative": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_gh_time_derivative": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_scalar_wave_upwind_penalty": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

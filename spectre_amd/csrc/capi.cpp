@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -17,6 +18,56 @@
 #include "kernels.hpp"
 #include "spectral.hpp"
 
+namespace sxd {
+// Device work arrays of the composite, mixed-p and adjoint paths: the
+// derivative array `du` and the per-point / per-face-slot temporaries.
+// Grown on demand (hipFree, then hipMalloc: a growth waits for the device),
+// or, when `fixed`, a caller-provided block that is never reallocated.
+struct Scratch {
+  double* du = nullptr;
+  double* temps = nullptr;
+  size_t du_doubles = 0, temps_doubles = 0;
+  bool fixed = false;
+  size_t fixed_doubles = 0;  // capacity of a fixed block
+
+  // Make room for du_n + temps_n doubles.  Returns SXD_OK,
+  // SXD_ERR_OUT_OF_MEMORY, or SXD_ERR_UNSUPPORTED for a fixed block that is
+  // too small.
+  int reserve(size_t du_n, size_t temps_n) {
+    if (fixed) {
+      if (du_n > fixed_doubles || temps_n > fixed_doubles - du_n) return SXD_ERR_UNSUPPORTED;
+      temps = du + du_n;
+      du_doubles = du_n;
+      temps_doubles = temps_n;
+      return SXD_OK;
+    }
+    if (du_doubles < du_n) {
+      if (du != nullptr) (void)hipFree(du);
+      du = nullptr;
+      du_doubles = 0;
+      if (hipMalloc(&du, du_n * sizeof(double)) != hipSuccess) return SXD_ERR_OUT_OF_MEMORY;
+      du_doubles = du_n;
+    }
+    if (temps_doubles < temps_n) {
+      if (temps != nullptr) (void)hipFree(temps);
+      temps = nullptr;
+      temps_doubles = 0;
+      if (hipMalloc(&temps, temps_n * sizeof(double)) != hipSuccess) return SXD_ERR_OUT_OF_MEMORY;
+      temps_doubles = temps_n;
+    }
+    return SXD_OK;
+  }
+  void release() {
+    if (!fixed) {
+      if (du != nullptr) (void)hipFree(du);
+      if (temps != nullptr) (void)hipFree(temps);
+    }
+    du = temps = nullptr;
+    du_doubles = temps_doubles = 0;
+  }
+};
+}  // namespace sxd
+
 struct sxd_domain {
   int device = 0;
   int num_elements = 0;
@@ -24,12 +75,11 @@ struct sxd_domain {
   int32_t* d_nbr = nullptr;
   // Some neighbour is -1 (an external face closed by a ghost state).
   bool has_boundary = false;
-  // Work arrays of the composite (non-fused) paths, allocated on first use.
-  double* d_scratch_du = nullptr;
-  double* d_scratch_temps = nullptr;
-  size_t scratch_du_doubles = 0;
-  size_t scratch_temps_doubles = 0;
+  // Work arrays of the composite (non-fused) paths, one set per stream that
+  // has called the domain, allocated on first use: calls on different
+  // streams never share one.
   std::mutex scratch_mutex;
+  std::vector<std::pair<void*, std::unique_ptr<sxd::Scratch>>> scratch;
   // Kernel choice per system (SXD_SYSTEM_*), SXD_KERNEL_AUTO until set, and
   // threads per workgroup of the mixed-p kernels.
   int kernel[2] = {SXD_KERNEL_AUTO, SXD_KERNEL_AUTO};
@@ -99,6 +149,8 @@ int finish(int rc, const char* where) {
     return fail(rc, "%s: kernel launch failed: %s", where, hipGetErrorString(err));
   }
   if (rc == SXD_ERR_UNSUPPORTED) return fail(rc, "%s: unsupported configuration", where);
+  if (rc == SXD_ERR_OUT_OF_MEMORY)
+    return fail(rc, "%s: device allocation of the work arrays failed", where);
   return fail(rc, "%s: failed with status %d", where, rc);
 }
 
@@ -200,6 +252,23 @@ int sxd_partial_derivatives(int num_elements, const int32_t extents[3], int nc,
   SXD_REQUIRE_DEVICE(du);
   return finish(launch_partial_derivatives(num_elements, extents, nc, u, inv_jac, du,
                                            nullptr, static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+int sxd_partial_derivatives_adjoint(int num_elements, const int32_t extents[3], int nc,
+                                    const double* dubar, const double* inv_jac, double* ubar,
+                                    void* stream) {
+  if (!valid_extents(extents))
+    return fail(SXD_ERR_INVALID_ARGUMENT, "extents must be 2..20 in each dimension");
+  if (num_elements < 0 || nc < 1)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "bad element or component count");
+  if (num_elements == 0) return ok();
+  SXD_REQUIRE_DEVICE(dubar);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  SXD_REQUIRE_DEVICE(ubar);
+  if (ubar == dubar) return fail(SXD_ERR_INVALID_ARGUMENT, "ubar must not alias dubar");
+  return finish(launch_partial_derivatives_adjoint(num_elements, extents, nc, dubar, inv_jac,
+                                                   ubar, static_cast<hipStream_t>(stream)),
                 __func__);
 }
 
@@ -460,32 +529,22 @@ int sxd_domain_destroy(sxd_domain* domain) {
   if (domain->d_list != nullptr) (void)hipFree(domain->d_list);
   if (domain->d_buckets != nullptr) (void)hipFree(domain->d_buckets);
   if (domain->d_nbr != nullptr) (void)hipFree(domain->d_nbr);
-  if (domain->d_scratch_du != nullptr) (void)hipFree(domain->d_scratch_du);
-  if (domain->d_scratch_temps != nullptr) (void)hipFree(domain->d_scratch_temps);
+  if (!domain->scratch.empty()) {
+    (void)hipDeviceSynchronize();
+    for (auto& s : domain->scratch) s.second->release();
+  }
   delete domain;
   return ok();
 }
 
 namespace {
-int ensure_scratch(sxd_domain* d, size_t du_doubles, size_t temps_doubles) {
+// The work arrays of `stream` on domain d (created empty on first use).
+extern "C++" Scratch& stream_scratch(sxd_domain* d, hipStream_t stream) {
   std::lock_guard<std::mutex> lock(d->scratch_mutex);
-  if (d->scratch_du_doubles < du_doubles) {
-    if (d->d_scratch_du != nullptr) (void)hipFree(d->d_scratch_du);
-    d->d_scratch_du = nullptr;
-    d->scratch_du_doubles = 0;
-    if (hipMalloc(&d->d_scratch_du, du_doubles * sizeof(double)) != hipSuccess)
-      return SXD_ERR_OUT_OF_MEMORY;
-    d->scratch_du_doubles = du_doubles;
-  }
-  if (d->scratch_temps_doubles < temps_doubles) {
-    if (d->d_scratch_temps != nullptr) (void)hipFree(d->d_scratch_temps);
-    d->d_scratch_temps = nullptr;
-    d->scratch_temps_doubles = 0;
-    if (hipMalloc(&d->d_scratch_temps, temps_doubles * sizeof(double)) != hipSuccess)
-      return SXD_ERR_OUT_OF_MEMORY;
-    d->scratch_temps_doubles = temps_doubles;
-  }
-  return SXD_OK;
+  for (auto& s : d->scratch)
+    if (s.first == static_cast<void*>(stream)) return *s.second;
+  d->scratch.emplace_back(static_cast<void*>(stream), std::make_unique<Scratch>());
+  return *d->scratch.back().second;
 }
 
 int check_domain(const sxd_domain* d) {
@@ -504,6 +563,24 @@ int check_periodic(const sxd_domain* d) {
   if (d->has_boundary)
     return fail(SXD_ERR_UNSUPPORTED, "the domain has external faces (neighbour -1); only "
                                      "the *_rhs_dirichlet calls evaluate it");
+  return SXD_OK;
+}
+
+int64_t domain_points(const sxd_domain* d) {
+  return d->mixed ? d->total_points
+                  : static_cast<int64_t>(d->num_elements) * d->ext[0] * d->ext[1] * d->ext[2];
+}
+
+// The whole-domain RHS kernels read neighbours' face values of u while other
+// workgroups write dt, so the two arrays must not overlap.
+int check_no_overlap(const sxd_domain* d, int nc, const double* u, const double* dt,
+                     const char* where) {
+  const int64_t n = static_cast<int64_t>(nc) * domain_points(d);
+  const uintptr_t a = reinterpret_cast<uintptr_t>(u), b = reinterpret_cast<uintptr_t>(dt);
+  const uintptr_t bytes = static_cast<uintptr_t>(n) * sizeof(double);
+  if (a < b + bytes && b < a + bytes)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "%s: dt overlaps u (in-place evaluation is not "
+                                          "supported)", where);
   return SXD_OK;
 }
 }  // namespace
@@ -537,14 +614,19 @@ int resolve_kernel(const sxd_domain* d, int system) {
   return k[0];
 }
 
-int run_mixed(sxd_domain* d, bool gh, int kernel, const double* u, const double* params,
-              const double* inv_jac, double* dt, hipStream_t stream,
+// Reserve work arrays (status only; finish() words the message).
+int reserve_scratch(Scratch& sc, size_t du_doubles, size_t temps_doubles) {
+  return sc.reserve(du_doubles, temps_doubles);
+}
+
+int run_mixed(sxd_domain* d, Scratch& sc, bool gh, int kernel, const double* u,
+              const double* params, const double* inv_jac, double* dt, hipStream_t stream,
               const double* ghost = nullptr, int bc_kind = kSwBcGhostArray) {
   double* scratch = nullptr;
   if (gh) {
-    if (ensure_scratch(d, 150 * static_cast<size_t>(d->total_points), 0) != SXD_OK)
-      return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
-    scratch = d->d_scratch_du;
+    if (int rc = reserve_scratch(sc, 150 * static_cast<size_t>(d->total_points), 0); rc != SXD_OK)
+      return rc;
+    scratch = sc.du;
   }
   const int threads = d->threads[gh ? SXD_SYSTEM_GH : SXD_SYSTEM_SCALAR_WAVE];
   if (kernel == SXD_KERNEL_MIXED_BATCHED)
@@ -560,42 +642,43 @@ int run_mixed(sxd_domain* d, bool gh, int kernel, const double* u, const double*
   return SXD_OK;
 }
 
-int run_sw(sxd_domain* d, int kernel, const double* u, const double* gamma2,
+int run_sw(sxd_domain* d, Scratch& sc, int kernel, const double* u, const double* gamma2,
            const double* inv_jac, double* dt, hipStream_t stream,
            const double* ghost = nullptr, int bc_kind = kSwBcGhostArray) {
   if (d->mixed) {
-    return run_mixed(d, false, kernel, u, gamma2, inv_jac, dt, stream, ghost, bc_kind);
+    return run_mixed(d, sc, false, kernel, u, gamma2, inv_jac, dt, stream, ghost, bc_kind);
   }
   double* scratch = nullptr;
   if (kernel == SXD_KERNEL_COMPOSITE) {
     const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
-    if (ensure_scratch(d, 15 * np * d->num_elements, 0) != SXD_OK)
-      return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
-    scratch = d->d_scratch_du;
+    if (int rc = reserve_scratch(sc, 15 * np * d->num_elements, 0); rc != SXD_OK) return rc;
+    scratch = sc.du;
   }
   return launch_sw_rhs(d->num_elements, d->ext, d->d_nbr, u, gamma2, inv_jac, dt,
                        kernel == SXD_KERNEL_FUSED, scratch, ghost, bc_kind, stream);
 }
 
-int run_gh(sxd_domain* d, int kernel, const double* u, const double* gammas,
+int run_gh(sxd_domain* d, Scratch& sc, int kernel, const double* u, const double* gammas,
            const double* inv_jac, double* dt, hipStream_t stream,
            const double* ghost = nullptr) {
-  if (d->mixed) return run_mixed(d, true, kernel, u, gammas, inv_jac, dt, stream, ghost);
+  if (d->mixed) return run_mixed(d, sc, true, kernel, u, gammas, inv_jac, dt, stream, ghost);
   if (kernel == SXD_KERNEL_FUSED || kernel == SXD_KERNEL_FUSED_MFMA)
     return launch_gh_rhs_fused(d->num_elements, d->ext, d->d_nbr, u, gammas, inv_jac, dt,
                                kernel == SXD_KERNEL_FUSED ? kGhFusedValu : kGhFusedMfma,
                                d->threads[SXD_SYSTEM_GH] == 512 ? 512 : 256, ghost, stream);
   const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
-  if (ensure_scratch(d, 150 * np * d->num_elements, 12 * np * d->num_elements) != SXD_OK)
-    return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
+  if (int rc = reserve_scratch(sc, 150 * np * d->num_elements, 12 * np * d->num_elements);
+      rc != SXD_OK)
+    return rc;
   return launch_gh_rhs_composite(d->num_elements, d->ext, d->d_nbr, u, gammas, inv_jac, dt,
-                                 d->d_scratch_du, d->d_scratch_temps, ghost, stream);
+                                 sc.du, sc.temps, ghost, stream);
 }
 
-int run_system(sxd_domain* d, int system, int kernel, const double* u, const double* params,
-               const double* inv_jac, double* dt, hipStream_t stream) {
-  return system == SXD_SYSTEM_SCALAR_WAVE ? run_sw(d, kernel, u, params, inv_jac, dt, stream)
-                                          : run_gh(d, kernel, u, params, inv_jac, dt, stream);
+int run_system(sxd_domain* d, Scratch& sc, int system, int kernel, const double* u,
+               const double* params, const double* inv_jac, double* dt, hipStream_t stream) {
+  return system == SXD_SYSTEM_SCALAR_WAVE
+             ? run_sw(d, sc, kernel, u, params, inv_jac, dt, stream)
+             : run_gh(d, sc, kernel, u, params, inv_jac, dt, stream);
 }
 }  // namespace
 
@@ -607,7 +690,9 @@ int sxd_scalar_wave_rhs(const sxd_domain* domain, const double* u, const double*
   SXD_REQUIRE_DEVICE(inv_jac);
   SXD_REQUIRE_DEVICE(dt);
   sxd_domain* d = const_cast<sxd_domain*>(domain);
-  return finish(run_sw(d, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,
+  if (int rc = check_no_overlap(d, 5, u, dt, __func__); rc != SXD_OK) return rc;
+  Scratch& sc = stream_scratch(d, static_cast<hipStream_t>(stream));
+  return finish(run_sw(d, sc, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,
                        static_cast<hipStream_t>(stream)),
                 __func__);
 }
@@ -622,7 +707,9 @@ int sxd_scalar_wave_rhs_dirichlet(const sxd_domain* domain, const double* u,
   SXD_REQUIRE_DEVICE(ghost);
   SXD_REQUIRE_DEVICE(dt);
   sxd_domain* d = const_cast<sxd_domain*>(domain);
-  return finish(run_sw(d, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,
+  if (int rc = check_no_overlap(d, 5, u, dt, __func__); rc != SXD_OK) return rc;
+  Scratch& sc = stream_scratch(d, static_cast<hipStream_t>(stream));
+  return finish(run_sw(d, sc, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,
                        static_cast<hipStream_t>(stream), ghost),
                 __func__);
 }
@@ -641,9 +728,11 @@ int sxd_scalar_wave_rhs_spherical_radiation(const sxd_domain* domain, const doub
     return fail(SXD_ERR_INVALID_ARGUMENT, "unknown spherical radiation type %d", type);
   if (type == SXD_SPHERICAL_RADIATION_BAYLISS_TURKEL) SXD_REQUIRE_DEVICE(coords);
   sxd_domain* d = const_cast<sxd_domain*>(domain);
+  if (int rc = check_no_overlap(d, 5, u, dt, __func__); rc != SXD_OK) return rc;
+  Scratch& sc = stream_scratch(d, static_cast<hipStream_t>(stream));
   const int kind = type == SXD_SPHERICAL_RADIATION_SOMMERFELD ? kSwBcSommerfeld
                                                               : kSwBcBaylissTurkel;
-  return finish(run_sw(d, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,
+  return finish(run_sw(d, sc, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,
                        static_cast<hipStream_t>(stream),
                        kind == kSwBcBaylissTurkel ? coords : nullptr, kind),
                 __func__);
@@ -663,8 +752,10 @@ int sxd_scalar_wave_rhs_constraint_preserving_spherical_radiation(
                 "unknown constraint-preserving spherical radiation type %d", type);
   if (type != SXD_CPSR_SOMMERFELD) SXD_REQUIRE_DEVICE(coords);
   sxd_domain* d = const_cast<sxd_domain*>(domain);
+  if (int rc = check_no_overlap(d, 5, u, dt, __func__); rc != SXD_OK) return rc;
+  Scratch& sc = stream_scratch(d, static_cast<hipStream_t>(stream));
   const int kind = kSwBcCpsrSommerfeld + (type - SXD_CPSR_SOMMERFELD);
-  return finish(run_sw(d, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,
+  return finish(run_sw(d, sc, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,
                        static_cast<hipStream_t>(stream),
                        type == SXD_CPSR_SOMMERFELD ? nullptr : coords, kind),
                 __func__);
@@ -678,7 +769,9 @@ int sxd_gh_rhs(const sxd_domain* domain, const double* u, const double* gammas,
   SXD_REQUIRE_DEVICE(inv_jac);
   SXD_REQUIRE_DEVICE(dt);
   sxd_domain* d = const_cast<sxd_domain*>(domain);
-  return finish(run_gh(d, resolve_kernel(d, SXD_SYSTEM_GH), u, gammas, inv_jac, dt,
+  if (int rc = check_no_overlap(d, 50, u, dt, __func__); rc != SXD_OK) return rc;
+  Scratch& sc = stream_scratch(d, static_cast<hipStream_t>(stream));
+  return finish(run_gh(d, sc, resolve_kernel(d, SXD_SYSTEM_GH), u, gammas, inv_jac, dt,
                        static_cast<hipStream_t>(stream)),
                 __func__);
 }
@@ -693,11 +786,13 @@ int sxd_gh_rhs_dirichlet(const sxd_domain* domain, const double* u, const double
   SXD_REQUIRE_DEVICE(ghost);
   SXD_REQUIRE_DEVICE(dt);
   sxd_domain* d = const_cast<sxd_domain*>(domain);
+  if (int rc = check_no_overlap(d, 50, u, dt, __func__); rc != SXD_OK) return rc;
+  Scratch& sc = stream_scratch(d, static_cast<hipStream_t>(stream));
   const int kernel = resolve_kernel(d, SXD_SYSTEM_GH);
   if (kernel == SXD_KERNEL_FUSED_MFMA)
     return fail(SXD_ERR_UNSUPPORTED, "external faces are not built for the MFMA engine; "
                                      "select the fused or composite kernel");
-  return finish(run_gh(d, kernel, u, gammas, inv_jac, dt, static_cast<hipStream_t>(stream),
+  return finish(run_gh(d, sc, kernel, u, gammas, inv_jac, dt, static_cast<hipStream_t>(stream),
                        ghost),
                 __func__);
 }
@@ -739,6 +834,10 @@ int sxd_domain_autotune(sxd_domain* domain, int system, const double* u, const d
   SXD_REQUIRE_DEVICE(inv_jac);
   SXD_REQUIRE_DEVICE(dt);
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (int rc = check_no_overlap(domain, system == SXD_SYSTEM_GH ? 50 : 5, u, dt, __func__);
+      rc != SXD_OK)
+    return rc;
+  Scratch& sc = stream_scratch(domain, s);
   hipEvent_t ev[2];
   if (hipEventCreate(&ev[0]) != hipSuccess) return fail(SXD_ERR_DEVICE, "hipEventCreate failed");
   if (hipEventCreate(&ev[1]) != hipSuccess) {
@@ -762,11 +861,11 @@ int sxd_domain_autotune(sxd_domain* domain, int system, const double* u, const d
     const int k = cand[ci];
     if (domain->mixed) domain->threads[system] = mixed_tiles[cj % ntiles];
     if (gh_fused_tiles) domain->threads[system] = fused_tiles[cj % ntiles];
-    rc = run_system(domain, system, k, u, params, inv_jac, dt, s);  // warm-up
+    rc = run_system(domain, sc, system, k, u, params, inv_jac, dt, s);  // warm-up
     if (rc != SXD_OK) break;
     if (hipEventRecord(ev[0], s) != hipSuccess) { rc = SXD_ERR_DEVICE; break; }
     for (int r = 0; r < repetitions && rc == SXD_OK; ++r)
-      rc = run_system(domain, system, k, u, params, inv_jac, dt, s);
+      rc = run_system(domain, sc, system, k, u, params, inv_jac, dt, s);
     if (rc != SXD_OK) break;
     float ms = 0.0f;
     if (hipEventRecord(ev[1], s) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
@@ -819,22 +918,24 @@ int sxd_gh_rhs_adjoint(const sxd_domain* domain, const double* u, const double* 
   SXD_REQUIRE_DEVICE(ubar);
   if (gbar == ubar || u == ubar) return fail(SXD_ERR_INVALID_ARGUMENT, "ubar must not alias");
   sxd_domain* d = const_cast<sxd_domain*>(domain);
+  Scratch& sc = stream_scratch(d, static_cast<hipStream_t>(stream));
   if (d->mixed) {
-    if (ensure_scratch(d, 150 * static_cast<size_t>(d->total_points), 0) != SXD_OK)
-      return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
+    if (int rc = reserve_scratch(sc, 150 * static_cast<size_t>(d->total_points), 0);
+        rc != SXD_OK)
+      return finish(rc, __func__);
     return finish(launch_mixed_gh_adjoint(d->num_elements, d->d_list, d->d_el, d->d_point_el,
                                           d->total_points, d->d_nbr,
                                           d->np_max, d->kf, d->km, u, gammas, inv_jac, gbar, ubar,
-                                          d->d_scratch_du, static_cast<hipStream_t>(stream)),
+                                          sc.du, static_cast<hipStream_t>(stream)),
                   __func__);
   }
   const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
   const size_t face_doubles = 300 * static_cast<size_t>(gh_adjoint_face_slots(d->ext));
-  if (ensure_scratch(d, 150 * np * d->num_elements, face_doubles * d->num_elements) != SXD_OK)
-    return fail(SXD_ERR_OUT_OF_MEMORY, "scratch allocation failed");
+  if (int rc = reserve_scratch(sc, 150 * np * d->num_elements, face_doubles * d->num_elements);
+      rc != SXD_OK)
+    return finish(rc, __func__);
   return finish(launch_gh_rhs_adjoint(d->num_elements, d->ext, d->d_nbr, u, gammas, inv_jac, gbar,
-                                      ubar, d->d_scratch_du, d->d_scratch_temps,
-                                      static_cast<hipStream_t>(stream)),
+                                      ubar, sc.du, sc.temps, static_cast<hipStream_t>(stream)),
                 __func__);
 }
 
@@ -862,6 +963,7 @@ struct sxd_rk3_stepper {
   double* d_rhs = nullptr;       // L0, L1, L2
   double* d_dt = nullptr;        // step size read by the update kernels
   double* h_dt = nullptr;        // pinned host copy of the step size
+  sxd::Scratch scratch;           // the stepper's own work arrays
   hipStream_t capture = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -876,6 +978,7 @@ void destroy_stepper(sxd_rk3_stepper* s) {
   if (s->d_rhs != nullptr) (void)hipFree(s->d_rhs);
   if (s->d_dt != nullptr) (void)hipFree(s->d_dt);
   if (s->h_dt != nullptr) (void)hipHostFree(s->h_dt);
+  s->scratch.release();
   delete s;
 }
 
@@ -890,8 +993,8 @@ int enqueue_rk3_step(sxd_rk3_stepper* s, const double* params, const double* inv
   const double* stage_in[3] = {s->u, s->d_stage, s->d_stage};
   double* stage_out[3] = {s->d_stage, s->d_stage, s->u};
   for (int k = 0; k < 3; ++k) {
-    int rc = run_system(s->domain, s->system, kernel, stage_in[k], params, inv_jac, s->d_rhs,
-                        stream);
+    int rc = run_system(s->domain, s->scratch, s->system, kernel, stage_in[k], params, inv_jac,
+                        s->d_rhs, stream);
     if (rc != SXD_OK) return rc;
     rc = launch_rk3_substep_dt(k, s->n, s->d_dt, s->u, s->d_stage, s->d_rhs, stage_out[k],
                                stream);
@@ -930,10 +1033,12 @@ int sxd_rk3_stepper_create(sxd_rk3_stepper** stepper, sxd_domain* domain, int sy
     destroy_stepper(s);
     return fail(SXD_ERR_DEVICE, "stream creation failed");
   }
-  // One RHS outside the capture allocates any work arrays and sets kernel
-  // attributes, so the captured region only enqueues kernels.
-  int rc = run_system(domain, system, resolve_kernel(domain, system), u, params, inv_jac,
-                      s->d_rhs, s->capture);
+  // One RHS outside the capture allocates the stepper's own work arrays and
+  // sets kernel attributes, so the captured region only enqueues kernels.
+  // The graph refers to those arrays only: calls on the domain afterwards
+  // (which may grow the domain's own work arrays) cannot move them.
+  int rc = run_system(domain, s->scratch, system, resolve_kernel(domain, system), u, params,
+                      inv_jac, s->d_rhs, s->capture);
   if (rc == SXD_OK && hipStreamSynchronize(s->capture) != hipSuccess) rc = SXD_ERR_DEVICE;
   if (rc != SXD_OK) {
     destroy_stepper(s);

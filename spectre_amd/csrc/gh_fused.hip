@@ -88,21 +88,42 @@ struct GhFusedLayout {
   static constexpr int ROWS = CPT * N;               // used rows (and K) of a tile
   static constexpr int KSTEPS = (ROWS + 3) / 4;
   static constexpr int TILES = (NF + 15) / 16;       // 16-pencil column tiles
-  // LDS (doubles)
-  static constexpr int kIntStride = 12;  // n[3] nv[3] lam[4] gamma2 lift
-  static constexpr int kExtStride = 12;  // n[3] nv[3] lam[4] gamma2, pad (16-byte rows)
+  // LDS (doubles).  Face records and corrections are field-major (field f
+  // of slot s at f * SLOTS + s): the lanes of a wave own consecutive slots,
+  // so their record accesses fall in distinct banks.
+  static constexpr int kIntFields = 12;  // n[3] nv[3] lam[4] gamma2 lift
+  static constexpr int kExtFields = 11;  // n[3] nv[3] lam[4] gamma2
   static constexpr int kFaceInt = 0;
-  static constexpr int kFaceExt = kFaceInt + SLOTS * kIntStride;
-  static constexpr int kStage = kFaceExt + SLOTS * kExtStride;
-  static constexpr int kCorrStride = 6;              // 5 corrections + pad (16-byte rows)
-  static constexpr int kCorr = kStage + (5 * NP + 1) / 2 * 2;  // 16-byte aligned
-  static constexpr int kDeriv = kCorr + SLOTS * kCorrStride;  // MFMA: logical derivatives
+  static constexpr int kFaceExt = kFaceInt + SLOTS * kIntFields;
+  static constexpr int kStage = (kFaceExt + SLOTS * kExtFields + 1) / 2 * 2;  // 16-byte aligned
+  static constexpr int kCorr = kStage + (5 * NP + 1) / 2 * 2;
+  static constexpr int kDeriv = kCorr + 5 * SLOTS;  // MFMA: logical derivatives
   // vector ALUs: second stage and correction buffers (odd passes), so a pass
   // can stage its fields while slower waves still read the previous pass's
-  static constexpr int kStage2 = kDeriv + (MFMA ? 9 * NP : 0);
+  static constexpr int kStage2 = (kDeriv + (MFMA ? 9 * NP : 0) + 1) / 2 * 2;
   static constexpr int kCorr2 = kStage2 + (MFMA ? 0 : (5 * NP + 1) / 2 * 2);
-  static constexpr int kDoubles = kCorr2 + (MFMA ? 0 : SLOTS * kCorrStride);
+  static constexpr int kDoubles = kCorr2 + (MFMA ? 0 : 5 * SLOTS);
+  // Staged rows (vector ALUs, N = 8) are swizzled: within row (j, k) the
+  // 16-byte pair s of xi values is stored at pair s ^ ((j >> 1) & 3).  The
+  // lanes of a wave read the xi rows j = 0..7 of one k (64 bytes apart) two
+  // values at a time; with the swizzle the eight rows' pairs fall in eight
+  // distinct groups of four banks instead of two.
+  static constexpr bool SWIZZLE = !MFMA && N == 8;
 };
+
+// Position within its row of xi index i of row j in the staged fields.
+template <int N, bool SWIZZLE>
+__device__ __forceinline__ int staged_i(int i, int j) {
+  if constexpr (SWIZZLE) return i ^ (2 * ((j >> 1) & 3));
+  return i;
+}
+
+// Staged position of volume point pt = i + N (j + N k).
+template <int N, bool SWIZZLE>
+__device__ __forceinline__ int staged_point(int pt) {
+  const int i = pt % N, j = (pt / N) % N;
+  return pt - i + staged_i<N, SWIZZLE>(i, j);
+}
 
 // Volume index of face point (a, b) of face `dir` (= 2 d + side).
 template <int N>
@@ -203,7 +224,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
 
   // points owned by this thread: pt[q] = t + q T
   bool act[P];
-  int pi_[P], pj[P], pk[P];
+  int pi_[P], pj[P], pk[P], spt[P];  // spt: staged position (vector-ALU engine)
 #pragma unroll
   for (int q = 0; q < P; ++q) {
     const int pt = t + q * T;
@@ -211,6 +232,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
     pi_[q] = pt % N;
     pj[q] = (pt / N) % N;
     pk[q] = act[q] ? pt / (N * N) : 0;
+    spt[q] = act[q] ? staged_point<N, L::SWIZZLE>(pt) : 0;
   }
 
   // ---- phase A -------------------------------------------------------------
@@ -244,7 +266,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
       const int side = idx[d] == 0 ? 0 : 1;
       const int a = d == 0 ? idx[1] : idx[0];
       const int b = d == 2 ? idx[1] : idx[2];
-      double* fi = smem + L::kFaceInt + ((2 * d + side) * NF + a + N * b) * L::kIntStride;
+      double* fi = smem + L::kFaceInt + (2 * d + side) * NF + a + N * b;
       GhFaceSide s;
       double mag;
 #pragma unroll
@@ -254,13 +276,13 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
       gh_char_speeds(s.lam, gamma1[q], st[q].lapse, st[q].shift, s.n);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        fi[c] = s.n[c];
-        fi[3 + c] = s.nv[c];
+        fi[c * L::SLOTS] = s.n[c];
+        fi[(3 + c) * L::SLOTS] = s.nv[c];
       }
 #pragma unroll
-      for (int c = 0; c < 4; ++c) fi[6 + c] = s.lam[c];
-      fi[10] = st[q].gamma2;
-      fi[11] = lift_factor(N, mag);
+      for (int c = 0; c < 4; ++c) fi[(6 + c) * L::SLOTS] = s.lam[c];
+      fi[10 * L::SLOTS] = st[q].gamma2;
+      fi[11 * L::SLOTS] = lift_factor(N, mag);
     }
   }
   // neighbour side of the face slots s = t + f T
@@ -307,15 +329,15 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
         gh_ghost_normal(s.n, s.nv, inv_sp);
         const double* go_ = gammas + e * 3 * NP + po;
         gh_char_speeds(s.lam, go_[NP], lapse, shift, s.n);
-        double* fe = smem + L::kFaceExt + slot * L::kExtStride;
+        double* fe = smem + L::kFaceExt + slot;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          fe[c] = s.n[c];
-          fe[3 + c] = s.nv[c];
+          fe[c * L::SLOTS] = s.n[c];
+          fe[(3 + c) * L::SLOTS] = s.nv[c];
         }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) fe[6 + c] = s.lam[c];
-        fe[10] = go_[2 * NP];
+        for (int c = 0; c < 4; ++c) fe[(6 + c) * L::SLOTS] = s.lam[c];
+        fe[10 * L::SLOTS] = go_[2 * NP];
         continue;
       }
     }
@@ -336,15 +358,15 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
     }
     gh_normalize(s.n, s.nv, &mag, inv_sp);
     gh_char_speeds(s.lam, gn[NP], lapse, shift, s.n);
-    double* fe = smem + L::kFaceExt + slot * L::kExtStride;
+    double* fe = smem + L::kFaceExt + slot;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      fe[c] = s.n[c];
-      fe[3 + c] = s.nv[c];
+      fe[c * L::SLOTS] = s.n[c];
+      fe[(3 + c) * L::SLOTS] = s.nv[c];
     }
 #pragma unroll
-    for (int c = 0; c < 4; ++c) fe[6 + c] = s.lam[c];
-    fe[10] = gn[2 * NP];
+    for (int c = 0; c < 4; ++c) fe[(6 + c) * L::SLOTS] = s.lam[c];
+    fe[10 * L::SLOTS] = gn[2 * NP];
   }
 
   // Derivative operands.  MFMA: the A operand block-diag(D, ..., D), step s
@@ -426,24 +448,24 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
       const int dir = slot / NF;
       const int fp = slot % NF;
       const int pv = face_volume_index<N>(dir, fp % N, fp / N);
-      const double* fi = smem + L::kFaceInt + slot * L::kIntStride;
-      const double* fe = smem + L::kFaceExt + slot * L::kExtStride;
+      const double* fi = smem + L::kFaceInt + slot;
+      const double* fe = smem + L::kFaceExt + slot;
       GhFaceSide si, se;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        si.n[c] = fi[c];
-        si.nv[c] = fi[3 + c];
-        se.n[c] = fe[c];
-        se.nv[c] = fe[3 + c];
+        si.n[c] = fi[c * L::SLOTS];
+        si.nv[c] = fi[(3 + c) * L::SLOTS];
+        se.n[c] = fe[c * L::SLOTS];
+        se.nv[c] = fe[(3 + c) * L::SLOTS];
       }
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        si.lam[c] = fi[6 + c];
-        se.lam[c] = fe[6 + c];
+        si.lam[c] = fi[(6 + c) * L::SLOTS];
+        se.lam[c] = fe[(6 + c) * L::SLOTS];
       }
-      si.gamma2 = fi[10];
-      se.gamma2 = fe[10];
-      const double fac = fi[11];
+      si.gamma2 = fi[10 * L::SLOTS];
+      se.gamma2 = fe[10 * L::SLOTS];
+      const double fac = fi[11 * L::SLOTS];
       double phi_i[3], phi_e[3], pki[13], pke[13], bc[5];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -454,7 +476,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
       gh_package_pair(pke, ext_v[5 * f], ext_v[5 * f + 1], phi_e, se);
       gh_boundary_terms_pair(bc, pki, si.lam, pke, se.lam);
 #pragma unroll
-      for (int c = 0; c < 5; ++c) corr[slot * L::kCorrStride + c] = bc[c] * fac;
+      for (int c = 0; c < 5; ++c) corr[c * L::SLOTS + slot] = bc[c] * fac;
     }
   };
   // add the corrections of the faces point q lies on (xi, eta, zeta order;
@@ -476,9 +498,9 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
       }
       const int a = d == 0 ? idx[1] : idx[0];
       const int b = d == 2 ? idx[1] : idx[2];
-      const double* cr = corr + ((2 * d + side) * NF + a + N * b) * L::kCorrStride;
+      const double* cr = corr + (2 * d + side) * NF + a + N * b;
 #pragma unroll
-      for (int c = 0; c < 5; ++c) out[c] = out[c] + cr[c];
+      for (int c = 0; c < 5; ++c) out[c] = out[c] + cr[c * L::SLOTS];
     }
   };
   if constexpr (MFMA) {
@@ -563,14 +585,9 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
           }
           const int a = d == 0 ? idx[1] : idx[0];
           const int b = d == 2 ? idx[1] : idx[2];
-          const double2* cr = reinterpret_cast<const double2*>(
-              cpp + ((2 * d + side) * NF + a + N * b) * L::kCorrStride);
-          const double2 c01 = cr[0], c23 = cr[1], c4x = cr[2];
-          prev[q][0] = prev[q][0] + c01.x;
-          prev[q][1] = prev[q][1] + c01.y;
-          prev[q][2] = prev[q][2] + c23.x;
-          prev[q][3] = prev[q][3] + c23.y;
-          prev[q][4] = prev[q][4] + c4x.x;
+          const double* cr = cpp + (2 * d + side) * NF + a + N * b;
+#pragma unroll
+          for (int c = 0; c < 5; ++c) prev[q][c] = prev[q][c] + cr[c * L::SLOTS];
         }
 #pragma unroll
         for (int c = 0; c < 5; ++c) dte[comp[c] * NP + t + q * T] = prev[q][c];
@@ -591,7 +608,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
       for (int q = 0; q < P; ++q)
         if (act[q]) {
 #pragma unroll
-          for (int c = 0; c < 5; ++c) stage[c * NP + t + q * T] = own_next[q][c];
+          for (int c = 0; c < 5; ++c) stage[c * NP + spt[q]] = own_next[q][c];
         }
       double ext_v[PF][5];
 #pragma unroll
@@ -608,23 +625,12 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
         const int slot = t + f * T;
         const int dir = slot / NF;
         const int fp = slot % NF;
-        const int pv = face_volume_index<N>(dir, fp % N, fp / N);
-        // both rows are 12 doubles on 16-byte boundaries: read in pairs
-        double fi[12], fe[12];
-        {
-          const double2* ri =
-              reinterpret_cast<const double2*>(smem + L::kFaceInt + slot * L::kIntStride);
-          const double2* re =
-              reinterpret_cast<const double2*>(smem + L::kFaceExt + slot * L::kExtStride);
+        const int pv = staged_point<N, L::SWIZZLE>(face_volume_index<N>(dir, fp % N, fp / N));
+        double fi[12], fe[11];
 #pragma unroll
-          for (int x = 0; x < 6; ++x) {
-            const double2 a = ri[x], b = re[x];
-            fi[2 * x] = a.x;
-            fi[2 * x + 1] = a.y;
-            fe[2 * x] = b.x;
-            fe[2 * x + 1] = b.y;
-          }
-        }
+        for (int x = 0; x < 12; ++x) fi[x] = smem[L::kFaceInt + x * L::SLOTS + slot];
+#pragma unroll
+        for (int x = 0; x < 11; ++x) fe[x] = smem[L::kFaceExt + x * L::SLOTS + slot];
         GhFaceSide si, se;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -651,7 +657,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
         gh_package_pair(pke, ext_v[f][0], ext_v[f][1], phi_e, se);
         gh_boundary_terms_pair(bc, pki, si.lam, pke, se.lam);
 #pragma unroll
-        for (int c = 0; c < 5; ++c) corr[slot * L::kCorrStride + c] = bc[c] * fac;
+        for (int c = 0; c < 5; ++c) corr[c * L::SLOTS + slot] = bc[c] * fac;
       }
       // volume terms of the owned points.  The logical derivatives of all
       // owned points come first: when a thread's two points share their
@@ -664,8 +670,9 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
         const double* sc = stage + c * NP;
         double zeta[N];
         if constexpr (kShareZeta) {
+          const int z0 = staged_i<N, L::SWIZZLE>(pi_[0], pj[0]) + N * pj[0];
 #pragma unroll
-          for (int l = 0; l < N; ++l) zeta[l] = sc[pi_[0] + N * (pj[0] + N * l)];
+          for (int l = 0; l < N; ++l) zeta[l] = sc[z0 + N * N * l];
         }
 #pragma unroll
         for (int q = 0; q < P; ++q) {
@@ -676,7 +683,8 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
           if constexpr (N % 2 == 0) {
 #pragma unroll
             for (int l = 0; l < N; l += 2) {
-              const double2 v = *reinterpret_cast<const double2*>(row + l);
+              const double2 v =
+                  *reinterpret_cast<const double2*>(row + staged_i<N, L::SWIZZLE>(l, j));
               xr[l] = v.x;
               xr[l + 1] = v.y;
             }
@@ -687,18 +695,20 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
           double d0 = Di[q][0] * xr[0];
 #pragma unroll
           for (int l = 1; l < N; ++l) d0 = fma(Di[q][l], xr[l], d0);
-          double d1 = Dj[q][0] * sc[i + N * N * k];
+          double d1 = Dj[q][0] * sc[staged_i<N, L::SWIZZLE>(i, 0) + N * N * k];
 #pragma unroll
-          for (int l = 1; l < N; ++l) d1 = fma(Dj[q][l], sc[i + N * (l + N * k)], d1);
+          for (int l = 1; l < N; ++l)
+            d1 = fma(Dj[q][l], sc[staged_i<N, L::SWIZZLE>(i, l) + N * (l + N * k)], d1);
           double d2;
           if constexpr (kShareZeta) {
             d2 = Dk[q][0] * zeta[0];
 #pragma unroll
             for (int l = 1; l < N; ++l) d2 = fma(Dk[q][l], zeta[l], d2);
           } else {
-            d2 = Dk[q][0] * sc[i + N * j];
+            const int z0 = staged_i<N, L::SWIZZLE>(i, j) + N * j;
+            d2 = Dk[q][0] * sc[z0];
 #pragma unroll
-            for (int l = 1; l < N; ++l) d2 = fma(Dk[q][l], sc[i + N * (j + N * l)], d2);
+            for (int l = 1; l < N; ++l) d2 = fma(Dk[q][l], sc[z0 + N * N * l], d2);
           }
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
@@ -713,7 +723,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
 #pragma unroll
       for (int q = 0; q < P; ++q) {
         if (!act[q]) continue;
-        const int pt = t + q * T;
+        const int pt = spt[q];
         // c = 0 g, 1 Pi, 2..4 Phi_n
         double phi[3], dphi[9];
 #pragma unroll

@@ -116,6 +116,75 @@ int launch_partial_derivatives(int num_elements, const int32_t ext[3], int nc,
 }
 
 // ---------------------------------------------------------------------------
+// Transpose of the partial derivatives for a fixed inverse Jacobian:
+//   xbar_j = sum_i (d xi^j / d x^i) dubar_{3c+i}              (i = 0, 1, 2)
+//   ubar_c = sum_{j=0..2} D_j^T xbar_j  (along dimension j)
+// Each point accumulates one chain: j = 0, 1, 2 in order and, within j,
+// l = 0 .. N_j - 1 (first term a product, then fused multiply-adds).
+// Grid: (E, component chunks); block 256; dynamic LDS: 3 Np doubles of xbar,
+// then the three D matrices.
+__global__ void __launch_bounds__(256)
+partial_derivatives_adjoint_kernel(int n0, int n1, int n2, int nc, int comps_per_block,
+                                   const double* __restrict__ dubar,
+                                   const double* __restrict__ inv_jac,
+                                   double* __restrict__ ubar, const double* __restrict__ dtab) {
+  extern __shared__ double s_x[];
+  const int np = n0 * n1 * n2;
+  const int64_t e = blockIdx.x;
+  const int c_begin = blockIdx.y * comps_per_block;
+  const int c_end = min(nc, c_begin + comps_per_block);
+  const int n[3] = {n0, n1, n2};
+  const double* Dm[3];
+  stage_diff_matrices(dtab, n, s_x + 3 * np, Dm);  // synchronised by the loop's first barrier
+  const double* ij = inv_jac + e * 9 * np;
+  for (int c = c_begin; c < c_end; ++c) {
+    const double* g = dubar + (e * 3 * nc + 3 * c) * np;
+    __syncthreads();
+    for (int p = threadIdx.x; p < np; p += blockDim.x) {
+      const double g0 = g[p], g1 = g[np + p], g2 = g[2 * np + p];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double s = ij[ij3(j, 0) * np + p] * g0;
+        s = fma(ij[ij3(j, 1) * np + p], g1, s);
+        s = fma(ij[ij3(j, 2) * np + p], g2, s);
+        s_x[j * np + p] = s;
+      }
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < np; p += blockDim.x) {
+      const int i = p % n0;
+      const int j = (p / n0) % n1;
+      const int k = p / (n0 * n1);
+      const double* x0 = s_x + n0 * (j + n1 * k);
+      double acc = Dm[0][i] * x0[0];
+      for (int l = 1; l < n0; ++l) acc = fma(Dm[0][l * n0 + i], x0[l], acc);
+      const double* x1 = s_x + np + i + n0 * n1 * k;
+      for (int l = 0; l < n1; ++l) acc = fma(Dm[1][l * n1 + j], x1[n0 * l], acc);
+      const double* x2 = s_x + 2 * np + i + n0 * j;
+      for (int l = 0; l < n2; ++l) acc = fma(Dm[2][l * n2 + k], x2[n0 * n1 * l], acc);
+      ubar[(e * nc + c) * np + p] = acc;
+    }
+  }
+}
+
+int launch_partial_derivatives_adjoint(int num_elements, const int32_t ext[3], int nc,
+                                       const double* dubar, const double* inv_jac,
+                                       double* ubar, hipStream_t stream) {
+  const double* dtab = device_diff_matrices();
+  if (dtab == nullptr) return SXD_ERR_DEVICE;
+  const int np = ext[0] * ext[1] * ext[2];
+  const int comps_per_block = nc < 4 ? nc : 4;
+  dim3 grid(num_elements, (nc + comps_per_block - 1) / comps_per_block);
+  const int d_doubles = ext[0] * ext[0] + ext[1] * ext[1] + ext[2] * ext[2];
+  const int lds = (3 * np + d_doubles) * static_cast<int>(sizeof(double));
+  if (!allow_lds(reinterpret_cast<const void*>(&partial_derivatives_adjoint_kernel), lds))
+    return SXD_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(partial_derivatives_adjoint_kernel, grid, dim3(256), lds, stream, ext[0],
+                     ext[1], ext[2], nc, comps_per_block, dubar, inv_jac, ubar, dtab);
+  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
+}
+
+// ---------------------------------------------------------------------------
 // Point-wise system terms on flat arrays of n points.
 __global__ void __launch_bounds__(256)
 sw_time_derivative_kernel(int64_t n, const double* __restrict__ u,

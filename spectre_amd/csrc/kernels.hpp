@@ -16,6 +16,12 @@ bool allow_lds(const void* kernel, int bytes);
 int launch_partial_derivatives(int num_elements, const int32_t ext[3], int nc,
                                const double* u, const double* inv_jac, double* du,
                                double* dlogical, hipStream_t stream);
+// ubar [E][nc][Np] = transpose of the partial derivatives applied to dubar
+// [E][3 nc][Np]; SXD_ERR_UNSUPPORTED if 3 Np doubles plus the D matrices
+// exceed the LDS.
+int launch_partial_derivatives_adjoint(int num_elements, const int32_t ext[3], int nc,
+                                       const double* dubar, const double* inv_jac,
+                                       double* ubar, hipStream_t stream);
 int launch_sw_time_derivative(int64_t n, const double* u, const double* du,
                               const double* gamma2, double* dt, double* gamma2_temp,
                               hipStream_t stream);

@@ -68,6 +68,29 @@ def partial_derivatives(u, extents, inv_jac, out=None):
     return out[0] if squeeze else out
 
 
+def partial_derivatives_adjoint(dubar, extents, inv_jac, out=None):
+    """Transpose of partial_derivatives for a fixed inverse Jacobian:
+    dubar [E, 3 nc, Np], inv_jac [E, 9, Np] -> ubar [E, nc, Np]."""
+    if dubar.dim() == 2:
+        dubar = dubar.unsqueeze(0)
+        inv_jac = inv_jac.unsqueeze(0)
+        squeeze = True
+    else:
+        squeeze = False
+    e, nc3, npts = dubar.shape
+    if npts != _np(extents) or nc3 % 3 != 0:
+        raise ValueError("dubar must be [E, 3 nc, Np] for these extents")
+    _require(dubar, "dubar")
+    _require(inv_jac, "inv_jac", (e, 9, npts))
+    if out is None:
+        out = torch.empty((e, nc3 // 3, npts), dtype=torch.float64, device=dubar.device)
+    _require(out, "out", (e, nc3 // 3, npts))
+    ext, ep = _extents(extents)
+    check(lib().sxd_partial_derivatives_adjoint(e, ep, nc3 // 3, _ptr(dubar), _ptr(inv_jac),
+                                                _ptr(out), _stream()))
+    return out[0] if squeeze else out
+
+
 def logical_partial_derivatives(u, extents, out=None):
     """u [E, nc, Np] -> [3, E, nc, Np] (d/dxi, d/deta, d/dzeta)."""
     if u.dim() == 2:
@@ -186,6 +209,21 @@ class Domain:
     @classmethod
     def from_brick(cls, brick):
         return cls(brick.extents, brick.neighbors)
+
+    @property
+    def handle(self):
+        """The sxd_domain pointer as an integer (for the pybind11 module)."""
+        if getattr(self, "_handle", None) is None or not self._handle.value:
+            raise SpectreAmdError(-1, "the domain is closed")
+        return int(self._handle.value)
+
+    def check_fields(self, system, u, params, inv_jac):
+        """Shape, dtype and device checks of a state and its parameters for
+        `system` ("scalar_wave" or "gh") on this domain."""
+        nc, n_par = (5, None) if system == "scalar_wave" else (50, 3)
+        _require(u, "u", self._shapes(nc))
+        _require(params, "params", self._scalar_shape() if n_par is None else self._shapes(n_par))
+        _require(inv_jac, "inv_jac", self._shapes(9))
 
     def close(self):
         if getattr(self, "_handle", None) is not None and self._handle.value:
@@ -384,6 +422,7 @@ class Domain:
 
 
 __all__ = ["SpectreAmdError", "partial_derivatives", "logical_partial_derivatives",
+           "partial_derivatives_adjoint",
            "scalar_wave_time_derivative", "gh_time_derivative", "scalar_wave_upwind_penalty",
            "gh_upwind_penalty", "lift_flux", "rk3_substep", "Domain"]
 

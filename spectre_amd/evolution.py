@@ -13,31 +13,43 @@ ScalarWave RHS is linear, so its step adjoint does not read those states and
 
 Reference: Time/TimeSteppers/Rk3HesthavenSsp.cpp:55-81 (the forward step);
 the reference has no adjoint.
-"""
-import ctypes
 
+Every device call of these Functions goes through the pybind11 module
+spectre_amd._Pybindings (the right-hand sides, their adjoints and the RK3
+substeps on the domain's handle), which calls the C ABI.
+"""
 import numpy as np
 import torch
 
+from . import _Pybindings as _pyb
 from . import dg
-from ._lib import check, lib
+
+
+def _rhs(domain, system):
+    """RHS callable fn(u, params, inv_jac, out) of `system` on `domain`."""
+    call = _pyb.scalar_wave_rhs if system == "scalar_wave" else _pyb.gh_rhs
+    return lambda u, params, inv_jac, out: call(domain.handle, u, params, inv_jac, out)
+
+
+def _sw_rhs_adjoint(domain):
+    return lambda gbar, gamma2, inv_jac, out: _pyb.scalar_wave_rhs_adjoint(
+        domain.handle, gbar, gamma2, inv_jac, out)
 
 
 def rk3_adjoint_substep(substep, dt, lam, a, mu, lam_out=None):
     """Transpose of one RK3 substep (sxd_rk3_adjoint_substep)."""
-    check(lib().sxd_rk3_adjoint_substep(int(substep), lam.numel(), float(dt), dg._ptr(lam),
-                                        dg._ptr(a), dg._ptr(mu), dg._ptr(lam_out), dg._stream()))
+    _pyb.rk3_adjoint_substep(int(substep), float(dt), lam, a, mu, lam_out)
     return mu, lam_out
 
 
 def rk3_step(domain, rhs, u, params, inv_jac, dt, stage, l):
     """One Rk3HesthavenSsp step of u in place (stage, l: work arrays)."""
     rhs(u, params, inv_jac, out=l)
-    dg.rk3_substep(0, dt, u, u, l, out=stage)
+    _pyb.rk3_substep(0, dt, u, u, l, stage)
     rhs(stage, params, inv_jac, out=l)
-    dg.rk3_substep(1, dt, u, stage, l, out=stage)
+    _pyb.rk3_substep(1, dt, u, stage, l, stage)
     rhs(stage, params, inv_jac, out=l)
-    dg.rk3_substep(2, dt, u, stage, l, out=u)
+    _pyb.rk3_substep(2, dt, u, stage, l, u)
 
 
 def rk3_step_adjoint(adjoint_rhs, lam3, params, inv_jac, dt, a, lam, mu):
@@ -57,13 +69,15 @@ class ScalarWaveEvolution(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u0, domain, gamma2, inv_jac, dt, steps, checkpoint_every=8,
                 recompute=False):
+        domain.check_fields("scalar_wave", u0, gamma2, inv_jac)
         u = u0.detach().clone()
         stage, l = torch.empty_like(u), torch.empty_like(u)
         checkpoints = []
+        rhs = _rhs(domain, "scalar_wave")
         for n in range(steps):
             if n % checkpoint_every == 0:
                 checkpoints.append(u.clone())
-            rk3_step(domain, domain.scalar_wave_rhs, u, gamma2, inv_jac, dt, stage, l)
+            rk3_step(domain, rhs, u, gamma2, inv_jac, dt, stage, l)
         ctx.domain, ctx.dt, ctx.steps = domain, dt, steps
         ctx.checkpoint_every, ctx.recompute = checkpoint_every, recompute
         ctx.checkpoints = checkpoints
@@ -86,10 +100,10 @@ class ScalarWaveEvolution(torch.autograd.Function):
                 states = []
                 for _ in range(first, last):
                     states.append(u.clone())
-                    rk3_step(domain, domain.scalar_wave_rhs, u, gamma2, inv_jac, dt, stage, l)
+                    rk3_step(domain, _rhs(domain, "scalar_wave"), u, gamma2, inv_jac, dt,
+                             stage, l)
             for _ in range(last - first):
-                rk3_step_adjoint(domain.scalar_wave_rhs_adjoint, lam3, gamma2, inv_jac, dt, a,
-                                 lam, mu)
+                rk3_step_adjoint(_sw_rhs_adjoint(domain), lam3, gamma2, inv_jac, dt, a, lam, mu)
                 lam3, mu = mu, lam3
         ctx.checkpoints = None
         return lam3, None, None, None, None, None, None, None
@@ -101,12 +115,12 @@ def rk3_step_states(rhs, u, params, inv_jac, dt, l, complete=True):
     step's result is not needed."""
     u0 = u.clone() if complete else u
     rhs(u, params, inv_jac, out=l)
-    u1 = dg.rk3_substep(0, dt, u, u, l)
+    u1 = _pyb.rk3_substep(0, dt, u, u, l, torch.empty_like(u))
     rhs(u1, params, inv_jac, out=l)
-    u2 = dg.rk3_substep(1, dt, u, u1, l)
+    u2 = _pyb.rk3_substep(1, dt, u, u1, l, torch.empty_like(u))
     if complete:
         rhs(u2, params, inv_jac, out=l)
-        dg.rk3_substep(2, dt, u, u2, l, out=u)
+        _pyb.rk3_substep(2, dt, u, u2, l, u)
     return u0, u1, u2
 
 
@@ -124,17 +138,19 @@ class GhEvolution(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, u0, domain, gammas, inv_jac, dt, steps, checkpoint_every=8, overlap=None):
+        domain.check_fields("gh", u0, gammas, inv_jac)
         u = u0.detach().clone()
         l = torch.empty_like(u)
         checkpoints, last_stages = [], []
         last_first = (max(steps - 1, 0) // checkpoint_every) * checkpoint_every
+        rhs = _rhs(domain, "gh")
         for n in range(steps):
             if n % checkpoint_every == 0:
                 checkpoints.append(u.clone())
             if n >= last_first:
-                last_stages.append(rk3_step_states(domain.gh_rhs, u, gammas, inv_jac, dt, l))
+                last_stages.append(rk3_step_states(rhs, u, gammas, inv_jac, dt, l))
             else:
-                rk3_step(domain, domain.gh_rhs, u, gammas, inv_jac, dt, torch.empty_like(u), l)
+                rk3_step(domain, rhs, u, gammas, inv_jac, dt, torch.empty_like(u), l)
         ctx.domain, ctx.dt, ctx.steps, ctx.k = domain, dt, steps, checkpoint_every
         # overlap the recompute with the adjoint (see backward): on by default
         # for mixed-p domains; the uniform fused RHS kernel fills whole CUs
@@ -172,7 +188,7 @@ class GhEvolution(torch.autograd.Function):
                 l2 = torch.empty_like(u)
                 # the segment's last step is not completed: its result is
                 # the next checkpoint
-                stages = [rk3_step_states(twin.gh_rhs, u, gammas, inv_jac, dt, l2,
+                stages = [rk3_step_states(_rhs(twin, "gh"), u, gammas, inv_jac, dt, l2,
                                           complete=n < last - 1)
                           for n in range(first, last)]
                 done = None
@@ -193,12 +209,13 @@ class GhEvolution(torch.autograd.Function):
                         for t in st:
                             t.record_stream(main)
                 pending = recompute(seg - 1) if seg > 0 and side is not None else None
+            h = domain.handle
             for s0, s1, s2 in reversed(stages):
-                domain.gh_rhs_adjoint(s2, gammas, inv_jac, lam3, out=a)
+                _pyb.gh_rhs_adjoint(h, s2, gammas, inv_jac, lam3, a)
                 rk3_adjoint_substep(2, dt, lam3, a, mu, lam)
-                domain.gh_rhs_adjoint(s1, gammas, inv_jac, lam, out=a)
+                _pyb.gh_rhs_adjoint(h, s1, gammas, inv_jac, lam, a)
                 rk3_adjoint_substep(1, dt, lam, a, mu, lam)
-                domain.gh_rhs_adjoint(s0, gammas, inv_jac, lam, out=a)
+                _pyb.gh_rhs_adjoint(h, s0, gammas, inv_jac, lam, a)
                 rk3_adjoint_substep(0, dt, lam, a, mu, None)
                 lam3, mu = mu, lam3
         ctx.checkpoints = ctx.last_stages = None

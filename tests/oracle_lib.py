@@ -70,6 +70,18 @@ def partial_derivatives(u, extents, inv_jac):
     return du
 
 
+def partial_derivatives_adjoint(dubar, extents, inv_jac):
+    """dubar [3 nc, Np], inv_jac [9, Np] -> ubar [nc, Np] (one element): the
+    transpose of partial_derivatives for this inverse Jacobian."""
+    dubar = np.ascontiguousarray(dubar, dtype=np.float64)
+    inv_jac = np.ascontiguousarray(inv_jac, dtype=np.float64)
+    nc = dubar.shape[0] // 3
+    ubar = np.empty((nc, dubar.shape[1]))
+    ext, ep = _i(extents)
+    _check(lib().sxo_partial_derivatives_adjoint(_d(dubar), nc, ep, _d(inv_jac), _d(ubar)))
+    return ubar
+
+
 def logical_partial_derivatives(u, extents):
     u = np.ascontiguousarray(u, dtype=np.float64)
     nc = u.shape[0]

@@ -520,9 +520,9 @@ def test_gh_rhs_mixed_p(kernel):
 
 
 def test_mixed_p_full_size_batched_equals_per_extents_and_oracle():
-    """1000 elements, extents from {4..12}: the batched launch, the per-extents
-    launches and the oracle agree (ScalarWave everywhere; GH on the GPU forms
-    and against the oracle on a sample of elements' dt blocks)."""
+    """ScalarWave on the full mixed-p workload (1000 elements, extents from
+    {4..12}): the autotuned batched launch and the per-extents launches both
+    equal the oracle bit for bit."""
     brick, u, g2, inv = wl.scalar_wave_plane_wave_mixed(seed=0)
     domain = dg.MixedDomain.from_brick(brick)
     ref = ol.sw_rhs_mixed(brick.extents, brick.neighbors, u, g2, inv)
@@ -533,6 +533,77 @@ def test_mixed_p_full_size_batched_equals_per_extents_and_oracle():
     assert np.array_equal(got, ref)
     domain.set_kernel("scalar_wave", "mixed_per_extents")
     assert np.array_equal(host(domain.scalar_wave_rhs(d_u, d_g2, d_inv)), ref)
+
+
+def _face_and_mortar_sizes(brick):
+    """Largest face (kf) and mortar (km) point counts of a mixed brick, by the
+    rule of sxd_domain_create_mixed (mortar extents = max per dimension)."""
+    ext = np.asarray(brick.extents).reshape(-1, 3)
+    kf = km = 0
+    for e in range(brick.num_elements):
+        for dirn in range(6):
+            d = dirn // 2
+            a = ext[e]
+            nb = brick.neighbors[e, dirn]
+            b = a if nb < 0 else ext[nb]
+            fa, fb = (a[1] if d == 0 else a[0]), (a[1] if d == 2 else a[2])
+            ga, gb = (b[1] if d == 0 else b[0]), (b[1] if d == 2 else b[2])
+            kf = max(kf, fa * fb, ga * gb)
+            km = max(km, max(fa, ga) * max(fb, gb))
+    return int(kf), int(km)
+
+
+def _gh_face_pass_uses_split_plan(kf, km):
+    """True where the GH mixed face passes run their split-buffer variant:
+    the per-workgroup LDS plan with a neighbour mortar buffer (18 kf + 51 km
+    doubles) fits as many workgroups per CU (160 KiB) as the plan without it
+    (18 kf + 34 km doubles)."""
+    split, fused = (18 * kf + 51 * km) * 8, (18 * kf + 34 * km) * 8
+    return 160 * 1024 // fused <= 160 * 1024 // split
+
+
+@pytest.mark.parametrize("kernel", ["mixed_batched", "mixed_per_extents"])
+def test_gh_rhs_mixed_p_full_size_matches_oracle(kernel):
+    """GH on the full mixed-p workload (1000 elements, extents from {4..12},
+    Kerr-Schild plus noise): 12 x 12 faces and mortars select the face-pass
+    variant with a neighbour mortar buffer; both launch forms equal the
+    oracle bit for bit."""
+    brick, u, gammas, inv = wl.gh_kerr_schild_mixed(seed=0)
+    kf, km = _face_and_mortar_sizes(brick)
+    assert (kf, km) == (144, 144)
+    assert _gh_face_pass_uses_split_plan(kf, km)
+    ref = ol.gh_rhs_mixed(brick.extents, brick.neighbors, u, gammas, inv)
+    domain = dg.MixedDomain.from_brick(brick)
+    domain.set_kernel("gh", kernel)
+    got = host(domain.gh_rhs(dev(u), dev(gammas), dev(inv)))
+    assert rel_err(got, ref) <= TOL
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("threads", [64, 128, 256])
+def test_gh_rhs_mixed_p_12x12_mortars(threads):
+    """A 2x2x2 brick whose element 0 has 12^3 points and the others extents
+    from {4..12}: every face of element 0 and its mortars hold 12 x 12 points
+    (the split-buffer face-pass variant); batched and per-extents launches
+    equal the oracle bit for bit at every tile width."""
+    ext = wl.random_extents(8, 4, 12, seed=11)
+    ext[0] = (12, 12, 12)
+    brick = wl.MixedBrick((3.0, -4.0, -4.0), (11.0, 4.0, 4.0), (2, 2, 2), ext)
+    u = brick.ragged([wl.gh_kerr_schild(brick.element_coordinates(e)[None])[0]
+                      for e in range(brick.num_elements)])
+    u *= 1.0 + 1e-6 * np.random.default_rng(11).standard_normal(u.shape)
+    gammas = brick.ragged([np.stack([np.ones(n), -np.ones(n), np.ones(n)])
+                           for n in brick.points])
+    inv = brick.inverse_jacobian()
+    kf, km = _face_and_mortar_sizes(brick)
+    assert (kf, km) == (144, 144) and _gh_face_pass_uses_split_plan(kf, km)
+    ref = ol.gh_rhs_mixed(brick.extents, brick.neighbors, u, gammas, inv)
+    domain = dg.MixedDomain.from_brick(brick)
+    domain.set_threads("gh", threads)
+    d_u, d_g, d_inv = dev(u), dev(gammas), dev(inv)
+    for kernel in ("mixed_batched", "mixed_per_extents"):
+        domain.set_kernel("gh", kernel)
+        assert np.array_equal(host(domain.gh_rhs(d_u, d_g, d_inv)), ref), kernel
 
 
 # ---- RK3 steps captured in a HIP graph ----------------------------------------
@@ -611,6 +682,74 @@ def test_rk3_graph_gh():
     stepper.step(0.002, 2)
     assert np.array_equal(host(d_u), ref)
     stepper.close()
+
+
+def test_rk3_graph_gh_mixed_p():
+    """GH on a mixed-p brick (extents 4..12) stepped by the HIP-graph RK3
+    stepper equals the oracle's RK3 bit for bit, with eager RHS calls on the
+    same domain between replays."""
+    brick, u, gammas, inv = wl.gh_kerr_schild_mixed(seed=4, elements=(2, 2, 2), lowest=4,
+                                                    highest=12)
+    ref = oracle_rk3(lambda v: ol.gh_rhs_mixed(brick.extents, brick.neighbors, v, gammas, inv),
+                     u, 0.002, 2)
+    domain = dg.MixedDomain.from_brick(brick)
+    d_u, d_g, d_inv = dev(u), dev(gammas), dev(inv)
+    stepper = dg.Rk3Stepper(domain, "gh", d_u, d_g, d_inv)
+    stepper.step(0.002, 1)
+    domain.gh_rhs(d_u, d_g, d_inv)
+    torch.cuda.synchronize()
+    stepper.step(0.002, 1)
+    assert np.array_equal(host(d_u), ref)
+    stepper.close()
+
+
+def test_rk3_stepper_keeps_its_work_arrays_when_the_domain_grows_its_own():
+    """A stepper on the composite GH path, then calls on the same domain that
+    need larger work arrays (the GH adjoint's face slots) and a ScalarWave
+    composite stepper followed by a GH call: the steppers' graphs still give
+    the oracle's RK3 bit for bit."""
+    brick, u, gammas, inv = wl.gh_kerr_schild_brick(seed=6, elements=(2, 2, 2), extents=(5, 5, 5))
+    ref = oracle_rk3(lambda v: ol.gh_rhs(brick.extents, brick.neighbors, v, gammas, inv), u,
+                     0.002, 2)
+    domain = dg.Domain.from_brick(brick)
+    domain.set_kernel("gh", "composite")
+    d_u, d_g, d_inv = dev(u), dev(gammas), dev(inv)
+    stepper = dg.Rk3Stepper(domain, "gh", d_u, d_g, d_inv)
+    stepper.step(0.002, 1)
+    torch.cuda.synchronize()
+    u_lin = dev(u)
+    gbar = dev(np.random.default_rng(6).standard_normal(u.shape))
+    domain.gh_rhs_adjoint(u_lin, d_g, d_inv, gbar)
+    torch.cuda.synchronize()
+    stepper.step(0.002, 1)
+    assert np.array_equal(host(d_u), ref)
+    stepper.close()
+
+    sb, su, sg2, sinv = wl.scalar_wave_plane_wave_3d(seed=6, elements=(2, 2, 2), extents=(5, 5, 5))
+    sref = oracle_rk3(lambda v: ol.sw_rhs(sb.extents, sb.neighbors, v, sg2, sinv), su, 0.01, 2)
+    sdomain = dg.Domain.from_brick(sb)
+    sdomain.set_kernel("scalar_wave", "composite")
+    sdomain.set_kernel("gh", "composite")
+    d_su = dev(su)
+    sstep = dg.Rk3Stepper(sdomain, "scalar_wave", d_su, dev(sg2), dev(sinv))
+    sstep.step(0.01, 1)
+    sdomain.gh_rhs(dev(u), d_g, d_inv)  # grows the domain's derivative array 10x
+    torch.cuda.synchronize()
+    sstep.step(0.01, 1)
+    assert np.array_equal(host(d_su), sref)
+    sstep.close()
+
+
+def test_rhs_rejects_dt_overlapping_u():
+    brick, u, gammas, inv = wl.gh_kerr_schild_brick(seed=0, elements=(2, 1, 1), extents=(4, 4, 4))
+    domain = dg.Domain.from_brick(brick)
+    d_u, d_g, d_inv = dev(u), dev(gammas), dev(inv)
+    with pytest.raises(dg.SpectreAmdError, match="overlaps"):
+        domain.gh_rhs(d_u, d_g, d_inv, out=d_u)
+    flat = torch.cat([d_u.flatten(), d_u.flatten()])
+    shifted = flat[64:64 + d_u.numel()].view_as(d_u)
+    with pytest.raises(dg.SpectreAmdError, match="overlaps"):
+        domain.gh_rhs(flat[:d_u.numel()].view_as(d_u), d_g, d_inv, out=shifted)
 
 
 # ---- ScalarWave adjoint ----------------------------------------------------------

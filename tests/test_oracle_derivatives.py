@@ -89,3 +89,37 @@ def test_batch_equals_single():
     batch = ol.partial_derivatives_batch(u, extents, inv_jac)
     for e in range(3):
         assert np.array_equal(batch[e], ol.partial_derivatives(u[e], extents, inv_jac[e]))
+
+
+@pytest.mark.parametrize("extents,nc", [((3, 4, 2), 2), ((4, 4, 4), 1), ((2, 5, 3), 3)])
+def test_adjoint_is_the_transpose_of_the_assembled_operator(extents, nc):
+    """The transposed partial derivatives equal A^T w, with A assembled column
+    by column from the forward oracle (unit vectors through
+    partial_derivatives) and a general inverse Jacobian: a transposed D index,
+    a swapped inverse-Jacobian index or a dropped direction fails this."""
+    npts = int(np.prod(extents))
+    rng = np.random.default_rng(sum(extents) + nc)
+    inv_jac = rng.uniform(-2.0, 2.0, (9, npts))
+    n_in = nc * npts
+    a = np.empty((3 * nc * npts, n_in))
+    for col in range(n_in):
+        e = np.zeros(n_in)
+        e[col] = 1.0
+        a[:, col] = ol.partial_derivatives(e.reshape(nc, npts), extents, inv_jac).ravel()
+    w = rng.standard_normal((3 * nc, npts))
+    got = ol.partial_derivatives_adjoint(w, extents, inv_jac)
+    ref = (a.T @ w.ravel()).reshape(nc, npts)
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def test_adjoint_dot_product_identity():
+    """<A u, w> = <u, A^T w> on a larger anisotropic mesh."""
+    extents = (7, 5, 9)
+    npts = int(np.prod(extents))
+    rng = np.random.default_rng(5)
+    inv_jac = rng.uniform(-2.0, 2.0, (9, npts))
+    u = rng.standard_normal((4, npts))
+    w = rng.standard_normal((12, npts))
+    lhs = float(np.sum(ol.partial_derivatives(u, extents, inv_jac) * w))
+    rhs = float(np.sum(u * ol.partial_derivatives_adjoint(w, extents, inv_jac)))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
