@@ -256,6 +256,22 @@ int sxd_scalar_wave_rhs_constraint_preserving_spherical_radiation(
 int sxd_gh_rhs(const sxd_domain* domain, const double* u, const double* gammas,
                const double* inv_jac, double* dt, void* stream);
 
+/* sxd_gh_rhs without any device allocation: the composite and mixed-p kernel
+ * organisations use a fixed work array of 16 Mi doubles (128 MiB) that the
+ * library holds on each device it runs on until the process ends, instead of
+ * the domain's per-stream work arrays.  No counterpart in the reference; it
+ * is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The domain (and so its
+ * device tables) must already exist.  SXD_ERR_UNSUPPORTED if the domain's
+ * work arrays exceed the fixed array: 150 doubles per grid point on mixed-p
+ * domains (about 111,000 points), 162 per point with the composite kernel,
+ * none with the fused kernels.  Every call on one device shares that device's
+ * array, which keeps the last call's contents: calls that overlap on one GPU
+ * (other streams, other domains, other threads) race and must be serialised
+ * by the caller.  Results equal sxd_gh_rhs bit for bit. */
+int sxd_gh_rhs_noalloc(const sxd_domain* domain, const double* u, const double* gammas,
+                       const double* inv_jac, double* dt, void* stream);
+
 /* sxd_gh_rhs on a domain with external faces (neighbour -1), closed by the
  * generalized-harmonic Dirichlet analytic ghost condition: on an external face
  * the exterior UpwindPenalty state is ghost's g_ab, Pi_ab, Phi_iab at the

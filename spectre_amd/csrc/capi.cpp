@@ -776,6 +776,40 @@ int sxd_gh_rhs(const sxd_domain* domain, const double* u, const double* gammas,
                 __func__);
 }
 
+int sxd_gh_rhs_noalloc(const sxd_domain* domain, const double* u, const double* gammas,
+                       const double* inv_jac, double* dt, void* stream) {
+  if (int rc = check_periodic(domain); rc != SXD_OK) return rc;
+  sxd_domain* d = const_cast<sxd_domain*>(domain);
+  const int kernel = resolve_kernel(d, SXD_SYSTEM_GH);
+  // work arrays the kernel organisation needs: 150 doubles per point (mixed
+  // p), 162 (composite), none (fused)
+  const size_t points = static_cast<size_t>(domain_points(d));
+  size_t need = 0;
+  if (d->mixed)
+    need = 150 * points;
+  else if (kernel == SXD_KERNEL_COMPOSITE)
+    need = 162 * points;
+  if (need > noalloc_scratch_doubles())
+    return fail(SXD_ERR_UNSUPPORTED,
+                "%s: the domain's work arrays need %zu doubles, above the fixed array's %zu "
+                "(16 Mi doubles, 128 MiB)",
+                __func__, need, noalloc_scratch_doubles());
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(gammas);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  SXD_REQUIRE_DEVICE(dt);
+  if (int rc = check_no_overlap(d, 50, u, dt, __func__); rc != SXD_OK) return rc;
+  Scratch sc;
+  sc.fixed = true;
+  sc.fixed_doubles = noalloc_scratch_doubles();
+  if (need > 0) {
+    if (int rc = noalloc_scratch(&sc.du); rc != SXD_OK)
+      return fail(rc, "%s: the fixed work array has no address on this device", __func__);
+  }
+  return finish(run_gh(d, sc, kernel, u, gammas, inv_jac, dt, static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
 int sxd_gh_rhs_dirichlet(const sxd_domain* domain, const double* u, const double* gammas,
                          const double* inv_jac, const double* ghost, double* dt,
                          void* stream) {

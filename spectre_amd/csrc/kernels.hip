@@ -10,7 +10,6 @@
 
 #include <cstdint>
 #include <mutex>
-#include <set>
 #include <utility>
 
 #include "device_common.hpp"
@@ -83,16 +82,24 @@ partial_derivatives_kernel(int n0, int n1, int n2, int nc, int comps_per_block,
 bool allow_lds(const void* kernel, int bytes) {
   if (bytes <= 64 * 1024) return true;
   if (bytes > 160 * 1024) return false;
+  // (device, kernel) pairs already enabled, in a fixed table (no allocation,
+  // so that the allocation-free entry point stays allocation-free)
+  struct Done {
+    int dev;
+    const void* kernel;
+  };
   static std::mutex mutex;
-  static std::set<std::pair<int, const void*>> done;
+  static Done done[256];
+  static int count = 0;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return false;
   std::lock_guard<std::mutex> lock(mutex);
-  if (done.count({dev, kernel}) != 0) return true;
+  for (int i = 0; i < count; ++i)
+    if (done[i].dev == dev && done[i].kernel == kernel) return true;
   if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) !=
       hipSuccess)
     return false;
-  done.insert({dev, kernel});
+  if (count < 256) done[count++] = {dev, kernel};
   return true;
 }
 

@@ -124,5 +124,9 @@ int launch_rk3_substep_dt(int substep, int64_t n, const double* dt, const double
                           const double* uk, const double* lk, double* out, hipStream_t stream);
 int launch_rk3_substep(int substep, int64_t n, double dt, const double* u0, const double* uk,
                        const double* lk, double* out, hipStream_t stream);
+// The allocation-free entry point's fixed work array (noalloc.hip): its
+// capacity in doubles, and its device address on the current device.
+size_t noalloc_scratch_doubles();
+int noalloc_scratch(double** ptr);
 
 }  // namespace sxd

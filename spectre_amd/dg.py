@@ -332,6 +332,19 @@ class Domain:
                                _stream()))
         return out
 
+    def gh_rhs_noalloc(self, u, gammas, inv_jac, out):
+        """gh_rhs through sxd_gh_rhs_noalloc: no device allocation (the
+        library's fixed per-device work array of 16 Mi doubles replaces the
+        domain's work arrays; calls that overlap on one GPU race).  `out` must
+        be given, since allocating it is the caller's business."""
+        _require(u, "u", self._shapes(50))
+        _require(gammas, "gammas", self._shapes(3))
+        _require(inv_jac, "inv_jac", self._shapes(9))
+        _require(out, "out", self._shapes(50))
+        check(lib().sxd_gh_rhs_noalloc(self._handle, _ptr(u), _ptr(gammas), _ptr(inv_jac),
+                                       _ptr(out), _stream()))
+        return out
+
     def gh_rhs_dirichlet(self, u, gammas, inv_jac, ghost, out=None):
         """gh_rhs with external faces (neighbour -1) closed by the Dirichlet
         analytic ghost state: ghost [E, 50, Np] holds the analytic g, Pi, Phi
