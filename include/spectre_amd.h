@@ -93,6 +93,56 @@ int sxd_partial_derivatives_adjoint(int num_elements, const int32_t extents[3], 
                                     const double* dubar, const double* inv_jac,
                                     double* ubar, void* stream);
 
+/* ---- Flux systems: divergence and moving-mesh volume terms ------------- */
+
+/* Fluxes use the layout of a Variables of flux tensors: per element 3 nc
+ * blocks of Np, block i + 3 c = F^i of component c (the flux's spatial upper
+ * index is its first, fastest index). */
+
+/* Strong-form divergence div [E][nc][Np] of the fluxes,
+ * div_c = sum_{i0} sum_d (d xi^d / d x^{i0}) d_d F^{i0}_c.
+ * Replaces divergence(not_null<Variables<div tags>*>,
+ * const Variables<flux tags>&, const Mesh<3>&, const InverseJacobian&)
+ * (NumericalAlgorithms/LinearOperators/Divergence.tpp:50-94).
+ * SXD_ERR_UNSUPPORTED if 3 Np doubles exceed the LDS (Np above ~6500). */
+int sxd_divergence(int num_elements, const int32_t extents[3], int nc, const double* fluxes,
+                   const double* inv_jac, double* div, void* stream);
+
+/* Weak-form divergence: with G_j = sum_i (det J d xi^j / d x^i) F^i, the weak
+ * flux differentiation matrix W_ab = D_ba w_b / w_a applied along each
+ * logical direction j and summed.  det_jac_times_inv_jac [E][9][Np] in the
+ * inverse-Jacobian layout.  Replaces weak_divergence
+ * (NumericalAlgorithms/LinearOperators/WeakDivergence.hpp:40-231, 3-d
+ * branch; matrix Spectral.cpp:462-485). */
+int sxd_weak_divergence(int num_elements, const int32_t extents[3], int nc,
+                        const double* fluxes, const double* det_jac_times_inv_jac, double* div,
+                        void* stream);
+
+/* The flux-variable part of the DG volume terms
+ * (Evolution/DiscontinuousGalerkin/Actions/VolumeTermsImpl.tpp:156-200,
+ * 264-291) for a system whose nc evolved components all have fluxes, on dt
+ * [E][nc][Np] (which holds the system's source terms on entry):
+ *  - moving mesh (mesh_velocity [E][3][Np] non-null): F^i_c -= u_c v^i
+ *    (fluxes updated in place, as the reference updates its flux buffer) and
+ *    dt_c -= u_c div v (div_mesh_velocity [E][Np]);
+ *  - then dt -= div F (SXD_DG_STRONG_INERTIAL, inv_jac) or
+ *    dt += det_inv_jac weak_div F (SXD_DG_WEAK_INERTIAL, det_jac_times_inv_jac
+ *    and det_inv_jac [E][Np]).
+ * Pointers a formulation does not use may be null. */
+enum { SXD_DG_STRONG_INERTIAL = 0, SXD_DG_WEAK_INERTIAL = 1 };
+int sxd_flux_volume_terms(int num_elements, const int32_t extents[3], int nc, const double* u,
+                          double* fluxes, const double* mesh_velocity,
+                          const double* div_mesh_velocity, int formulation,
+                          const double* inv_jac, const double* det_jac_times_inv_jac,
+                          const double* det_inv_jac, double* dt, void* stream);
+
+/* Moving-mesh term of evolved variables without fluxes (ScalarWave, GH):
+ * dt_c += v^i d_i u_c on num_points points; du [3 nc][n] in the
+ * partial_derivatives layout, mesh_velocity [3][n], dt [nc][n]
+ * (VolumeTermsImpl.tpp:235-262). */
+int sxd_mesh_velocity_nonconservative(int64_t num_points, int nc, const double* du,
+                                      const double* mesh_velocity, double* dt, void* stream);
+
 /* ---- Point-wise system terms ------------------------------------------ */
 
 /* ScalarWave::TimeDerivative<3>::apply over num_points points

@@ -9,6 +9,7 @@
 
 #include "gh_reference_impl.hpp"
 #include "sxo.hpp"
+#include "sxo_flux.hpp"
 
 namespace {
 bool bad_extents(const int* ext) {
@@ -46,6 +47,44 @@ int sxo_differentiation_matrix(int n, double* d) {
   if (n < 2 || n > 20) return -1;
   const auto& m = sxo::SpectralCache::get().D[n];
   std::memcpy(d, m.data(), m.size() * sizeof(double));
+  return 0;
+}
+
+int sxo_weak_flux_differentiation_matrix(int n, double* w) {
+  if (n < 2 || n > 20) return -1;
+  const std::vector<double> m = sxo::weak_flux_differentiation_matrix(n);
+  std::memcpy(w, m.data(), m.size() * sizeof(double));
+  return 0;
+}
+
+// One element: fluxes [3 nc][Np], inv_j / djij [9][Np] -> div [nc][Np].
+int sxo_divergence(const double* F, int nc, const int* ext, const double* inv_j, double* div) {
+  if (bad_extents(ext) || nc < 1) return -1;
+  sxo::divergence(F, nc, ext, inv_j, div);
+  return 0;
+}
+
+int sxo_weak_divergence(const double* F, int nc, const int* ext, const double* djij,
+                        double* div) {
+  if (bad_extents(ext) || nc < 1) return -1;
+  sxo::weak_divergence(F, nc, ext, djij, div);
+  return 0;
+}
+
+// One element; F updated in place, dt [nc][Np] updated in place.  v / div_v
+// may be null (static mesh); weak != 0 selects the weak form.
+int sxo_flux_volume_terms(int nc, const int* ext, const double* u, double* F, const double* v,
+                          const double* div_v, int weak, const double* inv_j,
+                          const double* djij, const double* det_inv_jac, double* dt) {
+  if (bad_extents(ext) || nc < 1) return -1;
+  sxo::flux_volume_terms(nc, ext, u, F, v, div_v, weak != 0, inv_j, djij, det_inv_jac, dt);
+  return 0;
+}
+
+int sxo_mesh_velocity_nonconservative(long n, int nc, const double* du, const double* v,
+                                      double* dt) {
+  if (n < 0 || nc < 1) return -1;
+  sxo::mesh_velocity_nonconservative(static_cast<size_t>(n), nc, du, v, dt);
   return 0;
 }
 

@@ -34,6 +34,11 @@ _SIGNATURES = {
     "sxd_partial_derivatives": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _vp, _vp, _vp, _vp]),
     "sxd_partial_derivatives_adjoint": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _vp, _vp,
                                                        _vp, _vp]),
+    "sxd_divergence": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _vp, _vp, _vp, _vp]),
+    "sxd_weak_divergence": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _vp, _vp, _vp, _vp]),
+    "sxd_flux_volume_terms": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _vp, _vp, _vp, _vp,
+                                             ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "sxd_mesh_velocity_nonconservative": (ctypes.c_int, [_i64, ctypes.c_int, _vp, _vp, _vp, _vp]),
     "sxd_scalar_wave_time_derivative": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_gh_time_derivative": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_scalar_wave_upwind_penalty": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

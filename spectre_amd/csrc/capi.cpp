@@ -198,6 +198,11 @@ const double* device_mortar_matrices() {
   return device_table(mortar_tables(), packed_mortar_matrices());
 }
 
+const double* device_weak_flux_matrices() {
+  static DeviceTables t;
+  return device_table(t, packed_weak_flux_matrices());
+}
+
 }  // namespace sxd
 
 using namespace sxd;
@@ -269,6 +274,90 @@ int sxd_partial_derivatives_adjoint(int num_elements, const int32_t extents[3], 
   if (ubar == dubar) return fail(SXD_ERR_INVALID_ARGUMENT, "ubar must not alias dubar");
   return finish(launch_partial_derivatives_adjoint(num_elements, extents, nc, dubar, inv_jac,
                                                    ubar, static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+namespace {
+int check_divergence_args(int num_elements, const int32_t* extents, int nc) {
+  if (!valid_extents(extents))
+    return fail(SXD_ERR_INVALID_ARGUMENT, "extents must be 2..20 in each dimension");
+  if (num_elements < 0 || nc < 1)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "bad element or component count");
+  if (flux_divergence_lds_bytes(extents) > 160 * 1024)
+    return fail(SXD_ERR_UNSUPPORTED, "3 Np doubles of fluxes do not fit the LDS");
+  return SXD_OK;
+}
+}  // namespace
+
+int sxd_divergence(int num_elements, const int32_t extents[3], int nc, const double* fluxes,
+                   const double* inv_jac, double* div, void* stream) {
+  if (int rc = check_divergence_args(num_elements, extents, nc); rc != SXD_OK) return rc;
+  if (num_elements == 0) return ok();
+  SXD_REQUIRE_DEVICE(fluxes);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  SXD_REQUIRE_DEVICE(div);
+  return finish(launch_flux_divergence(num_elements, extents, nc, false,
+                                       const_cast<double*>(fluxes), inv_jac, div, false,
+                                       nullptr, nullptr, nullptr, nullptr,
+                                       static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+int sxd_weak_divergence(int num_elements, const int32_t extents[3], int nc,
+                        const double* fluxes, const double* det_jac_times_inv_jac, double* div,
+                        void* stream) {
+  if (int rc = check_divergence_args(num_elements, extents, nc); rc != SXD_OK) return rc;
+  if (num_elements == 0) return ok();
+  SXD_REQUIRE_DEVICE(fluxes);
+  SXD_REQUIRE_DEVICE(det_jac_times_inv_jac);
+  SXD_REQUIRE_DEVICE(div);
+  return finish(launch_flux_divergence(num_elements, extents, nc, true,
+                                       const_cast<double*>(fluxes), det_jac_times_inv_jac, div,
+                                       false, nullptr, nullptr, nullptr, nullptr,
+                                       static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+int sxd_flux_volume_terms(int num_elements, const int32_t extents[3], int nc, const double* u,
+                          double* fluxes, const double* mesh_velocity,
+                          const double* div_mesh_velocity, int formulation,
+                          const double* inv_jac, const double* det_jac_times_inv_jac,
+                          const double* det_inv_jac, double* dt, void* stream) {
+  if (int rc = check_divergence_args(num_elements, extents, nc); rc != SXD_OK) return rc;
+  if (formulation != SXD_DG_STRONG_INERTIAL && formulation != SXD_DG_WEAK_INERTIAL)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "unknown DG formulation %d", formulation);
+  if (num_elements == 0) return ok();
+  SXD_REQUIRE_DEVICE(fluxes);
+  SXD_REQUIRE_DEVICE(dt);
+  const bool weak = formulation == SXD_DG_WEAK_INERTIAL;
+  if (weak) {
+    SXD_REQUIRE_DEVICE(det_jac_times_inv_jac);
+    SXD_REQUIRE_DEVICE(det_inv_jac);
+  } else {
+    SXD_REQUIRE_DEVICE(inv_jac);
+  }
+  if (mesh_velocity != nullptr) {
+    SXD_REQUIRE_DEVICE(mesh_velocity);
+    SXD_REQUIRE_DEVICE(div_mesh_velocity);
+    SXD_REQUIRE_DEVICE(u);
+  }
+  return finish(launch_flux_divergence(num_elements, extents, nc, weak, fluxes,
+                                       weak ? det_jac_times_inv_jac : inv_jac, dt, true, u,
+                                       mesh_velocity, div_mesh_velocity, det_inv_jac,
+                                       static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+int sxd_mesh_velocity_nonconservative(int64_t num_points, int nc, const double* du,
+                                      const double* mesh_velocity, double* dt, void* stream) {
+  if (num_points < 0 || nc < 1)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "bad point or component count");
+  if (num_points == 0) return ok();
+  SXD_REQUIRE_DEVICE(du);
+  SXD_REQUIRE_DEVICE(mesh_velocity);
+  SXD_REQUIRE_DEVICE(dt);
+  return finish(launch_mesh_velocity_nonconservative(num_points, nc, du, mesh_velocity, dt,
+                                                     static_cast<hipStream_t>(stream)),
                 __func__);
 }
 

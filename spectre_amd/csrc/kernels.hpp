@@ -124,6 +124,23 @@ int launch_rk3_substep_dt(int substep, int64_t n, const double* dt, const double
                           const double* uk, const double* lk, double* out, hipStream_t stream);
 int launch_rk3_substep(int substep, int64_t n, double dt, const double* u0, const double* uk,
                        const double* lk, double* out, hipStream_t stream);
+// Flux systems (divergence.hip).  Strong (weak = false; jac = inverse
+// Jacobian) or weak (jac = det J times the inverse Jacobian) divergence of
+// fluxes [E][3 nc][Np] (block i + 3 c = F^i_c).  terms = false: out = div.
+// terms = true: the volume terms of flux variables, on dt = out: with a mesh
+// velocity v [E][3][Np] (may be null; div_v [E][Np] then required) the
+// fluxes become F^i_c - u_c v^i (written back) and dt_c -= u_c div v, then
+// dt -= div (strong) or dt += div det_inv_jac [E][Np] (weak).
+// SXD_ERR_UNSUPPORTED if 3 Np doubles and the matrices exceed the LDS.
+int launch_flux_divergence(int num_elements, const int32_t ext[3], int nc, bool weak,
+                           double* fluxes, const double* jac, double* out, bool terms,
+                           const double* u, const double* v, const double* div_v,
+                           const double* det_inv_jac, hipStream_t stream);
+int flux_divergence_lds_bytes(const int32_t ext[3]);
+// dt_c += v^i d_i u_c on n points (du [3 nc][n], v [3][n], dt [nc][n]).
+int launch_mesh_velocity_nonconservative(int64_t n, int nc, const double* du, const double* v,
+                                         double* dt, hipStream_t stream);
+const double* device_weak_flux_matrices();
 // The allocation-free entry point's fixed work array (noalloc.hip): its
 // capacity in doubles, and its device address on the current device.
 size_t noalloc_scratch_doubles();

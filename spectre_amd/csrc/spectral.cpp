@@ -204,4 +204,23 @@ const std::vector<double>& packed_differentiation_matrices() {
   return table;
 }
 
+void lgl_weak_flux_differentiation_matrix(int n, double* out) {
+  double d[kMaxPoints * kMaxPoints], x[kMaxPoints], w[kMaxPoints];
+  lgl_differentiation_matrix(n, d);
+  lgl_points_and_weights(n, x, w);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) out[i * n + j] = d[j * n + i] * (w[j] / w[i]);
+}
+
+const std::vector<double>& packed_weak_flux_matrices() {
+  static std::vector<double> table;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    table.assign(kPackedDSize, 0.0);
+    for (int n = kMinPoints; n <= kMaxPoints; ++n)
+      lgl_weak_flux_differentiation_matrix(n, table.data() + packed_d_offset(n));
+  });
+  return table;
+}
+
 }  // namespace sxd

@@ -41,5 +41,10 @@ void mortar_interpolation_matrix(int nf, int nm, double* out);
 // Modal-truncation projection from nm LGL points onto nf (row-major [nf][nm]).
 void mortar_projection_matrix(int nf, int nm, double* out);
 const std::vector<double>& packed_mortar_matrices();
+// Weak flux differentiation matrix W_ij = D_ji (w_j / w_i), row-major
+// (Spectral.cpp:462-485), and all of them for n = 2..20 packed as the D
+// matrices (at packed_d_offset(n)).
+void lgl_weak_flux_differentiation_matrix(int n, double* out);
+const std::vector<double>& packed_weak_flux_matrices();
 
 }  // namespace sxd

@@ -107,6 +107,82 @@ def logical_partial_derivatives(u, extents, out=None):
     return out
 
 
+def _fluxes_shape(fluxes, extents):
+    if fluxes.dim() != 3 or fluxes.shape[1] % 3 != 0 or fluxes.shape[2] != _np(extents):
+        raise ValueError("fluxes must be [E, 3 nc, Np] for these extents")
+    return fluxes.shape[0], fluxes.shape[1] // 3, fluxes.shape[2]
+
+
+def divergence(fluxes, extents, inv_jac, out=None):
+    """Strong divergence: fluxes [E, 3 nc, Np] (block i + 3 c = F^i_c),
+    inv_jac [E, 9, Np] -> div [E, nc, Np]
+    (LinearOperators/Divergence.tpp:50-94)."""
+    e, nc, npts = _fluxes_shape(fluxes, extents)
+    _require(fluxes, "fluxes")
+    _require(inv_jac, "inv_jac", (e, 9, npts))
+    if out is None:
+        out = torch.empty((e, nc, npts), dtype=torch.float64, device=fluxes.device)
+    _require(out, "out", (e, nc, npts))
+    ext, ep = _extents(extents)
+    check(lib().sxd_divergence(e, ep, nc, _ptr(fluxes), _ptr(inv_jac), _ptr(out), _stream()))
+    return out
+
+
+def weak_divergence(fluxes, extents, det_jac_times_inv_jac, out=None):
+    """Weak divergence (LinearOperators/WeakDivergence.hpp:40-231): fluxes
+    [E, 3 nc, Np], det_jac_times_inv_jac [E, 9, Np] -> [E, nc, Np]."""
+    e, nc, npts = _fluxes_shape(fluxes, extents)
+    _require(fluxes, "fluxes")
+    _require(det_jac_times_inv_jac, "det_jac_times_inv_jac", (e, 9, npts))
+    if out is None:
+        out = torch.empty((e, nc, npts), dtype=torch.float64, device=fluxes.device)
+    _require(out, "out", (e, nc, npts))
+    ext, ep = _extents(extents)
+    check(lib().sxd_weak_divergence(e, ep, nc, _ptr(fluxes), _ptr(det_jac_times_inv_jac),
+                                    _ptr(out), _stream()))
+    return out
+
+
+def flux_volume_terms(u, fluxes, extents, dt, formulation="strong", inv_jac=None,
+                      det_jac_times_inv_jac=None, det_inv_jac=None, mesh_velocity=None,
+                      div_mesh_velocity=None):
+    """Volume terms of flux variables on dt [E, nc, Np] in place
+    (Evolution/DiscontinuousGalerkin/Actions/VolumeTermsImpl.tpp:156-200,
+    264-291): moving-mesh corrections of the fluxes (in place) and of dt,
+    then dt -= div F (strong) or dt += det_inv_jac weak_div F (weak)."""
+    e, nc, npts = _fluxes_shape(fluxes, extents)
+    _require(fluxes, "fluxes")
+    _require(dt, "dt", (e, nc, npts))
+    weak = {"strong": False, "weak": True}[formulation]
+    if weak:
+        _require(det_jac_times_inv_jac, "det_jac_times_inv_jac", (e, 9, npts))
+        _require(det_inv_jac, "det_inv_jac", (e, npts))
+    else:
+        _require(inv_jac, "inv_jac", (e, 9, npts))
+    if mesh_velocity is not None:
+        _require(mesh_velocity, "mesh_velocity", (e, 3, npts))
+        _require(div_mesh_velocity, "div_mesh_velocity", (e, npts))
+        _require(u, "u", (e, nc, npts))
+    ext, ep = _extents(extents)
+    check(lib().sxd_flux_volume_terms(
+        e, ep, nc, _ptr(u), _ptr(fluxes), _ptr(mesh_velocity), _ptr(div_mesh_velocity),
+        1 if weak else 0, _ptr(inv_jac), _ptr(det_jac_times_inv_jac), _ptr(det_inv_jac),
+        _ptr(dt), _stream()))
+    return dt
+
+
+def mesh_velocity_nonconservative(du, mesh_velocity, dt):
+    """dt_c += v^i d_i u_c in place: du [3 nc, n], mesh_velocity [3, n],
+    dt [nc, n] (VolumeTermsImpl.tpp:235-262)."""
+    nc, n = dt.shape
+    _require(du, "du", (3 * nc, n))
+    _require(mesh_velocity, "mesh_velocity", (3, n))
+    _require(dt, "dt", (nc, n))
+    check(lib().sxd_mesh_velocity_nonconservative(n, nc, _ptr(du), _ptr(mesh_velocity),
+                                                  _ptr(dt), _stream()))
+    return dt
+
+
 def scalar_wave_time_derivative(u, du, gamma2):
     """u [5, n], du [15, n], gamma2 [n] -> dt [5, n] (TimeDerivative.cpp:14-45)."""
     n = u.shape[-1]

@@ -91,6 +91,60 @@ def logical_partial_derivatives(u, extents):
     return out
 
 
+def weak_flux_differentiation_matrix(n):
+    w = np.empty((n, n))
+    _check(lib().sxo_weak_flux_differentiation_matrix(n, _d(w)))
+    return w
+
+
+def divergence(fluxes, extents, inv_jac):
+    """fluxes [3 nc, Np] (block i + 3 c), inv_jac [9, Np] -> [nc, Np] (one element)."""
+    fluxes = np.ascontiguousarray(fluxes, dtype=np.float64)
+    inv_jac = np.ascontiguousarray(inv_jac, dtype=np.float64)
+    nc = fluxes.shape[0] // 3
+    out = np.empty((nc, fluxes.shape[1]))
+    ext, ep = _i(extents)
+    _check(lib().sxo_divergence(_d(fluxes), nc, ep, _d(inv_jac), _d(out)))
+    return out
+
+
+def weak_divergence(fluxes, extents, djij):
+    fluxes = np.ascontiguousarray(fluxes, dtype=np.float64)
+    djij = np.ascontiguousarray(djij, dtype=np.float64)
+    nc = fluxes.shape[0] // 3
+    out = np.empty((nc, fluxes.shape[1]))
+    ext, ep = _i(extents)
+    _check(lib().sxo_weak_divergence(_d(fluxes), nc, ep, _d(djij), _d(out)))
+    return out
+
+
+def flux_volume_terms(u, fluxes, extents, dt, weak, inv_jac=None, djij=None, det_inv_jac=None,
+                      v=None, div_v=None):
+    """One element; returns (fluxes, dt) after the flux-variable volume terms
+    (inputs are copied, not modified)."""
+    f = np.array(fluxes, dtype=np.float64, order="C")
+    d = np.array(dt, dtype=np.float64, order="C")
+    nc = f.shape[0] // 3
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    keep = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None
+            for a in (v, div_v, inv_jac, djij, det_inv_jac)]
+    ptr = lambda a: None if a is None else _d(a)
+    ext, ep = _i(extents)
+    _check(lib().sxo_flux_volume_terms(nc, ep, _d(u), _d(f), ptr(keep[0]), ptr(keep[1]),
+                                       1 if weak else 0, ptr(keep[2]), ptr(keep[3]),
+                                       ptr(keep[4]), _d(d)))
+    return f, d
+
+
+def mesh_velocity_nonconservative(du, v, dt):
+    du = np.ascontiguousarray(du, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    d = np.array(dt, dtype=np.float64, order="C")
+    nc, n = d.shape
+    _check(lib().sxo_mesh_velocity_nonconservative(ctypes.c_long(n), nc, _d(du), _d(v), _d(d)))
+    return d
+
+
 def partial_derivatives_batch(u, extents, inv_jac):
     """u [E, nc, Np], inv_jac [E, 9, Np] -> du [E, 3 nc, Np]."""
     u = np.ascontiguousarray(u, dtype=np.float64)
