@@ -27,11 +27,12 @@ LLC_BYTES = 256 << 20  # MI355X Infinity Cache
 FP64_PEAK_TFLOPS = 78.6
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# (each scaled by what the counter reads for a plain copy of known size:
-# FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), for the workload's default kernel;
-# the summaries are profiles/gh_rhs_fused_kernel_gh_kerr_schild_512el_8cubed.md
-# and profiles/sw_rhs_fused_kernel_sw_plane_wave_64el_5cubed.md.
-MEASURED_TRAFFIC = {("gh", "fused"): 403619168.0, ("scalar_wave", "fused"): 2243413.0}
+# (each scaled by what the counter reads for probe kernels of known access
+# pattern: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), for the workload's default
+# kernel; the summaries are profiles/gh_rhs_fused_kernel_gh_kerr_schild_512el_8cubed.md,
+# profiles/hbm_probe_counter_calibration.md and
+# profiles/sw_rhs_fused_kernel_sw_plane_wave_64el_5cubed.md.
+MEASURED_TRAFFIC = {("gh", "fused"): 450124288.0, ("scalar_wave", "fused"): 2243413.0}
 
 
 def algorithmic_bytes_per_point(nc, n_aux, n):
@@ -77,7 +78,8 @@ def time_calls(fn, steps, warmup):
 
 def hbm_probe(mib, reps=5):
     """Probe kernels of known access pattern (spectre_amd/csrc/hbm_probe.hip):
-    a streaming read, a streaming write and a copy of `mib` MiB each, `reps`
+    a streaming read, a streaming write and a copy of `mib` MiB each, a read
+    with 8-byte loads and a gather of one double per 64-byte line, `reps`
     launches each on the current stream, timed with HIP events.  Profiled with
     rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE), the counters' KiB per
     dispatch against the known bytes give the factor that converts the
@@ -95,7 +97,9 @@ def hbm_probe(mib, reps=5):
     stream = torch.cuda.current_stream()
     out = {"probe_doubles": n}
     for mode, name, read, written in ((0, "read", n * 8, partial.numel() * 8),
-                                      (1, "write", 0, n * 8), (2, "copy", n * 8, n * 8)):
+                                      (1, "write", 0, n * 8), (2, "copy", n * 8, n * 8),
+                                      (3, "read8", n * 8, partial.numel() * 8),
+                                      (4, "gather64", n, partial.numel() * 8)):
         call = lambda: probe.sxd_probe_run(mode, ctypes.c_void_p(src.data_ptr()),
                                            ctypes.c_void_p(dst.data_ptr()),
                                            ctypes.c_void_p(partial.data_ptr()), n,

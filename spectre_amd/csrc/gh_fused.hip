@@ -27,8 +27,9 @@
 //     condition, GeneralizedHarmonic/BoundaryConditions/DirichletAnalytic.cpp:
 //     70-115, Actions/BoundaryConditionsImpl.hpp:427-586).
 //  B  for each symmetric pair (a,b) (10 passes): stage g_ab, Pi_ab, Phi_{i,ab}
-//     of the element in LDS (the next pass's values are prefetched into
-//     registers while the current pass computes); face slots form the lifted
+//     of the element in LDS (the next pass's values are prefetched while the
+//     current pass computes: at 8^3 by direct global-to-LDS copies into the
+//     idle stage buffer, otherwise into registers); face slots form the lifted
 //     UpwindPenalty correction; every point forms the 15 derivatives of the
 //     pair, contracts them with the inverse Jacobian, forms the five dt
 //     components of the pair, adds the corrections of the faces it lies on
@@ -95,13 +96,19 @@ struct GhFusedLayout {
   static constexpr int kExtFields = 11;  // n[3] nv[3] lam[4] gamma2
   static constexpr int kFaceInt = 0;
   static constexpr int kFaceExt = kFaceInt + SLOTS * kIntFields;
+  // Staging by direct global-to-LDS copies (global_load_lds_dwordx4) for the
+  // vector-ALU engine at 8^3: the next pass's fields go straight into the
+  // idle stage buffer, with no registers in between.  The stage then also
+  // holds the pass's parked dt Pi partial sum (component 5).
+  static constexpr bool DMA = !MFMA && N == 8;
+  static constexpr int STAGED = DMA ? 6 : 5;
   static constexpr int kStage = (kFaceExt + SLOTS * kExtFields + 1) / 2 * 2;  // 16-byte aligned
-  static constexpr int kCorr = kStage + (5 * NP + 1) / 2 * 2;
+  static constexpr int kCorr = kStage + (STAGED * NP + 1) / 2 * 2;
   static constexpr int kDeriv = kCorr + 5 * SLOTS;  // MFMA: logical derivatives
   // vector ALUs: second stage and correction buffers (odd passes), so a pass
   // can stage its fields while slower waves still read the previous pass's
   static constexpr int kStage2 = (kDeriv + (MFMA ? 9 * NP : 0) + 1) / 2 * 2;
-  static constexpr int kCorr2 = kStage2 + (MFMA ? 0 : (5 * NP + 1) / 2 * 2);
+  static constexpr int kCorr2 = kStage2 + (MFMA ? 0 : (STAGED * NP + 1) / 2 * 2);
   static constexpr int kDoubles = kCorr2 + (MFMA ? 0 : 5 * SLOTS);
   // Staged rows (vector ALUs, N = 8) are swizzled: within row (j, k) the
   // 16-byte pair s of xi values is stored at pair s ^ ((j >> 1) & 3).  The
@@ -250,6 +257,11 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
       const double* ga = gammas + e * 3 * NP + pt;
       gamma1[q] = ga[NP];
       gh_point_setup(st[q], up, ga[0], gamma1[q], ga[2 * NP]);
+      // dt Pi_ab through the Christoffel terms: parked in dt (row Pi_ab of
+      // this point, which only this thread touches) until pass ab reads it
+      // back, so that it holds no registers through the passes
+#pragma unroll
+      for (int r = 0; r < 10; ++r) dt[(e * 50 + 10 + r) * NP + pt] = st[q].dt_pi_partial[r];
 #pragma unroll
       for (int c = 0; c < 9; ++c) ij[q][c] = inv_jac[(e * 9 + c) * NP + pt];
     }
@@ -388,13 +400,19 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
       }
     } else {
 #pragma unroll
-      for (int q = 0; q < P; ++q)
+      for (int q = 0; q < P; ++q) {
+        // with N^2 = 64 every wave covers whole k-planes: its k is uniform,
+        // so its row of D is read by scalar loads and held in no VGPR
+        int kq = pk[q];
+        if constexpr (N * N == 64 && T % 64 == 0)
+          kq = __builtin_amdgcn_readfirstlane(t / 64) + q * (T / 64);
 #pragma unroll
         for (int l = 0; l < N; ++l) {
           Di[q][l] = D[pi_[q] * N + l];
           Dj[q][l] = D[pj[q] * N + l];
-          Dk[q][l] = D[pk[q] * N + l];
+          Dk[q][l] = D[kq * N + l];
         }
+      }
     }
   }
 
@@ -404,13 +422,17 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
   double* deriv = smem + L::kDeriv;
   double* dte = dt + e * 50 * NP;
   // prefetched fields of the next pass: own points and neighbour face points
-  double own_next[P][5], ext_next[PF][5];
+  double own_next[P][5], ext_next[PF][5], pin_next[P];
   auto load_pass = [&](int p) {
     const int comp[5] = {p, 10 + p, 20 + 3 * p, 21 + 3 * p, 22 + 3 * p};
+    if constexpr (!L::DMA) {
 #pragma unroll
-    for (int q = 0; q < P; ++q)
+      for (int q = 0; q < P; ++q) {
 #pragma unroll
-      for (int c = 0; c < 5; ++c) own_next[q][c] = act[q] ? ue[comp[c] * NP + t + q * T] : 0.0;
+        for (int c = 0; c < 5; ++c) own_next[q][c] = act[q] ? ue[comp[c] * NP + t + q * T] : 0.0;
+        pin_next[q] = act[q] ? dte[(10 + p) * NP + t + q * T] : 0.0;
+      }
+    }
 #pragma unroll
     for (int f = 0; f < PF; ++f)
 #pragma unroll
@@ -422,6 +444,29 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
           ext_next[f][c] = fact[f] ? u[nb_base[f] + comp[c] * NP] : 0.0;
         }
       }
+  };
+  // Direct copy of pass p's six staged components into the stage buffer at
+  // `buf`: 24 jobs of 64 lanes x 16 bytes (component c = job / 4, quarter
+  // job % 4), spread over the waves.  Lane L of a job fills 16-byte unit U = 64 (job %
+  // 4) + L of the component, i.e. pair U & 3 of staged row U >> 2; it reads
+  // the pair the swizzle stores there (pair (U & 3) ^ ((j >> 1) & 3) of the
+  // row, j = row % 8).
+  auto dma_stage = [&](int p, double* buf) {
+    if constexpr (L::DMA) {
+#pragma unroll
+      for (int m = 0; m < 24 / L::WAVES; ++m) {
+        const int job = wave + L::WAVES * m;  // wave-uniform
+        const int c = job >> 2, quarter = job & 3;
+        const int comp = c == 0 ? p : c == 1 ? 10 + p : c == 5 ? 10 + p : 20 + 3 * p + (c - 2);
+        const double* src = (c == 5 ? dte : ue) + comp * NP;
+        const int unit = 64 * quarter + lane;
+        const int row = unit >> 2;
+        const int src_unit = (row << 2) | ((unit & 3) ^ (((row & 7) >> 1) & 3));
+        __builtin_amdgcn_global_load_lds(
+            (const __attribute__((address_space(1))) void*)(src + 2 * src_unit),
+            (__attribute__((address_space(3))) void*)(buf + c * NP + 128 * quarter), 16, 0, 0);
+      }
+    }
   };
   // inertial derivatives of an owned point from its three logical ones
   auto contract = [&](int q, double d0, double d1, double d2, double* out3) {
@@ -509,14 +554,16 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
     for (int p = 0; p < 10; ++p) {
       lds_barrier();  // previous pass done with stage, corr and deriv
       // the pair's own values stay in registers for the point-wise terms
-      double own[P][5], ext_v[PF][5];
+      double own[P][5], ext_v[PF][5], pin[P];
 #pragma unroll
-      for (int q = 0; q < P; ++q)
+      for (int q = 0; q < P; ++q) {
+        pin[q] = pin_next[q];
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
           own[q][c] = own_next[q][c];
           if (act[q]) stage[c * NP + t + q * T] = own[q][c];
         }
+      }
 #pragma unroll
       for (int f = 0; f < PF; ++f)
 #pragma unroll
@@ -549,11 +596,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
 #pragma unroll
           for (int m = 0; m < 3; ++m) dphi[m + 3 * n] = d3[m];
         }
-        gh_point_pair(out[q], st[q], st[q].dt_pi_partial[0], own[q][1], own[q] + 2, dg[q],
-                      dpi[q], dphi);
-        // rotate so that pass p + 1 finds its partial sum at index 0
-#pragma unroll
-        for (int r = 0; r < 9; ++r) st[q].dt_pi_partial[r] = st[q].dt_pi_partial[r + 1];
+        gh_point_pair(out[q], st[q], pin[q], own[q][1], own[q] + 2, dg[q], dpi[q], dphi);
       }
 #pragma unroll
       for (int q = 0; q < P; ++q) {
@@ -593,6 +636,12 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
         for (int c = 0; c < 5; ++c) dte[comp[c] * NP + t + q * T] = prev[q][c];
       }
     };
+    if constexpr (L::DMA) {
+      // the parked dt Pi partial sums of every point are read by other waves'
+      // copies: complete and publish this thread's stores first
+      __syncthreads();
+      dma_stage(0, smem + L::kStage);
+    }
     load_pass(0);
 #pragma unroll 1
     for (int p = 0; p < 10; ++p) {
@@ -601,22 +650,40 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
       // (last read in pass p - 2) is free.  Pass p's corrections go to
       // correction buffer p & 1; pass p - 1's are read after this pass's
       // staging barrier (which ends their writes), before pass p + 1's
-      // barrier lets anyone write that buffer again.
+      // barrier lets anyone write that buffer again.  With direct copies,
+      // pass p's copy was issued during pass p - 1 (after that pass's
+      // barrier, which freed buffer p & 1); each wave waits for its own
+      // copies before the barrier that publishes them.
       stage = smem + ((p & 1) ? L::kStage2 : L::kStage);
       corr = smem + ((p & 1) ? L::kCorr2 : L::kCorr);
+      if constexpr (!L::DMA) {
 #pragma unroll
-      for (int q = 0; q < P; ++q)
-        if (act[q]) {
+        for (int q = 0; q < P; ++q)
+          if (act[q]) {
 #pragma unroll
-          for (int c = 0; c < 5; ++c) stage[c * NP + spt[q]] = own_next[q][c];
-        }
-      double ext_v[PF][5];
+            for (int c = 0; c < 5; ++c) stage[c * NP + spt[q]] = own_next[q][c];
+          }
+      }
+      double ext_v[PF][5], pin[P];
 #pragma unroll
       for (int f = 0; f < PF; ++f)
 #pragma unroll
         for (int c = 0; c < 5; ++c) ext_v[f][c] = ext_next[f][c];
-      if (p + 1 < 10) load_pass(p + 1);
-      lds_barrier();
+      if constexpr (L::DMA) {
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's copies landed
+        lds_barrier();
+        if (p + 1 < 10) {
+          dma_stage(p + 1, smem + (((p + 1) & 1) ? L::kStage2 : L::kStage));
+          load_pass(p + 1);
+        }
+#pragma unroll
+        for (int q = 0; q < P; ++q) pin[q] = stage[5 * NP + spt[q]];
+      } else {
+#pragma unroll
+        for (int q = 0; q < P; ++q) pin[q] = pin_next[q];
+        if (p + 1 < 10) load_pass(p + 1);
+        lds_barrier();
+      }
       if (p > 0) finish_pass(p - 1);
       // lifted boundary corrections of the face slots
 #pragma unroll
@@ -732,11 +799,7 @@ gh_rhs_fused_kernel(const int32_t* __restrict__ nbr, const double* __restrict__ 
 #pragma unroll
           for (int m = 0; m < 3; ++m) dphi[m + 3 * n] = dlog[q][3 * (2 + n) + m];
         }
-        gh_point_pair(out[q], st[q], st[q].dt_pi_partial[0], stage[NP + pt], phi, dlog[q],
-                      dlog[q] + 3, dphi);
-        // rotate so that pass p + 1 finds its partial sum at index 0
-#pragma unroll
-        for (int r = 0; r < 9; ++r) st[q].dt_pi_partial[r] = st[q].dt_pi_partial[r + 1];
+        gh_point_pair(out[q], st[q], pin[q], stage[NP + pt], phi, dlog[q], dlog[q] + 3, dphi);
       }
       // this pass's corrections are added after the next pass's staging
       // barrier, which also ends their writes
