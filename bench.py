@@ -212,6 +212,34 @@ def measure_mixed(system, steps, warmup, rk3_steps=5):
     }
 
 
+def measure_gauss(steps=50, warmup=5):
+    """ScalarWave RHS on Legendre-Gauss points (512 elements of 8^3, plane
+    wave on [-1,1]^3, composite kernels: Gauss derivatives, point-wise terms,
+    three face launches lifting onto whole pencils) against the LGL composite
+    and fused RHS of the same brick size."""
+    import numpy as np
+    import torch
+    from spectre_amd import dg, workloads as wl
+
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    brick, u, g2, inv, det_inv, face_inv = wl.scalar_wave_plane_wave_gauss(
+        seed=0, elements=(8, 8, 8), extents=(8, 8, 8))
+    d_u, d_g, d_inv, d_det, d_face = dev(u), dev(g2), dev(inv), dev(det_inv), dev(face_inv)
+    d_dt = torch.empty_like(d_u)
+    domain = dg.Domain.from_brick(brick)
+    ms = {"gauss_composite": time_calls(
+        lambda: domain.scalar_wave_rhs_gauss(d_u, d_g, d_inv, d_det, d_face, out=d_dt),
+        steps, warmup)}
+    for kernel in ("composite", "fused"):
+        domain.set_kernel("scalar_wave", kernel)
+        ms["lgl_" + kernel] = time_calls(lambda: domain.scalar_wave_rhs(d_u, d_g, d_inv, out=d_dt),
+                                         steps, warmup)
+    points = brick.num_elements * brick.num_points
+    return {"workload": "scalar_wave_plane_wave_gauss_512el_8^3", "grid_points": points,
+            "ms": {k: round(v, 6) for k, v in ms.items()},
+            "gauss_grid_points_per_s": round(points / (ms["gauss_composite"] * 1e-3), 1)}
+
+
 def measure_boundaries(steps=50, warmup=5):
     """Cost of the external boundary conditions: the RHS of a non-periodic
     brick whose outer faces are closed by a condition, against the periodic
@@ -419,7 +447,7 @@ def main():
                     help="also measure the mixed-p batched form (1) or not (0)")
     ap.add_argument("--only-extension", default="",
                     choices=["", "sw_graph", "mixed_sw", "mixed_gh", "adjoint", "gh_adjoint",
-                             "gh_adjoint_mixed", "boundaries"],
+                             "gh_adjoint_mixed", "boundaries", "gauss"],
                     help="run only this extension measurement (for profiling) and print "
                          "its JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -450,7 +478,8 @@ def main():
                 "adjoint": measure_adjoint,
                 "gh_adjoint": measure_gh_adjoint,
                 "gh_adjoint_mixed": lambda: measure_gh_adjoint(mixed=True),
-                "boundaries": measure_boundaries}
+                "boundaries": measure_boundaries,
+                "gauss": measure_gauss}
         print(json.dumps(only[args.only_extension]()))
         return
 
@@ -570,7 +599,8 @@ def main():
                       "adjoint": measure_adjoint(),
                       "gh_adjoint": measure_gh_adjoint(),
                       "gh_adjoint_mixed": measure_gh_adjoint(mixed=True),
-                      "boundaries": measure_boundaries()}
+                      "boundaries": measure_boundaries(),
+                      "gauss": measure_gauss()}
 
     result = {
         "metric": METRIC,

@@ -61,6 +61,23 @@ int sxd_lgl_points_and_weights(int n, double* x, double* w);
  * (NumericalAlgorithms/Spectral/Spectral.cpp:113-150, 859-878). */
 int sxd_lgl_differentiation_matrix(int n, double* d);
 
+/* Legendre-Gauss collocation points and weights for n points (2..20).
+ * Replaces Spectral::collocation_points / quadrature_weights for
+ * Basis::Legendre, Quadrature::Gauss
+ * (NumericalAlgorithms/Spectral/Legendre.cpp:106-148). */
+int sxd_lg_points_and_weights(int n, double* x, double* w);
+
+/* Differentiation matrix on the Legendre-Gauss points, row-major.
+ * Replaces Spectral::differentiation_matrix for Quadrature::Gauss
+ * (NumericalAlgorithms/Spectral/Spectral.cpp:113-150). */
+int sxd_lg_differentiation_matrix(int n, double* d);
+
+/* Boundary terms of the Gauss path, out [4][n]: the interpolation rows to
+ * xi = -1 and xi = +1 (Spectral::boundary_interpolation_matrices,
+ * Spectral.cpp:677-700) and the lifting terms l_j(-1) / w_j, l_j(+1) / w_j
+ * (Spectral::boundary_lifting_term, Spectral.cpp:738-768). */
+int sxd_lg_boundary_terms(int n, double* out);
+
 /* ---- Linear operators ------------------------------------------------- */
 
 /* Logical partial derivatives: dlogical = [3][E][nc][Np] (d/dxi, d/deta,
@@ -300,6 +317,22 @@ int sxd_scalar_wave_rhs_constraint_preserving_spherical_radiation(
     const sxd_domain* domain, const double* u, const double* gamma2,
     const double* inv_jac, const double* coords, int type, double* dt,
     void* stream);
+
+/* Full ScalarWave DG time derivative on Legendre-Gauss points (the
+ * Quadrature::Gauss branches of ComputeTimeDerivative and
+ * ApplyBoundaryCorrections: face values interpolated along the normal pencil,
+ * ProjectToBoundary.hpp:54-147; boundary corrections lifted onto whole
+ * pencils, ApplyBoundaryCorrections.hpp:939-986 and
+ * NumericalAlgorithms/DiscontinuousGalerkin/LiftFromBoundary.cpp:140-180).
+ * Uniform periodic domains.  u, dt [E][5][Np]; gamma2, det_inv_jac [E][Np]
+ * (det of the inverse Jacobian at the volume points); inv_jac [E][9][Np];
+ * face_inv_jac: per element the six faces in direction order (lower xi,
+ * upper xi, lower eta, ...), each 3 blocks of its Nf points (face-mesh order,
+ * remaining dimensions lowest first) holding d xi^d / d x^i of the face's
+ * dimension d, i = 0..2, as the map gives them on the face. */
+int sxd_scalar_wave_rhs_gauss(const sxd_domain* domain, const double* u, const double* gamma2,
+                              const double* inv_jac, const double* det_inv_jac,
+                              const double* face_inv_jac, double* dt, void* stream);
 
 /* Full generalized-harmonic DG time derivative (harmonic gauge).
  * u, dt [E][50][Np]; gammas [E][3][Np]; inv_jac [E][9][Np]. */

@@ -10,6 +10,7 @@
 #include "gh_reference_impl.hpp"
 #include "sxo.hpp"
 #include "sxo_flux.hpp"
+#include "sxo_gauss.hpp"
 
 namespace {
 bool bad_extents(const int* ext) {
@@ -47,6 +48,40 @@ int sxo_differentiation_matrix(int n, double* d) {
   if (n < 2 || n > 20) return -1;
   const auto& m = sxo::SpectralCache::get().D[n];
   std::memcpy(d, m.data(), m.size() * sizeof(double));
+  return 0;
+}
+
+int sxo_lg_points_and_weights(int n, double* x, double* w) {
+  if (n < 2 || n > 20) return -1;
+  std::vector<double> xv, wv;
+  sxo::lg_points_and_weights(n, &xv, &wv);
+  std::memcpy(x, xv.data(), n * sizeof(double));
+  std::memcpy(w, wv.data(), n * sizeof(double));
+  return 0;
+}
+
+int sxo_lg_differentiation_matrix(int n, double* d) {
+  if (n < 2 || n > 20) return -1;
+  const std::vector<double> m = sxo::lg_differentiation_matrix(n);
+  std::memcpy(d, m.data(), m.size() * sizeof(double));
+  return 0;
+}
+
+// out [4][n]: interpolation rows to -1 and +1, lifting terms at -1 and +1.
+int sxo_lg_boundary_terms(int n, double* out) {
+  if (n < 2 || n > 20) return -1;
+  const std::vector<double> m = sxo::lg_boundary_terms(n);
+  std::memcpy(out, m.data(), m.size() * sizeof(double));
+  return 0;
+}
+
+int sxo_sw_rhs_gauss(int num_elements, const int* ext, const int* nbr, const double* u,
+                     const double* gamma2, const double* inv_j, const double* det_inv_j,
+                     const double* face_inv_j, double* dt) {
+  if (bad_extents(ext) || num_elements < 1) return -1;
+  for (int i = 0; i < 6 * num_elements; ++i)
+    if (nbr[i] < 0 || nbr[i] >= num_elements) return -1;
+  sxo::sw_rhs_gauss(num_elements, ext, nbr, u, gamma2, inv_j, det_inv_j, face_inv_j, dt);
   return 0;
 }
 

@@ -30,6 +30,10 @@ _SIGNATURES = {
     "sxd_version": (ctypes.c_char_p, []),
     "sxd_lgl_points_and_weights": (ctypes.c_int, [ctypes.c_int, _dp, _dp]),
     "sxd_lgl_differentiation_matrix": (ctypes.c_int, [ctypes.c_int, _dp]),
+    "sxd_lg_points_and_weights": (ctypes.c_int, [ctypes.c_int, _dp, _dp]),
+    "sxd_lg_differentiation_matrix": (ctypes.c_int, [ctypes.c_int, _dp]),
+    "sxd_lg_boundary_terms": (ctypes.c_int, [ctypes.c_int, _dp]),
+    "sxd_scalar_wave_rhs_gauss": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_logical_partial_derivatives": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _vp, _vp, _vp]),
     "sxd_partial_derivatives": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _vp, _vp, _vp, _vp]),
     "sxd_partial_derivatives_adjoint": (ctypes.c_int, [ctypes.c_int, _ip, ctypes.c_int, _vp, _vp,

@@ -203,6 +203,11 @@ const double* device_weak_flux_matrices() {
   return device_table(t, packed_weak_flux_matrices());
 }
 
+const double* device_gauss_tables() {
+  static DeviceTables t;
+  return device_table(t, packed_gauss_tables());
+}
+
 }  // namespace sxd
 
 using namespace sxd;
@@ -227,6 +232,33 @@ int sxd_lgl_differentiation_matrix(int n, double* d) {
   if (d == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null output");
   const std::vector<double>& t = packed_differentiation_matrices();
   std::memcpy(d, t.data() + packed_d_offset(n), sizeof(double) * n * n);
+  return ok();
+}
+
+int sxd_lg_points_and_weights(int n, double* x, double* w) {
+  if (n < kMinPoints || n > kMaxPoints)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "Gauss quadrature here takes 2..20 points, got %d", n);
+  if (x == nullptr || w == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null output");
+  lg_points_and_weights(n, x, w);
+  return ok();
+}
+
+int sxd_lg_differentiation_matrix(int n, double* d) {
+  if (n < kMinPoints || n > kMaxPoints)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "Gauss quadrature here takes 2..20 points, got %d", n);
+  if (d == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null output");
+  const std::vector<double>& t = packed_gauss_tables();
+  std::memcpy(d, t.data() + packed_d_offset(n), sizeof(double) * n * n);
+  return ok();
+}
+
+int sxd_lg_boundary_terms(int n, double* out) {
+  if (n < kMinPoints || n > kMaxPoints)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "Gauss quadrature here takes 2..20 points, got %d", n);
+  if (out == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null output");
+  const std::vector<double>& t = packed_gauss_tables();
+  std::memcpy(out, t.data() + kPackedDSize + packed_gauss_boundary_offset(n),
+              sizeof(double) * 4 * n);
   return ok();
 }
 
@@ -847,6 +879,32 @@ int sxd_scalar_wave_rhs_constraint_preserving_spherical_radiation(
   return finish(run_sw(d, sc, resolve_kernel(d, SXD_SYSTEM_SCALAR_WAVE), u, gamma2, inv_jac, dt,
                        static_cast<hipStream_t>(stream),
                        type == SXD_CPSR_SOMMERFELD ? nullptr : coords, kind),
+                __func__);
+}
+
+int sxd_scalar_wave_rhs_gauss(const sxd_domain* domain, const double* u, const double* gamma2,
+                              const double* inv_jac, const double* det_inv_jac,
+                              const double* face_inv_jac, double* dt, void* stream) {
+  if (int rc = check_periodic(domain); rc != SXD_OK) return rc;
+  if (domain->mixed)
+    return fail(SXD_ERR_UNSUPPORTED, "%s: Gauss meshes take uniform domains", __func__);
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(gamma2);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  SXD_REQUIRE_DEVICE(det_inv_jac);
+  SXD_REQUIRE_DEVICE(face_inv_jac);
+  SXD_REQUIRE_DEVICE(dt);
+  sxd_domain* d = const_cast<sxd_domain*>(domain);
+  if (int rc = check_no_overlap(d, 5, u, dt, __func__); rc != SXD_OK) return rc;
+  if (device_gauss_tables() == nullptr)
+    return fail(SXD_ERR_DEVICE, "%s: upload of the Gauss tables failed", __func__);
+  Scratch& sc = stream_scratch(d, static_cast<hipStream_t>(stream));
+  const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
+  if (int rc = reserve_scratch(sc, 15 * np * d->num_elements, 0); rc != SXD_OK)
+    return finish(rc, __func__);
+  return finish(launch_sw_rhs_gauss(d->num_elements, d->ext, d->d_nbr, u, gamma2, inv_jac,
+                                    det_inv_jac, face_inv_jac, dt, sc.du,
+                                    static_cast<hipStream_t>(stream)),
                 __func__);
 }
 

@@ -105,8 +105,8 @@ bool allow_lds(const void* kernel, int bytes) {
 
 int launch_partial_derivatives(int num_elements, const int32_t ext[3], int nc,
                                const double* u, const double* inv_jac, double* du,
-                               double* dlogical, hipStream_t stream) {
-  const double* dtab = device_diff_matrices();
+                               double* dlogical, hipStream_t stream, const double* dtab) {
+  if (dtab == nullptr) dtab = device_diff_matrices();
   if (dtab == nullptr) return SXD_ERR_DEVICE;
   const int np = ext[0] * ext[1] * ext[2];
   const int comps_per_block = nc < 4 ? nc : 4;
@@ -794,6 +794,14 @@ int launch_sw_rhs(int num_elements, const int32_t ext[3], const int32_t* nbr,
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   return launch_face_passes(false, num_elements, ext, nbr, u, gamma2, inv_jac, dt, ghost,
                             bc_kind, scratch_du, stream);
+}
+
+int launch_sw_volume_batched(int64_t np, int64_t total, const double* u, const double* du,
+                             const double* gamma2, double* dt, hipStream_t stream) {
+  if (total == 0) return SXD_OK;
+  hipLaunchKernelGGL(sw_volume_batched_kernel, dim3(static_cast<unsigned>((total + 255) / 256)),
+                     dim3(256), 0, stream, static_cast<int>(np), total, u, du, gamma2, dt);
+  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
 }
 
 // ---------------------------------------------------------------------------

@@ -13,9 +13,24 @@ namespace sxd {
 // Enables `bytes` of dynamic LDS for `kernel` on the current device (needed
 // beyond 64 KiB; done once per device and kernel).  False if it cannot be.
 bool allow_lds(const void* kernel, int bytes);
+// dtab: packed 1-d differentiation matrices (packed_d_offset); null = the
+// LGL table.
 int launch_partial_derivatives(int num_elements, const int32_t ext[3], int nc,
                                const double* u, const double* inv_jac, double* du,
-                               double* dlogical, hipStream_t stream);
+                               double* dlogical, hipStream_t stream,
+                               const double* dtab = nullptr);
+// ScalarWave point-wise terms of a batch ([E][5][Np] u and dt, [E][15][Np] du,
+// [E][Np] gamma2; total = E Np).
+int launch_sw_volume_batched(int64_t np, int64_t total, const double* u, const double* du,
+                             const double* gamma2, double* dt, hipStream_t stream);
+// ScalarWave RHS on Legendre-Gauss points (gauss.hip); scratch_du [E][15][Np];
+// face_inv_jac per element: the six faces in direction order, each 3 rows of
+// its points' d xi^d / d x^i.
+const double* device_gauss_tables();
+int launch_sw_rhs_gauss(int num_elements, const int32_t ext[3], const int32_t* nbr,
+                        const double* u, const double* gamma2, const double* inv_jac,
+                        const double* det_inv_jac, const double* face_inv_jac, double* dt,
+                        double* scratch_du, hipStream_t stream);
 // ubar [E][nc][Np] = transpose of the partial derivatives applied to dubar
 // [E][3 nc][Np]; SXD_ERR_UNSUPPORTED if 3 Np doubles plus the D matrices
 // exceed the LDS.

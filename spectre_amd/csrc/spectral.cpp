@@ -223,4 +223,126 @@ const std::vector<double>& packed_weak_flux_matrices() {
   return table;
 }
 
+// ---------------------------------------------------------------------------
+// Legendre-Gauss (Kopriva Alg. 23, NumericalAlgorithms/Spectral/Legendre.cpp:
+// 106-148): the roots of L_n, bracketed as the reference brackets them for
+// TOMS748 (Legendre.cpp:127-131) and refined by safeguarded Newton steps with
+// L_n' from (1 - x^2) L_n' = n (L_{n-1} - x L_n); weights
+// w = 2 / ((1 - x^2) L_n'(x)^2).
+namespace {
+struct LegendreN {
+  double l, dl;  // L_n(x), L_n'(x)
+};
+
+LegendreN legendre_n(int n, double x) {
+  double prev = 1.0, cur = x;  // L_0, L_1
+  for (int k = 2; k <= n; ++k) {
+    const double next = ((2.0 * k - 1.0) * x * cur - (k - 1.0) * prev) / k;
+    prev = cur;
+    cur = next;
+  }
+  return {cur, n * (prev - x * cur) / (1.0 - x * x)};
+}
+
+double gauss_root(int n, int j) {
+  const double pi = 3.14159265358979323846;
+  double lo = -std::cos((2.0 * j) * pi / (2.0 * n));
+  double hi = -std::cos((2.0 * j + 2.0) * pi / (2.0 * n));
+  const double l_lo = legendre_n(n, lo).l;
+  double x = 0.5 * (lo + hi);
+  for (int it = 0; it < 200; ++it) {
+    const LegendreN v = legendre_n(n, x);
+    if (v.l == 0.0) break;
+    if ((v.l < 0.0) == (l_lo < 0.0)) {
+      lo = x;
+    } else {
+      hi = x;
+    }
+    double next = x - v.l / v.dl;
+    if (!(next > lo && next < hi)) next = 0.5 * (lo + hi);
+    const bool done = next == x || hi - lo <= 4.0 * 2.220446049250313e-16 * std::fabs(x);
+    x = next;
+    if (done) break;
+  }
+  const LegendreN v = legendre_n(n, x);
+  const double next = x - v.l / v.dl;
+  if (std::fabs(legendre_n(n, next).l) < std::fabs(v.l)) x = next;
+  return x;
+}
+}  // namespace
+
+void lg_points_and_weights(int n, double* x, double* w) {
+  for (int i = 0; i < n; ++i) x[i] = w[i] = 0.0;
+  for (int j = 0; j < n / 2; ++j) {
+    const double r = gauss_root(n, j);
+    const double dl = legendre_n(n, r).dl;
+    x[j] = r;
+    x[n - 1 - j] = -r;
+    w[j] = w[n - 1 - j] = 2.0 / (1.0 - r * r) / (dl * dl);
+  }
+  if (n % 2 == 1) {
+    const double dl = legendre_n(n, 0.0).dl;
+    x[n / 2] = 0.0;
+    w[n / 2] = 2.0 / (dl * dl);
+  }
+}
+
+namespace {
+// Barycentric weights (Kopriva Alg. 30, Spectral.cpp:83-101).
+void barycentric(int n, const double* x, double* b) {
+  for (int k = 0; k < n; ++k) b[k] = 1.0;
+  for (int j = 1; j < n; ++j)
+    for (int k = 0; k < j; ++k) {
+      b[k] *= x[k] - x[j];
+      b[j] *= x[j] - x[k];
+    }
+  for (int k = 0; k < n; ++k) b[k] = 1.0 / b[k];
+}
+}  // namespace
+
+void lg_differentiation_matrix(int n, double* d) {
+  double x[kMaxPoints], w[kMaxPoints], b[kMaxPoints];
+  lg_points_and_weights(n, x, w);
+  barycentric(n, x, b);
+  for (int i = 0; i < n; ++i) {
+    double diag = 0.0;
+    for (int j = 0; j < n; ++j) {
+      if (j == i) continue;
+      d[i * n + j] = b[j] / (b[i] * (x[i] - x[j]));
+      diag -= d[i * n + j];
+    }
+    d[i * n + i] = diag;
+  }
+}
+
+void lg_boundary_terms(int n, double* out) {
+  double x[kMaxPoints], w[kMaxPoints], b[kMaxPoints];
+  lg_points_and_weights(n, x, w);
+  barycentric(n, x, b);
+  for (int side = 0; side < 2; ++side) {
+    const double t = side == 0 ? -1.0 : 1.0;
+    double* row = out + side * n;
+    double sum = 0.0;
+    for (int j = 0; j < n; ++j) {
+      row[j] = b[j] / (t - x[j]);
+      sum += row[j];
+    }
+    for (int j = 0; j < n; ++j) row[j] /= sum;
+    for (int j = 0; j < n; ++j) out[(2 + side) * n + j] = row[j] / w[j];
+  }
+}
+
+const std::vector<double>& packed_gauss_tables() {
+  static std::vector<double> table;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    table.assign(kPackedGaussSize, 0.0);
+    for (int n = kMinPoints; n <= kMaxPoints; ++n) {
+      lg_differentiation_matrix(n, table.data() + packed_d_offset(n));
+      lg_boundary_terms(n, table.data() + kPackedDSize + packed_gauss_boundary_offset(n));
+    }
+  });
+  return table;
+}
+
 }  // namespace sxd

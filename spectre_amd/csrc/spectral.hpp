@@ -32,6 +32,26 @@ constexpr int packed_mortar_offset(int nf, int nm) {
 constexpr int kPackedMortarSize = packed_mortar_offset(kMaxPoints + 1, 0);
 
 void lgl_points_and_weights(int n, double* x, double* w);
+
+// Legendre-Gauss (NumericalAlgorithms/Spectral/Legendre.cpp:106-148):
+// points, weights, differentiation matrix (row-major), and the boundary
+// terms: out[0..n) interpolation to xi = -1, [n..2n) to xi = +1
+// (boundary_interpolation_matrices, Spectral.cpp:677-700), [2n..3n) and
+// [3n..4n) the lifting terms l_j(-1) / w_j and l_j(+1) / w_j
+// (boundary_lifting_term, Spectral.cpp:738-768).
+void lg_points_and_weights(int n, double* x, double* w);
+void lg_differentiation_matrix(int n, double* d);
+void lg_boundary_terms(int n, double* out);
+// Packed device table of the Gauss path: the D matrices for n = 2..20 at
+// packed_d_offset(n), then the boundary terms at
+// kPackedDSize + packed_gauss_boundary_offset(n).
+constexpr int packed_gauss_boundary_offset(int n) {
+  int off = 0;
+  for (int m = kMinPoints; m < n; ++m) off += 4 * m;
+  return off;
+}
+constexpr int kPackedGaussSize = kPackedDSize + packed_gauss_boundary_offset(kMaxPoints + 1);
+const std::vector<double>& packed_gauss_tables();
 void lgl_differentiation_matrix(int n, double* d);  // row-major
 // All D matrices for n = 2..20, row-major, at packed_d_offset(n).
 const std::vector<double>& packed_differentiation_matrices();

@@ -397,6 +397,25 @@ class Domain:
             _ptr(out), _stream()))
         return out
 
+    def scalar_wave_rhs_gauss(self, u, gamma2, inv_jac, det_inv_jac, face_inv_jac, out=None):
+        """ScalarWave RHS on Legendre-Gauss points (uniform periodic domain):
+        det_inv_jac [E, Np]; face_inv_jac [E, F] with the six faces in
+        direction order, each 3 rows of its points' d xi^d / d x^i."""
+        _require(u, "u", self._shapes(5))
+        _require(gamma2, "gamma2", self._scalar_shape())
+        _require(inv_jac, "inv_jac", self._shapes(9))
+        _require(det_inv_jac, "det_inv_jac", self._scalar_shape())
+        n0, n1, n2 = (int(x) for x in self.extents)
+        face_block = 2 * 3 * (n1 * n2 + n0 * n2 + n0 * n1)
+        _require(face_inv_jac, "face_inv_jac", (self.num_elements, face_block))
+        if out is None:
+            out = torch.empty_like(u)
+        _require(out, "out", self._shapes(5))
+        check(lib().sxd_scalar_wave_rhs_gauss(self._handle, _ptr(u), _ptr(gamma2), _ptr(inv_jac),
+                                              _ptr(det_inv_jac), _ptr(face_inv_jac), _ptr(out),
+                                              _stream()))
+        return out
+
     def gh_rhs(self, u, gammas, inv_jac, out=None):
         _require(u, "u", self._shapes(50))
         _require(gammas, "gammas", self._shapes(3))

@@ -1,4 +1,5 @@
-"""Legendre-Gauss-Lobatto tables from the native library (host functions).
+"""Legendre-Gauss-Lobatto and Legendre-Gauss tables from the native library
+(host functions).
 
 Mirrors spectre.Spectral's collocation_points / quadrature_weights /
 differentiation_matrix for Basis.Legendre, Quadrature.GaussLobatto
@@ -38,6 +39,44 @@ def differentiation_matrix(n):
     check(lib().sxd_lgl_differentiation_matrix(n, d.ctypes.data_as(_dp)))
     d.setflags(write=False)
     return d
+
+
+@functools.lru_cache(maxsize=None)
+def _gauss_points_and_weights(n):
+    x = np.empty(n)
+    w = np.empty(n)
+    check(lib().sxd_lg_points_and_weights(n, x.ctypes.data_as(_dp), w.ctypes.data_as(_dp)))
+    x.setflags(write=False)
+    w.setflags(write=False)
+    return x, w
+
+
+def lg_points(n):
+    """Legendre-Gauss points (Quadrature.Gauss)."""
+    return _gauss_points_and_weights(n)[0]
+
+
+def lg_weights(n):
+    return _gauss_points_and_weights(n)[1]
+
+
+@functools.lru_cache(maxsize=None)
+def lg_differentiation_matrix(n):
+    d = np.empty((n, n))
+    check(lib().sxd_lg_differentiation_matrix(n, d.ctypes.data_as(_dp)))
+    d.setflags(write=False)
+    return d
+
+
+@functools.lru_cache(maxsize=None)
+def lg_boundary_terms(n):
+    """[4, n]: boundary interpolation rows to xi = -1 and +1
+    (boundary_interpolation_matrices) and lifting terms l_j(-1) / w_j,
+    l_j(+1) / w_j (boundary_lifting_term)."""
+    out = np.empty((4, n))
+    check(lib().sxd_lg_boundary_terms(n, out.ctypes.data_as(_dp)))
+    out.setflags(write=False)
+    return out
 
 
 # Names used by spectre.Spectral

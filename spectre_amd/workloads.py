@@ -197,10 +197,13 @@ class Brick:
     def widths(self):
         return tuple((self.upper[d] - self.lower[d]) / self.elements[d] for d in range(3))
 
+    def _points(self, n):
+        return spectral.lgl_points(n)
+
     def coordinates(self):
         """Inertial coordinates [E][3][Np]."""
         n0, n1, n2 = self.extents
-        xs = [spectral.lgl_points(n) for n in self.extents]
+        xs = [self._points(n) for n in self.extents]
         ex, ey, ez = self.elements
         h = self.widths
         e = np.arange(self.num_elements)
@@ -219,6 +222,44 @@ class Brick:
         for d in range(3):
             inv[:, d + 3 * d, :] = 2.0 / self.widths[d]
         return inv
+
+
+class GaussBrick(Brick):
+    """A periodic brick of affine elements on Legendre-Gauss points (no grid
+    point on a face)."""
+
+    def _points(self, n):
+        return spectral.lg_points(n)
+
+    def det_inverse_jacobian(self):
+        """[E][Np]: det(d xi / d x) = prod_d 2 / h_d."""
+        return np.full((self.num_elements, self.num_points),
+                       (2.0 / self.widths[0]) * (2.0 / self.widths[1]) * (2.0 / self.widths[2]))
+
+    def face_inverse_jacobian(self):
+        """[E][face block]: the six faces in direction order, each 3 rows of
+        its face points holding d xi^d / d x^i (d the face's dimension) as the
+        affine map gives them on the face: 2 / h_d for i = d, else 0."""
+        blocks = []
+        for dir_ in range(6):
+            d = dir_ // 2
+            nf = self.num_points // self.extents[d]
+            rows = np.zeros((3, nf))
+            rows[d] = 2.0 / self.widths[d]
+            blocks.append(rows.ravel())
+        one = np.concatenate(blocks)
+        return np.tile(one, (self.num_elements, 1))
+
+
+def scalar_wave_plane_wave_gauss(seed=0, elements=(4, 4, 4), extents=(5, 5, 5)):
+    """Plane wave on the periodic brick [-1,1]^3 of Legendre-Gauss elements
+    with gamma2 = 1.  Returns (brick, u, gamma2, inv_jac, det_inv_jac,
+    face_inv_jac)."""
+    brick = GaussBrick((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), elements, extents)
+    u, _ = plane_wave(brick.coordinates(), seed)
+    gamma2 = np.ones((brick.num_elements, brick.num_points))
+    return (brick, u, gamma2, brick.inverse_jacobian(), brick.det_inverse_jacobian(),
+            brick.face_inverse_jacobian())
 
 
 class MixedBrick:

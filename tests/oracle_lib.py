@@ -91,6 +91,45 @@ def logical_partial_derivatives(u, extents):
     return out
 
 
+def lg_points_and_weights(n):
+    """Legendre-Gauss points and weights."""
+    x = np.empty(n)
+    w = np.empty(n)
+    _check(lib().sxo_lg_points_and_weights(n, _d(x), _d(w)))
+    return x, w
+
+
+def lg_differentiation_matrix(n):
+    d = np.empty((n, n))
+    _check(lib().sxo_lg_differentiation_matrix(n, _d(d)))
+    return d
+
+
+def lg_boundary_terms(n):
+    """[4, n]: interpolation rows to -1 and +1, lifting terms at -1 and +1."""
+    out = np.empty((4, n))
+    _check(lib().sxo_lg_boundary_terms(n, _d(out)))
+    return out
+
+
+def sw_rhs_gauss(extents, neighbors, u, gamma2, inv_jac, det_inv_jac, face_inv_jac):
+    """ScalarWave DG RHS on Legendre-Gauss points (periodic, uniform extents):
+    u [E,5,Np], gamma2 [E,Np], inv_jac [E,9,Np], det_inv_jac [E,Np],
+    face_inv_jac [E, face block] -> dt [E,5,Np]."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    gamma2 = np.ascontiguousarray(gamma2, dtype=np.float64)
+    inv_jac = np.ascontiguousarray(inv_jac, dtype=np.float64)
+    det_inv_jac = np.ascontiguousarray(det_inv_jac, dtype=np.float64)
+    face_inv_jac = np.ascontiguousarray(face_inv_jac, dtype=np.float64)
+    e = u.shape[0]
+    dt = np.empty_like(u)
+    ext, ep = _i(extents)
+    nb, nbp = _i(np.asarray(neighbors).reshape(-1))
+    _check(lib().sxo_sw_rhs_gauss(e, ep, nbp, _d(u), _d(gamma2), _d(inv_jac), _d(det_inv_jac),
+                                  _d(face_inv_jac), _d(dt)))
+    return dt
+
+
 def weak_flux_differentiation_matrix(n):
     w = np.empty((n, n))
     _check(lib().sxo_weak_flux_differentiation_matrix(n, _d(w)))
