@@ -78,6 +78,25 @@ int sxd_lg_differentiation_matrix(int n, double* d);
  * (Spectral::boundary_lifting_term, Spectral.cpp:738-768). */
 int sxd_lg_boundary_terms(int n, double* out);
 
+/* The part of a parent mesh a child mesh (a mortar) covers
+ * (Spectral::ChildSize, NumericalAlgorithms/Spectral/Projection.hpp:22-30). */
+enum { SXD_CHILD_FULL = 1, SXD_CHILD_UPPER_HALF = 2, SXD_CHILD_LOWER_HALF = 3 };
+
+/* LGL prolongation from n_parent to n_child >= n_parent points on the part
+ * child_size of the parent interval, row-major [n_child][n_parent].
+ * Replaces Spectral::projection_matrix_parent_to_child(const Mesh<1>&,
+ * const Mesh<1>&, ChildSize) (Spectral/Projection.cpp:283-362). */
+int sxd_projection_matrix_parent_to_child(int n_parent, int n_child, int child_size,
+                                          double* out);
+
+/* LGL restriction (non-massive operand) of data on n_child >= n_parent
+ * points covering the part child_size of the parent interval, row-major
+ * [n_parent][n_child].  Replaces Spectral::projection_matrix_child_to_parent
+ * (Spectral/Projection.cpp:36-240; the mortar Vandermonde inverse is the
+ * exact discrete one, w_j P_k(x_j) / gamma_k). */
+int sxd_projection_matrix_child_to_parent(int n_child, int n_parent, int child_size,
+                                          double* out);
+
 /* ---- Linear operators ------------------------------------------------- */
 
 /* Logical partial derivatives: dlogical = [3][E][nc][Np] (d/dxi, d/deta,
@@ -253,6 +272,37 @@ int sxd_domain_destroy(sxd_domain* domain);
  * sxd_gh_rhs_dirichlet only. */
 int sxd_domain_create_mixed(sxd_domain** domain, int num_elements,
                             const int32_t* extents, const int32_t* neighbors);
+/* A set of elements coupled through explicitly listed mortars: neighbours
+ * whose logical axes are oriented differently (OrientationMap,
+ * Domain/Structure/OrientationMap.hpp) and h-refined faces covered by two or
+ * four mortars (2:1 balance), on top of per-element extents as in
+ * sxd_domain_create_mixed (ragged layout).  Replaces the mortar bookkeeping
+ * of ComputeTimeDerivative.hpp:690-774 (orient_variables_on_slice,
+ * Domain/Structure/OrientationMapHelpers.cpp:372-400) and
+ * ApplyBoundaryCorrections.hpp:797-1043 with
+ * projection_matrix_parent_to_child / _child_to_parent for the Full,
+ * UpperHalf and LowerHalf mortar sizes (Spectral/Projection.cpp).
+ * records: host array [num_records][8] of int32, one per (element,
+ * direction, neighbour) -- the reference's DirectionalId keyed mortars:
+ *   { e, dir, en, o0, o1, o2, size_a, size_b }
+ * dir = 2 d + side of e's logical frame; o_j = the neighbour's logical
+ * direction 2 k + side that e's upper logical axis j maps to (the mapped
+ * directions of the OrientationMap; aligned = {1, 3, 5}); size_a, size_b =
+ * the part of e's face the mortar covers along the face's two dimensions
+ * (lower dimension first): SXD_CHILD_FULL, SXD_CHILD_UPPER_HALF,
+ * SXD_CHILD_LOWER_HALF (Spectral::ChildSize; dg::mortar_size,
+ * NumericalAlgorithms/DiscontinuousGalerkin/MortarHelpers.cpp:49-74).  Every
+ * record needs the neighbour's record back (direction o(dir opposite)), with
+ * the inverse orientation; the mortar mesh is the larger extents of the two
+ * faces per dimension; the mortars of a face must cover it exactly once; no
+ * external faces.  Corrections reach dt face by face in direction order and,
+ * within a face, in the order of the records.  sxd_scalar_wave_rhs,
+ * sxd_gh_rhs, sxd_domain_autotune and the RK3 stepper take such a domain
+ * (kernel SXD_KERNEL_MORTAR); the boundary-condition, Gauss, adjoint and
+ * allocation-free calls return SXD_ERR_UNSUPPORTED.  SXD_ERR_INVALID_ARGUMENT
+ * (with the reason in sxd_last_error) for an inconsistent record list. */
+int sxd_domain_create_mortars(sxd_domain** domain, int num_elements, const int32_t* extents,
+                              int num_records, const int32_t* records);
 /* Total grid points of the domain (sum of Np_e). */
 int sxd_domain_num_points(const sxd_domain* domain, int64_t* points);
 
@@ -385,7 +435,10 @@ enum {
   /* mixed-p domains: */
   SXD_KERNEL_MIXED_BATCHED = 4,    /* one launch per kernel over all elements,
                                       ordered by decreasing cost */
-  SXD_KERNEL_MIXED_PER_EXTENTS = 5 /* one launch per kernel and group of equal extents */
+  SXD_KERNEL_MIXED_PER_EXTENTS = 5, /* one launch per kernel and group of equal extents */
+  /* mortar domains: */
+  SXD_KERNEL_MORTAR = 6 /* ragged volume kernels, then package, projection,
+                           boundary-term and lift launches over the mortars */
 };
 enum { SXD_SYSTEM_SCALAR_WAVE = 0, SXD_SYSTEM_GH = 1 };
 

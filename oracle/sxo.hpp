@@ -1375,6 +1375,44 @@ inline void to_mortar(int npk, int na, int nb, int ma, int mb, const T* pk_own, 
                       mortar_interpolation_matrix(mb, *kb), pk_nb, nb_m->data());
 }
 
+// Volume terms of every element of a ragged layout (VolumeTermsImpl.tpp:
+// 72-153: partial derivatives, then the point-wise time derivative).
+template <class T>
+void sw_volume_ragged(const RaggedLayout& L, const T* u, const T* gamma2, const T* inv_j, T* dt) {
+  for (size_t e = 0; e < L.np.size(); ++e) {
+    const size_t np = L.np[e], off = L.off[e];
+    std::vector<T> du(15 * np);
+    partial_derivatives(u + 5 * off, 5, L.ext[e].data(), inv_j + 9 * off, du.data());
+    for (size_t p = 0; p < np; ++p) {
+      T up[5], dup[15], dtp[5];
+      for (int c = 0; c < 5; ++c) up[c] = u[5 * off + c * np + p];
+      for (int c = 0; c < 15; ++c) dup[c] = du[c * np + p];
+      sw_time_derivative_point(dtp, up, dup, gamma2[off + p]);
+      for (int c = 0; c < 5; ++c) dt[5 * off + c * np + p] = dtp[c];
+    }
+  }
+}
+
+// GH volume terms of a ragged layout; temps receives each point's face
+// temporaries (gammas [3][Np_e] per element).
+template <class T>
+void gh_volume_ragged(const RaggedLayout& L, const T* u, const T* gammas, const T* inv_j, T* dt,
+                      GhFaceTemps<T>* temps) {
+  for (size_t e = 0; e < L.np.size(); ++e) {
+    const size_t np = L.np[e], off = L.off[e];
+    std::vector<T> du(150 * np);
+    partial_derivatives(u + 50 * off, 50, L.ext[e].data(), inv_j + 9 * off, du.data());
+    const T* ga = gammas + 3 * off;
+    for (size_t p = 0; p < np; ++p) {
+      T up[50], dup[150], dtp[50];
+      for (int c = 0; c < 50; ++c) up[c] = u[50 * off + c * np + p];
+      for (int c = 0; c < 150; ++c) dup[c] = du[c * np + p];
+      gh_time_derivative_point(dtp, &temps[off + p], up, dup, ga[p], ga[np + p], ga[2 * np + p]);
+      for (int c = 0; c < 50; ++c) dt[50 * off + c * np + p] = dtp[c];
+    }
+  }
+}
+
 // Full ScalarWave DG RHS on a mixed-p set of elements (ragged layout).
 // External faces (neighbour < 0) take the exterior state as in sw_rhs_faces
 // (bc_kind kSwGhostArray: ghost ragged [5][Np_e] per element, read at the
@@ -1388,18 +1426,7 @@ void sw_rhs_mixed(int num_elements, const int* extents, const int* nbr, const T*
                   const T* gamma2, const T* inv_j, T* dt, const T* ghost = nullptr,
                   int bc_kind = kSwGhostArray) {
   const RaggedLayout L(num_elements, extents);
-  for (int e = 0; e < num_elements; ++e) {
-    const size_t np = L.np[e], off = L.off[e];
-    std::vector<T> du(15 * np);
-    partial_derivatives(u + 5 * off, 5, L.ext[e].data(), inv_j + 9 * off, du.data());
-    for (size_t p = 0; p < np; ++p) {
-      T up[5], dup[15], dtp[5];
-      for (int c = 0; c < 5; ++c) up[c] = u[5 * off + c * np + p];
-      for (int c = 0; c < 15; ++c) dup[c] = du[c * np + p];
-      sw_time_derivative_point(dtp, up, dup, gamma2[off + p]);
-      for (int c = 0; c < 5; ++c) dt[5 * off + c * np + p] = dtp[c];
-    }
-  }
+  sw_volume_ragged(L, u, gamma2, inv_j, dt);
   const bool cpsr = bc_kind >= kSwCpsrSommerfeld;
   std::vector<T> du_e;
   for (int e = 0; e < num_elements; ++e) {
@@ -1527,19 +1554,7 @@ void gh_rhs_mixed(int num_elements, const int* extents, const int* nbr, const T*
                   const T* gammas, const T* inv_j, T* dt, const T* ghost = nullptr) {
   const RaggedLayout L(num_elements, extents);
   std::vector<GhFaceTemps<T>> temps(L.total);
-  for (int e = 0; e < num_elements; ++e) {
-    const size_t np = L.np[e], off = L.off[e];
-    std::vector<T> du(150 * np);
-    partial_derivatives(u + 50 * off, 50, L.ext[e].data(), inv_j + 9 * off, du.data());
-    const T* ga = gammas + 3 * off;
-    for (size_t p = 0; p < np; ++p) {
-      T up[50], dup[150], dtp[50];
-      for (int c = 0; c < 50; ++c) up[c] = u[50 * off + c * np + p];
-      for (int c = 0; c < 150; ++c) dup[c] = du[c * np + p];
-      gh_time_derivative_point(dtp, &temps[off + p], up, dup, ga[p], ga[np + p], ga[2 * np + p]);
-      for (int c = 0; c < 50; ++c) dt[50 * off + c * np + p] = dtp[c];
-    }
-  }
+  gh_volume_ragged(L, u, gammas, inv_j, dt, temps.data());
   for (int e = 0; e < num_elements; ++e) {
     const size_t np = L.np[e], off = L.off[e];
     for (int step = 0; step < 12; ++step) {

@@ -11,6 +11,7 @@
 #include "sxo.hpp"
 #include "sxo_flux.hpp"
 #include "sxo_gauss.hpp"
+#include "sxo_mortar.hpp"
 
 namespace {
 bool bad_extents(const int* ext) {
@@ -627,6 +628,53 @@ int sxo_rk3_substep(int substep, long n, double dt_step, const double* u0,
                     const double* uk, const double* lk, double* out) {
   if (substep < 0 || substep > 2) return -1;
   sxo::rk3_substep(substep, static_cast<size_t>(n), dt_step, u0, uk, lk, out);
+  return 0;
+}
+
+// Non-conforming mortars (sxo_mortar.hpp).
+int sxo_half_interpolation_matrix(int n_face, int n_mortar, int upper, double* out) {
+  if (n_face < 2 || n_mortar < n_face || n_mortar > 20) return -1;
+  const std::vector<double> m = sxo::half_interpolation_matrix(n_face, n_mortar, upper != 0);
+  std::memcpy(out, m.data(), m.size() * sizeof(double));
+  return 0;
+}
+
+int sxo_half_projection_matrix(int n_face, int n_mortar, int upper, double* out) {
+  if (n_face < 2 || n_mortar < n_face || n_mortar > 20) return -1;
+  const std::vector<double> m = sxo::half_projection_matrix(n_face, n_mortar, upper != 0);
+  std::memcpy(out, m.data(), m.size() * sizeof(double));
+  return 0;
+}
+
+int sxo_oriented_offset_on_slice(int n1, int n2, int sliced_dim, const int* mapped, int* out) {
+  if (n1 < 1 || n2 < 1 || sliced_dim < 0 || sliced_dim > 2) return -1;
+  sxo::Orientation o{{mapped[0], mapped[1], mapped[2]}};
+  const std::vector<int> v = sxo::oriented_offset_on_slice(n1, n2, sliced_dim, o);
+  std::memcpy(out, v.data(), v.size() * sizeof(int));
+  return 0;
+}
+
+int sxo_sw_rhs_mortars(int num_elements, const int* extents, int num_records, const int* records,
+                       const double* u, const double* gamma2, const double* inv_j, double* dt) {
+  for (int e = 0; e < num_elements; ++e)
+    if (bad_extents(extents + 3 * e)) return -1;
+  try {
+    sxo::rhs_mortars<false>(num_elements, extents, num_records, records, u, gamma2, inv_j, dt);
+  } catch (const std::exception&) {
+    return -2;
+  }
+  return 0;
+}
+
+int sxo_gh_rhs_mortars(int num_elements, const int* extents, int num_records, const int* records,
+                       const double* u, const double* gammas, const double* inv_j, double* dt) {
+  for (int e = 0; e < num_elements; ++e)
+    if (bad_extents(extents + 3 * e)) return -1;
+  try {
+    sxo::rhs_mortars<true>(num_elements, extents, num_records, records, u, gammas, inv_j, dt);
+  } catch (const std::exception&) {
+    return -2;
+  }
   return 0;
 }
 

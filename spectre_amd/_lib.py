@@ -61,6 +61,12 @@ _SIGNATURES = {
     "sxd_gh_rhs_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_rk3_substep": (ctypes.c_int, [ctypes.c_int, _i64, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]),
     "sxd_domain_create_mixed": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _ip, _ip]),
+    "sxd_domain_create_mortars": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _ip,
+                                                 ctypes.c_int, _ip]),
+    "sxd_projection_matrix_parent_to_child": (ctypes.c_int, [ctypes.c_int, ctypes.c_int,
+                                                             ctypes.c_int, _dp]),
+    "sxd_projection_matrix_child_to_parent": (ctypes.c_int, [ctypes.c_int, ctypes.c_int,
+                                                             ctypes.c_int, _dp]),
     "sxd_domain_num_points": (ctypes.c_int, [_vp, ctypes.POINTER(_i64)]),
     "sxd_domain_set_threads": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "sxd_domain_get_threads": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
@@ -80,7 +86,7 @@ _SIGNATURES = {
 
 # Kernel organisations of the whole-domain right-hand sides (SXD_KERNEL_*)
 KERNELS = {"auto": 0, "composite": 1, "fused": 2, "fused_mfma": 3, "mixed_batched": 4,
-           "mixed_per_extents": 5}
+           "mixed_per_extents": 5, "mortar": 6}
 SYSTEMS = {"scalar_wave": 0, "gh": 1}
 
 

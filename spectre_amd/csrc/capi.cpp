@@ -93,6 +93,10 @@ struct sxd_domain {
   int32_t* d_buckets = nullptr;       // elements grouped by extents
   int32_t* d_point_el = nullptr;      // element of each grid point
   std::vector<std::pair<int, int>> buckets;  // (start, count) in d_buckets
+  // Mortar domains (non-aligned neighbours, h-refined faces): the ragged
+  // element set above (its neighbour table unused) and the mortar tables.
+  bool mortars = false;
+  sxd::MortarPlan plan;
 };
 
 namespace sxd {
@@ -198,6 +202,11 @@ const double* device_mortar_matrices() {
   return device_table(mortar_tables(), packed_mortar_matrices());
 }
 
+const double* device_half_mortar_matrices() {
+  static DeviceTables t;
+  return device_table(t, packed_half_mortar_matrices());
+}
+
 const double* device_weak_flux_matrices() {
   static DeviceTables t;
   return device_table(t, packed_weak_flux_matrices());
@@ -249,6 +258,34 @@ int sxd_lg_differentiation_matrix(int n, double* d) {
   if (d == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null output");
   const std::vector<double>& t = packed_gauss_tables();
   std::memcpy(d, t.data() + packed_d_offset(n), sizeof(double) * n * n);
+  return ok();
+}
+
+int sxd_projection_matrix_parent_to_child(int n_parent, int n_child, int child_size,
+                                          double* out) {
+  if (n_parent < kMinPoints || n_child < n_parent || n_child > kMaxPoints)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "need 2 <= n_parent <= n_child <= 20");
+  if (child_size < SXD_CHILD_FULL || child_size > SXD_CHILD_LOWER_HALF)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "unknown child size %d", child_size);
+  if (out == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null output");
+  if (child_size == SXD_CHILD_FULL)
+    mortar_interpolation_matrix(n_parent, n_child, out);
+  else
+    half_mortar_interpolation_matrix(n_parent, n_child, child_size == SXD_CHILD_UPPER_HALF, out);
+  return ok();
+}
+
+int sxd_projection_matrix_child_to_parent(int n_child, int n_parent, int child_size,
+                                          double* out) {
+  if (n_parent < kMinPoints || n_child < n_parent || n_child > kMaxPoints)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "need 2 <= n_parent <= n_child <= 20");
+  if (child_size < SXD_CHILD_FULL || child_size > SXD_CHILD_LOWER_HALF)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "unknown child size %d", child_size);
+  if (out == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null output");
+  if (child_size == SXD_CHILD_FULL)
+    mortar_projection_matrix(n_parent, n_child, out);
+  else
+    half_mortar_projection_matrix(n_parent, n_child, child_size == SXD_CHILD_UPPER_HALF, out);
   return ok();
 }
 
@@ -615,6 +652,38 @@ int sxd_domain_create_mixed(sxd_domain** domain, int num_elements, const int32_t
   return ok();
 }
 
+int sxd_domain_create_mortars(sxd_domain** domain, int num_elements, const int32_t* extents,
+                              int num_records, const int32_t* records) {
+  if (domain == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null domain output");
+  *domain = nullptr;
+  if (num_elements < 1) return fail(SXD_ERR_INVALID_ARGUMENT, "need at least one element");
+  if (extents == nullptr || records == nullptr)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "null extents or record list");
+  for (int e = 0; e < num_elements; ++e)
+    if (!valid_extents(extents + 3 * e))
+      return fail(SXD_ERR_INVALID_ARGUMENT, "extents of element %d must be 2..20", e);
+  // the ragged element set: a neighbour table of self-references, which no
+  // mortar-domain path reads
+  std::vector<int32_t> self(6 * static_cast<size_t>(num_elements));
+  for (size_t i = 0; i < self.size(); ++i) self[i] = static_cast<int32_t>(i / 6);
+  sxd_domain* d = nullptr;
+  if (int rc = sxd_domain_create_mixed(&d, num_elements, extents, self.data()); rc != SXD_OK)
+    return rc;
+  std::string error;
+  if (int rc = build_mortar_plan(num_elements, extents, num_records, records, &d->plan, &error);
+      rc != SXD_OK) {
+    sxd_domain_destroy(d);
+    return fail(rc, "%s", error.c_str());
+  }
+  d->mortars = true;
+  if (device_half_mortar_matrices() == nullptr) {
+    sxd_domain_destroy(d);
+    return fail(SXD_ERR_DEVICE, "upload of the half-interval mortar tables failed");
+  }
+  *domain = d;
+  return ok();
+}
+
 int sxd_domain_num_points(const sxd_domain* domain, int64_t* points) {
   if (domain == nullptr || points == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null argument");
   *points = domain->mixed ? domain->total_points
@@ -650,6 +719,7 @@ int sxd_domain_destroy(sxd_domain* domain) {
   if (domain->d_list != nullptr) (void)hipFree(domain->d_list);
   if (domain->d_buckets != nullptr) (void)hipFree(domain->d_buckets);
   if (domain->d_nbr != nullptr) (void)hipFree(domain->d_nbr);
+  if (domain->mortars) destroy_mortar_plan(&domain->plan);
   if (!domain->scratch.empty()) {
     (void)hipDeviceSynchronize();
     for (auto& s : domain->scratch) s.second->release();
@@ -711,6 +781,10 @@ namespace {
 // their number (at most 3).
 int candidate_kernels(const sxd_domain* d, int system, int* k) {
   int n = 0;
+  if (d->mortars) {
+    k[n++] = SXD_KERNEL_MORTAR;
+    return n;
+  }
   if (d->mixed) {
     k[n++] = SXD_KERNEL_MIXED_BATCHED;
     k[n++] = SXD_KERNEL_MIXED_PER_EXTENTS;
@@ -740,9 +814,27 @@ int reserve_scratch(Scratch& sc, size_t du_doubles, size_t temps_doubles) {
   return sc.reserve(du_doubles, temps_doubles);
 }
 
+// Volume terms on the ragged element set, then the corrections of every
+// mortar; work arrays: the derivatives (GH) and the mortar buffers.
+int run_mortars(sxd_domain* d, Scratch& sc, bool gh, const double* u, const double* params,
+                const double* inv_jac, double* dt, hipStream_t stream) {
+  const size_t du = gh ? 150 * static_cast<size_t>(d->total_points) : 0;
+  if (int rc = reserve_scratch(sc, du, mortar_work_doubles(gh, d->plan)); rc != SXD_OK) return rc;
+  const int threads = d->threads[gh ? SXD_SYSTEM_GH : SXD_SYSTEM_SCALAR_WAVE];
+  if (int rc = launch_mixed_volume(gh, d->num_elements, d->d_list, d->d_el, d->np_max, u, params,
+                                   inv_jac, dt, sc.du, threads, stream);
+      rc != SXD_OK)
+    return rc;
+  return launch_mortar_faces(gh, d->plan, d->d_el, u, params, inv_jac, dt, sc.temps, stream);
+}
+
 int run_mixed(sxd_domain* d, Scratch& sc, bool gh, int kernel, const double* u,
               const double* params, const double* inv_jac, double* dt, hipStream_t stream,
               const double* ghost = nullptr, int bc_kind = kSwBcGhostArray) {
+  if (d->mortars) {
+    if (ghost != nullptr || bc_kind != kSwBcGhostArray) return SXD_ERR_UNSUPPORTED;
+    return run_mortars(d, sc, gh, u, params, inv_jac, dt, stream);
+  }
   double* scratch = nullptr;
   if (gh) {
     if (int rc = reserve_scratch(sc, 150 * static_cast<size_t>(d->total_points), 0); rc != SXD_OK)
@@ -926,6 +1018,8 @@ int sxd_gh_rhs(const sxd_domain* domain, const double* u, const double* gammas,
 int sxd_gh_rhs_noalloc(const sxd_domain* domain, const double* u, const double* gammas,
                        const double* inv_jac, double* dt, void* stream) {
   if (int rc = check_periodic(domain); rc != SXD_OK) return rc;
+  if (domain->mortars)
+    return fail(SXD_ERR_UNSUPPORTED, "%s: not built for mortar domains", __func__);
   sxd_domain* d = const_cast<sxd_domain*>(domain);
   const int kernel = resolve_kernel(d, SXD_SYSTEM_GH);
   // work arrays the kernel organisation needs: 150 doubles per point (mixed
@@ -1098,6 +1192,8 @@ int sxd_gh_rhs_adjoint(const sxd_domain* domain, const double* u, const double* 
   SXD_REQUIRE_DEVICE(gbar);
   SXD_REQUIRE_DEVICE(ubar);
   if (gbar == ubar || u == ubar) return fail(SXD_ERR_INVALID_ARGUMENT, "ubar must not alias");
+  if (domain->mortars)
+    return fail(SXD_ERR_UNSUPPORTED, "%s: not built for mortar domains", __func__);
   sxd_domain* d = const_cast<sxd_domain*>(domain);
   Scratch& sc = stream_scratch(d, static_cast<hipStream_t>(stream));
   if (d->mixed) {

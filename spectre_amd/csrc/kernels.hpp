@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 
 #include "../../include/spectre_amd.h"
 
@@ -109,6 +110,53 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
                      const double* params, const double* inv_jac, double* dt, double* scratch_du,
                      int threads, hipStream_t stream, const double* ghost = nullptr,
                      int bc_kind = kSwBcGhostArray);
+// The volume terms alone of launch_mixed_rhs (dt = point-wise terms).
+int launch_mixed_volume(bool gh, int count, const int32_t* list, const MixedElement* el,
+                        int np_max, const double* u, const double* params,
+                        const double* inv_jac, double* dt, double* scratch_du, int threads,
+                        hipStream_t stream);
+// Non-conforming mortars (mortar.hip): rotated neighbours and h-refined faces.
+// Spectral::ChildSize (NumericalAlgorithms/Spectral/Projection.hpp:22-30).
+constexpr int kChildFull = 1, kChildUpperHalf = 2, kChildLowerHalf = 3;
+const double* device_half_mortar_matrices();
+// A face of an element with at least one mortar: its records are
+// [rec_begin, rec_begin + rec_count) of the record table; pt_off is the
+// offset of its points among all faces' points.
+struct MortarFace {
+  int32_t e, dir, nf, rec_begin, rec_count;
+  int64_t pt_off;
+};
+// One side of a mortar.  na, nb: own face extents; ka, kb: mortar extents in
+// the own frame; size_a, size_b: ChildSize of the mortar on the own face;
+// mortar_off: offset of its own projected package among the projected
+// mortars' points (-1: the mortar is the face); rec_off: offset of its
+// corrections and gather indices among all records' mortar points.
+struct MortarRecord {
+  int32_t face, partner, na, nb, ka, kb, size_a, size_b;
+  int64_t mortar_off, rec_off;
+};
+struct MortarPlan {
+  int num_faces = 0, num_records = 0, num_projected = 0;
+  int64_t face_points = 0, projected_points = 0, record_points = 0;
+  int dim_begin[4] = {0, 0, 0, 0};  // faces of dimension d: [dim_begin[d], dim_begin[d+1])
+  int kf = 0, km = 0;
+  MortarFace* d_faces = nullptr;
+  MortarRecord* d_records = nullptr;
+  int32_t* d_gather = nullptr;
+  int32_t* d_projected = nullptr;
+};
+// Decode and check a record list ({e, dir, en, o0, o1, o2, size_a, size_b}
+// per record) and upload its tables; SXD_ERR_INVALID_ARGUMENT with *error
+// set on an inconsistent list.
+int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
+                      const int32_t* records, MortarPlan* plan, std::string* error);
+void destroy_mortar_plan(MortarPlan* plan);
+size_t mortar_work_doubles(bool gh, const MortarPlan& plan);
+// Boundary corrections of every mortar added into dt (after the volume
+// terms); work: mortar_work_doubles(gh, plan) doubles.
+int launch_mortar_faces(bool gh, const MortarPlan& plan, const MixedElement* el, const double* u,
+                        const double* params, const double* inv_jac, double* dt, double* work,
+                        hipStream_t stream);
 // Adjoint of the GH RHS on a mixed-p domain, linearised at u: ubar =
 // (d dt / d u)^T gbar over the work list; scratch_du = 150 x total points.
 int mixed_face_adjoint_lds_bytes(int kf, int km);

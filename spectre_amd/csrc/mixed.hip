@@ -1173,6 +1173,33 @@ bool enable_large_lds() {
 
 int mixed_face_lds_bytes(int kf, int km) { return FacePlan(kf, km, 13).total * 8; }
 
+int launch_mixed_volume(bool gh, int count, const int32_t* list, const MixedElement* el,
+                        int np_max, const double* u, const double* params,
+                        const double* inv_jac, double* dt, double* scratch_du, int threads,
+                        hipStream_t stream) {
+  if (count == 0) return SXD_OK;
+  const double* dtab = device_diff_matrices();
+  if (dtab == nullptr) return SXD_ERR_DEVICE;
+  if (threads != 64 && threads != 128 && threads != 256) return SXD_ERR_INVALID_ARGUMENT;
+  if (!enable_large_lds()) return SXD_ERR_DEVICE;
+  if (!gh) {
+    if (5 * np_max * 8 > 160 * 1024) return SXD_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL(sw_mixed_volume_kernel<false>, dim3(count), dim3(threads),
+                       5 * np_max * 8, stream, el, list, u, params, inv_jac, dt, dtab, nullptr,
+                       kSwBcGhostArray, nullptr);
+    return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
+  }
+  if (scratch_du == nullptr) return SXD_ERR_INVALID_ARGUMENT;
+  hipLaunchKernelGGL(ragged_partial_derivatives_kernel,
+                     dim3(count, (50 + kRaggedChunk - 1) / kRaggedChunk), dim3(threads),
+                     (kStagedDiffDoubles + np_max) * 8, stream, el, list, 50, u, inv_jac,
+                     scratch_du, dtab);
+  if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  hipLaunchKernelGGL(gh_mixed_pointwise_kernel, dim3(count), dim3(threads), 0, stream, el, list,
+                     u, scratch_du, params, dt);
+  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
+}
+
 int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement* el,
                      const int32_t* nbr, int np_max, int kf, int km, const double* u,
                      const double* params, const double* inv_jac, double* dt, double* scratch_du,

@@ -1,0 +1,551 @@
+// Non-conforming mortars (CDNA4, FP64): faces whose neighbour has a different
+// orientation of its logical axes, and h-refined (2:1) faces covered by two or
+// four mortars, for the ScalarWave and GH DG right-hand sides on ragged
+// (mixed-p) element sets.
+//
+// The reference's data flow (ComputeTimeDerivative.hpp:690-774,
+// InternalMortarDataImpl.hpp:230-318, ApplyBoundaryCorrections.hpp:797-1043):
+// each element packages its face data (dg_package_data) on its own face mesh,
+// projects the package to every mortar of the face with its own mortar sizes
+// (projection_matrix_parent_to_child, dimension a then b; identity where the
+// size is Full and the extents agree) and sends it to the neighbour oriented
+// into the neighbour's point order (orient_variables_on_slice,
+// Domain/Structure/OrientationMapHelpers.cpp:372-400).  The receiver forms
+// dg_boundary_terms at each mortar point, projects the correction back to
+// its face (projection_matrix_child_to_parent) and lifts it (dg::lift_flux),
+// one mortar after another.
+//
+// Here every element's packages stay in HBM and the receiver gathers the
+// partner's package through a per-record index table (the partner's
+// oriented offsets, inverted on the host), so no data is reordered:
+//   1. mortar_package_kernel: one workgroup per face, the package of every
+//      face point and |n| (SW 13 values, GH 10 pairs x 13 + 4 speeds);
+//   2. mortar_project_kernel: records whose mortar differs from the face,
+//      one output (mortar point, component) per work item;
+//   3. mortar_boundary_terms_kernel: one workgroup per record, the
+//      correction at every mortar point from the own and the gathered
+//      partner package;
+//   4. mortar_lift_kernel: one launch per dimension (both sides; their points
+//      differ), one thread per face point, which projects and lifts the
+//      face's mortars in record order, so every dt point sees its faces in
+//      direction order.
+// Matrix products are sums over the input index, first term a product, then
+// fused multiply-adds, dimension a before b, as in the oracle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "device_common.hpp"
+#include "gh_point.hpp"
+#include "kernels.hpp"
+#include "sw_point.hpp"
+
+namespace sxd {
+namespace {
+
+__device__ __forceinline__ int face_point_index(const MixedElement& m, int d, int side, int a,
+                                                int b) {
+  const int n0 = m.n[0], n1 = m.n[1];
+  const int nd = d == 0 ? n0 : (d == 1 ? n1 : m.n[2]);
+  const int fixed = side == 0 ? 0 : nd - 1;
+  if (d == 0) return fixed + n0 * (a + n1 * b);
+  if (d == 1) return a + n0 * (fixed + n1 * b);
+  return a + n0 * (b + n1 * fixed);
+}
+
+// Face -> mortar matrix of one dimension ([n_mortar][n_face]) or nullptr for
+// the identity.
+__device__ __forceinline__ const double* to_mortar(const double* full, const double* half, int nf,
+                                                   int nm, int size) {
+  if (size == kChildFull)
+    return nf == nm ? nullptr : full + mortar_table_offset(nf, nm);
+  return half + (size == kChildUpperHalf ? 0 : kPackedMortarSize) + mortar_table_offset(nf, nm);
+}
+
+// Mortar -> face matrix of one dimension ([n_face][n_mortar]) or nullptr.
+__device__ __forceinline__ const double* from_mortar(const double* full, const double* half,
+                                                     int nf, int nm, int size) {
+  if (size == kChildFull)
+    return nf == nm ? nullptr : full + mortar_table_offset(nf, nm) + nf * nm;
+  return half + (size == kChildUpperHalf ? 0 : kPackedMortarSize) + mortar_table_offset(nf, nm) +
+         nf * nm;
+}
+
+// Output point (ra, rb) of the two-dimensional product Mb (Ma in): in is
+// [ib][ia] (a fastest), Ma [oa][ia], Mb [ob][ib], nullptr = identity.  The
+// dimension-a sums are complete before the dimension-b sum uses them.
+__device__ __forceinline__ double apply_2d(const double* in, int ia, int ib, const double* Ma,
+                                           const double* Mb, int ra, int rb) {
+  auto pass_a = [&](int lb) {
+    const double* v = in + ia * lb;
+    if (Ma == nullptr) return v[ra];
+    const double* row = Ma + ra * ia;
+    double s = row[0] * v[0];
+    for (int l = 1; l < ia; ++l) s = fma(row[l], v[l], s);
+    return s;
+  };
+  if (Mb == nullptr) return pass_a(rb);
+  const double* row = Mb + rb * ib;
+  double s = row[0] * pass_a(0);
+  for (int l = 1; l < ib; ++l) s = fma(row[l], pass_a(l), s);
+  return s;
+}
+
+template <bool GH>
+__global__ void __launch_bounds__(64)
+mortar_package_kernel(const MixedElement* __restrict__ el, const MortarFace* __restrict__ faces,
+                      const double* __restrict__ u, const double* __restrict__ params,
+                      const double* __restrict__ inv_jac, double* __restrict__ pkf,
+                      double* __restrict__ mag) {
+  constexpr int NPK = GH ? 134 : 13;
+  const MortarFace fr = faces[blockIdx.x];
+  const MixedElement m = el[fr.e];
+  const int d = fr.dir >> 1, side = fr.dir & 1;
+  const int na = d == 0 ? m.n[1] : m.n[0];
+  const int nf = fr.nf;
+  const int np = m.np;
+  double* out = pkf + static_cast<int64_t>(NPK) * fr.pt_off;
+  for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+    const int p = face_point_index(m, d, side, f % na, f / na);
+    double n[3], mg;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      n[i] = inv_jac[9 * m.off + ij3(d, i) * np + p];
+      if (side == 0) n[i] = n[i] * -1.0;
+    }
+    if constexpr (!GH) {
+      flat_normalize(n, &mg);
+      double up[5], pk[13];
+#pragma unroll
+      for (int c = 0; c < 5; ++c) up[c] = u[5 * m.off + c * np + p];
+      sw_package(pk, up, params[m.off + p], n);
+#pragma unroll
+      for (int c = 0; c < 13; ++c) out[c * nf + f] = pk[c];
+    } else {
+      const double* ue = u + 50 * m.off + p;
+      double g[10];
+#pragma unroll
+      for (int c = 0; c < 10; ++c) g[c] = ue[c * np];
+      double lapse, shift[3], inv_sp[6];
+      gh_lapse_shift(&lapse, shift, inv_sp, g);
+      const double* ga = params + 3 * m.off + p;
+      GhFaceSide s;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) s.n[i] = n[i];
+      gh_normalize(s.n, s.nv, &mg, inv_sp);
+      gh_char_speeds(s.lam, ga[np], lapse, shift, s.n);
+      s.gamma2 = ga[2 * np];
+      for (int pr = 0; pr < 10; ++pr) {
+        double phi[3], pk[13];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) phi[i] = ue[(20 + i + 3 * pr) * np];
+        gh_package_pair(pk, g[pr], ue[(10 + pr) * np], phi, s);
+#pragma unroll
+        for (int c = 0; c < 13; ++c) out[(13 * pr + c) * nf + f] = pk[c];
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) out[(130 + c) * nf + f] = s.lam[c];
+    }
+    mag[fr.pt_off + f] = mg;
+  }
+}
+
+// Components per workgroup of the projection kernel.
+constexpr int kProjectChunk = 16;
+
+template <int NPK>
+__global__ void __launch_bounds__(128)
+mortar_project_kernel(const MortarRecord* __restrict__ recs, const int32_t* __restrict__ list,
+                      const MortarFace* __restrict__ faces, const double* __restrict__ pkf,
+                      double* __restrict__ pkm, const double* __restrict__ full,
+                      const double* __restrict__ half) {
+  const MortarRecord r = recs[list[blockIdx.x]];
+  const MortarFace fr = faces[r.face];
+  const double* Ma = to_mortar(full, half, r.na, r.ka, r.size_a);
+  const double* Mb = to_mortar(full, half, r.nb, r.kb, r.size_b);
+  const int nf = r.na * r.nb, km = r.ka * r.kb;
+  const double* in = pkf + static_cast<int64_t>(NPK) * fr.pt_off;
+  double* out = pkm + static_cast<int64_t>(NPK) * r.mortar_off;
+  const int c0 = blockIdx.y * kProjectChunk;
+  const int cn = min(kProjectChunk, NPK - c0);
+  for (int x = threadIdx.x; x < cn * km; x += blockDim.x) {
+    const int c = c0 + x / km, k = x % km;
+    out[c * km + k] = apply_2d(in + c * nf, r.na, r.nb, Ma, Mb, k % r.ka, k / r.ka);
+  }
+}
+
+// Package block of a record on its mortar: its own projected block, or the
+// face block where the mortar equals the face.
+template <int NPK>
+__device__ __forceinline__ const double* mortar_package(const MortarRecord& r,
+                                                        const MortarFace* faces,
+                                                        const double* pkf, const double* pkm) {
+  return r.mortar_off >= 0 ? pkm + static_cast<int64_t>(NPK) * r.mortar_off
+                           : pkf + static_cast<int64_t>(NPK) * faces[r.face].pt_off;
+}
+
+template <bool GH>
+__global__ void __launch_bounds__(64)
+mortar_boundary_terms_kernel(const MortarRecord* __restrict__ recs,
+                             const MortarFace* __restrict__ faces,
+                             const int32_t* __restrict__ gather, const double* __restrict__ pkf,
+                             const double* __restrict__ pkm, double* __restrict__ bcm) {
+  constexpr int NPK = GH ? 134 : 13;
+  constexpr int NC = GH ? 50 : 5;
+  const MortarRecord r = recs[blockIdx.x];
+  const MortarRecord pr_rec = recs[r.partner];
+  const double* own = mortar_package<NPK>(r, faces, pkf, pkm);
+  const double* ext = mortar_package<NPK>(pr_rec, faces, pkf, pkm);
+  const int km = r.ka * r.kb;
+  double* out = bcm + static_cast<int64_t>(NC) * r.rec_off;
+  for (int k = threadIdx.x; k < km; k += blockDim.x) {
+    const int s = gather[r.rec_off + k];
+    if constexpr (!GH) {
+      double in[13], ex[13], bc[5];
+#pragma unroll
+      for (int c = 0; c < 13; ++c) {
+        in[c] = own[c * km + k];
+        ex[c] = ext[c * km + s];
+      }
+      sw_boundary_terms(bc, in, ex);
+#pragma unroll
+      for (int c = 0; c < 5; ++c) out[c * km + k] = bc[c];
+    } else {
+      double lam_in[4], lam_ex[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        lam_in[c] = own[(130 + c) * km + k];
+        lam_ex[c] = ext[(130 + c) * km + s];
+      }
+      for (int pr = 0; pr < 10; ++pr) {
+        double in[13], ex[13], bc[5];
+#pragma unroll
+        for (int c = 0; c < 13; ++c) {
+          in[c] = own[(13 * pr + c) * km + k];
+          ex[c] = ext[(13 * pr + c) * km + s];
+        }
+        gh_boundary_terms_pair(bc, in, lam_in, ex, lam_ex);
+        out[pr * km + k] = bc[0];
+        out[(10 + pr) * km + k] = bc[1];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) out[(20 + 3 * pr + i) * km + k] = bc[2 + i];
+      }
+    }
+  }
+}
+
+template <bool GH>
+__global__ void __launch_bounds__(64)
+mortar_lift_kernel(const MixedElement* __restrict__ el, const MortarFace* __restrict__ faces,
+                   int face_begin, const MortarRecord* __restrict__ recs,
+                   const double* __restrict__ mag, const double* __restrict__ bcm,
+                   double* __restrict__ dt, const double* __restrict__ full,
+                   const double* __restrict__ half) {
+  constexpr int NC = GH ? 50 : 5;
+  const MortarFace fr = faces[face_begin + blockIdx.x];
+  const MixedElement m = el[fr.e];
+  const int d = fr.dir >> 1, side = fr.dir & 1;
+  const int na = d == 0 ? m.n[1] : m.n[0];
+  const int nd = d == 0 ? m.n[0] : (d == 1 ? m.n[1] : m.n[2]);
+  for (int f = threadIdx.x; f < fr.nf; f += blockDim.x) {
+    const int fa = f % na, fb = f / na;
+    const int p = face_point_index(m, d, side, fa, fb);
+    const double fac = lift_factor(nd, mag[fr.pt_off + f]);
+    double* target = dt + NC * m.off + p;
+    for (int q = fr.rec_begin; q < fr.rec_begin + fr.rec_count; ++q) {
+      const MortarRecord r = recs[q];
+      const double* Pa = from_mortar(full, half, r.na, r.ka, r.size_a);
+      const double* Pb = from_mortar(full, half, r.nb, r.kb, r.size_b);
+      const int km = r.ka * r.kb;
+      const double* src = bcm + static_cast<int64_t>(NC) * r.rec_off;
+      for (int c = 0; c < NC; ++c) {
+        const double v = apply_2d(src + c * km, r.ka, r.kb, Pa, Pb, fa, fb);
+        target[c * m.np] = target[c * m.np] + v * fac;
+      }
+    }
+  }
+}
+
+// ---- host: the record list ------------------------------------------------------
+
+int map_direction(const int o[3], int dir) {
+  const int m = o[dir / 2];
+  return (dir % 2 == 1) ? m : (m ^ 1);
+}
+
+bool valid_orientation(const int o[3]) {
+  bool seen[3] = {false, false, false};
+  for (int i = 0; i < 3; ++i) {
+    if (o[i] < 0 || o[i] > 5 || seen[o[i] / 2]) return false;
+    seen[o[i] / 2] = true;
+  }
+  return true;
+}
+
+// oriented_offset_on_slice (OrientationMapHelpers.cpp:219-241 with
+// compute_offset_permutation :42-78): slice point i1 + n1 i2 of this side
+// lies at the returned index of the neighbour's slice.
+std::vector<int32_t> oriented_offsets(int n1, int n2, int sliced_dim, const int o[3]) {
+  const int dims[2] = {sliced_dim == 0 ? 1 : 0, sliced_dim == 2 ? 1 : 2};
+  const bool transposed = o[dims[0]] / 2 > o[dims[1]] / 2;
+  const bool a1 = o[dims[0]] % 2 == 1, a2 = o[dims[1]] % 2 == 1;
+  std::vector<int32_t> out(static_cast<size_t>(n1) * n2);
+  for (int i2 = 0; i2 < n2; ++i2)
+    for (int i1 = 0; i1 < n1; ++i1) {
+      const int j1 = a1 ? i1 : n1 - 1 - i1, j2 = a2 ? i2 : n2 - 1 - i2;
+      out[i1 + n1 * i2] = transposed ? j2 + n2 * j1 : j1 + n1 * j2;
+    }
+  return out;
+}
+
+}  // namespace
+
+int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
+                      const int32_t* records, MortarPlan* plan, std::string* error) {
+  *plan = MortarPlan{};
+  auto bad = [&](const std::string& msg) {
+    *error = msg;
+    return SXD_ERR_INVALID_ARGUMENT;
+  };
+  if (num_records < 1) return bad("need at least one mortar record");
+  struct Host {
+    int e, dir, en, o[3], size[2], caller, partner = -1, na = 0, nb = 0, ka = 0, kb = 0;
+  };
+  std::vector<Host> h(num_records);
+  std::map<std::array<int, 3>, int> index;
+  for (int i = 0; i < num_records; ++i) {
+    const int32_t* q = records + 8 * i;
+    Host& r = h[i];
+    r.e = q[0];
+    r.dir = q[1];
+    r.en = q[2];
+    for (int k = 0; k < 3; ++k) r.o[k] = q[3 + k];
+    r.size[0] = q[6];
+    r.size[1] = q[7];
+    r.caller = i;
+    if (r.e < 0 || r.e >= num_elements || r.en < 0 || r.en >= num_elements)
+      return bad("record " + std::to_string(i) + ": element index out of range");
+    if (r.dir < 0 || r.dir > 5) return bad("record " + std::to_string(i) + ": direction not 0..5");
+    if (!valid_orientation(r.o))
+      return bad("record " + std::to_string(i) + ": orientation is not a signed permutation");
+    for (int k = 0; k < 2; ++k)
+      if (r.size[k] < kChildFull || r.size[k] > kChildLowerHalf)
+        return bad("record " + std::to_string(i) + ": mortar size not 1..3");
+    if (!index.emplace(std::array<int, 3>{r.e, r.dir, r.en}, i).second)
+      return bad("record " + std::to_string(i) + ": duplicate (element, direction, neighbour)");
+    const int32_t* ext = extents + 3 * r.e;
+    const int d = r.dir / 2;
+    r.na = d == 0 ? ext[1] : ext[0];
+    r.nb = d == 2 ? ext[1] : ext[2];
+  }
+  for (int i = 0; i < num_records; ++i) {
+    Host& r = h[i];
+    const int dir_n = map_direction(r.o, r.dir ^ 1);
+    const auto it = index.find({r.en, dir_n, r.e});
+    if (it == index.end())
+      return bad("record " + std::to_string(i) + ": the neighbour has no record back");
+    r.partner = it->second;
+    const Host& p = h[r.partner];
+    for (int k = 0; k < 3; ++k) {
+      const int j = r.o[k] / 2;
+      if (p.o[j] != 2 * k + r.o[k] % 2)
+        return bad("record " + std::to_string(i) + ": orientations of the two sides not inverse");
+    }
+    const int d = r.dir / 2, dn = dir_n / 2;
+    const int dims[2] = {d == 0 ? 1 : 0, d == 2 ? 1 : 2};
+    const int pdims[2] = {dn == 0 ? 1 : 0, dn == 2 ? 1 : 2};
+    int pn[2];
+    for (int k = 0; k < 2; ++k) {
+      const int slot = r.o[dims[k]] / 2 == pdims[0] ? 0 : 1;
+      pn[k] = slot == 0 ? p.na : p.nb;
+      if (r.size[k] != kChildFull && p.size[slot] != kChildFull)
+        return bad("record " + std::to_string(i) + ": a mortar dimension halved on both sides");
+    }
+    r.ka = std::max(r.na, pn[0]);
+    r.kb = std::max(r.nb, pn[1]);
+  }
+  for (int i = 0; i < num_records; ++i) {
+    const Host& r = h[i];
+    const Host& p = h[r.partner];
+    const int d = r.dir / 2;
+    const bool transposed = r.o[d == 0 ? 1 : 0] / 2 > r.o[d == 2 ? 1 : 2] / 2;
+    if ((transposed ? p.kb : p.ka) != r.ka || (transposed ? p.ka : p.kb) != r.kb)
+      return bad("record " + std::to_string(i) + ": the two sides' mortar meshes disagree");
+  }
+  // faces: records grouped by (dimension, element, side), caller order within
+  std::vector<int> order(num_records);
+  for (int i = 0; i < num_records; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    const auto kx = std::make_tuple(h[x].dir / 2, h[x].e, h[x].dir % 2);
+    const auto ky = std::make_tuple(h[y].dir / 2, h[y].e, h[y].dir % 2);
+    return kx < ky;
+  });
+  std::vector<int> pos(num_records);
+  for (int i = 0; i < num_records; ++i) pos[order[i]] = i;
+  std::vector<MortarFace> faces;
+  std::vector<MortarRecord> recs(num_records);
+  std::vector<int32_t> gather, proj;
+  int64_t face_pts = 0, proj_pts = 0, rec_pts = 0;
+  int dim_begin[4] = {0, 0, 0, 0};
+  for (int i = 0; i < num_records; ++i) {
+    const Host& r = h[order[i]];
+    if (faces.empty() || faces.back().e != r.e || faces.back().dir != r.dir) {
+      MortarFace f{};
+      f.e = r.e;
+      f.dir = r.dir;
+      f.nf = r.na * r.nb;
+      f.rec_begin = i;
+      f.rec_count = 0;
+      f.pt_off = face_pts;
+      face_pts += f.nf;
+      faces.push_back(f);
+      for (int d = r.dir / 2 + 1; d <= 3; ++d) dim_begin[d] = static_cast<int>(faces.size());
+    }
+    MortarFace& f = faces.back();
+    ++f.rec_count;
+    MortarRecord& m = recs[i];
+    m.face = static_cast<int32_t>(faces.size() - 1);
+    m.partner = pos[r.partner];
+    m.na = r.na;
+    m.nb = r.nb;
+    m.ka = r.ka;
+    m.kb = r.kb;
+    m.size_a = r.size[0];
+    m.size_b = r.size[1];
+    const bool identity = r.size[0] == kChildFull && r.size[1] == kChildFull && r.na == r.ka &&
+                          r.nb == r.kb;
+    m.mortar_off = identity ? -1 : proj_pts;
+    if (!identity) {
+      proj_pts += static_cast<int64_t>(r.ka) * r.kb;
+      proj.push_back(i);
+    }
+    m.rec_off = rec_pts;
+    rec_pts += static_cast<int64_t>(r.ka) * r.kb;
+    // my mortar point k reads the partner's point offsets[k]: the partner's
+    // oriented offsets map its points to mine, mine map mine to its
+    const std::vector<int32_t> g = oriented_offsets(r.ka, r.kb, r.dir / 2, r.o);
+    gather.insert(gather.end(), g.begin(), g.end());
+  }
+  // every face covered exactly once by its mortars
+  for (const MortarFace& f : faces) {
+    int quarters = 0;
+    for (int q = f.rec_begin; q < f.rec_begin + f.rec_count; ++q)
+      quarters += (recs[q].size_a == kChildFull ? 2 : 1) * (recs[q].size_b == kChildFull ? 2 : 1);
+    if (quarters != 4)
+      return bad("face " + std::to_string(f.dir) + " of element " + std::to_string(f.e) +
+                 ": its mortars do not cover it exactly once");
+  }
+  std::vector<bool> has_face(6 * static_cast<size_t>(num_elements), false);
+  for (const MortarFace& f : faces) has_face[6 * static_cast<size_t>(f.e) + f.dir] = true;
+  for (size_t i = 0; i < has_face.size(); ++i)
+    if (!has_face[i])
+      return bad("element " + std::to_string(i / 6) + " has no mortar in direction " +
+                 std::to_string(i % 6));
+  int kf = 0, km = 0;
+  for (const MortarRecord& m : recs) {
+    kf = std::max(kf, m.na * m.nb);
+    km = std::max(km, m.ka * m.kb);
+  }
+  plan->num_faces = static_cast<int>(faces.size());
+  plan->num_records = num_records;
+  plan->num_projected = static_cast<int>(proj.size());
+  plan->face_points = face_pts;
+  plan->projected_points = proj_pts;
+  plan->record_points = rec_pts;
+  for (int d = 0; d < 4; ++d) plan->dim_begin[d] = dim_begin[d];
+  plan->kf = kf;
+  plan->km = km;
+  int rc = SXD_OK;
+  auto upload = [&](auto** dst, const auto& v) {
+    if (rc != SXD_OK || v.empty()) return;
+    if (hipMalloc(dst, v.size() * sizeof(v[0])) != hipSuccess) {
+      rc = SXD_ERR_OUT_OF_MEMORY;
+      return;
+    }
+    if (hipMemcpy(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice) != hipSuccess)
+      rc = SXD_ERR_DEVICE;
+  };
+  upload(&plan->d_faces, faces);
+  upload(&plan->d_records, recs);
+  upload(&plan->d_gather, gather);
+  upload(&plan->d_projected, proj);
+  if (rc != SXD_OK) {
+    destroy_mortar_plan(plan);
+    *error = "device set-up of the mortar tables failed";
+  }
+  return rc;
+}
+
+void destroy_mortar_plan(MortarPlan* plan) {
+  if (plan->d_faces != nullptr) (void)hipFree(plan->d_faces);
+  if (plan->d_records != nullptr) (void)hipFree(plan->d_records);
+  if (plan->d_gather != nullptr) (void)hipFree(plan->d_gather);
+  if (plan->d_projected != nullptr) (void)hipFree(plan->d_projected);
+  *plan = MortarPlan{};
+}
+
+size_t mortar_work_doubles(bool gh, const MortarPlan& plan) {
+  const size_t npk = gh ? 134 : 13, nc = gh ? 50 : 5;
+  return npk * static_cast<size_t>(plan.face_points + plan.projected_points) +
+         static_cast<size_t>(plan.face_points) + nc * static_cast<size_t>(plan.record_points);
+}
+
+int launch_mortar_faces(bool gh, const MortarPlan& plan, const MixedElement* el, const double* u,
+                        const double* params, const double* inv_jac, double* dt, double* work,
+                        hipStream_t stream) {
+  const double* full = device_mortar_matrices();
+  const double* half = device_half_mortar_matrices();
+  if (full == nullptr || half == nullptr) return SXD_ERR_DEVICE;
+  if (work == nullptr) return SXD_ERR_INVALID_ARGUMENT;
+  const int npk = gh ? 134 : 13;
+  double* pkf = work;
+  double* pkm = pkf + static_cast<size_t>(npk) * plan.face_points;
+  double* mag = pkm + static_cast<size_t>(npk) * plan.projected_points;
+  double* bcm = mag + plan.face_points;
+  if (gh)
+    hipLaunchKernelGGL(mortar_package_kernel<true>, dim3(plan.num_faces), dim3(64), 0, stream, el,
+                       plan.d_faces, u, params, inv_jac, pkf, mag);
+  else
+    hipLaunchKernelGGL(mortar_package_kernel<false>, dim3(plan.num_faces), dim3(64), 0, stream,
+                       el, plan.d_faces, u, params, inv_jac, pkf, mag);
+  if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  if (plan.num_projected > 0) {
+    const dim3 grid(plan.num_projected, (npk + kProjectChunk - 1) / kProjectChunk);
+    if (gh)
+      hipLaunchKernelGGL(mortar_project_kernel<134>, grid, dim3(128), 0, stream, plan.d_records,
+                         plan.d_projected, plan.d_faces, pkf, pkm, full, half);
+    else
+      hipLaunchKernelGGL(mortar_project_kernel<13>, grid, dim3(128), 0, stream, plan.d_records,
+                         plan.d_projected, plan.d_faces, pkf, pkm, full, half);
+    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  }
+  if (gh)
+    hipLaunchKernelGGL(mortar_boundary_terms_kernel<true>, dim3(plan.num_records), dim3(64), 0,
+                       stream, plan.d_records, plan.d_faces, plan.d_gather, pkf, pkm, bcm);
+  else
+    hipLaunchKernelGGL(mortar_boundary_terms_kernel<false>, dim3(plan.num_records), dim3(64), 0,
+                       stream, plan.d_records, plan.d_faces, plan.d_gather, pkf, pkm, bcm);
+  if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  for (int d = 0; d < 3; ++d) {
+    const int count = plan.dim_begin[d + 1] - plan.dim_begin[d];
+    if (count == 0) continue;
+    if (gh)
+      hipLaunchKernelGGL(mortar_lift_kernel<true>, dim3(count), dim3(64), 0, stream, el,
+                         plan.d_faces, plan.dim_begin[d], plan.d_records, mag, bcm, dt, full,
+                         half);
+    else
+      hipLaunchKernelGGL(mortar_lift_kernel<false>, dim3(count), dim3(64), 0, stream, el,
+                         plan.d_faces, plan.dim_begin[d], plan.d_records, mag, bcm, dt, full,
+                         half);
+    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  }
+  return SXD_OK;
+}
+
+}  // namespace sxd

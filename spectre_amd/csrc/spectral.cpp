@@ -19,6 +19,7 @@
 #include "spectral.hpp"
 
 #include <cmath>
+#include <cstddef>
 #include <mutex>
 
 namespace sxd {
@@ -129,10 +130,12 @@ double legendre_p(int k, double x) {
 }
 }  // namespace
 
-void mortar_interpolation_matrix(int nf, int nm, double* out) {
-  double xf[kMaxPoints], wf[kMaxPoints], xm[kMaxPoints], wm[kMaxPoints], b[kMaxPoints];
+namespace {
+// Barycentric interpolation from the nf LGL points to nt targets (row-major
+// [nt][nf]); a target equal within roundoff to a node gets the unit row.
+void interpolation_rows(int nf, const double* target, int nt, double* out) {
+  double xf[kMaxPoints], wf[kMaxPoints], b[kMaxPoints];
   lgl_points_and_weights(nf, xf, wf);
-  lgl_points_and_weights(nm, xm, wm);
   for (int k = 0; k < nf; ++k) b[k] = 1.0;
   for (int j = 1; j < nf; ++j)
     for (int k = 0; k < j; ++k) {
@@ -141,12 +144,13 @@ void mortar_interpolation_matrix(int nf, int nm, double* out) {
     }
   for (int k = 0; k < nf; ++k) b[k] = 1.0 / b[k];
   const double eps = 100.0 * 2.220446049250313e-16;
-  for (int r = 0; r < nm; ++r) {
+  for (int r = 0; r < nt; ++r) {
     double* row = out + r * nf;
     bool match = false;
     for (int j = 0; j < nf; ++j) {
       row[j] = 0.0;
-      if (std::fabs(xm[r] - xf[j]) < (std::fmax(std::fabs(xm[r]), std::fabs(xf[j])) + 1.0) * eps) {
+      if (std::fabs(target[r] - xf[j]) <
+          (std::fmax(std::fabs(target[r]), std::fabs(xf[j])) + 1.0) * eps) {
         row[j] = 1.0;
         match = true;
       }
@@ -154,11 +158,18 @@ void mortar_interpolation_matrix(int nf, int nm, double* out) {
     if (match) continue;
     double sum = 0.0;
     for (int j = 0; j < nf; ++j) {
-      row[j] = b[j] / (xm[r] - xf[j]);
+      row[j] = b[j] / (target[r] - xf[j]);
       sum += row[j];
     }
     for (int j = 0; j < nf; ++j) row[j] /= sum;
   }
+}
+}  // namespace
+
+void mortar_interpolation_matrix(int nf, int nm, double* out) {
+  double xm[kMaxPoints], wm[kMaxPoints];
+  lgl_points_and_weights(nm, xm, wm);
+  interpolation_rows(nf, xm, nm, out);
 }
 
 void mortar_projection_matrix(int nf, int nm, double* out) {
@@ -189,6 +200,77 @@ const std::vector<double>& packed_mortar_matrices() {
         mortar_interpolation_matrix(nf, nm, blk);
         mortar_projection_matrix(nf, nm, blk + nf * nm);
       }
+  });
+  return table;
+}
+
+void half_mortar_interpolation_matrix(int nf, int nm, bool upper, double* out) {
+  double xm[kMaxPoints], wm[kMaxPoints], t[kMaxPoints];
+  lgl_points_and_weights(nm, xm, wm);
+  for (int k = 0; k < nm; ++k) t[k] = upper ? 0.5 * (xm[k] + 1.0) : 0.5 * (xm[k] - 1.0);
+  interpolation_rows(nf, t, nm, out);
+}
+
+namespace {
+// T(large, small) of the upper-half projection (Projection.cpp:150-170).
+double half_spectral_transformation(int large, int small) {
+  double result = 1.0;
+  for (int i = (large - small) / 2; i > 0; --i) {
+    const double num = static_cast<double>(static_cast<size_t>(large + small + 3 - 2 * i) *
+                                           static_cast<size_t>(large + small + 2 - 2 * i) *
+                                           static_cast<size_t>(large - small + 2 - 2 * i) *
+                                           static_cast<size_t>(large - small + 1 - 2 * i));
+    const double den = static_cast<double>(static_cast<size_t>(2 * i) *
+                                           static_cast<size_t>(2 * large + 1 - 2 * i) *
+                                           static_cast<size_t>(large + 2 - 2 * i) *
+                                           static_cast<size_t>(large + 1 - 2 * i));
+    result = 1.0 - result * num / den;
+  }
+  for (int i = 1; i <= large - small; ++i)
+    result *= 1.0 + static_cast<double>(large + small + 1) / static_cast<double>(i);
+  return result / std::pow(2.0, large + 1);
+}
+}  // namespace
+
+void half_mortar_projection_matrix(int nf, int nm, bool upper, double* out) {
+  double xf[kMaxPoints], wf[kMaxPoints], xm[kMaxPoints], wm[kMaxPoints];
+  lgl_points_and_weights(nf, xf, wf);
+  lgl_points_and_weights(nm, xm, wm);
+  double vinv[kMaxPoints * kMaxPoints], temp[kMaxPoints * kMaxPoints];
+  for (int k = 0; k < nm; ++k) {
+    const double gamma = k < nm - 1 ? 2.0 / (2.0 * k + 1.0) : 2.0 / (nm - 1.0);
+    for (int j = 0; j < nm; ++j) vinv[k * nm + j] = wm[j] * legendre_p(k, xm[j]) / gamma;
+  }
+  for (int x = 0; x < nf * nf; ++x) temp[x] = 0.0;
+  for (int j = 0; j < nf; ++j)
+    for (int k = j; k < nf; ++k) {
+      const double t = half_spectral_transformation(k, j);
+      for (int i = 0; i < nf; ++i) temp[i * nf + j] = std::fma(legendre_p(k, xf[i]), t, temp[i * nf + j]);
+    }
+  for (int i = 0; i < nf; ++i)
+    for (int j = 0; j < nm; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < nf; ++k) s = std::fma(temp[i * nf + k], vinv[k * nm + j], s);
+      // the lower half by symmetry: Pr_lower(i, j) = Pr_upper(nf-1-i, nm-1-j)
+      if (upper)
+        out[i * nm + j] = s;
+      else
+        out[(nf - 1 - i) * nm + (nm - 1 - j)] = s;
+    }
+}
+
+const std::vector<double>& packed_half_mortar_matrices() {
+  static std::vector<double> table;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    table.assign(2 * static_cast<size_t>(kPackedMortarSize), 0.0);
+    for (int half = 0; half < 2; ++half)
+      for (int nf = kMinPoints; nf <= kMaxPoints; ++nf)
+        for (int nm = nf; nm <= kMaxPoints; ++nm) {
+          double* blk = table.data() + half * kPackedMortarSize + packed_mortar_offset(nf, nm);
+          half_mortar_interpolation_matrix(nf, nm, half == 0, blk);
+          half_mortar_projection_matrix(nf, nm, half == 0, blk + nf * nm);
+        }
   });
   return table;
 }

@@ -61,6 +61,15 @@ void mortar_interpolation_matrix(int nf, int nm, double* out);
 // Modal-truncation projection from nm LGL points onto nf (row-major [nf][nm]).
 void mortar_projection_matrix(int nf, int nm, double* out);
 const std::vector<double>& packed_mortar_matrices();
+// h-refined mortars (Spectral::ChildSize UpperHalf / LowerHalf,
+// NumericalAlgorithms/Spectral/Projection.cpp): interpolation from the nf
+// face points to the nm mortar points of the upper or lower half of the face
+// ([nm][nf]), and the projection of data on that half back onto the face
+// ([nf][nm]).  Packed as the full-size mortar tables, the upper-half block
+// table first, then the lower-half one (2 kPackedMortarSize doubles).
+void half_mortar_interpolation_matrix(int nf, int nm, bool upper, double* out);
+void half_mortar_projection_matrix(int nf, int nm, bool upper, double* out);
+const std::vector<double>& packed_half_mortar_matrices();
 // Weak flux differentiation matrix W_ij = D_ji (w_j / w_i), row-major
 // (Spectral.cpp:462-485), and all of them for n = 2..20 packed as the D
 // matrices (at packed_d_offset(n)).

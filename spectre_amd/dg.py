@@ -532,7 +532,8 @@ class Domain:
 __all__ = ["SpectreAmdError", "partial_derivatives", "logical_partial_derivatives",
            "partial_derivatives_adjoint",
            "scalar_wave_time_derivative", "gh_time_derivative", "scalar_wave_upwind_penalty",
-           "gh_upwind_penalty", "lift_flux", "rk3_substep", "Domain"]
+           "gh_upwind_penalty", "lift_flux", "rk3_substep", "Domain", "MixedDomain",
+           "MortarDomain"]
 
 
 class MixedDomain(Domain):
@@ -567,6 +568,36 @@ class MixedDomain(Domain):
     def _scalar_shape(self):
         return (self.total_points,)
 
+
+
+class MortarDomain(MixedDomain):
+    """Element set coupled through explicit mortar records
+    (sxd_domain_create_mortars): neighbours with rotated logical frames and
+    h-refined faces, per-element extents; ragged fields as MixedDomain.
+    records [R, 8]: {e, dir, en, o0, o1, o2, size_a, size_b}."""
+
+    def __init__(self, extents, records):
+        ext = np.ascontiguousarray(extents, dtype=np.int32)
+        rec = np.ascontiguousarray(records, dtype=np.int32)
+        if ext.ndim != 2 or ext.shape[1] != 3 or rec.ndim != 2 or rec.shape[1] != 8:
+            raise ValueError("need extents [E, 3] and records [R, 8]")
+        self.extents = ext
+        self.records = rec.copy()
+        self.num_elements = ext.shape[0]
+        self._args = (ext.copy(), rec.copy())
+        handle = ctypes.c_void_p()
+        check(lib().sxd_domain_create_mortars(ctypes.byref(handle), self.num_elements,
+                                              ext.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                              rec.shape[0],
+                                              rec.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        self._handle = handle
+        total = ctypes.c_int64()
+        check(lib().sxd_domain_num_points(self._handle, ctypes.byref(total)))
+        self.total_points = total.value
+
+    @classmethod
+    def from_brick(cls, brick):
+        return cls(brick.extents, brick.records)
 
 
 class Rk3Stepper:

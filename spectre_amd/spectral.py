@@ -79,6 +79,31 @@ def lg_boundary_terms(n):
     return out
 
 
+FULL, UPPER_HALF, LOWER_HALF = 1, 2, 3  # Spectral::ChildSize
+
+
+@functools.lru_cache(maxsize=None)
+def projection_matrix_parent_to_child(n_parent, n_child, child_size):
+    """[n_child, n_parent]: interpolation from the parent's LGL points to the
+    child's on the part child_size of the parent interval."""
+    out = np.empty((n_child, n_parent))
+    check(lib().sxd_projection_matrix_parent_to_child(n_parent, n_child, child_size,
+                                                      out.ctypes.data_as(_dp)))
+    out.setflags(write=False)
+    return out
+
+
+@functools.lru_cache(maxsize=None)
+def projection_matrix_child_to_parent(n_child, n_parent, child_size):
+    """[n_parent, n_child]: L2 projection of data on the child's points
+    (covering child_size of the parent interval) onto the parent's."""
+    out = np.empty((n_parent, n_child))
+    check(lib().sxd_projection_matrix_child_to_parent(n_child, n_parent, child_size,
+                                                      out.ctypes.data_as(_dp)))
+    out.setflags(write=False)
+    return out
+
+
 # Names used by spectre.Spectral
 collocation_points = lgl_points
 quadrature_weights = lgl_weights

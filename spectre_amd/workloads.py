@@ -638,3 +638,206 @@ def gh_adjoint_workload(seed=0, elements=(6, 5, 10), extents=(6, 6, 6), noise=1.
     lapse, shift, _ = _lapse_shift_inverse([u[:, p] for p in range(10)])
     speed = float(np.max(np.abs(lapse) + np.sqrt(np.sum(shift ** 2, axis=0))))
     return brick, u, gammas, inv, cfl_step(brick, 0.25, speed)
+
+
+# ---------------------------------------------------------------------------
+# Non-conforming domains: h-refined (2:1) elements and elements whose logical
+# axes are rotated against their neighbours'.
+ROTATIONS = [(perm, signs) for perm in ((0, 1, 2), (1, 2, 0), (2, 0, 1), (0, 2, 1), (2, 1, 0),
+                                        (1, 0, 2))
+             for signs in ((1, 1, 1), (1, 1, -1), (1, -1, 1), (1, -1, -1), (-1, 1, 1),
+                           (-1, 1, -1), (-1, -1, 1), (-1, -1, -1))
+             if (1 if perm in ((0, 1, 2), (1, 2, 0), (2, 0, 1)) else -1)
+             * signs[0] * signs[1] * signs[2] == 1]
+"""The 24 proper rotations of the logical frame: (perm, signs) with logical
+axis j along physical axis perm[j], in direction signs[j]."""
+
+
+class MortarBrick:
+    """A periodic brick of affine elements with 2:1 h-refinement and rotated
+    logical frames, described by mortar records.
+
+    The domain is Bx x By x Bz base cells; cell c (c = cx + Bx (cy + By cz)) is
+    split into 2, 4 or 8 children along the dimensions refine[c] (children in
+    the order of their bits, the lowest split dimension fastest).  Element e
+    has physical LGL extents phys_extents[e] (per physical axis) and a logical
+    frame rotations[e] = (perm, signs): logical axis j runs along physical axis
+    perm[j] in direction signs[j], so its logical extents are
+    phys_extents[e][perm[j]].  Fields use the ragged Variables layout of the
+    logical extents (xi fastest).
+
+    records [R][8] (int32): {e, dir, en, o0, o1, o2, size_a, size_b} per
+    (element, logical direction, neighbour): o_j is the neighbour's logical
+    direction 2 k + side that e's upper logical axis j maps to, size_a/b the
+    part of e's face the mortar covers along the face's two logical
+    dimensions (1 Full, 2 UpperHalf, 3 LowerHalf; dg::mortar_size,
+    NumericalAlgorithms/DiscontinuousGalerkin/MortarHelpers.cpp:49-74).
+    Within a face the records follow the neighbours' physical position
+    (lowest tangential axis fastest)."""
+
+    def __init__(self, lower, upper, base, refine=None, phys_extents=None, rotations=None):
+        self.lower, self.upper, self.base = tuple(lower), tuple(upper), tuple(base)
+        refine = refine or {}
+        boxes = []  # (lo[3], len[3]) in half-cell units
+        for c in range(int(np.prod(base))):
+            cell = (c % base[0], (c // base[0]) % base[1], c // (base[0] * base[1]))
+            dims = sorted(refine.get(c, ()))
+            for child in range(1 << len(dims)):
+                lo = [2 * cell[d] for d in range(3)]
+                ln = [2, 2, 2]
+                for bit, d in enumerate(dims):
+                    ln[d] = 1
+                    lo[d] += (child >> bit) & 1
+                boxes.append((tuple(lo), tuple(ln)))
+        self.boxes = boxes
+        E = len(boxes)
+        self.phys_extents = (np.full((E, 3), 5, dtype=np.int32) if phys_extents is None
+                             else np.ascontiguousarray(phys_extents, dtype=np.int32).reshape(E, 3))
+        self.rotations = ([((0, 1, 2), (1, 1, 1))] * E if rotations is None else list(rotations))
+        self.extents = np.array([[self.phys_extents[e][self.rotations[e][0][j]] for j in range(3)]
+                                 for e in range(E)], dtype=np.int32)
+        self.points = np.prod(self.extents, axis=1).astype(np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(self.points)]).astype(np.int64)
+        self.records = self._records()
+
+    @property
+    def num_elements(self):
+        return len(self.boxes)
+
+    @property
+    def total_points(self):
+        return int(self.offsets[-1])
+
+    def _unit(self, d):
+        return (self.upper[d] - self.lower[d]) / (2 * self.base[d])
+
+    def _records(self):
+        W = [2 * b for b in self.base]
+        recs = []
+        for e, (lo, ln) in enumerate(self.boxes):
+            perm, sg = self.rotations[e]
+            for dir_ in range(6):
+                j, side_l = divmod(dir_, 2)
+                P = perm[j]
+                side_p = side_l if sg[j] > 0 else 1 - side_l
+                plane = (lo[P] + ln[P]) % W[P] if side_p else lo[P] % W[P]
+                tang = [d for d in range(3) if d != P]
+                found = []
+                for en, (nlo, nln) in enumerate(self.boxes):
+                    nplane = nlo[P] % W[P] if side_p else (nlo[P] + nln[P]) % W[P]
+                    if nplane != plane:
+                        continue
+                    if all(max(lo[t], nlo[t]) < min(lo[t] + ln[t], nlo[t] + nln[t]) for t in tang):
+                        found.append(en)
+                if not found:
+                    raise ValueError(f"no neighbour of element {e} in direction {dir_}")
+                found.sort(key=lambda en: tuple(self.boxes[en][0][t] for t in reversed(tang)))
+                for en in found:
+                    nperm, nsg = self.rotations[en]
+                    o = []
+                    for jj in range(3):
+                        k = nperm.index(perm[jj])
+                        o.append(2 * k + (1 if nsg[k] == sg[jj] else 0))
+                    fdims = [d for d in range(3) if d != j]
+                    sizes = []
+                    for jj in fdims:
+                        Q = perm[jj]
+                        nlo, nln = self.boxes[en][0][Q], self.boxes[en][1][Q]
+                        if nln >= ln[Q]:
+                            sizes.append(1)
+                            continue
+                        upper_phys = nlo > lo[Q]
+                        sizes.append(2 if upper_phys == (sg[jj] > 0) else 3)
+                    recs.append([e, dir_, en] + o + sizes)
+        return np.array(recs, dtype=np.int32)
+
+    def element_coordinates(self, e):
+        """Inertial coordinates [3][Np_e] of element e, in its logical point order."""
+        lo, ln = self.boxes[e]
+        perm, sg = self.rotations[e]
+        n = self.extents[e]
+        xs = [spectral.lgl_points(int(v)) for v in n]
+        k, j, i = np.meshgrid(np.arange(n[2]), np.arange(n[1]), np.arange(n[0]), indexing="ij")
+        logical = [xs[0][i.ravel()], xs[1][j.ravel()], xs[2][k.ravel()]]
+        out = np.empty((3, int(self.points[e])))
+        for jj in range(3):
+            P = perm[jj]
+            h = self._unit(P)
+            out[P] = self.lower[P] + h * lo[P] + 0.5 * h * ln[P] * (1.0 + sg[jj] * logical[jj])
+        return out
+
+    def coordinates(self):
+        return self.ragged([self.element_coordinates(e) for e in range(self.num_elements)])
+
+    def inverse_jacobian(self):
+        """Ragged [9][Np_e]: d xi^j / d x^i = signs[j] 2 / width along perm[j]."""
+        out = []
+        for e in range(self.num_elements):
+            _, ln = self.boxes[e]
+            perm, sg = self.rotations[e]
+            inv = np.zeros((9, int(self.points[e])))
+            for jj in range(3):
+                P = perm[jj]
+                inv[jj + 3 * P] = sg[jj] * 2.0 / (self._unit(P) * ln[P])
+            out.append(inv)
+        return self.ragged(out)
+
+    def aligned_point_order(self, e):
+        """For each point of element e (logical order), its index in the
+        physical-axis order of the same element unrotated (x fastest)."""
+        perm, sg = self.rotations[e]
+        n = self.extents[e]
+        pn = self.phys_extents[e]
+        k, j, i = np.meshgrid(np.arange(n[2]), np.arange(n[1]), np.arange(n[0]), indexing="ij")
+        idx = [i.ravel(), j.ravel(), k.ravel()]
+        phys = [None, None, None]
+        for jj in range(3):
+            P = perm[jj]
+            phys[P] = idx[jj] if sg[jj] > 0 else n[jj] - 1 - idx[jj]
+        return phys[0] + pn[0] * (phys[1] + pn[1] * phys[2])
+
+    def ragged(self, per_element):
+        return np.ascontiguousarray(np.concatenate([np.asarray(a).ravel() for a in per_element]))
+
+    def split(self, flat, nc):
+        return [flat[nc * self.offsets[e]:nc * self.offsets[e + 1]].reshape(nc, -1)
+                for e in range(self.num_elements)]
+
+
+def scalar_wave_mortar_workload(seed=0, base=(2, 2, 2), refine=None, lowest=4, highest=6,
+                                rotate=True, k=(1.0, -2.0, 1.0)):
+    """ScalarWave plane wave psi = sin(k.x) on the periodic MortarBrick
+    [0, 2 pi]^3 (refine: cell -> split dimensions; extents from
+    {lowest..highest}; random proper rotations when rotate), gamma2 = 1.
+    Returns (brick, u, gamma2, inv_jac) in the ragged layout."""
+    rng = np.random.default_rng(seed)
+    E = sum(1 << len((refine or {}).get(c, ())) for c in range(int(np.prod(base))))
+    ext = rng.integers(lowest, highest + 1, size=(E, 3))
+    rots = [ROTATIONS[i] for i in rng.integers(0, len(ROTATIONS), size=E)] if rotate else None
+    brick = MortarBrick((0.0, 0.0, 0.0), (2 * np.pi,) * 3, base, refine, ext, rots)
+    kv = np.asarray(k, dtype=np.float64)
+    w = np.linalg.norm(kv)
+    per = []
+    for e in range(brick.num_elements):
+        x = brick.element_coordinates(e)
+        ph = kv @ x
+        per.append(np.stack([np.sin(ph), w * np.cos(ph)] + [kv[i] * np.cos(ph) for i in range(3)]))
+    u = brick.ragged(per)
+    return brick, u, np.ones(brick.total_points), brick.inverse_jacobian()
+
+
+def gh_mortar_workload(seed=0, base=(2, 2, 2), refine=None, lowest=4, highest=6, rotate=True,
+                       noise=1.0e-4, lower=(3.0, -4.0, -4.0), upper=(11.0, 4.0, 4.0)):
+    """Kerr-Schild GH state (with relative noise) on a periodic MortarBrick;
+    gammas (1, -1, 1).  Returns (brick, u, gammas, inv_jac), ragged."""
+    rng = np.random.default_rng(seed)
+    E = sum(1 << len((refine or {}).get(c, ())) for c in range(int(np.prod(base))))
+    ext = rng.integers(lowest, highest + 1, size=(E, 3))
+    rots = [ROTATIONS[i] for i in rng.integers(0, len(ROTATIONS), size=E)] if rotate else None
+    brick = MortarBrick(lower, upper, base, refine, ext, rots)
+    u = brick.ragged([gh_kerr_schild(brick.element_coordinates(e)[None])[0]
+                      for e in range(brick.num_elements)])
+    if noise:
+        u *= 1.0 + noise * rng.standard_normal(u.shape)
+    gammas = brick.ragged([np.stack([np.ones(n), -np.ones(n), np.ones(n)]) for n in brick.points])
+    return brick, u, gammas, brick.inverse_jacobian()

@@ -537,3 +537,42 @@ def gh_evolve_vjp(extents, neighbors, u0, gammas, inv_jac, dt, steps, gbar_final
     _check(lib().sxo_gh_evolve_vjp(u0.shape[0], ep, nbp, _d(u0), _d(gammas), _d(inv_jac),
                                    ctypes.c_double(dt), int(steps), _d(gbar_final), _d(out)))
     return out
+
+
+# Non-conforming mortars: orientations and h-refinement.
+def half_interpolation_matrix(n_face, n_mortar, upper):
+    out = np.empty((n_mortar, n_face))
+    _check(lib().sxo_half_interpolation_matrix(n_face, n_mortar, int(bool(upper)), _d(out)))
+    return out
+
+
+def half_projection_matrix(n_face, n_mortar, upper):
+    out = np.empty((n_face, n_mortar))
+    _check(lib().sxo_half_projection_matrix(n_face, n_mortar, int(bool(upper)), _d(out)))
+    return out
+
+
+def oriented_offset_on_slice(n1, n2, sliced_dim, mapped):
+    out = np.empty(n1 * n2, dtype=np.int32)
+    m, mp = _i(mapped)
+    _check(lib().sxo_oriented_offset_on_slice(n1, n2, sliced_dim, mp,
+                                              out.ctypes.data_as(_ip)))
+    return out
+
+
+def _rhs_mortars(fn, extents, records, u, params, inv_jac):
+    u, params, inv_jac = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, params, inv_jac))
+    dt = np.empty_like(u)
+    ext, ep = _i(np.asarray(extents).ravel())
+    rec, rp = _i(np.asarray(records).ravel())
+    _check(fn(len(ext) // 3, ep, len(rec) // 8, rp, _d(u), _d(params), _d(inv_jac), _d(dt)))
+    return dt
+
+
+def sw_rhs_mortars(extents, records, u, gamma2, inv_jac):
+    """ScalarWave RHS on a mortar domain (ragged layout; records [R][8])."""
+    return _rhs_mortars(lib().sxo_sw_rhs_mortars, extents, records, u, gamma2, inv_jac)
+
+
+def gh_rhs_mortars(extents, records, u, gammas, inv_jac):
+    return _rhs_mortars(lib().sxo_gh_rhs_mortars, extents, records, u, gammas, inv_jac)
