@@ -303,6 +303,57 @@ int sxd_domain_create_mixed(sxd_domain** domain, int num_elements,
  * (with the reason in sxd_last_error) for an inconsistent record list. */
 int sxd_domain_create_mortars(sxd_domain** domain, int num_elements, const int32_t* extents,
                               int num_records, const int32_t* records);
+/* One rank's part of a mortar domain, for a run split over devices or nodes:
+ * the elements e with partition[e] == rank of the global element set and
+ * record list above (the same arrays on every rank), in global order, with
+ * the ragged layout of those elements only.  A mortar whose two sides live
+ * on different ranks is coupled by boundary messages -- the
+ * BoundaryData / BoundaryMessage exchange of ComputeTimeDerivative.hpp:
+ * 690-774 (Evolution/DiscontinuousGalerkin/BoundaryData.hpp,
+ * Messages/BoundaryMessage.hpp) -- so its RHS runs as sxd_rhs_begin, the
+ * exchange, sxd_boundary_messages_unpack per sending rank, sxd_rhs_end;
+ * sxd_scalar_wave_rhs / sxd_gh_rhs return SXD_ERR_UNSUPPORTED while such
+ * mortars exist. */
+int sxd_domain_create_mortars_partition(sxd_domain** domain, int num_elements,
+                                        const int32_t* extents, int num_records,
+                                        const int32_t* records, const int32_t* partition,
+                                        int rank);
+/* The send buffer of a partitioned mortar domain for `system`: one group per
+ * receiving rank (ascending), each a directory (int64 message count, int64
+ * byte offset of every message within the group; padded to 64 bytes) and
+ * its messages.  A message is a 64-byte header of 16 int32 -- magic
+ * 0x53584442, system, sender element, receiver element, receiver direction,
+ * sender direction, sender extents (3), mortar extents in the receiver's
+ * frame (2), packaged components (13 ScalarWave, 134 GH), data doubles,
+ * reserved, time step id (low, high) -- and the sender's packaged mortar
+ * data (dg_package_data projected to the mortar) in the receiver's point
+ * order (orient_variables_on_slice), padded to 64 bytes.  Fills
+ * *num_peers, the total size, and (when non-null, num_peers entries each)
+ * the ranks, byte offsets and sizes of the groups: the ranges an RCCL
+ * ncclSend to each rank would carry. */
+int sxd_boundary_messages_layout(const sxd_domain* domain, int system, int* num_peers,
+                                 int32_t* ranks, int64_t* offsets, int64_t* bytes,
+                                 int64_t* total_bytes);
+/* Number of messages this rank receives from `rank` per RHS. */
+int sxd_boundary_messages_expected(const sxd_domain* domain, int rank, int* count);
+/* First half of a partitioned RHS: volume terms into dt, the packages of
+ * every mortar, and the outgoing messages into send (device, the layout's
+ * total size; stamped with time_step_id). */
+int sxd_rhs_begin(const sxd_domain* domain, int system, const double* u, const double* params,
+                  const double* inv_jac, double* dt, void* send, int64_t time_step_id,
+                  void* stream);
+/* Take the group of messages received from sender_rank (device, bytes
+ * long).  Every header is checked (magic, system, time step id, that the
+ * mortar exists on this rank with the same mesh, no mortar twice); a failed
+ * check returns SXD_ERR_INVALID_ARGUMENT naming the message and the reason.
+ * Synchronizes the stream. */
+int sxd_boundary_messages_unpack(const sxd_domain* domain, int system, int sender_rank,
+                                 const void* recv, int64_t bytes, void* stream);
+/* Second half: the corrections of every mortar into dt, once every remote
+ * mortar's message has arrived (SXD_ERR_INVALID_ARGUMENT otherwise).  Same
+ * stream as sxd_rhs_begin. */
+int sxd_rhs_end(const sxd_domain* domain, int system, double* dt, void* stream);
+
 /* Total grid points of the domain (sum of Np_e). */
 int sxd_domain_num_points(const sxd_domain* domain, int64_t* points);
 

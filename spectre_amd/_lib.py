@@ -63,6 +63,16 @@ _SIGNATURES = {
     "sxd_domain_create_mixed": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _ip, _ip]),
     "sxd_domain_create_mortars": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _ip,
                                                  ctypes.c_int, _ip]),
+    "sxd_domain_create_mortars_partition": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _ip,
+                                                           ctypes.c_int, _ip, _ip, ctypes.c_int]),
+    "sxd_boundary_messages_layout": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip,
+                                                    ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                                                    ctypes.POINTER(_i64)]),
+    "sxd_boundary_messages_expected": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+    "sxd_rhs_begin": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "sxd_boundary_messages_unpack": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _i64,
+                                                    _vp]),
+    "sxd_rhs_end": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "sxd_projection_matrix_parent_to_child": (ctypes.c_int, [ctypes.c_int, ctypes.c_int,
                                                              ctypes.c_int, _dp]),
     "sxd_projection_matrix_child_to_parent": (ctypes.c_int, [ctypes.c_int, ctypes.c_int,

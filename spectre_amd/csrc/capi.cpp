@@ -29,6 +29,15 @@ struct Scratch {
   size_t du_doubles = 0, temps_doubles = 0;
   bool fixed = false;
   size_t fixed_doubles = 0;  // capacity of a fixed block
+  // A partitioned mortar RHS between its begin and end calls: the system,
+  // the time step id of the exchange, and which receive slots are filled;
+  // the message status array of the unpack calls.
+  bool begun = false;
+  int phase_system = -1;
+  int64_t step = 0;
+  std::vector<uint8_t> filled;
+  int32_t* d_status = nullptr;
+  size_t status_n = 0;
 
   // Make room for du_n + temps_n doubles.  Returns SXD_OK,
   // SXD_ERR_OUT_OF_MEMORY, or SXD_ERR_UNSUPPORTED for a fixed block that is
@@ -62,6 +71,9 @@ struct Scratch {
       if (du != nullptr) (void)hipFree(du);
       if (temps != nullptr) (void)hipFree(temps);
     }
+    if (d_status != nullptr) (void)hipFree(d_status);
+    d_status = nullptr;
+    status_n = 0;
     du = temps = nullptr;
     du_doubles = temps_doubles = 0;
   }
@@ -652,8 +664,10 @@ int sxd_domain_create_mixed(sxd_domain** domain, int num_elements, const int32_t
   return ok();
 }
 
-int sxd_domain_create_mortars(sxd_domain** domain, int num_elements, const int32_t* extents,
-                              int num_records, const int32_t* records) {
+namespace {
+int create_mortar_domain(sxd_domain** domain, int num_elements, const int32_t* extents,
+                         int num_records, const int32_t* records, const int32_t* partition,
+                         int rank) {
   if (domain == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null domain output");
   *domain = nullptr;
   if (num_elements < 1) return fail(SXD_ERR_INVALID_ARGUMENT, "need at least one element");
@@ -662,15 +676,23 @@ int sxd_domain_create_mortars(sxd_domain** domain, int num_elements, const int32
   for (int e = 0; e < num_elements; ++e)
     if (!valid_extents(extents + 3 * e))
       return fail(SXD_ERR_INVALID_ARGUMENT, "extents of element %d must be 2..20", e);
-  // the ragged element set: a neighbour table of self-references, which no
-  // mortar-domain path reads
-  std::vector<int32_t> self(6 * static_cast<size_t>(num_elements));
+  // the ragged set of this rank's elements (all of them without a
+  // partition), with a neighbour table of self-references that no mortar
+  // path reads
+  std::vector<int32_t> local_ext;
+  for (int e = 0; e < num_elements; ++e)
+    if (partition == nullptr || partition[e] == rank)
+      local_ext.insert(local_ext.end(), extents + 3 * e, extents + 3 * e + 3);
+  const int owned = static_cast<int>(local_ext.size() / 3);
+  if (owned == 0) return fail(SXD_ERR_INVALID_ARGUMENT, "rank %d owns no element", rank);
+  std::vector<int32_t> self(6 * static_cast<size_t>(owned));
   for (size_t i = 0; i < self.size(); ++i) self[i] = static_cast<int32_t>(i / 6);
   sxd_domain* d = nullptr;
-  if (int rc = sxd_domain_create_mixed(&d, num_elements, extents, self.data()); rc != SXD_OK)
+  if (int rc = sxd_domain_create_mixed(&d, owned, local_ext.data(), self.data()); rc != SXD_OK)
     return rc;
   std::string error;
-  if (int rc = build_mortar_plan(num_elements, extents, num_records, records, &d->plan, &error);
+  if (int rc = build_mortar_plan(num_elements, extents, num_records, records, partition, rank,
+                                 &d->plan, &error);
       rc != SXD_OK) {
     sxd_domain_destroy(d);
     return fail(rc, "%s", error.c_str());
@@ -682,6 +704,24 @@ int sxd_domain_create_mortars(sxd_domain** domain, int num_elements, const int32
   }
   *domain = d;
   return ok();
+}
+}  // namespace
+
+int sxd_domain_create_mortars(sxd_domain** domain, int num_elements, const int32_t* extents,
+                              int num_records, const int32_t* records) {
+  return create_mortar_domain(domain, num_elements, extents, num_records, records, nullptr, 0);
+}
+
+int sxd_domain_create_mortars_partition(sxd_domain** domain, int num_elements,
+                                        const int32_t* extents, int num_records,
+                                        const int32_t* records, const int32_t* partition,
+                                        int rank) {
+  if (partition == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null partition");
+  for (int e = 0; e < num_elements; ++e)
+    if (partition[e] < 0)
+      return fail(SXD_ERR_INVALID_ARGUMENT, "element %d has a negative rank", e);
+  return create_mortar_domain(domain, num_elements, extents, num_records, records, partition,
+                              rank);
 }
 
 int sxd_domain_num_points(const sxd_domain* domain, int64_t* points) {
@@ -1069,6 +1109,145 @@ int sxd_gh_rhs_dirichlet(const sxd_domain* domain, const double* u, const double
                                      "select the fused or composite kernel");
   return finish(run_gh(d, sc, kernel, u, gammas, inv_jac, dt, static_cast<hipStream_t>(stream),
                        ghost),
+                __func__);
+}
+
+// ---- partitioned mortar RHS around a boundary-message exchange -------------
+
+int sxd_boundary_messages_layout(const sxd_domain* domain, int system, int* num_peers,
+                                 int32_t* ranks, int64_t* offsets, int64_t* bytes,
+                                 int64_t* total_bytes) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (!domain->mortars) return fail(SXD_ERR_UNSUPPORTED, "not a mortar domain");
+  if (system != SXD_SYSTEM_SCALAR_WAVE && system != SXD_SYSTEM_GH)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "unknown system %d", system);
+  if (num_peers == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null output");
+  const auto& peers = domain->plan.peers;
+  *num_peers = static_cast<int>(peers.size());
+  if (total_bytes != nullptr) *total_bytes = domain->plan.message_bytes[system];
+  for (size_t i = 0; i < peers.size(); ++i) {
+    if (ranks != nullptr) ranks[i] = peers[i].rank;
+    if (offsets != nullptr) offsets[i] = peers[i].offset[system];
+    if (bytes != nullptr) bytes[i] = peers[i].bytes[system];
+  }
+  return ok();
+}
+
+int sxd_boundary_messages_expected(const sxd_domain* domain, int rank, int* count) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (!domain->mortars) return fail(SXD_ERR_UNSUPPORTED, "not a mortar domain");
+  if (count == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null output");
+  const auto it = domain->plan.expected.find(rank);
+  *count = it == domain->plan.expected.end() ? 0 : it->second;
+  return ok();
+}
+
+int sxd_rhs_begin(const sxd_domain* domain, int system, const double* u, const double* params,
+                  const double* inv_jac, double* dt, void* send, int64_t time_step_id,
+                  void* stream) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (!domain->mortars) return fail(SXD_ERR_UNSUPPORTED, "%s: not a mortar domain", __func__);
+  if (system != SXD_SYSTEM_SCALAR_WAVE && system != SXD_SYSTEM_GH)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "unknown system %d", system);
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(params);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  SXD_REQUIRE_DEVICE(dt);
+  if (domain->plan.message_bytes[system] > 0) SXD_REQUIRE_DEVICE(send);
+  sxd_domain* d = const_cast<sxd_domain*>(domain);
+  const bool gh = system == SXD_SYSTEM_GH;
+  if (int rc = check_no_overlap(d, gh ? 50 : 5, u, dt, __func__); rc != SXD_OK) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Scratch& sc = stream_scratch(d, s);
+  const size_t du = gh ? 150 * static_cast<size_t>(d->total_points) : 0;
+  int rc = reserve_scratch(sc, du, mortar_work_doubles(gh, d->plan));
+  if (rc == SXD_OK)
+    rc = launch_mixed_volume(gh, d->num_elements, d->d_list, d->d_el, d->np_max, u, params,
+                             inv_jac, dt, sc.du, d->threads[system], s);
+  if (rc == SXD_OK)
+    rc = launch_mortar_packages(gh, d->plan, d->d_el, u, params, inv_jac, sc.temps, s);
+  if (rc == SXD_OK) rc = launch_mortar_pack(gh, d->plan, sc.temps, send, time_step_id, s);
+  if (rc != SXD_OK) return finish(rc, __func__);
+  sc.begun = true;
+  sc.phase_system = system;
+  sc.step = time_step_id;
+  sc.filled.assign(d->plan.num_remote, 0);
+  return ok();
+}
+
+int sxd_boundary_messages_unpack(const sxd_domain* domain, int system, int sender_rank,
+                                 const void* recv, int64_t bytes, void* stream) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (!domain->mortars) return fail(SXD_ERR_UNSUPPORTED, "%s: not a mortar domain", __func__);
+  sxd_domain* d = const_cast<sxd_domain*>(domain);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Scratch& sc = stream_scratch(d, s);
+  if (!sc.begun || sc.phase_system != system)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "%s: no sxd_rhs_begin of this system on this stream",
+                __func__);
+  const auto it = d->plan.expected.find(sender_rank);
+  const int count = it == d->plan.expected.end() ? 0 : it->second;
+  if (count == 0)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "%s: no message is expected from rank %d", __func__,
+                sender_rank);
+  SXD_REQUIRE_DEVICE(recv);
+  if (bytes < directory_bytes(count))
+    return fail(SXD_ERR_INVALID_ARGUMENT, "%s: %lld bytes cannot hold %d messages", __func__,
+                static_cast<long long>(bytes), count);
+  if (sc.status_n < 2 * static_cast<size_t>(count)) {
+    if (sc.d_status != nullptr) (void)hipFree(sc.d_status);
+    sc.d_status = nullptr;
+    sc.status_n = 0;
+    if (hipMalloc(&sc.d_status, 2 * static_cast<size_t>(count) * sizeof(int32_t)) != hipSuccess)
+      return fail(SXD_ERR_OUT_OF_MEMORY, "%s: status array", __func__);
+    sc.status_n = 2 * static_cast<size_t>(count);
+  }
+  int rc = launch_mortar_unpack(system == SXD_SYSTEM_GH, d->plan, sc.temps, recv, bytes, count,
+                                sc.step, sc.d_status, s);
+  if (rc != SXD_OK) return finish(rc, __func__);
+  std::vector<int32_t> status(2 * static_cast<size_t>(count));
+  if (hipStreamSynchronize(s) != hipSuccess ||
+      hipMemcpy(status.data(), sc.d_status, status.size() * sizeof(int32_t),
+                hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(SXD_ERR_DEVICE, "%s: reading the message status failed", __func__);
+  static const char* const reasons[] = {"",
+                                        "the group's message count is not the expected one",
+                                        "a message extends past the buffer",
+                                        "a header has no message magic",
+                                        "a message is for another system",
+                                        "a message carries another time step id",
+                                        "a message is for a mortar this rank does not have",
+                                        "a message's mortar mesh differs from the receiver's"};
+  for (int i = 0; i < count; ++i) {
+    const int code = status[2 * i], slot = status[2 * i + 1];
+    if (code != 0)
+      return fail(SXD_ERR_INVALID_ARGUMENT, "%s: message %d from rank %d: %s", __func__, i,
+                  sender_rank, reasons[code]);
+    if (sc.filled[slot] != 0)
+      return fail(SXD_ERR_INVALID_ARGUMENT, "%s: message %d from rank %d: mortar already received",
+                  __func__, i, sender_rank);
+    sc.filled[slot] = 1;
+  }
+  return ok();
+}
+
+int sxd_rhs_end(const sxd_domain* domain, int system, double* dt, void* stream) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (!domain->mortars) return fail(SXD_ERR_UNSUPPORTED, "%s: not a mortar domain", __func__);
+  SXD_REQUIRE_DEVICE(dt);
+  sxd_domain* d = const_cast<sxd_domain*>(domain);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Scratch& sc = stream_scratch(d, s);
+  if (!sc.begun || sc.phase_system != system)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "%s: no sxd_rhs_begin of this system on this stream",
+                __func__);
+  for (size_t i = 0; i < sc.filled.size(); ++i)
+    if (sc.filled[i] == 0)
+      return fail(SXD_ERR_INVALID_ARGUMENT, "%s: %zu remote mortars still lack their message",
+                  __func__, static_cast<size_t>(std::count(sc.filled.begin(), sc.filled.end(), 0)));
+  sc.begun = false;
+  return finish(launch_mortar_corrections(system == SXD_SYSTEM_GH, d->plan, d->d_el, dt, sc.temps,
+                                          s),
                 __func__);
 }
 

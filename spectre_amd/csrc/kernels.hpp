@@ -5,7 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
+#include <vector>
 
 #include "../../include/spectre_amd.h"
 
@@ -131,25 +133,75 @@ struct MortarFace {
 // mortar_off: offset of its own projected package among the projected
 // mortars' points (-1: the mortar is the face); rec_off: offset of its
 // corrections and gather indices among all records' mortar points.
+// partner < 0: the partner is in another partition (rank); its package
+// arrives as a boundary message into the receive slot -1 - partner, at
+// recv_off among the received mortar points.
 struct MortarRecord {
   int32_t face, partner, na, nb, ka, kb, size_a, size_b;
-  int64_t mortar_off, rec_off;
+  int64_t mortar_off, rec_off, recv_off;
+};
+// Boundary messages (the BoundaryMessage of
+// Evolution/DiscontinuousGalerkin/Messages/BoundaryMessage.hpp for DG mortar
+// data).  The send buffer holds one group per receiving rank; a group is a
+// directory (int64 count, int64 byte offset of each message within the
+// group, padded to 64 bytes) followed by its messages; a message is a
+// 64-byte header of 16 int32 and the packaged mortar data (doubles, padded to
+// 64 bytes), oriented into the receiver's mortar point order.  Header:
+//   0 magic, 1 system, 2 sender element, 3 receiver element,
+//   4 receiver direction, 5 sender direction, 6-8 sender volume extents,
+//   9-10 mortar extents in the receiver's frame, 11 packaged components,
+//   12 data doubles, 13 reserved, 14-15 time step id (low, high 32 bits).
+constexpr int32_t kMessageMagic = 0x53584442;  // "SXDB"
+constexpr int64_t kMessageHeaderBytes = 64;
+__host__ __device__ inline int64_t message_data_bytes(int64_t doubles) {
+  return (8 * doubles + 63) / 64 * 64;
+}
+__host__ __device__ inline int64_t message_data_bytes(bool gh, int64_t km) {
+  return message_data_bytes((gh ? 134 : 13) * km);
+}
+__host__ __device__ inline int64_t directory_bytes(int64_t count) {
+  return (8 * (count + 1) + 63) / 64 * 64;
+}
+// status codes of a received message
+constexpr int kMessageWrongCount = 1, kMessageTruncated = 2, kMessageBadMagic = 3,
+              kMessageWrongSystem = 4, kMessageWrongStep = 5, kMessageUnknownMortar = 6,
+              kMessageWrongMesh = 7;
+struct MortarMessage {
+  int32_t record, km, peer, index, group_count, pad;
+  int64_t group_offset[2], offset[2];  // per system: SW, GH
+  int32_t header[16];
+};
+struct MortarRecvKey {
+  int32_t recv_e, recv_dir, send_e, slot, ka, kb, send_rank, pad;
+  int64_t recv_off;
+};
+struct MortarPeer {
+  int rank = 0, count = 0;
+  int64_t offset[2] = {0, 0}, bytes[2] = {0, 0};
 };
 struct MortarPlan {
-  int num_faces = 0, num_records = 0, num_projected = 0;
-  int64_t face_points = 0, projected_points = 0, record_points = 0;
+  int num_faces = 0, num_records = 0, num_projected = 0, num_remote = 0;
+  int64_t face_points = 0, projected_points = 0, record_points = 0, remote_points = 0;
+  int64_t message_bytes[2] = {0, 0};  // the whole send buffer, per system
+  std::vector<MortarPeer> peers;      // outgoing groups, ascending rank
+  std::map<int, int> expected;        // messages expected from each rank
   int dim_begin[4] = {0, 0, 0, 0};  // faces of dimension d: [dim_begin[d], dim_begin[d+1])
   int kf = 0, km = 0;
   MortarFace* d_faces = nullptr;
   MortarRecord* d_records = nullptr;
   int32_t* d_gather = nullptr;
   int32_t* d_projected = nullptr;
+  MortarMessage* d_messages = nullptr;
+  MortarRecvKey* d_keys = nullptr;
 };
 // Decode and check a record list ({e, dir, en, o0, o1, o2, size_a, size_b}
 // per record) and upload its tables; SXD_ERR_INVALID_ARGUMENT with *error
 // set on an inconsistent list.
+// partition: the owning rank of every element (nullptr: all elements, rank
+// ignored); the plan covers the elements of `rank`.
 int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
-                      const int32_t* records, MortarPlan* plan, std::string* error);
+                      const int32_t* records, const int32_t* partition, int rank,
+                      MortarPlan* plan, std::string* error);
 void destroy_mortar_plan(MortarPlan* plan);
 size_t mortar_work_doubles(bool gh, const MortarPlan& plan);
 // Boundary corrections of every mortar added into dt (after the volume
@@ -157,6 +209,22 @@ size_t mortar_work_doubles(bool gh, const MortarPlan& plan);
 int launch_mortar_faces(bool gh, const MortarPlan& plan, const MixedElement* el, const double* u,
                         const double* params, const double* inv_jac, double* dt, double* work,
                         hipStream_t stream);
+// The two halves of launch_mortar_faces around a boundary-message exchange:
+// the packages of every mortar, then (with the remote packages received) the
+// corrections.
+int launch_mortar_packages(bool gh, const MortarPlan& plan, const MixedElement* el,
+                           const double* u, const double* params, const double* inv_jac,
+                           double* work, hipStream_t stream);
+int launch_mortar_corrections(bool gh, const MortarPlan& plan, const MixedElement* el,
+                              double* dt, double* work, hipStream_t stream);
+// Write every outgoing message into send (plan.message_bytes[gh] bytes).
+int launch_mortar_pack(bool gh, const MortarPlan& plan, double* work, void* send, int64_t step,
+                       hipStream_t stream);
+// Unpack the `count` messages of one sending rank's group (bytes long) into
+// the receive slots; status: 2 count int32 (code, slot) per message.
+int launch_mortar_unpack(bool gh, const MortarPlan& plan, double* work, const void* recv,
+                         int64_t bytes, int count, int64_t step, int32_t* status,
+                         hipStream_t stream);
 // Adjoint of the GH RHS on a mixed-p domain, linearised at u: ubar =
 // (d dt / d u)^T gbar over the work list; scratch_du = 150 x total points.
 int mixed_face_adjoint_lds_bytes(int kf, int km);

@@ -195,17 +195,20 @@ __global__ void __launch_bounds__(64)
 mortar_boundary_terms_kernel(const MortarRecord* __restrict__ recs,
                              const MortarFace* __restrict__ faces,
                              const int32_t* __restrict__ gather, const double* __restrict__ pkf,
-                             const double* __restrict__ pkm, double* __restrict__ bcm) {
+                             const double* __restrict__ pkm, const double* __restrict__ recv,
+                             double* __restrict__ bcm) {
   constexpr int NPK = GH ? 134 : 13;
   constexpr int NC = GH ? 50 : 5;
   const MortarRecord r = recs[blockIdx.x];
-  const MortarRecord pr_rec = recs[r.partner];
+  const bool remote = r.partner < 0;
   const double* own = mortar_package<NPK>(r, faces, pkf, pkm);
-  const double* ext = mortar_package<NPK>(pr_rec, faces, pkf, pkm);
+  // a partner in another partition: its package arrived in the own point order
+  const double* ext = remote ? recv + static_cast<int64_t>(NPK) * r.recv_off
+                             : mortar_package<NPK>(recs[r.partner], faces, pkf, pkm);
   const int km = r.ka * r.kb;
   double* out = bcm + static_cast<int64_t>(NC) * r.rec_off;
   for (int k = threadIdx.x; k < km; k += blockDim.x) {
-    const int s = gather[r.rec_off + k];
+    const int s = remote ? k : gather[r.rec_off + k];
     if constexpr (!GH) {
       double in[13], ex[13], bc[5];
 #pragma unroll
@@ -272,6 +275,121 @@ mortar_lift_kernel(const MixedElement* __restrict__ el, const MortarFace* __rest
   }
 }
 
+// Outgoing boundary messages, one workgroup each, into the send buffer's
+// group of the receiving rank: the group's directory entry (and, for the
+// first message of a group, the message count), the header, then the
+// record's mortar package oriented into the receiver's point order (the
+// sender orients, ComputeTimeDerivative.hpp:710-721): point s of the own
+// mortar goes to the receiver's point gather[s].
+template <int NPK>
+__global__ void __launch_bounds__(64)
+mortar_pack_kernel(const MortarMessage* __restrict__ msgs, const MortarRecord* __restrict__ recs,
+                   const MortarFace* __restrict__ faces, const int32_t* __restrict__ gather,
+                   const double* __restrict__ pkf, const double* __restrict__ pkm,
+                   unsigned char* __restrict__ send, int system, int32_t step_lo,
+                   int32_t step_hi) {
+  const MortarMessage msg = msgs[blockIdx.x];
+  unsigned char* group = send + msg.group_offset[system];
+  unsigned char* base = group + msg.offset[system];
+  int32_t* hd = reinterpret_cast<int32_t*>(base);
+  const int km = msg.km;
+  if (threadIdx.x < 16) {
+    int32_t v = msg.header[threadIdx.x];
+    if (threadIdx.x == 1) v = system;
+    if (threadIdx.x == 11) v = NPK;
+    if (threadIdx.x == 12) v = NPK * km;
+    if (threadIdx.x == 14) v = step_lo;
+    if (threadIdx.x == 15) v = step_hi;
+    hd[threadIdx.x] = v;
+  } else if (threadIdx.x == 16) {
+    int64_t* dir = reinterpret_cast<int64_t*>(group);
+    dir[1 + msg.index] = msg.offset[system];
+    if (msg.index == 0) dir[0] = msg.group_count;
+  }
+  const MortarRecord r = recs[msg.record];
+  const double* own = mortar_package<NPK>(r, faces, pkf, pkm);
+  double* data = reinterpret_cast<double*>(base + kMessageHeaderBytes);
+  for (int x = threadIdx.x; x < NPK * km; x += blockDim.x) {
+    const int c = x / km, k = x % km;
+    data[c * km + gather[r.rec_off + k]] = own[x];
+  }
+}
+
+// The messages of one sending rank's group, one workgroup each (grid = the
+// number of messages this rank expects from it): find the receiving slot by
+// (receiver element, receiver direction, sender element), check the header
+// and copy the package.  status[2 i] = 0 or the failed check, status[2 i + 1]
+// = the slot.
+template <int NPK>
+__global__ void __launch_bounds__(64)
+mortar_unpack_kernel(const unsigned char* __restrict__ group, int64_t bytes,
+                     const MortarRecvKey* __restrict__ keys, int num_keys,
+                     double* __restrict__ recv, int system, int32_t step_lo, int32_t step_hi,
+                     int32_t* __restrict__ status) {
+  __shared__ int64_t s_off, s_dst;
+  __shared__ int32_t s_n, s_code;
+  if (threadIdx.x == 0) {
+    const int64_t* dir = reinterpret_cast<const int64_t*>(group);
+    int code = 0, slot = -1, n = 0;
+    int64_t off = 0, dst = 0;
+    if (directory_bytes(gridDim.x) > bytes || dir[0] != static_cast<int64_t>(gridDim.x)) {
+      code = kMessageWrongCount;
+    } else {
+      off = dir[1 + blockIdx.x];
+      if (off < directory_bytes(gridDim.x) || off % 8 != 0 || off + kMessageHeaderBytes > bytes)
+        code = kMessageTruncated;
+    }
+    const int32_t* hd = reinterpret_cast<const int32_t*>(group + off);
+    if (code == 0) {
+      if (hd[0] != kMessageMagic) code = kMessageBadMagic;
+      else if (hd[1] != system || hd[11] != NPK) code = kMessageWrongSystem;
+      else if (hd[14] != step_lo || hd[15] != step_hi) code = kMessageWrongStep;
+      else if (hd[12] < 0 || off + kMessageHeaderBytes + 8 * static_cast<int64_t>(hd[12]) > bytes)
+        code = kMessageTruncated;
+    }
+    if (code == 0) {
+      int lo = 0, hi = num_keys - 1, found = -1;
+      while (lo <= hi) {
+        const int mid = (lo + hi) / 2;
+        const MortarRecvKey& k = keys[mid];
+        const int c = k.recv_e != hd[3] ? (k.recv_e < hd[3] ? -1 : 1)
+                      : k.recv_dir != hd[4] ? (k.recv_dir < hd[4] ? -1 : 1)
+                      : k.send_e != hd[2] ? (k.send_e < hd[2] ? -1 : 1)
+                                          : 0;
+        if (c == 0) {
+          found = mid;
+          break;
+        }
+        if (c < 0) lo = mid + 1;
+        else hi = mid - 1;
+      }
+      if (found < 0) {
+        code = kMessageUnknownMortar;
+      } else {
+        const MortarRecvKey& k = keys[found];
+        if (hd[9] != k.ka || hd[10] != k.kb || hd[12] != NPK * k.ka * k.kb) {
+          code = kMessageWrongMesh;
+        } else {
+          slot = k.slot;
+          dst = k.recv_off;
+          n = hd[12];
+        }
+      }
+    }
+    s_off = off;
+    s_dst = dst;
+    s_n = n;
+    s_code = code;
+    status[2 * blockIdx.x] = code;
+    status[2 * blockIdx.x + 1] = slot;
+  }
+  __syncthreads();
+  if (s_code != 0) return;
+  const double* data = reinterpret_cast<const double*>(group + s_off + kMessageHeaderBytes);
+  double* out = recv + NPK * s_dst;
+  for (int x = threadIdx.x; x < s_n; x += blockDim.x) out[x] = data[x];
+}
+
 // ---- host: the record list ------------------------------------------------------
 
 int map_direction(const int o[3], int dir) {
@@ -307,7 +425,8 @@ std::vector<int32_t> oriented_offsets(int n1, int n2, int sliced_dim, const int 
 }  // namespace
 
 int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
-                      const int32_t* records, MortarPlan* plan, std::string* error) {
+                      const int32_t* records, const int32_t* partition, int rank,
+                      MortarPlan* plan, std::string* error) {
   *plan = MortarPlan{};
   auto bad = [&](const std::string& msg) {
     *error = msg;
@@ -378,26 +497,39 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
     if ((transposed ? p.kb : p.ka) != r.ka || (transposed ? p.ka : p.kb) != r.kb)
       return bad("record " + std::to_string(i) + ": the two sides' mortar meshes disagree");
   }
-  // faces: records grouped by (dimension, element, side), caller order within
-  std::vector<int> order(num_records);
-  for (int i = 0; i < num_records; ++i) order[i] = i;
+  // the partition: owned elements in global order get local indices
+  std::vector<int32_t> local(num_elements, -1);
+  int num_owned = 0;
+  for (int e = 0; e < num_elements; ++e)
+    if (partition == nullptr || partition[e] == rank) local[e] = num_owned++;
+  if (num_owned == 0) return bad("rank " + std::to_string(rank) + " owns no element");
+  // faces: the owned elements' records grouped by (dimension, element,
+  // side), caller order within
+  std::vector<int> order;
+  for (int i = 0; i < num_records; ++i)
+    if (local[h[i].e] >= 0) order.push_back(i);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
     const auto kx = std::make_tuple(h[x].dir / 2, h[x].e, h[x].dir % 2);
     const auto ky = std::make_tuple(h[y].dir / 2, h[y].e, h[y].dir % 2);
     return kx < ky;
   });
-  std::vector<int> pos(num_records);
-  for (int i = 0; i < num_records; ++i) pos[order[i]] = i;
+  const int kept = static_cast<int>(order.size());
+  std::vector<int> pos(num_records, -1);
+  for (int i = 0; i < kept; ++i) pos[order[i]] = i;
   std::vector<MortarFace> faces;
-  std::vector<MortarRecord> recs(num_records);
+  std::vector<MortarRecord> recs(kept);
   std::vector<int32_t> gather, proj;
-  int64_t face_pts = 0, proj_pts = 0, rec_pts = 0;
+  std::vector<MortarMessage> messages;
+  std::vector<MortarRecvKey> keys;
+  int64_t face_pts = 0, proj_pts = 0, rec_pts = 0, recv_pts = 0;
+  int64_t msg_bytes[2] = {0, 0};
   int dim_begin[4] = {0, 0, 0, 0};
-  for (int i = 0; i < num_records; ++i) {
+  for (int i = 0; i < kept; ++i) {
     const Host& r = h[order[i]];
-    if (faces.empty() || faces.back().e != r.e || faces.back().dir != r.dir) {
+    const int le = local[r.e];
+    if (faces.empty() || faces.back().e != le || faces.back().dir != r.dir) {
       MortarFace f{};
-      f.e = r.e;
+      f.e = le;
       f.dir = r.dir;
       f.nf = r.na * r.nb;
       f.rec_begin = i;
@@ -411,27 +543,98 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
     ++f.rec_count;
     MortarRecord& m = recs[i];
     m.face = static_cast<int32_t>(faces.size() - 1);
-    m.partner = pos[r.partner];
     m.na = r.na;
     m.nb = r.nb;
     m.ka = r.ka;
     m.kb = r.kb;
     m.size_a = r.size[0];
     m.size_b = r.size[1];
+    const int64_t km = static_cast<int64_t>(r.ka) * r.kb;
     const bool identity = r.size[0] == kChildFull && r.size[1] == kChildFull && r.na == r.ka &&
                           r.nb == r.kb;
     m.mortar_off = identity ? -1 : proj_pts;
     if (!identity) {
-      proj_pts += static_cast<int64_t>(r.ka) * r.kb;
+      proj_pts += km;
       proj.push_back(i);
     }
     m.rec_off = rec_pts;
-    rec_pts += static_cast<int64_t>(r.ka) * r.kb;
+    rec_pts += km;
     // my mortar point k reads the partner's point offsets[k]: the partner's
     // oriented offsets map its points to mine, mine map mine to its
     const std::vector<int32_t> g = oriented_offsets(r.ka, r.kb, r.dir / 2, r.o);
     gather.insert(gather.end(), g.begin(), g.end());
+    const Host& p = h[r.partner];
+    if (local[p.e] >= 0) {
+      m.partner = pos[r.partner];
+      m.recv_off = -1;
+      continue;
+    }
+    // the partner lives in another partition: its package arrives as a
+    // message (slot = the order of such records), and this side's package
+    // leaves as one, oriented into the partner's point order
+    const int slot = static_cast<int>(keys.size());
+    m.partner = -1 - slot;
+    m.recv_off = recv_pts;
+    recv_pts += km;
+    MortarRecvKey key{};
+    key.recv_e = r.e;
+    key.recv_dir = r.dir;
+    key.send_e = p.e;
+    key.slot = slot;
+    key.ka = r.ka;
+    key.kb = r.kb;
+    key.recv_off = m.recv_off;
+    key.send_rank = partition == nullptr ? 0 : partition[p.e];
+    keys.push_back(key);
+    MortarMessage msg{};
+    msg.record = i;
+    msg.km = static_cast<int32_t>(km);
+    msg.peer = key.send_rank;  // the partner's rank receives this side's package
+    int32_t* hd = msg.header;
+    hd[0] = kMessageMagic;
+    hd[2] = r.e;
+    hd[3] = p.e;
+    hd[4] = p.dir;
+    hd[5] = r.dir;
+    for (int k = 0; k < 3; ++k) hd[6 + k] = extents[3 * r.e + k];
+    hd[9] = p.ka;
+    hd[10] = p.kb;
+    messages.push_back(msg);
   }
+  // outgoing messages grouped by receiving rank (ascending), record order
+  // within a group; each group starts with its directory
+  std::stable_sort(messages.begin(), messages.end(),
+                   [](const MortarMessage& x, const MortarMessage& y) { return x.peer < y.peer; });
+  for (size_t i = 0; i < messages.size();) {
+    size_t j = i;
+    while (j < messages.size() && messages[j].peer == messages[i].peer) ++j;
+    MortarPeer peer{};
+    peer.rank = messages[i].peer;
+    peer.count = static_cast<int>(j - i);
+    for (int sys = 0; sys < 2; ++sys) {
+      peer.offset[sys] = msg_bytes[sys];
+      int64_t off = directory_bytes(peer.count);
+      for (size_t q = i; q < j; ++q) {
+        messages[q].group_offset[sys] = peer.offset[sys];
+        messages[q].offset[sys] = off;
+        off += kMessageHeaderBytes + message_data_bytes(sys == 1, messages[q].km);
+      }
+      peer.bytes[sys] = off;
+      msg_bytes[sys] += off;
+    }
+    for (size_t q = i; q < j; ++q) {
+      messages[q].index = static_cast<int32_t>(q - i);
+      messages[q].group_count = peer.count;
+    }
+    plan->peers.push_back(peer);
+    i = j;
+  }
+  // incoming: the number of messages expected from each sending rank
+  for (const MortarRecvKey& k : keys) ++plan->expected[k.send_rank];
+  std::sort(keys.begin(), keys.end(), [](const MortarRecvKey& x, const MortarRecvKey& y) {
+    return std::make_tuple(x.recv_e, x.recv_dir, x.send_e) <
+           std::make_tuple(y.recv_e, y.recv_dir, y.send_e);
+  });
   // every face covered exactly once by its mortars
   for (const MortarFace& f : faces) {
     int quarters = 0;
@@ -441,7 +644,7 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
       return bad("face " + std::to_string(f.dir) + " of element " + std::to_string(f.e) +
                  ": its mortars do not cover it exactly once");
   }
-  std::vector<bool> has_face(6 * static_cast<size_t>(num_elements), false);
+  std::vector<bool> has_face(6 * static_cast<size_t>(num_owned), false);
   for (const MortarFace& f : faces) has_face[6 * static_cast<size_t>(f.e) + f.dir] = true;
   for (size_t i = 0; i < has_face.size(); ++i)
     if (!has_face[i])
@@ -453,7 +656,11 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
     km = std::max(km, m.ka * m.kb);
   }
   plan->num_faces = static_cast<int>(faces.size());
-  plan->num_records = num_records;
+  plan->num_records = kept;
+  plan->num_remote = static_cast<int>(keys.size());
+  plan->remote_points = recv_pts;
+  plan->message_bytes[0] = msg_bytes[0];
+  plan->message_bytes[1] = msg_bytes[1];
   plan->num_projected = static_cast<int>(proj.size());
   plan->face_points = face_pts;
   plan->projected_points = proj_pts;
@@ -475,6 +682,8 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
   upload(&plan->d_records, recs);
   upload(&plan->d_gather, gather);
   upload(&plan->d_projected, proj);
+  upload(&plan->d_messages, messages);
+  upload(&plan->d_keys, keys);
   if (rc != SXD_OK) {
     destroy_mortar_plan(plan);
     *error = "device set-up of the mortar tables failed";
@@ -487,65 +696,136 @@ void destroy_mortar_plan(MortarPlan* plan) {
   if (plan->d_records != nullptr) (void)hipFree(plan->d_records);
   if (plan->d_gather != nullptr) (void)hipFree(plan->d_gather);
   if (plan->d_projected != nullptr) (void)hipFree(plan->d_projected);
+  if (plan->d_messages != nullptr) (void)hipFree(plan->d_messages);
+  if (plan->d_keys != nullptr) (void)hipFree(plan->d_keys);
   *plan = MortarPlan{};
 }
 
 size_t mortar_work_doubles(bool gh, const MortarPlan& plan) {
   const size_t npk = gh ? 134 : 13, nc = gh ? 50 : 5;
-  return npk * static_cast<size_t>(plan.face_points + plan.projected_points) +
+  return npk * static_cast<size_t>(plan.face_points + plan.projected_points +
+                                   plan.remote_points) +
          static_cast<size_t>(plan.face_points) + nc * static_cast<size_t>(plan.record_points);
 }
 
-int launch_mortar_faces(bool gh, const MortarPlan& plan, const MixedElement* el, const double* u,
-                        const double* params, const double* inv_jac, double* dt, double* work,
-                        hipStream_t stream) {
+namespace {
+struct WorkLayout {
+  double *pkf, *pkm, *mag, *bcm, *recv;
+  WorkLayout(bool gh, const MortarPlan& plan, double* work) {
+    const size_t npk = gh ? 134 : 13, nc = gh ? 50 : 5;
+    pkf = work;
+    pkm = pkf + npk * plan.face_points;
+    mag = pkm + npk * plan.projected_points;
+    bcm = mag + plan.face_points;
+    recv = bcm + nc * plan.record_points;
+  }
+};
+}  // namespace
+
+int launch_mortar_packages(bool gh, const MortarPlan& plan, const MixedElement* el,
+                           const double* u, const double* params, const double* inv_jac,
+                           double* work, hipStream_t stream) {
   const double* full = device_mortar_matrices();
   const double* half = device_half_mortar_matrices();
   if (full == nullptr || half == nullptr) return SXD_ERR_DEVICE;
   if (work == nullptr) return SXD_ERR_INVALID_ARGUMENT;
   const int npk = gh ? 134 : 13;
-  double* pkf = work;
-  double* pkm = pkf + static_cast<size_t>(npk) * plan.face_points;
-  double* mag = pkm + static_cast<size_t>(npk) * plan.projected_points;
-  double* bcm = mag + plan.face_points;
+  const WorkLayout w(gh, plan, work);
   if (gh)
     hipLaunchKernelGGL(mortar_package_kernel<true>, dim3(plan.num_faces), dim3(64), 0, stream, el,
-                       plan.d_faces, u, params, inv_jac, pkf, mag);
+                       plan.d_faces, u, params, inv_jac, w.pkf, w.mag);
   else
     hipLaunchKernelGGL(mortar_package_kernel<false>, dim3(plan.num_faces), dim3(64), 0, stream,
-                       el, plan.d_faces, u, params, inv_jac, pkf, mag);
+                       el, plan.d_faces, u, params, inv_jac, w.pkf, w.mag);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   if (plan.num_projected > 0) {
     const dim3 grid(plan.num_projected, (npk + kProjectChunk - 1) / kProjectChunk);
     if (gh)
       hipLaunchKernelGGL(mortar_project_kernel<134>, grid, dim3(128), 0, stream, plan.d_records,
-                         plan.d_projected, plan.d_faces, pkf, pkm, full, half);
+                         plan.d_projected, plan.d_faces, w.pkf, w.pkm, full, half);
     else
       hipLaunchKernelGGL(mortar_project_kernel<13>, grid, dim3(128), 0, stream, plan.d_records,
-                         plan.d_projected, plan.d_faces, pkf, pkm, full, half);
+                         plan.d_projected, plan.d_faces, w.pkf, w.pkm, full, half);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   }
+  return SXD_OK;
+}
+
+int launch_mortar_corrections(bool gh, const MortarPlan& plan, const MixedElement* el,
+                              double* dt, double* work, hipStream_t stream) {
+  const double* full = device_mortar_matrices();
+  const double* half = device_half_mortar_matrices();
+  if (full == nullptr || half == nullptr) return SXD_ERR_DEVICE;
+  if (work == nullptr) return SXD_ERR_INVALID_ARGUMENT;
+  const WorkLayout w(gh, plan, work);
   if (gh)
     hipLaunchKernelGGL(mortar_boundary_terms_kernel<true>, dim3(plan.num_records), dim3(64), 0,
-                       stream, plan.d_records, plan.d_faces, plan.d_gather, pkf, pkm, bcm);
+                       stream, plan.d_records, plan.d_faces, plan.d_gather, w.pkf, w.pkm, w.recv,
+                       w.bcm);
   else
     hipLaunchKernelGGL(mortar_boundary_terms_kernel<false>, dim3(plan.num_records), dim3(64), 0,
-                       stream, plan.d_records, plan.d_faces, plan.d_gather, pkf, pkm, bcm);
+                       stream, plan.d_records, plan.d_faces, plan.d_gather, w.pkf, w.pkm, w.recv,
+                       w.bcm);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   for (int d = 0; d < 3; ++d) {
     const int count = plan.dim_begin[d + 1] - plan.dim_begin[d];
     if (count == 0) continue;
     if (gh)
       hipLaunchKernelGGL(mortar_lift_kernel<true>, dim3(count), dim3(64), 0, stream, el,
-                         plan.d_faces, plan.dim_begin[d], plan.d_records, mag, bcm, dt, full,
+                         plan.d_faces, plan.dim_begin[d], plan.d_records, w.mag, w.bcm, dt, full,
                          half);
     else
       hipLaunchKernelGGL(mortar_lift_kernel<false>, dim3(count), dim3(64), 0, stream, el,
-                         plan.d_faces, plan.dim_begin[d], plan.d_records, mag, bcm, dt, full,
+                         plan.d_faces, plan.dim_begin[d], plan.d_records, w.mag, w.bcm, dt, full,
                          half);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   }
   return SXD_OK;
+}
+
+int launch_mortar_faces(bool gh, const MortarPlan& plan, const MixedElement* el, const double* u,
+                        const double* params, const double* inv_jac, double* dt, double* work,
+                        hipStream_t stream) {
+  if (plan.num_remote > 0) return SXD_ERR_UNSUPPORTED;
+  const int rc = launch_mortar_packages(gh, plan, el, u, params, inv_jac, work, stream);
+  return rc != SXD_OK ? rc : launch_mortar_corrections(gh, plan, el, dt, work, stream);
+}
+
+int launch_mortar_pack(bool gh, const MortarPlan& plan, double* work, void* send, int64_t step,
+                       hipStream_t stream) {
+  if (plan.num_remote == 0) return SXD_OK;
+  if (work == nullptr || send == nullptr) return SXD_ERR_INVALID_ARGUMENT;
+  const WorkLayout w(gh, plan, work);
+  const int32_t lo = static_cast<int32_t>(static_cast<uint64_t>(step) & 0xffffffffu);
+  const int32_t hi = static_cast<int32_t>(static_cast<uint64_t>(step) >> 32);
+  unsigned char* out = static_cast<unsigned char*>(send);
+  if (gh)
+    hipLaunchKernelGGL(mortar_pack_kernel<134>, dim3(plan.num_remote), dim3(64), 0, stream,
+                       plan.d_messages, plan.d_records, plan.d_faces, plan.d_gather, w.pkf, w.pkm,
+                       out, 1, lo, hi);
+  else
+    hipLaunchKernelGGL(mortar_pack_kernel<13>, dim3(plan.num_remote), dim3(64), 0, stream,
+                       plan.d_messages, plan.d_records, plan.d_faces, plan.d_gather, w.pkf, w.pkm,
+                       out, 0, lo, hi);
+  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
+}
+
+int launch_mortar_unpack(bool gh, const MortarPlan& plan, double* work, const void* recv,
+                         int64_t bytes, int count, int64_t step, int32_t* status,
+                         hipStream_t stream) {
+  if (count == 0) return SXD_OK;
+  if (work == nullptr || recv == nullptr || status == nullptr) return SXD_ERR_INVALID_ARGUMENT;
+  const WorkLayout w(gh, plan, work);
+  const int32_t lo = static_cast<int32_t>(static_cast<uint64_t>(step) & 0xffffffffu);
+  const int32_t hi = static_cast<int32_t>(static_cast<uint64_t>(step) >> 32);
+  const unsigned char* in = static_cast<const unsigned char*>(recv);
+  if (gh)
+    hipLaunchKernelGGL(mortar_unpack_kernel<134>, dim3(count), dim3(64), 0, stream, in, bytes,
+                       plan.d_keys, plan.num_remote, w.recv, 1, lo, hi, status);
+  else
+    hipLaunchKernelGGL(mortar_unpack_kernel<13>, dim3(count), dim3(64), 0, stream, in, bytes,
+                       plan.d_keys, plan.num_remote, w.recv, 0, lo, hi, status);
+  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
 }
 
 }  // namespace sxd

@@ -574,30 +574,98 @@ class MortarDomain(MixedDomain):
     """Element set coupled through explicit mortar records
     (sxd_domain_create_mortars): neighbours with rotated logical frames and
     h-refined faces, per-element extents; ragged fields as MixedDomain.
-    records [R, 8]: {e, dir, en, o0, o1, o2, size_a, size_b}."""
+    records [R, 8]: {e, dir, en, o0, o1, o2, size_a, size_b}.
 
-    def __init__(self, extents, records):
+    With partition (the owning rank of every element) the domain is rank
+    `rank`'s part (sxd_domain_create_mortars_partition): fields cover its
+    elements only, and mortars shared with other ranks are coupled by
+    boundary messages: rhs_begin, the exchange of the send buffer's groups
+    (message_layout), unpack per sending rank, rhs_end."""
+
+    def __init__(self, extents, records, partition=None, rank=0):
         ext = np.ascontiguousarray(extents, dtype=np.int32)
         rec = np.ascontiguousarray(records, dtype=np.int32)
         if ext.ndim != 2 or ext.shape[1] != 3 or rec.ndim != 2 or rec.shape[1] != 8:
             raise ValueError("need extents [E, 3] and records [R, 8]")
-        self.extents = ext
         self.records = rec.copy()
-        self.num_elements = ext.shape[0]
-        self._args = (ext.copy(), rec.copy())
+        self.rank = int(rank)
         handle = ctypes.c_void_p()
-        check(lib().sxd_domain_create_mortars(ctypes.byref(handle), self.num_elements,
-                                              ext.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                                              rec.shape[0],
-                                              rec.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        ip = ctypes.POINTER(ctypes.c_int32)
+        if partition is None:
+            self.extents = ext
+            check(lib().sxd_domain_create_mortars(ctypes.byref(handle), ext.shape[0],
+                                                  ext.ctypes.data_as(ip), rec.shape[0],
+                                                  rec.ctypes.data_as(ip)))
+        else:
+            part = np.ascontiguousarray(partition, dtype=np.int32)
+            if part.shape != (ext.shape[0],):
+                raise ValueError("need one rank per element")
+            self.partition = part.copy()
+            self.extents = ext[part == self.rank]
+            check(lib().sxd_domain_create_mortars_partition(
+                ctypes.byref(handle), ext.shape[0], ext.ctypes.data_as(ip), rec.shape[0],
+                rec.ctypes.data_as(ip), part.ctypes.data_as(ip), self.rank))
+        self.num_elements = self.extents.shape[0]
+        self._args = (ext.copy(), rec.copy())
         self._handle = handle
         total = ctypes.c_int64()
         check(lib().sxd_domain_num_points(self._handle, ctypes.byref(total)))
         self.total_points = total.value
 
     @classmethod
-    def from_brick(cls, brick):
-        return cls(brick.extents, brick.records)
+    def from_brick(cls, brick, partition=None, rank=0):
+        return cls(brick.extents, brick.records, partition, rank)
+
+    def message_layout(self, system):
+        """[(rank, byte offset, bytes)] of the send buffer's groups, and its
+        total size."""
+        n = ctypes.c_int()
+        total = ctypes.c_int64()
+        sysid = SYSTEMS[system]
+        check(lib().sxd_boundary_messages_layout(self._handle, sysid, ctypes.byref(n), None, None,
+                                                 None, ctypes.byref(total)))
+        ranks = np.zeros(n.value, dtype=np.int32)
+        offs = np.zeros(n.value, dtype=np.int64)
+        sizes = np.zeros(n.value, dtype=np.int64)
+        i64 = ctypes.POINTER(ctypes.c_int64)
+        check(lib().sxd_boundary_messages_layout(
+            self._handle, sysid, ctypes.byref(n), ranks.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            offs.ctypes.data_as(i64), sizes.ctypes.data_as(i64), ctypes.byref(total)))
+        return [(int(r), int(o), int(b)) for r, o, b in zip(ranks, offs, sizes)], total.value
+
+    def messages_expected(self, rank):
+        c = ctypes.c_int()
+        check(lib().sxd_boundary_messages_expected(self._handle, int(rank), ctypes.byref(c)))
+        return c.value
+
+    def rhs_begin(self, system, u, params, inv_jac, out=None, step=0):
+        """Volume terms into out and the outgoing messages; returns (out,
+        send buffer as a uint8 tensor)."""
+        nc, n_par = (5, None) if system == "scalar_wave" else (50, 3)
+        _require(u, "u", self._shapes(nc))
+        _require(params, "params", self._scalar_shape() if n_par is None else self._shapes(n_par))
+        _require(inv_jac, "inv_jac", self._shapes(9))
+        if out is None:
+            out = torch.empty_like(u)
+        _require(out, "out", self._shapes(nc))
+        _, total = self.message_layout(system)
+        send = torch.empty(max(total, 1), dtype=torch.uint8, device=u.device)
+        check(lib().sxd_rhs_begin(self._handle, SYSTEMS[system], _ptr(u), _ptr(params),
+                                  _ptr(inv_jac), _ptr(out), ctypes.c_void_p(send.data_ptr()),
+                                  int(step), _stream()))
+        return out, send
+
+    def unpack(self, system, sender_rank, group):
+        """Take the message group (a uint8 device tensor) sent by sender_rank."""
+        if group.dtype != torch.uint8 or not group.is_cuda or not group.is_contiguous():
+            raise ValueError("the message group must be a contiguous uint8 device tensor")
+        check(lib().sxd_boundary_messages_unpack(self._handle, SYSTEMS[system], int(sender_rank),
+                                                 ctypes.c_void_p(group.data_ptr()),
+                                                 group.numel(), _stream()))
+
+    def rhs_end(self, system, out):
+        check(lib().sxd_rhs_end(self._handle, SYSTEMS[system], _ptr(out), _stream()))
+        return out
 
 
 class Rk3Stepper:
