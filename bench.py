@@ -148,10 +148,11 @@ def measure_sw_graph(calls=1000, reps=5):
 
 def measure_mixed(system, steps, warmup, rk3_steps=5):
     """Mixed-p workload (1000 elements of a 10x10x10 brick, extents drawn from
-    {4..12} per element and dimension): the batched work-list launch against
-    one launch per group of equal extents, with the autotuner's threads per
-    workgroup, and one RK3 step replayed from a HIP graph against the same
-    step enqueued call by call."""
+    {4..12} per element and dimension): the two batched organisations over
+    the whole domain (the mortar kernels, and the mixed-p face kernels)
+    against one launch per group of equal extents, with the autotuner's
+    threads per workgroup, and one RK3 step replayed from a HIP graph (the
+    autotuned organisation) against the same step enqueued call by call."""
     import numpy as np
     import torch
     from spectre_amd import dg, workloads as wl
@@ -172,10 +173,10 @@ def measure_mixed(system, steps, warmup, rk3_steps=5):
     chosen, _ = domain.autotune(system, d_u, d_p, d_inv, d_dt, repetitions=5)
     threads = domain.threads(system)
     ms = {}
-    for kernel in ("mixed_batched", "mixed_per_extents"):
+    for kernel in ("mortar", "mixed_batched", "mixed_per_extents"):
         domain.set_kernel(system, kernel)
         ms[kernel] = time_calls(lambda: rhs(d_u, d_p, d_inv, out=d_dt), steps, warmup)
-    domain.set_kernel(system, "mixed_batched")
+    domain.set_kernel(system, chosen)
     # one RK3 step: graph replay against call-by-call enqueueing
     dt_step = 1e-3
     d_u2 = d_u.clone()
@@ -195,13 +196,15 @@ def measure_mixed(system, steps, warmup, rk3_steps=5):
     eager_ms = time_calls(eager_step, rk3_steps, 1)
     points = brick.total_points
     alg = mixed_algorithmic_bytes(brick, nc, n_aux)
-    b = ms["mixed_batched"]
+    b = min(ms["mortar"], ms["mixed_batched"])
     return {
         "workload": f"{system}_mixed_p_1000el_extents_4..12",
         "grid_points": points,
         "extents_groups": len({tuple(e) for e in brick.extents.tolist()}),
         "autotuned": {"kernel": chosen, "threads_per_workgroup": threads},
         "batched_ms": round(b, 6),
+        "mortar_ms": round(ms["mortar"], 6),
+        "mixed_batched_ms": round(ms["mixed_batched"], 6),
         "per_extents_ms": round(ms["mixed_per_extents"], 6),
         "batch_gain": round(ms["mixed_per_extents"] / b, 3),
         "grid_points_per_s": round(points / (b * 1e-3), 1),
@@ -238,6 +241,76 @@ def measure_gauss(steps=50, warmup=5):
     return {"workload": "scalar_wave_plane_wave_gauss_512el_8^3", "grid_points": points,
             "ms": {k: round(v, 6) for k, v in ms.items()},
             "gauss_grid_points_per_s": round(points / (ms["gauss_composite"] * 1e-3), 1)}
+
+
+def measure_mortars(steps=30, warmup=3):
+    """The non-conforming mortar path: a periodic brick of 8^3 cells, every
+    16th cell split in all three dimensions and every 16th (offset 5) in one,
+    elements rotated at random, extents 6..8 (768 elements); ScalarWave
+    (plane wave) and GH (Kerr-Schild).  And the cost of the general path on
+    a conforming, aligned 512 x 8^3 brick: the mortar kernels against the
+    mixed-p batched and the fused kernels of the same brick."""
+    import numpy as np
+    import torch
+    from spectre_amd import dg, workloads as wl
+
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    refine = {c: (0, 1, 2) for c in range(0, 512, 16)}
+    refine.update({c: (1,) for c in range(5, 512, 16)})
+    out = {}
+    brick, u, g2, inv = wl.scalar_wave_mortar_workload(seed=0, base=(8, 8, 8), refine=refine,
+                                                       lowest=6, highest=8)
+    domain = dg.MortarDomain.from_brick(brick)
+    d_u, d_g, d_i = dev(u), dev(g2), dev(inv)
+    d_dt = torch.empty_like(d_u)
+    ms = time_calls(lambda: domain.scalar_wave_rhs(d_u, d_g, d_i, out=d_dt), steps, warmup)
+    out["sw_refined_rotated"] = {"elements": brick.num_elements, "mortars": len(brick.records),
+                                 "grid_points": brick.total_points, "ms": round(ms, 6),
+                                 "grid_points_per_s": round(brick.total_points / (ms * 1e-3), 1)}
+    brick, u, ga, inv = wl.gh_mortar_workload(seed=0, base=(8, 8, 8), refine=refine, lowest=6,
+                                              highest=8, noise=1.0e-6)
+    domain = dg.MortarDomain.from_brick(brick)
+    d_u, d_g, d_i = dev(u), dev(ga), dev(inv)
+    d_dt = torch.empty_like(d_u)
+    ms = time_calls(lambda: domain.gh_rhs(d_u, d_g, d_i, out=d_dt), steps, warmup)
+    out["gh_refined_rotated"] = {"elements": brick.num_elements, "mortars": len(brick.records),
+                                 "grid_points": brick.total_points, "ms": round(ms, 6),
+                                 "grid_points_per_s": round(brick.total_points / (ms * 1e-3), 1)}
+    # conforming aligned 512 x 8^3: mortar kernels vs mixed batched vs fused
+    for system in ("scalar_wave", "gh"):
+        if system == "scalar_wave":
+            mb, u, par, inv = wl.scalar_wave_plane_wave_mixed(seed=0, elements=(8, 8, 8),
+                                                             lowest=8, highest=8)
+        else:
+            mb, u, par, inv = wl.gh_kerr_schild_mixed(seed=0, elements=(8, 8, 8), lowest=8,
+                                                     highest=8)
+        nb = np.asarray(mb.neighbors)
+        rec = np.array([[e, d, nb[e, d], 1, 3, 5, 1, 1] for e in range(nb.shape[0])
+                        for d in range(6)], dtype=np.int32)
+        d_u, d_p, d_i = dev(u), dev(par), dev(inv)
+        d_dt = torch.empty_like(d_u)
+        rhs = (lambda dm: (lambda: dm.scalar_wave_rhs(d_u, d_p, d_i, out=d_dt))) \
+            if system == "scalar_wave" else (lambda dm: (lambda: dm.gh_rhs(d_u, d_p, d_i, out=d_dt)))
+        mo = dg.MortarDomain(mb.extents, rec)
+        mx = dg.MixedDomain.from_brick(mb)
+        mx.set_kernel(system, "mixed_batched")
+        res = {"mortar": time_calls(rhs(mo), steps, warmup),
+               "mixed_batched": time_calls(rhs(mx), steps, warmup)}
+        if system == "scalar_wave":
+            ub, uu, ug, ui = wl.scalar_wave_plane_wave_3d(seed=0, elements=(8, 8, 8),
+                                                          extents=(8, 8, 8))
+        else:
+            ub, uu, ug, ui = wl.gh_kerr_schild_brick(seed=0, elements=(8, 8, 8),
+                                                     extents=(8, 8, 8))
+        ud = dg.Domain.from_brick(ub)
+        ud.set_kernel(system, "fused")
+        e_u, e_g, e_i = dev(uu), dev(ug), dev(ui)
+        e_dt = torch.empty_like(e_u)
+        fn = (lambda: ud.scalar_wave_rhs(e_u, e_g, e_i, out=e_dt)) if system == "scalar_wave" \
+            else (lambda: ud.gh_rhs(e_u, e_g, e_i, out=e_dt))
+        res["fused"] = time_calls(fn, steps, warmup)
+        out[system + "_conforming_512el_8^3_ms"] = {k: round(v, 6) for k, v in res.items()}
+    return out
 
 
 def measure_boundaries(steps=50, warmup=5):
@@ -447,7 +520,7 @@ def main():
                     help="also measure the mixed-p batched form (1) or not (0)")
     ap.add_argument("--only-extension", default="",
                     choices=["", "sw_graph", "mixed_sw", "mixed_gh", "adjoint", "gh_adjoint",
-                             "gh_adjoint_mixed", "boundaries", "gauss"],
+                             "gh_adjoint_mixed", "boundaries", "gauss", "mortars"],
                     help="run only this extension measurement (for profiling) and print "
                          "its JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -479,7 +552,8 @@ def main():
                 "gh_adjoint": measure_gh_adjoint,
                 "gh_adjoint_mixed": lambda: measure_gh_adjoint(mixed=True),
                 "boundaries": measure_boundaries,
-                "gauss": measure_gauss}
+                "gauss": measure_gauss,
+                "mortars": measure_mortars}
         print(json.dumps(only[args.only_extension]()))
         return
 
@@ -600,7 +674,8 @@ def main():
                       "gh_adjoint": measure_gh_adjoint(),
                       "gh_adjoint_mixed": measure_gh_adjoint(mixed=True),
                       "boundaries": measure_boundaries(),
-                      "gauss": measure_gauss()}
+                      "gauss": measure_gauss(),
+                      "mortars": measure_mortars()}
 
     result = {
         "metric": METRIC,

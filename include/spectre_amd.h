@@ -449,7 +449,9 @@ int sxd_gh_rhs(const sxd_domain* domain, const double* u, const double* gammas,
  * device tables) must already exist.  SXD_ERR_UNSUPPORTED if the domain's
  * work arrays exceed the fixed array: 150 doubles per grid point on mixed-p
  * domains (about 111,000 points), 162 per point with the composite kernel,
- * none with the fused kernels.  Every call on one device shares that device's
+ * none with the fused kernels (mixed-p domains run their batched mixed-p
+ * kernels here, also where SXD_KERNEL_MORTAR is selected; mortar domains
+ * return SXD_ERR_UNSUPPORTED).  Every call on one device shares that device's
  * array, which keeps the last call's contents: calls that overlap on one GPU
  * (other streams, other domains, other threads) race and must be serialised
  * by the caller.  Results equal sxd_gh_rhs bit for bit. */
@@ -487,7 +489,7 @@ enum {
   SXD_KERNEL_MIXED_BATCHED = 4,    /* one launch per kernel over all elements,
                                       ordered by decreasing cost */
   SXD_KERNEL_MIXED_PER_EXTENTS = 5, /* one launch per kernel and group of equal extents */
-  /* mortar domains: */
+  /* mortar domains, and periodic mixed-p domains (the default there): */
   SXD_KERNEL_MORTAR = 6 /* ragged volume kernels, then package, projection,
                            boundary-term and lift launches over the mortars */
 };

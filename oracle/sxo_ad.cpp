@@ -145,6 +145,30 @@ int sxo_gh_evolve_vjp(int num_elements, const int* ext, const int* nbr, const do
   return 0;
 }
 
+// The same on a mixed-p set of elements (ragged layout).
+int sxo_gh_evolve_mixed_vjp(int num_elements, const int* extents, const int* nbr,
+                            const double* u0, const double* gammas, const double* inv_j,
+                            double dt, int steps, const double* gbar_final, double* ubar0) {
+  const size_t np = ragged_points(num_elements, extents);
+  const size_t n = 50 * np;
+  auto ga = constants(gammas, 3 * np);
+  auto ij = constants(inv_j, 9 * np);
+  tape_vjp(n, u0, n, gbar_final, ubar0, [&](const Rev* x, Rev* y) {
+    std::vector<Rev> u(x, x + n), u1(n), u2(n), l(n);
+    for (int s = 0; s < steps; ++s) {
+      sxo::gh_rhs_mixed(num_elements, extents, nbr, u.data(), ga.data(), ij.data(), l.data());
+      sxo::rk3_substep(0, n, dt, u.data(), u.data(), l.data(), u1.data());
+      sxo::gh_rhs_mixed(num_elements, extents, nbr, u1.data(), ga.data(), ij.data(), l.data());
+      sxo::rk3_substep(1, n, dt, u.data(), u1.data(), l.data(), u2.data());
+      sxo::gh_rhs_mixed(num_elements, extents, nbr, u2.data(), ga.data(), ij.data(), l.data());
+      sxo::rk3_substep(2, n, dt, u.data(), u2.data(), l.data(), u1.data());
+      u.swap(u1);
+    }
+    for (size_t i = 0; i < n; ++i) y[i] = u[i];
+  });
+  return 0;
+}
+
 // ScalarWave DG RHS by the tape (cross-check of the hand-written adjoint).
 int sxo_sw_rhs_vjp(int num_elements, const int* ext, const int* nbr, const double* u,
                    const double* gamma2, const double* inv_j, const double* gbar, double* ubar) {

@@ -108,6 +108,9 @@ struct sxd_domain {
   // Mortar domains (non-aligned neighbours, h-refined faces): the ragged
   // element set above (its neighbour table unused) and the mortar tables.
   bool mortars = false;
+  // A periodic mixed-p domain also holds the mortar tables of its
+  // neighbour table (SXD_KERNEL_MORTAR).
+  bool conforming_plan = false;
   sxd::MortarPlan plan;
 };
 
@@ -565,8 +568,21 @@ int sxd_domain_create(sxd_domain** domain, int num_elements, const int32_t exten
   return ok();
 }
 
+namespace {
+int create_mixed_domain(sxd_domain** domain, int num_elements, const int32_t* extents,
+                        const int32_t* neighbors, bool conforming_plan);
+}  // namespace
+
 int sxd_domain_create_mixed(sxd_domain** domain, int num_elements, const int32_t* extents,
                             const int32_t* neighbors) {
+  return create_mixed_domain(domain, num_elements, extents, neighbors, true);
+}
+
+namespace {
+// conforming_plan: also build the mortar tables of the neighbour table
+// (aligned, Full) when there is no external face, for SXD_KERNEL_MORTAR.
+int create_mixed_domain(sxd_domain** domain, int num_elements, const int32_t* extents,
+                        const int32_t* neighbors, bool conforming_plan) {
   if (domain == nullptr) return fail(SXD_ERR_INVALID_ARGUMENT, "null domain output");
   *domain = nullptr;
   if (num_elements < 1) return fail(SXD_ERR_INVALID_ARGUMENT, "need at least one element");
@@ -660,9 +676,29 @@ int sxd_domain_create_mixed(sxd_domain** domain, int num_elements, const int32_t
     sxd_domain_destroy(d);
     return fail(rc, "mixed domain set-up failed");
   }
+  if (conforming_plan && !has_boundary) {
+    std::vector<int32_t> records(48 * static_cast<size_t>(num_elements));
+    for (int e = 0; e < num_elements; ++e)
+      for (int dir = 0; dir < 6; ++dir) {
+        int32_t* q = records.data() + 8 * (6 * static_cast<size_t>(e) + dir);
+        const int32_t rec[8] = {e, dir, neighbors[6 * e + dir], 1, 3, 5, SXD_CHILD_FULL,
+                                SXD_CHILD_FULL};
+        std::memcpy(q, rec, sizeof(rec));
+      }
+    std::string error;
+    rc = build_mortar_plan(num_elements, extents, 6 * num_elements, records.data(), nullptr, 0,
+                           &d->plan, &error);
+    if (rc == SXD_OK && device_half_mortar_matrices() == nullptr) rc = SXD_ERR_DEVICE;
+    if (rc != SXD_OK) {
+      sxd_domain_destroy(d);
+      return fail(rc, "mortar tables of the mixed domain: %s", error.c_str());
+    }
+    d->conforming_plan = true;
+  }
   *domain = d;
   return ok();
 }
+}  // namespace
 
 namespace {
 int create_mortar_domain(sxd_domain** domain, int num_elements, const int32_t* extents,
@@ -688,7 +724,8 @@ int create_mortar_domain(sxd_domain** domain, int num_elements, const int32_t* e
   std::vector<int32_t> self(6 * static_cast<size_t>(owned));
   for (size_t i = 0; i < self.size(); ++i) self[i] = static_cast<int32_t>(i / 6);
   sxd_domain* d = nullptr;
-  if (int rc = sxd_domain_create_mixed(&d, owned, local_ext.data(), self.data()); rc != SXD_OK)
+  if (int rc = create_mixed_domain(&d, owned, local_ext.data(), self.data(), false);
+      rc != SXD_OK)
     return rc;
   std::string error;
   if (int rc = build_mortar_plan(num_elements, extents, num_records, records, partition, rank,
@@ -759,7 +796,7 @@ int sxd_domain_destroy(sxd_domain* domain) {
   if (domain->d_list != nullptr) (void)hipFree(domain->d_list);
   if (domain->d_buckets != nullptr) (void)hipFree(domain->d_buckets);
   if (domain->d_nbr != nullptr) (void)hipFree(domain->d_nbr);
-  if (domain->mortars) destroy_mortar_plan(&domain->plan);
+  if (domain->mortars || domain->conforming_plan) destroy_mortar_plan(&domain->plan);
   if (!domain->scratch.empty()) {
     (void)hipDeviceSynchronize();
     for (auto& s : domain->scratch) s.second->release();
@@ -826,6 +863,7 @@ int candidate_kernels(const sxd_domain* d, int system, int* k) {
     return n;
   }
   if (d->mixed) {
+    if (d->conforming_plan) k[n++] = SXD_KERNEL_MORTAR;
     k[n++] = SXD_KERNEL_MIXED_BATCHED;
     k[n++] = SXD_KERNEL_MIXED_PER_EXTENTS;
     return n;
@@ -874,6 +912,13 @@ int run_mixed(sxd_domain* d, Scratch& sc, bool gh, int kernel, const double* u,
   if (d->mortars) {
     if (ghost != nullptr || bc_kind != kSwBcGhostArray) return SXD_ERR_UNSUPPORTED;
     return run_mortars(d, sc, gh, u, params, inv_jac, dt, stream);
+  }
+  if (kernel == SXD_KERNEL_MORTAR) {
+    // external faces take the mixed-p kernels (no mortar plan is built then)
+    if (!d->conforming_plan || ghost != nullptr || bc_kind != kSwBcGhostArray)
+      kernel = SXD_KERNEL_MIXED_BATCHED;
+    else
+      return run_mortars(d, sc, gh, u, params, inv_jac, dt, stream);
   }
   double* scratch = nullptr;
   if (gh) {
@@ -1061,7 +1106,10 @@ int sxd_gh_rhs_noalloc(const sxd_domain* domain, const double* u, const double* 
   if (domain->mortars)
     return fail(SXD_ERR_UNSUPPORTED, "%s: not built for mortar domains", __func__);
   sxd_domain* d = const_cast<sxd_domain*>(domain);
-  const int kernel = resolve_kernel(d, SXD_SYSTEM_GH);
+  int kernel = resolve_kernel(d, SXD_SYSTEM_GH);
+  // the mortar organisation's package buffers do not fit the fixed array at
+  // useful sizes: mixed-p domains run the batched mixed-p kernels here
+  if (kernel == SXD_KERNEL_MORTAR) kernel = SXD_KERNEL_MIXED_BATCHED;
   // work arrays the kernel organisation needs: 150 doubles per point (mixed
   // p), 162 (composite), none (fused)
   const size_t points = static_cast<size_t>(domain_points(d));

@@ -187,6 +187,7 @@ struct MortarPlan {
   std::map<int, int> expected;        // messages expected from each rank
   int dim_begin[4] = {0, 0, 0, 0};  // faces of dimension d: [dim_begin[d], dim_begin[d+1])
   int kf = 0, km = 0;
+  int tmp_max = 1;  // largest ka nb or na kb: doubles per component of a staged pass
   MortarFace* d_faces = nullptr;
   MortarRecord* d_records = nullptr;
   int32_t* d_gather = nullptr;

@@ -21,14 +21,16 @@
 //   1. mortar_package_kernel: one workgroup per face, the package of every
 //      face point and |n| (SW 13 values, GH 10 pairs x 13 + 4 speeds);
 //   2. mortar_project_kernel: records whose mortar differs from the face,
-//      one output (mortar point, component) per work item;
+//      one workgroup per (record, component chunk): the dimension-a sums
+//      staged in LDS, then one output (mortar point, component) per work item;
 //   3. mortar_boundary_terms_kernel: one workgroup per record, the
 //      correction at every mortar point from the own and the gathered
 //      partner package;
 //   4. mortar_lift_kernel: one launch per dimension (both sides; their points
-//      differ), one thread per face point, which projects and lifts the
-//      face's mortars in record order, so every dt point sees its faces in
-//      direction order.
+//      differ), one workgroup per face: each of its mortars in record order
+//      is projected back (dimension-a sums staged in LDS, then one work item
+//      per (component, face point)) and lifted, so every dt point sees its
+//      faces in direction order and a face's mortars in record order.
 // Matrix products are sums over the input index, first term a product, then
 // fused multiply-adds, dimension a before b, as in the oracle.
 #include <hip/hip_runtime.h>
@@ -77,28 +79,8 @@ __device__ __forceinline__ const double* from_mortar(const double* full, const d
          nf * nm;
 }
 
-// Output point (ra, rb) of the two-dimensional product Mb (Ma in): in is
-// [ib][ia] (a fastest), Ma [oa][ia], Mb [ob][ib], nullptr = identity.  The
-// dimension-a sums are complete before the dimension-b sum uses them.
-__device__ __forceinline__ double apply_2d(const double* in, int ia, int ib, const double* Ma,
-                                           const double* Mb, int ra, int rb) {
-  auto pass_a = [&](int lb) {
-    const double* v = in + ia * lb;
-    if (Ma == nullptr) return v[ra];
-    const double* row = Ma + ra * ia;
-    double s = row[0] * v[0];
-    for (int l = 1; l < ia; ++l) s = fma(row[l], v[l], s);
-    return s;
-  };
-  if (Mb == nullptr) return pass_a(rb);
-  const double* row = Mb + rb * ib;
-  double s = row[0] * pass_a(0);
-  for (int l = 1; l < ib; ++l) s = fma(row[l], pass_a(l), s);
-  return s;
-}
-
 template <bool GH>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(128)
 mortar_package_kernel(const MixedElement* __restrict__ el, const MortarFace* __restrict__ faces,
                       const double* __restrict__ u, const double* __restrict__ params,
                       const double* __restrict__ inv_jac, double* __restrict__ pkf,
@@ -156,27 +138,67 @@ mortar_package_kernel(const MixedElement* __restrict__ el, const MortarFace* __r
   }
 }
 
-// Components per workgroup of the projection kernel.
-constexpr int kProjectChunk = 16;
+// Components staged at once: at most `cap`, and no more than fit 32 KiB of
+// LDS at the largest staged pass (several workgroups per CU).
+constexpr int kProjectCap = 16, kLiftCap = 25;
+inline int staged_chunk(int nc, int tmp_max, int cap) {
+  return std::max(1, std::min({nc, cap, 32768 / (8 * tmp_max)}));
+}
 
+// Dimension-a pass of apply_2d for a chunk of cn components, into LDS:
+// tmp[c][lb][ra] (oa x ib per component) = sum_l Ma[ra][l] in[c][l + ia lb]
+// (in itself where Ma is the identity).  in: component stride `stride`.
+__device__ __forceinline__ void stage_pass_a(const double* in, int stride, int cn, int ia, int ib,
+                                             int oa, const double* Ma, double* tmp) {
+  for (int x = threadIdx.x; x < cn * oa * ib; x += blockDim.x) {
+    const int c = x / (oa * ib), y = x % (oa * ib), lb = y / oa, ra = y % oa;
+    const double* v = in + c * stride + ia * lb;
+    double s;
+    if (Ma == nullptr) {
+      s = v[ra];
+    } else {
+      const double* row = Ma + ra * ia;
+      s = row[0] * v[0];
+      for (int l = 1; l < ia; ++l) s = fma(row[l], v[l], s);
+    }
+    tmp[x] = s;
+  }
+}
+
+// Dimension-b pass at output (ra, rb) of component c from the staged tmp.
+__device__ __forceinline__ double pass_b(const double* tmp, int c, int oa, int ib,
+                                         const double* Mb, int ra, int rb) {
+  const double* v = tmp + c * oa * ib + ra;
+  if (Mb == nullptr) return v[oa * rb];
+  const double* row = Mb + rb * ib;
+  double s = row[0] * v[0];
+  for (int l = 1; l < ib; ++l) s = fma(row[l], v[oa * l], s);
+  return s;
+}
+
+// Face -> mortar projection of the records that need one; grid (record,
+// component chunk of `chunk`); dynamic LDS: chunk x (ka nb) doubles.
 template <int NPK>
-__global__ void __launch_bounds__(128)
+__global__ void __launch_bounds__(256)
 mortar_project_kernel(const MortarRecord* __restrict__ recs, const int32_t* __restrict__ list,
                       const MortarFace* __restrict__ faces, const double* __restrict__ pkf,
                       double* __restrict__ pkm, const double* __restrict__ full,
-                      const double* __restrict__ half) {
+                      const double* __restrict__ half, int chunk) {
+  extern __shared__ double s_tmp[];
   const MortarRecord r = recs[list[blockIdx.x]];
   const MortarFace fr = faces[r.face];
   const double* Ma = to_mortar(full, half, r.na, r.ka, r.size_a);
   const double* Mb = to_mortar(full, half, r.nb, r.kb, r.size_b);
   const int nf = r.na * r.nb, km = r.ka * r.kb;
-  const double* in = pkf + static_cast<int64_t>(NPK) * fr.pt_off;
-  double* out = pkm + static_cast<int64_t>(NPK) * r.mortar_off;
-  const int c0 = blockIdx.y * kProjectChunk;
-  const int cn = min(kProjectChunk, NPK - c0);
+  const int c0 = blockIdx.y * chunk;
+  const int cn = min(chunk, NPK - c0);
+  const double* in = pkf + static_cast<int64_t>(NPK) * fr.pt_off + c0 * nf;
+  double* out = pkm + static_cast<int64_t>(NPK) * r.mortar_off + c0 * km;
+  stage_pass_a(in, nf, cn, r.na, r.nb, r.ka, Ma, s_tmp);
+  __syncthreads();
   for (int x = threadIdx.x; x < cn * km; x += blockDim.x) {
-    const int c = c0 + x / km, k = x % km;
-    out[c * km + k] = apply_2d(in + c * nf, r.na, r.nb, Ma, Mb, k % r.ka, k / r.ka);
+    const int c = x / km, k = x % km;
+    out[c * km + k] = pass_b(s_tmp, c, r.ka, r.nb, Mb, k % r.ka, k / r.ka);
   }
 }
 
@@ -243,35 +265,58 @@ mortar_boundary_terms_kernel(const MortarRecord* __restrict__ recs,
   }
 }
 
+// Mortar -> face projection and lift of the faces [face_begin, ...) of one
+// dimension, one workgroup per face: its mortars in record order, each
+// projected (dimension a staged in LDS for `chunk` components at a time,
+// then dimension b per (component, face point) work item) and added into
+// dt.  Dynamic LDS: chunk x (na kb) doubles.
 template <bool GH>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(256)
 mortar_lift_kernel(const MixedElement* __restrict__ el, const MortarFace* __restrict__ faces,
                    int face_begin, const MortarRecord* __restrict__ recs,
                    const double* __restrict__ mag, const double* __restrict__ bcm,
                    double* __restrict__ dt, const double* __restrict__ full,
-                   const double* __restrict__ half) {
+                   const double* __restrict__ half, int chunk) {
   constexpr int NC = GH ? 50 : 5;
+  extern __shared__ double s_tmp[];
   const MortarFace fr = faces[face_begin + blockIdx.x];
   const MixedElement m = el[fr.e];
   const int d = fr.dir >> 1, side = fr.dir & 1;
   const int na = d == 0 ? m.n[1] : m.n[0];
   const int nd = d == 0 ? m.n[0] : (d == 1 ? m.n[1] : m.n[2]);
-  for (int f = threadIdx.x; f < fr.nf; f += blockDim.x) {
-    const int fa = f % na, fb = f / na;
-    const int p = face_point_index(m, d, side, fa, fb);
-    const double fac = lift_factor(nd, mag[fr.pt_off + f]);
-    double* target = dt + NC * m.off + p;
-    for (int q = fr.rec_begin; q < fr.rec_begin + fr.rec_count; ++q) {
-      const MortarRecord r = recs[q];
-      const double* Pa = from_mortar(full, half, r.na, r.ka, r.size_a);
-      const double* Pb = from_mortar(full, half, r.nb, r.kb, r.size_b);
-      const int km = r.ka * r.kb;
-      const double* src = bcm + static_cast<int64_t>(NC) * r.rec_off;
-      for (int c = 0; c < NC; ++c) {
-        const double v = apply_2d(src + c * km, r.ka, r.kb, Pa, Pb, fa, fb);
-        target[c * m.np] = target[c * m.np] + v * fac;
+  const int nf = fr.nf;
+  double* target0 = dt + NC * m.off;
+  for (int q = fr.rec_begin; q < fr.rec_begin + fr.rec_count; ++q) {
+    const MortarRecord r = recs[q];
+    const double* Pa = from_mortar(full, half, r.na, r.ka, r.size_a);
+    const double* Pb = from_mortar(full, half, r.nb, r.kb, r.size_b);
+    const int km = r.ka * r.kb;
+    const double* src = bcm + static_cast<int64_t>(NC) * r.rec_off;
+    if (Pa == nullptr && Pb == nullptr) {
+      // the mortar is the face
+      for (int x = threadIdx.x; x < NC * nf; x += blockDim.x) {
+        const int c = x / nf, f = x % nf;
+        const int p = face_point_index(m, d, side, f % na, f / na);
+        double& t = target0[c * m.np + p];
+        t = t + src[c * km + f] * lift_factor(nd, mag[fr.pt_off + f]);
       }
+    } else {
+      for (int c0 = 0; c0 < NC; c0 += chunk) {
+        const int cn = min(chunk, NC - c0);
+        stage_pass_a(src + c0 * km, km, cn, r.ka, r.kb, r.na, Pa, s_tmp);
+        __syncthreads();
+        for (int x = threadIdx.x; x < cn * nf; x += blockDim.x) {
+          const int c = x / nf, f = x % nf;
+          const int fa = f % na, fb = f / na;
+          const int p = face_point_index(m, d, side, fa, fb);
+          double& t = target0[(c0 + c) * m.np + p];
+          t = t + pass_b(s_tmp, c, r.na, r.kb, Pb, fa, fb) * lift_factor(nd, mag[fr.pt_off + f]);
+        }
+        __syncthreads();  // s_tmp reused; this mortar's dt updates precede the next one's
+      }
+      continue;
     }
+    __syncthreads();  // this mortar's dt updates precede the next one's
   }
 }
 
@@ -650,10 +695,11 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
     if (!has_face[i])
       return bad("element " + std::to_string(i / 6) + " has no mortar in direction " +
                  std::to_string(i % 6));
-  int kf = 0, km = 0;
+  int kf = 0, km = 0, tmp_max = 1;
   for (const MortarRecord& m : recs) {
     kf = std::max(kf, m.na * m.nb);
     km = std::max(km, m.ka * m.kb);
+    tmp_max = std::max({tmp_max, m.ka * m.nb, m.na * m.kb});
   }
   plan->num_faces = static_cast<int>(faces.size());
   plan->num_records = kept;
@@ -668,6 +714,7 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
   for (int d = 0; d < 4; ++d) plan->dim_begin[d] = dim_begin[d];
   plan->kf = kf;
   plan->km = km;
+  plan->tmp_max = tmp_max;
   int rc = SXD_OK;
   auto upload = [&](auto** dst, const auto& v) {
     if (rc != SXD_OK || v.empty()) return;
@@ -732,20 +779,22 @@ int launch_mortar_packages(bool gh, const MortarPlan& plan, const MixedElement* 
   const int npk = gh ? 134 : 13;
   const WorkLayout w(gh, plan, work);
   if (gh)
-    hipLaunchKernelGGL(mortar_package_kernel<true>, dim3(plan.num_faces), dim3(64), 0, stream, el,
+    hipLaunchKernelGGL(mortar_package_kernel<true>, dim3(plan.num_faces), dim3(128), 0, stream, el,
                        plan.d_faces, u, params, inv_jac, w.pkf, w.mag);
   else
-    hipLaunchKernelGGL(mortar_package_kernel<false>, dim3(plan.num_faces), dim3(64), 0, stream,
+    hipLaunchKernelGGL(mortar_package_kernel<false>, dim3(plan.num_faces), dim3(128), 0, stream,
                        el, plan.d_faces, u, params, inv_jac, w.pkf, w.mag);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   if (plan.num_projected > 0) {
-    const dim3 grid(plan.num_projected, (npk + kProjectChunk - 1) / kProjectChunk);
+    const int chunk = staged_chunk(npk, plan.tmp_max, kProjectCap);
+    const dim3 grid(plan.num_projected, (npk + chunk - 1) / chunk);
+    const size_t lds = sizeof(double) * chunk * plan.tmp_max;
     if (gh)
-      hipLaunchKernelGGL(mortar_project_kernel<134>, grid, dim3(128), 0, stream, plan.d_records,
-                         plan.d_projected, plan.d_faces, w.pkf, w.pkm, full, half);
+      hipLaunchKernelGGL(mortar_project_kernel<134>, grid, dim3(256), lds, stream, plan.d_records,
+                         plan.d_projected, plan.d_faces, w.pkf, w.pkm, full, half, chunk);
     else
-      hipLaunchKernelGGL(mortar_project_kernel<13>, grid, dim3(128), 0, stream, plan.d_records,
-                         plan.d_projected, plan.d_faces, w.pkf, w.pkm, full, half);
+      hipLaunchKernelGGL(mortar_project_kernel<13>, grid, dim3(256), lds, stream, plan.d_records,
+                         plan.d_projected, plan.d_faces, w.pkf, w.pkm, full, half, chunk);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   }
   return SXD_OK;
@@ -767,17 +816,19 @@ int launch_mortar_corrections(bool gh, const MortarPlan& plan, const MixedElemen
                        stream, plan.d_records, plan.d_faces, plan.d_gather, w.pkf, w.pkm, w.recv,
                        w.bcm);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  const int chunk = staged_chunk(gh ? 50 : 5, plan.tmp_max, kLiftCap);
+  const size_t lds = sizeof(double) * chunk * plan.tmp_max;
   for (int d = 0; d < 3; ++d) {
     const int count = plan.dim_begin[d + 1] - plan.dim_begin[d];
     if (count == 0) continue;
     if (gh)
-      hipLaunchKernelGGL(mortar_lift_kernel<true>, dim3(count), dim3(64), 0, stream, el,
+      hipLaunchKernelGGL(mortar_lift_kernel<true>, dim3(count), dim3(256), lds, stream, el,
                          plan.d_faces, plan.dim_begin[d], plan.d_records, w.mag, w.bcm, dt, full,
-                         half);
+                         half, chunk);
     else
-      hipLaunchKernelGGL(mortar_lift_kernel<false>, dim3(count), dim3(64), 0, stream, el,
+      hipLaunchKernelGGL(mortar_lift_kernel<false>, dim3(count), dim3(256), lds, stream, el,
                          plan.d_faces, plan.dim_begin[d], plan.d_records, w.mag, w.bcm, dt, full,
-                         half);
+                         half, chunk);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   }
   return SXD_OK;

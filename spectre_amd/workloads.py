@@ -713,6 +713,8 @@ class MortarBrick:
 
     def _records(self):
         W = [2 * b for b in self.base]
+        LO = np.array([b[0] for b in self.boxes])
+        LN = np.array([b[1] for b in self.boxes])
         recs = []
         for e, (lo, ln) in enumerate(self.boxes):
             perm, sg = self.rotations[e]
@@ -722,13 +724,11 @@ class MortarBrick:
                 side_p = side_l if sg[j] > 0 else 1 - side_l
                 plane = (lo[P] + ln[P]) % W[P] if side_p else lo[P] % W[P]
                 tang = [d for d in range(3) if d != P]
-                found = []
-                for en, (nlo, nln) in enumerate(self.boxes):
-                    nplane = nlo[P] % W[P] if side_p else (nlo[P] + nln[P]) % W[P]
-                    if nplane != plane:
-                        continue
-                    if all(max(lo[t], nlo[t]) < min(lo[t] + ln[t], nlo[t] + nln[t]) for t in tang):
-                        found.append(en)
+                nplane = LO[:, P] % W[P] if side_p else (LO[:, P] + LN[:, P]) % W[P]
+                hit = nplane == plane
+                for t in tang:
+                    hit &= np.maximum(lo[t], LO[:, t]) < np.minimum(lo[t] + ln[t], LO[:, t] + LN[:, t])
+                found = [int(x) for x in np.flatnonzero(hit)]
                 if not found:
                     raise ValueError(f"no neighbour of element {e} in direction {dir_}")
                 found.sort(key=lambda en: tuple(self.boxes[en][0][t] for t in reversed(tang)))

@@ -539,6 +539,19 @@ def gh_evolve_vjp(extents, neighbors, u0, gammas, inv_jac, dt, steps, gbar_final
     return out
 
 
+def gh_evolve_mixed_vjp(extents, neighbors, u0, gammas, inv_jac, dt, steps, gbar_final):
+    """gh_evolve_vjp on a mixed-p set of elements (ragged layout)."""
+    u0, gammas, inv_jac, gbar_final = (np.ascontiguousarray(a, dtype=np.float64)
+                                       for a in (u0, gammas, inv_jac, gbar_final))
+    out = np.empty_like(u0)
+    ext, ep = _i(np.asarray(extents).ravel())
+    nb, nbp = _i(np.asarray(neighbors).ravel())
+    _check(lib().sxo_gh_evolve_mixed_vjp(len(ext) // 3, ep, nbp, _d(u0), _d(gammas), _d(inv_jac),
+                                         ctypes.c_double(dt), int(steps), _d(gbar_final),
+                                         _d(out)))
+    return out
+
+
 # Non-conforming mortars: orientations and h-refinement.
 def half_interpolation_matrix(n_face, n_mortar, upper):
     out = np.empty((n_mortar, n_face))

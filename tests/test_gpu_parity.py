@@ -494,7 +494,7 @@ def test_invalid_arguments_are_reported():
 
 
 # ---- mixed p ------------------------------------------------------------------
-@pytest.mark.parametrize("kernel", ["mixed_batched", "mixed_per_extents"])
+@pytest.mark.parametrize("kernel", ["mixed_batched", "mixed_per_extents", "mortar"])
 @pytest.mark.parametrize("threads", [64, 256])
 def test_scalar_wave_rhs_mixed_p(kernel, threads):
     brick, u, g2, inv = wl.scalar_wave_plane_wave_mixed(seed=1, elements=(3, 3, 2))
@@ -507,7 +507,7 @@ def test_scalar_wave_rhs_mixed_p(kernel, threads):
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("kernel", ["mixed_batched", "mixed_per_extents"])
+@pytest.mark.parametrize("kernel", ["mixed_batched", "mixed_per_extents", "mortar"])
 def test_gh_rhs_mixed_p(kernel):
     brick, u, gammas, inv = wl.gh_kerr_schild_mixed(seed=2, elements=(2, 2, 2), lowest=4,
                                                     highest=9)
@@ -531,8 +531,9 @@ def test_mixed_p_full_size_batched_equals_per_extents_and_oracle():
                                  repetitions=2)
     got = host(domain.scalar_wave_rhs(d_u, d_g2, d_inv))
     assert np.array_equal(got, ref)
-    domain.set_kernel("scalar_wave", "mixed_per_extents")
-    assert np.array_equal(host(domain.scalar_wave_rhs(d_u, d_g2, d_inv)), ref)
+    for k in ("mixed_per_extents", "mortar"):
+        domain.set_kernel("scalar_wave", k)
+        assert np.array_equal(host(domain.scalar_wave_rhs(d_u, d_g2, d_inv)), ref), k
 
 
 def _face_and_mortar_sizes(brick):
@@ -562,12 +563,12 @@ def _gh_face_pass_uses_split_plan(kf, km):
     return 160 * 1024 // fused <= 160 * 1024 // split
 
 
-@pytest.mark.parametrize("kernel", ["mixed_batched", "mixed_per_extents"])
+@pytest.mark.parametrize("kernel", ["mixed_batched", "mixed_per_extents", "mortar"])
 def test_gh_rhs_mixed_p_full_size_matches_oracle(kernel):
     """GH on the full mixed-p workload (1000 elements, extents from {4..12},
     Kerr-Schild plus noise): 12 x 12 faces and mortars select the face-pass
-    variant with a neighbour mortar buffer; both launch forms equal the
-    oracle bit for bit."""
+    variant with a neighbour mortar buffer; the batched and per-extents
+    launches and the mortar organisation equal the oracle bit for bit."""
     brick, u, gammas, inv = wl.gh_kerr_schild_mixed(seed=0)
     kf, km = _face_and_mortar_sizes(brick)
     assert (kf, km) == (144, 144)
@@ -1020,6 +1021,61 @@ def test_gh_gradient_is_the_same_with_the_recompute_overlapped(mixed):
         grads.append(host(grad))
     assert np.array_equal(grads[0], grads[1])
     assert np.array_equal(grads[0], grads[2])
+
+
+def test_gh_gradient_mixed_p_config4_size_matches_central_differences():
+    """Config 4 at its stated size: J = |C_a|_2 after 20 RK3 steps on the
+    300-element mixed-p workload (extents 6..10), checkpoints every 8 steps;
+    dJ/du0 against central differences (eps = 1e-6) on 4 seeded directions,
+    relative error <= 1e-7."""
+    from spectre_amd import evolution as ev
+    brick, u, gammas, inv, dt = wl.gh_adjoint_workload_mixed()
+    assert brick.num_elements == 300
+    domain = dg.MixedDomain.from_brick(brick)
+    d_g, d_inv = dev(gammas), dev(inv)
+    index = ev.ragged_point_index(brick.extents, 50).to(d_inv.device)
+    mass = ev.ragged_mass_weights(brick.extents, d_inv)
+    u0 = dev(u).requires_grad_(True)
+    uf = ev.GhEvolution.apply(u0, domain, d_g, d_inv, dt, 20, 8)
+    ev.ragged_gauge_constraint_norm(uf, index, mass).backward()
+    grad = u0.grad.detach()
+    rng = np.random.default_rng(12)
+    eps = 1e-6
+    worst = 0.0
+    for i in range(4):
+        print(f"config-4 FD direction {i}", flush=True)
+        v = dev(rng.standard_normal(u.shape) * np.abs(u))
+        js = []
+        for sgn in (1.0, -1.0):
+            with torch.no_grad():
+                uf = ev.GhEvolution.apply(dev(u) + sgn * eps * v, domain, d_g, d_inv, dt, 20, 8)
+                js.append(float(ev.ragged_gauge_constraint_norm(uf, index, mass)))
+        fd = (js[0] - js[1]) / (2 * eps)
+        ad = float(torch.sum(grad * v))
+        worst = max(worst, abs(fd - ad) / max(abs(ad), 1e-300))
+    print(f"config 4 (300 el, 20 steps): worst relative FD/adjoint difference: {worst:.3e}")
+    assert worst <= 1e-7
+
+
+def test_gh_gradient_mixed_p_matches_transpose_of_cpu_evolution():
+    """Two elements of different extents (a mixed-p mortar between them):
+    the GPU gradient through 3 RK3 steps equals the oracle's reverse-mode
+    tape through the same CPU evolution, i.e. the transpose of the assembled
+    CPU operator applied to a vector."""
+    from spectre_amd import evolution as ev
+    brick, u, gammas, inv, dt = wl.gh_adjoint_workload_mixed(seed=2, elements=(2, 1, 1),
+                                                             lowest=3, highest=4, noise=1e-3)
+    assert len({tuple(x) for x in brick.extents.tolist()}) == 2
+    domain = dg.MixedDomain.from_brick(brick)
+    d_g, d_inv = dev(gammas), dev(inv)
+    u0 = dev(u).requires_grad_(True)
+    uf = ev.GhEvolution.apply(u0, domain, d_g, d_inv, dt, 3, 2)
+    gbar = np.random.default_rng(13).standard_normal(u.shape)
+    (grad,) = torch.autograd.grad(uf, u0, grad_outputs=dev(gbar))
+    ref = ol.gh_evolve_mixed_vjp(brick.extents, brick.neighbors, u, gammas, inv, dt, 3, gbar)
+    err = rel_err(host(grad), ref)
+    print(f"mixed-p GH evolution gradient vs tape: {err:.3e}")
+    assert err <= 1e-12
 
 
 def test_gh_gradient_matches_transpose_of_cpu_evolution():
