@@ -495,6 +495,47 @@ enum {
 };
 enum { SXD_SYSTEM_SCALAR_WAVE = 0, SXD_SYSTEM_GH = 1 };
 
+/* GH constraint-preserving Bjorhus boundary condition, ConstraintPreserving
+ * type on a static mesh: gh::BoundaryConditions::ConstraintPreservingBjorhus::
+ * dg_time_derivative (Evolution/Systems/GeneralizedHarmonic/
+ * BoundaryConditions/Bjorhus.cpp:67-383, BjorhusImpl.cpp:9-160, 497-532).
+ * Point-wise on num_points points, every argument a device array in the
+ * reference's tensor storage (component blocks of num_points): the outward
+ * unit normal covector [3], g_ab [10], Pi_ab [10], Phi_iab [30], inertial
+ * coordinates [3], gamma1, gamma2, lapse, shift [3], inverse spacetime metric
+ * [10], spacetime unit normal vector [4], three-index constraint [30], gauge
+ * source H_a [4], its spacetime derivative d_a H_b [16, a + 4 b], the volume
+ * time derivatives of g [10], Pi [10], Phi [30], and d_i Pi_ab [30],
+ * d_j Phi_iab [90, j + 3 i + 9 ab]; outputs the corrections to dt g [10],
+ * dt Pi [10], dt Phi [30].  Each point is treated as a face of its own (the
+ * corrections vanish where no speed is negative).  The
+ * ConstraintPreservingPhysical type (Weyl-tensor terms) is not built. */
+int sxd_gh_bjorhus_corrections(
+    int64_t num_points, const double* normal_covector, const double* spacetime_metric,
+    const double* pi, const double* phi, const double* coords, const double* gamma1,
+    const double* gamma2, const double* lapse, const double* shift,
+    const double* inverse_spacetime_metric, const double* spacetime_unit_normal_vector,
+    const double* three_index_constraint, const double* gauge_source,
+    const double* spacetime_deriv_gauge_source, const double* dt_spacetime_metric,
+    const double* dt_pi, const double* dt_phi, const double* d_pi, const double* d_phi,
+    double* dt_spacetime_metric_correction, double* dt_pi_correction, double* dt_phi_correction,
+    void* stream);
+
+/* Full GH DG time derivative of a uniform domain whose external faces
+ * (neighbour -1) carry the ConstraintPreservingBjorhus condition (harmonic
+ * gauge: H_a = 0): volume terms, then on each external face, one dimension
+ * after another, the Bjorhus correction of the face's points (from the
+ * volume time derivatives and derivatives at the point; no correction on a
+ * face without a negative characteristic speed), then the UpwindPenalty
+ * corrections of the internal faces (Actions/BoundaryConditionsImpl.hpp:
+ * 586-660, a time-derivative condition has no upwind term).  coords
+ * [E][3][Np] inertial coordinates (the gauge condition's 1/r).  Composite
+ * kernels; SXD_ERR_UNSUPPORTED on mixed-p and mortar domains.  Results
+ * agree with the oracle to 1e-12 (the corrections are tensor sums whose order
+ * is not the reference's). */
+int sxd_gh_rhs_bjorhus(const sxd_domain* domain, const double* u, const double* gammas,
+                       const double* inv_jac, const double* coords, double* dt, void* stream);
+
 /* Pin the kernel used by sxd_scalar_wave_rhs / sxd_gh_rhs on this domain.
  * SXD_ERR_UNSUPPORTED if the kernel cannot run these extents.  Not
  * thread-safe against concurrent RHS calls on the same domain. */

@@ -1006,7 +1006,7 @@ inline void gh_ghost_normal(T n[3], T nv[3], const T inv_sp[6]) {
 template <class T>
 void gh_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
                   const T* u, const T* inv_j, T* dt, const GhFaceTemps<T>* temps,
-                  const T* ghost = nullptr) {
+                  const T* ghost = nullptr, bool skip_external = false) {
   const size_t np = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
   // external faces first, then internal ones, each in direction order (as in
   // sw_rhs_faces)
@@ -1015,6 +1015,8 @@ void gh_rhs_faces(int e_begin, int e_end, const int ext[3], const int* nbr,
       const int dir = step % 6;
       const int en = nbr[6 * e + dir];
       if ((en < 0) != (step < 6)) continue;
+      // faces closed by a time-derivative condition get no upwind term
+      if (en < 0 && skip_external) continue;
       const int d = dir / 2, side = dir % 2;
       const Face fi = make_face(ext, d, side);
       const Face fe = make_face(ext, d, 1 - side);

@@ -489,6 +489,20 @@ int sxo_gh_rhs(int num_elements, const int* ext, const int* nbr, const double* u
   return 0;
 }
 
+// Volume terms and the internal faces' corrections only: the part of the GH
+// RHS that a time-derivative condition on the external faces (Bjorhus)
+// leaves to the upwind scheme.
+int sxo_gh_rhs_internal_faces(int num_elements, const int* ext, const int* nbr, const double* u,
+                              const double* gammas, const double* inv_j, double* dt) {
+  if (bad_extents(ext) || num_elements < 1) return -1;
+  const size_t np = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
+  std::vector<sxo::GhFaceTemps<double>> temps(static_cast<size_t>(num_elements) * np);
+  sxo::gh_rhs_volume(0, num_elements, ext, u, gammas, inv_j, dt, temps.data());
+  sxo::gh_rhs_faces(0, num_elements, ext, nbr, u, inv_j, dt, temps.data(),
+                    static_cast<const double*>(nullptr), true);
+  return 0;
+}
+
 int sxo_sw_rhs_adjoint(int num_elements, const int* ext, const int* nbr, const double* gbar,
                        const double* gamma2, const double* inv_j, double* ubar) {
   if (bad_extents(ext) || num_elements < 1) return -1;

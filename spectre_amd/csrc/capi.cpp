@@ -1299,6 +1299,51 @@ int sxd_rhs_end(const sxd_domain* domain, int system, double* dt, void* stream) 
                 __func__);
 }
 
+int sxd_gh_bjorhus_corrections(
+    int64_t num_points, const double* normal_covector, const double* spacetime_metric,
+    const double* pi, const double* phi, const double* coords, const double* gamma1,
+    const double* gamma2, const double* lapse, const double* shift,
+    const double* inverse_spacetime_metric, const double* spacetime_unit_normal_vector,
+    const double* three_index_constraint, const double* gauge_source,
+    const double* spacetime_deriv_gauge_source, const double* dt_spacetime_metric,
+    const double* dt_pi, const double* dt_phi, const double* d_pi, const double* d_phi,
+    double* dt_spacetime_metric_correction, double* dt_pi_correction, double* dt_phi_correction,
+    void* stream) {
+  if (num_points < 0) return fail(SXD_ERR_INVALID_ARGUMENT, "negative point count");
+  const double* in[19] = {normal_covector, spacetime_metric, pi, phi, coords, gamma1, gamma2,
+                          lapse, shift, inverse_spacetime_metric, spacetime_unit_normal_vector,
+                          three_index_constraint, gauge_source, spacetime_deriv_gauge_source,
+                          dt_spacetime_metric, dt_pi, dt_phi, d_pi, d_phi};
+  double* out[3] = {dt_spacetime_metric_correction, dt_pi_correction, dt_phi_correction};
+  for (const double* p : in) SXD_REQUIRE_DEVICE(p);
+  for (double* p : out) SXD_REQUIRE_DEVICE(p);
+  return finish(launch_gh_bjorhus_points(num_points, in, out, static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
+int sxd_gh_rhs_bjorhus(const sxd_domain* domain, const double* u, const double* gammas,
+                       const double* inv_jac, const double* coords, double* dt, void* stream) {
+  if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (domain->mixed)
+    return fail(SXD_ERR_UNSUPPORTED, "%s: takes uniform domains (composite kernels)", __func__);
+  SXD_REQUIRE_DEVICE(u);
+  SXD_REQUIRE_DEVICE(gammas);
+  SXD_REQUIRE_DEVICE(inv_jac);
+  SXD_REQUIRE_DEVICE(coords);
+  SXD_REQUIRE_DEVICE(dt);
+  sxd_domain* d = const_cast<sxd_domain*>(domain);
+  if (int rc = check_no_overlap(d, 50, u, dt, __func__); rc != SXD_OK) return rc;
+  Scratch& sc = stream_scratch(d, static_cast<hipStream_t>(stream));
+  const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
+  if (int rc = reserve_scratch(sc, 150 * np * d->num_elements, 12 * np * d->num_elements);
+      rc != SXD_OK)
+    return finish(rc, __func__);
+  return finish(launch_gh_rhs_bjorhus(d->num_elements, d->ext, d->d_nbr, u, gammas, inv_jac,
+                                      coords, dt, sc.du, sc.temps,
+                                      static_cast<hipStream_t>(stream)),
+                __func__);
+}
+
 int sxd_domain_set_kernel(sxd_domain* domain, int system, int kernel) {
   if (int rc = check_domain(domain); rc != SXD_OK) return rc;
   if (system != SXD_SYSTEM_SCALAR_WAVE && system != SXD_SYSTEM_GH)

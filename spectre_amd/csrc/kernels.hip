@@ -583,8 +583,12 @@ int launch_face_passes(bool gh, int num_elements, const int32_t ext[3], const in
                        hipStream_t stream) {
   // With external faces: one sweep over the external faces, then one over the
   // internal ones (the order in which the corrections reach dt).
-  const bool external = ghost != nullptr || bc_kind != kSwBcGhostArray;
-  const int phases[2] = {external ? kFacesExternal : kFacesAll, kFacesInternal};
+  // A time-derivative condition on the external faces (GH Bjorhus) has no
+  // upwind term there: the internal sweep only.
+  const bool td_only = bc_kind == kGhBcBjorhus;
+  const bool external = !td_only && (ghost != nullptr || bc_kind != kSwBcGhostArray);
+  const int phases[2] = {td_only ? kFacesInternal : external ? kFacesExternal : kFacesAll,
+                         kFacesInternal};
   for (int ph = 0; ph < (external ? 2 : 1); ++ph) {
     for (int d = 0; d < 3; ++d) {
       const int na = d == 0 ? ext[1] : ext[0];
@@ -869,6 +873,26 @@ int launch_gh_rhs_composite(int num_elements, const int32_t ext[3], const int32_
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   return launch_face_passes(true, num_elements, ext, nbr, u, scratch_temps, inv_jac, dt,
                             ghost, kSwBcGhostArray, nullptr, stream);
+}
+
+int launch_gh_rhs_bjorhus(int num_elements, const int32_t ext[3], const int32_t* nbr,
+                          const double* u, const double* gammas, const double* inv_jac,
+                          const double* coords, double* dt, double* scratch_du,
+                          double* scratch_temps, hipStream_t stream) {
+  const int np = ext[0] * ext[1] * ext[2];
+  int rc = launch_partial_derivatives(num_elements, ext, 50, u, inv_jac, scratch_du, nullptr,
+                                      stream);
+  if (rc != SXD_OK) return rc;
+  const int64_t total = static_cast<int64_t>(num_elements) * np;
+  hipLaunchKernelGGL(gh_volume_batched_kernel, dim3(static_cast<unsigned>((total + 127) / 128)),
+                     dim3(128), 0, stream, np, total, u, scratch_du, gammas, dt,
+                     scratch_temps);
+  if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  rc = launch_gh_bjorhus_faces(num_elements, ext, nbr, u, scratch_du, gammas, inv_jac, coords, dt,
+                               stream);
+  if (rc != SXD_OK) return rc;
+  return launch_face_passes(true, num_elements, ext, nbr, u, scratch_temps, inv_jac, dt, nullptr,
+                            kGhBcBjorhus, nullptr, stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -66,6 +66,9 @@ constexpr int kSwBcGhostArray = 0, kSwBcSommerfeld = 1, kSwBcBaylissTurkel = 2;
 // upwind term on the face, dt Pi += correction); ghost = coordinates for the
 // Bayliss-Turkel types.
 constexpr int kSwBcCpsrSommerfeld = 3, kSwBcCpsrFirstOrderBt = 4, kSwBcCpsrSecondOrderBt = 5;
+// GH constraint-preserving Bjorhus (a time-derivative condition, no upwind
+// term on the external faces).
+constexpr int kGhBcBjorhus = 10;
 // fused: one launch per RHS (requires sw_fused_supported); otherwise the
 // composite path, which needs scratch_du [E][15][Np].  Faces whose neighbour
 // index is negative take their exterior state by bc_kind: kSwBcGhostArray
@@ -273,6 +276,26 @@ int flux_divergence_lds_bytes(const int32_t ext[3]);
 int launch_mesh_velocity_nonconservative(int64_t n, int nc, const double* du, const double* v,
                                          double* dt, hipStream_t stream);
 const double* device_weak_flux_matrices();
+// GH constraint-preserving Bjorhus condition (bjorhus.hip).  Point-wise: in =
+// {normal [3], g [10], pi [10], phi [30], coords [3], gamma1, gamma2, lapse,
+// shift [3], inverse spacetime metric [10], normal vector [4], three-index
+// constraint [30], H [4], d H [16] (a + 4 b = d_a H_b), dt g [10], dt pi
+// [10], dt phi [30], d pi [30], d phi [90]} (component blocks of n points),
+// out = {dt g, dt pi, dt phi} corrections; every point is its own face.
+int launch_gh_bjorhus_points(int64_t n, const double* const in[19], double* const out[3],
+                             hipStream_t stream);
+// The corrections of the external faces of a uniform domain added into dt
+// (volume terms already there), one dimension after another.
+int launch_gh_bjorhus_faces(int num_elements, const int32_t ext[3], const int32_t* nbr,
+                            const double* u, const double* du, const double* gammas,
+                            const double* inv_jac, const double* coords, double* dt,
+                            hipStream_t stream);
+// The whole GH RHS of a uniform domain whose external faces carry the
+// Bjorhus condition (composite kernels; scratch as launch_gh_rhs_composite).
+int launch_gh_rhs_bjorhus(int num_elements, const int32_t ext[3], const int32_t* nbr,
+                          const double* u, const double* gammas, const double* inv_jac,
+                          const double* coords, double* dt, double* scratch_du,
+                          double* scratch_temps, hipStream_t stream);
 // The allocation-free entry point's fixed work array (noalloc.hip): its
 // capacity in doubles, and its device address on the current device.
 size_t noalloc_scratch_doubles();

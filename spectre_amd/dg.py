@@ -234,6 +234,29 @@ def gh_upwind_penalty(u_int, temps_int, n_int, u_ext, temps_ext, n_ext):
     return bc
 
 
+def gh_bjorhus_corrections(normal_covector, spacetime_metric, pi, phi, coords, gamma1, gamma2,
+                           lapse, shift, inverse_spacetime_metric, spacetime_unit_normal_vector,
+                           three_index_constraint, gauge_source, spacetime_deriv_gauge_source,
+                           dt_spacetime_metric, dt_pi, dt_phi, d_pi, d_phi):
+    """Point-wise ConstraintPreservingBjorhus corrections
+    (sxd_gh_bjorhus_corrections): device tensors [components, n] in the
+    reference's storage; returns (dt g, dt Pi, dt Phi) corrections."""
+    n = spacetime_metric.shape[-1]
+    comps = [3, 10, 10, 30, 3, 1, 1, 1, 3, 10, 4, 30, 4, 16, 10, 10, 30, 30, 90]
+    args = [normal_covector, spacetime_metric, pi, phi, coords, gamma1, gamma2, lapse, shift,
+            inverse_spacetime_metric, spacetime_unit_normal_vector, three_index_constraint,
+            gauge_source, spacetime_deriv_gauge_source, dt_spacetime_metric, dt_pi, dt_phi,
+            d_pi, d_phi]
+    for a, c in zip(args, comps):
+        _require(a, "input", (c, n) if c > 1 else (n,))
+    outs = [torch.empty((10, n), dtype=torch.float64, device=spacetime_metric.device),
+            torch.empty((10, n), dtype=torch.float64, device=spacetime_metric.device),
+            torch.empty((30, n), dtype=torch.float64, device=spacetime_metric.device)]
+    check(lib().sxd_gh_bjorhus_corrections(n, *[_ptr(a) for a in args],
+                                           *[_ptr(o) for o in outs], _stream()))
+    return tuple(outs)
+
+
 def lift_flux(bc, extent_perpendicular, magnitude_of_normal):
     """In place: bc [nc, n] *= -0.5 N (N - 1) |n| (LiftFlux.hpp:41-62)."""
     nc, n = bc.shape
@@ -456,6 +479,21 @@ class Domain:
                                          _ptr(ghost), _ptr(out), _stream()))
         return out
 
+
+    def gh_rhs_bjorhus(self, u, gammas, inv_jac, coords, out=None):
+        """GH RHS with the constraint-preserving Bjorhus condition on the
+        external faces (sxd_gh_rhs_bjorhus; uniform domains); coords: the
+        inertial coordinates, shaped like inv_jac with 3 components."""
+        _require(u, "u", self._shapes(50))
+        _require(gammas, "gammas", self._shapes(3))
+        _require(inv_jac, "inv_jac", self._shapes(9))
+        _require(coords, "coords", self._shapes(3))
+        if out is None:
+            out = torch.empty_like(u)
+        _require(out, "out", self._shapes(50))
+        check(lib().sxd_gh_rhs_bjorhus(self._handle, _ptr(u), _ptr(gammas), _ptr(inv_jac),
+                                       _ptr(coords), _ptr(out), _stream()))
+        return out
     def scalar_wave_rhs_adjoint(self, gbar, gamma2, inv_jac, out=None):
         """Transpose of scalar_wave_rhs (linear in u): out = A^T gbar."""
         _require(gbar, "gbar", self._shapes(5))
@@ -532,7 +570,7 @@ class Domain:
 __all__ = ["SpectreAmdError", "partial_derivatives", "logical_partial_derivatives",
            "partial_derivatives_adjoint",
            "scalar_wave_time_derivative", "gh_time_derivative", "scalar_wave_upwind_penalty",
-           "gh_upwind_penalty", "lift_flux", "rk3_substep", "Domain", "MixedDomain",
+           "gh_upwind_penalty", "gh_bjorhus_corrections", "lift_flux", "rk3_substep", "Domain", "MixedDomain",
            "MortarDomain"]
 
 

@@ -344,6 +344,20 @@ def gh_rhs(extents, neighbors, u, gammas, inv_jac):
     return dt
 
 
+def gh_rhs_internal_faces(extents, neighbors, u, gammas, inv_jac):
+    """Volume terms plus the internal faces' corrections (external faces
+    skipped)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    gammas = np.ascontiguousarray(gammas, dtype=np.float64)
+    inv_jac = np.ascontiguousarray(inv_jac, dtype=np.float64)
+    dt = np.empty_like(u)
+    ext, ep = _i(extents)
+    nb, nbp = _i(neighbors)
+    _check(lib().sxo_gh_rhs_internal_faces(u.shape[0], ep, nbp, _d(u), _d(gammas), _d(inv_jac),
+                                           _d(dt)))
+    return dt
+
+
 def gh_rhs_threads(threads, extents, neighbors, u, gammas, inv_jac, out=None):
     """gh_rhs with elements split over host threads (CPU baseline)."""
     u = np.ascontiguousarray(u, dtype=np.float64)

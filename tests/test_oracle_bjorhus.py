@@ -1,0 +1,82 @@
+"""Pins of the numpy restatement of the GH constraint-preserving Bjorhus
+boundary condition (oracle/gh_bjorhus.py, ConstraintPreserving type, static
+mesh):
+
+* the 48 golden vectors written from the reference's own Python test oracle
+  (tests/Unit/Evolution/Systems/GeneralizedHarmonic/BoundaryConditions/
+  Bjorhus.py) by tests/golden/make_bjorhus_vectors.py: consistent random
+  points with both signs of every characteristic speed, and points without
+  any incoming speed (corrections zero);
+* the construction itself: the corrected time derivative's characteristic
+  projections take the prescribed values -- v_plus frozen to zero and
+  v_psi, v_zero shifted by the constraint-damping terms -- where the speeds
+  are negative, and the projections of fields with positive speeds are
+  untouched.
+"""
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+from bjorhus_oracle import bjorhus
+
+HERE = pathlib.Path(__file__).resolve().parent
+CASES = json.loads((HERE / "golden" / "bjorhus_vectors.json").read_text())["cases"]
+
+
+def inputs(case):
+    i = {k: np.asarray(v, dtype=np.float64) for k, v in case["inputs"].items()}
+    P = lambda a: a[..., None]
+    return (P(i["normal_covector"]), P(i["spacetime_metric"]), P(i["pi"]), P(i["phi"]),
+            P(i["inertial_coords"]), np.array([i["gamma1"]]), np.array([i["gamma2"]]),
+            np.array([i["lapse"]]), P(i["shift"]), P(i["inverse_spacetime_metric"]),
+            P(i["spacetime_unit_normal_vector"]), P(i["three_index_constraint"]),
+            P(i["gauge_source"]), P(i["spacetime_deriv_gauge_source"]),
+            P(i["dt_spacetime_metric"]), P(i["dt_pi"]), P(i["dt_phi"]), P(i["d_pi"]),
+            P(i["d_phi"]))
+
+
+@pytest.mark.parametrize("k", range(len(CASES)))
+def test_golden_vectors_of_the_reference(k):
+    c = CASES[k]
+    dg, dpi, dphi, _ = bjorhus.corrections(*inputs(c))
+    for got, key in ((dg, "dt_spacetime_metric"), (dpi, "dt_pi"), (dphi, "dt_phi")):
+        ref = np.asarray(c[key])
+        np.testing.assert_allclose(got[..., 0], ref, rtol=0, atol=1e-13 * max(1.0, np.abs(ref).max()))
+
+
+def char_fields(dt_g, dt_pi, dt_phi, n_dn, n_up, gamma2):
+    nd = np.einsum("ip,iabp->abp", n_up, dt_phi)
+    return (dt_g, dt_phi - np.einsum("ip,abp->iabp", n_dn, nd), dt_pi + nd - gamma2 * dt_g,
+            dt_pi - nd - gamma2 * dt_g)
+
+
+def test_corrected_projections_take_the_prescribed_values():
+    checked = 0
+    for c in CASES:
+        args = inputs(c)
+        (n, g, pi, phi, x, g1, g2, lapse, shift, G, t_up, C3, H, dH, dtg, dtpi, dtphi,
+         dpi_, dphi_) = args
+        dg, dpi, dphi, speeds = bjorhus.corrections(*args)
+        if np.min(speeds) >= 0:
+            assert not np.any(dg) and not np.any(dpi) and not np.any(dphi)
+            continue
+        S = G[1:, 1:] + np.einsum("ip,jp->ijp", shift, shift) / lapse ** 2
+        n_up = np.einsum("ijp,ip->jp", S, n)
+        before = char_fields(dtg, dtpi, dtphi, n, n_up, g2)
+        after = char_fields(dtg + dg, dtpi + dpi, dtphi + dphi, n, n_up, g2)
+        lam = speeds[:, 0]
+        if lam[2] <= 0:   # v_plus frozen
+            np.testing.assert_allclose(after[2], 0.0, atol=1e-12)
+        else:
+            np.testing.assert_allclose(after[2], before[2], atol=1e-12)
+        if lam[0] <= 0:   # v_psi shifted by lambda_0 n^i C_iab
+            np.testing.assert_allclose(after[0] - before[0],
+                                       lam[0] * np.einsum("ip,iabp->abp", n_up, C3), atol=1e-12)
+        else:
+            np.testing.assert_allclose(after[0], before[0], atol=1e-12)
+        if lam[1] > 0:
+            np.testing.assert_allclose(after[1], before[1], atol=1e-12)
+        checked += 1
+    assert checked >= 24
