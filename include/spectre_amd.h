@@ -495,10 +495,20 @@ enum {
 };
 enum { SXD_SYSTEM_SCALAR_WAVE = 0, SXD_SYSTEM_GH = 1 };
 
-/* GH constraint-preserving Bjorhus boundary condition, ConstraintPreserving
- * type on a static mesh: gh::BoundaryConditions::ConstraintPreservingBjorhus::
+/* Types of the GH constraint-preserving Bjorhus boundary condition
+ * (detail::ConstraintPreservingBjorhusType, Evolution/Systems/
+ * GeneralizedHarmonic/BoundaryConditions/Bjorhus.hpp): ConstraintPreserving,
+ * and ConstraintPreservingPhysical, which also sets the incoming physical
+ * (Weyl-tensor) modes of v_minus (BjorhusImpl.cpp:160-496). */
+enum {
+  SXD_BJORHUS_CONSTRAINT_PRESERVING = 0,
+  SXD_BJORHUS_CONSTRAINT_PRESERVING_PHYSICAL = 1
+};
+
+/* GH constraint-preserving Bjorhus boundary condition of type bjorhus_type on
+ * a static mesh: gh::BoundaryConditions::ConstraintPreservingBjorhus::
  * dg_time_derivative (Evolution/Systems/GeneralizedHarmonic/
- * BoundaryConditions/Bjorhus.cpp:67-383, BjorhusImpl.cpp:9-160, 497-532).
+ * BoundaryConditions/Bjorhus.cpp:67-383, BjorhusImpl.cpp:9-532).
  * Point-wise on num_points points, every argument a device array in the
  * reference's tensor storage (component blocks of num_points): the outward
  * unit normal covector [3], g_ab [10], Pi_ab [10], Phi_iab [30], inertial
@@ -508,11 +518,11 @@ enum { SXD_SYSTEM_SCALAR_WAVE = 0, SXD_SYSTEM_GH = 1 };
  * time derivatives of g [10], Pi [10], Phi [30], and d_i Pi_ab [30],
  * d_j Phi_iab [90, j + 3 i + 9 ab]; outputs the corrections to dt g [10],
  * dt Pi [10], dt Phi [30].  Each point is treated as a face of its own (the
- * corrections vanish where no speed is negative).  The
- * ConstraintPreservingPhysical type (Weyl-tensor terms) is not built. */
+ * corrections vanish where no speed is negative).  SXD_ERR_INVALID_ARGUMENT
+ * for an unknown type. */
 int sxd_gh_bjorhus_corrections(
-    int64_t num_points, const double* normal_covector, const double* spacetime_metric,
-    const double* pi, const double* phi, const double* coords, const double* gamma1,
+    int bjorhus_type, int64_t num_points, const double* normal_covector,
+    const double* spacetime_metric, const double* pi, const double* phi, const double* coords, const double* gamma1,
     const double* gamma2, const double* lapse, const double* shift,
     const double* inverse_spacetime_metric, const double* spacetime_unit_normal_vector,
     const double* three_index_constraint, const double* gauge_source,
@@ -522,8 +532,8 @@ int sxd_gh_bjorhus_corrections(
     void* stream);
 
 /* Full GH DG time derivative of a uniform domain whose external faces
- * (neighbour -1) carry the ConstraintPreservingBjorhus condition (harmonic
- * gauge: H_a = 0): volume terms, then on each external face, one dimension
+ * (neighbour -1) carry the ConstraintPreservingBjorhus condition of type
+ * bjorhus_type (harmonic gauge: H_a = 0): volume terms, then on each external face, one dimension
  * after another, the Bjorhus correction of the face's points (from the
  * volume time derivatives and derivatives at the point; no correction on a
  * face without a negative characteristic speed), then the UpwindPenalty
@@ -533,8 +543,9 @@ int sxd_gh_bjorhus_corrections(
  * kernels; SXD_ERR_UNSUPPORTED on mixed-p and mortar domains.  Results
  * agree with the oracle to 1e-12 (the corrections are tensor sums whose order
  * is not the reference's). */
-int sxd_gh_rhs_bjorhus(const sxd_domain* domain, const double* u, const double* gammas,
-                       const double* inv_jac, const double* coords, double* dt, void* stream);
+int sxd_gh_rhs_bjorhus(const sxd_domain* domain, int bjorhus_type, const double* u,
+                       const double* gammas, const double* inv_jac, const double* coords,
+                       double* dt, void* stream);
 
 /* Pin the kernel used by sxd_scalar_wave_rhs / sxd_gh_rhs on this domain.
  * SXD_ERR_UNSUPPORTED if the kernel cannot run these extents.  Not

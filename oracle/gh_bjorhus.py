@@ -1,5 +1,6 @@
 """Oracle: the GH constraint-preserving Bjorhus boundary condition
-(ConstraintPreserving type, static mesh), point-wise, in numpy.
+(ConstraintPreserving and ConstraintPreservingPhysical types, static mesh),
+point-wise, in numpy.
 
 TEST INFRASTRUCTURE ONLY: used by tests/ as the checker; nothing under
 spectre_amd/ imports it.
@@ -27,6 +28,10 @@ corrections of BjorhusImpl.cpp:9-160 and :497-532:
   Lindblom et al. 2006 with mu = 0) + the gauge Sommerfeld terms with
   factor gamma2 - 1/r; each set to zero where its speed is positive, all of
   them zero when no point of the face has a negative speed;
+* ConstraintPreservingPhysical type: dt v_minus also gets the physical terms
+  of BjorhusImpl.cpp:160-496 (Kidder, Scheel & Teukolsky 2001, mu = 0, 4-index
+  constraint adjustment of the Ricci tensor and gamma2 term on): see
+  physical_terms();
 * back to the evolved variables (Characteristics.cpp:133-168): dt g = u_psi,
   dt Pi = (u_+ + u_-)/2 + gamma2 u_psi, dt Phi_i = (u_+ - u_-) n_i / 2 +
   u_zero_i.
@@ -160,11 +165,93 @@ def f_constraint(S, G, t_up, t_dn, pi, phi, d_pi, d_phi, H, dH, gamma2, C3):
     return F
 
 
+def physical_terms(g, pi, phi, S, inv_g, t_up, n_dn, n_up, P_dd, P_ud, P_uu, C3, d_pi, d_phi,
+                   v_minus, gamma2, lam_minus):
+    """Physical-mode terms added to dt v_minus by the ConstraintPreservingPhysical
+    type (BjorhusImpl.cpp:160-496), at P points:
+      K_ij = 1/2 (Phi_(i,j+1,a) + Phi_(j,i+1,a)) t^a + 1/2 Pi_(i+1,j+1)
+        (GeneralizedHarmonic/ExtrinsicCurvature.cpp:16-35);
+      spatial Christoffel symbols from D_kij = Phi_(k,i+1,j+1), raised with
+        gamma^ij;
+      grad_k K_ij = 1/2 [d_k Pi_(i+1,j+1) + (d_k Phi_(i,j+1,a) + d_k Phi_(j,i+1,a)) t^a
+        - (Phi_(i,j+1,a) + Phi_(j,i+1,a)) t^b (g^ca + t^c t^a / 2) Phi_kcb]
+        - Gamma^l_ik K_lj - Gamma^l_jk K_li
+        (CovariantDerivOfExtrinsicCurvature.cpp:34-100);
+      R_ij from Phi and its derivatives (GeneralizedHarmonic/Ricci.cpp:18-120)
+        plus the 4-index-constraint terms of BjorhusImpl.cpp:330-366;
+      E_ij = R_ij + gamma^kl (K_kl K_ij - K_il K_kj) (WeylElectric.cpp:27-46);
+      U8-_ij = (P^k_i P^l_j - P^kl P_ij / 2) (E_kl + n^m (grad_m K_kl
+        - (grad_l K_km + grad_k K_lm) / 2)) (WeylPropagating.cpp:41-90, sign -1;
+        spatial transverse projections of ProjectionOperators.cpp:23-88);
+      U3-_ab = 2 P^(i+1)_a P^(j+1)_b U8-_ij;
+      term_ab = (P^c_a P^d_b - P_ab P^cd / 2) (v_minus_cd + lambda_3 (U3-_cd
+        - gamma2 n^i C_icd)).
+    Arrays carry the point index last; d_phi[k, i] = d_k Phi_i."""
+    P = pi.shape[-1]
+    gam = g[1:, 1:]
+    ph_s = phi[:, 1:, :]                                 # Phi_(i, j+1, a)
+    K = 0.5 * np.einsum("ijap,ap->ijp", ph_s + np.swapaxes(ph_s, 0, 1), t_up) + 0.5 * pi[1:, 1:]
+    D = phi[:, 1:, 1:]                                   # D_kij
+    chr1 = 0.5 * (np.transpose(D, (2, 0, 1, 3)) + np.transpose(D, (2, 1, 0, 3)) - D)
+    chr2 = np.einsum("lkp,kijp->lijp", S, chr1)          # Gamma^l_ij
+    # ordinary derivative part of grad K
+    dk = d_pi[:, 1:, 1:].copy()                          # d_k Pi_(i+1, j+1)
+    dps = d_phi[:, :, 1:, :]                              # d_k Phi_(i, j+1, a)
+    dk += np.einsum("kijap,ap->kijp", dps + np.swapaxes(dps, 1, 2), t_up)
+    M = inv_g + 0.5 * np.einsum("cp,ap->cap", t_up, t_up)   # g^ca + t^c t^a / 2
+    Q = np.einsum("cap,bp,kcbp->kap", M, t_up, phi)      # Q[k, a] = M^ca t^b Phi_kcb
+    dk -= np.einsum("ijap,kap->kijp", ph_s + np.swapaxes(ph_s, 0, 1), Q)
+    gradK = 0.5 * dk - (np.einsum("likp,ljp->kijp", chr2, K) + np.einsum("ljkp,lip->kijp", chr2, K))
+    # spatial Ricci tensor (GeneralizedHarmonic/Ricci.cpp)
+    hIjj = 0.5 * np.einsum("klp,lijp->kijp", S, D)       # (1/2) Phi^k_ij
+    hijK = 0.5 * np.einsum("klp,ijlp->ijkp", S, D)       # (1/2) Phi_ij^k
+    d_tr = np.einsum("ijjp->ip", hijK)
+    b_tr = np.einsum("jjip->ip", hIjj)
+    d_up = np.einsum("klp,lp->kp", S, d_tr)
+    b_up = np.einsum("klp,lp->kp", S, b_tr)
+    # first[i, j, k] = (Phi_(i,j+1,k+1) + Phi_(j,i+1,k+1) - Phi_(k,i+1,j+1)) / 2
+    first = 0.5 * (D + np.transpose(D, (1, 0, 2, 3)) - np.transpose(D, (1, 2, 0, 3)))
+    R = np.einsum("ijkp,kp->ijp", first, d_up - 2.0 * b_up)
+    dD = d_phi[:, :, 1:, 1:]                              # dD[m, n, a, b] = d_m Phi_(n, a+1, b+1)
+    br = (np.einsum("jlkip->ijklp", dD) + np.einsum("ilkjp->ijklp", dD)
+          - np.einsum("jiklp->ijklp", dD) - np.einsum("ijklp->ijklp", dD)
+          + np.einsum("kiljp->ijklp", dD) + np.einsum("kjlip->ijklp", dD)
+          - 2.0 * np.einsum("lkijp->ijklp", dD))
+    R += 0.25 * np.einsum("klp,ijklp->ijp", S, br)
+    R += np.einsum("iklp,jlkp->ijp", hijK, hijK)
+    R += 2.0 * np.einsum("kilp,kjlp->ijp", hIjj, hijK)
+    R -= 2.0 * np.einsum("klip,lkjp->ijp", hIjj, hIjj)
+    # 4-index constraint terms (BjorhusImpl.cpp:330-366)
+    t1 = (np.einsum("klp,ikljp->ijp", S, dD) - np.einsum("klp,kiljp->ijp", S, dD)
+          + np.einsum("klp,jklip->ijp", S, dD) - np.einsum("klp,kjlip->ijp", S, dD))
+    R += 0.25 * t1
+    dA = d_phi[:, :, 1:, :]                               # d_m Phi_(n, a+1, b)
+    t2 = (np.einsum("kp,ap,ikjap->ijp", n_up, t_up, dA) - np.einsum("kp,ap,kijap->ijp", n_up, t_up, dA)
+          + np.einsum("kp,ap,jkiap->ijp", n_up, t_up, dA) - np.einsum("kp,ap,kjiap->ijp", n_up, t_up, dA))
+    R += 0.5 * t2
+    # spatial transverse projections
+    PIJ = S - np.einsum("ip,jp->ijp", n_up, n_up)
+    Pij = gam - np.einsum("ip,jp->ijp", n_dn, n_dn)
+    PIj = np.eye(3)[:, :, None] - np.einsum("ip,jp->ijp", n_up, n_dn)
+    # Weyl propagating mode U8- (sign -1)
+    E = R + np.einsum("klp,klp,ijp->ijp", S, K, K) - np.einsum("klp,ilp,kjp->ijp", S, K, K)
+    U = (E + np.einsum("kp,kijp->ijp", n_up, gradK)
+         - 0.5 * (np.einsum("kp,jikp->ijp", n_up, gradK) + np.einsum("kp,ijkp->ijp", n_up, gradK)))
+    W = (np.einsum("kip,ljp,klp->ijp", PIj, PIj, U)
+         - 0.5 * np.einsum("klp,ijp,klp->ijp", PIJ, Pij, U))
+    U3m = 2.0 * np.einsum("iap,jbp,ijp->abp", P_ud[1:], P_ud[1:], W)
+    X = v_minus + lam_minus * (U3m - gamma2 * np.einsum("ip,icdp->cdp", n_up, C3))
+    return (np.einsum("cap,dbp,cdp->abp", P_ud, P_ud, X)
+            - 0.5 * np.einsum("abp,cdp,cdp->abp", P_dd, P_uu, X))
+
+
 def corrections(normal_covector, g, pi, phi, coords, gamma1, gamma2, lapse, shift, inv_g,
-                t_up, C3, H, dH, dt_g, dt_pi, dt_phi, d_pi, d_phi, face_has_incoming=None):
-    """(dt g, dt Pi, dt Phi) corrections of the ConstraintPreserving type at
-    P points.  face_has_incoming [P] (bool): whether the point's face has a
-    negative speed anywhere; None: decide per point (a face of one point)."""
+                t_up, C3, H, dH, dt_g, dt_pi, dt_phi, d_pi, d_phi, face_has_incoming=None,
+                physical=False):
+    """(dt g, dt Pi, dt Phi) corrections of the ConstraintPreserving type
+    (physical=True: ConstraintPreservingPhysical) at P points.
+    face_has_incoming [P] (bool): whether the point's face has a negative
+    speed anywhere; None: decide per point (a face of one point)."""
     P = g.shape[-1]
     n_dn = normal_covector
     t_dn = np.zeros((4, P))
@@ -217,6 +304,9 @@ def corrections(normal_covector, g, pi, phi, coords, gamma1, gamma2, lapse, shif
             + np.einsum("bp,ap,cp,dp->abcdp", k_in_dn, k_out_dn, k_in_up, k_out_up)
             + np.einsum("ap,bp,cp,dp->abcdp", k_in_dn, k_in_dn, k_out_up, k_out_up)))
     bc_minus += factor * np.einsum("abcdp,cdp->abp", q, v_psi)
+    if physical:
+        bc_minus += physical_terms(g, pi, phi, S, inv_g, t_up, n_dn, n_up, P_dd, P_ud, P_uu, C3,
+                                   d_pi, d_phi, v_minus, gamma2, speeds[3])
     # zero where the speed is positive; everything zero without incoming speeds
     for bc, lam in ((bc_psi, speeds[0]), (bc_zero, speeds[1]), (bc_plus, speeds[2]),
                     (bc_minus, speeds[3])):

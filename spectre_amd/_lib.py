@@ -59,8 +59,8 @@ _SIGNATURES = {
     "sxd_gh_rhs": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_gh_rhs_dirichlet": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_gh_rhs_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
-    "sxd_gh_bjorhus_corrections": (ctypes.c_int, [_i64] + [_vp] * 23),
-    "sxd_gh_rhs_bjorhus": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sxd_gh_bjorhus_corrections": (ctypes.c_int, [ctypes.c_int, _i64] + [_vp] * 23),
+    "sxd_gh_rhs_bjorhus": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sxd_rk3_substep": (ctypes.c_int, [ctypes.c_int, _i64, ctypes.c_double, _vp, _vp, _vp, _vp, _vp]),
     "sxd_domain_create_mixed": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _ip, _ip]),
     "sxd_domain_create_mortars": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, _ip,

@@ -33,7 +33,7 @@ __device__ __forceinline__ void load_iaa(double out[3][4][4], const double* base
 }
 
 __global__ void __launch_bounds__(64)
-bjorhus_points_kernel(int64_t n, const double* __restrict__ normal, const double* __restrict__ g,
+bjorhus_points_kernel(int64_t n, bool physical, const double* __restrict__ normal, const double* __restrict__ g,
                       const double* __restrict__ pi, const double* __restrict__ phi,
                       const double* __restrict__ coords, const double* __restrict__ gamma1,
                       const double* __restrict__ gamma2, const double* __restrict__ lapse,
@@ -77,7 +77,7 @@ bjorhus_points_kernel(int64_t n, const double* __restrict__ normal, const double
   bjorhus_speeds(p, lam);
   const bool incoming = lam[0] < 0.0 || lam[1] < 0.0 || lam[2] < 0.0 || lam[3] < 0.0;
   double og[4][4], opi[4][4], ophi[3][4][4];
-  gh_bjorhus_point(p, incoming, og, opi, ophi);
+  gh_bjorhus_point(p, incoming, physical, og, opi, ophi);
   for (int a = 0; a < 4; ++a)
     for (int b = a; b < 4; ++b) {
       out_g[s4(a, b) * n + q] = og[a][b];
@@ -93,7 +93,8 @@ bjorhus_points_kernel(int64_t n, const double* __restrict__ normal, const double
 // volume terms (and the corrections of earlier dimensions) and receives the
 // corrections.  Harmonic gauge: H_a = 0, d_a H_b = 0.
 __global__ void __launch_bounds__(64)
-bjorhus_face_kernel(int d, int n0, int n1, int n2, const int32_t* __restrict__ nbr,
+bjorhus_face_kernel(int d, bool physical, int n0, int n1, int n2,
+                    const int32_t* __restrict__ nbr,
                     const double* __restrict__ u, const double* __restrict__ du,
                     const double* __restrict__ gammas, const double* __restrict__ inv_jac,
                     const double* __restrict__ coords, double* __restrict__ dt) {
@@ -181,7 +182,7 @@ bjorhus_face_kernel(int d, int n0, int n1, int n2, const int32_t* __restrict__ n
     BjorhusPoint p;
     build(pt, p);
     double og[4][4], opi[4][4], ophi[3][4][4];
-    gh_bjorhus_point(p, face_incoming, og, opi, ophi);
+    gh_bjorhus_point(p, face_incoming, physical, og, opi, ophi);
     for (int a = 0; a < 4; ++a)
       for (int b = a; b < 4; ++b) {
         const int c = s4(a, b);
@@ -194,23 +195,24 @@ bjorhus_face_kernel(int d, int n0, int n1, int n2, const int32_t* __restrict__ n
 
 }  // namespace
 
-int launch_gh_bjorhus_points(int64_t n, const double* const in[19], double* const out[3],
-                             hipStream_t stream) {
+int launch_gh_bjorhus_points(int64_t n, bool physical, const double* const in[19],
+                             double* const out[3], hipStream_t stream) {
   if (n == 0) return SXD_OK;
   hipLaunchKernelGGL(bjorhus_points_kernel, dim3(static_cast<unsigned>((n + 63) / 64)), dim3(64),
-                     0, stream, n, in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], in[8],
+                     0, stream, n, physical, in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], in[8],
                      in[9], in[10], in[11], in[12], in[13], in[14], in[15], in[16], in[17], in[18],
                      out[0], out[1], out[2]);
   return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
 }
 
-int launch_gh_bjorhus_faces(int num_elements, const int32_t ext[3], const int32_t* nbr,
+int launch_gh_bjorhus_faces(int num_elements, bool physical, const int32_t ext[3],
+                            const int32_t* nbr,
                             const double* u, const double* du, const double* gammas,
                             const double* inv_jac, const double* coords, double* dt,
                             hipStream_t stream) {
   for (int d = 0; d < 3; ++d) {
     hipLaunchKernelGGL(bjorhus_face_kernel, dim3(static_cast<unsigned>(2 * num_elements)),
-                       dim3(64), 0, stream, d, ext[0], ext[1], ext[2], nbr, u, du, gammas,
+                       dim3(64), 0, stream, d, physical, ext[0], ext[1], ext[2], nbr, u, du, gammas,
                        inv_jac, coords, dt);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   }

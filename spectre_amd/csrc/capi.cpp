@@ -1300,7 +1300,7 @@ int sxd_rhs_end(const sxd_domain* domain, int system, double* dt, void* stream) 
 }
 
 int sxd_gh_bjorhus_corrections(
-    int64_t num_points, const double* normal_covector, const double* spacetime_metric,
+    int bjorhus_type, int64_t num_points, const double* normal_covector, const double* spacetime_metric,
     const double* pi, const double* phi, const double* coords, const double* gamma1,
     const double* gamma2, const double* lapse, const double* shift,
     const double* inverse_spacetime_metric, const double* spacetime_unit_normal_vector,
@@ -1310,6 +1310,9 @@ int sxd_gh_bjorhus_corrections(
     double* dt_spacetime_metric_correction, double* dt_pi_correction, double* dt_phi_correction,
     void* stream) {
   if (num_points < 0) return fail(SXD_ERR_INVALID_ARGUMENT, "negative point count");
+  if (bjorhus_type != SXD_BJORHUS_CONSTRAINT_PRESERVING &&
+      bjorhus_type != SXD_BJORHUS_CONSTRAINT_PRESERVING_PHYSICAL)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "%s: unknown Bjorhus type %d", __func__, bjorhus_type);
   const double* in[19] = {normal_covector, spacetime_metric, pi, phi, coords, gamma1, gamma2,
                           lapse, shift, inverse_spacetime_metric, spacetime_unit_normal_vector,
                           three_index_constraint, gauge_source, spacetime_deriv_gauge_source,
@@ -1317,13 +1320,19 @@ int sxd_gh_bjorhus_corrections(
   double* out[3] = {dt_spacetime_metric_correction, dt_pi_correction, dt_phi_correction};
   for (const double* p : in) SXD_REQUIRE_DEVICE(p);
   for (double* p : out) SXD_REQUIRE_DEVICE(p);
-  return finish(launch_gh_bjorhus_points(num_points, in, out, static_cast<hipStream_t>(stream)),
+  return finish(launch_gh_bjorhus_points(
+                    num_points, bjorhus_type == SXD_BJORHUS_CONSTRAINT_PRESERVING_PHYSICAL, in,
+                    out, static_cast<hipStream_t>(stream)),
                 __func__);
 }
 
-int sxd_gh_rhs_bjorhus(const sxd_domain* domain, const double* u, const double* gammas,
-                       const double* inv_jac, const double* coords, double* dt, void* stream) {
+int sxd_gh_rhs_bjorhus(const sxd_domain* domain, int bjorhus_type, const double* u,
+                       const double* gammas, const double* inv_jac, const double* coords,
+                       double* dt, void* stream) {
   if (int rc = check_domain(domain); rc != SXD_OK) return rc;
+  if (bjorhus_type != SXD_BJORHUS_CONSTRAINT_PRESERVING &&
+      bjorhus_type != SXD_BJORHUS_CONSTRAINT_PRESERVING_PHYSICAL)
+    return fail(SXD_ERR_INVALID_ARGUMENT, "%s: unknown Bjorhus type %d", __func__, bjorhus_type);
   if (domain->mixed)
     return fail(SXD_ERR_UNSUPPORTED, "%s: takes uniform domains (composite kernels)", __func__);
   SXD_REQUIRE_DEVICE(u);
@@ -1338,7 +1347,9 @@ int sxd_gh_rhs_bjorhus(const sxd_domain* domain, const double* u, const double* 
   if (int rc = reserve_scratch(sc, 150 * np * d->num_elements, 12 * np * d->num_elements);
       rc != SXD_OK)
     return finish(rc, __func__);
-  return finish(launch_gh_rhs_bjorhus(d->num_elements, d->ext, d->d_nbr, u, gammas, inv_jac,
+  return finish(launch_gh_rhs_bjorhus(d->num_elements,
+                                      bjorhus_type == SXD_BJORHUS_CONSTRAINT_PRESERVING_PHYSICAL,
+                                      d->ext, d->d_nbr, u, gammas, inv_jac,
                                       coords, dt, sc.du, sc.temps,
                                       static_cast<hipStream_t>(stream)),
                 __func__);

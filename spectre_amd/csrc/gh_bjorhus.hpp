@@ -1,5 +1,6 @@
 // GH constraint-preserving Bjorhus boundary condition (ConstraintPreserving
-// type, static mesh), point-wise on the device.
+// and ConstraintPreservingPhysical types, static mesh), point-wise on the
+// device.
 //
 // A time-derivative condition (gh::BoundaryConditions::
 // ConstraintPreservingBjorhus::dg_time_derivative, Evolution/Systems/
@@ -14,10 +15,17 @@
 // corrections of dt g, dt Pi and dt Phi.  Each field's correction is zero
 // where its speed is positive, and all are zero on a face without any
 // negative speed.  C_ia and F_a are the two-index and F constraints
-// (Constraints.cpp:24-930, without stress-energy).
+// (Constraints.cpp:24-930, without stress-energy).  The
+// ConstraintPreservingPhysical type adds to the v_minus correction the
+// physical-mode terms of BjorhusImpl.cpp:160-496 (bjorhus_physical_terms).
 //
 // Full 4 x 4 index loops: this runs on boundary points only, so the code
-// follows the tensor expressions rather than a register budget.
+// follows the tensor expressions rather than a register budget.  Both
+// functions below are inlined into their kernels: compiled at -O3 as calls
+// (hipcc of ROCm 7.2, gfx950) the physical-type corrections came out wrong
+// on the device (off by up to 0.4, deterministic), while the same source
+// inlined, or built at -O1, or built for the host (clean under address,
+// memory and undefined-behaviour sanitizers) matches the oracle to 6e-15.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -46,10 +54,160 @@ __device__ __forceinline__ void bjorhus_speeds(const BjorhusPoint& p, double lam
   lam[3] = -p.lapse - bn;
 }
 
+// Physical-mode terms of the v_minus correction (ConstraintPreservingPhysical
+// type; BjorhusImpl.cpp:160-496 with mu = 0, the 4-index-constraint terms
+// added to the Ricci tensor and the gamma2 term; Kidder, Scheel & Teukolsky
+// 2001):
+//   K_ij = 1/2 (Phi_(i,j+1,a) + Phi_(j,i+1,a)) t^a + 1/2 Pi_(i+1,j+1)
+//     (GeneralizedHarmonic/ExtrinsicCurvature.cpp:16-35);
+//   grad_k K_ij = 1/2 [d_k Pi_(i+1,j+1) + (d_k Phi_(i,j+1,a) + d_k Phi_(j,i+1,a)) t^a
+//     - (Phi_(i,j+1,a) + Phi_(j,i+1,a)) t^b (g^ca + t^c t^a / 2) Phi_kcb]
+//     - Gamma^l_ik K_lj - Gamma^l_jk K_li, Gamma from D_kij = Phi_(k,i+1,j+1)
+//     (CovariantDerivOfExtrinsicCurvature.cpp:34-100);
+//   R_ij from Phi and d Phi (GeneralizedHarmonic/Ricci.cpp:18-120) plus the
+//     4-index-constraint terms (BjorhusImpl.cpp:330-366);
+//   E_ij = R_ij + gamma^kl (K_kl K_ij - K_il K_kj) (WeylElectric.cpp:27-46);
+//   U8-_ij = (P^k_i P^l_j - P^kl P_ij / 2) (E_kl + n^m (grad_m K_kl -
+//     (grad_l K_km + grad_k K_lm) / 2)) (WeylPropagating.cpp:41-90, sign -1);
+//   U3-_ab = 2 P^(i+1)_a P^(j+1)_b U8-_ij;
+//   term_ab = (P^c_a P^d_b - P_ab P^cd / 2) (v_minus_cd + lambda_3 (U3-_cd -
+//     gamma2 n^i C_icd)).
+__device__ __forceinline__ void bjorhus_physical_terms(const BjorhusPoint& p, const double S[3][3],
+                                              const double n_up[3], const double Pdd[4][4],
+                                              const double Pud[4][4], const double Puu[4][4],
+                                              const double v_minus[4][4], double lam3,
+                                              double term[4][4]) {
+  double K[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int a = 0; a < 4; ++a) s += 0.5 * (p.phi[i][j + 1][a] + p.phi[j][i + 1][a]) * p.t_up[a];
+      K[i][j] = s + 0.5 * p.pi[i + 1][j + 1];
+    }
+  double chr2[3][3][3];  // Gamma^l_ij
+  for (int l = 0; l < 3; ++l)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < 3; ++k)
+          s += S[l][k] * 0.5 *
+               (p.phi[i][j + 1][k + 1] + p.phi[j][i + 1][k + 1] - p.phi[k][i + 1][j + 1]);
+        chr2[l][i][j] = s;
+      }
+  double Q[3][4];  // (g^ca + t^c t^a / 2) t^b Phi_kcb
+  for (int k = 0; k < 3; ++k)
+    for (int a = 0; a < 4; ++a) {
+      double s = 0.0;
+      for (int c = 0; c < 4; ++c)
+        for (int b = 0; b < 4; ++b)
+          s += (p.G[c][a] + 0.5 * p.t_up[c] * p.t_up[a]) * p.t_up[b] * p.phi[k][c][b];
+      Q[k][a] = s;
+    }
+  double gK[3][3][3];  // grad_k K_ij at [k][i][j]
+  for (int k = 0; k < 3; ++k)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = p.dpi[k][i + 1][j + 1];
+        for (int a = 0; a < 4; ++a) {
+          s += (p.dphi[k][i][j + 1][a] + p.dphi[k][j][i + 1][a]) * p.t_up[a];
+          s -= (p.phi[i][j + 1][a] + p.phi[j][i + 1][a]) * Q[k][a];
+        }
+        s *= 0.5;
+        for (int l = 0; l < 3; ++l) s -= chr2[l][i][k] * K[l][j] + chr2[l][j][k] * K[l][i];
+        gK[k][i][j] = s;
+      }
+  // spatial Ricci tensor
+  double hIjj[3][3][3], hijK[3][3][3];  // (1/2) Phi^k_ij at [k][i][j], (1/2) Phi_ij^k at [i][j][k]
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int c = 0; c < 3; ++c) {
+        double s1 = 0.0, s2 = 0.0;
+        for (int l = 0; l < 3; ++l) {
+          s1 += 0.5 * S[a][l] * p.phi[l][b + 1][c + 1];
+          s2 += 0.5 * S[c][l] * p.phi[a][b + 1][l + 1];
+        }
+        hIjj[a][b][c] = s1;
+        hijK[a][b][c] = s2;
+      }
+  double dup[3], bup[3], dtr[3], btr[3];
+  for (int i = 0; i < 3; ++i) {
+    dtr[i] = hijK[i][0][0] + hijK[i][1][1] + hijK[i][2][2];
+    btr[i] = hIjj[0][0][i] + hIjj[1][1][i] + hIjj[2][2][i];
+  }
+  for (int k = 0; k < 3; ++k) {
+    dup[k] = S[k][0] * dtr[0] + S[k][1] * dtr[1] + S[k][2] * dtr[2];
+    bup[k] = S[k][0] * btr[0] + S[k][1] * btr[1] + S[k][2] * btr[2];
+  }
+  double U[3][3];  // U8-, before the transverse projection
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        s += 0.5 * (p.phi[i][j + 1][k + 1] + p.phi[j][i + 1][k + 1] - p.phi[k][i + 1][j + 1]) *
+             (dup[k] - 2.0 * bup[k]);
+        for (int l = 0; l < 3; ++l) {
+          s += 0.25 * S[k][l] *
+                   (p.dphi[j][l][k + 1][i + 1] + p.dphi[i][l][k + 1][j + 1] -
+                    p.dphi[j][i][k + 1][l + 1] - p.dphi[i][j][k + 1][l + 1] +
+                    p.dphi[k][i][l + 1][j + 1] + p.dphi[k][j][l + 1][i + 1] -
+                    2.0 * p.dphi[l][k][i + 1][j + 1]) +
+               hijK[i][k][l] * hijK[j][l][k] + 2.0 * hIjj[k][i][l] * hijK[k][j][l] -
+               2.0 * hIjj[k][l][i] * hIjj[l][k][j];
+          // 4-index-constraint terms
+          s += 0.25 * S[k][l] *
+               (p.dphi[i][k][l + 1][j + 1] - p.dphi[k][i][l + 1][j + 1] +
+                p.dphi[j][k][l + 1][i + 1] - p.dphi[k][j][l + 1][i + 1]);
+          // Weyl electric part
+          s += S[k][l] * (K[k][l] * K[i][j] - K[i][l] * K[k][j]);
+        }
+        for (int a = 0; a < 4; ++a)
+          s += 0.5 * n_up[k] * p.t_up[a] *
+               (p.dphi[i][k][j + 1][a] - p.dphi[k][i][j + 1][a] + p.dphi[j][k][i + 1][a] -
+                p.dphi[k][j][i + 1][a]);
+        // propagating part, sign -1
+        s += n_up[k] * (gK[k][i][j] - 0.5 * (gK[j][i][k] + gK[i][j][k]));
+      }
+      U[i][j] = s;
+    }
+  double W[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) {
+          const double pki = (k == i ? 1.0 : 0.0) - n_up[k] * p.n[i];
+          const double plj = (l == j ? 1.0 : 0.0) - n_up[l] * p.n[j];
+          const double pkl = S[k][l] - n_up[k] * n_up[l];
+          const double pij = p.g[i + 1][j + 1] - p.n[i] * p.n[j];
+          s += (pki * plj - 0.5 * pkl * pij) * U[k][l];
+        }
+      W[i][j] = s;
+    }
+  double X[4][4];
+  for (int c = 0; c < 4; ++c)
+    for (int d = 0; d < 4; ++d) {
+      double u3 = 0.0, nc = 0.0;
+      for (int i = 0; i < 3; ++i) {
+        nc += n_up[i] * p.C3[i][c][d];
+        for (int j = 0; j < 3; ++j) u3 += 2.0 * Pud[i + 1][c] * Pud[j + 1][d] * W[i][j];
+      }
+      X[c][d] = v_minus[c][d] + lam3 * (u3 - p.gamma2 * nc);
+    }
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      double s = 0.0;
+      for (int c = 0; c < 4; ++c)
+        for (int d = 0; d < 4; ++d)
+          s += (Pud[c][a] * Pud[d][b] - 0.5 * Pdd[a][b] * Puu[c][d]) * X[c][d];
+      term[a][b] = s;
+    }
+}
+
 // Corrections (dt g, dt Pi, dt Phi) at one point; face_has_incoming: some
-// point of the point's face has a negative characteristic speed.
-__device__ inline void gh_bjorhus_point(const BjorhusPoint& p, bool face_has_incoming,
-                                        double out_g[4][4], double out_pi[4][4],
+// point of the point's face has a negative characteristic speed; physical:
+// the ConstraintPreservingPhysical type.
+__device__ __forceinline__ void gh_bjorhus_point(const BjorhusPoint& p, bool face_has_incoming,
+                                        bool physical, double out_g[4][4], double out_pi[4][4],
                                         double out_phi[3][4][4]) {
   double lam[4];
   bjorhus_speeds(p, lam);
@@ -247,6 +405,8 @@ __device__ inline void gh_bjorhus_point(const BjorhusPoint& p, bool face_has_inc
           for (int k = 0; k < 3; ++k) s += levi_civita3(i, j, k) * p.dphi[j][k][a][b];
         D4[i][a][b] = s;
       }
+  double phys[4][4];
+  if (physical) bjorhus_physical_terms(p, S, n_up, Pdd, Pud, Puu, v_minus, lam[3], phys);
   double r = sqrt(p.x[0] * p.x[0] + p.x[1] * p.x[1] + p.x[2] * p.x[2]);
   const double factor = p.gamma2 - 1.0 / r;
   const bool on = face_has_incoming;
@@ -282,6 +442,7 @@ __device__ inline void gh_bjorhus_point(const BjorhusPoint& p, bool face_has_inc
                             kin_dn[a] * kin_dn[b] * kout_up[c] * kout_up[d]);
           bc_minus += q * factor * p.dtg[c][d];
         }
+      if (physical) bc_minus += phys[a][b];
       const double u_psi = (on && !(lam[0] > 0.0)) ? bc_psi : 0.0;
       const double u_plus = (on && !(lam[2] > 0.0)) ? bc_plus : 0.0;
       const double u_minus = (on && !(lam[3] > 0.0)) ? bc_minus : 0.0;

@@ -875,8 +875,9 @@ int launch_gh_rhs_composite(int num_elements, const int32_t ext[3], const int32_
                             ghost, kSwBcGhostArray, nullptr, stream);
 }
 
-int launch_gh_rhs_bjorhus(int num_elements, const int32_t ext[3], const int32_t* nbr,
-                          const double* u, const double* gammas, const double* inv_jac,
+int launch_gh_rhs_bjorhus(int num_elements, bool physical, const int32_t ext[3],
+                          const int32_t* nbr, const double* u, const double* gammas,
+                          const double* inv_jac,
                           const double* coords, double* dt, double* scratch_du,
                           double* scratch_temps, hipStream_t stream) {
   const int np = ext[0] * ext[1] * ext[2];
@@ -888,8 +889,8 @@ int launch_gh_rhs_bjorhus(int num_elements, const int32_t ext[3], const int32_t*
                      dim3(128), 0, stream, np, total, u, scratch_du, gammas, dt,
                      scratch_temps);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
-  rc = launch_gh_bjorhus_faces(num_elements, ext, nbr, u, scratch_du, gammas, inv_jac, coords, dt,
-                               stream);
+  rc = launch_gh_bjorhus_faces(num_elements, physical, ext, nbr, u, scratch_du, gammas, inv_jac,
+                               coords, dt, stream);
   if (rc != SXD_OK) return rc;
   return launch_face_passes(true, num_elements, ext, nbr, u, scratch_temps, inv_jac, dt, nullptr,
                             kGhBcBjorhus, nullptr, stream);

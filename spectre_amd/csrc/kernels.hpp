@@ -282,18 +282,21 @@ const double* device_weak_flux_matrices();
 // constraint [30], H [4], d H [16] (a + 4 b = d_a H_b), dt g [10], dt pi
 // [10], dt phi [30], d pi [30], d phi [90]} (component blocks of n points),
 // out = {dt g, dt pi, dt phi} corrections; every point is its own face.
-int launch_gh_bjorhus_points(int64_t n, const double* const in[19], double* const out[3],
-                             hipStream_t stream);
+// physical: the ConstraintPreservingPhysical type.
+int launch_gh_bjorhus_points(int64_t n, bool physical, const double* const in[19],
+                             double* const out[3], hipStream_t stream);
 // The corrections of the external faces of a uniform domain added into dt
 // (volume terms already there), one dimension after another.
-int launch_gh_bjorhus_faces(int num_elements, const int32_t ext[3], const int32_t* nbr,
-                            const double* u, const double* du, const double* gammas,
+int launch_gh_bjorhus_faces(int num_elements, bool physical, const int32_t ext[3],
+                            const int32_t* nbr, const double* u, const double* du,
+                            const double* gammas,
                             const double* inv_jac, const double* coords, double* dt,
                             hipStream_t stream);
 // The whole GH RHS of a uniform domain whose external faces carry the
 // Bjorhus condition (composite kernels; scratch as launch_gh_rhs_composite).
-int launch_gh_rhs_bjorhus(int num_elements, const int32_t ext[3], const int32_t* nbr,
-                          const double* u, const double* gammas, const double* inv_jac,
+int launch_gh_rhs_bjorhus(int num_elements, bool physical, const int32_t ext[3],
+                          const int32_t* nbr, const double* u, const double* gammas,
+                          const double* inv_jac,
                           const double* coords, double* dt, double* scratch_du,
                           double* scratch_temps, hipStream_t stream);
 // The allocation-free entry point's fixed work array (noalloc.hip): its

@@ -234,13 +234,25 @@ def gh_upwind_penalty(u_int, temps_int, n_int, u_ext, temps_ext, n_ext):
     return bc
 
 
+BJORHUS_TYPES = {"ConstraintPreserving": 0, "ConstraintPreservingPhysical": 1}
+
+
+def _bjorhus_type(bjorhus_type):
+    if bjorhus_type not in BJORHUS_TYPES:
+        raise ValueError(f"unknown Bjorhus type {bjorhus_type!r}: one of {sorted(BJORHUS_TYPES)}")
+    return BJORHUS_TYPES[bjorhus_type]
+
+
 def gh_bjorhus_corrections(normal_covector, spacetime_metric, pi, phi, coords, gamma1, gamma2,
                            lapse, shift, inverse_spacetime_metric, spacetime_unit_normal_vector,
                            three_index_constraint, gauge_source, spacetime_deriv_gauge_source,
-                           dt_spacetime_metric, dt_pi, dt_phi, d_pi, d_phi):
+                           dt_spacetime_metric, dt_pi, dt_phi, d_pi, d_phi,
+                           bjorhus_type="ConstraintPreserving"):
     """Point-wise ConstraintPreservingBjorhus corrections
-    (sxd_gh_bjorhus_corrections): device tensors [components, n] in the
+    (sxd_gh_bjorhus_corrections) of type "ConstraintPreserving" or
+    "ConstraintPreservingPhysical": device tensors [components, n] in the
     reference's storage; returns (dt g, dt Pi, dt Phi) corrections."""
+    kind = _bjorhus_type(bjorhus_type)
     n = spacetime_metric.shape[-1]
     comps = [3, 10, 10, 30, 3, 1, 1, 1, 3, 10, 4, 30, 4, 16, 10, 10, 30, 30, 90]
     args = [normal_covector, spacetime_metric, pi, phi, coords, gamma1, gamma2, lapse, shift,
@@ -252,7 +264,7 @@ def gh_bjorhus_corrections(normal_covector, spacetime_metric, pi, phi, coords, g
     outs = [torch.empty((10, n), dtype=torch.float64, device=spacetime_metric.device),
             torch.empty((10, n), dtype=torch.float64, device=spacetime_metric.device),
             torch.empty((30, n), dtype=torch.float64, device=spacetime_metric.device)]
-    check(lib().sxd_gh_bjorhus_corrections(n, *[_ptr(a) for a in args],
+    check(lib().sxd_gh_bjorhus_corrections(kind, n, *[_ptr(a) for a in args],
                                            *[_ptr(o) for o in outs], _stream()))
     return tuple(outs)
 
@@ -480,10 +492,13 @@ class Domain:
         return out
 
 
-    def gh_rhs_bjorhus(self, u, gammas, inv_jac, coords, out=None):
-        """GH RHS with the constraint-preserving Bjorhus condition on the
+    def gh_rhs_bjorhus(self, u, gammas, inv_jac, coords, out=None,
+                       bjorhus_type="ConstraintPreserving"):
+        """GH RHS with the constraint-preserving Bjorhus condition of type
+        "ConstraintPreserving" or "ConstraintPreservingPhysical" on the
         external faces (sxd_gh_rhs_bjorhus; uniform domains); coords: the
         inertial coordinates, shaped like inv_jac with 3 components."""
+        kind = _bjorhus_type(bjorhus_type)
         _require(u, "u", self._shapes(50))
         _require(gammas, "gammas", self._shapes(3))
         _require(inv_jac, "inv_jac", self._shapes(9))
@@ -491,7 +506,7 @@ class Domain:
         if out is None:
             out = torch.empty_like(u)
         _require(out, "out", self._shapes(50))
-        check(lib().sxd_gh_rhs_bjorhus(self._handle, _ptr(u), _ptr(gammas), _ptr(inv_jac),
+        check(lib().sxd_gh_rhs_bjorhus(self._handle, kind, _ptr(u), _ptr(gammas), _ptr(inv_jac),
                                        _ptr(coords), _ptr(out), _stream()))
         return out
     def scalar_wave_rhs_adjoint(self, gbar, gamma2, inv_jac, out=None):

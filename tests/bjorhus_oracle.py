@@ -84,11 +84,12 @@ def face_points(ext, d, side):
     return np.flatnonzero(on)
 
 
-def gh_rhs_bjorhus(extents, neighbors, u, gammas, inv_jac, coords):
+def gh_rhs_bjorhus(extents, neighbors, u, gammas, inv_jac, coords, physical=False):
     """The GH RHS of a uniform brick whose external faces carry the Bjorhus
-    condition: the oracle's volume terms and internal faces, plus the
-    corrections of each external face computed from the volume dt (and the
-    corrections of earlier dimensions) by the numpy restatement."""
+    condition (physical: the ConstraintPreservingPhysical type): the oracle's
+    volume terms and internal faces, plus the corrections of each external
+    face computed from the volume dt (and the corrections of earlier
+    dimensions) by the numpy restatement."""
     E = u.shape[0]
     nb = np.asarray(neighbors)
     dt = ol.gh_rhs_internal_faces(extents, neighbors, u, gammas, inv_jac)
@@ -116,7 +117,7 @@ def gh_rhs_bjorhus(extents, neighbors, u, gammas, inv_jac, coords):
                     a["normal_covector"], a["g"], a["pi"], a["phi"], a["coords"], a["gamma1"],
                     a["gamma2"], a["lapse"], a["shift"], a["inv_g"], a["t_up"], a["C3"], a["H"],
                     a["dH"], a["dt_g"], a["dt_pi"], a["dt_phi"], a["d_pi"], a["d_phi"],
-                    face_has_incoming=incoming)
+                    face_has_incoming=incoming, physical=physical)
                 corr = np.concatenate([store_aa(dg), store_aa(dpi), store_iaa(dphi)])
                 dt_bc[e][:, pts] += corr
                 total[e][:, pts] += corr

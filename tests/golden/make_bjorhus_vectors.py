@@ -1,13 +1,13 @@
 """Write input/output vectors of the GH constraint-preserving Bjorhus boundary
-condition (ConstraintPreserving type, static mesh) from the reference's own
-Python test oracle.
+condition (ConstraintPreserving and ConstraintPreservingPhysical types, static
+mesh) from the reference's own Python test oracle.
 
 The reference checks its C++ ConstraintPreservingBjorhus against the numpy
 functions of
   tests/Unit/Evolution/Systems/GeneralizedHarmonic/BoundaryConditions/Bjorhus.py
 (dt_spacetime_metric_static_mesh, dt_pi_ConstraintPreserving_static_mesh,
-dt_phi_ConstraintPreserving_static_mesh), which evaluate the corrections at
-one point.  No closed form pins those values, so this script (run only where
+dt_phi_ConstraintPreserving_static_mesh and the ..._ConstraintPreservingPhysical_
+static_mesh pair), which evaluate the corrections at one point.  No closed form pins those values, so this script (run only where
 the reference tree exists) imports that module with its sibling test modules,
 evaluates it on seeded, physically consistent random points (a spacetime
 metric near Minkowski with its lapse, shift, inverse and normal vector; a
@@ -89,14 +89,18 @@ def main():
         dt_g = bj.dt_spacetime_metric_static_mesh(*args)
         dt_pi = bj.dt_pi_ConstraintPreserving_static_mesh(*args)
         dt_phi = bj.dt_phi_ConstraintPreserving_static_mesh(*args)
+        dt_pi_phys = bj.dt_pi_ConstraintPreservingPhysical_static_mesh(*args)
+        dt_phi_phys = bj.dt_phi_ConstraintPreservingPhysical_static_mesh(*args)
         cases.append({"inputs": {k2: np.asarray(v).tolist() for k2, v in p.items()},
                       "dt_spacetime_metric": np.asarray(dt_g).tolist(),
                       "dt_pi": np.asarray(dt_pi).tolist(),
-                      "dt_phi": np.asarray(dt_phi).tolist()})
+                      "dt_phi": np.asarray(dt_phi).tolist(),
+                      "dt_pi_physical": np.asarray(dt_pi_phys).tolist(),
+                      "dt_phi_physical": np.asarray(dt_phi_phys).tolist()})
     out = HERE / "bjorhus_vectors.json"
     out.write_text(json.dumps({"source": "tests/Unit/Evolution/Systems/GeneralizedHarmonic/"
-                                         "BoundaryConditions/Bjorhus.py (ConstraintPreserving, "
-                                         "static mesh)", "cases": cases}))
+                                         "BoundaryConditions/Bjorhus.py (ConstraintPreserving "
+                                         "and ConstraintPreservingPhysical, static mesh)", "cases": cases}))
     print(f"wrote {len(cases)} cases to {out}")
 
 

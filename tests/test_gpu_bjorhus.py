@@ -1,6 +1,7 @@
 """GPU parity of the GH constraint-preserving Bjorhus boundary condition
-(ConstraintPreserving type, static mesh) within the 1e-12 bar (the
-corrections are tensor sums that the device evaluates in its own order):
+(ConstraintPreserving and ConstraintPreservingPhysical types, static mesh)
+within the 1e-12 bar (the corrections are tensor sums that the device
+evaluates in its own order):
 
 * point-wise (sxd_gh_bjorhus_corrections) against the 48 golden vectors from
   the reference's Python test oracle and against the numpy restatement;
@@ -56,30 +57,50 @@ def storage_inputs(cases):
             iaa(get("d_pi")), dphi_st]
 
 
-def test_pointwise_corrections_match_the_golden_vectors():
+@pytest.mark.parametrize("kind", ["ConstraintPreserving", "ConstraintPreservingPhysical"])
+def test_pointwise_corrections_match_the_golden_vectors(kind):
     cases = json.loads((HERE / "golden" / "bjorhus_vectors.json").read_text())["cases"]
     args = [dev(a) for a in storage_inputs(cases)]
-    dgc, dpic, dphic = (host(t) for t in dg.gh_bjorhus_corrections(*args))
+    dgc, dpic, dphic = (host(t) for t in dg.gh_bjorhus_corrections(*args, bjorhus_type=kind))
+    suffix = "_physical" if kind == "ConstraintPreservingPhysical" else ""
     ref_g = bo.store_aa(np.stack([np.asarray(c["dt_spacetime_metric"]) for c in cases], -1))
-    ref_pi = bo.store_aa(np.stack([np.asarray(c["dt_pi"]) for c in cases], -1))
-    ref_phi = bo.store_iaa(np.stack([np.asarray(c["dt_phi"]) for c in cases], -1))
+    ref_pi = bo.store_aa(np.stack([np.asarray(c["dt_pi" + suffix]) for c in cases], -1))
+    ref_phi = bo.store_iaa(np.stack([np.asarray(c["dt_phi" + suffix]) for c in cases], -1))
     for got, ref in ((dgc, ref_g), (dpic, ref_pi), (dphic, ref_phi)):
         scale = np.maximum(1.0, np.abs(ref).max(axis=0))
         assert np.max(np.abs(got - ref) / scale) <= 1e-12
     assert np.count_nonzero(np.abs(ref_pi).max(axis=0)) >= 24
 
 
+def test_unknown_bjorhus_type_is_rejected():
+    cases = json.loads((HERE / "golden" / "bjorhus_vectors.json").read_text())["cases"][:2]
+    args = [dev(a) for a in storage_inputs(cases)]
+    with pytest.raises(ValueError):
+        dg.gh_bjorhus_corrections(*args, bjorhus_type="Physical")
+    outs = [torch.empty((c, 2), dtype=torch.float64, device="cuda") for c in (10, 10, 30)]
+    rc = dg.lib().sxd_gh_bjorhus_corrections(7, 2, *[dg._ptr(a) for a in args],
+                                             *[dg._ptr(o) for o in outs], None)
+    assert rc == -1  # SXD_ERR_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("kind", ["ConstraintPreserving", "ConstraintPreservingPhysical"])
 @pytest.mark.parametrize("elements,extents", [((2, 2, 2), (5, 5, 5)), ((2, 1, 2), (6, 4, 5))])
-def test_gh_rhs_bjorhus_matches_oracle(elements, extents):
-    print(f"bjorhus rhs {elements} x {extents}", flush=True)
+def test_gh_rhs_bjorhus_matches_oracle(elements, extents, kind):
+    print(f"bjorhus rhs {kind} {elements} x {extents}", flush=True)
     brick, u, gammas, inv, _ = wl.gh_kerr_schild_dirichlet(seed=3, elements=elements,
                                                            extents=extents, noise=1e-4)
     coords = brick.coordinates()
-    ref = bo.gh_rhs_bjorhus(brick.extents, brick.neighbors, u, gammas, inv, coords)
+    physical = kind == "ConstraintPreservingPhysical"
+    ref = bo.gh_rhs_bjorhus(brick.extents, brick.neighbors, u, gammas, inv, coords,
+                            physical=physical)
+    if physical:
+        other = bo.gh_rhs_bjorhus(brick.extents, brick.neighbors, u, gammas, inv, coords)
+        assert np.abs(ref - other).max() > 1e-6 * np.abs(ref).max()
     interior_only = bo.ol.gh_rhs_internal_faces(brick.extents, brick.neighbors, u, gammas, inv)
     assert np.abs(ref - interior_only).max() > 1e-6 * np.abs(ref).max()
     domain = dg.Domain.from_brick(brick)
-    got = host(domain.gh_rhs_bjorhus(dev(u), dev(gammas), dev(inv), dev(coords)))
+    got = host(domain.gh_rhs_bjorhus(dev(u), dev(gammas), dev(inv), dev(coords),
+                                     bjorhus_type=kind))
     err = np.abs(got - ref).max() / np.abs(ref).max()
     print(f"relative max-norm difference {err:.3e}")
     assert err <= 1e-12

@@ -1,6 +1,6 @@
 """Pins of the numpy restatement of the GH constraint-preserving Bjorhus
-boundary condition (oracle/gh_bjorhus.py, ConstraintPreserving type, static
-mesh):
+boundary condition (oracle/gh_bjorhus.py, ConstraintPreserving and
+ConstraintPreservingPhysical types, static mesh):
 
 * the 48 golden vectors written from the reference's own Python test oracle
   (tests/Unit/Evolution/Systems/GeneralizedHarmonic/BoundaryConditions/
@@ -11,7 +11,10 @@ mesh):
   projections take the prescribed values -- v_plus frozen to zero and
   v_psi, v_zero shifted by the constraint-damping terms -- where the speeds
   are negative, and the projections of fields with positive speeds are
-  untouched.
+  untouched;
+* the ConstraintPreservingPhysical type changes only the v_minus projection:
+  the v_psi, v_zero and v_plus projections of its corrections equal those of
+  the ConstraintPreserving type.
 """
 import json
 import pathlib
@@ -44,6 +47,31 @@ def test_golden_vectors_of_the_reference(k):
     for got, key in ((dg, "dt_spacetime_metric"), (dpi, "dt_pi"), (dphi, "dt_phi")):
         ref = np.asarray(c[key])
         np.testing.assert_allclose(got[..., 0], ref, rtol=0, atol=1e-13 * max(1.0, np.abs(ref).max()))
+
+
+@pytest.mark.parametrize("k", range(len(CASES)))
+def test_golden_vectors_of_the_reference_physical_type(k):
+    c = CASES[k]
+    dg, dpi, dphi, _ = bjorhus.corrections(*inputs(c), physical=True)
+    for got, key in ((dg, "dt_spacetime_metric"), (dpi, "dt_pi_physical"),
+                     (dphi, "dt_phi_physical")):
+        ref = np.asarray(c[key])
+        np.testing.assert_allclose(got[..., 0], ref, rtol=0, atol=1e-13 * max(1.0, np.abs(ref).max()))
+
+
+def test_physical_type_changes_only_v_minus():
+    changed = 0
+    for c in CASES:
+        args = inputs(c)
+        n, G, shift, lapse, g2 = args[0], args[9], args[8], args[7], args[6]
+        S = G[1:, 1:] + np.einsum("ip,jp->ijp", shift, shift) / lapse ** 2
+        n_up = np.einsum("ijp,ip->jp", S, n)
+        cp = char_fields(*bjorhus.corrections(*args)[:3], n, n_up, g2)
+        ph = char_fields(*bjorhus.corrections(*args, physical=True)[:3], n, n_up, g2)
+        for a, b in zip(cp[:3], ph[:3]):
+            np.testing.assert_allclose(a, b, rtol=0, atol=1e-13)
+        changed += int(np.abs(cp[3] - ph[3]).max() > 1e-3)
+    assert changed >= len(CASES) // 2
 
 
 def char_fields(dt_g, dt_pi, dt_phi, n_dn, n_up, gamma2):
