@@ -538,9 +538,11 @@ int sxd_gh_bjorhus_corrections(
  * volume time derivatives and derivatives at the point; no correction on a
  * face without a negative characteristic speed), then the UpwindPenalty
  * corrections of the internal faces (Actions/BoundaryConditionsImpl.hpp:
- * 586-660, a time-derivative condition has no upwind term).  coords
- * [E][3][Np] inertial coordinates (the gauge condition's 1/r).  Composite
- * kernels; SXD_ERR_UNSUPPORTED on mixed-p and mortar domains.  Results
+ * 586-660, a time-derivative condition has no upwind term).  coords:
+ * inertial coordinates (the gauge condition's 1/r), [E][3][Np] on uniform
+ * domains, ragged [3][Np_e] on mixed-p ones.  Uniform domains run the
+ * composite kernels, mixed-p domains the mixed-p volume and face-pass
+ * kernels whatever kernel is set; SXD_ERR_UNSUPPORTED on mortar domains.  Results
  * agree with the oracle to 1e-12 (the corrections are tensor sums whose order
  * is not the reference's). */
 int sxd_gh_rhs_bjorhus(const sxd_domain* domain, int bjorhus_type, const double* u,

@@ -1551,9 +1551,12 @@ void sw_rhs_mixed(int num_elements, const int* extents, const int* nbr, const T*
 // [3][Np_e] per element.  External faces (neighbour < 0) take the exterior
 // state from `ghost` (ragged [50][Np_e]) at the face's own points, as in
 // gh_rhs_faces, with no mortar; they are corrected before the internal faces.
+// skip_external: external faces get no correction (a time-derivative
+// condition applied by the caller).
 template <class T>
 void gh_rhs_mixed(int num_elements, const int* extents, const int* nbr, const T* u,
-                  const T* gammas, const T* inv_j, T* dt, const T* ghost = nullptr) {
+                  const T* gammas, const T* inv_j, T* dt, const T* ghost = nullptr,
+                  bool skip_external = false) {
   const RaggedLayout L(num_elements, extents);
   std::vector<GhFaceTemps<T>> temps(L.total);
   gh_volume_ragged(L, u, gammas, inv_j, dt, temps.data());
@@ -1564,6 +1567,7 @@ void gh_rhs_mixed(int num_elements, const int* extents, const int* nbr, const T*
       const int en = nbr[6 * e + dir];
       if ((en < 0) != (step < 6)) continue;
       const int d = dir / 2, side = dir % 2;
+      if (en < 0 && skip_external) continue;
       if (en < 0) {
         const Face fi = make_face(L.ext[e].data(), d, side);
         const T* ux = ghost + 50 * off;

@@ -599,6 +599,15 @@ int sxo_gh_rhs_mixed_dirichlet(int num_elements, const int* extents, const int* 
   return 0;
 }
 
+int sxo_gh_rhs_mixed_internal_faces(int num_elements, const int* extents, const int* nbr,
+                                    const double* u, const double* gammas,
+                                    const double* inv_j, double* dt) {
+  if (bad_ragged(num_elements, extents, nbr, true)) return -1;
+  sxo::gh_rhs_mixed(num_elements, extents, nbr, u, gammas, inv_j, dt,
+                    static_cast<const double*>(nullptr), true);
+  return 0;
+}
+
 int sxo_gh_rhs_mixed(int num_elements, const int* extents, const int* nbr, const double* u,
                      const double* gammas, const double* inv_j, double* dt) {
   if (bad_ragged(num_elements, extents, nbr)) return -1;

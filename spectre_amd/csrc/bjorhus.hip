@@ -1,7 +1,7 @@
 // GH constraint-preserving Bjorhus boundary condition on the device: the
 // point-wise corrections (gh_bjorhus.hpp) for arrays of points, and the
-// external faces of a uniform GH domain, applied after the volume terms and
-// before the internal faces' corrections, one dimension after another
+// external faces of a uniform or mixed-p GH domain, applied after the volume
+// terms and before the internal faces' corrections, one dimension after another
 // (Evolution/DiscontinuousGalerkin/Actions/BoundaryConditionsImpl.hpp:
 // 586-660: each external direction's time-derivative correction is added to
 // dt before the next direction projects dt to its face).
@@ -86,28 +86,21 @@ bjorhus_points_kernel(int64_t n, bool physical, const double* __restrict__ norma
     }
 }
 
-// The external faces normal to dimension d of a uniform GH domain: one
-// workgroup per (element, side); faces with a neighbour return at once.
-// u [E][50][Np], du [E][150][Np] (block 3 c + i = d_i u_c), gammas
-// [E][3][Np], inv_jac [E][9][Np], coords [E][3][Np]; dt [E][50][Np] holds the
-// volume terms (and the corrections of earlier dimensions) and receives the
-// corrections.  Harmonic gauge: H_a = 0, d_a H_b = 0.
-__global__ void __launch_bounds__(64)
-bjorhus_face_kernel(int d, bool physical, int n0, int n1, int n2,
-                    const int32_t* __restrict__ nbr,
-                    const double* __restrict__ u, const double* __restrict__ du,
-                    const double* __restrict__ gammas, const double* __restrict__ inv_jac,
-                    const double* __restrict__ coords, double* __restrict__ dt) {
-  const int64_t e = blockIdx.x >> 1;
-  const int side = blockIdx.x & 1;
-  if (nbr[6 * e + 2 * d + side] >= 0) return;  // uniform over the workgroup
-  const int ext[3] = {n0, n1, n2};
-  const int na = d == 0 ? n1 : n0, nb = d == 2 ? n1 : n2;
-  const int np = n0 * n1 * n2;
-  const double* ue = u + e * 50 * np;
-  const double* due = du + e * 150 * np;
-  const double* ga = gammas + e * 3 * np;
-  double* dte = dt + e * 50 * np;
+// The corrections of one external face (dimension d, side) of one element
+// with extents ext and np points, added into its dt: one workgroup (64
+// threads, one wave).  Element blocks: u [50][np], du [150][np] (block
+// 3 c + i = d_i u_c), gammas [3][np], inv_jac [9][np], coords [3][np], dt
+// [50][np] holding the volume terms (and the corrections of earlier
+// dimensions).  Harmonic gauge: H_a = 0, d_a H_b = 0.
+__device__ __forceinline__ void bjorhus_face(int d, int side, bool physical, const int ext[3],
+                                             int np, const double* __restrict__ ue,
+                                             const double* __restrict__ due,
+                                             const double* __restrict__ ga,
+                                             const double* __restrict__ ije,
+                                             const double* __restrict__ xe,
+                                             double* __restrict__ dte) {
+  const int n0 = ext[0], n1 = ext[1];
+  const int na = d == 0 ? n1 : n0, nb = d == 2 ? n1 : ext[2];
   auto point_of = [&](int f) {
     int idx[3];
     idx[d] = side == 0 ? 0 : ext[d] - 1;
@@ -134,7 +127,7 @@ bjorhus_face_kernel(int d, bool physical, int n0, int n1, int n2,
     // outward unit normal covector
     double nn[3];
     for (int i = 0; i < 3; ++i) {
-      nn[i] = inv_jac[(e * 9 + ij3(d, i)) * np + pt];
+      nn[i] = ije[ij3(d, i) * np + pt];
       if (side == 0) nn[i] = nn[i] * -1.0;
     }
     double m = 0.0;
@@ -144,7 +137,7 @@ bjorhus_face_kernel(int d, bool physical, int n0, int n1, int n2,
     for (int i = 0; i < 3; ++i) p.n[i] = nn[i] / m;
     p.gamma1 = ga[np + pt];
     p.gamma2 = ga[2 * np + pt];
-    for (int i = 0; i < 3; ++i) p.x[i] = coords[(e * 3 + i) * np + pt];
+    for (int i = 0; i < 3; ++i) p.x[i] = xe[i * np + pt];
     for (int a = 0; a < 4; ++a) {
       p.H[a] = 0.0;
       for (int b = 0; b < 4; ++b) {
@@ -193,6 +186,42 @@ bjorhus_face_kernel(int d, bool physical, int n0, int n1, int n2,
   }
 }
 
+// The external faces normal to dimension d of a uniform GH domain: one
+// workgroup per (element, side); faces with a neighbour return at once.
+// Arrays [E][components][Np].
+__global__ void __launch_bounds__(64)
+bjorhus_face_kernel(int d, bool physical, int n0, int n1, int n2,
+                    const int32_t* __restrict__ nbr,
+                    const double* __restrict__ u, const double* __restrict__ du,
+                    const double* __restrict__ gammas, const double* __restrict__ inv_jac,
+                    const double* __restrict__ coords, double* __restrict__ dt) {
+  const int64_t e = blockIdx.x >> 1;
+  const int side = blockIdx.x & 1;
+  if (nbr[6 * e + 2 * d + side] >= 0) return;  // uniform over the workgroup
+  const int ext[3] = {n0, n1, n2};
+  const int np = n0 * n1 * n2;
+  bjorhus_face(d, side, physical, ext, np, u + e * 50 * np, du + e * 150 * np,
+               gammas + e * 3 * np, inv_jac + e * 9 * np, coords + e * 3 * np,
+               dt + e * 50 * np);
+}
+
+// The same on a mixed-p domain: workgroup (x, side) takes element list[x]
+// (ragged arrays: element blocks at the element's point offset).
+__global__ void __launch_bounds__(64)
+bjorhus_mixed_face_kernel(int d, bool physical, const MixedElement* __restrict__ el,
+                          const int32_t* __restrict__ list, const int32_t* __restrict__ nbr,
+                          const double* __restrict__ u, const double* __restrict__ du,
+                          const double* __restrict__ gammas, const double* __restrict__ inv_jac,
+                          const double* __restrict__ coords, double* __restrict__ dt) {
+  const int e = list[blockIdx.x];
+  const int side = blockIdx.y;
+  if (nbr[6 * e + 2 * d + side] >= 0) return;  // uniform over the workgroup
+  const MixedElement m = el[e];
+  const int ext[3] = {m.n[0], m.n[1], m.n[2]};
+  bjorhus_face(d, side, physical, ext, m.np, u + 50 * m.off, du + 150 * m.off,
+               gammas + 3 * m.off, inv_jac + 9 * m.off, coords + 3 * m.off, dt + 50 * m.off);
+}
+
 }  // namespace
 
 int launch_gh_bjorhus_points(int64_t n, bool physical, const double* const in[19],
@@ -214,6 +243,19 @@ int launch_gh_bjorhus_faces(int num_elements, bool physical, const int32_t ext[3
     hipLaunchKernelGGL(bjorhus_face_kernel, dim3(static_cast<unsigned>(2 * num_elements)),
                        dim3(64), 0, stream, d, physical, ext[0], ext[1], ext[2], nbr, u, du, gammas,
                        inv_jac, coords, dt);
+    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  }
+  return SXD_OK;
+}
+
+int launch_gh_bjorhus_mixed_faces(int count, bool physical, const int32_t* list,
+                                  const MixedElement* el, const int32_t* nbr, const double* u,
+                                  const double* du, const double* gammas, const double* inv_jac,
+                                  const double* coords, double* dt, hipStream_t stream) {
+  if (count == 0) return SXD_OK;
+  for (int d = 0; d < 3; ++d) {
+    hipLaunchKernelGGL(bjorhus_mixed_face_kernel, dim3(static_cast<unsigned>(count), 2), dim3(64),
+                       0, stream, d, physical, el, list, nbr, u, du, gammas, inv_jac, coords, dt);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   }
   return SXD_OK;

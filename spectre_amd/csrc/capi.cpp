@@ -1333,8 +1333,9 @@ int sxd_gh_rhs_bjorhus(const sxd_domain* domain, int bjorhus_type, const double*
   if (bjorhus_type != SXD_BJORHUS_CONSTRAINT_PRESERVING &&
       bjorhus_type != SXD_BJORHUS_CONSTRAINT_PRESERVING_PHYSICAL)
     return fail(SXD_ERR_INVALID_ARGUMENT, "%s: unknown Bjorhus type %d", __func__, bjorhus_type);
-  if (domain->mixed)
-    return fail(SXD_ERR_UNSUPPORTED, "%s: takes uniform domains (composite kernels)", __func__);
+  if (domain->mortars)
+    return fail(SXD_ERR_UNSUPPORTED, "%s: mortar domains have no external faces", __func__);
+  const bool physical = bjorhus_type == SXD_BJORHUS_CONSTRAINT_PRESERVING_PHYSICAL;
   SXD_REQUIRE_DEVICE(u);
   SXD_REQUIRE_DEVICE(gammas);
   SXD_REQUIRE_DEVICE(inv_jac);
@@ -1343,15 +1344,31 @@ int sxd_gh_rhs_bjorhus(const sxd_domain* domain, int bjorhus_type, const double*
   sxd_domain* d = const_cast<sxd_domain*>(domain);
   if (int rc = check_no_overlap(d, 50, u, dt, __func__); rc != SXD_OK) return rc;
   Scratch& sc = stream_scratch(d, static_cast<hipStream_t>(stream));
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (d->mixed) {
+    // the mixed-p kernels with their face passes (the mortar organisation
+    // has no external faces): volume terms, the condition on the external
+    // faces, then the internal faces
+    if (int rc = reserve_scratch(sc, 150 * static_cast<size_t>(d->total_points), 0); rc != SXD_OK)
+      return finish(rc, __func__);
+    const int threads = d->threads[SXD_SYSTEM_GH];
+    int rc = launch_mixed_volume(true, d->num_elements, d->d_list, d->d_el, d->np_max, u, gammas,
+                                 inv_jac, dt, sc.du, threads, s);
+    if (rc == SXD_OK)
+      rc = launch_gh_bjorhus_mixed_faces(d->num_elements, physical, d->d_list, d->d_el, d->d_nbr,
+                                         u, sc.du, gammas, inv_jac, coords, dt, s);
+    if (rc == SXD_OK)
+      rc = launch_mixed_gh_face_passes(d->num_elements, d->d_list, d->d_el, d->d_nbr, d->kf,
+                                       d->km, u, gammas, inv_jac, dt, threads, nullptr,
+                                       kFacesInternal, s);
+    return finish(rc, __func__);
+  }
   const size_t np = static_cast<size_t>(d->ext[0]) * d->ext[1] * d->ext[2];
   if (int rc = reserve_scratch(sc, 150 * np * d->num_elements, 12 * np * d->num_elements);
       rc != SXD_OK)
     return finish(rc, __func__);
-  return finish(launch_gh_rhs_bjorhus(d->num_elements,
-                                      bjorhus_type == SXD_BJORHUS_CONSTRAINT_PRESERVING_PHYSICAL,
-                                      d->ext, d->d_nbr, u, gammas, inv_jac,
-                                      coords, dt, sc.du, sc.temps,
-                                      static_cast<hipStream_t>(stream)),
+  return finish(launch_gh_rhs_bjorhus(d->num_elements, physical, d->ext, d->d_nbr, u, gammas,
+                                      inv_jac, coords, dt, sc.du, sc.temps, s),
                 __func__);
 }
 

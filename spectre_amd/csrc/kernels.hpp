@@ -120,6 +120,14 @@ int launch_mixed_volume(bool gh, int count, const int32_t* list, const MixedElem
                         int np_max, const double* u, const double* params,
                         const double* inv_jac, double* dt, double* scratch_du, int threads,
                         hipStream_t stream);
+// The GH face corrections alone of launch_mixed_rhs, one launch per dimension,
+// for the faces that `phase` selects (kFacesAll, kFacesExternal with the ghost
+// state, kFacesInternal).
+int launch_mixed_gh_face_passes(int count, const int32_t* list, const MixedElement* el,
+                                const int32_t* nbr, int kf, int km, const double* u,
+                                const double* params, const double* inv_jac, double* dt,
+                                int threads, const double* ghost, int phase,
+                                hipStream_t stream);
 // Non-conforming mortars (mortar.hip): rotated neighbours and h-refined faces.
 // Spectral::ChildSize (NumericalAlgorithms/Spectral/Projection.hpp:22-30).
 constexpr int kChildFull = 1, kChildUpperHalf = 2, kChildLowerHalf = 3;
@@ -292,6 +300,13 @@ int launch_gh_bjorhus_faces(int num_elements, bool physical, const int32_t ext[3
                             const double* gammas,
                             const double* inv_jac, const double* coords, double* dt,
                             hipStream_t stream);
+// The corrections of the external faces of the `count` mixed-p elements in
+// `list` (ragged arrays, du as launch_mixed_volume leaves it) added into dt,
+// one dimension after another.
+int launch_gh_bjorhus_mixed_faces(int count, bool physical, const int32_t* list,
+                                  const MixedElement* el, const int32_t* nbr, const double* u,
+                                  const double* du, const double* gammas, const double* inv_jac,
+                                  const double* coords, double* dt, hipStream_t stream);
 // The whole GH RHS of a uniform domain whose external faces carry the
 // Bjorhus condition (composite kernels; scratch as launch_gh_rhs_composite).
 int launch_gh_rhs_bjorhus(int num_elements, bool physical, const int32_t ext[3],

@@ -1239,26 +1239,45 @@ int launch_mixed_rhs(bool gh, int count, const int32_t* list, const MixedElement
   hipLaunchKernelGGL(gh_mixed_pointwise_kernel, dim3(count), dim3(threads), 0, stream, el, list,
                      u, scratch_du, params, dt);
   if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  // with external faces: a sweep over the external faces, then one over the
+  // internal ones (the order in which the corrections reach dt)
+  if (ghost != nullptr) {
+    const int rc = launch_mixed_gh_face_passes(count, list, el, nbr, kf, km, u, params, inv_jac,
+                                               dt, threads, ghost, kFacesExternal, stream);
+    if (rc != SXD_OK) return rc;
+    return launch_mixed_gh_face_passes(count, list, el, nbr, kf, km, u, params, inv_jac, dt,
+                                       threads, ghost, kFacesInternal, stream);
+  }
+  return launch_mixed_gh_face_passes(count, list, el, nbr, kf, km, u, params, inv_jac, dt,
+                                     threads, nullptr, kFacesAll, stream);
+}
+
+int launch_mixed_gh_face_passes(int count, const int32_t* list, const MixedElement* el,
+                                const int32_t* nbr, int kf, int km, const double* u,
+                                const double* params, const double* inv_jac, double* dt,
+                                int threads, const double* ghost, int phase,
+                                hipStream_t stream) {
+  if (count == 0) return SXD_OK;
+  const double* mtab = device_mortar_matrices();
+  if (mtab == nullptr) return SXD_ERR_DEVICE;
+  if (threads != 64 && threads != 128 && threads != 256) return SXD_ERR_INVALID_ARGUMENT;
+  if (mixed_face_lds_bytes(kf, km) > 160 * 1024) return SXD_ERR_UNSUPPORTED;
+  if (!enable_large_lds()) return SXD_ERR_DEVICE;
   // the fused neighbour interpolation only where its smaller LDS plan fits
   // more workgroups on a CU
   const int lds_split = FacePassPlan<false>(kf, km).total * 8;
   const int lds_fused = FacePassPlan<true>(kf, km).total * 8;
   const bool fused_nb = 160 * 1024 / lds_fused > 160 * 1024 / lds_split;
-  // with external faces: a sweep over the external faces, then one over the
-  // internal ones (the order in which the corrections reach dt)
-  const int phases[2] = {ghost != nullptr ? kFacesExternal : kFacesAll, kFacesInternal};
-  for (int ph = 0; ph < (ghost != nullptr ? 2 : 1); ++ph) {
-    for (int d = 0; d < 3; ++d) {
-      if (fused_nb)
-        hipLaunchKernelGGL(gh_mixed_face_pass_kernel<true>, dim3(count, 2, 10), dim3(threads),
-                           lds_fused, stream, d, el, list, nbr, u, params, inv_jac, dt, mtab, kf,
-                           km, ghost, phases[ph]);
-      else
-        hipLaunchKernelGGL(gh_mixed_face_pass_kernel<false>, dim3(count, 2, 10), dim3(threads),
-                           lds_split, stream, d, el, list, nbr, u, params, inv_jac, dt, mtab, kf,
-                           km, ghost, phases[ph]);
-      if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
-    }
+  for (int d = 0; d < 3; ++d) {
+    if (fused_nb)
+      hipLaunchKernelGGL(gh_mixed_face_pass_kernel<true>, dim3(count, 2, 10), dim3(threads),
+                         lds_fused, stream, d, el, list, nbr, u, params, inv_jac, dt, mtab, kf,
+                         km, ghost, phase);
+    else
+      hipLaunchKernelGGL(gh_mixed_face_pass_kernel<false>, dim3(count, 2, 10), dim3(threads),
+                         lds_split, stream, d, el, list, nbr, u, params, inv_jac, dt, mtab, kf,
+                         km, ghost, phase);
+    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   }
   return SXD_OK;
 }

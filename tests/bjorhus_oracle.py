@@ -84,6 +84,24 @@ def face_points(ext, d, side):
     return np.flatnonzero(on)
 
 
+def face_corrections(ext, d, side, u, du, dt, gammas, inv_jac, coords, physical):
+    """Storage-order corrections [50, face points] of one external face of one
+    element ([nc][Np] blocks, dt the current time derivative) and the face's
+    volume point indices."""
+    pts = face_points(ext, d, side)
+    nrm = np.stack([inv_jac[d + 3 * i, pts] for i in range(3)])
+    if side == 0:
+        nrm = -nrm
+    a = point_inputs(u[:, pts], du[:, pts], dt[:, pts], gammas[:, pts], coords[:, pts], nrm)
+    args = (a["normal_covector"], a["g"], a["pi"], a["phi"], a["coords"], a["gamma1"],
+            a["gamma2"], a["lapse"], a["shift"], a["inv_g"], a["t_up"], a["C3"], a["H"],
+            a["dH"], a["dt_g"], a["dt_pi"], a["dt_phi"], a["d_pi"], a["d_phi"])
+    speeds = bjorhus.corrections(*args)[3]
+    incoming = np.full(pts.size, bool(np.min(speeds) < 0.0))
+    dg, dpi, dphi, _ = bjorhus.corrections(*args, face_has_incoming=incoming, physical=physical)
+    return np.concatenate([store_aa(dg), store_aa(dpi), store_iaa(dphi)]), pts
+
+
 def gh_rhs_bjorhus(extents, neighbors, u, gammas, inv_jac, coords, physical=False):
     """The GH RHS of a uniform brick whose external faces carry the Bjorhus
     condition (physical: the ConstraintPreservingPhysical type): the oracle's
@@ -101,24 +119,32 @@ def gh_rhs_bjorhus(extents, neighbors, u, gammas, inv_jac, coords, physical=Fals
             for side in range(2):
                 if nb[e, 2 * d + side] >= 0:
                     continue
-                pts = face_points(extents, d, side)
-                nrm = np.stack([inv_jac[e, d + 3 * i, pts] for i in range(3)])
-                if side == 0:
-                    nrm = -nrm
-                inp = point_inputs(u[e][:, pts], du[e][:, pts], dt_bc[e][:, pts],
-                                   gammas[e][:, pts], coords[e][:, pts], nrm)
-                a = inp
-                speeds_probe = bjorhus.corrections(
-                    a["normal_covector"], a["g"], a["pi"], a["phi"], a["coords"], a["gamma1"],
-                    a["gamma2"], a["lapse"], a["shift"], a["inv_g"], a["t_up"], a["C3"], a["H"],
-                    a["dH"], a["dt_g"], a["dt_pi"], a["dt_phi"], a["d_pi"], a["d_phi"])[3]
-                incoming = np.full(pts.size, bool(np.min(speeds_probe) < 0.0))
-                dg, dpi, dphi, _ = bjorhus.corrections(
-                    a["normal_covector"], a["g"], a["pi"], a["phi"], a["coords"], a["gamma1"],
-                    a["gamma2"], a["lapse"], a["shift"], a["inv_g"], a["t_up"], a["C3"], a["H"],
-                    a["dH"], a["dt_g"], a["dt_pi"], a["dt_phi"], a["d_pi"], a["d_phi"],
-                    face_has_incoming=incoming, physical=physical)
-                corr = np.concatenate([store_aa(dg), store_aa(dpi), store_iaa(dphi)])
+                corr, pts = face_corrections(extents, d, side, u[e], du[e], dt_bc[e], gammas[e],
+                                             inv_jac[e], coords[e], physical)
                 dt_bc[e][:, pts] += corr
                 total[e][:, pts] += corr
     return dt + total
+
+
+def gh_rhs_bjorhus_mixed(brick, u, gammas, inv_jac, coords, physical=False):
+    """The same on a mixed-p brick (ragged layout): the oracle's mixed-p volume
+    terms and internal faces, plus each element's external-face corrections
+    in dimension order."""
+    nb = np.asarray(brick.neighbors).reshape(-1, 6)
+    dt = ol.gh_rhs_mixed_internal_faces(brick.extents, brick.neighbors, u, gammas, inv_jac)
+    us, gs, js, xs = (brick.split(a, nc) for a, nc in ((u, 50), (gammas, 3), (inv_jac, 9),
+                                                      (coords, 3)))
+    out = brick.split(dt, 50)
+    for e in range(brick.num_elements):
+        ext = tuple(int(v) for v in brick.extents[e])
+        du = ol.partial_derivatives(us[e], ext, js[e])
+        dt_bc = ol.gh_time_derivative(us[e], du, gs[e])
+        for d in range(3):
+            for side in range(2):
+                if nb[e, 2 * d + side] >= 0:
+                    continue
+                corr, pts = face_corrections(ext, d, side, us[e], du, dt_bc, gs[e], js[e], xs[e],
+                                             physical)
+                dt_bc[:, pts] += corr
+                out[e][:, pts] += corr
+    return brick.ragged(out)

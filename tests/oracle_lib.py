@@ -473,6 +473,18 @@ def sw_rhs_mixed(extents, neighbors, u, gamma2, inv_jac):
     return dt
 
 
+def gh_rhs_mixed_internal_faces(extents, neighbors, u, gammas, inv_jac):
+    """Mixed-p volume terms plus the internal faces' corrections (external
+    faces skipped), ragged layout."""
+    u, gammas, inv_jac = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, gammas, inv_jac))
+    dt = np.empty_like(u)
+    ext, ep = _i(np.asarray(extents).ravel())
+    nb, nbp = _i(np.asarray(neighbors).ravel())
+    _check(lib().sxo_gh_rhs_mixed_internal_faces(len(ext) // 3, ep, nbp, _d(u), _d(gammas),
+                                                 _d(inv_jac), _d(dt)))
+    return dt
+
+
 def gh_rhs_mixed(extents, neighbors, u, gammas, inv_jac):
     u, gammas, inv_jac = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, gammas, inv_jac))
     dt = np.empty_like(u)

@@ -9,7 +9,8 @@ evaluates in its own order):
   constraints and hence the corrections do not vanish) with the condition on
   all outer faces (sxd_gh_rhs_bjorhus) against the oracle assembled from the
   C++ restatement's volume and internal-face terms and the numpy corrections,
-  face by face in direction order.
+  face by face in direction order; uniform and mixed-p (extents 3..6)
+  bricks.
 """
 import json
 import pathlib
@@ -99,6 +100,29 @@ def test_gh_rhs_bjorhus_matches_oracle(elements, extents, kind):
     interior_only = bo.ol.gh_rhs_internal_faces(brick.extents, brick.neighbors, u, gammas, inv)
     assert np.abs(ref - interior_only).max() > 1e-6 * np.abs(ref).max()
     domain = dg.Domain.from_brick(brick)
+    got = host(domain.gh_rhs_bjorhus(dev(u), dev(gammas), dev(inv), dev(coords),
+                                     bjorhus_type=kind))
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"relative max-norm difference {err:.3e}")
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("kind", ["ConstraintPreserving", "ConstraintPreservingPhysical"])
+@pytest.mark.parametrize("threads", [64, 256])
+def test_gh_rhs_bjorhus_mixed_p_matches_oracle(kind, threads):
+    print(f"bjorhus mixed-p rhs {kind} threads {threads}", flush=True)
+    brick, u, gammas, inv, _ = wl.gh_kerr_schild_mixed_dirichlet(
+        seed=7, elements=(2, 2, 2), lowest=3, highest=6, lower=(3.0, -2.0, 0.5),
+        upper=(7.0, 2.0, 4.5))
+    rng = np.random.default_rng(12)
+    gammas = gammas * rng.uniform(0.5, 1.5, gammas.shape)
+    coords = brick.coordinates()
+    ref = bo.gh_rhs_bjorhus_mixed(brick, u, gammas, inv, coords,
+                                  physical=kind == "ConstraintPreservingPhysical")
+    internal = bo.ol.gh_rhs_mixed_internal_faces(brick.extents, brick.neighbors, u, gammas, inv)
+    assert np.abs(ref - internal).max() > 1e-6 * np.abs(ref).max()
+    domain = dg.MixedDomain.from_brick(brick)
+    domain.set_threads("gh", threads)
     got = host(domain.gh_rhs_bjorhus(dev(u), dev(gammas), dev(inv), dev(coords),
                                      bjorhus_type=kind))
     err = np.abs(got - ref).max() / np.abs(ref).max()

@@ -22,6 +22,7 @@ import pathlib
 import numpy as np
 import pytest
 
+import bjorhus_oracle as bo
 from bjorhus_oracle import bjorhus
 
 HERE = pathlib.Path(__file__).resolve().parent
@@ -108,3 +109,20 @@ def test_corrected_projections_take_the_prescribed_values():
             np.testing.assert_allclose(after[1], before[1], atol=1e-12)
         checked += 1
     assert checked >= 24
+
+
+def test_mixed_p_assembly_equals_the_uniform_one_at_equal_extents():
+    """The mixed-p Bjorhus RHS of the tests (ragged layout) reduces to the
+    uniform one when every element has the same extents."""
+    from spectre_amd import workloads as wl
+    b, u, g, inv, _ = wl.gh_kerr_schild_mixed_dirichlet(
+        seed=7, elements=(2, 1, 2), lowest=4, highest=4, lower=(3.0, -2.0, 0.5),
+        upper=(7.0, 2.0, 4.5))
+    x = b.coordinates()
+    U = lambda a, nc: np.stack(b.split(a, nc))
+    nb = np.asarray(b.neighbors).reshape(-1, 6)
+    for physical in (False, True):
+        mixed = U(bo.gh_rhs_bjorhus_mixed(b, u, g, inv, x, physical=physical), 50)
+        uniform = bo.gh_rhs_bjorhus((4, 4, 4), nb, U(u, 50), U(g, 3), U(inv, 9), U(x, 3),
+                                    physical=physical)
+        assert np.abs(mixed - uniform).max() <= 1e-14 * np.abs(uniform).max()
