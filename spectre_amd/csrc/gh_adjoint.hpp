@@ -28,9 +28,17 @@ namespace sxd {
 // Makes a pointer's value opaque to the optimiser: loads through it after
 // this point are not merged with loads made before it.  A long point-wise
 // adjoint re-reads its inputs (from cache) in each phase instead of keeping
-// every value it read alive in registers across phases.
+// every value it read alive in registers across phases.  On the device the
+// pointers are global memory (kernel arguments): the opaque value is a
+// global-space pointer, so the re-reads stay global loads rather than flat
+// ones (which also occupy the LDS counter).
 #if defined(__HIP_DEVICE_COMPILE__)
-#define SXD_REREAD(p) asm volatile("" : "+v"(p))
+#define SXD_REREAD(p)                                                     \
+  do {                                                                    \
+    auto sxd_q_ = (const __attribute__((address_space(1))) double*)(p);   \
+    asm volatile("" : "+v"(sxd_q_));                                      \
+    (p) = (const double*)sxd_q_;                                          \
+  } while (0)
 #else
 #define SXD_REREAD(p) asm volatile("" : "+r"(p))
 #endif
