@@ -9,6 +9,7 @@
 
 #include "gh_reference_impl.hpp"
 #include "sxo.hpp"
+#include "sxo_refstyle.hpp"
 #include "sxo_flux.hpp"
 #include "sxo_gauss.hpp"
 #include "sxo_mortar.hpp"
@@ -627,6 +628,24 @@ int sxo_gh_rhs_threads(int threads, int num_elements, const int* ext, const int*
   std::vector<sxo::GhFaceTemps<double>> temps(static_cast<size_t>(num_elements) * np);
   run_split(threads, num_elements, [&](int b, int e) {
     sxo::gh_rhs_volume(b, e, ext, u, gammas, inv_j, dt, temps.data());
+  });
+  run_split(threads, num_elements, [&](int b, int e) {
+    sxo::gh_rhs_faces(b, e, ext, nbr, u, inv_j, dt, temps.data());
+  });
+  return 0;
+}
+
+// The same RHS with the volume terms arranged as the reference computes them
+// on a CPU (sxo_refstyle.hpp: transposes and matrix products per dimension,
+// materialised temporaries): the CPU baseline of the benchmark.
+int sxo_gh_rhs_refstyle_threads(int threads, int num_elements, const int* ext, const int* nbr,
+                                const double* u, const double* gammas, const double* inv_j,
+                                double* dt) {
+  if (bad_extents(ext) || num_elements < 1) return -1;
+  const size_t np = static_cast<size_t>(ext[0]) * ext[1] * ext[2];
+  std::vector<sxo::GhFaceTemps<double>> temps(static_cast<size_t>(num_elements) * np);
+  run_split(threads, num_elements, [&](int b, int e) {
+    sxo::refstyle::gh_rhs_volume(b, e, ext, u, gammas, inv_j, dt, temps.data());
   });
   run_split(threads, num_elements, [&](int b, int e) {
     sxo::gh_rhs_faces(b, e, ext, nbr, u, inv_j, dt, temps.data());

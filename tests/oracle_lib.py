@@ -358,6 +358,21 @@ def gh_rhs_internal_faces(extents, neighbors, u, gammas, inv_jac):
     return dt
 
 
+def gh_rhs_refstyle_threads(threads, extents, neighbors, u, gammas, inv_jac, out=None):
+    """gh_rhs with the volume terms arranged as the reference computes them on
+    a CPU (transposes and matrix products, materialised temporaries), elements
+    split over host threads: the CPU baseline."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    gammas = np.ascontiguousarray(gammas, dtype=np.float64)
+    inv_jac = np.ascontiguousarray(inv_jac, dtype=np.float64)
+    dt = np.empty_like(u) if out is None else out
+    ext, ep = _i(extents)
+    nb, nbp = _i(neighbors)
+    _check(lib().sxo_gh_rhs_refstyle_threads(threads, u.shape[0], ep, nbp, _d(u), _d(gammas),
+                                             _d(inv_jac), _d(dt)))
+    return dt
+
+
 def gh_rhs_threads(threads, extents, neighbors, u, gammas, inv_jac, out=None):
     """gh_rhs with elements split over host threads (CPU baseline)."""
     u = np.ascontiguousarray(u, dtype=np.float64)
