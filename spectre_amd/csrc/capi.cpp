@@ -38,6 +38,29 @@ struct Scratch {
   std::vector<uint8_t> filled;
   int32_t* d_status = nullptr;
   size_t status_n = 0;
+  // A second stream of the calling stream's work (mortar organisation: the
+  // face packages and boundary terms run beside the volume terms), with the
+  // fork and join events; created on first use outside a graph capture.
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+
+  // The side stream and its events; false where they cannot be created.
+  bool ensure_side() {
+    if (side != nullptr) return true;
+    hipStream_t s = nullptr;
+    hipEvent_t f = nullptr, j = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return false;
+    if (hipEventCreateWithFlags(&f, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&j, hipEventDisableTiming) != hipSuccess) {
+      if (f != nullptr) (void)hipEventDestroy(f);
+      (void)hipStreamDestroy(s);
+      return false;
+    }
+    side = s;
+    fork = f;
+    join = j;
+    return true;
+  }
 
   // Make room for du_n + temps_n doubles.  Returns SXD_OK,
   // SXD_ERR_OUT_OF_MEMORY, or SXD_ERR_UNSUPPORTED for a fixed block that is
@@ -74,6 +97,13 @@ struct Scratch {
     if (d_status != nullptr) (void)hipFree(d_status);
     d_status = nullptr;
     status_n = 0;
+    if (side != nullptr) {
+      (void)hipEventDestroy(fork);
+      (void)hipEventDestroy(join);
+      (void)hipStreamDestroy(side);
+    }
+    side = nullptr;
+    fork = join = nullptr;
     du = temps = nullptr;
     du_doubles = temps_doubles = 0;
   }
@@ -899,11 +929,38 @@ int run_mortars(sxd_domain* d, Scratch& sc, bool gh, const double* u, const doub
   const size_t du = gh ? 150 * static_cast<size_t>(d->total_points) : 0;
   if (int rc = reserve_scratch(sc, du, mortar_work_doubles(gh, d->plan)); rc != SXD_OK) return rc;
   const int threads = d->threads[gh ? SXD_SYSTEM_GH : SXD_SYSTEM_SCALAR_WAVE];
-  if (int rc = launch_mixed_volume(gh, d->num_elements, d->d_list, d->d_el, d->np_max, u, params,
-                                   inv_jac, dt, sc.du, threads, stream);
-      rc != SXD_OK)
-    return rc;
-  return launch_mortar_faces(gh, d->plan, d->d_el, u, params, inv_jac, dt, sc.temps, stream);
+  auto volume = [&](hipStream_t s) {
+    return launch_mixed_volume(gh, d->num_elements, d->d_list, d->d_el, d->np_max, u, params,
+                               inv_jac, dt, sc.du, threads, s);
+  };
+  // The packages and boundary terms read u only and write the work array:
+  // they run on a second stream beside the volume terms, and the lift (which
+  // adds into dt after them) waits for both.  Inside a graph capture the
+  // same fork and join become graph edges; a stream first seen inside a
+  // capture (no side stream yet) runs everything in order.
+  hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream, &capture) != hipSuccess) return SXD_ERR_DEVICE;
+  const bool fork = d->plan.num_remote == 0 &&
+                    (sc.side != nullptr ||
+                     (capture == hipStreamCaptureStatusNone && sc.ensure_side()));
+  if (!fork) {
+    if (int rc = volume(stream); rc != SXD_OK) return rc;
+    return launch_mortar_faces(gh, d->plan, d->d_el, u, params, inv_jac, dt, sc.temps, stream);
+  }
+  if (hipEventRecord(sc.fork, stream) != hipSuccess ||
+      hipStreamWaitEvent(sc.side, sc.fork, 0) != hipSuccess)
+    return SXD_ERR_DEVICE;
+  int rc = launch_mortar_packages(gh, d->plan, d->d_el, u, params, inv_jac, sc.temps, sc.side);
+  if (rc == SXD_OK) rc = launch_mortar_boundary_terms(gh, d->plan, sc.temps, sc.side);
+  const int rc_volume = volume(stream);
+  // join in every case, so that no work of this call is left on the side
+  // stream after it returns
+  if (hipEventRecord(sc.join, sc.side) != hipSuccess ||
+      hipStreamWaitEvent(stream, sc.join, 0) != hipSuccess)
+    return SXD_ERR_DEVICE;
+  if (rc != SXD_OK) return rc;
+  if (rc_volume != SXD_OK) return rc_volume;
+  return launch_mortar_lift(gh, d->plan, d->d_el, dt, sc.temps, stream);
 }
 
 int run_mixed(sxd_domain* d, Scratch& sc, bool gh, int kernel, const double* u,

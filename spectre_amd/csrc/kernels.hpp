@@ -229,6 +229,12 @@ int launch_mortar_packages(bool gh, const MortarPlan& plan, const MixedElement* 
                            double* work, hipStream_t stream);
 int launch_mortar_corrections(bool gh, const MortarPlan& plan, const MixedElement* el,
                               double* dt, double* work, hipStream_t stream);
+// The corrections in two steps: the boundary terms on every mortar (into the
+// work array; they do not touch dt), then their projection and lift into dt.
+int launch_mortar_boundary_terms(bool gh, const MortarPlan& plan, double* work,
+                                 hipStream_t stream);
+int launch_mortar_lift(bool gh, const MortarPlan& plan, const MixedElement* el, double* dt,
+                       double* work, hipStream_t stream);
 // Write every outgoing message into send (plan.message_bytes[gh] bytes).
 int launch_mortar_pack(bool gh, const MortarPlan& plan, double* work, void* send, int64_t step,
                        hipStream_t stream);

@@ -800,11 +800,8 @@ int launch_mortar_packages(bool gh, const MortarPlan& plan, const MixedElement* 
   return SXD_OK;
 }
 
-int launch_mortar_corrections(bool gh, const MortarPlan& plan, const MixedElement* el,
-                              double* dt, double* work, hipStream_t stream) {
-  const double* full = device_mortar_matrices();
-  const double* half = device_half_mortar_matrices();
-  if (full == nullptr || half == nullptr) return SXD_ERR_DEVICE;
+int launch_mortar_boundary_terms(bool gh, const MortarPlan& plan, double* work,
+                                 hipStream_t stream) {
   if (work == nullptr) return SXD_ERR_INVALID_ARGUMENT;
   const WorkLayout w(gh, plan, work);
   if (gh)
@@ -815,7 +812,16 @@ int launch_mortar_corrections(bool gh, const MortarPlan& plan, const MixedElemen
     hipLaunchKernelGGL(mortar_boundary_terms_kernel<false>, dim3(plan.num_records), dim3(64), 0,
                        stream, plan.d_records, plan.d_faces, plan.d_gather, w.pkf, w.pkm, w.recv,
                        w.bcm);
-  if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
+}
+
+int launch_mortar_lift(bool gh, const MortarPlan& plan, const MixedElement* el, double* dt,
+                       double* work, hipStream_t stream) {
+  const double* full = device_mortar_matrices();
+  const double* half = device_half_mortar_matrices();
+  if (full == nullptr || half == nullptr) return SXD_ERR_DEVICE;
+  if (work == nullptr) return SXD_ERR_INVALID_ARGUMENT;
+  const WorkLayout w(gh, plan, work);
   const int chunk = staged_chunk(gh ? 50 : 5, plan.tmp_max, kLiftCap);
   const size_t lds = sizeof(double) * chunk * plan.tmp_max;
   for (int d = 0; d < 3; ++d) {
@@ -832,6 +838,12 @@ int launch_mortar_corrections(bool gh, const MortarPlan& plan, const MixedElemen
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   }
   return SXD_OK;
+}
+
+int launch_mortar_corrections(bool gh, const MortarPlan& plan, const MixedElement* el,
+                              double* dt, double* work, hipStream_t stream) {
+  const int rc = launch_mortar_boundary_terms(gh, plan, work, stream);
+  return rc != SXD_OK ? rc : launch_mortar_lift(gh, plan, el, dt, work, stream);
 }
 
 int launch_mortar_faces(bool gh, const MortarPlan& plan, const MixedElement* el, const double* u,
