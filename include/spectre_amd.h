@@ -491,7 +491,9 @@ enum {
   SXD_KERNEL_MIXED_PER_EXTENTS = 5, /* one launch per kernel and group of equal extents */
   /* mortar domains, and periodic mixed-p domains (the default there): */
   SXD_KERNEL_MORTAR = 6 /* ragged volume kernels, then package, projection,
-                           boundary-term and lift launches over the mortars */
+                           boundary-term and lift launches over the mortars (the
+                           packages and boundary terms on an internal second
+                           stream, joined to the caller's stream by an event) */
 };
 enum { SXD_SYSTEM_SCALAR_WAVE = 0, SXD_SYSTEM_GH = 1 };
 
