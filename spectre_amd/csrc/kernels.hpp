@@ -192,6 +192,12 @@ struct MortarPeer {
 };
 struct MortarPlan {
   int num_faces = 0, num_records = 0, num_projected = 0, num_remote = 0;
+  // faces whose single mortar differs from the face (at most kDirectPoints
+  // points) are packaged and projected in one kernel; the others are
+  // packaged on the face, and their projected mortars (num_projected) are
+  // projected from the face package
+  int num_package_faces = 0, num_direct = 0;
+  int direct_nf = 0, direct_tmp = 1;  // largest face and ka nb among the direct faces
   int64_t face_points = 0, projected_points = 0, record_points = 0, remote_points = 0;
   int64_t message_bytes[2] = {0, 0};  // the whole send buffer, per system
   std::vector<MortarPeer> peers;      // outgoing groups, ascending rank
@@ -203,6 +209,8 @@ struct MortarPlan {
   MortarRecord* d_records = nullptr;
   int32_t* d_gather = nullptr;
   int32_t* d_projected = nullptr;
+  int32_t* d_package_faces = nullptr;
+  int32_t* d_direct = nullptr;
   MortarMessage* d_messages = nullptr;
   MortarRecvKey* d_keys = nullptr;
 };

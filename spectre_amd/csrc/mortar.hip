@@ -20,7 +20,11 @@
 // oriented offsets, inverted on the host), so no data is reordered:
 //   1. mortar_package_kernel: one workgroup per face, the package of every
 //      face point and |n| (SW 13 values, GH 10 pairs x 13 + 4 speeds);
-//   2. mortar_project_kernel: records whose mortar differs from the face,
+//      mortar_direct_kernel instead for a face whose single mortar differs
+//      from it: each group of package values formed in LDS and projected to
+//      the mortar at once (steps 1 and 2 without the face package in HBM);
+//   2. mortar_project_kernel: the other records whose mortar differs from
+//      the face,
 //      one workgroup per (record, component chunk): the dimension-a sums
 //      staged in LDS, then one output (mortar point, component) per work item;
 //   3. mortar_boundary_terms_kernel: one workgroup per record, the
@@ -79,62 +83,124 @@ __device__ __forceinline__ const double* from_mortar(const double* full, const d
          nf * nm;
 }
 
+// Face points of a face packaged and projected in one workgroup (one per
+// thread).  The mixed-p LDS plan already keeps faces below this (at most 193
+// points), so every face with a single differing mortar qualifies.
+constexpr int kDirectPoints = 256;
+
+// One face point's package inputs: GH keeps the side's normal, speeds and
+// gamma2 and the point's u (g_ab is read again per pair); ScalarWave keeps the
+// normalised normal, u and the parameter.
+template <bool GH>
+struct PackagePoint;
+template <>
+struct PackagePoint<true> {
+  GhFaceSide s;
+  const double* ue;  // u at the point, component stride np
+  int np;
+};
+template <>
+struct PackagePoint<false> {
+  double n[3], up[5], param;
+};
+
+// Package component groups: GH 10 symmetric pairs of 13 values, then the 4
+// speeds; ScalarWave one group of 13.
+template <bool GH>
+constexpr int kPackageGroups = GH ? 11 : 1;
+__device__ __forceinline__ int package_group_begin(bool gh, int gr) { return gh ? 13 * gr : 0; }
+__device__ __forceinline__ int package_group_size(bool gh, int gr) {
+  return gh && gr == 10 ? 4 : 13;
+}
+
+// Face point f of face fr (element m): fills pt and returns |n|.
+template <bool GH>
+__device__ __forceinline__ double package_setup(const MixedElement& m, const MortarFace& fr, int f,
+                                                const double* u, const double* params,
+                                                const double* inv_jac, PackagePoint<GH>& pt) {
+  const int d = fr.dir >> 1, side = fr.dir & 1;
+  const int na = d == 0 ? m.n[1] : m.n[0];
+  const int np = m.np;
+  const int p = face_point_index(m, d, side, f % na, f / na);
+  double n[3], mg;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    n[i] = inv_jac[9 * m.off + ij3(d, i) * np + p];
+    if (side == 0) n[i] = n[i] * -1.0;
+  }
+  if constexpr (!GH) {
+    flat_normalize(n, &mg);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) pt.n[i] = n[i];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) pt.up[c] = u[5 * m.off + c * np + p];
+    pt.param = params[m.off + p];
+  } else {
+    pt.ue = u + 50 * m.off + p;
+    pt.np = np;
+    double g[10];
+#pragma unroll
+    for (int c = 0; c < 10; ++c) g[c] = pt.ue[c * np];
+    double lapse, shift[3], inv_sp[6];
+    gh_lapse_shift(&lapse, shift, inv_sp, g);
+    const double* ga = params + 3 * m.off + p;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) pt.s.n[i] = n[i];
+    gh_normalize(pt.s.n, pt.s.nv, &mg, inv_sp);
+    gh_char_speeds(pt.s.lam, ga[np], lapse, shift, pt.s.n);
+    pt.s.gamma2 = ga[2 * np];
+  }
+  return mg;
+}
+
+// Package group gr of a point (package_group_size values into pk).
+template <bool GH>
+__device__ __forceinline__ void package_group(const PackagePoint<GH>& pt, int gr, double pk[13]) {
+  if constexpr (!GH) {
+    sw_package(pk, pt.up, pt.param, pt.n);
+  } else if (gr < 10) {
+    const int np = pt.np;
+    double phi[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) phi[i] = pt.ue[(20 + i + 3 * gr) * np];
+    gh_package_pair(pk, pt.ue[gr * np], pt.ue[(10 + gr) * np], phi, pt.s);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) pk[c] = pt.s.lam[c];
+  }
+}
+
+// Speed c of a GH point (package component 130 + c).
+__device__ __forceinline__ double package_speed(const PackagePoint<true>& pt, int c) {
+  return pt.s.lam[c];
+}
+__device__ __forceinline__ double package_speed(const PackagePoint<false>&, int) { return 0.0; }
+
+// Package of every point of the faces in `list` (one workgroup per face) and
+// |n| of each point.
 template <bool GH>
 __global__ void __launch_bounds__(128)
 mortar_package_kernel(const MixedElement* __restrict__ el, const MortarFace* __restrict__ faces,
-                      const double* __restrict__ u, const double* __restrict__ params,
-                      const double* __restrict__ inv_jac, double* __restrict__ pkf,
-                      double* __restrict__ mag) {
+                      const int32_t* __restrict__ list, const double* __restrict__ u,
+                      const double* __restrict__ params, const double* __restrict__ inv_jac,
+                      double* __restrict__ pkf, double* __restrict__ mag) {
   constexpr int NPK = GH ? 134 : 13;
-  const MortarFace fr = faces[blockIdx.x];
+  const MortarFace fr = faces[list[blockIdx.x]];
   const MixedElement m = el[fr.e];
-  const int d = fr.dir >> 1, side = fr.dir & 1;
-  const int na = d == 0 ? m.n[1] : m.n[0];
   const int nf = fr.nf;
-  const int np = m.np;
   double* out = pkf + static_cast<int64_t>(NPK) * fr.pt_off;
   for (int f = threadIdx.x; f < nf; f += blockDim.x) {
-    const int p = face_point_index(m, d, side, f % na, f / na);
-    double n[3], mg;
+    PackagePoint<GH> pt;
+    mag[fr.pt_off + f] = package_setup<GH>(m, fr, f, u, params, inv_jac, pt);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      n[i] = inv_jac[9 * m.off + ij3(d, i) * np + p];
-      if (side == 0) n[i] = n[i] * -1.0;
+    for (int gr = 0; gr < kPackageGroups<GH>; ++gr) {
+      double pk[13];
+      package_group<GH>(pt, gr, pk);
+      const int c0 = package_group_begin(GH, gr), cn = package_group_size(GH, gr);
+#pragma unroll
+      for (int c = 0; c < 13; ++c)
+        if (c < cn) out[(c0 + c) * nf + f] = pk[c];
     }
-    if constexpr (!GH) {
-      flat_normalize(n, &mg);
-      double up[5], pk[13];
-#pragma unroll
-      for (int c = 0; c < 5; ++c) up[c] = u[5 * m.off + c * np + p];
-      sw_package(pk, up, params[m.off + p], n);
-#pragma unroll
-      for (int c = 0; c < 13; ++c) out[c * nf + f] = pk[c];
-    } else {
-      const double* ue = u + 50 * m.off + p;
-      double g[10];
-#pragma unroll
-      for (int c = 0; c < 10; ++c) g[c] = ue[c * np];
-      double lapse, shift[3], inv_sp[6];
-      gh_lapse_shift(&lapse, shift, inv_sp, g);
-      const double* ga = params + 3 * m.off + p;
-      GhFaceSide s;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) s.n[i] = n[i];
-      gh_normalize(s.n, s.nv, &mg, inv_sp);
-      gh_char_speeds(s.lam, ga[np], lapse, shift, s.n);
-      s.gamma2 = ga[2 * np];
-      for (int pr = 0; pr < 10; ++pr) {
-        double phi[3], pk[13];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) phi[i] = ue[(20 + i + 3 * pr) * np];
-        gh_package_pair(pk, g[pr], ue[(10 + pr) * np], phi, s);
-#pragma unroll
-        for (int c = 0; c < 13; ++c) out[(13 * pr + c) * nf + f] = pk[c];
-      }
-#pragma unroll
-      for (int c = 0; c < 4; ++c) out[(130 + c) * nf + f] = s.lam[c];
-    }
-    mag[fr.pt_off + f] = mg;
   }
 }
 
@@ -199,6 +265,57 @@ mortar_project_kernel(const MortarRecord* __restrict__ recs, const int32_t* __re
   for (int x = threadIdx.x; x < cn * km; x += blockDim.x) {
     const int c = x / km, k = x % km;
     out[c * km + k] = pass_b(s_tmp, c, r.ka, r.nb, Mb, k % r.ka, k / r.ka);
+  }
+}
+
+// Package and projection together for the faces in `list`: faces with a
+// single mortar that differs from the face and at most 256 points (one per
+// thread).  Each package group is formed in LDS and projected onto the
+// mortar at once, so the face package never goes to HBM; the arithmetic is
+// mortar_package_kernel's followed by mortar_project_kernel's.  Dynamic LDS:
+// 13 x (nf_max + tmp_max) doubles.
+template <bool GH>
+__global__ void __launch_bounds__(kDirectPoints)
+mortar_direct_kernel(const MixedElement* __restrict__ el, const MortarFace* __restrict__ faces,
+                     const int32_t* __restrict__ list, const MortarRecord* __restrict__ recs,
+                     const double* __restrict__ u, const double* __restrict__ params,
+                     const double* __restrict__ inv_jac, double* __restrict__ pkm,
+                     double* __restrict__ mag, const double* __restrict__ full,
+                     const double* __restrict__ half, int nf_max) {
+  constexpr int NPK = GH ? 134 : 13;
+  extern __shared__ double s_lds[];
+  double* s_pk = s_lds;                 // [13][nf] of the current group
+  double* s_tmp = s_lds + 13 * nf_max;  // dimension-a sums
+  const MortarFace fr = faces[list[blockIdx.x]];
+  const MortarRecord r = recs[fr.rec_begin];
+  const MixedElement m = el[fr.e];
+  const int nf = fr.nf, ka = r.ka, km = r.ka * r.kb;
+  const int f = threadIdx.x;
+  PackagePoint<GH> pt{};
+  if (f < nf) mag[fr.pt_off + f] = package_setup<GH>(m, fr, f, u, params, inv_jac, pt);
+  const double* Ma = to_mortar(full, half, r.na, r.ka, r.size_a);
+  const double* Mb = to_mortar(full, half, r.nb, r.kb, r.size_b);
+  double* out = pkm + static_cast<int64_t>(NPK) * r.mortar_off;
+  for (int gr = 0; gr < kPackageGroups<GH>; ++gr) {
+    const int c0 = package_group_begin(GH, gr), cn = package_group_size(GH, gr);
+    if (f < nf) {
+      if (GH && gr == 10) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) s_pk[c * nf + f] = package_speed(pt, c);
+      } else {
+        double pk[13];
+        package_group<GH>(pt, gr, pk);
+#pragma unroll
+        for (int c = 0; c < 13; ++c) s_pk[c * nf + f] = pk[c];
+      }
+    }
+    __syncthreads();  // the group's package is complete
+    stage_pass_a(s_pk, nf, cn, r.na, r.nb, ka, Ma, s_tmp);
+    __syncthreads();  // dimension-a sums complete; s_pk free for the next group
+    for (int x = threadIdx.x; x < cn * km; x += blockDim.x) {
+      const int c = x / km, k = x % km;
+      out[(c0 + c) * km + k] = pass_b(s_tmp, c, ka, r.nb, Mb, k % ka, k / ka);
+    }
   }
 }
 
@@ -695,6 +812,25 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
     if (!has_face[i])
       return bad("element " + std::to_string(i / 6) + " has no mortar in direction " +
                  std::to_string(i % 6));
+  // faces packaged and projected together: one mortar, which differs from
+  // the face; the project kernel takes the other projected mortars
+  std::vector<int32_t> package_faces, direct, projected;
+  int direct_nf = 0, direct_tmp = 1;
+  for (size_t i = 0; i < faces.size(); ++i) {
+    const MortarFace& f = faces[i];
+    const MortarRecord& r = recs[f.rec_begin];
+    if (f.rec_count == 1 && r.mortar_off >= 0 && f.nf <= kDirectPoints) {
+      direct.push_back(static_cast<int32_t>(i));
+      direct_nf = std::max(direct_nf, f.nf);
+      direct_tmp = std::max(direct_tmp, r.ka * r.nb);
+    } else {
+      package_faces.push_back(static_cast<int32_t>(i));
+    }
+  }
+  for (const int32_t q : proj) {
+    const MortarFace& f = faces[recs[q].face];
+    if (!(f.rec_count == 1 && f.nf <= kDirectPoints)) projected.push_back(q);
+  }
   int kf = 0, km = 0, tmp_max = 1;
   for (const MortarRecord& m : recs) {
     kf = std::max(kf, m.na * m.nb);
@@ -707,7 +843,11 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
   plan->remote_points = recv_pts;
   plan->message_bytes[0] = msg_bytes[0];
   plan->message_bytes[1] = msg_bytes[1];
-  plan->num_projected = static_cast<int>(proj.size());
+  plan->num_projected = static_cast<int>(projected.size());
+  plan->num_package_faces = static_cast<int>(package_faces.size());
+  plan->num_direct = static_cast<int>(direct.size());
+  plan->direct_nf = direct_nf;
+  plan->direct_tmp = direct_tmp;
   plan->face_points = face_pts;
   plan->projected_points = proj_pts;
   plan->record_points = rec_pts;
@@ -728,7 +868,9 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
   upload(&plan->d_faces, faces);
   upload(&plan->d_records, recs);
   upload(&plan->d_gather, gather);
-  upload(&plan->d_projected, proj);
+  upload(&plan->d_projected, projected);
+  upload(&plan->d_package_faces, package_faces);
+  upload(&plan->d_direct, direct);
   upload(&plan->d_messages, messages);
   upload(&plan->d_keys, keys);
   if (rc != SXD_OK) {
@@ -743,6 +885,8 @@ void destroy_mortar_plan(MortarPlan* plan) {
   if (plan->d_records != nullptr) (void)hipFree(plan->d_records);
   if (plan->d_gather != nullptr) (void)hipFree(plan->d_gather);
   if (plan->d_projected != nullptr) (void)hipFree(plan->d_projected);
+  if (plan->d_package_faces != nullptr) (void)hipFree(plan->d_package_faces);
+  if (plan->d_direct != nullptr) (void)hipFree(plan->d_direct);
   if (plan->d_messages != nullptr) (void)hipFree(plan->d_messages);
   if (plan->d_keys != nullptr) (void)hipFree(plan->d_keys);
   *plan = MortarPlan{};
@@ -778,13 +922,29 @@ int launch_mortar_packages(bool gh, const MortarPlan& plan, const MixedElement* 
   if (work == nullptr) return SXD_ERR_INVALID_ARGUMENT;
   const int npk = gh ? 134 : 13;
   const WorkLayout w(gh, plan, work);
-  if (gh)
-    hipLaunchKernelGGL(mortar_package_kernel<true>, dim3(plan.num_faces), dim3(128), 0, stream, el,
-                       plan.d_faces, u, params, inv_jac, w.pkf, w.mag);
-  else
-    hipLaunchKernelGGL(mortar_package_kernel<false>, dim3(plan.num_faces), dim3(128), 0, stream,
-                       el, plan.d_faces, u, params, inv_jac, w.pkf, w.mag);
-  if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  if (plan.num_package_faces > 0) {
+    if (gh)
+      hipLaunchKernelGGL(mortar_package_kernel<true>, dim3(plan.num_package_faces), dim3(128), 0,
+                         stream, el, plan.d_faces, plan.d_package_faces, u, params, inv_jac,
+                         w.pkf, w.mag);
+    else
+      hipLaunchKernelGGL(mortar_package_kernel<false>, dim3(plan.num_package_faces), dim3(128), 0,
+                         stream, el, plan.d_faces, plan.d_package_faces, u, params, inv_jac,
+                         w.pkf, w.mag);
+    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  }
+  if (plan.num_direct > 0) {
+    const size_t lds = sizeof(double) * 13 * (plan.direct_nf + plan.direct_tmp);
+    if (gh)
+      hipLaunchKernelGGL(mortar_direct_kernel<true>, dim3(plan.num_direct), dim3(kDirectPoints),
+                         lds, stream, el, plan.d_faces, plan.d_direct, plan.d_records, u, params,
+                         inv_jac, w.pkm, w.mag, full, half, plan.direct_nf);
+    else
+      hipLaunchKernelGGL(mortar_direct_kernel<false>, dim3(plan.num_direct), dim3(kDirectPoints),
+                         lds, stream, el, plan.d_faces, plan.d_direct, plan.d_records, u, params,
+                         inv_jac, w.pkm, w.mag, full, half, plan.direct_nf);
+    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  }
   if (plan.num_projected > 0) {
     const int chunk = staged_chunk(npk, plan.tmp_max, kProjectCap);
     const dim3 grid(plan.num_projected, (npk + chunk - 1) / chunk);
