@@ -197,7 +197,8 @@ struct MortarPlan {
   // packaged on the face, and their projected mortars (num_projected) are
   // projected from the face package
   int num_package_faces = 0, num_direct = 0;
-  int direct_nf = 0, direct_tmp = 1;  // largest face and ka nb among the direct faces
+  // largest face, ka nb, and ka na + kb nb (matrix doubles) among the direct faces
+  int direct_nf = 0, direct_tmp = 1, direct_mat = 0;
   int64_t face_points = 0, projected_points = 0, record_points = 0, remote_points = 0;
   int64_t message_bytes[2] = {0, 0};  // the whole send buffer, per system
   std::vector<MortarPeer> peers;      // outgoing groups, ascending rank

@@ -273,7 +273,7 @@ mortar_project_kernel(const MortarRecord* __restrict__ recs, const int32_t* __re
 // thread).  Each package group is formed in LDS and projected onto the
 // mortar at once, so the face package never goes to HBM; the arithmetic is
 // mortar_package_kernel's followed by mortar_project_kernel's.  Dynamic LDS:
-// 13 x (nf_max + tmp_max) doubles.
+// 13 x (nf_max + tmp_max) + mat_max doubles (the face-to-mortar matrices).
 template <bool GH>
 __global__ void __launch_bounds__(kDirectPoints)
 mortar_direct_kernel(const MixedElement* __restrict__ el, const MortarFace* __restrict__ faces,
@@ -281,20 +281,28 @@ mortar_direct_kernel(const MixedElement* __restrict__ el, const MortarFace* __re
                      const double* __restrict__ u, const double* __restrict__ params,
                      const double* __restrict__ inv_jac, double* __restrict__ pkm,
                      double* __restrict__ mag, const double* __restrict__ full,
-                     const double* __restrict__ half, int nf_max) {
+                     const double* __restrict__ half, int nf_max, int tmp_max) {
   constexpr int NPK = GH ? 134 : 13;
   extern __shared__ double s_lds[];
   double* s_pk = s_lds;                 // [13][nf] of the current group
   double* s_tmp = s_lds + 13 * nf_max;  // dimension-a sums
+  double* s_ma = s_tmp + 13 * tmp_max;  // the two matrices, read by every group
   const MortarFace fr = faces[list[blockIdx.x]];
   const MortarRecord r = recs[fr.rec_begin];
   const MixedElement m = el[fr.e];
   const int nf = fr.nf, ka = r.ka, km = r.ka * r.kb;
   const int f = threadIdx.x;
-  PackagePoint<GH> pt{};
-  if (f < nf) mag[fr.pt_off + f] = package_setup<GH>(m, fr, f, u, params, inv_jac, pt);
   const double* Ma = to_mortar(full, half, r.na, r.ka, r.size_a);
   const double* Mb = to_mortar(full, half, r.nb, r.kb, r.size_b);
+  double* s_mb = s_ma + (Ma != nullptr ? ka * r.na : 0);
+  if (Ma != nullptr)
+    for (int i = f; i < ka * r.na; i += blockDim.x) s_ma[i] = Ma[i];
+  if (Mb != nullptr)
+    for (int i = f; i < r.kb * r.nb; i += blockDim.x) s_mb[i] = Mb[i];
+  if (Ma != nullptr) Ma = s_ma;
+  if (Mb != nullptr) Mb = s_mb;
+  PackagePoint<GH> pt{};
+  if (f < nf) mag[fr.pt_off + f] = package_setup<GH>(m, fr, f, u, params, inv_jac, pt);
   double* out = pkm + static_cast<int64_t>(NPK) * r.mortar_off;
   for (int gr = 0; gr < kPackageGroups<GH>; ++gr) {
     const int c0 = package_group_begin(GH, gr), cn = package_group_size(GH, gr);
@@ -309,7 +317,7 @@ mortar_direct_kernel(const MixedElement* __restrict__ el, const MortarFace* __re
         for (int c = 0; c < 13; ++c) s_pk[c * nf + f] = pk[c];
       }
     }
-    __syncthreads();  // the group's package is complete
+    __syncthreads();  // the group's package (and, first, the matrices) complete
     stage_pass_a(s_pk, nf, cn, r.na, r.nb, ka, Ma, s_tmp);
     __syncthreads();  // dimension-a sums complete; s_pk free for the next group
     for (int x = threadIdx.x; x < cn * km; x += blockDim.x) {
@@ -815,7 +823,7 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
   // faces packaged and projected together: one mortar, which differs from
   // the face; the project kernel takes the other projected mortars
   std::vector<int32_t> package_faces, direct, projected;
-  int direct_nf = 0, direct_tmp = 1;
+  int direct_nf = 0, direct_tmp = 1, direct_mat = 0;
   for (size_t i = 0; i < faces.size(); ++i) {
     const MortarFace& f = faces[i];
     const MortarRecord& r = recs[f.rec_begin];
@@ -823,6 +831,7 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
       direct.push_back(static_cast<int32_t>(i));
       direct_nf = std::max(direct_nf, f.nf);
       direct_tmp = std::max(direct_tmp, r.ka * r.nb);
+      direct_mat = std::max(direct_mat, r.ka * r.na + r.kb * r.nb);
     } else {
       package_faces.push_back(static_cast<int32_t>(i));
     }
@@ -848,6 +857,7 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
   plan->num_direct = static_cast<int>(direct.size());
   plan->direct_nf = direct_nf;
   plan->direct_tmp = direct_tmp;
+  plan->direct_mat = direct_mat;
   plan->face_points = face_pts;
   plan->projected_points = proj_pts;
   plan->record_points = rec_pts;
@@ -934,15 +944,16 @@ int launch_mortar_packages(bool gh, const MortarPlan& plan, const MixedElement* 
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   }
   if (plan.num_direct > 0) {
-    const size_t lds = sizeof(double) * 13 * (plan.direct_nf + plan.direct_tmp);
+    const size_t lds =
+        sizeof(double) * (13 * (plan.direct_nf + plan.direct_tmp) + plan.direct_mat);
     if (gh)
       hipLaunchKernelGGL(mortar_direct_kernel<true>, dim3(plan.num_direct), dim3(kDirectPoints),
                          lds, stream, el, plan.d_faces, plan.d_direct, plan.d_records, u, params,
-                         inv_jac, w.pkm, w.mag, full, half, plan.direct_nf);
+                         inv_jac, w.pkm, w.mag, full, half, plan.direct_nf, plan.direct_tmp);
     else
       hipLaunchKernelGGL(mortar_direct_kernel<false>, dim3(plan.num_direct), dim3(kDirectPoints),
                          lds, stream, el, plan.d_faces, plan.d_direct, plan.d_records, u, params,
-                         inv_jac, w.pkm, w.mag, full, half, plan.direct_nf);
+                         inv_jac, w.pkm, w.mag, full, half, plan.direct_nf, plan.direct_tmp);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   }
   if (plan.num_projected > 0) {
