@@ -206,6 +206,7 @@ struct MortarPlan {
   int dim_begin[4] = {0, 0, 0, 0};  // faces of dimension d: [dim_begin[d], dim_begin[d+1])
   int kf = 0, km = 0;
   int tmp_max = 1;  // largest ka nb or na kb: doubles per component of a staged pass
+  int mat_max = 0;  // largest ka na + kb nb: doubles of a record's two matrices
   MortarFace* d_faces = nullptr;
   MortarRecord* d_records = nullptr;
   int32_t* d_gather = nullptr;

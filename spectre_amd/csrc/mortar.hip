@@ -242,24 +242,43 @@ __device__ __forceinline__ double pass_b(const double* tmp, int c, int oa, int i
   return s;
 }
 
+// Copies a record's two matrices (na x ka and nb x kb elements, either may
+// be the identity: nullptr) into LDS at s_mat and points Ma and Mb there.
+// The caller synchronises before the first read.
+__device__ __forceinline__ void stage_matrices(const double*& Ma, int size_a, const double*& Mb,
+                                               int size_b, double* s_mat) {
+  double* s_mb = s_mat + (Ma != nullptr ? size_a : 0);
+  if (Ma != nullptr) {
+    for (int i = threadIdx.x; i < size_a; i += blockDim.x) s_mat[i] = Ma[i];
+    Ma = s_mat;
+  }
+  if (Mb != nullptr) {
+    for (int i = threadIdx.x; i < size_b; i += blockDim.x) s_mb[i] = Mb[i];
+    Mb = s_mb;
+  }
+}
+
 // Face -> mortar projection of the records that need one; grid (record,
-// component chunk of `chunk`); dynamic LDS: chunk x (ka nb) doubles.
+// component chunk of `chunk`); dynamic LDS: chunk x tmp_max doubles of
+// dimension-a sums, then the record's two matrices (mat_max doubles).
 template <int NPK>
 __global__ void __launch_bounds__(256)
 mortar_project_kernel(const MortarRecord* __restrict__ recs, const int32_t* __restrict__ list,
                       const MortarFace* __restrict__ faces, const double* __restrict__ pkf,
                       double* __restrict__ pkm, const double* __restrict__ full,
-                      const double* __restrict__ half, int chunk) {
+                      const double* __restrict__ half, int chunk, int tmp_max) {
   extern __shared__ double s_tmp[];
   const MortarRecord r = recs[list[blockIdx.x]];
   const MortarFace fr = faces[r.face];
   const double* Ma = to_mortar(full, half, r.na, r.ka, r.size_a);
   const double* Mb = to_mortar(full, half, r.nb, r.kb, r.size_b);
+  stage_matrices(Ma, r.ka * r.na, Mb, r.kb * r.nb, s_tmp + chunk * tmp_max);
   const int nf = r.na * r.nb, km = r.ka * r.kb;
   const int c0 = blockIdx.y * chunk;
   const int cn = min(chunk, NPK - c0);
   const double* in = pkf + static_cast<int64_t>(NPK) * fr.pt_off + c0 * nf;
   double* out = pkm + static_cast<int64_t>(NPK) * r.mortar_off + c0 * km;
+  __syncthreads();  // matrices staged
   stage_pass_a(in, nf, cn, r.na, r.nb, r.ka, Ma, s_tmp);
   __syncthreads();
   for (int x = threadIdx.x; x < cn * km; x += blockDim.x) {
@@ -294,13 +313,7 @@ mortar_direct_kernel(const MixedElement* __restrict__ el, const MortarFace* __re
   const int f = threadIdx.x;
   const double* Ma = to_mortar(full, half, r.na, r.ka, r.size_a);
   const double* Mb = to_mortar(full, half, r.nb, r.kb, r.size_b);
-  double* s_mb = s_ma + (Ma != nullptr ? ka * r.na : 0);
-  if (Ma != nullptr)
-    for (int i = f; i < ka * r.na; i += blockDim.x) s_ma[i] = Ma[i];
-  if (Mb != nullptr)
-    for (int i = f; i < r.kb * r.nb; i += blockDim.x) s_mb[i] = Mb[i];
-  if (Ma != nullptr) Ma = s_ma;
-  if (Mb != nullptr) Mb = s_mb;
+  stage_matrices(Ma, ka * r.na, Mb, r.kb * r.nb, s_ma);
   PackagePoint<GH> pt{};
   if (f < nf) mag[fr.pt_off + f] = package_setup<GH>(m, fr, f, u, params, inv_jac, pt);
   double* out = pkm + static_cast<int64_t>(NPK) * r.mortar_off;
@@ -394,14 +407,15 @@ mortar_boundary_terms_kernel(const MortarRecord* __restrict__ recs,
 // dimension, one workgroup per face: its mortars in record order, each
 // projected (dimension a staged in LDS for `chunk` components at a time,
 // then dimension b per (component, face point) work item) and added into
-// dt.  Dynamic LDS: chunk x (na kb) doubles.
+// dt.  Dynamic LDS: chunk x tmp_max doubles of dimension-a sums, then the
+// record's two matrices (mat_max doubles).
 template <bool GH>
 __global__ void __launch_bounds__(256)
 mortar_lift_kernel(const MixedElement* __restrict__ el, const MortarFace* __restrict__ faces,
                    int face_begin, const MortarRecord* __restrict__ recs,
                    const double* __restrict__ mag, const double* __restrict__ bcm,
                    double* __restrict__ dt, const double* __restrict__ full,
-                   const double* __restrict__ half, int chunk) {
+                   const double* __restrict__ half, int chunk, int tmp_max) {
   constexpr int NC = GH ? 50 : 5;
   extern __shared__ double s_tmp[];
   const MortarFace fr = faces[face_begin + blockIdx.x];
@@ -426,6 +440,9 @@ mortar_lift_kernel(const MixedElement* __restrict__ el, const MortarFace* __rest
         t = t + src[c * km + f] * lift_factor(nd, mag[fr.pt_off + f]);
       }
     } else {
+      // the previous mortar's last barrier ended its reads of the matrices
+      stage_matrices(Pa, r.na * r.ka, Pb, r.nb * r.kb, s_tmp + chunk * tmp_max);
+      __syncthreads();
       for (int c0 = 0; c0 < NC; c0 += chunk) {
         const int cn = min(chunk, NC - c0);
         stage_pass_a(src + c0 * km, km, cn, r.ka, r.kb, r.na, Pa, s_tmp);
@@ -840,11 +857,12 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
     const MortarFace& f = faces[recs[q].face];
     if (!(f.rec_count == 1 && f.nf <= kDirectPoints)) projected.push_back(q);
   }
-  int kf = 0, km = 0, tmp_max = 1;
+  int kf = 0, km = 0, tmp_max = 1, mat_max = 0;
   for (const MortarRecord& m : recs) {
     kf = std::max(kf, m.na * m.nb);
     km = std::max(km, m.ka * m.kb);
     tmp_max = std::max({tmp_max, m.ka * m.nb, m.na * m.kb});
+    mat_max = std::max(mat_max, m.ka * m.na + m.kb * m.nb);
   }
   plan->num_faces = static_cast<int>(faces.size());
   plan->num_records = kept;
@@ -865,6 +883,7 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
   plan->kf = kf;
   plan->km = km;
   plan->tmp_max = tmp_max;
+  plan->mat_max = mat_max;
   int rc = SXD_OK;
   auto upload = [&](auto** dst, const auto& v) {
     if (rc != SXD_OK || v.empty()) return;
@@ -959,13 +978,15 @@ int launch_mortar_packages(bool gh, const MortarPlan& plan, const MixedElement* 
   if (plan.num_projected > 0) {
     const int chunk = staged_chunk(npk, plan.tmp_max, kProjectCap);
     const dim3 grid(plan.num_projected, (npk + chunk - 1) / chunk);
-    const size_t lds = sizeof(double) * chunk * plan.tmp_max;
+    const size_t lds = sizeof(double) * (chunk * plan.tmp_max + plan.mat_max);
     if (gh)
       hipLaunchKernelGGL(mortar_project_kernel<134>, grid, dim3(256), lds, stream, plan.d_records,
-                         plan.d_projected, plan.d_faces, w.pkf, w.pkm, full, half, chunk);
+                         plan.d_projected, plan.d_faces, w.pkf, w.pkm, full, half, chunk,
+                         plan.tmp_max);
     else
       hipLaunchKernelGGL(mortar_project_kernel<13>, grid, dim3(256), lds, stream, plan.d_records,
-                         plan.d_projected, plan.d_faces, w.pkf, w.pkm, full, half, chunk);
+                         plan.d_projected, plan.d_faces, w.pkf, w.pkm, full, half, chunk,
+                         plan.tmp_max);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   }
   return SXD_OK;
@@ -994,18 +1015,18 @@ int launch_mortar_lift(bool gh, const MortarPlan& plan, const MixedElement* el, 
   if (work == nullptr) return SXD_ERR_INVALID_ARGUMENT;
   const WorkLayout w(gh, plan, work);
   const int chunk = staged_chunk(gh ? 50 : 5, plan.tmp_max, kLiftCap);
-  const size_t lds = sizeof(double) * chunk * plan.tmp_max;
+  const size_t lds = sizeof(double) * (chunk * plan.tmp_max + plan.mat_max);
   for (int d = 0; d < 3; ++d) {
     const int count = plan.dim_begin[d + 1] - plan.dim_begin[d];
     if (count == 0) continue;
     if (gh)
       hipLaunchKernelGGL(mortar_lift_kernel<true>, dim3(count), dim3(256), lds, stream, el,
                          plan.d_faces, plan.dim_begin[d], plan.d_records, w.mag, w.bcm, dt, full,
-                         half, chunk);
+                         half, chunk, plan.tmp_max);
     else
       hipLaunchKernelGGL(mortar_lift_kernel<false>, dim3(count), dim3(256), lds, stream, el,
                          plan.d_faces, plan.dim_begin[d], plan.d_records, w.mag, w.bcm, dt, full,
-                         half, chunk);
+                         half, chunk, plan.tmp_max);
     if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
   }
   return SXD_OK;
