@@ -212,6 +212,11 @@ struct MortarPlan {
   int32_t* d_gather = nullptr;
   int32_t* d_projected = nullptr;
   int32_t* d_package_faces = nullptr;
+  // boundary terms: one workgroup per local mortar (the record before its
+  // partner) for both sides, one per record whose partner is remote
+  int num_pairs = 0, num_remote_records = 0;
+  int32_t* d_pairs = nullptr;
+  int32_t* d_remote_records = nullptr;
   int32_t* d_direct = nullptr;
   MortarMessage* d_messages = nullptr;
   MortarRecvKey* d_keys = nullptr;

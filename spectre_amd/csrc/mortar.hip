@@ -27,9 +27,10 @@
 //      the face,
 //      one workgroup per (record, component chunk): the dimension-a sums
 //      staged in LDS, then one output (mortar point, component) per work item;
-//   3. mortar_boundary_terms_kernel: one workgroup per record, the
-//      correction at every mortar point from the own and the gathered
-//      partner package;
+//   3. mortar_boundary_pairs_kernel: one workgroup per mortar with both
+//      sides local, the corrections of both records at every mortar point
+//      from the two packages (each read once); mortar_boundary_terms_kernel
+//      for a record whose partner's package arrived in a message;
 //   4. mortar_lift_kernel: one launch per dimension (both sides; their points
 //      differ), one workgroup per face: each of its mortars in record order
 //      is projected back (dimension-a sums staged in LDS, then one work item
@@ -353,13 +354,14 @@ __device__ __forceinline__ const double* mortar_package(const MortarRecord& r,
 template <bool GH>
 __global__ void __launch_bounds__(64)
 mortar_boundary_terms_kernel(const MortarRecord* __restrict__ recs,
+                             const int32_t* __restrict__ list,
                              const MortarFace* __restrict__ faces,
                              const int32_t* __restrict__ gather, const double* __restrict__ pkf,
                              const double* __restrict__ pkm, const double* __restrict__ recv,
                              double* __restrict__ bcm) {
   constexpr int NPK = GH ? 134 : 13;
   constexpr int NC = GH ? 50 : 5;
-  const MortarRecord r = recs[blockIdx.x];
+  const MortarRecord r = recs[list[blockIdx.x]];
   const bool remote = r.partner < 0;
   const double* own = mortar_package<NPK>(r, faces, pkf, pkm);
   // a partner in another partition: its package arrived in the own point order
@@ -398,6 +400,72 @@ mortar_boundary_terms_kernel(const MortarRecord* __restrict__ recs,
         out[(10 + pr) * km + k] = bc[1];
 #pragma unroll
         for (int i = 0; i < 3; ++i) out[(20 + 3 * pr + i) * km + k] = bc[2 + i];
+      }
+    }
+  }
+}
+
+// Boundary terms of both records of a mortar whose sides are both local, one
+// workgroup per mortar (the record in `list` and its partner): each package
+// is read once, and each side gets the correction mortar_boundary_terms_kernel
+// forms for it (its own package interior, the other exterior), at its own
+// point order (own point k is the partner's point gather[k], and the
+// partner's gather maps it back).
+template <bool GH>
+__global__ void __launch_bounds__(64)
+mortar_boundary_pairs_kernel(const MortarRecord* __restrict__ recs,
+                             const int32_t* __restrict__ list,
+                             const MortarFace* __restrict__ faces,
+                             const int32_t* __restrict__ gather, const double* __restrict__ pkf,
+                             const double* __restrict__ pkm, double* __restrict__ bcm) {
+  constexpr int NPK = GH ? 134 : 13;
+  constexpr int NC = GH ? 50 : 5;
+  const MortarRecord r = recs[list[blockIdx.x]];
+  const MortarRecord q = recs[r.partner];
+  const double* own = mortar_package<NPK>(r, faces, pkf, pkm);
+  const double* ext = mortar_package<NPK>(q, faces, pkf, pkm);
+  const int km = r.ka * r.kb;
+  double* out = bcm + static_cast<int64_t>(NC) * r.rec_off;
+  double* out_q = bcm + static_cast<int64_t>(NC) * q.rec_off;
+  for (int k = threadIdx.x; k < km; k += blockDim.x) {
+    const int s = gather[r.rec_off + k];
+    if constexpr (!GH) {
+      double in[13], ex[13], bc[5];
+#pragma unroll
+      for (int c = 0; c < 13; ++c) {
+        in[c] = own[c * km + k];
+        ex[c] = ext[c * km + s];
+      }
+      sw_boundary_terms(bc, in, ex);
+#pragma unroll
+      for (int c = 0; c < 5; ++c) out[c * km + k] = bc[c];
+      sw_boundary_terms(bc, ex, in);
+#pragma unroll
+      for (int c = 0; c < 5; ++c) out_q[c * km + s] = bc[c];
+    } else {
+      double lam_in[4], lam_ex[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        lam_in[c] = own[(130 + c) * km + k];
+        lam_ex[c] = ext[(130 + c) * km + s];
+      }
+      for (int pr = 0; pr < 10; ++pr) {
+        double in[13], ex[13], bc[5];
+#pragma unroll
+        for (int c = 0; c < 13; ++c) {
+          in[c] = own[(13 * pr + c) * km + k];
+          ex[c] = ext[(13 * pr + c) * km + s];
+        }
+        gh_boundary_terms_pair(bc, in, lam_in, ex, lam_ex);
+        out[pr * km + k] = bc[0];
+        out[(10 + pr) * km + k] = bc[1];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) out[(20 + 3 * pr + i) * km + k] = bc[2 + i];
+        gh_boundary_terms_pair(bc, ex, lam_ex, in, lam_in);
+        out_q[pr * km + s] = bc[0];
+        out_q[(10 + pr) * km + s] = bc[1];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) out_q[(20 + 3 * pr + i) * km + s] = bc[2 + i];
       }
     }
   }
@@ -857,6 +925,13 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
     const MortarFace& f = faces[recs[q].face];
     if (!(f.rec_count == 1 && f.nf <= kDirectPoints)) projected.push_back(q);
   }
+  std::vector<int32_t> pairs, remote_records;
+  for (int i = 0; i < kept; ++i) {
+    if (recs[i].partner < 0)
+      remote_records.push_back(i);
+    else if (recs[i].partner > i)
+      pairs.push_back(i);
+  }
   int kf = 0, km = 0, tmp_max = 1, mat_max = 0;
   for (const MortarRecord& m : recs) {
     kf = std::max(kf, m.na * m.nb);
@@ -873,6 +948,8 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
   plan->num_projected = static_cast<int>(projected.size());
   plan->num_package_faces = static_cast<int>(package_faces.size());
   plan->num_direct = static_cast<int>(direct.size());
+  plan->num_pairs = static_cast<int>(pairs.size());
+  plan->num_remote_records = static_cast<int>(remote_records.size());
   plan->direct_nf = direct_nf;
   plan->direct_tmp = direct_tmp;
   plan->direct_mat = direct_mat;
@@ -900,6 +977,8 @@ int build_mortar_plan(int num_elements, const int32_t* extents, int num_records,
   upload(&plan->d_projected, projected);
   upload(&plan->d_package_faces, package_faces);
   upload(&plan->d_direct, direct);
+  upload(&plan->d_pairs, pairs);
+  upload(&plan->d_remote_records, remote_records);
   upload(&plan->d_messages, messages);
   upload(&plan->d_keys, keys);
   if (rc != SXD_OK) {
@@ -916,6 +995,8 @@ void destroy_mortar_plan(MortarPlan* plan) {
   if (plan->d_projected != nullptr) (void)hipFree(plan->d_projected);
   if (plan->d_package_faces != nullptr) (void)hipFree(plan->d_package_faces);
   if (plan->d_direct != nullptr) (void)hipFree(plan->d_direct);
+  if (plan->d_pairs != nullptr) (void)hipFree(plan->d_pairs);
+  if (plan->d_remote_records != nullptr) (void)hipFree(plan->d_remote_records);
   if (plan->d_messages != nullptr) (void)hipFree(plan->d_messages);
   if (plan->d_keys != nullptr) (void)hipFree(plan->d_keys);
   *plan = MortarPlan{};
@@ -996,15 +1077,29 @@ int launch_mortar_boundary_terms(bool gh, const MortarPlan& plan, double* work,
                                  hipStream_t stream) {
   if (work == nullptr) return SXD_ERR_INVALID_ARGUMENT;
   const WorkLayout w(gh, plan, work);
-  if (gh)
-    hipLaunchKernelGGL(mortar_boundary_terms_kernel<true>, dim3(plan.num_records), dim3(64), 0,
-                       stream, plan.d_records, plan.d_faces, plan.d_gather, w.pkf, w.pkm, w.recv,
-                       w.bcm);
-  else
-    hipLaunchKernelGGL(mortar_boundary_terms_kernel<false>, dim3(plan.num_records), dim3(64), 0,
-                       stream, plan.d_records, plan.d_faces, plan.d_gather, w.pkf, w.pkm, w.recv,
-                       w.bcm);
-  return hipGetLastError() == hipSuccess ? SXD_OK : SXD_ERR_DEVICE;
+  if (plan.num_pairs > 0) {
+    if (gh)
+      hipLaunchKernelGGL(mortar_boundary_pairs_kernel<true>, dim3(plan.num_pairs), dim3(64), 0,
+                         stream, plan.d_records, plan.d_pairs, plan.d_faces, plan.d_gather, w.pkf,
+                         w.pkm, w.bcm);
+    else
+      hipLaunchKernelGGL(mortar_boundary_pairs_kernel<false>, dim3(plan.num_pairs), dim3(64), 0,
+                         stream, plan.d_records, plan.d_pairs, plan.d_faces, plan.d_gather, w.pkf,
+                         w.pkm, w.bcm);
+    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  }
+  if (plan.num_remote_records > 0) {
+    if (gh)
+      hipLaunchKernelGGL(mortar_boundary_terms_kernel<true>, dim3(plan.num_remote_records),
+                         dim3(64), 0, stream, plan.d_records, plan.d_remote_records, plan.d_faces,
+                         plan.d_gather, w.pkf, w.pkm, w.recv, w.bcm);
+    else
+      hipLaunchKernelGGL(mortar_boundary_terms_kernel<false>, dim3(plan.num_remote_records),
+                         dim3(64), 0, stream, plan.d_records, plan.d_remote_records, plan.d_faces,
+                         plan.d_gather, w.pkf, w.pkm, w.recv, w.bcm);
+    if (hipGetLastError() != hipSuccess) return SXD_ERR_DEVICE;
+  }
+  return SXD_OK;
 }
 
 int launch_mortar_lift(bool gh, const MortarPlan& plan, const MixedElement* el, double* dt,
